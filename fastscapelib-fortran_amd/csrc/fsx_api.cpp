@@ -1,0 +1,547 @@
+// fsx_api.cpp -- C ABI of the MI355X landscape step: context lifetime, parameter blocks, field
+// transfer and the per-step launch sequence (FastScape_Execute_Step, src/FastScape_api.f90:201-263).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/fastscape_hip.h"
+#include "fsx_state.h"
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(x)                                                                     \
+  do {                                                                                \
+    hipError_t e_ = (x);                                                              \
+    if (e_ != hipSuccess) return fail(FSX_EHIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+struct fsx_ctx {
+  DevState S;
+  std::vector<InstParams> hp;
+  std::vector<int> steps;
+  std::vector<double> xl, yl;
+  std::vector<int> ibc;
+  std::vector<int> chunk_base;
+  std::vector<void*> allocs;
+  hipStream_t stream = nullptr;
+  bool dirty = true;
+  long long bytes = 0;
+};
+
+extern "C" const char* fsx_last_error(void) { return g_err.c_str(); }
+extern "C" const char* fsx_version(void) { return "fastscape-mi355x 0.1"; }
+
+template <typename T>
+static int dalloc(fsx_ctx* c, T** p, long long n) {
+  if (n <= 0) n = 1;
+  void* q = nullptr;
+  const size_t bytes = sizeof(T) * (size_t)n;
+  hipError_t e = hipMalloc(&q, bytes);
+  if (e != hipSuccess) return fail(FSX_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  e = hipMemset(q, 0, bytes);
+  if (e != hipSuccess) return fail(FSX_EHIP, std::string("hipMemset: ") + hipGetErrorString(e));
+  c->allocs.push_back(q);
+  c->bytes += (long long)bytes;
+  *p = (T*)q;
+  return 0;
+}
+
+// SetBC, src/FastScape_ctx.f90:565-593: write(cbc,'(i4)') ibc; digits 4/2/1/3 = left/right/bottom/top.
+static void decode_bc(InstParams& P, int ibc) {
+  char tmp[32];
+  snprintf(tmp, sizeof tmp, "%4d", ibc);
+  if (strlen(tmp) > 4) strcpy(tmp, "****");
+  P.fix_left = tmp[3] == '1';
+  P.fix_right = tmp[1] == '1';
+  P.fix_bottom = tmp[0] == '1';
+  P.fix_top = tmp[2] == '1';
+  P.i1 = P.fix_left ? 1 : 0;
+  P.i2 = P.fix_right ? P.nx - 2 : P.nx - 1;
+  P.j1 = P.fix_bottom ? 1 : 0;
+  P.j2 = P.fix_top ? P.ny - 2 : P.ny - 1;
+  P.xcyc = !P.fix_left && !P.fix_right;
+  P.ycyc = !P.fix_bottom && !P.fix_top;
+}
+
+// dx = xl/(nx-1), dy = yl/(ny-1) (src/FlowRouting.f90:52-53); neighbour distances
+// sqrt((dx*ii)**2+(dy*jj)**2) (:347) evaluated here once, on the host's correctly rounded sqrt.
+static void geometry(InstParams& P, double xl, double yl) {
+  P.dx = xl / (double)(P.nx - 1);
+  P.dy = yl / (double)(P.ny - 1);
+  for (int jj = -1; jj <= 1; jj++)
+    for (int ii = -1; ii <= 1; ii++) {
+      const double a = P.dx * (double)ii, b = P.dy * (double)jj;
+      P.len[(ii + 1) + 3 * (jj + 1)] = std::sqrt(a * a + b * b);
+    }
+  P.dxdy = P.dx * P.dy;
+}
+
+static int check_inst(fsx_ctx* c, int inst) {
+  if (!c) return fail(FSX_EINVAL, "null context");
+  if (inst < 0 || inst >= c->S.ninst) return fail(FSX_EINVAL, "instance id out of range");
+  return 0;
+}
+
+static int upload_params(fsx_ctx* c) {
+  if (!c->dirty) return 0;
+  HIPCHK(hipMemcpyAsync(c->S.P, c->hp.data(), sizeof(InstParams) * c->hp.size(), hipMemcpyHostToDevice, c->stream));
+  c->dirty = false;
+  return 0;
+}
+
+static int upload_bc(fsx_ctx* c, int inst) {
+  const InstParams& P = c->hp[inst];
+  std::vector<uint8_t> bc((size_t)P.nn, 0);
+  for (int j = 0; j < P.ny; j++)
+    for (int i = 0; i < P.nx; i++) {
+      const bool f = (P.fix_left && i == 0) || (P.fix_right && i == P.nx - 1) || (P.fix_bottom && j == 0) ||
+                     (P.fix_top && j == P.ny - 1);
+      bc[(size_t)j * P.nx + i] = f;
+    }
+  HIPCHK(hipMemcpy(c->S.bc + P.off, bc.data(), bc.size(), hipMemcpyHostToDevice));
+  return 0;
+}
+
+extern "C" int fsx_create(int ninst, const int* nx, const int* ny, fsx_ctx** out) {
+  if (!out || ninst <= 0 || !nx || !ny) return fail(FSX_EINVAL, "fsx_create: bad arguments");
+  *out = nullptr;
+  fsx_ctx* c = new fsx_ctx();
+  DevState& S = c->S;
+  S.ninst = ninst;
+  c->hp.resize(ninst);
+  c->steps.assign(ninst, 0);
+  c->xl.assign(ninst, 0.0);
+  c->yl.assign(ninst, 0.0);
+  c->ibc.assign(ninst, 1111);
+  long long off = 0;
+  std::vector<Chunk> chunks;
+  c->chunk_base.push_back(0);
+  for (int k = 0; k < ninst; k++) {
+    if (nx[k] < 3 || ny[k] < 3) {
+      delete c;
+      return fail(FSX_EINVAL, "fsx_create: nx and ny must be >= 3");
+    }
+    InstParams& P = c->hp[k];
+    memset(&P, 0, sizeof P);
+    P.nx = nx[k];
+    P.ny = ny[k];
+    P.nn = nx[k] * ny[k];
+    P.off = (int)off;
+    P.active = 1;
+    P.newton_tol = 1.0e-3;
+    P.p = 1.0;
+    decode_bc(P, 1111);
+    geometry(P, (double)(P.nx - 1), (double)(P.ny - 1));
+    for (int s = 0; s < P.nn; s += FSX_CHUNK) chunks.push_back(Chunk{k, s});
+    c->chunk_base.push_back((int)chunks.size());
+    off += P.nn;
+    S.max_nn = std::max(S.max_nn, P.nn);
+    S.max_nx = std::max(S.max_nx, P.nx);
+    S.max_ny = std::max(S.max_ny, P.ny);
+  }
+  if (off > 300000000LL) {
+    delete c;
+    return fail(FSX_EINVAL, "fsx_create: batch too large");
+  }
+  S.ntot = off;
+  S.nchunks = (int)chunks.size();
+  const long long N = off;
+  int rc = 0;
+  hipError_t he = hipStreamCreate(&c->stream);
+  if (he != hipSuccess) { delete c; return fail(FSX_EHIP, "hipStreamCreate failed"); }
+#define A(p, n) if (!rc) rc = dalloc(c, &S.p, (n))
+  A(P, ninst); A(chunks, S.nchunks);
+  A(h, N); A(b, N); A(u, N); A(kf, N); A(kd, N); A(precip, N); A(etot, N); A(erate, N); A(area, N);
+  A(length, N); A(hwater, N); A(lake_depth, N); A(sedflux, N); A(ht, N); A(water, N); A(zint, N); A(zintp, N);
+  A(gam, N); A(dh, N); A(hp, N); A(lwv, N); A(lsed, N); A(kfint, N); A(kdint, N); A(g, N); A(pexp, N); A(elev, N);
+  A(rec, N); A(lbl, N); A(basin, N); A(rootrank, N); A(order, N); A(lstart, N + ninst); A(lsill, N); A(stack, N);
+  A(dmask, N); A(bc, N);
+  A(outlet, N); A(bsize, N); A(soff, N); A(ecount, N); A(eoff, N); A(comp, N); A(bpar, N); A(bnew, N); A(best, N);
+  A(tdeg, N); A(toff, N); A(tadj, 2 * N); A(bfront, N);
+  A(cb0, FSX_EDGE_CAP * N); A(cb1, FSX_EDGE_CAP * N); A(cn0, FSX_EDGE_CAP * N); A(cn1, FSX_EDGE_CAP * N);
+  A(sid, FSX_EDGE_CAP * N); A(npos, FSX_EDGE_CAP * N); A(Lasc, FSX_EDGE_CAP * N); A(Rasc, FSX_EDGE_CAP * N);
+  A(segs, FSX_EDGE_CAP * N); A(cw, FSX_EDGE_CAP * N); A(skey, FSX_EDGE_CAP * N); A(intree, FSX_EDGE_CAP * N);
+  A(sB, FSX_SLAB_CAP * N); A(sp, FSX_SLAB_CAP * N); A(sq, FSX_SLAB_CAP * N); A(sw, FSX_SLAB_CAP * N);
+  A(nbasins, ninst); A(basin0, ninst); A(nconn, ninst); A(nlev, ninst); A(err, ninst); A(ngs, ninst);
+  A(ntree, ninst); A(chunkcnt, S.nchunks); A(newton, ninst); A(scal, ninst); A(flags, 64); A(cbase, ninst + 1);
+  A(nsegs, ninst); A(segctr, ninst);
+#undef A
+  if (rc) { fsx_destroy(c); return rc; }
+  he = hipMemcpy(S.chunks, chunks.data(), sizeof(Chunk) * chunks.size(), hipMemcpyHostToDevice);
+  if (he == hipSuccess)
+    he = hipMemcpy(S.cbase, c->chunk_base.data(), sizeof(int) * c->chunk_base.size(), hipMemcpyHostToDevice);
+  if (he != hipSuccess) { fsx_destroy(c); return fail(FSX_EHIP, "upload of the chunk table failed"); }
+  // SetUp defaults (src/FastScape_ctx.f90:81-97): precip = 1, BC 1111
+  std::vector<double> ones((size_t)S.max_nn, 1.0);
+  for (int k = 0; k < ninst; k++) {
+    const InstParams& P = c->hp[k];
+    he = hipMemcpy(S.precip + P.off, ones.data(), sizeof(double) * P.nn, hipMemcpyHostToDevice);
+    if (he != hipSuccess) { fsx_destroy(c); return fail(FSX_EHIP, "precip upload failed"); }
+    if ((rc = upload_bc(c, k))) { fsx_destroy(c); return rc; }
+  }
+  *out = c;
+  return 0;
+}
+
+extern "C" int fsx_destroy(fsx_ctx* c) {
+  if (!c) return 0;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (void* p : c->allocs) (void)hipFree(p);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return 0;
+}
+
+extern "C" int fsx_set_xl_yl(fsx_ctx* c, int inst, double xl, double yl) {
+  if (int r = check_inst(c, inst)) return r;
+  c->xl[inst] = xl;
+  c->yl[inst] = yl;
+  geometry(c->hp[inst], xl, yl);
+  c->dirty = true;
+  return 0;
+}
+
+extern "C" int fsx_set_dt(fsx_ctx* c, int inst, double dt) {
+  if (int r = check_inst(c, inst)) return r;
+  c->hp[inst].dt = dt;
+  c->dirty = true;
+  return 0;
+}
+
+extern "C" int fsx_set_bc(fsx_ctx* c, int inst, int ibc) {
+  if (int r = check_inst(c, inst)) return r;
+  c->ibc[inst] = ibc;
+  decode_bc(c->hp[inst], ibc);
+  c->dirty = true;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return upload_bc(c, inst);
+}
+
+static int put(fsx_ctx* c, double* dst, const double* src, int n, int on_device) {
+  HIPCHK(hipMemcpyAsync(dst, src, sizeof(double) * (size_t)n, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                        c->stream));
+  if (!on_device) HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+extern "C" int fsx_set_erosional_parameters(fsx_ctx* c, int inst, const double* kf, double kfsed, double m, double n,
+                                            const double* kd, double kdsed, double g1, double g2, double p,
+                                            int on_device) {
+  if (int r = check_inst(c, inst)) return r;
+  if (!kf || !kd) return fail(FSX_EINVAL, "kf and kd are required");
+  InstParams& P = c->hp[inst];
+  if (int r = put(c, c->S.kf + P.off, kf, P.nn, on_device)) return r;
+  if (int r = put(c, c->S.kd + P.off, kd, P.nn, on_device)) return r;
+  P.run_spl = 1;
+  P.kfsed = kfsed;
+  P.m = m;
+  P.n = n;
+  P.kdsed = kdsed;
+  P.g1 = g1;
+  P.g2 = g2;
+  P.p = p;
+  P.sfd = 0;
+  if (p < -1.5) {
+    P.sfd = 1;
+    P.p = 1.0;
+  }
+  // runDiffusion if maxval(kd) > tiny or kdsed > tiny (sticky, as the reference)
+  double kdmax = -1.0e308;
+  if (on_device) {
+    std::vector<double> tmp((size_t)P.nn);
+    HIPCHK(hipMemcpy(tmp.data(), kd, sizeof(double) * P.nn, hipMemcpyDeviceToHost));
+    for (double x : tmp) kdmax = std::max(kdmax, x);
+  } else {
+    for (int i = 0; i < P.nn; i++) kdmax = std::max(kdmax, kd[i]);
+  }
+  const double tiny = 2.2250738585072014e-308;
+  if (kdmax > tiny || kdsed > tiny) P.run_diff = 1;
+  if (!P.sfd) return fail(FSX_EINVAL, "multiple flow direction (p >= -1.5) is not available in this build");
+  c->dirty = true;
+  return 0;
+}
+
+extern "C" int fsx_set_newton_tol(fsx_ctx* c, int inst, double tol) {
+  if (int r = check_inst(c, inst)) return r;
+  if (!(tol > 0.0)) return fail(FSX_EINVAL, "Newton tolerance must be > 0");
+  c->hp[inst].newton_tol = tol;
+  c->dirty = true;
+  return 0;
+}
+
+extern "C" int fsx_init_h(fsx_ctx* c, int inst, const double* h, int on_device) {
+  if (int r = check_inst(c, inst)) return r;
+  if (!h) return fail(FSX_EINVAL, "null h");
+  const InstParams& P = c->hp[inst];
+  if (int r = put(c, c->S.h + P.off, h, P.nn, on_device)) return r;
+  return put(c, c->S.b + P.off, h, P.nn, on_device);
+}
+
+extern "C" int fsx_set_field(fsx_ctx* c, int inst, int field, const double* src, int on_device) {
+  if (int r = check_inst(c, inst)) return r;
+  if (!src) return fail(FSX_EINVAL, "null source");
+  InstParams& P = c->hp[inst];
+  double* dst = nullptr;
+  switch (field) {
+    case FSX_H: dst = c->S.h; break;
+    case FSX_B: dst = c->S.b; break;
+    case FSX_U: dst = c->S.u; P.run_uplift = 1; c->dirty = true; break;
+    case FSX_PRECIP: dst = c->S.precip; break;
+    default: return fail(FSX_EINVAL, "field cannot be set");
+  }
+  return put(c, dst + P.off, src, P.nn, on_device);
+}
+
+__global__ void k_add_layers(double* __restrict__ h, double* __restrict__ b, const double* __restrict__ dh, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    h[i] = h[i] + dh[i];
+    b[i] = b[i] + dh[i];
+  }
+}
+
+extern "C" int fsx_set_all_layers(fsx_ctx* c, int inst, const double* dh, int on_device) {
+  if (int r = check_inst(c, inst)) return r;
+  if (!dh) return fail(FSX_EINVAL, "null dh");
+  const InstParams& P = c->hp[inst];
+  if (int r = put(c, c->S.zint + P.off, dh, P.nn, on_device)) return r;
+  k_add_layers<<<(P.nn + 255) / 256, 256, 0, c->stream>>>(c->S.h + P.off, c->S.b + P.off, c->S.zint + P.off, P.nn);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+extern "C" int fsx_reset_cumulative_erosion(fsx_ctx* c, int inst) {
+  if (int r = check_inst(c, inst)) return r;
+  const InstParams& P = c->hp[inst];
+  HIPCHK(hipMemsetAsync(c->S.etot + P.off, 0, sizeof(double) * P.nn, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+extern "C" int fsx_set_active(fsx_ctx* c, int inst, int active) {
+  if (int r = check_inst(c, inst)) return r;
+  c->hp[inst].active = active ? 1 : 0;
+  c->dirty = true;
+  return 0;
+}
+
+static int check_errors(fsx_ctx* c) {
+  std::vector<int> e(c->S.ninst);
+  HIPCHK(hipMemcpyAsync(e.data(), c->S.err, sizeof(int) * e.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int k = 0; k < c->S.ninst; k++) {
+    if (!e[k]) continue;
+    HIPCHK(hipMemset(c->S.err, 0, sizeof(int) * e.size()));
+    std::string why;
+    if (e[k] & FSX_ERR_NCONN) why += " basin graph exceeds 6*nbasins edges;";
+    if (e[k] & FSX_ERR_NO_OUTLET) why += " no basin drains to a fixed boundary node;";
+    if (e[k] & FSX_ERR_TRIDAG) why += " zero pivot in the diffusion solve;";
+    if (e[k] & FSX_ERR_NEWTON) why += " Newton iteration did not converge;";
+    if (e[k] & FSX_ERR_SORTSTACK) why += " edge-sort segment stack overflow;";
+    if (e[k] & FSX_ERR_LEVELS) why += " receiver graph is not a forest;";
+    if (e[k] & FSX_ERR_MFD) why += " multiple-flow order incomplete;";
+    return fail(FSX_ESTEP, "instance " + std::to_string(k) + ":" + why);
+  }
+  return 0;
+}
+
+static void enqueue_routing(fsx_ctx* c, hipStream_t st) {
+  DevState& S = c->S;
+  launch_receivers(S, st);
+  launch_donors(S, st);
+  launch_local_minima(S, st);
+  launch_donors(S, st);
+  launch_levels_fill(S, st);
+  launch_area(S, st);
+}
+
+static void enqueue_step(fsx_ctx* c, hipStream_t st) {
+  DevState& S = c->S;
+  launch_uplift(S, st);
+  enqueue_routing(c, st);
+  launch_spl_prep(S, st);
+  launch_spl(S, st);
+  launch_spl_post(S, st);
+  launch_diffusion(S, st);
+}
+
+extern "C" int fsx_execute_steps(fsx_ctx* c, int nsteps, void* stream, int sync) {
+  if (!c) return fail(FSX_EINVAL, "null context");
+  if (nsteps < 0) return fail(FSX_EINVAL, "nsteps < 0");
+  hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+  if (stream && c->dirty) HIPCHK(hipStreamSynchronize(c->stream));
+  if (int r = upload_params(c)) return r;
+  if (stream) HIPCHK(hipStreamSynchronize(c->stream));
+  for (int s = 0; s < nsteps; s++) enqueue_step(c, st);
+  HIPCHK(hipGetLastError());
+  for (int k = 0; k < c->S.ninst; k++)
+    if (c->hp[k].active) c->steps[k] += nsteps;
+  if (sync) {
+    HIPCHK(hipStreamSynchronize(st));
+    return check_errors(c);
+  }
+  return 0;
+}
+
+extern "C" int fsx_synchronize(fsx_ctx* c) {
+  if (!c) return fail(FSX_EINVAL, "null context");
+  HIPCHK(hipDeviceSynchronize());
+  return check_errors(c);
+}
+
+extern "C" int fsx_route(fsx_ctx* c, void* stream, int sync) {
+  if (!c) return fail(FSX_EINVAL, "null context");
+  hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+  if (int r = upload_params(c)) return r;
+  if (stream) HIPCHK(hipStreamSynchronize(c->stream));
+  enqueue_routing(c, st);
+  HIPCHK(hipGetLastError());
+  if (sync) {
+    HIPCHK(hipStreamSynchronize(st));
+    return check_errors(c);
+  }
+  return 0;
+}
+
+extern "C" int fsx_copy_field(fsx_ctx* c, int inst, int field, double* dst, int on_device) {
+  if (int r = check_inst(c, inst)) return r;
+  if (!dst) return fail(FSX_EINVAL, "null destination");
+  const InstParams& P = c->hp[inst];
+  const double* src = nullptr;
+  switch (field) {
+    case FSX_H: src = c->S.h; break;
+    case FSX_B: src = c->S.b; break;
+    case FSX_U: src = c->S.u; break;
+    case FSX_PRECIP: src = c->S.precip; break;
+    case FSX_ETOT: src = c->S.etot; break;
+    case FSX_ERATE: src = c->S.erate; break;
+    case FSX_AREA: src = c->S.area; break;
+    case FSX_LAKE_DEPTH: src = c->S.lake_depth; break;
+    case FSX_SEDFLUX: src = c->S.sedflux; break;
+    case FSX_LENGTH: src = c->S.length; break;
+    case FSX_HWATER: src = c->S.hwater; break;
+    case FSX_KF: src = c->S.kf; break;
+    case FSX_KD: src = c->S.kd; break;
+    default: return fail(FSX_EINVAL, "unknown field");
+  }
+  HIPCHK(hipMemcpyAsync(dst, src + P.off, sizeof(double) * P.nn, on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+__global__ void k_expand_donors(const InstParams* __restrict__ PP, int inst, const uint8_t* __restrict__ dmask,
+                                int* __restrict__ ndon, int* __restrict__ don) {
+  const InstParams& P = PP[inst];
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= P.nn) return;
+  int d[8];
+  const int nd = donors_sorted(P, v, dmask[P.off + v], d);
+  if (ndon) ndon[v] = nd;
+  if (don)
+    for (int k = 0; k < 8; k++) don[8 * v + k] = k < nd ? d[k] : -1;
+}
+
+extern "C" int fsx_copy_ifield(fsx_ctx* c, int inst, int field, int* dst, int on_device) {
+  if (int r = check_inst(c, inst)) return r;
+  if (!dst) return fail(FSX_EINVAL, "null destination");
+  const InstParams& P = c->hp[inst];
+  const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  int* tmp = c->S.sB;  // 8*ntot ints of scratch, free outside a step
+  switch (field) {
+    case FSX_REC:
+      HIPCHK(hipMemcpyAsync(dst, c->S.rec + P.off, sizeof(int) * P.nn, kind, c->stream));
+      break;
+    case FSX_ORDER:
+      HIPCHK(hipMemcpyAsync(dst, c->S.order + P.off, sizeof(int) * P.nn, kind, c->stream));
+      break;
+    case FSX_NDON:
+      k_expand_donors<<<(P.nn + 255) / 256, 256, 0, c->stream>>>(c->S.P, inst, c->S.dmask, tmp, nullptr);
+      HIPCHK(hipMemcpyAsync(dst, tmp, sizeof(int) * P.nn, kind, c->stream));
+      break;
+    case FSX_DON:
+      k_expand_donors<<<(P.nn + 255) / 256, 256, 0, c->stream>>>(c->S.P, inst, c->S.dmask, nullptr, tmp);
+      HIPCHK(hipMemcpyAsync(dst, tmp, sizeof(int) * 8 * (size_t)P.nn, kind, c->stream));
+      break;
+    default:
+      return fail(FSX_EINVAL, "unknown integer field");
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+extern "C" int fsx_get_step(fsx_ctx* c, int inst, int* step) {
+  if (int r = check_inst(c, inst)) return r;
+  if (step) *step = c->steps[inst];
+  return 0;
+}
+
+extern "C" int fsx_get_sizes(fsx_ctx* c, int inst, int* nx, int* ny) {
+  if (int r = check_inst(c, inst)) return r;
+  if (nx) *nx = c->hp[inst].nx;
+  if (ny) *ny = c->hp[inst].ny;
+  return 0;
+}
+
+extern "C" int fsx_get_stats(fsx_ctx* c, int inst, long long* out) {
+  if (int r = check_inst(c, inst)) return r;
+  int v[5];
+  long long nw = 0;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(&v[0], c->S.nbasins + inst, sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(&v[1], c->S.nconn + inst, sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(&v[2], c->S.ntree + inst, sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(&v[3], c->S.nlev + inst, sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(&v[4], c->S.ngs + inst, sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(&nw, c->S.newton + inst, sizeof(long long), hipMemcpyDeviceToHost));
+  for (int k = 0; k < 5; k++) out[k] = v[k];
+  out[5] = nw;
+  return 0;
+}
+
+extern "C" int fsx_memory(fsx_ctx* c, long long* bytes, long long* per_node) {
+  if (!c) return fail(FSX_EINVAL, "null context");
+  if (bytes) *bytes = c->bytes;
+  if (per_node) *per_node = c->S.ntot ? c->bytes / c->S.ntot : 0;
+  return 0;
+}
+
+extern "C" int fsx_indexx(int n, const double* keys, int* perm) {
+  if (n < 0 || (n > 0 && (!keys || !perm))) return fail(FSX_EINVAL, "fsx_indexx: bad arguments");
+  if (n == 0) return 0;
+  // a one-landscape context sized so that its edge capacity holds n keys
+  const int nn = (n + FSX_EDGE_CAP - 1) / FSX_EDGE_CAP + 3;
+  int nx = 3, ny = std::max(3, (nn + 2) / 3);
+  fsx_ctx* c = nullptr;
+  if (int r = fsx_create(1, &nx, &ny, &c)) return r;
+  c->hp[0].run_spl = 1;
+  c->dirty = true;
+  int rc = upload_params(c);
+  hipError_t e = hipSuccess;
+  if (!rc) e = hipMemcpy(c->S.cw, keys, sizeof(double) * n, hipMemcpyHostToDevice);
+  if (!rc && e == hipSuccess) e = hipMemcpy(c->S.nconn, &n, sizeof(int), hipMemcpyHostToDevice);
+  if (!rc && e == hipSuccess) {
+    launch_nr_sort(c->S, c->stream);
+    e = hipGetLastError();
+  }
+  std::vector<int> sid(n);
+  if (!rc && e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (!rc && e == hipSuccess) e = hipMemcpy(sid.data(), c->S.sid, sizeof(int) * n, hipMemcpyDeviceToHost);
+  if (!rc && e != hipSuccess) rc = fail(FSX_EHIP, std::string("fsx_indexx: ") + hipGetErrorString(e));
+  if (!rc) rc = check_errors(c);
+  if (!rc)
+    for (int p = 0; p < n; p++) perm[p] = sid[p] + 1;
+  fsx_destroy(c);
+  return rc;
+}

@@ -1,0 +1,760 @@
+// fsx_basins.hip -- closed-basin rerouting (LocalMinima, src/LocalMinima.f90:1-132) on the GPU.
+//
+//  1. basin labels: root of every node by pointer jumping; basin id = rank of the root among the
+//     roots in ascending index (compute_basins numbers basins in stack order, whose roots come in
+//     ascending index, :136-169).
+//  2. passes: one thread per closed basin walks its nodes in the reference's depth-first preorder
+//     (stackless: donors ascending, climbing back through receivers) and scans the 8 neighbours in
+//     the order of :239-240, keeping per neighbour basin the first strictly lowest pass (:284-317).
+//     Edge order = basins in id order, each basin's edges in discovery order, one virtual edge
+//     (-1e10) per open basin other than basin0 at that basin's place (:263-276).
+//  3. edge order of loc_min_3_indexx (:947-1046), reproduced exactly by a parallel form of its
+//     partition step (below), including the order it gives equal weights.
+//  4. Boruvka minimum spanning forest keyed by that order == Kruskal in that order (:333-385).
+//  5. orientation from basin0 (order_tree, :389-533) by a breadth-first walk of the tree.
+//  6. receiver path reversal per tree edge (correct_receivers_v2, :601-662), one thread per edge;
+//     paths lie in disjoint basins, so edges are independent.
+#include <climits>
+
+#include "fsx_scan.h"
+#include "fsx_state.h"
+
+#define NODE_PROLOGUE                        \
+  const Chunk c = chunks[blockIdx.x];        \
+  const InstParams& P = PP[c.inst];          \
+  const int v = c.start + (int)threadIdx.x;  \
+  const bool live = P.active && P.run_spl && v < P.nn;
+
+__global__ void k_fill_int(int* __restrict__ a, long long n, int val) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    a[i] = val;
+}
+
+// ---- 1. labels ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(FSX_CHUNK) void k_lbl_init(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
+                                                        const int* __restrict__ rec, int* __restrict__ lbl) {
+  NODE_PROLOGUE
+  if (!live) return;
+  lbl[P.off + v] = rec[P.off + v];
+}
+
+__global__ __launch_bounds__(FSX_CHUNK) void k_lbl_jump(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
+                                                        int* __restrict__ lbl, int* __restrict__ flags, int iter) {
+  if (iter > 0 && flags[iter - 1] == 0) return;
+  NODE_PROLOGUE
+  if (!live) return;
+  int* L = lbl + P.off;
+  const int x = L[v];
+  const int y = L[x];
+  if (x != y) {
+    L[v] = y;
+    flags[iter] = 1;
+  }
+}
+
+// roots per chunk
+__global__ __launch_bounds__(FSX_CHUNK) void k_root_count(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
+                                                          const int* __restrict__ rec, int* __restrict__ chunkcnt) {
+  __shared__ int lds[FSX_CHUNK / 64];
+  NODE_PROLOGUE
+  const int f = (live && rec[P.off + v] == v) ? 1 : 0;
+  const int t = block_sum<FSX_CHUNK>(f, lds);
+  if (threadIdx.x == 0) chunkcnt[blockIdx.x] = t;
+}
+
+// exclusive scan of the per-chunk root counts of each landscape; nbasins; basin0 reset
+__global__ __launch_bounds__(FSX_WG) void k_root_scan(const InstParams* __restrict__ PP, int* __restrict__ chunkcnt,
+                                                      const int* __restrict__ cbase, int* __restrict__ nbasins,
+                                                      int* __restrict__ basin0) {
+  __shared__ int lds[FSX_WG / 64];
+  const int inst = blockIdx.x;
+  const InstParams& P = PP[inst];
+  if (!P.active || !P.run_spl) return;
+  const int c0 = cbase[inst], c1 = cbase[inst + 1];
+  int carry = 0;
+  for (int b = c0; b < c1; b += FSX_WG) {
+    const int i = b + threadIdx.x;
+    const int x = i < c1 ? chunkcnt[i] : 0;
+    int tot;
+    const int incl = block_incl_scan<FSX_WG>(x, lds, tot);
+    if (i < c1) chunkcnt[i] = carry + incl - x;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) {
+    nbasins[inst] = carry;
+    basin0[inst] = INT_MAX;
+  }
+}
+
+__global__ __launch_bounds__(FSX_CHUNK) void k_root_rank(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
+                                                         const int* __restrict__ rec, const uint8_t* __restrict__ bc,
+                                                         const int* __restrict__ chunkcnt, int* __restrict__ rootrank,
+                                                         int* __restrict__ outlet, int* __restrict__ basin0) {
+  __shared__ int lds[FSX_CHUNK / 64];
+  NODE_PROLOGUE
+  const int f = (live && rec[P.off + v] == v) ? 1 : 0;
+  int tot;
+  const int incl = block_incl_scan<FSX_CHUNK>(f, lds, tot);
+  if (f) {
+    const int r = chunkcnt[blockIdx.x] + incl - 1;
+    rootrank[P.off + v] = r;
+    outlet[P.off + r] = v;
+    if (bc[P.off + v]) atomicMin(basin0 + c.inst, r);
+  }
+}
+
+__global__ __launch_bounds__(FSX_CHUNK) void k_basin_label(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
+                                                           const int* __restrict__ lbl, const int* __restrict__ rootrank,
+                                                           const uint8_t* __restrict__ bc, int* __restrict__ basin,
+                                                           int* __restrict__ bsize) {
+  NODE_PROLOGUE
+  if (!live) return;
+  const int r = lbl[P.off + v];
+  const int b = rootrank[P.off + r];
+  basin[P.off + v] = b;
+  if (!bc[P.off + r]) atomicAdd(bsize + P.off + b, 1);
+}
+
+// exclusive scan of closed-basin sizes -> candidate slab offsets; clears edge counts
+__global__ __launch_bounds__(FSX_WG) void k_bsize_scan(const InstParams* __restrict__ PP, const int* __restrict__ nbasins,
+                                                       const int* __restrict__ bsize, int* __restrict__ soff,
+                                                       int* __restrict__ ecount) {
+  __shared__ int lds[FSX_WG / 64];
+  const int inst = blockIdx.x;
+  const InstParams& P = PP[inst];
+  if (!P.active || !P.run_spl) return;
+  const int nb = nbasins[inst];
+  int carry = 0;
+  for (int b0 = 0; b0 < nb; b0 += FSX_WG) {
+    const int b = b0 + threadIdx.x;
+    const int x = b < nb ? bsize[P.off + b] : 0;
+    int tot;
+    const int incl = block_incl_scan<FSX_WG>(x, lds, tot);
+    if (b < nb) {
+      soff[P.off + b] = carry + incl - x;
+      ecount[P.off + b] = 0;
+    }
+    carry += tot;
+  }
+}
+
+// ---- 2. passes (connect_basins) -------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_basin_walk(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
+                                                   const int* __restrict__ nbasins, const int* __restrict__ outlet,
+                                                   const uint8_t* __restrict__ bc, const int* __restrict__ rec,
+                                                   const uint8_t* __restrict__ dmask, const int* __restrict__ basin,
+                                                   const double* __restrict__ h, const int* __restrict__ soff,
+                                                   const int* __restrict__ bsize, int* __restrict__ ecount,
+                                                   int* __restrict__ sB, int* __restrict__ sp, int* __restrict__ sq,
+                                                   double* __restrict__ sw, int* __restrict__ err) {
+  const Chunk c = chunks[blockIdx.x >> 2];
+  const InstParams& P = PP[c.inst];
+  const int b = c.start + (int)((blockIdx.x & 3) * 64 + threadIdx.x);
+  if (!P.active || !P.run_spl || b >= nbasins[c.inst]) return;
+  const int off = P.off, nx = P.nx, ny = P.ny;
+  const int r = outlet[off + b];
+  if (bc[off + r]) return;  // open basin: no scan
+  const int* R = rec + off;
+  const uint8_t* M = dmask + off;
+  const int* BS = basin + off;
+  const int* OUT = outlet + off;
+  const uint8_t* BC = bc + off;
+  const double* H = h + off;
+  const long long sbase = (long long)FSX_SLAB_CAP * (off + soff[off + b]);
+  int* TB = sB + sbase;
+  int* TP = sp + sbase;
+  int* TQ = sq + sbase;
+  double* TW = sw + sbase;
+  constexpr int II[8] = {-1, 1, 0, 0, -1, 1, -1, 1};
+  constexpr int JJ[8] = {0, 0, -1, 1, -1, -1, 1, 1};
+  int ne = 0;
+  int v = r;
+  const int cap = FSX_SLAB_CAP * bsize[off + b];
+  for (int visits = 0;; visits++) {
+    if (visits > P.nn) { atomicOr(err + c.inst, FSX_ERR_LEVELS); break; }
+    // scan the 8 neighbours of v (no wrap), src/LocalMinima.f90:281-320
+    const int i = v % nx, j = v / nx;
+    const double hv = H[v];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const int ki = i + II[k], kj = j + JJ[k];
+      if (ki < 0 || ki >= nx || kj < 0 || kj >= ny) continue;
+      const int q = kj * nx + ki;
+      const int B = BS[q];
+      if (b >= B && !BC[OUT[B]]) continue;
+      const double hq = H[q];
+      const double w = hv > hq ? hv : hq;
+      int t = 0;
+      while (t < ne && TB[t] != B) t++;
+      if (t == ne) {
+        if (ne >= cap) { atomicOr(err + c.inst, FSX_ERR_NCONN); continue; }
+        TB[t] = B; TP[t] = v; TQ[t] = q; TW[t] = w;
+        ne++;
+      } else if (w < TW[t]) {
+        TP[t] = v; TQ[t] = q; TW[t] = w;
+      }
+    }
+    // next node in depth-first preorder: first donor, else next sibling up the receiver chain
+    int d[8];
+    if (donors_sorted(P, v, M[v], d) > 0) { v = d[0]; continue; }
+    bool found = false;
+    while (v != r) {
+      const int p = R[v];
+      const int np = donors_sorted(P, p, M[p], d);
+      int nxt = -1;
+      for (int k = 0; k < np; k++)
+        if (d[k] > v) { nxt = d[k]; break; }
+      if (nxt >= 0) { v = nxt; found = true; break; }
+      v = p;
+    }
+    if (!found) break;
+  }
+  ecount[off + b] = ne;
+}
+
+// edge offsets: basin order, open basins (except basin0) one virtual edge each
+__global__ __launch_bounds__(FSX_WG) void k_edge_scan(const InstParams* __restrict__ PP, const int* __restrict__ nbasins,
+                                                      const int* __restrict__ basin0, const int* __restrict__ outlet,
+                                                      const uint8_t* __restrict__ bc, const int* __restrict__ ecount,
+                                                      int* __restrict__ eoff, int* __restrict__ nconn, int* __restrict__ err) {
+  __shared__ int lds[FSX_WG / 64];
+  const int inst = blockIdx.x;
+  const InstParams& P = PP[inst];
+  if (!P.active || !P.run_spl) return;
+  const int nb = nbasins[inst], b0 = basin0[inst];
+  int carry = 0;
+  for (int s = 0; s < nb; s += FSX_WG) {
+    const int b = s + threadIdx.x;
+    int x = 0;
+    if (b < nb) {
+      const bool open = bc[P.off + outlet[P.off + b]];
+      x = open ? (b == b0 ? 0 : 1) : ecount[P.off + b];
+    }
+    int tot;
+    const int incl = block_incl_scan<FSX_WG>(x, lds, tot);
+    if (b < nb) eoff[P.off + b] = carry + incl - x;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) {
+    if (b0 == INT_MAX) {
+      atomicOr(err + inst, FSX_ERR_NO_OUTLET);
+      nconn[inst] = 0;
+    } else if (carry > FSX_EDGE_CAP * nb) {
+      atomicOr(err + inst, FSX_ERR_NCONN);
+      nconn[inst] = 0;
+    } else {
+      nconn[inst] = carry;
+    }
+  }
+}
+
+__global__ __launch_bounds__(FSX_CHUNK) void k_edge_fill(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
+                                                         const int* __restrict__ nbasins, const int* __restrict__ basin0,
+                                                         const int* __restrict__ nconn, const int* __restrict__ outlet,
+                                                         const uint8_t* __restrict__ bc, const int* __restrict__ ecount,
+                                                         const int* __restrict__ eoff, const int* __restrict__ soff,
+                                                         const int* __restrict__ sB, const int* __restrict__ sp,
+                                                         const int* __restrict__ sq, const double* __restrict__ sw,
+                                                         int* __restrict__ cb0, int* __restrict__ cb1, int* __restrict__ cn0,
+                                                         int* __restrict__ cn1, double* __restrict__ cw) {
+  const Chunk c = chunks[blockIdx.x];
+  const InstParams& P = PP[c.inst];
+  const int b = c.start + (int)threadIdx.x;
+  if (!P.active || !P.run_spl || b >= nbasins[c.inst] || nconn[c.inst] == 0) return;
+  const long long eb = (long long)FSX_EDGE_CAP * P.off;
+  const int e0 = eoff[P.off + b];
+  if (bc[P.off + outlet[P.off + b]]) {
+    if (b != basin0[c.inst]) {
+      cb0[eb + e0] = basin0[c.inst]; cb1[eb + e0] = b;
+      cn0[eb + e0] = -1; cn1[eb + e0] = -1;
+      cw[eb + e0] = -1.0e10;
+    }
+    return;
+  }
+  const long long sbase = (long long)FSX_SLAB_CAP * (P.off + soff[P.off + b]);
+  const int ne = ecount[P.off + b];
+  for (int k = 0; k < ne; k++) {
+    cb0[eb + e0 + k] = b; cb1[eb + e0 + k] = sB[sbase + k];
+    cn0[eb + e0 + k] = sp[sbase + k]; cn1[eb + e0 + k] = sq[sbase + k];
+    cw[eb + e0 + k] = sw[sbase + k];
+  }
+}
+
+// ---- 3. loc_min_3_indexx order --------------------------------------------------------------------
+// One partition step of the Numerical Recipes quicksort on positions [l, ir] (0-based), computed by a
+// group of NT threads.  After the median of three, with pivot a = key[l]:
+//   L = ascending positions p in [l+2, ir]   with key[p] >= a   (stops of the upward scan)
+//   R = descending positions q in [l+1, ir-1] with key[q] <= a   (stops of the downward scan)
+// The scans swap L[t] with R[t] exactly for t = 1..T, T = #{t : L[t] <= R[t]} (a self-swap at
+// L[T] == R[T] included); the crossing indices are then
+//   self-swap: i* = L[T]+1, j* = L[T]-1;  else i* = min(L[T+1], R[T]), j* = max(R[T+1], L[T])
+// with R[0] = +inf and L[0] = l+1; the pivot moves to j* and the element there to l.  Subranges
+// [l, j*-1] and [i*, ir] follow.  Segments of <= 7 keys get the reference's stable insertion sort.
+// Every step is checked against the sequential routine on tie-heavy inputs in the tests.
+#define NR_ELT 8
+#define NR_BIG 2048
+
+template <int NT>
+__device__ __forceinline__ void grp_sync() {
+  if (NT == 64) {
+    __threadfence_block();
+    __builtin_amdgcn_wave_barrier();
+  } else {
+    __syncthreads();
+  }
+}
+
+template <int NT>
+__device__ __forceinline__ int grp_scan(int x, int* lds, int& total) {
+  if (NT == 64) {
+    const int v = wave_incl_scan(x);
+    total = __shfl(v, 63, 64);
+    return v;
+  }
+  return block_incl_scan<NT>(x, lds, total);
+}
+
+template <int NT>
+__device__ __forceinline__ int grp_rank() {
+  return NT == 64 ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
+}
+
+__device__ __forceinline__ void kswap(double* K, int* I, int p, int q) {
+  const double tk = K[p]; K[p] = K[q]; K[q] = tk;
+  const int ti = I[p]; I[p] = I[q]; I[q] = ti;
+}
+
+__device__ void nr_insertion(double* K, int* I, int l, int ir) {
+  for (int j = l + 1; j <= ir; j++) {
+    const double a = K[j];
+    const int ia = I[j];
+    int i = j - 1;
+    while (i >= l && K[i] > a) {
+      K[i + 1] = K[i];
+      I[i + 1] = I[i];
+      i--;
+    }
+    K[i + 1] = a;
+    I[i + 1] = ia;
+  }
+}
+
+// returns (istar, jstar) in every thread of the group; sh must be per-group shared scratch (>= 4 ints)
+template <int NT>
+__device__ void nr_partition(double* K, int* I, int* LA, int* RA, int l, int ir, int* lds, int* sh, double* shd,
+                             int& istar, int& jstar) {
+  const int rk = grp_rank<NT>();
+  if (rk == 0) {
+    const int k = (l + ir) >> 1;
+    kswap(K, I, k, l + 1);
+    if (K[l + 1] > K[ir]) kswap(K, I, l + 1, ir);
+    if (K[l] > K[ir]) kswap(K, I, l, ir);
+    if (K[l + 1] > K[l]) kswap(K, I, l + 1, l);
+    shd[0] = K[l];
+  }
+  grp_sync<NT>();
+  const double a = shd[0];
+  int cL = 0, cR = 0;
+  for (int cb = l + 1; cb <= ir; cb += NT * NR_ELT) {
+    const int p0 = cb + rk * NR_ELT;
+    int fl = 0, fr = 0;  // bit masks over the NR_ELT positions of this thread
+    for (int e = 0; e < NR_ELT; e++) {
+      const int p = p0 + e;
+      if (p <= ir) {
+        const double kp = K[p];
+        if (p >= l + 2 && kp >= a) fl |= 1 << e;
+        if (p <= ir - 1 && kp <= a) fr |= 1 << e;
+      }
+    }
+    const int nl = __popc(fl), nr = __popc(fr);
+    int tot;
+    const int incl = grp_scan<NT>(nl | (nr << 16), lds, tot);
+    int rl = cL + (incl & 0xffff) - nl, rr = cR + (incl >> 16) - nr;
+    for (int e = 0; e < NR_ELT; e++) {
+      if (fl & (1 << e)) LA[l + rl++] = p0 + e;
+      if (fr & (1 << e)) RA[l + rr++] = p0 + e;
+    }
+    cL += tot & 0xffff;
+    cR += tot >> 16;
+  }
+  grp_sync<NT>();
+  const int totR = cR, mm = cL < cR ? cL : cR;
+  int cnt = 0;
+  for (int t = 1 + rk; t <= mm; t += NT) cnt += (LA[l + t - 1] <= RA[l + totR - t]) ? 1 : 0;
+  int T;
+  grp_scan<NT>(cnt, lds, T);
+  for (int t = 1 + rk; t <= T; t += NT) {
+    const int p = LA[l + t - 1], q = RA[l + totR - t];
+    if (p != q) kswap(K, I, p, q);
+  }
+  grp_sync<NT>();
+  if (rk == 0) {
+    int is, js;
+    if (T >= 1 && LA[l + T - 1] == RA[l + totR - T]) {
+      is = LA[l + T - 1] + 1;
+      js = LA[l + T - 1] - 1;
+    } else {
+      is = LA[l + T];
+      js = RA[l + totR - T - 1];
+      if (T >= 1) {
+        const int rt = RA[l + totR - T], lt = LA[l + T - 1];
+        is = is < rt ? is : rt;
+        js = js > lt ? js : lt;
+      }
+    }
+    const double pk = K[l];
+    const int pi = I[l];
+    K[l] = K[js]; I[l] = I[js];
+    K[js] = pk; I[js] = pi;
+    sh[0] = is;
+    sh[1] = js;
+  }
+  grp_sync<NT>();
+  istar = sh[0];
+  jstar = sh[1];
+  grp_sync<NT>();
+}
+
+// Large segments, one workgroup per landscape; segments of <= NR_BIG keys go to a list.
+__global__ __launch_bounds__(FSX_WG) void k_nr_big(const InstParams* __restrict__ PP, const int* __restrict__ nconn,
+                                                   const double* __restrict__ cw, double* __restrict__ skey,
+                                                   int* __restrict__ sid, int* __restrict__ Lasc, int* __restrict__ Rasc,
+                                                   int* __restrict__ segs, int* __restrict__ nsegs, int* __restrict__ err) {
+  __shared__ int lds[FSX_WG / 64];
+  __shared__ int sh[4];
+  __shared__ double shd[1];
+  __shared__ int bst[2 * 128];
+  __shared__ int s_nb, s_ns;
+  const int inst = blockIdx.x;
+  const InstParams& P = PP[inst];
+  if (!P.active || !P.run_spl) return;
+  const int n = nconn[inst];
+  const long long eb = (long long)FSX_EDGE_CAP * P.off;
+  double* K = skey + eb;
+  int* I = sid + eb;
+  int* LA = Lasc + eb;
+  int* RA = Rasc + eb;
+  int* SG = segs + eb;
+  for (int p = threadIdx.x; p < n; p += FSX_WG) { K[p] = cw[eb + p]; I[p] = p; }
+  if (threadIdx.x == 0) {
+    s_nb = 0;
+    s_ns = 0;
+    if (n > NR_BIG) { bst[0] = 0; bst[1] = n - 1; s_nb = 1; }
+    else if (n >= 2) { SG[0] = 0; SG[1] = n - 1; s_ns = 1; }
+  }
+  __syncthreads();
+  while (s_nb > 0) {
+    const int l = bst[2 * (s_nb - 1)], ir = bst[2 * (s_nb - 1) + 1];
+    __syncthreads();
+    int is, js;
+    nr_partition<FSX_WG>(K, I, LA, RA, l, ir, lds, sh, shd, is, js);
+    if (threadIdx.x == 0) {
+      s_nb--;
+      const int a0 = l, a1 = js - 1, b0 = is, b1 = ir;
+      const int segl[2][2] = {{a0, a1}, {b0, b1}};
+      for (int s = 0; s < 2; s++) {
+        const int sl = segl[s][0], sr = segl[s][1], sz = sr - sl + 1;
+        if (sz > NR_BIG) {
+          if (s_nb < 128) { bst[2 * s_nb] = sl; bst[2 * s_nb + 1] = sr; s_nb++; }
+          else atomicOr(err + inst, FSX_ERR_SORTSTACK);
+        } else if (sz >= 2) {
+          SG[2 * s_ns] = sl; SG[2 * s_ns + 1] = sr; s_ns++;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) nsegs[inst] = s_ns;
+}
+
+// Segments of <= NR_BIG keys: each wave takes a segment and finishes it depth first.
+#define NR_WSTACK 64
+__global__ __launch_bounds__(256) void k_nr_small(const InstParams* __restrict__ PP, const int* __restrict__ nsegs,
+                                                  int* __restrict__ segctr, double* __restrict__ skey,
+                                                  int* __restrict__ sid, int* __restrict__ Lasc, int* __restrict__ Rasc,
+                                                  const int* __restrict__ segs, int* __restrict__ err) {
+  __shared__ int sh[4][4];
+  __shared__ double shd[4][1];
+  __shared__ int wst[4][2 * NR_WSTACK];
+  const int inst = blockIdx.y;
+  const InstParams& P = PP[inst];
+  if (!P.active || !P.run_spl) return;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long long eb = (long long)FSX_EDGE_CAP * P.off;
+  double* K = skey + eb;
+  int* I = sid + eb;
+  int* LA = Lasc + eb;
+  int* RA = Rasc + eb;
+  const int* SG = segs + eb;
+  const int ns = nsegs[inst];
+  for (;;) {
+    int s = 0;
+    if (lane == 0) s = atomicAdd(segctr + inst, 1);
+    s = __shfl(s, 0, 64);
+    if (s >= ns) break;
+    int top = 0;
+    if (lane == 0) { wst[w][0] = SG[2 * s]; wst[w][1] = SG[2 * s + 1]; }
+    top = 1;
+    grp_sync<64>();
+    while (top > 0) {
+      top--;
+      const int l = wst[w][2 * top], ir = wst[w][2 * top + 1];
+      grp_sync<64>();
+      if (ir - l < 7) {
+        if (lane == 0) nr_insertion(K, I, l, ir);
+        grp_sync<64>();
+        continue;
+      }
+      int is, js;
+      nr_partition<64>(K, I, LA, RA, l, ir, nullptr, sh[w], shd[w], is, js);
+      if (lane == 0) {
+        if (js - 1 - l + 1 >= 2) {
+          if (top < NR_WSTACK) { wst[w][2 * top] = l; wst[w][2 * top + 1] = js - 1; }
+          else atomicOr(err + inst, FSX_ERR_SORTSTACK);
+        }
+      }
+      if (js - l >= 2) top++;
+      if (lane == 0) {
+        if (ir - is + 1 >= 2) {
+          if (top < NR_WSTACK) { wst[w][2 * top] = is; wst[w][2 * top + 1] = ir; }
+          else atomicOr(err + inst, FSX_ERR_SORTSTACK);
+        }
+      }
+      if (ir - is + 1 >= 2) top++;
+      if (top > NR_WSTACK) top = NR_WSTACK;
+      grp_sync<64>();
+    }
+  }
+}
+
+__global__ __launch_bounds__(FSX_CHUNK) void k_nr_pos(const InstParams* __restrict__ PP, const int* __restrict__ nconn,
+                                                      const int* __restrict__ sid, int* __restrict__ npos) {
+  const int inst = blockIdx.y;
+  const InstParams& P = PP[inst];
+  if (!P.active || !P.run_spl) return;
+  const long long eb = (long long)FSX_EDGE_CAP * P.off;
+  const int n = nconn[inst];
+  for (int p = blockIdx.x * FSX_CHUNK + threadIdx.x; p < n; p += gridDim.x * FSX_CHUNK) npos[eb + sid[eb + p]] = p;
+}
+
+// ---- 4. Boruvka -----------------------------------------------------------------------------------
+__global__ __launch_bounds__(FSX_WG) void k_boruvka(const InstParams* __restrict__ PP, const int* __restrict__ nbasins,
+                                                    const int* __restrict__ nconn, const int* __restrict__ cb0,
+                                                    const int* __restrict__ cb1, const int* __restrict__ npos,
+                                                    const int* __restrict__ sid, int* __restrict__ comp,
+                                                    int* __restrict__ best, int* __restrict__ bnew,
+                                                    uint8_t* __restrict__ intree, int* __restrict__ ntree) {
+  __shared__ int s_any, s_chg;
+  const int inst = blockIdx.x;
+  const InstParams& P = PP[inst];
+  if (!P.active || !P.run_spl) return;
+  const int nb = nbasins[inst], n = nconn[inst], tid = threadIdx.x;
+  const long long eb = (long long)FSX_EDGE_CAP * P.off;
+  const int* E0 = cb0 + eb;
+  const int* E1 = cb1 + eb;
+  const int* NP = npos + eb;
+  const int* SI = sid + eb;
+  uint8_t* IT = intree + eb;
+  int* C = comp + P.off;
+  int* BST = best + P.off;
+  int* BN = bnew + P.off;
+  for (int b = tid; b < nb; b += FSX_WG) C[b] = b;
+  for (int e = tid; e < n; e += FSX_WG) IT[e] = 0;
+  __syncthreads();
+  if (n == 0) return;
+  for (int round = 0; round < 64; round++) {
+    for (int b = tid; b < nb; b += FSX_WG) BST[b] = INT_MAX;
+    if (tid == 0) s_any = 0;
+    __syncthreads();
+    for (int e = tid; e < n; e += FSX_WG) {
+      const int c0 = C[E0[e]], c1 = C[E1[e]];
+      if (c0 != c1) {
+        const int k = NP[e];
+        atomicMin(BST + c0, k);
+        atomicMin(BST + c1, k);
+      }
+    }
+    __syncthreads();
+    for (int cc = tid; cc < nb; cc += FSX_WG) {
+      if (C[cc] != cc) continue;
+      BN[cc] = cc;
+      const int k = BST[cc];
+      if (k == INT_MAX) continue;
+      const int e = SI[k];
+      IT[e] = 1;
+      const int c0 = C[E0[e]], c1 = C[E1[e]];
+      const int o = c0 == cc ? c1 : c0;
+      BN[cc] = (BST[o] == k) ? (cc < o ? cc : o) : o;
+      s_any = 1;
+    }
+    __syncthreads();
+    if (!s_any) break;
+    for (;;) {
+      if (tid == 0) s_chg = 0;
+      __syncthreads();
+      for (int cc = tid; cc < nb; cc += FSX_WG) {
+        if (C[cc] != cc) continue;
+        const int x = BN[cc], y = BN[x];
+        if (x != y) { BN[cc] = y; s_chg = 1; }
+      }
+      __syncthreads();
+      if (!s_chg) break;
+    }
+    for (int b = tid; b < nb; b += FSX_WG) C[b] = BN[C[b]];
+    __syncthreads();
+  }
+  int cnt = 0;
+  for (int e = tid; e < n; e += FSX_WG) cnt += IT[e];
+  if (cnt) atomicAdd(ntree + inst, cnt);
+}
+
+// ---- 5. orientation from basin0 (order_tree) --------------------------------------------------------
+__global__ __launch_bounds__(FSX_WG) void k_orient(const InstParams* __restrict__ PP, const int* __restrict__ nbasins,
+                                                   const int* __restrict__ nconn, const int* __restrict__ basin0,
+                                                   int* __restrict__ cb0, int* __restrict__ cb1, int* __restrict__ cn0,
+                                                   int* __restrict__ cn1, const uint8_t* __restrict__ intree,
+                                                   int* __restrict__ tdeg, int* __restrict__ toff, int* __restrict__ cur,
+                                                   int* __restrict__ tadj, int* __restrict__ pe, int* __restrict__ front) {
+  __shared__ int lds[FSX_WG / 64];
+  __shared__ int s_end;
+  const int inst = blockIdx.x;
+  const InstParams& P = PP[inst];
+  if (!P.active || !P.run_spl) return;
+  const int nb = nbasins[inst], n = nconn[inst], tid = threadIdx.x;
+  if (n == 0) return;
+  const long long eb = (long long)FSX_EDGE_CAP * P.off;
+  int* E0 = cb0 + eb;
+  int* E1 = cb1 + eb;
+  int* N0 = cn0 + eb;
+  int* N1 = cn1 + eb;
+  const uint8_t* IT = intree + eb;
+  int* DG = tdeg + P.off;
+  int* OF = toff + P.off;
+  int* CU = cur + P.off;
+  int* ADJ = tadj + P.off * 2;  // 2*(nb-1) entries <= 2*nn
+  int* PE = pe + P.off;
+  int* FR = front + P.off;
+  for (int b = tid; b < nb; b += FSX_WG) { DG[b] = 0; CU[b] = 0; PE[b] = -1; }
+  __syncthreads();
+  for (int e = tid; e < n; e += FSX_WG)
+    if (IT[e]) { atomicAdd(DG + E0[e], 1); atomicAdd(DG + E1[e], 1); }
+  __syncthreads();
+  int carry = 0;
+  for (int s = 0; s < nb; s += FSX_WG) {
+    const int b = s + tid;
+    const int x = b < nb ? DG[b] : 0;
+    int tot;
+    const int incl = block_incl_scan<FSX_WG>(x, lds, tot);
+    if (b < nb) OF[b] = carry + incl - x;
+    carry += tot;
+  }
+  __syncthreads();
+  for (int e = tid; e < n; e += FSX_WG)
+    if (IT[e]) {
+      const int a = E0[e], c = E1[e];
+      ADJ[OF[a] + atomicAdd(CU + a, 1)] = e;
+      ADJ[OF[c] + atomicAdd(CU + c, 1)] = e;
+    }
+  __syncthreads();
+  const int root = basin0[inst];
+  if (tid == 0) {
+    FR[0] = root;
+    PE[root] = -2;
+    s_end = 1;
+  }
+  __syncthreads();
+  int fs = 0, fe = 1;
+  while (fs < fe) {
+    for (int idx = fs + tid; idx < fe; idx += FSX_WG) {
+      const int B = FR[idx];
+      for (int q = OF[B]; q < OF[B] + DG[B]; q++) {
+        const int e = ADJ[q];
+        const int o = E0[e] == B ? E1[e] : E0[e];
+        if (PE[o] == -1) {
+          PE[o] = e;
+          FR[atomicAdd(&s_end, 1)] = o;
+          if (E0[e] != B) {  // make (n1,p1) the parent side
+            E0[e] = B; E1[e] = o;
+            const int t = N0[e]; N0[e] = N1[e]; N1[e] = t;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    fs = fe;
+    fe = s_end;
+    __syncthreads();
+  }
+}
+
+// ---- 6. receiver path reversal (correct_receivers_v2) ------------------------------------------------
+__global__ __launch_bounds__(FSX_CHUNK) void k_correct(const InstParams* __restrict__ PP, const int* __restrict__ nconn,
+                                                       const int* __restrict__ cb1, const int* __restrict__ cn0,
+                                                       const int* __restrict__ cn1, const uint8_t* __restrict__ intree,
+                                                       const int* __restrict__ outlet, int* __restrict__ rec,
+                                                       double* __restrict__ length, int* __restrict__ err) {
+  const int inst = blockIdx.y;
+  const InstParams& P = PP[inst];
+  if (!P.active || !P.run_spl) return;
+  const int n = nconn[inst];
+  const long long eb = (long long)FSX_EDGE_CAP * P.off;
+  int* R = rec + P.off;
+  double* LEN = length + P.off;
+  for (int e = blockIdx.x * FSX_CHUNK + threadIdx.x; e < n; e += gridDim.x * FSX_CHUNK) {
+    if (!intree[eb + e]) continue;
+    const int node_from = cn1[eb + e];
+    if (node_from == -1) continue;
+    const int node_to = cn0[eb + e];
+    const int outlet_from = outlet[P.off + cb1[eb + e]];
+    int next = R[node_from], cur = node_from;
+    double prev = LEN[node_from];
+    R[node_from] = node_to;
+    const int ii = node_from % P.nx - node_to % P.nx, jj = node_from / P.nx - node_to / P.nx;
+    LEN[node_from] = P.len[(ii + 1) + 3 * (jj + 1)];  // sqrt(ddx*ddx+ddy*ddy), :643-646
+    int guard = 0;
+    while (cur != outlet_from) {
+      if (++guard > P.nn) { atomicOr(err + inst, FSX_ERR_LEVELS); break; }
+      const int rn = R[next];
+      R[next] = cur;
+      const double t = prev;
+      prev = LEN[next];
+      LEN[next] = t;
+      cur = next;
+      next = rn;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------------
+void launch_nr_sort(DevState& S, hipStream_t st) {
+  k_fill_int<<<1, 256, 0, st>>>(S.segctr, S.ninst, 0);
+  k_nr_big<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.nconn, S.cw, S.skey, S.sid, S.Lasc, S.Rasc, S.segs, S.nsegs, S.err);
+  k_nr_small<<<dim3(8, S.ninst), 256, 0, st>>>(S.P, S.nsegs, S.segctr, S.skey, S.sid, S.Lasc, S.Rasc, S.segs, S.err);
+  k_nr_pos<<<dim3(16, S.ninst), FSX_CHUNK, 0, st>>>(S.P, S.nconn, S.sid, S.npos);
+}
+
+void launch_local_minima(DevState& S, hipStream_t st) {
+  const int nb = S.nchunks;
+  k_lbl_init<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.rec, S.lbl);
+  k_fill_int<<<1, 64, 0, st>>>(S.flags, 64, 0);
+  for (int it = 0; it < 26; it++) k_lbl_jump<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.lbl, S.flags, it);
+  k_root_count<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.rec, S.chunkcnt);
+  k_root_scan<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.chunkcnt, S.cbase, S.nbasins, S.basin0);
+  k_root_rank<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.rec, S.bc, S.chunkcnt, S.rootrank, S.outlet, S.basin0);
+  k_fill_int<<<1024, 256, 0, st>>>(S.bsize, S.ntot, 0);
+  k_basin_label<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.lbl, S.rootrank, S.bc, S.basin, S.bsize);
+  k_bsize_scan<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.nbasins, S.bsize, S.soff, S.ecount);
+  k_basin_walk<<<nb * 4, 64, 0, st>>>(S.P, S.chunks, S.nbasins, S.outlet, S.bc, S.rec, S.dmask, S.basin, S.h, S.soff,
+                                       S.bsize, S.ecount, S.sB, S.sp, S.sq, S.sw, S.err);
+  k_edge_scan<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.nbasins, S.basin0, S.outlet, S.bc, S.ecount, S.eoff, S.nconn, S.err);
+  k_edge_fill<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.nbasins, S.basin0, S.nconn, S.outlet, S.bc, S.ecount, S.eoff,
+                                         S.soff, S.sB, S.sp, S.sq, S.sw, S.cb0, S.cb1, S.cn0, S.cn1, S.cw);
+  launch_nr_sort(S, st);
+  k_fill_int<<<1, 256, 0, st>>>(S.ntree, S.ninst, 0);
+  k_boruvka<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.nbasins, S.nconn, S.cb0, S.cb1, S.npos, S.sid, S.comp, S.best, S.bnew,
+                                         S.intree, S.ntree);
+  k_orient<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.nbasins, S.nconn, S.basin0, S.cb0, S.cb1, S.cn0, S.cn1, S.intree, S.tdeg,
+                                        S.toff, S.bsize, S.tadj, S.bpar, S.bfront);
+  k_correct<<<dim3(16, S.ninst), FSX_CHUNK, 0, st>>>(S.P, S.nconn, S.cb1, S.cn0, S.cn1, S.intree, S.outlet, S.rec,
+                                                      S.length, S.err);
+}

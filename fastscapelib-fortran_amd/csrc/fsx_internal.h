@@ -1,0 +1,124 @@
+// fsx_internal.h -- shared layout, parameter block and device helpers of the MI355X landscape step.
+//
+// A context holds a batch of independent landscapes ("instances").  Every per-node array is one
+// flat allocation over the batch; instance k owns nodes [P[k].off, P[k].off + P[k].nn).  Node
+// indices stored inside arrays (receivers, labels, pass nodes) are LOCAL to their instance, so
+// every per-instance algorithm reads like the single-landscape reference.
+//
+// Per-basin arrays reuse the node offsets (a landscape has at most nn basins); per-edge arrays use
+// EDGE_CAP*off (the reference sizes its basin graph as 6*nbasins, src/LocalMinima.f90:59).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define FSX_EDGE_CAP 6   // edges per node of capacity (nconn_max = 6*nbasins in the reference)
+#define FSX_SLAB_CAP 8   // candidate-pass slots per closed-basin node
+#define FSX_CHUNK 256    // nodes per block in node-parallel kernels
+#define FSX_WG 1024      // threads of the per-landscape workgroups
+
+// device-side error bits, one word per instance
+enum {
+  FSX_ERR_NCONN = 1,       // more basin-graph edges than 6*nbasins (reference would overflow)
+  FSX_ERR_NO_OUTLET = 2,   // no basin drains to a fixed node (reference reads basin(-1))
+  FSX_ERR_TRIDAG = 4,      // zero pivot in the tridiagonal solve (reference STOPs)
+  FSX_ERR_NEWTON = 8,      // Newton iteration did not converge within the cap
+  FSX_ERR_SORTSTACK = 16,  // segment stack of the edge sort overflowed
+  FSX_ERR_LEVELS = 32,     // receiver graph has a cycle (stack incomplete)
+  FSX_ERR_MFD = 64,        // multiple-flow-direction order incomplete (reference STOPs)
+};
+
+struct InstParams {
+  int nx, ny, nn, off;       // grid, node count, node offset in the batch arrays
+  int i1, i2, j1, j2;        // 0-based inclusive receiver-search bounds (SetBC)
+  int xcyc, ycyc;            // cyclic flags (SetBC)
+  int fix_left, fix_right, fix_bottom, fix_top;  // BC digits == '1'
+  int run_uplift, run_spl, run_diff, sfd;
+  int active;
+  int pad0;
+  double dx, dy, dt;
+  double len[9];             // neighbour distance by direction (ii+1)+3*(jj+1), host-computed
+  double dxdy;               // dx*dy
+  double kfsed, m, n, kdsed, g1, g2, p, newton_tol;
+};
+
+struct Chunk {
+  int inst, start;
+};
+
+// ------------------------------------------------------------------------------------------
+// Neighbour of (i,j) in direction (ii,jj) with cyclic wrap else clamp, as find_receiver
+// (src/FlowRouting.f90:336-345).  0-based.
+__device__ __forceinline__ int nbr_wc(const InstParams& P, int i, int j, int ii, int jj) {
+  int iii = i + ii;
+  if (iii < 0 && P.xcyc) iii += P.nx;
+  if (iii < 0) iii = 0;
+  if (iii > P.nx - 1 && P.xcyc) iii -= P.nx;
+  if (iii > P.nx - 1) iii = P.nx - 1;
+  int jjj = j + jj;
+  if (jjj < 0 && P.ycyc) jjj += P.ny;
+  if (jjj < 0) jjj = 0;
+  if (jjj > P.ny - 1 && P.ycyc) jjj -= P.ny;
+  if (jjj > P.ny - 1) jjj = P.ny - 1;
+  return iii + jjj * P.nx;
+}
+
+// Distinct wrap/clamp neighbours of v (excluding v), in direction order d = (ii+1)+3*(jj+1).
+// Receivers of every node (before and after closed-basin rerouting) are among these, so the
+// donors of v are exactly the q in this set with rec[q] == v.  Returns the count; q[] and dir[].
+__device__ __forceinline__ int neighbours_wc(const InstParams& P, int v, int* q, int* dir) {
+  int i = v % P.nx, j = v / P.nx, c = 0;
+  for (int jj = -1; jj <= 1; jj++)
+    for (int ii = -1; ii <= 1; ii++) {
+      if (ii == 0 && jj == 0) continue;
+      int w = nbr_wc(P, i, j, ii, jj);
+      if (w == v) continue;
+      bool dup = false;
+      for (int t = 0; t < c; t++) dup |= (q[t] == w);
+      if (dup) continue;
+      q[c] = w;
+      dir[c] = (ii + 1) + 3 * (jj + 1);
+      c++;
+    }
+  return c;
+}
+
+// bit index of direction d in the 8-bit donor mask
+__device__ __forceinline__ int dbit(int d) { return d < 4 ? d : d - 1; }
+
+// Donors of v in ascending local index (the reference's donor order, src/FlowRouting.f90:379-385).
+__device__ __forceinline__ int donors_sorted(const InstParams& P, int v, unsigned mask, int* d) {
+  if (!mask) return 0;
+  int q[8], dir[8];
+  int c = neighbours_wc(P, v, q, dir), nd = 0;
+  for (int t = 0; t < c; t++)
+    if (mask & (1u << dbit(dir[t]))) {
+      int x = q[t], s = nd++;
+      while (s > 0 && d[s - 1] > x) { d[s] = d[s - 1]; s--; }
+      d[s] = x;
+    }
+  return nd;
+}
+
+// order-preserving 64-bit image of a double (for atomic min on doubles)
+__device__ __forceinline__ unsigned long long dkey(double x) {
+  unsigned long long u = (unsigned long long)__double_as_longlong(x);
+  return (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
+}
+
+// ------------------------------------------------------------------------------------------
+// Host-side launch wrappers (defined in the .hip files).
+struct DevState;
+void launch_uplift(DevState& S, hipStream_t st);
+void launch_receivers(DevState& S, hipStream_t st);
+void launch_donors(DevState& S, hipStream_t st);
+void launch_local_minima(DevState& S, hipStream_t st);
+void launch_nr_sort(DevState& S, hipStream_t st);
+void launch_spl_prep(DevState& S, hipStream_t st);
+void launch_spl_post(DevState& S, hipStream_t st);
+void launch_expand_donors(DevState& S, hipStream_t st);
+void launch_levels_fill(DevState& S, hipStream_t st);
+void launch_area(DevState& S, hipStream_t st);
+void launch_spl(DevState& S, hipStream_t st);
+void launch_diffusion(DevState& S, hipStream_t st);
+void launch_mfd(DevState& S, hipStream_t st);

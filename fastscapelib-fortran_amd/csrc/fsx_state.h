@@ -1,0 +1,52 @@
+// fsx_state.h -- device buffers of one context (a batch of landscapes), owned by the library.
+#pragma once
+#include "fsx_internal.h"
+
+struct DevState {
+  int ninst = 0;
+  long long ntot = 0;     // total nodes over the batch
+  int nchunks = 0;        // FSX_CHUNK-node blocks covering every instance
+  int max_nn = 0, max_nx = 0, max_ny = 0;
+  int any_mfd = 0, any_gs = 0;
+  InstParams* P = nullptr;  // [ninst]
+  Chunk* chunks = nullptr;  // [nchunks]
+
+  // ---- per node (ntot) ----
+  double *h = nullptr, *b = nullptr, *u = nullptr, *kf = nullptr, *kd = nullptr, *precip = nullptr;
+  double *etot = nullptr, *erate = nullptr, *area = nullptr, *length = nullptr, *hwater = nullptr;
+  double *lake_depth = nullptr, *sedflux = nullptr, *ht = nullptr, *water = nullptr, *zint = nullptr;
+  double *zintp = nullptr, *gam = nullptr, *dh = nullptr, *hp = nullptr, *lwv = nullptr, *lsed = nullptr;
+  double *kfint = nullptr, *kdint = nullptr, *g = nullptr, *pexp = nullptr, *elev = nullptr;
+  int *rec = nullptr, *lbl = nullptr, *basin = nullptr, *rootrank = nullptr, *order = nullptr;
+  int *lstart = nullptr, *lsill = nullptr, *stack = nullptr;
+  uint8_t *dmask = nullptr, *bc = nullptr;
+
+  // ---- per basin (node offsets) ----
+  int *outlet = nullptr, *bsize = nullptr, *soff = nullptr, *ecount = nullptr, *eoff = nullptr;
+  int *comp = nullptr, *bpar = nullptr, *bnew = nullptr, *best = nullptr, *tdeg = nullptr, *toff = nullptr;
+  int *tadj = nullptr, *bfront = nullptr;
+
+  // ---- per edge (FSX_EDGE_CAP * node offsets) ----
+  int *cb0 = nullptr, *cb1 = nullptr, *cn0 = nullptr, *cn1 = nullptr, *sid = nullptr, *npos = nullptr;
+  int *Lasc = nullptr, *Rasc = nullptr, *segs = nullptr;
+  double *cw = nullptr, *skey = nullptr;
+  uint8_t *intree = nullptr;
+
+  // ---- candidate-pass slab (FSX_SLAB_CAP * node offsets) ----
+  int *sB = nullptr, *sp = nullptr, *sq = nullptr;
+  double *sw = nullptr;
+
+  // ---- multiple flow direction (8 slots per node, allocated when any instance uses it) ----
+  int *mrec = nullptr, *mstack = nullptr, *mndon = nullptr, *mvis = nullptr;
+  uint8_t *mnrec = nullptr;
+  double *mwrec = nullptr, *mlrec = nullptr;
+
+  // ---- per instance ----
+  int *nbasins = nullptr, *basin0 = nullptr, *nconn = nullptr, *nlev = nullptr, *err = nullptr;
+  int *ngs = nullptr, *ntree = nullptr, *chunkcnt = nullptr;
+  long long *newton = nullptr;
+  double *scal = nullptr;      // per-instance scalar scratch (tolerances, GS error)
+  int *flags = nullptr;        // convergence flags of iterative kernels (64)
+  int *cbase = nullptr;        // first chunk of each instance [ninst+1]
+  int *nsegs = nullptr, *segctr = nullptr;  // edge-sort segment lists [ninst]
+};

@@ -1,0 +1,312 @@
+"""Drop-in for the f2py module ``fastscapelib_fortran`` (src_python/fastscapelib_fortran/__init__.py:1),
+running FastScape_Execute_Step on an MI355X through the C ABI of libfastscape_hip.so.
+
+The free functions keep the reference's call surface and f2py argument conventions
+(src/FastScape_api.f90, usage as examples/FastScape_test.ipynb):
+
+* scalars by value: ``fastscape_set_nx_ny(nx, ny)``, ``fastscape_set_dt(dt)``, ``fastscape_set_bc(ibc)``;
+* ``intent(in)`` arrays are copied in: ``fastscape_init_h(h)``, ``fastscape_set_u(u)``, ...;
+* copy-outs fill a caller-owned float64 array in place and return None: ``fastscape_copy_h(h)``;
+* ``fastscape_get_step()`` and ``fastscape_get_sizes()`` return their values.
+
+Documented divergences: call-order and size errors raise ``RuntimeError``/``ValueError`` where the
+reference STOPs the process; ``fastscape_setup`` starts from h = 0 rather than Fortran random
+numbers (every run sets h with ``fastscape_init_h``); routines outside the per-step update
+(advection, marine, stratigraphy, VTK, flexure, chi/slope/curvature read-outs) raise
+``NotImplementedError``.
+
+``Batch`` runs many independent landscapes of different sizes in one set of kernel launches.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import FastscapeError, check, host_f64, indexx, lib, ptr
+
+__all__ = [
+    "fastscape_init", "fastscape_set_nx_ny", "fastscape_setup", "fastscape_set_xl_yl", "fastscape_set_dt",
+    "fastscape_init_h", "fastscape_set_erosional_parameters", "fastscape_set_u", "fastscape_set_bc",
+    "fastscape_execute_step", "fastscape_copy_h", "fastscape_copy_basement", "fastscape_copy_total_erosion",
+    "fastscape_copy_erosion_rate", "fastscape_copy_drainage_area", "fastscape_copy_lake_depth",
+    "fastscape_get_step", "fastscape_get_sizes", "fastscape_destroy", "fastscape_set_h", "fastscape_set_basement",
+    "fastscape_set_precip", "fastscape_set_all_layers", "fastscape_reset_cumulative_erosion", "fastscape_view",
+    "fastscape_debug", "Batch", "FastscapeError", "indexx",
+]
+
+
+class Batch:
+    """Independent landscapes resident in HBM, stepped together.
+
+    ``sizes`` is a sequence of (nx, ny).  Setters take the instance id first and otherwise mirror
+    the fastscape_* functions.  Arrays may be numpy (host) or torch CUDA tensors (device, no copy
+    through the host).
+    """
+
+    def __init__(self, sizes):
+        sizes = [(int(a), int(b)) for a, b in sizes]
+        if not sizes:
+            raise ValueError("Batch needs at least one landscape")
+        n = len(sizes)
+        nx = (C.c_int * n)(*[s[0] for s in sizes])
+        ny = (C.c_int * n)(*[s[1] for s in sizes])
+        h = C.c_void_p()
+        check(lib.fsx_create(n, nx, ny, C.byref(h)), "fsx_create")
+        self._h = h
+        self.sizes = sizes
+        self.nn = [a * b for a, b in sizes]
+
+    # -- lifetime --
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib.fsx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __len__(self):
+        return len(self.sizes)
+
+    # -- helpers --
+    def _src(self, k, x, name):
+        """(pointer, on_device, keepalive) for an input array of instance k."""
+        if hasattr(x, "data_ptr") and getattr(x, "is_cuda", False):
+            import torch
+            t = x.reshape(-1)
+            if t.dtype != torch.float64:
+                t = t.to(torch.float64)
+            t = t.contiguous()
+            if t.numel() < self.nn[k]:
+                raise ValueError(f"{name}: expected at least {self.nn[k]} values")
+            return C.c_void_p(t.data_ptr()), 1, t
+        a = host_f64(x, self.nn[k], name)
+        return ptr(a), 0, a
+
+    # -- setters --
+    def set_xl_yl(self, k, xl, yl): check(lib.fsx_set_xl_yl(self._h, k, float(xl), float(yl)), "set_xl_yl")
+    def set_dt(self, k, dt): check(lib.fsx_set_dt(self._h, k, float(dt)), "set_dt")
+    def set_bc(self, k, ibc): check(lib.fsx_set_bc(self._h, k, int(ibc)), "set_bc")
+    def set_newton_tol(self, k, tol): check(lib.fsx_set_newton_tol(self._h, k, float(tol)), "set_newton_tol")
+    def set_active(self, k, on): check(lib.fsx_set_active(self._h, k, 1 if on else 0), "set_active")
+
+    def set_erosional_parameters(self, k, kf, kfsed, m, n, kd, kdsed, g1, g2, p):
+        pkf, dkf, a1 = self._src(k, kf, "kf")
+        pkd, dkd, a2 = self._src(k, kd, "kd")
+        if dkf != dkd:
+            a1 = host_f64(kf.cpu().numpy() if dkf else kf, self.nn[k], "kf")
+            a2 = host_f64(kd.cpu().numpy() if dkd else kd, self.nn[k], "kd")
+            pkf, pkd, dkf = ptr(a1), ptr(a2), 0
+        check(lib.fsx_set_erosional_parameters(self._h, k, pkf, float(kfsed), float(m), float(n), pkd, float(kdsed),
+                                               float(g1), float(g2), float(p), dkf), "set_erosional_parameters")
+
+    def init_h(self, k, h):
+        p, d, _a = self._src(k, h, "h")
+        check(lib.fsx_init_h(self._h, k, p, d), "init_h")
+
+    def set_field(self, k, field, x, name="field"):
+        p, d, _a = self._src(k, x, name)
+        check(lib.fsx_set_field(self._h, k, field, p, d), "set_field")
+
+    def set_u(self, k, u): self.set_field(k, _lib.FSX_U, u, "u")
+    def set_h(self, k, h): self.set_field(k, _lib.FSX_H, h, "h")
+    def set_basement(self, k, b): self.set_field(k, _lib.FSX_B, b, "b")
+    def set_precip(self, k, p): self.set_field(k, _lib.FSX_PRECIP, p, "precip")
+
+    def set_all_layers(self, k, dh):
+        p, d, _a = self._src(k, dh, "dh")
+        check(lib.fsx_set_all_layers(self._h, k, p, d), "set_all_layers")
+
+    def reset_cumulative_erosion(self, k): check(lib.fsx_reset_cumulative_erosion(self._h, k), "reset")
+
+    # -- stepping --
+    def step(self, nsteps=1, stream=None, sync=True):
+        s = C.c_void_p(stream) if isinstance(stream, int) else (stream or C.c_void_p())
+        check(lib.fsx_execute_steps(self._h, int(nsteps), s, 1 if sync else 0), "execute_steps")
+
+    def synchronize(self): check(lib.fsx_synchronize(self._h), "synchronize")
+
+    def route(self, sync=True):
+        check(lib.fsx_route(self._h, C.c_void_p(), 1 if sync else 0), "route")
+
+    # -- read-outs --
+    def field(self, k, field, out=None):
+        if out is None:
+            out = np.empty(self.nn[k], np.float64)
+        if hasattr(out, "data_ptr") and getattr(out, "is_cuda", False):
+            check(lib.fsx_copy_field(self._h, k, field, C.c_void_p(out.data_ptr()), 1), "copy_field")
+            return out
+        if out.dtype != np.float64 or not out.flags.c_contiguous or out.size < self.nn[k]:
+            raise ValueError(f"copy target must be a contiguous float64 array of >= {self.nn[k]} values")
+        check(lib.fsx_copy_field(self._h, k, field, ptr(out), 0), "copy_field")
+        return out
+
+    def ifield(self, k, field):
+        n = self.nn[k] * (8 if field in (_lib.FSX_DON, _lib.FSX_MREC) else 1)
+        out = np.empty(n, np.int32)
+        check(lib.fsx_copy_ifield(self._h, k, field, ptr(out), 0), "copy_ifield")
+        return out.reshape(self.nn[k], 8) if n != self.nn[k] else out
+
+    def h(self, k): return self.field(k, _lib.FSX_H)
+    def rec(self, k): return self.ifield(k, _lib.FSX_REC)
+    def donors(self, k): return self.ifield(k, _lib.FSX_NDON), self.ifield(k, _lib.FSX_DON)
+    def order(self, k): return self.ifield(k, _lib.FSX_ORDER)
+
+    def get_step(self, k):
+        s = C.c_int()
+        check(lib.fsx_get_step(self._h, k, C.byref(s)), "get_step")
+        return s.value
+
+    def stats(self, k):
+        o = (C.c_longlong * 6)()
+        check(lib.fsx_get_stats(self._h, k, o), "get_stats")
+        return dict(zip(["nbasins", "nconn", "ntree", "levels", "ngs", "newton_iters"], list(o)))
+
+    def memory(self):
+        a, b = C.c_longlong(), C.c_longlong()
+        check(lib.fsx_memory(self._h, C.byref(a), C.byref(b)), "memory")
+        return a.value, b.value
+
+
+# ---- the f2py-style module state (one landscape, as the reference's FastScapeContext) ----------
+class _State:
+    def __init__(self):
+        self.nx = 0
+        self.ny = 0
+        self.batch: Batch | None = None
+
+
+_S = _State()
+
+
+def _need():
+    if _S.batch is None:
+        raise RuntimeError("You need to run SetUp first")
+    return _S.batch
+
+
+def fastscape_init():
+    """FastScape_Init (src/FastScape_api.f90:146-156)."""
+    if _S.batch is not None:
+        _S.batch.close()
+    _S.__init__()
+
+
+def fastscape_set_nx_ny(nx, ny):
+    """FastScape_Set_NX_NY (src/FastScape_api.f90:475-487)."""
+    _S.nx, _S.ny = int(nx), int(ny)
+
+
+def fastscape_setup():
+    """FastScape_Setup (src/FastScape_api.f90:160-170; SetUp src/FastScape_ctx.f90:57-113)."""
+    if _S.nx == 0:
+        raise RuntimeError("FastScapeSetup - You need to set nx first")
+    if _S.ny == 0:
+        raise RuntimeError("FastScapeSetup - You need to set ny first")
+    if _S.batch is not None:
+        _S.batch.close()
+    _S.batch = Batch([(_S.nx, _S.ny)])
+
+
+def fastscape_destroy():
+    """FastScape_Destroy (src/FastScape_api.f90:174-184)."""
+    if _S.batch is not None:
+        _S.batch.close()
+        _S.batch = None
+
+
+def fastscape_set_xl_yl(xl, yl): _need().set_xl_yl(0, xl, yl)
+def fastscape_set_dt(dt): _need().set_dt(0, dt)
+def fastscape_set_bc(ibc): _need().set_bc(0, ibc)
+def fastscape_init_h(h): _need().init_h(0, h)
+def fastscape_set_u(u): _need().set_u(0, u)
+def fastscape_set_h(h): _need().set_h(0, h)
+def fastscape_set_basement(b): _need().set_basement(0, b)
+def fastscape_set_precip(p): _need().set_precip(0, p)
+def fastscape_set_all_layers(dh): _need().set_all_layers(0, dh)
+def fastscape_reset_cumulative_erosion(): _need().reset_cumulative_erosion(0)
+
+
+def fastscape_set_erosional_parameters(kf, kfsed, m, n, kd, kdsed, g1, g2, p):
+    """FastScape_Set_Erosional_Parameters (src/FastScape_api.f90:523-536)."""
+    _need().set_erosional_parameters(0, kf, kfsed, m, n, kd, kdsed, g1, g2, p)
+
+
+def fastscape_execute_step():
+    """FastScape_Execute_Step (src/FastScape_api.f90:201-263)."""
+    _need().step(1)
+
+
+def _copy_into(field, out):
+    b = _need()
+    if not isinstance(out, np.ndarray) or out.dtype != np.float64 or not out.flags.c_contiguous:
+        raise ValueError("copy target must be a contiguous float64 numpy array (f2py intent(inout))")
+    if out.size < b.nn[0]:
+        raise ValueError(f"copy target holds {out.size} values, need {b.nn[0]}")
+    b.field(0, field, out.reshape(-1))
+
+
+def fastscape_copy_h(h): _copy_into(_lib.FSX_H, h)
+def fastscape_copy_basement(b): _copy_into(_lib.FSX_B, b)
+def fastscape_copy_total_erosion(e): _copy_into(_lib.FSX_ETOT, e)
+def fastscape_copy_erosion_rate(e): _copy_into(_lib.FSX_ERATE, e)
+def fastscape_copy_drainage_area(a): _copy_into(_lib.FSX_AREA, a)
+def fastscape_copy_lake_depth(d): _copy_into(_lib.FSX_LAKE_DEPTH, d)
+
+
+def fastscape_get_step():
+    """FastScape_Get_Step (src/FastScape_api.f90:572-584)."""
+    return _need().get_step(0)
+
+
+def fastscape_get_sizes():
+    """FastScape_Get_Sizes (src/FastScape_api.f90:556-568)."""
+    return _S.nx, _S.ny
+
+
+def fastscape_view():
+    """FastScape_View (src/FastScape_ctx.f90:378-393), summary of the resident state."""
+    b = _need()
+    h = b.h(0)
+    print("FastScapeContext:")
+    print("nx,ny", _S.nx, _S.ny)
+    print("nn", _S.nx * _S.ny)
+    print("step", b.get_step(0))
+    print("h", h.min(), h.mean(), h.max())
+
+
+def fastscape_debug():
+    """FastScape_Debug (src/FastScape_ctx.f90:513-561): routing statistics of the last step."""
+    b = _need()
+    st = b.stats(0)
+    rec = b.rec(0)
+    print("--------------------------------------------------------")
+    print("Time step", b.get_step(0))
+    print("Total number of nodes (nx*ny)", _S.nx * _S.ny)
+    print("Total number of self donors", int((rec == np.arange(rec.size)).sum()))
+    print("Number of basins / basin-graph edges / MST edges", st["nbasins"], st["nconn"], st["ntree"])
+    print("Number of Gauss-Siedel iterations (SPL)", st["ngs"])
+
+
+def _outside(name):
+    def f(*a, **k):
+        raise NotImplementedError(f"{name} is outside the per-step update implemented on the GPU")
+    f.__name__ = name
+    return f
+
+
+for _n in ["fastscape_set_v", "fastscape_set_marine_parameters", "fastscape_strati", "fastscape_vtk",
+           "fastscape_get_fluxes", "fastscape_init_f", "fastscape_copy_f", "fastscape_copy_chi",
+           "fastscape_copy_slope", "fastscape_copy_curvature", "fastscape_copy_catchment"]:
+    globals()[_n] = _outside(_n)

@@ -1,0 +1,105 @@
+"""ctypes binding of libfastscape_hip.so (the C ABI declared in include/fastscape_hip.h).
+
+The library is the product path: there is no CPU fallback.  Importing this module on a machine
+where the library is missing or cannot initialise HIP raises immediately.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libfastscape_hip.so")
+
+# field ids (include/fastscape_hip.h)
+FSX_H, FSX_B, FSX_U, FSX_PRECIP, FSX_ETOT, FSX_ERATE, FSX_AREA, FSX_LAKE_DEPTH, FSX_SEDFLUX, FSX_LENGTH, \
+    FSX_HWATER, FSX_KF, FSX_KD = range(13)
+FSX_REC, FSX_NDON, FSX_DON, FSX_ORDER, FSX_MREC, FSX_MNREC = range(6)
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_vp = C.c_void_p
+
+
+class FastscapeError(RuntimeError):
+    """A C-ABI call failed; the message is the library's last error."""
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `make -C fastscapelib-fortran_amd/csrc` "
+                          "(or __graft_entry__.build()); there is no CPU fallback")
+    lib = C.CDLL(LIB_PATH)
+    sig = {
+        "fsx_create": [C.c_int, _ip, _ip, C.POINTER(_vp)],
+        "fsx_destroy": [_vp],
+        "fsx_set_xl_yl": [_vp, C.c_int, C.c_double, C.c_double],
+        "fsx_set_dt": [_vp, C.c_int, C.c_double],
+        "fsx_set_bc": [_vp, C.c_int, C.c_int],
+        "fsx_set_erosional_parameters": [_vp, C.c_int, _vp, C.c_double, C.c_double, C.c_double, _vp, C.c_double,
+                                         C.c_double, C.c_double, C.c_double, C.c_int],
+        "fsx_set_newton_tol": [_vp, C.c_int, C.c_double],
+        "fsx_init_h": [_vp, C.c_int, _vp, C.c_int],
+        "fsx_set_field": [_vp, C.c_int, C.c_int, _vp, C.c_int],
+        "fsx_set_all_layers": [_vp, C.c_int, _vp, C.c_int],
+        "fsx_reset_cumulative_erosion": [_vp, C.c_int],
+        "fsx_set_active": [_vp, C.c_int, C.c_int],
+        "fsx_execute_steps": [_vp, C.c_int, _vp, C.c_int],
+        "fsx_synchronize": [_vp],
+        "fsx_copy_field": [_vp, C.c_int, C.c_int, _vp, C.c_int],
+        "fsx_copy_ifield": [_vp, C.c_int, C.c_int, _vp, C.c_int],
+        "fsx_get_step": [_vp, C.c_int, _ip],
+        "fsx_get_sizes": [_vp, C.c_int, _ip, _ip],
+        "fsx_route": [_vp, _vp, C.c_int],
+        "fsx_indexx": [C.c_int, _dp, _ip],
+        "fsx_get_stats": [_vp, C.c_int, C.POINTER(C.c_longlong)],
+        "fsx_memory": [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)],
+    }
+    for name, args in sig.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    lib.fsx_last_error.restype = C.c_char_p
+    lib.fsx_last_error.argtypes = []
+    lib.fsx_version.restype = C.c_char_p
+    return lib
+
+
+lib = _load()
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != 0:
+        msg = lib.fsx_last_error().decode(errors="replace")
+        raise FastscapeError(f"{what}: {msg} (status {rc})" if what else f"{msg} (status {rc})")
+
+
+def host_f64(x, n: int, name: str) -> np.ndarray:
+    """A contiguous float64 host copy of x with at least n entries (broadcast from a scalar)."""
+    a = np.asarray(x, dtype=np.float64)
+    if a.ndim == 0:
+        a = np.full(n, float(a))
+    a = np.ascontiguousarray(a.reshape(-1))
+    if a.size < n:
+        raise ValueError(f"{name}: expected at least {n} values, got {a.size}")
+    return a
+
+
+def ptr(a: np.ndarray):
+    return C.c_void_p(a.ctypes.data)
+
+
+def dev_ptr(t) -> C.c_void_p:
+    """Device pointer of a torch tensor (float64/int32, contiguous, on the GPU)."""
+    if not t.is_contiguous():
+        raise ValueError("device tensor must be contiguous")
+    return C.c_void_p(t.data_ptr())
+
+
+def indexx(keys) -> np.ndarray:
+    """loc_min_3_indexx on the device: the reference's 1-based sort permutation of keys."""
+    k = np.ascontiguousarray(np.asarray(keys, dtype=np.float64).reshape(-1))
+    out = np.empty(k.size, np.int32)
+    check(lib.fsx_indexx(k.size, k.ctypes.data_as(_dp), out.ctypes.data_as(_ip)), "fsx_indexx")
+    return out

@@ -1,0 +1,138 @@
+/*
+ * fastscape_hip.h -- C ABI of the MI355X implementation of fastscapelib-fortran's per-step
+ * landscape update (FastScape_Execute_Step and the state it reads and writes).
+ *
+ * A context (fsx_ctx) holds a batch of independent landscapes ("instances") resident in HBM; a
+ * single-landscape context is the drop-in for the reference's module state
+ * (src/FastScape_ctx.f90:9-36).  Instance ids run from 0 to ninst-1.
+ *
+ * Conventions
+ *  - Every entry point returns 0 on success or a negative FSX_E* code; fsx_last_error() returns a
+ *    thread-local message for the last failure.  (The reference STOPs the process instead.)
+ *  - Node arrays are nx*ny doubles with x fastest, as the reference's h(nx*ny).
+ *  - `on_device` != 0 means the pointer is HIP device memory (e.g. a torch tensor's data_ptr());
+ *    0 means host memory.  Pointers are not retained after the call returns.
+ *  - Node indices returned by copy-outs are 0-based (the reference's are 1-based).
+ *  - `stream` may be NULL (the context's own stream).
+ */
+#ifndef FASTSCAPE_HIP_H
+#define FASTSCAPE_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fsx_ctx fsx_ctx;
+
+enum {
+  FSX_OK = 0,
+  FSX_EINVAL = -1,   /* bad argument, size or call order                              */
+  FSX_EHIP = -2,     /* HIP runtime error                                              */
+  FSX_ESTEP = -3,    /* a step failed on the device (see fsx_last_error for the cause) */
+  FSX_ENOMEM = -4,
+};
+
+/* field ids of fsx_set_field / fsx_copy_field */
+enum {
+  FSX_H = 0,          /* topography h                     Set_H / Copy_H            */
+  FSX_B = 1,          /* basement b                       Set_Basement / Copy_Basement */
+  FSX_U = 2,          /* uplift rate u                    Set_U                     */
+  FSX_PRECIP = 3,     /* precipitation                    Set_Precip                */
+  FSX_ETOT = 4,       /* total erosion                    Copy_Total_Erosion        */
+  FSX_ERATE = 5,      /* erosion rate                     Copy_Erosion_Rate         */
+  FSX_AREA = 6,       /* drainage area                    Copy_Drainage_Area        */
+  FSX_LAKE_DEPTH = 7, /* lake depth                       Copy_Lake_Depth           */
+  FSX_SEDFLUX = 8,    /* sediment flux of the last step (Sedflux, no reference copy-out) */
+  FSX_LENGTH = 9,     /* distance to receiver (length, internal)                      */
+  FSX_HWATER = 10,    /* lake surface (hwater, internal)                              */
+  FSX_KF = 11,        /* kf (write through fsx_set_erosional_parameters)             */
+  FSX_KD = 12,        /* kd (idem)                                                    */
+};
+/* integer field ids of fsx_copy_ifield */
+enum {
+  FSX_REC = 0,    /* receiver after closed-basin rerouting, 0-based (rec)               */
+  FSX_NDON = 1,   /* donor count (ndon)                                                  */
+  FSX_DON = 2,    /* donors, 8 slots per node ascending, unused = -1 (don(8,nn))         */
+  FSX_ORDER = 3,  /* base-level-first topological order of the nodes (replaces stack)    */
+  FSX_MREC = 4,   /* multiple-flow receivers, 8 slots per node, unused = -1 (mrec)       */
+  FSX_MNREC = 5,  /* multiple-flow receiver count (mnrec)                                */
+};
+
+/* Thread-local text of the last error. */
+const char *fsx_last_error(void);
+/* Library version string. */
+const char *fsx_version(void);
+
+/* FastScape_Init + Set_NX_NY + Setup (src/FastScape_api.f90:146-170,475-487; SetUp,
+ * src/FastScape_ctx.f90:57-113) for ninst landscapes of sizes nx[k] x ny[k] (each >= 3).
+ * Defaults as SetUp: u=0, etot=0, precip=1, BC 1111, no process active; h = 0 (not random). */
+int fsx_create(int ninst, const int *nx, const int *ny, fsx_ctx **out);
+/* FastScape_Destroy (src/FastScape_api.f90:174-184) */
+int fsx_destroy(fsx_ctx *ctx);
+
+/* FastScape_Set_XL_YL (src/FastScape_api.f90:491-503) */
+int fsx_set_xl_yl(fsx_ctx *ctx, int inst, double xl, double yl);
+/* FastScape_Set_DT (src/FastScape_api.f90:507-519) */
+int fsx_set_dt(fsx_ctx *ctx, int inst, double dt);
+/* FastScape_Set_BC (src/FastScape_api.f90:602-614; SetBC src/FastScape_ctx.f90:565-593) */
+int fsx_set_bc(fsx_ctx *ctx, int inst, int ibc);
+/* FastScape_Set_Erosional_Parameters (src/FastScape_api.f90:523-536; SetErosionalParam
+ * src/FastScape_ctx.f90:423-450): p < -1.5 selects single flow direction. */
+int fsx_set_erosional_parameters(fsx_ctx *ctx, int inst, const double *kf, double kfsed, double m, double n,
+                                 const double *kd, double kdsed, double g1, double g2, double p, int on_device);
+/* Newton tolerance of the n != 1 stream-power solve.  No reference setter: the reference
+ * hard-codes 1e-3 (src/StreamPowerLaw.f90:344), which is the default here. */
+int fsx_set_newton_tol(fsx_ctx *ctx, int inst, double tol);
+/* FastScape_Init_H (src/FastScape_api.f90:267-279; InitH src/FastScape_ctx.f90:339-350): h, and b = h */
+int fsx_init_h(fsx_ctx *ctx, int inst, const double *h, int on_device);
+/* FastScape_Set_U / Set_H / Set_Basement / Set_Precip (src/FastScape_api.f90:618-630,664-676,
+ * 696-724): field one of FSX_U, FSX_H, FSX_B, FSX_PRECIP.  Setting FSX_U switches uplift on. */
+int fsx_set_field(fsx_ctx *ctx, int inst, int field, const double *src, int on_device);
+/* FastScape_Set_All_Layers (src/FastScape_api.f90:680-692): h += dh, b += dh */
+int fsx_set_all_layers(fsx_ctx *ctx, int inst, const double *dh, int on_device);
+/* FastScape_Reset_Cumulative_Erosion (src/FastScape_api.f90:650-660) */
+int fsx_reset_cumulative_erosion(fsx_ctx *ctx, int inst);
+/* Include (1) or exclude (0) an instance from fsx_execute_steps (all are active at creation). */
+int fsx_set_active(fsx_ctx *ctx, int inst, int active);
+
+/* FastScape_Execute_Step (src/FastScape_api.f90:201-263), nsteps times, on every active
+ * instance: Uplift, then routing + LocalMinima + accumulation + stream power, then Diffusion.
+ * Asynchronous on `stream` unless `sync` != 0; device-side failures surface as FSX_ESTEP at the
+ * next synchronous call. */
+int fsx_execute_steps(fsx_ctx *ctx, int nsteps, void *stream, int sync);
+/* Wait for queued steps and report device-side failures. */
+int fsx_synchronize(fsx_ctx *ctx);
+
+/* FastScape_Copy_H / Copy_Basement / Copy_Total_Erosion / Copy_Erosion_Rate /
+ * Copy_Drainage_Area / Copy_Lake_Depth (src/FastScape_api.f90:299-471) and internal fields. */
+int fsx_copy_field(fsx_ctx *ctx, int inst, int field, double *dst, int on_device);
+/* Integer state after the last step: receivers, donors, order (see FSX_REC...). dst holds nn
+ * ints (8*nn for FSX_DON and FSX_MREC). */
+int fsx_copy_ifield(fsx_ctx *ctx, int inst, int field, int *dst, int on_device);
+
+/* FastScape_Get_Step (src/FastScape_api.f90:572-584) */
+int fsx_get_step(fsx_ctx *ctx, int inst, int *step);
+/* FastScape_Get_Sizes (src/FastScape_api.f90:556-568) */
+int fsx_get_sizes(fsx_ctx *ctx, int inst, int *nx, int *ny);
+
+/* Routing only, on the current h (no uplift, no erosion): find_receiver + find_donor +
+ * find_stack + LocalMinima + lake fill + accumulation, as FlowRoutingSingleFlowDirection and
+ * FlowAccumulationSingleFlowDirection (src/FlowRouting.f90:43-93,123-143), or their multiple-flow
+ * counterparts (:3-39,97-119).  Copy the result with fsx_copy_ifield / fsx_copy_field. */
+int fsx_route(fsx_ctx *ctx, void *stream, int sync);
+
+/* loc_min_3_indexx (src/LocalMinima.f90:947-1046) on n keys: writes the 1-based permutation
+ * (the reference's indx) into perm.  Host arrays; runs the device sort. */
+int fsx_indexx(int n, const double *keys, int *perm);
+
+/* Statistics of the last step of an instance: out[0]=nbasins, out[1]=nconn, out[2]=ntree,
+ * out[3]=levels, out[4]=Gauss-Seidel iterations, out[5]=Newton iterations (total). */
+int fsx_get_stats(fsx_ctx *ctx, int inst, long long *out6);
+
+/* Device-resident memory of a context (bytes) and the bytes per node the layout costs. */
+int fsx_memory(fsx_ctx *ctx, long long *bytes, long long *bytes_per_node);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

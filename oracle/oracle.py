@@ -1,0 +1,345 @@
+"""ctypes views of the two CPU checkers.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+leg.  The product path (fastscapelib-fortran_amd/) never imports this module.
+
+* ``COracle``   -- oracle/liboracle.so, the C restatement (handle based, any number of instances).
+* ``FortranRef`` -- oracle/_ref/libfsref.so, the reference Fortran compiled unmodified from its own
+  sources by ``make -C oracle ref``.  Its state is module-global (src/FastScape_ctx.f90:9-36), so one
+  instance exists per process; calling ``setup`` again replaces it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+C_LIB = os.path.join(HERE, "liboracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "libfsref.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def build_c(force: bool = False) -> str:
+    """Compile the C restatement (make -C oracle c)."""
+    if force or not os.path.exists(C_LIB):
+        subprocess.run(["make", "-s", "-C", HERE, "c"], check=True)
+    return C_LIB
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_LIB)
+
+
+class Stats(C.Structure):
+    _fields_ = [("nbasins", C.c_int), ("nopen", C.c_int), ("nconn", C.c_int), ("ntree", C.c_int),
+                ("nlocmin_before", C.c_int), ("nlocmin_after", C.c_int), ("max_tie", C.c_int),
+                ("depth", C.c_int), ("ngs", C.c_int), ("newton_iters", C.c_long)]
+
+    def asdict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_clib = None
+
+
+def _c():
+    global _clib
+    if _clib is None:
+        lib = C.CDLL(build_c())
+        lib.fso_create.restype = C.c_void_p
+        lib.fso_create.argtypes = [C.c_int, C.c_int]
+        lib.fso_destroy.argtypes = [C.c_void_p]
+        for name, args in [("fso_set_xl_yl", [C.c_double, C.c_double]), ("fso_set_dt", [C.c_double]),
+                           ("fso_set_bc", [C.c_int]), ("fso_set_u", [_dp]), ("fso_set_precip", [_dp]),
+                           ("fso_init_h", [_dp]), ("fso_set_h", [_dp]), ("fso_set_basement", [_dp]),
+                           ("fso_set_newton_tol", [C.c_double]),
+                           ("fso_set_erosional_parameters", [_dp, C.c_double, C.c_double, C.c_double, _dp,
+                                                             C.c_double, C.c_double, C.c_double, C.c_double]),
+                           ("fso_copy_field", [C.c_int, _dp]), ("fso_copy_ifield", [C.c_int, _ip]),
+                           ("fso_copy_don", [_ip]), ("fso_copy_mrec", [_ip, _dp, _dp]),
+                           ("fso_reset_cumulative_erosion", [])]:
+            f = getattr(lib, name)
+            f.argtypes = [C.c_void_p] + args
+            f.restype = None
+        lib.fso_execute_step.argtypes = [C.c_void_p]
+        lib.fso_execute_step.restype = C.c_int
+        lib.fso_get_step.argtypes = [C.c_void_p]
+        lib.fso_get_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
+        lib.fso_find_receiver.argtypes = [_dp, C.c_int, C.c_int, C.c_double, C.c_double, _ip, _dp] + [C.c_int] * 6
+        lib.fso_find_donor.argtypes = [_ip, C.c_int, _ip, _ip]
+        lib.fso_find_stack.argtypes = [_ip, _ip, _ip, C.c_int, _ip]
+        lib.fso_indexx.argtypes = [C.c_int, _dp, _ip]
+        lib.fso_bc_decode.argtypes = [C.c_int, C.c_int, C.c_int] + [_ip] * 6 + [C.c_void_p]
+        lib.fso_local_minima.argtypes = [_ip, _ip, C.c_void_p, _ip, _ip, _dp, _dp, C.c_int, C.c_int,
+                                         C.c_double, C.c_double, C.c_void_p]
+        lib.fso_local_minima.restype = C.c_int
+        lib.fso_connect_basins_dump.argtypes = [_ip, _ip, C.c_void_p, _dp, C.c_int, C.c_int, C.c_int, _ip,
+                                                _ip, _ip, _dp]
+        lib.fso_connect_basins_dump.restype = C.c_int
+        _clib = lib
+    return _clib
+
+
+FIELDS = {"h": 0, "b": 1, "etot": 2, "erate": 3, "area": 4, "lake_depth": 5, "length": 6, "hwater": 7,
+          "sedflux": 8}
+IFIELDS = {"rec": 0, "ndon": 1, "stack": 2, "mnrec": 3, "mstack": 4}
+
+
+class COracle:
+    """One landscape in the C restatement; method names follow the fastscape_* call surface."""
+
+    def __init__(self, nx: int, ny: int):
+        self.lib = _c()
+        self.nx, self.ny, self.nn = nx, ny, nx * ny
+        self.h_ = self.lib.fso_create(nx, ny)
+        if not self.h_:
+            raise ValueError("nx, ny must be >= 3")
+
+    def __del__(self):
+        if getattr(self, "h_", None):
+            self.lib.fso_destroy(self.h_)
+            self.h_ = None
+
+    def _arr(self, x):
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(x, dtype=np.float64), (self.nn,)))
+        return a
+
+    def set_xl_yl(self, xl, yl): self.lib.fso_set_xl_yl(self.h_, xl, yl)
+    def set_dt(self, dt): self.lib.fso_set_dt(self.h_, dt)
+    def set_bc(self, ibc): self.lib.fso_set_bc(self.h_, int(ibc))
+    def set_u(self, u): self.lib.fso_set_u(self.h_, _d(self._arr(u)))
+    def set_precip(self, p): self.lib.fso_set_precip(self.h_, _d(self._arr(p)))
+    def init_h(self, h): self.lib.fso_init_h(self.h_, _d(self._arr(h)))
+    def set_h(self, h): self.lib.fso_set_h(self.h_, _d(self._arr(h)))
+    def set_basement(self, b): self.lib.fso_set_basement(self.h_, _d(self._arr(b)))
+    def set_newton_tol(self, t): self.lib.fso_set_newton_tol(self.h_, t)
+    def reset_cumulative_erosion(self): self.lib.fso_reset_cumulative_erosion(self.h_)
+
+    def set_erosional_parameters(self, kf, kfsed, m, n, kd, kdsed, g1, g2, p):
+        self.lib.fso_set_erosional_parameters(self.h_, _d(self._arr(kf)), kfsed, m, n, _d(self._arr(kd)),
+                                              kdsed, g1, g2, p)
+
+    def execute_step(self):
+        rc = self.lib.fso_execute_step(self.h_)
+        if rc:
+            raise RuntimeError(f"oracle step failed with status {rc}")
+
+    def get_step(self): return self.lib.fso_get_step(self.h_)
+
+    def stats(self):
+        st = Stats()
+        self.lib.fso_get_stats(self.h_, C.byref(st))
+        return st.asdict()
+
+    def field(self, name):
+        out = np.empty(self.nn, np.float64)
+        self.lib.fso_copy_field(self.h_, FIELDS[name], _d(out))
+        return out
+
+    def ifield(self, name):
+        out = np.empty(self.nn, np.int32)
+        self.lib.fso_copy_ifield(self.h_, IFIELDS[name], _i(out))
+        return out
+
+    def donors(self):
+        """(ndon, don[nn,8]) with unused slots set to -1."""
+        ndon = self.ifield("ndon")
+        don = np.empty((self.nn, 8), np.int32)
+        self.lib.fso_copy_don(self.h_, _i(don))
+        don[np.arange(8)[None, :] >= ndon[:, None]] = -1
+        return ndon, don
+
+    def mrec(self):
+        mrec = np.empty((self.nn, 8), np.int32)
+        mw = np.empty((self.nn, 8), np.float64)
+        ml = np.empty((self.nn, 8), np.float64)
+        self.lib.fso_copy_mrec(self.h_, _i(mrec), _d(mw), _d(ml))
+        return mrec, mw, ml
+
+
+# ---- stateless routines ----------------------------------------------------------------------
+
+def bc_decode(ibc, nx, ny):
+    v = [C.c_int() for _ in range(6)]
+    bc = np.zeros(nx * ny, np.uint8)
+    _c().fso_bc_decode(ibc, nx, ny, *[C.byref(x) for x in v], bc.ctypes.data_as(C.c_void_p))
+    return tuple(x.value for x in v), bc
+
+
+def find_receiver(h, nx, ny, dx, dy, ibc=1111):
+    (i1, i2, j1, j2, xc, yc), _ = bc_decode(ibc, nx, ny)
+    h = np.ascontiguousarray(h, np.float64)
+    rec = np.empty(nx * ny, np.int32)
+    length = np.empty(nx * ny, np.float64)
+    _c().fso_find_receiver(_d(h), nx, ny, dx, dy, _i(rec), _d(length), i1, i2, j1, j2, xc, yc)
+    return rec, length
+
+
+def find_donor(rec):
+    nn = rec.size
+    ndon = np.empty(nn, np.int32)
+    don = np.full((nn, 8), -1, np.int32)
+    _c().fso_find_donor(_i(np.ascontiguousarray(rec, np.int32)), nn, _i(ndon), _i(don))
+    don[np.arange(8)[None, :] >= ndon[:, None]] = -1
+    return ndon, don
+
+
+def find_stack(rec, ndon, don):
+    nn = rec.size
+    stack = np.empty(nn, np.int32)
+    d = np.ascontiguousarray(don, np.int32)
+    _c().fso_find_stack(_i(np.ascontiguousarray(rec, np.int32)), _i(d), _i(np.ascontiguousarray(ndon, np.int32)),
+                        nn, _i(stack))
+    return stack
+
+
+def indexx(arr):
+    arr = np.ascontiguousarray(arr, np.float64)
+    out = np.empty(arr.size, np.int32)
+    _c().fso_indexx(arr.size, _d(arr), _i(out))
+    return out  # 1-based
+
+
+def local_minima(h, nx, ny, dx, dy, ibc=1111):
+    """Receivers + LocalMinima on h: returns rec, length, ndon, don, stack, stats."""
+    rec, length = find_receiver(h, nx, ny, dx, dy, ibc)
+    ndon, don = find_donor(rec)
+    stack = find_stack(rec, ndon, don)
+    _, bc = bc_decode(ibc, nx, ny)
+    st = Stats()
+    h = np.ascontiguousarray(h, np.float64)
+    rc = _c().fso_local_minima(_i(stack), _i(rec), bc.ctypes.data_as(C.c_void_p), _i(ndon), _i(don), _d(h),
+                               _d(length), nx, ny, dx, dy, C.byref(st))
+    if rc:
+        raise RuntimeError(f"local_minima status {rc}")
+    don[np.arange(8)[None, :] >= ndon[:, None]] = -1
+    return rec, length, ndon, don, stack, st.asdict()
+
+
+def connect_basins(h, nx, ny, dx, dy, ibc=1111):
+    """Basin graph as connect_basins builds it (before the MST): basin0, cb, cn, cw."""
+    rec, length = find_receiver(h, nx, ny, dx, dy, ibc)
+    ndon, don = find_donor(rec)
+    stack = find_stack(rec, ndon, don)
+    _, bc = bc_decode(ibc, nx, ny)
+    cap = 8 * nx * ny
+    cb = np.empty((cap, 2), np.int32)
+    cn = np.empty((cap, 2), np.int32)
+    cw = np.empty(cap, np.float64)
+    b0 = C.c_int()
+    h = np.ascontiguousarray(h, np.float64)
+    nc = _c().fso_connect_basins_dump(_i(stack), _i(rec), bc.ctypes.data_as(C.c_void_p), _d(h), nx, ny, cap,
+                                      C.byref(b0), _i(cb), _i(cn), _d(cw))
+    return b0.value, cb[:nc].copy(), cn[:nc].copy(), cw[:nc].copy()
+
+
+# ---- the reference Fortran -------------------------------------------------------------------
+
+class FortranRef:
+    """The reference library compiled from its own sources.  Arguments go by reference; node indices
+    in integer outputs are converted to 0-based."""
+
+    _lib = None
+
+    def __init__(self):
+        if FortranRef._lib is None:
+            if not ref_available():
+                raise FileNotFoundError(REF_LIB)
+            FortranRef._lib = C.CDLL(REF_LIB)
+        self.lib = FortranRef._lib
+        self.nn = 0
+
+    @staticmethod
+    def _ri(v): return C.byref(C.c_int(int(v)))
+
+    @staticmethod
+    def _rd(v): return C.byref(C.c_double(float(v)))
+
+    def _arr(self, x):
+        return np.ascontiguousarray(np.broadcast_to(np.asarray(x, dtype=np.float64), (self.nn,))).copy()
+
+    def setup(self, nx, ny):
+        self.nx, self.ny, self.nn = nx, ny, nx * ny
+        self.lib.fastscape_init_()
+        self.lib.fastscape_set_nx_ny_(self._ri(nx), self._ri(ny))
+        self.lib.fastscape_setup_()
+
+    def set_xl_yl(self, xl, yl): self.lib.fastscape_set_xl_yl_(self._rd(xl), self._rd(yl))
+    def set_dt(self, dt): self.lib.fastscape_set_dt_(self._rd(dt))
+    def set_bc(self, ibc): self.lib.fastscape_set_bc_(self._ri(ibc))
+    def set_u(self, u): self.lib.fastscape_set_u_(_d(self._arr(u)))
+    def set_precip(self, p): self.lib.fastscape_set_precip_(_d(self._arr(p)))
+    def init_h(self, h): self.lib.fastscape_init_h_(_d(self._arr(h)))
+
+    def set_erosional_parameters(self, kf, kfsed, m, n, kd, kdsed, g1, g2, p):
+        self.lib.fastscape_set_erosional_parameters_(_d(self._arr(kf)), self._rd(kfsed), self._rd(m), self._rd(n),
+                                                     _d(self._arr(kd)), self._rd(kdsed), self._rd(g1),
+                                                     self._rd(g2), self._rd(p))
+
+    def execute_step(self): self.lib.fastscape_execute_step_()
+
+    def get_step(self):
+        s = C.c_int()
+        self.lib.fastscape_get_step_(C.byref(s))
+        return s.value
+
+    _COPY = {"h": "fastscape_copy_h_", "b": "fastscape_copy_basement_", "etot": "fastscape_copy_total_erosion_",
+             "erate": "fastscape_copy_erosion_rate_", "area": "fastscape_copy_drainage_area_",
+             "lake_depth": "fastscape_copy_lake_depth_"}
+
+    def field(self, name):
+        out = np.empty(self.nn, np.float64)
+        getattr(self.lib, self._COPY[name])(_d(out))
+        return out
+
+    # -- internal routines, called on caller-owned arrays --
+    def find_receiver(self, h, nx, ny, dx, dy, ibc=1111):
+        (i1, i2, j1, j2, xc, yc), _ = bc_decode(ibc, nx, ny)
+        h = np.ascontiguousarray(h, np.float64)
+        rec = np.empty(nx * ny, np.int32)
+        length = np.empty(nx * ny, np.float64)
+        self.lib.find_receiver_(_d(h), self._ri(nx), self._ri(ny), self._rd(dx), self._rd(dy), _i(rec), _d(length),
+                                self._ri(i1), self._ri(i2), self._ri(j1), self._ri(j2), self._ri(xc), self._ri(yc))
+        return rec - 1, length
+
+    def routing(self, h, nx, ny, dx, dy, ibc=1111):
+        """find_receiver + find_donor + find_stack + LocalMinima on h, as FlowRoutingSingleFlowDirection."""
+        (i1, i2, j1, j2, xc, yc), bc = bc_decode(ibc, nx, ny)
+        nn = nx * ny
+        h = np.ascontiguousarray(h, np.float64).copy()
+        rec = np.empty(nn, np.int32)
+        length = np.empty(nn, np.float64)
+        self.lib.find_receiver_(_d(h), self._ri(nx), self._ri(ny), self._rd(dx), self._rd(dy), _i(rec), _d(length),
+                                self._ri(i1), self._ri(i2), self._ri(j1), self._ri(j2), self._ri(xc), self._ri(yc))
+        ndon = np.zeros(nn, np.int32)
+        don = np.zeros((nn, 8), np.int32)  # Fortran don(8,nn): 8 contiguous slots per node
+        self.lib.find_donor_(_i(rec), self._ri(nn), _i(ndon), _i(don))
+        stack = np.zeros(nn, np.int32)
+        c0 = np.zeros(nn)
+        c1 = np.zeros(nn)
+        self.lib.find_stack_(_i(rec), _i(don), _i(ndon), self._ri(nn), _d(c0), _i(stack), _d(c1))
+        lbc = bc.astype(np.int32)
+        self.lib.localminima_(_i(stack), _i(rec), _i(lbc), _i(ndon), _i(don), _d(h), _d(length), self._ri(nx),
+                              self._ri(ny), self._rd(dx), self._rd(dy))
+        don = don - 1
+        don[np.arange(8)[None, :] >= ndon[:, None]] = -1
+        return rec - 1, length, ndon, don, stack - 1
+
+    def indexx(self, arr):
+        arr = np.ascontiguousarray(arr, np.float64)
+        out = np.empty(arr.size, np.int32)
+        self.lib.loc_min_3_indexx_(self._ri(arr.size), _d(arr), _i(out))
+        return out  # 1-based
