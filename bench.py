@@ -1,0 +1,221 @@
+#!/usr/bin/env python3
+"""Throughput of the MI355X landscape step (FastScape_Execute_Step) on one GPU.
+
+A "step" is one FastScape_Execute_Step over every landscape of a batch resident in HBM:
+uplift, D8 receivers, closed-basin rerouting, drainage area, implicit stream power, ADI diffusion.
+The default workload is the uniform batch: 64 independent 513x513 landscapes (dx = 200 m), seeds
+0..63, h ~ U[0,1), kf log-uniform in [2e-6, 2e-5], uplift U ~ U[5e-4, 2e-3] m/yr inside and 0 on the
+fixed boundary, m = 0.4, n = 1, kd = 1e-2, dt = 1e3 yr.  The same landscape run alone gives the plain
+one-at-a-time figure.  Synthetic, seeded inputs; FP64 throughout.
+
+Prints ONE JSON line: metric, value (batched cell-steps/s), plain rate and the batched/plain ratio,
+the roofline of the dominant kernel (HIP events recorded around it on the stream it runs on during
+the timed region), and the reference Fortran timed on this host's cores on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import multiprocessing as mp
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "fastscapelib-fortran_amd"))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "cell-steps/s batched+served vs plain on 1 GPU; p50/p99 run latency"
+
+# Algorithmic (compulsory) bytes of the single-kernel phases: every input read once and every
+# output written once, per unit (node of the batch unless stated).  See DESIGN.md.
+ALG_BYTES_PER_NODE = {
+    "uplift": 40,        # R h,u,b  W h,b
+    "receivers": 20,     # R h  W rec(i32), length
+    "donors": 13,        # R rec  W dmask(u8), first, next
+    "lm_walk": 24,       # R h, basin, first, next, rec (nodes of closed basins ~ all nodes)
+    "area_prep": 61,     # R J, dmask, pos, precip  W A0, donor slots(8 x i32)
+    "area_sweep": 60,    # R A0, donor slots, order  W Al, area
+    "spl_prep": 48,      # R h, b, kf  W kfint, g, ht
+    "spl_fact": 72,      # R J, rec, pos(x2), area, length, kfint, ht  W rp, F, HTl
+    "spl_sweep": 56,     # R order, rp, F, HTl, Hl  W h, hwater, lake_depth
+    "spl_post": 64,      # R h, b, ht, etot  W b, etot, erate, sedflux
+}
+ALG_BYTES_PER_EDGE = {"lm_mst": 13, "lm_orient": 33}  # per basin-graph edge
+
+
+def uniform_batch(ninst=64, n=513, dx=200.0):
+    """Uniform batch recipe; per landscape: h noise, then kf, then U, from PCG64(seed)."""
+    out = []
+    for seed in range(ninst):
+        rng = np.random.default_rng(seed)
+        nn = n * n
+        h = rng.random(nn)
+        kf = 10.0 ** rng.uniform(np.log10(2e-6), np.log10(2e-5))
+        U = rng.uniform(5e-4, 2e-3)
+        u = np.full(nn, U)
+        e = np.zeros((n, n), bool)
+        e[0, :] = e[-1, :] = e[:, 0] = e[:, -1] = True
+        u[e.ravel()] = 0.0
+        out.append(dict(nx=n, ny=n, xl=dx * (n - 1), yl=dx * (n - 1), h=h, u=u, kf=kf, m=0.4, n=1.0, kd=1e-2,
+                        dt=1e3, ibc=1111))
+    return out
+
+
+def configure(obj, L, *k):
+    obj.set_xl_yl(*k, L["xl"], L["yl"])
+    obj.set_dt(*k, L["dt"])
+    obj.init_h(*k, L["h"])
+    nn = L["nx"] * L["ny"]
+    obj.set_erosional_parameters(*k, np.full(nn, L["kf"]), -1.0, L["m"], L["n"], np.full(nn, L["kd"]), -1.0, 0.0,
+                                 0.0, -2.0)
+    obj.set_u(*k, L["u"])
+    obj.set_bc(*k, L["ibc"])
+
+
+def _sync():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+
+
+def run_gpu(fs, Ls, steps, warmup, profile):
+    with fs.Batch([(L["nx"], L["ny"]) for L in Ls]) as b:
+        for k, L in enumerate(Ls):
+            configure(b, L, k)
+        if warmup:
+            b.step(warmup)
+        if profile:
+            b.profiling(True)
+            b.reset_timers()
+        _sync()
+        t0 = time.perf_counter()
+        b.step(steps)
+        _sync()
+        dt = time.perf_counter() - t0
+        timers = b.timers() if profile else None
+        stats = [b.stats(k) for k in range(len(Ls))]
+        mem = b.memory()
+    return dt, timers, stats, mem
+
+
+# ---- CPU baseline: the reference Fortran (oracle/_ref), one landscape per process -------------------
+def _cpu_worker(args):
+    L, steps, use_ref = args
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    if use_ref:
+        o = O.FortranRef()
+        o.setup(L["nx"], L["ny"])
+    else:
+        o = O.COracle(L["nx"], L["ny"])
+    t0 = time.perf_counter()
+    configure(o, L)
+    for _ in range(steps):
+        o.execute_step()
+    o.field("h")
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(Ls, steps, procs):
+    ref_lib = os.path.join(ROOT, "oracle", "_ref", "libfsref.so")
+    use_ref = os.path.exists(ref_lib)
+    if not use_ref:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        O.build_c()
+    jobs = [(Ls[i % len(Ls)], steps, use_ref) for i in range(procs)]
+    ctx = mp.get_context("spawn")
+    t0 = time.perf_counter()
+    with ctx.Pool(procs) as pool:
+        times = pool.map(_cpu_worker, jobs)
+    makespan = time.perf_counter() - t0
+    cells = sum(L["nx"] * L["ny"] for L, _, _ in jobs)
+    return {"value": cells * steps / makespan, "unit": "cell-steps/s", "cores": procs,
+            "kind": "reference" if use_ref else "port",
+            "sample": f"{procs} x {Ls[0]['nx']}x{Ls[0]['ny']} landscapes of the batch, {steps} steps each, one "
+                      f"process per core ({'unmodified reference Fortran, flang -O2' if use_ref else 'C port'}); "
+                      f"makespan {makespan:.1f} s, per-process {min(times):.1f}-{max(times):.1f} s"}
+
+
+def roofline(timers, stats, ncells):
+    """Dominant single-kernel phase by time; achieved = algorithmic bytes per launch / mean launch time."""
+    best = None
+    for ph, (ms, cnt) in timers.items():
+        if cnt == 0:
+            continue
+        if ph in ALG_BYTES_PER_NODE:
+            bytes_per_launch = ALG_BYTES_PER_NODE[ph] * ncells
+        elif ph in ALG_BYTES_PER_EDGE:
+            bytes_per_launch = ALG_BYTES_PER_EDGE[ph] * sum(s["nconn"] for s in stats)
+        else:
+            continue
+        launches = cnt * (2 if ph == "donors" else 1)
+        avg_ms = ms / launches
+        if best is None or ms > best[1]:
+            best = (ph, ms, avg_ms, bytes_per_launch)
+    ph, _, avg_ms, nbytes = best
+    achieved = nbytes / (avg_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": ph,
+            "avg_launch_ms": round(avg_ms, 4), "alg_bytes_per_launch": int(nbytes)}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--ninst", type=int, default=64)
+    ap.add_argument("--n", type=int, default=513)
+    ap.add_argument("--plain-steps", type=int, default=40)
+    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--cpu-procs", type=int, default=0, help="0: min(8, host cores)")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (landscapes shard "
+              "across GPUs as independent batches, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import fastscapelib_fortran as fs
+
+    Ls = uniform_batch(args.ninst, args.n)
+    ncells = sum(L["nx"] * L["ny"] for L in Ls)
+    dt, timers, stats, mem = run_gpu(fs, Ls, args.steps, args.warmup, profile=True)
+    value = ncells * args.steps / dt
+
+    # plain path: one landscape at a time (the first of the batch)
+    pdt, _, _, _ = run_gpu(fs, Ls[:1], args.plain_steps, 1, profile=False)
+    plain = Ls[0]["nx"] * Ls[0]["ny"] * args.plain_steps / pdt
+
+    phases = {p: round(ms / max(1, args.steps), 4) for p, (ms, c) in timers.items() if c}
+    out = {
+        "metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded uniform-noise landscapes)",
+        "config": {"workload": f"uniform batch {args.ninst} x {args.n}x{args.n} D8 n=1 g=0 kd=1e-2 fixed BC",
+                   "landscapes": args.ninst, "cells": ncells},
+        "plain": {"value": plain, "ms_per_step": pdt / args.plain_steps * 1e3,
+                  "workload": f"1 x {args.n}x{args.n}, same parameters"},
+        "batched_over_plain": value / plain,
+        "roofline": roofline(timers, stats, ncells),
+        "phase_ms_per_step": phases,
+        "step_model_frac": round(336.0 * ncells / (dt / args.steps) / 1e9 / HBM_PEAK_GBS, 5),
+        "hbm_bytes_resident": mem[0],
+        "basins_per_landscape": int(np.mean([s["nbasins"] for s in stats])),
+        "levels_per_landscape": int(np.mean([s["levels"] for s in stats])),
+    }
+    if not args.no_cpu:
+        procs = args.cpu_procs or min(8, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(Ls, args.cpu_steps, procs)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -14,6 +14,20 @@
 
 static thread_local std::string g_err;
 
+// Values of one level kept in LDS by the sweeps (wider levels fall back to global memory); sized
+// once from the device's LDS per workgroup.
+int g_sweep_cap = 0;
+
+static int configure_sweeps() {
+  if (g_sweep_cap) return 0;
+  int dev = 0, smem = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return -1;
+  if (hipDeviceGetAttribute(&smem, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return -1;
+  if (sweep_set_lds(smem) != 0) return -1;
+  g_sweep_cap = smem;
+  return 0;
+}
+
 static int fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
@@ -36,7 +50,53 @@ struct fsx_ctx {
   hipStream_t stream = nullptr;
   bool dirty = true;
   long long bytes = 0;
+  // phase timers: a ring of (start, stop) event pairs, accumulated every kRing steps
+  bool prof = false;
+  std::vector<hipEvent_t> ev;  // kRing * PH_COUNT * 2
+  int ring_used = 0;
+  std::vector<unsigned char> rec_mask;  // which pairs of the ring were recorded
+  double acc_ms[PH_COUNT] = {0};
+  long long acc_n[PH_COUNT] = {0};
 };
+static const int kRing = 64;
+
+static int flush_timers(fsx_ctx* c, hipStream_t st) {
+  if (!c->prof || c->ring_used == 0) return 0;
+  HIPCHK(hipStreamSynchronize(st));
+  for (int s = 0; s < c->ring_used; s++)
+    for (int p = 0; p < PH_COUNT; p++) {
+      const int k = s * PH_COUNT + p;
+      if (!c->rec_mask[k]) continue;
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, c->ev[2 * k], c->ev[2 * k + 1]));
+      c->acc_ms[p] += ms;
+      c->acc_n[p] += 1;
+      c->rec_mask[k] = 0;
+    }
+  c->ring_used = 0;
+  return 0;
+}
+
+struct PhaseMark {
+  fsx_ctx* c;
+  hipStream_t st;
+  int k;
+  PhaseMark(fsx_ctx* c_, hipStream_t st_, int ph) : c(c_), st(st_), k(-1) {
+    if (!c->prof) return;
+    k = c->ring_used * PH_COUNT + ph;
+    (void)hipEventRecord(c->ev[2 * k], st);
+  }
+  ~PhaseMark() {
+    if (k < 0) return;
+    (void)hipEventRecord(c->ev[2 * k + 1], st);
+    c->rec_mask[k] = 1;
+  }
+};
+#define PHASE(ph, call)                    \
+  do {                                     \
+    PhaseMark pm_(c, st, ph);              \
+    call;                                  \
+  } while (0)
 
 extern "C" const char* fsx_last_error(void) { return g_err.c_str(); }
 extern "C" const char* fsx_version(void) { return "fastscape-mi355x 0.1"; }
@@ -157,6 +217,7 @@ extern "C" int fsx_create(int ninst, const int* nx, const int* ny, fsx_ctx** out
   S.nchunks = (int)chunks.size();
   const long long N = off;
   int rc = 0;
+  if (configure_sweeps() != 0) { delete c; return fail(FSX_EHIP, "cannot configure LDS of the sweep kernels"); }
   hipError_t he = hipStreamCreate(&c->stream);
   if (he != hipSuccess) { delete c; return fail(FSX_EHIP, "hipStreamCreate failed"); }
 #define A(p, n) if (!rc) rc = dalloc(c, &S.p, (n))
@@ -164,7 +225,7 @@ extern "C" int fsx_create(int ninst, const int* nx, const int* ny, fsx_ctx** out
   A(h, N); A(b, N); A(u, N); A(kf, N); A(kd, N); A(precip, N); A(etot, N); A(erate, N); A(area, N);
   A(length, N); A(hwater, N); A(lake_depth, N); A(sedflux, N); A(ht, N); A(water, N); A(zint, N); A(zintp, N);
   A(gam, N); A(dh, N); A(hp, N); A(lwv, N); A(lsed, N); A(kfint, N); A(kdint, N); A(g, N); A(pexp, N); A(elev, N);
-  A(rec, N); A(lbl, N); A(basin, N); A(rootrank, N); A(order, N); A(lstart, N + ninst); A(lsill, N); A(stack, N);
+  A(rec, N); A(basin, N); A(rootrank, N); A(order, N); A(lstart, N + ninst); A(lsill, N);
   A(dmask, N); A(bc, N);
   A(outlet, N); A(bsize, N); A(soff, N); A(ecount, N); A(eoff, N); A(comp, N); A(bpar, N); A(bnew, N); A(best, N);
   A(tdeg, N); A(toff, N); A(tadj, 2 * N); A(bfront, N);
@@ -174,7 +235,9 @@ extern "C" int fsx_create(int ninst, const int* nx, const int* ny, fsx_ctx** out
   A(sB, FSX_SLAB_CAP * N); A(sp, FSX_SLAB_CAP * N); A(sq, FSX_SLAB_CAP * N); A(sw, FSX_SLAB_CAP * N);
   A(nbasins, ninst); A(basin0, ninst); A(nconn, ninst); A(nlev, ninst); A(err, ninst); A(ngs, ninst);
   A(ntree, ninst); A(chunkcnt, S.nchunks); A(newton, ninst); A(scal, ninst); A(flags, 64); A(cbase, ninst + 1);
-  A(nsegs, ninst); A(segctr, ninst);
+  A(nsegs, ninst); A(segctr, ninst); A(maxd, ninst);
+  A(J, N); A(ngroups, ninst); A(grec, 2 * (N + ninst)); A(first, N); A(next, N); A(gmeta, N + ninst); A(pos, N); A(lcur, N + ninst); A(rp, N); A(LSl, N); A(dn, 8 * N);
+  A(A0, N); A(Al, N); A(F, N); A(HTl, N); A(Hl, N); A(Wl, N); A(HWl, N); A(HPl, N); A(DHl, N); A(ELl, N);
 #undef A
   if (rc) { fsx_destroy(c); return rc; }
   he = hipMemcpy(S.chunks, chunks.data(), sizeof(Chunk) * chunks.size(), hipMemcpyHostToDevice);
@@ -196,6 +259,7 @@ extern "C" int fsx_create(int ninst, const int* nx, const int* ny, fsx_ctx** out
 extern "C" int fsx_destroy(fsx_ctx* c) {
   if (!c) return 0;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& e : c->ev) (void)hipEventDestroy(e);
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -358,22 +422,38 @@ static int check_errors(fsx_ctx* c) {
 
 static void enqueue_routing(fsx_ctx* c, hipStream_t st) {
   DevState& S = c->S;
-  launch_receivers(S, st);
-  launch_donors(S, st);
-  launch_local_minima(S, st);
-  launch_donors(S, st);
-  launch_levels_fill(S, st);
-  launch_area(S, st);
+  PHASE(PH_RECEIVERS, launch_receivers(S, st));
+  PHASE(PH_DONORS, launch_donors(S, st));
+  // LocalMinima, src/LocalMinima.f90:1-132
+  PHASE(PH_LM_LABELS, launch_lm_labels(S, st));
+  PHASE(PH_LM_BASINS, launch_lm_basins(S, st));
+  PHASE(PH_LM_WALK, launch_lm_walk(S, st));
+  PHASE(PH_LM_EDGES, launch_lm_edges(S, st));
+  PHASE(PH_LM_SORT, launch_nr_sort(S, st));
+  PHASE(PH_LM_MST, launch_lm_mst(S, st));
+  PHASE(PH_LM_ORIENT, launch_lm_orient(S, st));
+  PHASE(PH_LM_CORRECT, launch_lm_correct(S, st));
+  PHASE(PH_DONORS, launch_donors(S, st));
+  PHASE(PH_LEVELS, launch_levels(S, st));
+  PHASE(PH_AREA_PREP, launch_area_prep(S, st));
+  PHASE(PH_AREA_SWEEP, launch_area_sweep(S, st));
+}
+
+static void enqueue_route_only(fsx_ctx* c, hipStream_t st) {
+  enqueue_routing(c, st);
+  launch_spl_prep(c->S, st);
+  launch_fill(c->S, st);  // lake surface / depth without eroding
 }
 
 static void enqueue_step(fsx_ctx* c, hipStream_t st) {
   DevState& S = c->S;
-  launch_uplift(S, st);
+  PHASE(PH_UPLIFT, launch_uplift(S, st));
   enqueue_routing(c, st);
-  launch_spl_prep(S, st);
-  launch_spl(S, st);
-  launch_spl_post(S, st);
-  launch_diffusion(S, st);
+  PHASE(PH_SPL_PREP, launch_spl_prep(S, st));
+  PHASE(PH_SPL_FACT, launch_spl_fact(S, st));
+  PHASE(PH_SPL_SWEEP, launch_spl_sweep(S, st));  // also fills hwater / lake_depth
+  PHASE(PH_SPL_POST, launch_spl_post(S, st));
+  PHASE(PH_DIFFUSION, launch_diffusion(S, st));
 }
 
 extern "C" int fsx_execute_steps(fsx_ctx* c, int nsteps, void* stream, int sync) {
@@ -383,7 +463,11 @@ extern "C" int fsx_execute_steps(fsx_ctx* c, int nsteps, void* stream, int sync)
   if (stream && c->dirty) HIPCHK(hipStreamSynchronize(c->stream));
   if (int r = upload_params(c)) return r;
   if (stream) HIPCHK(hipStreamSynchronize(c->stream));
-  for (int s = 0; s < nsteps; s++) enqueue_step(c, st);
+  for (int s = 0; s < nsteps; s++) {
+    enqueue_step(c, st);
+    if (c->prof && ++c->ring_used == kRing)
+      if (int r = flush_timers(c, st)) return r;
+  }
   HIPCHK(hipGetLastError());
   for (int k = 0; k < c->S.ninst; k++)
     if (c->hp[k].active) c->steps[k] += nsteps;
@@ -405,7 +489,7 @@ extern "C" int fsx_route(fsx_ctx* c, void* stream, int sync) {
   hipStream_t st = stream ? (hipStream_t)stream : c->stream;
   if (int r = upload_params(c)) return r;
   if (stream) HIPCHK(hipStreamSynchronize(c->stream));
-  enqueue_routing(c, st);
+  enqueue_route_only(c, st);
   HIPCHK(hipGetLastError());
   if (sync) {
     HIPCHK(hipStreamSynchronize(st));
@@ -507,6 +591,37 @@ extern "C" int fsx_get_stats(fsx_ctx* c, int inst, long long* out) {
   HIPCHK(hipMemcpy(&nw, c->S.newton + inst, sizeof(long long), hipMemcpyDeviceToHost));
   for (int k = 0; k < 5; k++) out[k] = v[k];
   out[5] = nw;
+  return 0;
+}
+
+extern "C" int fsx_set_profiling(fsx_ctx* c, int on) {
+  if (!c) return fail(FSX_EINVAL, "null context");
+  if (on && c->ev.empty()) {
+    c->ev.resize((size_t)kRing * PH_COUNT * 2);
+    for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
+    c->rec_mask.assign((size_t)kRing * PH_COUNT, 0);
+  }
+  if (!on && c->prof) {
+    if (int r = flush_timers(c, c->stream)) return r;
+  }
+  c->prof = on != 0;
+  return 0;
+}
+
+extern "C" int fsx_get_timers(fsx_ctx* c, double* ms, long long* counts, int n) {
+  if (!c) return fail(FSX_EINVAL, "null context");
+  HIPCHK(hipDeviceSynchronize());
+  if (int r = flush_timers(c, c->stream)) return r;
+  for (int p = 0; p < n && p < PH_COUNT; p++) {
+    if (ms) ms[p] = c->acc_ms[p];
+    if (counts) counts[p] = c->acc_n[p];
+  }
+  return PH_COUNT;
+}
+
+extern "C" int fsx_reset_timers(fsx_ctx* c) {
+  if (!c) return fail(FSX_EINVAL, "null context");
+  for (int p = 0; p < PH_COUNT; p++) { c->acc_ms[p] = 0; c->acc_n[p] = 0; }
   return 0;
 }
 

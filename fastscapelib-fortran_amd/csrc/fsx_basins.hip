@@ -14,6 +14,7 @@
 //  5. orientation from basin0 (order_tree, :389-533) by a breadth-first walk of the tree.
 //  6. receiver path reversal per tree edge (correct_receivers_v2, :601-662), one thread per edge;
 //     paths lie in disjoint basins, so edges are independent.
+#include <algorithm>
 #include <climits>
 
 #include "fsx_scan.h"
@@ -30,28 +31,12 @@ __global__ void k_fill_int(int* __restrict__ a, long long n, int val) {
     a[i] = val;
 }
 
-// ---- 1. labels ---------------------------------------------------------------------------------
-__global__ __launch_bounds__(FSX_CHUNK) void k_lbl_init(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
-                                                        const int* __restrict__ rec, int* __restrict__ lbl) {
-  NODE_PROLOGUE
-  if (!live) return;
-  lbl[P.off + v] = rec[P.off + v];
+void launch_fill_int(int* a, long long n, int val, hipStream_t st) {
+  const int blocks = (int)std::min<long long>(1024, (n + 255) / 256 > 0 ? (n + 255) / 256 : 1);
+  k_fill_int<<<blocks, 256, 0, st>>>(a, n, val);
 }
 
-__global__ __launch_bounds__(FSX_CHUNK) void k_lbl_jump(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
-                                                        int* __restrict__ lbl, int* __restrict__ flags, int iter) {
-  if (iter > 0 && flags[iter - 1] == 0) return;
-  NODE_PROLOGUE
-  if (!live) return;
-  int* L = lbl + P.off;
-  const int x = L[v];
-  const int y = L[x];
-  if (x != y) {
-    L[v] = y;
-    flags[iter] = 1;
-  }
-}
-
+// ---- 1. labels: root of every node from the pointer-jumping pass (fsx_tree.hip) ----------------
 // roots per chunk
 __global__ __launch_bounds__(FSX_CHUNK) void k_root_count(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
                                                           const int* __restrict__ rec, int* __restrict__ chunkcnt) {
@@ -104,15 +89,16 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_root_rank(const InstParams* __res
 }
 
 __global__ __launch_bounds__(FSX_CHUNK) void k_basin_label(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
-                                                           const int* __restrict__ lbl, const int* __restrict__ rootrank,
-                                                           const uint8_t* __restrict__ bc, int* __restrict__ basin,
-                                                           int* __restrict__ bsize) {
+                                                           const unsigned long long* __restrict__ J,
+                                                           const int* __restrict__ rootrank, const uint8_t* __restrict__ bc,
+                                                           int* __restrict__ basin, int* __restrict__ bsize) {
   NODE_PROLOGUE
   if (!live) return;
-  const int r = lbl[P.off + v];
+  const int r = (int)(unsigned)(J[P.off + v] & 0xffffffffu);
   const int b = rootrank[P.off + r];
-  basin[P.off + v] = b;
-  if (!bc[P.off + r]) atomicAdd(bsize + P.off + b, 1);
+  const bool closed = !bc[P.off + r];
+  basin[P.off + v] = closed ? b : -(b + 1);  // signed: open basins negative
+  if (closed) atomicAdd(bsize + P.off + b, 1);
 }
 
 // exclusive scan of closed-basin sizes -> candidate slab offsets; clears edge counts
@@ -139,77 +125,102 @@ __global__ __launch_bounds__(FSX_WG) void k_bsize_scan(const InstParams* __restr
 }
 
 // ---- 2. passes (connect_basins) -------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_basin_walk(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
-                                                   const int* __restrict__ nbasins, const int* __restrict__ outlet,
-                                                   const uint8_t* __restrict__ bc, const int* __restrict__ rec,
-                                                   const uint8_t* __restrict__ dmask, const int* __restrict__ basin,
-                                                   const double* __restrict__ h, const int* __restrict__ soff,
-                                                   const int* __restrict__ bsize, int* __restrict__ ecount,
-                                                   int* __restrict__ sB, int* __restrict__ sp, int* __restrict__ sq,
-                                                   double* __restrict__ sw, int* __restrict__ err) {
-  const Chunk c = chunks[blockIdx.x >> 2];
-  const InstParams& P = PP[c.inst];
-  const int b = c.start + (int)((blockIdx.x & 3) * 64 + threadIdx.x);
-  if (!P.active || !P.run_spl || b >= nbasins[c.inst]) return;
-  const int off = P.off, nx = P.nx, ny = P.ny;
-  const int r = outlet[off + b];
-  if (bc[off + r]) return;  // open basin: no scan
-  const int* R = rec + off;
-  const uint8_t* M = dmask + off;
-  const int* BS = basin + off;
-  const int* OUT = outlet + off;
-  const uint8_t* BC = bc + off;
-  const double* H = h + off;
-  const long long sbase = (long long)FSX_SLAB_CAP * (off + soff[off + b]);
-  int* TB = sB + sbase;
-  int* TP = sp + sbase;
-  int* TQ = sq + sbase;
-  double* TW = sw + sbase;
+#define WALK_TCAP 16  // pass-table entries per thread kept in LDS; further entries live in the slab
+#define WALK_GX 64  // blocks per landscape; each thread walks basins b = x, x + WALK_GX*64, ...
+__global__ __launch_bounds__(64) void k_basin_walk(const InstParams* __restrict__ PP, const int* __restrict__ nbasins,
+                                                   const uint8_t* __restrict__ bc, const int* __restrict__ outlet,
+                                                   const int* __restrict__ rec, const int* __restrict__ first,
+                                                   const int* __restrict__ next,
+                                                   const int* __restrict__ basin, const double* __restrict__ h,
+                                                   const int* __restrict__ soff, const int* __restrict__ bsize,
+                                                   int* __restrict__ ecount, int* __restrict__ sB, int* __restrict__ sp,
+                                                   int* __restrict__ sq, double* __restrict__ sw, int* __restrict__ err) {
+  __shared__ int lB[WALK_TCAP * 64];
+  __shared__ double lW[WALK_TCAP * 64];
+  int* LB = lB + threadIdx.x;
+  double* LW = lW + threadIdx.x;
   constexpr int II[8] = {-1, 1, 0, 0, -1, 1, -1, 1};
   constexpr int JJ[8] = {0, 0, -1, 1, -1, -1, 1, 1};
-  int ne = 0;
-  int v = r;
-  const int cap = FSX_SLAB_CAP * bsize[off + b];
-  for (int visits = 0;; visits++) {
-    if (visits > P.nn) { atomicOr(err + c.inst, FSX_ERR_LEVELS); break; }
-    // scan the 8 neighbours of v (no wrap), src/LocalMinima.f90:281-320
-    const int i = v % nx, j = v / nx;
-    const double hv = H[v];
+  const int inst = blockIdx.y;
+  const InstParams& P = PP[inst];
+  if (!P.active || !P.run_spl) return;
+  const int nb = nbasins[inst];
+  const int2 it = make_int2(inst, 0);
+  for (int b = blockIdx.x * 64 + threadIdx.x; b < nb; b += WALK_GX * 64) {
+    const int off = P.off, nx = P.nx, ny = P.ny;
+    if (bc[off + outlet[off + b]]) continue;  // open basin: no scan
+    const int r = outlet[off + b];
+    const int* R = rec + off;
+    const int* FI = first + off;
+    const int* NX = next + off;
+    const int* BS = basin + off;
+    const double* H = h + off;
+    const long long sbase = (long long)FSX_SLAB_CAP * (off + soff[off + b]);
+    int* TB = sB + sbase;
+    int* TP = sp + sbase;
+    int* TQ = sq + sbase;
+    double* TW = sw + sbase;
+    const int cap = FSX_SLAB_CAP * bsize[off + b];
+    int ne = 0;
+    int v = r;
+    for (int visits = 0;; visits++) {
+      if (visits > P.nn) { atomicOr(err + it.x, FSX_ERR_LEVELS); break; }
+      // scan the 8 neighbours of v (no wrap), src/LocalMinima.f90:281-320
+      const int i = v % nx, j = v / nx;
+      const double hv = H[v];
+      const int fv = FI[v];
+      int qs[8], bs[8];
+      double hq[8];
 #pragma unroll
-    for (int k = 0; k < 8; k++) {
-      const int ki = i + II[k], kj = j + JJ[k];
-      if (ki < 0 || ki >= nx || kj < 0 || kj >= ny) continue;
-      const int q = kj * nx + ki;
-      const int B = BS[q];
-      if (b >= B && !BC[OUT[B]]) continue;
-      const double hq = H[q];
-      const double w = hv > hq ? hv : hq;
-      int t = 0;
-      while (t < ne && TB[t] != B) t++;
-      if (t == ne) {
-        if (ne >= cap) { atomicOr(err + c.inst, FSX_ERR_NCONN); continue; }
-        TB[t] = B; TP[t] = v; TQ[t] = q; TW[t] = w;
-        ne++;
-      } else if (w < TW[t]) {
-        TP[t] = v; TQ[t] = q; TW[t] = w;
+      for (int k = 0; k < 8; k++) {
+        const int ki = i + II[k], kj = j + JJ[k];
+        const bool in = !(ki < 0 || ki >= nx || kj < 0 || kj >= ny);
+        qs[k] = in ? kj * nx + ki : -1;
+        bs[k] = in ? BS[qs[k]] : 0;
+        hq[k] = in ? H[qs[k]] : 0.0;
       }
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        if (qs[k] < 0) continue;
+        const int Bs = bs[k];
+        const bool closedB = Bs >= 0;
+        const int B = closedB ? Bs : -(Bs + 1);
+        if (b >= B && closedB) continue;
+        const double w = hv > hq[k] ? hv : hq[k];
+        int t = 0;
+        const int nl = ne < WALK_TCAP ? ne : WALK_TCAP;
+        while (t < nl && LB[64 * t] != B) t++;
+        if (t == nl) {
+          while (t < ne && TB[t] != B) t++;
+        }
+        if (t == ne) {
+          if (ne >= cap) { atomicOr(err + it.x, FSX_ERR_NCONN); continue; }
+          if (t < WALK_TCAP) { LB[64 * t] = B; LW[64 * t] = w; }
+          else { TB[t] = B; TW[t] = w; }
+          TP[t] = v; TQ[t] = qs[k];
+          ne++;
+        } else {
+          const double tw = t < WALK_TCAP ? LW[64 * t] : TW[t];
+          if (w < tw) {
+            if (t < WALK_TCAP) LW[64 * t] = w; else TW[t] = w;
+            TP[t] = v; TQ[t] = qs[k];
+          }
+        }
+      }
+      // next node in depth-first preorder: first donor, else next sibling up the receiver chain
+      if (fv >= 0) { v = fv; continue; }
+      bool found = false;
+      while (v != r) {
+        const int nx_ = NX[v];
+        if (nx_ >= 0) { v = nx_; found = true; break; }
+        v = R[v];
+      }
+      if (!found) break;
     }
-    // next node in depth-first preorder: first donor, else next sibling up the receiver chain
-    int d[8];
-    if (donors_sorted(P, v, M[v], d) > 0) { v = d[0]; continue; }
-    bool found = false;
-    while (v != r) {
-      const int p = R[v];
-      const int np = donors_sorted(P, p, M[p], d);
-      int nxt = -1;
-      for (int k = 0; k < np; k++)
-        if (d[k] > v) { nxt = d[k]; break; }
-      if (nxt >= 0) { v = nxt; found = true; break; }
-      v = p;
-    }
-    if (!found) break;
+    const int nl = ne < WALK_TCAP ? ne : WALK_TCAP;
+    for (int t = 0; t < nl; t++) { TB[t] = LB[64 * t]; TW[t] = LW[64 * t]; }
+    ecount[off + b] = ne;
   }
-  ecount[off + b] = ne;
 }
 
 // edge offsets: basin order, open basins (except basin0) one virtual edge each
@@ -727,34 +738,49 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_correct(const InstParams* __restr
 
 // ---------------------------------------------------------------------------------------------------
 void launch_nr_sort(DevState& S, hipStream_t st) {
-  k_fill_int<<<1, 256, 0, st>>>(S.segctr, S.ninst, 0);
+  launch_fill_int(S.segctr, S.ninst, 0, st);
   k_nr_big<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.nconn, S.cw, S.skey, S.sid, S.Lasc, S.Rasc, S.segs, S.nsegs, S.err);
   k_nr_small<<<dim3(8, S.ninst), 256, 0, st>>>(S.P, S.nsegs, S.segctr, S.skey, S.sid, S.Lasc, S.Rasc, S.segs, S.err);
   k_nr_pos<<<dim3(16, S.ninst), FSX_CHUNK, 0, st>>>(S.P, S.nconn, S.sid, S.npos);
 }
 
-void launch_local_minima(DevState& S, hipStream_t st) {
+void launch_lm_labels(DevState& S, hipStream_t st) {
+  launch_jump(S, st);  // J = (depth, root) of the steepest-descent forest
+}
+
+void launch_lm_basins(DevState& S, hipStream_t st) {
   const int nb = S.nchunks;
-  k_lbl_init<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.rec, S.lbl);
-  k_fill_int<<<1, 64, 0, st>>>(S.flags, 64, 0);
-  for (int it = 0; it < 26; it++) k_lbl_jump<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.lbl, S.flags, it);
   k_root_count<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.rec, S.chunkcnt);
   k_root_scan<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.chunkcnt, S.cbase, S.nbasins, S.basin0);
   k_root_rank<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.rec, S.bc, S.chunkcnt, S.rootrank, S.outlet, S.basin0);
-  k_fill_int<<<1024, 256, 0, st>>>(S.bsize, S.ntot, 0);
-  k_basin_label<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.lbl, S.rootrank, S.bc, S.basin, S.bsize);
+  launch_fill_int(S.bsize, S.ntot, 0, st);
+  k_basin_label<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.J, S.rootrank, S.bc, S.basin, S.bsize);
   k_bsize_scan<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.nbasins, S.bsize, S.soff, S.ecount);
-  k_basin_walk<<<nb * 4, 64, 0, st>>>(S.P, S.chunks, S.nbasins, S.outlet, S.bc, S.rec, S.dmask, S.basin, S.h, S.soff,
-                                       S.bsize, S.ecount, S.sB, S.sp, S.sq, S.sw, S.err);
+}
+
+void launch_lm_walk(DevState& S, hipStream_t st) {
+  k_basin_walk<<<dim3(WALK_GX, S.ninst), 64, 0, st>>>(S.P, S.nbasins, S.bc, S.outlet, S.rec, S.first, S.next, S.basin, S.h,
+                                                       S.soff, S.bsize, S.ecount, S.sB, S.sp, S.sq, S.sw, S.err);
+}
+
+void launch_lm_edges(DevState& S, hipStream_t st) {
   k_edge_scan<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.nbasins, S.basin0, S.outlet, S.bc, S.ecount, S.eoff, S.nconn, S.err);
-  k_edge_fill<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.nbasins, S.basin0, S.nconn, S.outlet, S.bc, S.ecount, S.eoff,
-                                         S.soff, S.sB, S.sp, S.sq, S.sw, S.cb0, S.cb1, S.cn0, S.cn1, S.cw);
-  launch_nr_sort(S, st);
-  k_fill_int<<<1, 256, 0, st>>>(S.ntree, S.ninst, 0);
+  k_edge_fill<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.nbasins, S.basin0, S.nconn, S.outlet, S.bc, S.ecount,
+                                                S.eoff, S.soff, S.sB, S.sp, S.sq, S.sw, S.cb0, S.cb1, S.cn0, S.cn1, S.cw);
+}
+
+void launch_lm_mst(DevState& S, hipStream_t st) {
+  launch_fill_int(S.ntree, S.ninst, 0, st);
   k_boruvka<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.nbasins, S.nconn, S.cb0, S.cb1, S.npos, S.sid, S.comp, S.best, S.bnew,
                                          S.intree, S.ntree);
+}
+
+void launch_lm_orient(DevState& S, hipStream_t st) {
   k_orient<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.nbasins, S.nconn, S.basin0, S.cb0, S.cb1, S.cn0, S.cn1, S.intree, S.tdeg,
                                         S.toff, S.bsize, S.tadj, S.bpar, S.bfront);
+}
+
+void launch_lm_correct(DevState& S, hipStream_t st) {
   k_correct<<<dim3(16, S.ninst), FSX_CHUNK, 0, st>>>(S.P, S.nconn, S.cb1, S.cn0, S.cn1, S.intree, S.outlet, S.rec,
                                                       S.length, S.err);
 }

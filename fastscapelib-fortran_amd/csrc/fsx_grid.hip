@@ -57,17 +57,29 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_receivers(const InstParams* __res
 
 // Donor masks: bit dbit(d) set when the neighbour of v in direction d drains to v.  Listing the
 // set bits in ascending neighbour index reproduces find_donor (src/FlowRouting.f90:366-389).
+// Also threads the donors as first[v] (smallest donor) and next[d] (next larger donor of the same
+// receiver), which lets a depth-first preorder walk advance with one load per step.
 __global__ __launch_bounds__(FSX_CHUNK) void k_donors(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
-                                                      const int* __restrict__ rec, uint8_t* __restrict__ dmask) {
+                                                      const int* __restrict__ rec, uint8_t* __restrict__ dmask,
+                                                      int* __restrict__ first, int* __restrict__ next) {
   NODE_KERNEL_PROLOGUE
   if (!P.run_spl) return;
   const int* ri = rec + P.off;
   int q[8], dir[8];
   const int c8 = neighbours_wc(P, v, q, dir);
   unsigned m = 0;
+  int d[8], nd = 0;
   for (int t = 0; t < c8; t++)
-    if (ri[q[t]] == v) m |= 1u << dbit(dir[t]);
+    if (ri[q[t]] == v) {
+      m |= 1u << dbit(dir[t]);
+      int x = q[t], s = nd++;
+      while (s > 0 && d[s - 1] > x) { d[s] = d[s - 1]; s--; }
+      d[s] = x;
+    }
   dmask[g_] = (uint8_t)m;
+  first[g_] = nd ? d[0] : -1;
+  for (int k = 0; k < nd; k++) next[P.off + d[k]] = k + 1 < nd ? d[k + 1] : -1;
+  if (ri[v] == v) next[g_] = -1;
 }
 
 // Stream-power set-up, src/StreamPowerLaw.f90:236-250: g, kfint, ht.
@@ -290,7 +302,7 @@ void launch_receivers(DevState& S, hipStream_t st) {
   k_receivers<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.h, S.rec, S.length);
 }
 void launch_donors(DevState& S, hipStream_t st) {
-  k_donors<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.rec, S.dmask);
+  k_donors<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.rec, S.dmask, S.first, S.next);
 }
 void launch_spl_prep(DevState& S, hipStream_t st) {
   k_spl_prep<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.h, S.b, S.kf, S.kfint, S.g, S.ht);

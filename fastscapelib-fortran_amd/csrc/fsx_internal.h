@@ -112,13 +112,52 @@ struct DevState;
 void launch_uplift(DevState& S, hipStream_t st);
 void launch_receivers(DevState& S, hipStream_t st);
 void launch_donors(DevState& S, hipStream_t st);
-void launch_local_minima(DevState& S, hipStream_t st);
+void launch_lm_labels(DevState& S, hipStream_t st);
+void launch_lm_basins(DevState& S, hipStream_t st);
+void launch_lm_walk(DevState& S, hipStream_t st);
+void launch_lm_edges(DevState& S, hipStream_t st);
+void launch_lm_mst(DevState& S, hipStream_t st);
+void launch_lm_orient(DevState& S, hipStream_t st);
+void launch_lm_correct(DevState& S, hipStream_t st);
 void launch_nr_sort(DevState& S, hipStream_t st);
 void launch_spl_prep(DevState& S, hipStream_t st);
 void launch_spl_post(DevState& S, hipStream_t st);
 void launch_expand_donors(DevState& S, hipStream_t st);
-void launch_levels_fill(DevState& S, hipStream_t st);
+void launch_levels(DevState& S, hipStream_t st);
+void launch_jump(DevState& S, hipStream_t st);
+void launch_fill_int(int* a, long long n, int val, hipStream_t st);
 void launch_area(DevState& S, hipStream_t st);
+void launch_area_prep(DevState& S, hipStream_t st);
+void launch_area_sweep(DevState& S, hipStream_t st);
+void launch_spl_fact(DevState& S, hipStream_t st);
+void launch_spl_sweep(DevState& S, hipStream_t st);
 void launch_spl(DevState& S, hipStream_t st);
 void launch_diffusion(DevState& S, hipStream_t st);
 void launch_mfd(DevState& S, hipStream_t st);
+int sweep_set_lds(int smem_max);
+void launch_fill(DevState& S, hipStream_t st);
+
+// Phases timed with HIP events when profiling is on (the reference times Uplift, SPL and Diffusion
+// with cpu_time, src/FastScape_api.f90:207-257).  Phases marked * are exactly one kernel.
+enum Phase {
+  PH_UPLIFT = 0,   // * k_uplift
+  PH_RECEIVERS,    // * k_receivers
+  PH_DONORS,       // * k_donors (before and after rerouting)
+  PH_LM_LABELS,    //   pointer jumping for basin roots
+  PH_LM_BASINS,    //   basin ids, sizes
+  PH_LM_WALK,      // * k_basin_walk
+  PH_LM_EDGES,     //   basin-graph assembly
+  PH_LM_SORT,      //   edge order (k_nr_big, k_nr_small, k_nr_pos)
+  PH_LM_MST,       // * k_boruvka (+ a fill)
+  PH_LM_ORIENT,    // * k_orient
+  PH_LM_CORRECT,   // * k_correct
+  PH_LEVELS,       //   depth by pointer jumping + k_level_sort
+  PH_AREA_PREP,    // * k_area_prep
+  PH_AREA_SWEEP,   // * k_area_sweep
+  PH_SPL_PREP,     // * k_spl_prep
+  PH_SPL_FACT,     // * k_spl_fact
+  PH_SPL_SWEEP,    // * k_spl_sweep
+  PH_SPL_POST,     // * k_spl_post
+  PH_DIFFUSION,    //   k_adi_prep, k_adi_x, k_adi_y, k_adi_post
+  PH_COUNT
+};

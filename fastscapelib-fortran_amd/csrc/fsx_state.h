@@ -17,9 +17,14 @@ struct DevState {
   double *lake_depth = nullptr, *sedflux = nullptr, *ht = nullptr, *water = nullptr, *zint = nullptr;
   double *zintp = nullptr, *gam = nullptr, *dh = nullptr, *hp = nullptr, *lwv = nullptr, *lsed = nullptr;
   double *kfint = nullptr, *kdint = nullptr, *g = nullptr, *pexp = nullptr, *elev = nullptr;
-  int *rec = nullptr, *lbl = nullptr, *basin = nullptr, *rootrank = nullptr, *order = nullptr;
-  int *lstart = nullptr, *lsill = nullptr, *stack = nullptr;
+  int *rec = nullptr, *basin = nullptr, *rootrank = nullptr, *order = nullptr;
+  int *lstart = nullptr, *lsill = nullptr;
   uint8_t *dmask = nullptr, *bc = nullptr;
+  unsigned long long* J = nullptr;  // (distance, ancestor) pairs of the pointer-jumping passes
+  int *pos = nullptr, *lcur = nullptr, *rp = nullptr, *LSl = nullptr, *dn = nullptr;
+  // per-position (level order) working arrays of the sweeps
+  double *A0 = nullptr, *Al = nullptr, *F = nullptr, *HTl = nullptr, *Hl = nullptr, *Wl = nullptr, *HWl = nullptr;
+  double *HPl = nullptr, *DHl = nullptr, *ELl = nullptr;
 
   // ---- per basin (node offsets) ----
   int *outlet = nullptr, *bsize = nullptr, *soff = nullptr, *ecount = nullptr, *eoff = nullptr;
@@ -43,7 +48,12 @@ struct DevState {
 
   // ---- per instance ----
   int *nbasins = nullptr, *basin0 = nullptr, *nconn = nullptr, *nlev = nullptr, *err = nullptr;
-  int *ngs = nullptr, *ntree = nullptr, *chunkcnt = nullptr;
+  int *ngs = nullptr, *ntree = nullptr, *chunkcnt = nullptr, *maxd = nullptr, *ngroups = nullptr;
+  int *gmeta = nullptr;
+  int4* grec = nullptr;  // per-group records of the sweeps, 2 per group
+  int *first = nullptr, *next = nullptr;  // smallest donor / next larger sibling (-1: none)
+  int2* closed_list = nullptr;  // (instance, basin) of every closed basin
+  int* nclosed = nullptr;  // level groups of the sweeps (node offsets + instance, nn+1 per landscape)
   long long *newton = nullptr;
   double *scal = nullptr;      // per-instance scalar scratch (tolerances, GS error)
   int *flags = nullptr;        // convergence flags of iterative kernels (64)

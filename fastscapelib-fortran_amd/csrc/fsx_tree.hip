@@ -1,298 +1,679 @@
 // fsx_tree.hip -- dependency-ordered sweeps over the receiver forest of each landscape.
 //
-// The reference walks a depth-first stack serially (src/FlowRouting.f90:393-443).  Here one
-// 1024-thread workgroup per landscape builds a breadth-first level order (level 0 = base-level
-// nodes, level l+1 = donors of level l) and sweeps it: top-down for the lake fill and the
-// stream-power solve (each node needs only its receiver), bottom-up for drainage area (each node
-// pulls its donors in descending index, which is the reference's summation order,
-// src/FlowRouting.f90:136-139).  Level order is deterministic: positions come from a workgroup
-// scan, not atomics.
+// The reference walks a depth-first stack serially (src/FlowRouting.f90:393-443).  Here:
+//  * depth of every node (hops to its base-level node) by pointer jumping over (distance, ancestor)
+//    pairs packed in 64 bits, node-parallel over the whole batch;
+//  * a counting sort by depth gives the level order (level 0 = base-level nodes) and each node's
+//    position in it; node-parallel prep kernels turn the per-node inputs into per-position records
+//    (receiver position in the previous level, donor positions in the next level, stream-power
+//    factor) so that the sweeps read them coalesced;
+//  * one workgroup per landscape then sweeps the levels, keeping the values of the adjacent level in
+//    LDS: bottom-up for drainage area (each node pulls its donors in descending index, the
+//    reference's summation order, src/FlowRouting.f90:136-139) and top-down for the lake fill and
+//    the stream-power solve (each node needs only its receiver).
+// Nodes of one level are independent, so their order inside the level does not affect any value.
+#include <type_traits>
+
 #include "fsx_scan.h"
 #include "fsx_state.h"
 
-// BFS level order over the receiver forest, fused with the lake-surface fill of
-// FlowRoutingSingleFlowDirection (src/FlowRouting.f90:73-89) and find_mult_rec (:183-198):
-// hwater(d) = hwater(rec) + 1e-8 where hwater(rec) > h(d), else h(d); lake_depth = hwater - h.
-__global__ __launch_bounds__(FSX_WG) void k_levels_fill(const InstParams* __restrict__ PP, const int* __restrict__ rec,
-                                                        const uint8_t* __restrict__ dmask, const double* __restrict__ h,
-                                                        double* __restrict__ hwater, double* __restrict__ lake_depth,
-                                                        int* __restrict__ order, int* __restrict__ lstart,
-                                                        int* __restrict__ nlev, int* __restrict__ err) {
+#define NODE_PROLOGUE                         \
+  const Chunk c = chunks[blockIdx.x];         \
+  const InstParams& P = PP[c.inst];           \
+  const int v = c.start + (int)threadIdx.x;   \
+  if (!P.active || !P.run_spl || v >= P.nn) return; \
+  const long long g_ = (long long)P.off + v;
+
+__device__ __forceinline__ unsigned long long jpack(unsigned dist, unsigned nxt) {
+  return ((unsigned long long)dist << 32) | nxt;
+}
+__device__ __forceinline__ int jnext(unsigned long long w) { return (int)(unsigned)(w & 0xffffffffu); }
+__device__ __forceinline__ int jdist(unsigned long long w) { return (int)(w >> 32); }
+
+// ---- depth / root by pointer jumping ----------------------------------------------------------
+__global__ __launch_bounds__(FSX_CHUNK) void k_jump_init(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
+                                                         const int* __restrict__ rec, unsigned long long* __restrict__ J) {
+  NODE_PROLOGUE
+  const int r = rec[g_];
+  J[g_] = jpack(r == v ? 0u : 1u, (unsigned)r);
+}
+
+// J[v] = (distance to ancestor, ancestor).  One jump replaces the ancestor by its own ancestor;
+// a 64-bit load/store keeps the pair consistent, so jumping in place is safe.
+__global__ __launch_bounds__(FSX_CHUNK) void k_jump(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
+                                                    unsigned long long* __restrict__ J, int* __restrict__ flags, int iter) {
+  if (iter > 0 && flags[iter - 1] == 0) return;
+  NODE_PROLOGUE
+  unsigned long long* Ji = J + P.off;
+  const unsigned long long w = Ji[v];
+  const int x = jnext(w);
+  const unsigned long long wx = Ji[x];
+  if (jnext(wx) != x) {
+    Ji[v] = jpack((unsigned)(jdist(w) + jdist(wx)), (unsigned)jnext(wx));
+    flags[iter] = 1;
+  }
+}
+
+// ---- level order (counting sort by depth, one workgroup per landscape) ---------------------------
+// Counts live in LDS (levels >= LVL_HCAP spill to global memory).  The same kernel partitions the
+// levels into groups of consecutive levels that start inside one bucket of GRP_BUCKET positions
+// (so a group has at most GRP_BUCKET levels and GRP_STG nodes; a level wider than GRP_HALF forms its
+// own group); the sweeps stage one group at a time into LDS.
+#define LVL_HCAP 32768
+#define GRP_HALF 1024                // a level wider than this is a group of its own
+#define GRP_BUCKET 1024              // levels starting in one bucket of positions share a group
+#define GRP_STG (GRP_HALF + GRP_BUCKET)  // positions of a group (staging capacity)
+
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+__global__ __launch_bounds__(FSX_WG) void k_level_sort(const InstParams* __restrict__ PP,
+                                                       const unsigned long long* __restrict__ J,
+                                                       const int* __restrict__ flags, int* __restrict__ lstart,
+                                                       int* __restrict__ lcur, int* __restrict__ nlev,
+                                                       int* __restrict__ gmeta, int* __restrict__ ngroups,
+                                                       int4* __restrict__ grec, int* __restrict__ order,
+                                                       int* __restrict__ pos, int* __restrict__ err) {
+  extern __shared__ int hist[];
   __shared__ int lds[FSX_WG / 64];
+  __shared__ int s_max;
   const int inst = blockIdx.x;
   const InstParams& P = PP[inst];
   if (!P.active || !P.run_spl) return;
   const int nn = P.nn, tid = threadIdx.x;
-  const int* R = rec + P.off;
-  const uint8_t* M = dmask + P.off;
-  const double* H = h + P.off;
-  double* HW = hwater + P.off;
-  double* LD = lake_depth + P.off;
-  int* ord = order + P.off;
-  int* ls = lstart + P.off + inst;  // nn+1 entries per landscape
-  int cnt = 0;
-  for (int base = 0; base < nn; base += FSX_WG) {
-    const int v = base + tid;
-    const int f = (v < nn && R[v] == v) ? 1 : 0;
-    int tot;
-    const int incl = block_incl_scan<FSX_WG>(f, lds, tot);
-    if (f) {
-      ord[cnt + incl - 1] = v;
-      HW[v] = H[v];
-      LD[v] = 0.0;
-    }
-    cnt += tot;
+  const long long off = P.off;
+  int* ls = lstart + off + inst;  // nn+1 entries
+  int* lc = lcur + off + inst;
+  if (tid == 0) {
+    s_max = 0;
+    if (flags[25]) atomicOr(err + inst, FSX_ERR_LEVELS);  // jumping did not converge: receiver cycle
   }
-  int fs = 0, fe = cnt, lvl = 0;
-  while (fs < fe) {
-    if (tid == 0) ls[lvl] = fs;
-    int nend = fe;
-    for (int base = fs; base < fe; base += FSX_WG) {
-      const int idx = base + tid;
-      int d[8], nd = 0, v = -1;
-      if (idx < fe) {
-        v = ord[idx];
-        nd = donors_sorted(P, v, M[v], d);
-      }
-      int tot;
-      const int incl = block_incl_scan<FSX_WG>(nd, lds, tot);
-      if (nd) {
-        const double hw = HW[v];
-        int pos = nend + incl - nd;
-        for (int k = 0; k < nd; k++) {
-          const int x = d[k];
-          ord[pos + k] = x;
-          const double hx = H[x];
-          const double w = hw > hx ? hw + 1.0e-8 : hx;
-          HW[x] = w;
-          LD[x] = w - hx;
-        }
-      }
-      nend += tot;
+  __syncthreads();
+  int mx = 0;
+  for (int v = tid; v < nn; v += FSX_WG) {
+    int d = jdist(J[off + v]);
+    d = d < 0 ? 0 : (d >= nn ? nn - 1 : d);
+    mx = d > mx ? d : mx;
+  }
+  for (int o = 32; o > 0; o >>= 1) { const int y = __shfl_xor(mx, o, 64); mx = y > mx ? y : mx; }
+  if ((tid & 63) == 0) atomicMax(&s_max, mx);
+  __syncthreads();
+  const int L = s_max + 1;
+  const int LH = L < LVL_HCAP ? L : LVL_HCAP;
+  for (int l = tid; l < LH; l += FSX_WG) hist[l] = 0;
+  for (int l = LVL_HCAP + tid; l < L; l += FSX_WG) lc[l] = 0;
+  __syncthreads();
+  for (int v = tid; v < nn; v += FSX_WG) {
+    int d = jdist(J[off + v]);
+    d = d < 0 ? 0 : (d >= nn ? nn - 1 : d);
+    if (d < LVL_HCAP) atomicAdd(hist + d, 1); else atomicAdd(lc + d, 1);
+  }
+  __syncthreads();
+  int carry = 0;
+  for (int s0 = 0; s0 < L; s0 += FSX_WG) {
+    const int l = s0 + tid;
+    const int x = l < L ? (l < LVL_HCAP ? hist[l] : lc[l]) : 0;
+    int tot;
+    const int incl = block_incl_scan<FSX_WG>(x, lds, tot);
+    if (l < L) {
+      const int st = carry + incl - x;
+      ls[l] = st;
+      if (l < LVL_HCAP) hist[l] = st; else lc[l] = st;
     }
-    __syncthreads();
-    fs = fe;
-    fe = nend;
-    lvl++;
+    carry += tot;
   }
   if (tid == 0) {
-    ls[lvl] = fs;
-    nlev[inst] = lvl;
-    if (fs != nn) atomicOr(err + inst, FSX_ERR_LEVELS);
+    ls[L] = carry;
+    nlev[inst] = L;
+  }
+  __syncthreads();
+  for (int v = tid; v < nn; v += FSX_WG) {
+    int d = jdist(J[off + v]);
+    d = d < 0 ? 0 : (d >= nn ? nn - 1 : d);
+    const int p = d < LVL_HCAP ? atomicAdd(hist + d, 1) : atomicAdd(lc + d, 1);
+    if (p < nn) {
+      order[off + p] = v;
+      pos[off + v] = p;
+    }
+  }
+  __syncthreads();
+  // level groups: a new group starts at level 0, at or after a wide level, and where the start
+  // position crosses a multiple of GRP_HALF
+  int* gm = gmeta + off + inst;  // <= L+1 entries
+  int gcarry = 0;
+  for (int s0 = 0; s0 < L; s0 += FSX_WG) {
+    const int l = s0 + tid;
+    int f = 0;
+    if (l < L) {
+      const int a0 = ls[l], w = ls[l + 1] - a0;
+      if (l == 0) f = 1;
+      else {
+        const int pa = ls[l - 1], pw = a0 - pa;
+        f = (w > GRP_HALF) || (pw > GRP_HALF) || (a0 / GRP_BUCKET != pa / GRP_BUCKET);
+      }
+    }
+    int tot;
+    const int incl = block_incl_scan<FSX_WG>(f, lds, tot);
+    if (f) gm[gcarry + incl - 1] = l;
+    gcarry += tot;
+  }
+  if (tid == 0) {
+    gm[gcarry] = L;
+    ngroups[inst] = gcarry;
+  }
+  __syncthreads();
+  // per-group records (lo, hi, p0, p1) + (pm, p2, -, -): level range, position range, start of the
+  // level below and end of the level above
+  int4* gr = grec + 2 * (off + inst);
+  for (int g = tid; g < gcarry; g += FSX_WG) {
+    const int lo = gm[g], hi = gm[g + 1];
+    const int p0 = ls[lo], p1 = ls[hi];
+    gr[2 * g] = make_int4(lo, hi, p0, p1);
+    gr[2 * g + 1] = make_int4(lo > 0 ? ls[lo - 1] : 0, hi < L ? ls[hi + 1] : p1, 0, 0);
   }
 }
 
-// FlowAccumulationSingleFlowDirection, src/FlowRouting.f90:123-143.  a(v) = dx*dy*precip(v),
-// then its donors' areas added from the highest donor index down; a base-level node adds its own
-// area to itself last (the reference's a(rec(ijk)) += a(ijk) with rec(ijk) == ijk).
-__global__ __launch_bounds__(FSX_WG) void k_area_sfd(const InstParams* __restrict__ PP, const int* __restrict__ rec,
-                                                     const uint8_t* __restrict__ dmask, const double* __restrict__ precip,
-                                                     const int* __restrict__ order, const int* __restrict__ lstart,
-                                                     const int* __restrict__ nlev, double* __restrict__ area) {
+// ---- per-position records -----------------------------------------------------------------------
+// Drainage-area inputs: own term dx*dy*precip and the positions (relative to the next level's start)
+// of the donors in descending node index; unused slots -1.
+__global__ __launch_bounds__(FSX_CHUNK) void k_area_prep(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
+                                                         const unsigned long long* __restrict__ J,
+                                                         const uint8_t* __restrict__ dmask, const int* __restrict__ pos,
+                                                         const int* __restrict__ lstart, const double* __restrict__ precip,
+                                                         double* __restrict__ A0, int* __restrict__ dn) {
+  NODE_PROLOGUE
+  const int p = pos[g_];
+  const int d = jdist(J[g_]);
+  const long long gp = (long long)P.off + p;
+  A0[gp] = P.dxdy * precip[g_];
+  int ds[8];
+  const int nd = donors_sorted(P, v, dmask[g_], ds);
+  const int base = lstart[P.off + c.inst + d + 1];
+  int* out = dn + 8 * gp;
+#pragma unroll
+  for (int k = 0; k < 8; k++) out[k] = k < nd ? pos[P.off + ds[nd - 1 - k]] - base : -1;
+}
+
+// Stream-power inputs per position: receiver position in the previous level, ht, and
+// fact = kf*dt*a**m/length (n == 1, src/StreamPowerLaw.f90:310) or kf*dt*a**m/length**n with the
+// activity test ht(ijk) > ht(ijr) of the Newton branch (:350-351; -1 marks an inactive node).
+__global__ __launch_bounds__(FSX_CHUNK) void k_spl_fact(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
+                                                        const unsigned long long* __restrict__ J, const int* __restrict__ rec,
+                                                        const int* __restrict__ pos, const int* __restrict__ lstart,
+                                                        const double* __restrict__ area, const double* __restrict__ length,
+                                                        const double* __restrict__ kfint, const double* __restrict__ ht,
+                                                        int* __restrict__ rp, double* __restrict__ F, double* __restrict__ HTl) {
+  NODE_PROLOGUE
+  if (!P.sfd) return;
+  const int p = pos[g_];
+  const int d = jdist(J[g_]);
+  const long long gp = (long long)P.off + p;
+  const int r = rec[g_];
+  const double t = ht[g_];
+  HTl[gp] = t;
+  if (d == 0) {
+    rp[gp] = -1;
+    F[gp] = 0.0;
+    return;
+  }
+  rp[gp] = pos[P.off + r] - lstart[P.off + c.inst + d - 1];
+  const double am = pow(area[g_], P.m);
+  const double n = P.n;
+  if (fabs(n - 1.0) < 2.2250738585072014e-308) {
+    F[gp] = kfint[g_] * P.dt * am / length[g_];
+  } else {
+    F[gp] = (t > ht[P.off + r]) ? kfint[g_] * P.dt * am / pow(length[g_], n) : -1.0;
+  }
+}
+
+// ---- staged level groups ---------------------------------------------------------------------------
+// The sweeps visit level groups (k_level_sort).  Group g covers levels [lo, hi) and positions
+// [p0, p1).  Its per-position records are prefetched into registers one group ahead (two slots per
+// thread cover GRP_STG positions) and written to an LDS staging area when the group starts, so the
+// only global-memory latency left on the critical path is one load per group, hidden behind the
+// previous group.  Levels inside a group are separated by LDS-only barriers.
+struct GroupMeta {
+  int lo, hi, p0, p1;
+  int pm;  // start of level lo-1 (0 when lo == 0)
+  int p2;  // end of level hi (== p1 when hi is the last level)
+};
+
+// ---- bottom-up: drainage area -------------------------------------------------------------------
+// FlowAccumulationSingleFlowDirection, src/FlowRouting.f90:123-143: a = dx*dy*precip, donors added
+// from the highest index down; a base-level node finally adds its own area to itself (:138 with
+// rec(ijk) == ijk), doubling it.
+struct AreaRec {
+  double a;
+  int d[8];
+  int o;
+};
+
+__device__ __forceinline__ void area_load(AreaRec& r, long long gp, const double* A0, const int* dn, const int* order) {
+  r.a = A0[gp];
+  const int4* q = reinterpret_cast<const int4*>(dn + 8 * gp);
+  const int4 x = q[0], y = q[1];
+  r.d[0] = x.x; r.d[1] = x.y; r.d[2] = x.z; r.d[3] = x.w;
+  r.d[4] = y.x; r.d[5] = y.y; r.d[6] = y.z; r.d[7] = y.w;
+  r.o = order[gp];
+}
+
+__global__ __launch_bounds__(FSX_WG) void k_area_sweep(const InstParams* __restrict__ PP, const int* __restrict__ lstart,
+                                                       const int4* __restrict__ grec, const int* __restrict__ ngroups,
+                                                       const int* __restrict__ order, const double* __restrict__ A0,
+                                                       const int* __restrict__ dn, double* __restrict__ Al,
+                                                       double* __restrict__ area, int vcap) {
+  extern __shared__ double sm[];
+  // level values double-buffered at sm[0..vcap) and sm[vcap..2*vcap); offsets, not a pointer array,
+  // so every access stays an LDS (ds_*) instruction
+  double* sA = sm + 2 * vcap;
+  int* sD = reinterpret_cast<int*>(sA + GRP_STG);
+  int* sO = sD + 8 * GRP_STG;
+  int* sL = sO + GRP_STG;  // level starts of the group, GRP_BUCKET+1
   const int inst = blockIdx.x;
   const InstParams& P = PP[inst];
   if (!P.active || !P.run_spl || !P.sfd) return;
-  const int* R = rec + P.off;
-  const uint8_t* M = dmask + P.off;
-  const double* PR = precip + P.off;
-  const int* ord = order + P.off;
-  const int* ls = lstart + P.off + inst;
-  double* A = area + P.off;
-  for (int l = nlev[inst] - 1; l >= 0; l--) {
-    const int b0 = ls[l], b1 = ls[l + 1];
-    for (int idx = b0 + (int)threadIdx.x; idx < b1; idx += FSX_WG) {
-      const int v = ord[idx];
-      double a = P.dxdy * PR[v];
-      int d[8];
-      const int nd = donors_sorted(P, v, M[v], d);
-      for (int k = nd - 1; k >= 0; k--) a = a + A[d[k]];
-      if (R[v] == v) a = a + a;
-      A[v] = a;
+  const int tid = threadIdx.x;
+  const long long off = P.off;
+  const int* ls = lstart + off + inst;
+  const int4* gr = grec + 2 * (off + inst);
+  const int G = ngroups[inst];
+  auto meta = [&](int g) {
+    GroupMeta m;
+    if (g < 0 || g >= G) { m.lo = m.hi = m.p0 = m.p1 = m.pm = m.p2 = 0; return m; }
+    const int4 a = gr[2 * g], b = gr[2 * g + 1];
+    m.lo = a.x; m.hi = a.y; m.p0 = a.z; m.p1 = a.w; m.pm = b.x; m.p2 = b.y;
+    return m;
+  };
+  AreaRec r0, r1;
+  int rl = 0;
+  auto prefetch = [&](const GroupMeta& m) {
+    const int np = m.p1 - m.p0;
+    if (np > GRP_STG) return;
+    if (tid < np) area_load(r0, off + m.p0 + tid, A0, dn, order);
+    if (tid + FSX_WG < np) area_load(r1, off + m.p0 + tid + FSX_WG, A0, dn, order);
+    if (tid < m.hi - m.lo) rl = ls[m.lo + tid];
+  };
+  GroupMeta cur = meta(G - 1);
+  GroupMeta nxt_m = meta(G - 2);
+  prefetch(cur);
+  for (int g = G - 1; g >= 0; g--) {
+    const int np = cur.p1 - cur.p0;
+    const bool staged = np <= GRP_STG;
+    if (staged) {
+      if (tid < np) {
+        sA[tid] = r0.a;
+#pragma unroll
+        for (int k = 0; k < 8; k++) sD[8 * tid + k] = r0.d[k];
+        sO[tid] = r0.o;
+      }
+      if (tid + FSX_WG < np) {
+        const int t = tid + FSX_WG;
+        sA[t] = r1.a;
+#pragma unroll
+        for (int k = 0; k < 8; k++) sD[8 * t + k] = r1.d[k];
+        sO[t] = r1.o;
+      }
+      if (tid < cur.hi - cur.lo) sL[tid] = rl;
+      if (tid == 0) sL[cur.hi - cur.lo] = cur.p1;
     }
-    __syncthreads();
+    lds_barrier();
+    const GroupMeta after = meta(g - 2);  // two groups ahead: in flight while this group runs
+    if (g > 0) prefetch(nxt_m);
+    for (int l = cur.hi - 1; l >= cur.lo; l--) {
+      const int b0 = staged ? sL[l - cur.lo] : cur.p0;
+      const int w = (staged ? sL[l - cur.lo + 1] : cur.p1) - b0;
+      const int bn = b0 + w;
+      const int wn = (l + 1 < cur.hi) ? sL[l - cur.lo + 2] - bn : cur.p2 - bn;
+      double* cv = sm + ((l & 1) ? vcap : 0);
+      const double* nv = sm + ((l & 1) ? 0 : vcap);
+      const bool nin = wn <= vcap;
+      // FAST: records staged and the level above in LDS -- the body touches no global loads, so no
+      // wait on in-flight prefetches or stores is needed between levels
+      auto body = [&](auto fast_tag) {
+        constexpr bool FAST = decltype(fast_tag)::value;
+        for (int i = tid; i < w; i += FSX_WG) {
+          double a;
+          int q[8], o;
+          if constexpr (FAST) {
+            const int si = b0 - cur.p0 + i;
+            a = sA[si];
+#pragma unroll
+            for (int k = 0; k < 8; k++) q[k] = sD[8 * si + k];
+            o = sO[si];
+          } else {
+            AreaRec r;
+            area_load(r, off + b0 + i, A0, dn, order);
+            a = r.a;
+#pragma unroll
+            for (int k = 0; k < 8; k++) q[k] = r.d[k];
+            o = r.o;
+          }
+          // unused slots add an exact +0.0, so the sum keeps the reference's order
+          double dv[8];
+#pragma unroll
+          for (int k = 0; k < 8; k++) {
+            if constexpr (FAST) dv[k] = q[k] < 0 ? 0.0 : nv[q[k]];
+            else dv[k] = q[k] < 0 ? 0.0 : (nin ? nv[q[k]] : Al[off + bn + q[k]]);
+          }
+#pragma unroll
+          for (int k = 0; k < 8; k++) a = a + dv[k];
+          if (l == 0) a = a + a;
+          if (w <= vcap) cv[i] = a;
+          Al[off + b0 + i] = a;
+          area[off + o] = a;
+        }
+      };
+      if (staged && nin) body(std::true_type{});
+      else body(std::false_type{});
+      if (w > vcap) __syncthreads(); else lds_barrier();
+    }
+    cur = nxt_m;
+    nxt_m = after;
   }
 }
 
-// Implicit stream-power law along single receivers, src/StreamPowerLaw.f90:210-400.
-// One workgroup per landscape runs the Gauss-Seidel loop (one pass when g == 0): the erosion
-// push dh (bottom-up, donors pulled in descending index as the reverse-stack push adds them),
-// then elev, then the top-down solve: closed form for n == 1 (:306-314), relaxed Newton with
-// omega = 0.875/n otherwise (:343-358); lakes keep elev and accumulate their water volume.
-__global__ __launch_bounds__(FSX_WG) void k_spl_sfd(const InstParams* __restrict__ PP, const int* __restrict__ rec,
-                                                    const uint8_t* __restrict__ dmask, const uint8_t* __restrict__ bc,
-                                                    const int* __restrict__ order, const int* __restrict__ lstart,
-                                                    const int* __restrict__ nlev, const double* __restrict__ area,
-                                                    const double* __restrict__ length, const double* __restrict__ kfint,
-                                                    const double* __restrict__ g, const double* __restrict__ ht,
-                                                    double* __restrict__ h, double* __restrict__ hp, double* __restrict__ dh,
-                                                    double* __restrict__ elev, double* __restrict__ water,
-                                                    int* __restrict__ lsill, double* __restrict__ lwv,
-                                                    double* __restrict__ lsed, double* __restrict__ scal,
-                                                    int* __restrict__ ngs_out, long long* __restrict__ newton_out,
-                                                    int* __restrict__ err) {
+// ---- top-down: lake fill + stream power ----------------------------------------------------------
+// Implicit stream-power law along single receivers, src/StreamPowerLaw.f90:210-400, with the lake
+// surface fill of FlowRoutingSingleFlowDirection (src/FlowRouting.f90:73-89) fused into the first
+// pass (fill_only: the fill alone, h untouched).  The Gauss-Seidel loop (one pass when g == 0
+// everywhere) alternates the bottom-up erosion push dh (:266-283), elev (:285-289) and the
+// top-down solve: closed form for n == 1 (:306-314), relaxed Newton with omega = 0.875/n otherwise
+// (:343-358); lakes keep elev and collect their water volume.
+struct SplRec {
+  int o, q;
+  double f, t, h;
+};
+
+__device__ __forceinline__ void spl_load(SplRec& r, long long gp, const int* order, const int* rp, const double* F,
+                                         const double* HTl, const double* Hl) {
+  r.o = order[gp];
+  r.q = rp[gp];
+  r.f = F[gp];
+  r.t = HTl[gp];
+  r.h = Hl[gp];
+}
+
+__global__ __launch_bounds__(FSX_WG) void k_spl_sweep(
+    const InstParams* __restrict__ PP, const int* __restrict__ lstart, const int* __restrict__ nlev,
+    const int4* __restrict__ grec, const int* __restrict__ ngroups, const int* __restrict__ order,
+    const int* __restrict__ rp, const double* __restrict__ F, const double* __restrict__ HTl,
+    const int* __restrict__ dn, const uint8_t* __restrict__ bc, const double* __restrict__ g,
+    const double* __restrict__ Al, double* __restrict__ h, double* __restrict__ hwater, double* __restrict__ lake_depth,
+    double* __restrict__ water, int* __restrict__ lsill, double* __restrict__ lwv, double* __restrict__ lsed,
+    double* __restrict__ Hl, double* __restrict__ Wl, double* __restrict__ HWl, int* __restrict__ LSl,
+    double* __restrict__ HPl, double* __restrict__ DHl, double* __restrict__ ELl, double* __restrict__ scal,
+    int* __restrict__ ngs_out, long long* __restrict__ newton_out, int* __restrict__ err, int vcap, int fill_only) {
+  extern __shared__ double sm[];
   __shared__ double red[FSX_WG / 64];
   __shared__ double s_tol, s_err, s_gmax;
-  __shared__ long long s_newton;
+  __shared__ unsigned long long s_newton;
   const int inst = blockIdx.x;
   const InstParams& P = PP[inst];
   if (!P.active || !P.run_spl || !P.sfd) return;
-  const int nn = P.nn, tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  const int* R = rec + P.off;
-  const uint8_t* M = dmask + P.off;
-  const uint8_t* BC = bc + P.off;
-  const int* ord = order + P.off;
-  const int* ls = lstart + P.off + inst;
+  // per-level double buffers (h, water, hwater, lake sill) addressed by offset from the LDS base
+#define BH(par) (sm + (par) * vcap)
+#define BW(par) (sm + (2 + (par)) * vcap)
+#define BHW(par) (sm + (4 + (par)) * vcap)
+#define BLS(par) (reinterpret_cast<int*>(sm + 6 * vcap) + (par) * vcap)
+  double* sF = sm + 7 * vcap;  // staging: GRP_STG records
+  double* sT = sF + GRP_STG;
+  double* sH = sT + GRP_STG;
+  int* sO = reinterpret_cast<int*>(sH + GRP_STG);
+  int* sQ = sO + GRP_STG;
+  int* sL = sQ + GRP_STG;  // GRP_BUCKET+1 level starts
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int nn = P.nn;
+  const long long off = P.off;
+  const int* ls = lstart + off + inst;
+  const int4* gr = grec + 2 * (off + inst);
   const int L = nlev[inst];
-  const double* A = area + P.off;
-  const double* LEN = length + P.off;
-  const double* KF = kfint + P.off;
-  const double* G = g + P.off;
-  const double* HT = ht + P.off;
-  double* H = h + P.off;
-  double* HP = hp + P.off;
-  double* DH = dh + P.off;
-  double* EL = elev + P.off;
-  double* W = water + P.off;
-  int* LS = lsill + P.off;
-  double* LW = lwv + P.off;
-  double* LSD = lsed + P.off;
-  const double dt = P.dt, m = P.m, n = P.n;
+  const int G = ngroups[inst];
+  const double n = P.n;
   const bool n_is_one = fabs(n - 1.0) < 2.2250738585072014e-308;
 
-  // tol = 1e-4*(maxval(abs(h))+1); gmax = maxval(g)
-  double hm = 0.0, gm = -1.7976931348623157e308;
-  for (int v = tid; v < nn; v += FSX_WG) {
-    hm = fmax(hm, fabs(H[v]));
-    gm = fmax(gm, G[v]);
-    LSD[v] = 0.0;
-    LS[v] = -1;
+  // tol = 1e-4*(maxval(abs(h))+1), gmax = maxval(g); level-ordered working copy of h
+  double hm = 0.0, gmx = -1.7976931348623157e308;
+  for (int p = tid; p < nn; p += FSX_WG) {
+    const int v = order[off + p];
+    const double hv = h[off + v];
+    hm = fmax(hm, fabs(hv));
+    gmx = fmax(gmx, g[off + v]);
+    Hl[off + p] = hv;
+    LSl[off + p] = -1;
+    lsed[off + v] = 0.0;
   }
   for (int o = 32; o > 0; o >>= 1) {
     hm = fmax(hm, __shfl_xor(hm, o, 64));
-    gm = fmax(gm, __shfl_xor(gm, o, 64));
+    gmx = fmax(gmx, __shfl_xor(gmx, o, 64));
   }
   if (lane == 0) red[wid] = hm;
   __syncthreads();
   if (tid == 0) {
     double x = red[0];
-    for (int w = 1; w < FSX_WG / 64; w++) x = fmax(x, red[w]);
+    for (int q = 1; q < FSX_WG / 64; q++) x = fmax(x, red[q]);
     s_tol = 1.0e-4 * (x + 1.0);
     s_newton = 0;
   }
   __syncthreads();
-  if (lane == 0) red[wid] = gm;
+  if (lane == 0) red[wid] = gmx;
   __syncthreads();
   if (tid == 0) {
     double x = red[0];
-    for (int w = 1; w < FSX_WG / 64; w++) x = fmax(x, red[w]);
+    for (int q = 1; q < FSX_WG / 64; q++) x = fmax(x, red[q]);
     s_gmax = x;
     s_err = 2.0 * s_tol;
   }
   __syncthreads();
   const bool gzero = s_gmax < 2.2250738585072014e-308;
-  const double dxdy_g = 1.0;  // placeholder to keep the expression order explicit below
-  (void)dxdy_g;
   int ngs = 0;
-  long long my_newton = 0;
+  unsigned long long my_newton = 0;
+  auto meta = [&](int gg) {
+    GroupMeta m;
+    if (gg < 0 || gg >= G) { m.lo = m.hi = m.p0 = m.p1 = m.pm = m.p2 = 0; return m; }
+    const int4 a = gr[2 * gg], b = gr[2 * gg + 1];
+    m.lo = a.x; m.hi = a.y; m.p0 = a.z; m.p1 = a.w; m.pm = b.x; m.p2 = b.y;
+    return m;
+  };
   while (s_err > s_tol && ngs < 99) {
     ngs++;
-    if (gzero) {
-      // g == 0: dh == 0 exactly and elev == ht, the push only recomputes zeros
-      for (int v = tid; v < nn; v += FSX_WG) { HP[v] = H[v]; EL[v] = HT[v]; }
-    } else {
-      for (int v = tid; v < nn; v += FSX_WG) HP[v] = H[v];
+    const bool first = ngs == 1;
+    if (!gzero && !fill_only) {
+      for (int p = tid; p < nn; p += FSX_WG) HPl[off + p] = Hl[off + p];
       __syncthreads();
-      // dh push, src/StreamPowerLaw.f90:266-283, bottom-up
+      // erosion push, bottom-up (donors in descending index); lake sediment of lake sills
       for (int l = L - 1; l >= 0; l--) {
-        for (int idx = ls[l] + tid; idx < ls[l + 1]; idx += FSX_WG) {
-          const int v = ord[idx];
-          const double own = HT[v] - HP[v];
+        const int b0 = ls[l], w = ls[l + 1] - b0;
+        const int bn = ls[l + 1], wn = (l + 1 < L) ? ls[l + 2] - bn : 0;
+        double* cv = BH(l & 1);
+        const double* nv = BH((l + 1) & 1);
+        const bool nin = wn <= vcap;
+        for (int i = tid; i < w; i += FSX_WG) {
+          const long long gp = off + b0 + i;
+          const int v = order[gp];
+          const double own = HTl[gp] - HPl[gp];
           double s = own;
-          int d[8];
-          const int nd = donors_sorted(P, v, M[v], d);
-          for (int k = nd - 1; k >= 0; k--) s = s + DH[d[k]];
-          if (R[v] != v) {
-            const double t = s - own;
-            if (LS[v] == v) LSD[v] = t <= 0.0 ? 0.0 : t;
-            DH[v] = t + own;
-          } else {
-            DH[v] = s;
-            LSD[v] = s;
+          const int* dd = dn + 8 * gp;
+          for (int k = 0; k < 8; k++) {
+            const int q = dd[k];
+            if (q < 0) break;
+            s = s + (nin ? nv[q] : DHl[off + bn + q]);
           }
+          double dv;
+          if (l > 0) {
+            const double t = s - own;
+            if (LSl[gp] == v) lsed[off + v] = t <= 0.0 ? 0.0 : t;
+            dv = t + own;
+          } else {
+            dv = s;
+            lsed[off + v] = s;
+          }
+          if (w <= vcap) cv[i] = dv;
+          DHl[gp] = dv;
         }
         __syncthreads();
       }
-      for (int v = tid; v < nn; v += FSX_WG) {
-        const double t = HT[v];
-        EL[v] = BC[v] ? t : t + (DH[v] - (t - HP[v])) * G[v] * P.dx * P.dy / A[v];
-      }
-    }
-    __syncthreads();
-    // solve, base level up
-    for (int l = 0; l < L; l++) {
-      for (int idx = ls[l] + tid; idx < ls[l + 1]; idx += FSX_WG) {
-        const int v = ord[idx];
-        const int r = R[v];
-        if (r == v) {
-          W[v] = HT[v];
-          LS[v] = v;
-          LW[v] = 0.0;
-          continue;
-        }
-        const double w_rcv = W[r];
-        const double e = EL[v];
-        double hv;
-        if (e > w_rcv) {
-          if (n_is_one) {
-            double f = e, df = 1.0;
-            const double fact = KF[v] * dt * pow(A[v], m) / LEN[v];
-            f = f + fact * H[r];
-            df = df + fact;
-            hv = f / df;
-          } else {
-            const double omega = 0.875 / n, tolp = P.newton_tol;
-            double errp = 2.0 * tolp;
-            const double h0 = e;
-            hv = H[v];
-            const double hr = H[r];
-            const bool act = HT[v] > HT[r];
-            const double fact = act ? KF[v] * dt * pow(A[v], m) / pow(LEN[v], n) : 0.0;
-            int it = 0;
-            while (errp > tolp) {
-              double f = hv - h0, df = 1.0;
-              if (act) {
-                const double x = fmax(0.0, hv - hr);
-                f = f + fact * pow(x, n);
-                df = df + fact * n * pow(x, n - 1.0);
-              }
-              const double hn = hv - f / df;
-              errp = fabs(hn - hv);
-              hv = hv * (1.0 - omega) + hn * omega;
-              if (++it > 100000) { atomicOr(err + inst, FSX_ERR_NEWTON); break; }
-            }
-            my_newton += it;
-          }
-          LS[v] = v;
-          LW[v] = 0.0;
-          if (hv < w_rcv) hv = w_rcv;
-        } else {
-          hv = e;
-          const int sl = LS[r];
-          LS[v] = sl;
-          if (!gzero && sl >= 0) atomicAdd(LW + sl, w_rcv - hv);
-        }
-        H[v] = hv;
-        W[v] = w_rcv > hv ? w_rcv : hv;
+      for (int p = tid; p < nn; p += FSX_WG) {
+        const long long gp = off + p;
+        const int v = order[gp];
+        const double t = HTl[gp];
+        ELl[gp] = bc[off + v] ? t : t + (DHl[gp] - (t - HPl[gp])) * g[off + v] * P.dx * P.dy / Al[gp];
       }
       __syncthreads();
     }
-    // err = sqrt(sum((h-hp)**2)/nn), zero when g == 0 everywhere
-    if (gzero) {
+    // solve, base level up, group by group
+    SplRec r0, r1;
+    int rl = 0;
+    auto prefetch = [&](const GroupMeta& m) {
+      const int np = m.p1 - m.p0;
+      if (np > GRP_STG) return;
+      if (tid < np) spl_load(r0, off + m.p0 + tid, order, rp, F, HTl, Hl);
+      if (tid + FSX_WG < np) spl_load(r1, off + m.p0 + tid + FSX_WG, order, rp, F, HTl, Hl);
+      if (tid < m.hi - m.lo) rl = ls[m.lo + tid];
+    };
+    GroupMeta cur = meta(0);
+    GroupMeta nxt_m = meta(1);
+    prefetch(cur);
+    for (int gg = 0; gg < G; gg++) {
+      const int np = cur.p1 - cur.p0;
+      const bool staged = np <= GRP_STG;
+      if (staged) {
+        if (tid < np) { sO[tid] = r0.o; sQ[tid] = r0.q; sF[tid] = r0.f; sT[tid] = r0.t; sH[tid] = r0.h; }
+        if (tid + FSX_WG < np) {
+          const int t = tid + FSX_WG;
+          sO[t] = r1.o; sQ[t] = r1.q; sF[t] = r1.f; sT[t] = r1.t; sH[t] = r1.h;
+        }
+        if (tid < cur.hi - cur.lo) sL[tid] = rl;
+        if (tid == 0) sL[cur.hi - cur.lo] = cur.p1;
+      }
+      lds_barrier();
+      const GroupMeta after = meta(gg + 2);
+      if (gg + 1 < G) prefetch(nxt_m);
+      for (int l = cur.lo; l < cur.hi; l++) {
+        const int b0 = staged ? sL[l - cur.lo] : cur.p0;
+        const int w = (staged ? sL[l - cur.lo + 1] : cur.p1) - b0;
+        const int bp = l == cur.lo ? cur.pm : sL[l - 1 - cur.lo], wp = l > 0 ? b0 - bp : 0;
+        const int cb = l & 1, pb = cb ^ 1;
+        const bool pin = wp <= vcap;
+        // FAST: staged records, receiver level in LDS, g == 0 -- no global loads in the body
+        auto body = [&](auto fast_tag) {
+          constexpr bool FAST = decltype(fast_tag)::value;
+          for (int i = tid; i < w; i += FSX_WG) {
+            SplRec rr;
+            if constexpr (FAST) {
+              const int si = b0 - cur.p0 + i;
+              rr.o = sO[si]; rr.q = sQ[si]; rr.f = sF[si]; rr.t = sT[si]; rr.h = sH[si];
+            } else if (staged) {
+              const int si = b0 - cur.p0 + i;
+              rr.o = sO[si]; rr.q = sQ[si]; rr.f = sF[si]; rr.t = sT[si]; rr.h = sH[si];
+            } else {
+              spl_load(rr, off + b0 + i, order, rp, F, HTl, Hl);
+            }
+            const int v = rr.o;
+            const double t = rr.t;
+            double hv = rr.h, wv, hw = t;
+            int sl = v;
+            if (l == 0) {
+              wv = t;
+              if (!gzero && !fill_only) lwv[off + v] = 0.0;
+              if (first) lake_depth[off + v] = 0.0;
+            } else {
+              const int q = rr.q;
+              double hr, w_rcv;
+              if constexpr (FAST) { hr = BH(pb)[q]; w_rcv = BW(pb)[q]; }
+              else { hr = pin ? BH(pb)[q] : Hl[off + bp + q]; w_rcv = pin ? BW(pb)[q] : Wl[off + bp + q]; }
+              if (first) {
+                double hwr;
+                if constexpr (FAST) hwr = BHW(pb)[q];
+                else hwr = pin ? BHW(pb)[q] : HWl[off + bp + q];
+                hw = hwr > t ? hwr + 1.0e-8 : t;
+                lake_depth[off + v] = hw - t;
+              }
+              if (fill_only) {
+                wv = w_rcv;
+              } else {
+                double e;
+                if constexpr (FAST) e = t;
+                else e = gzero ? t : ELl[off + b0 + i];
+                if (e > w_rcv) {
+                  const double fact = rr.f;
+                  if (n_is_one) {
+                    double f = e, df = 1.0;
+                    f = f + fact * hr;
+                    df = df + fact;
+                    hv = f / df;
+                  } else {
+                    const double omega = 0.875 / n, tolp = P.newton_tol;
+                    double errp = 2.0 * tolp;
+                    const double h0 = e;
+                    const bool act = fact >= 0.0;
+                    int it = 0;
+                    while (errp > tolp) {
+                      double f = hv - h0, df = 1.0;
+                      if (act) {
+                        const double x = fmax(0.0, hv - hr);
+                        f = f + fact * pow(x, n);
+                        df = df + fact * n * pow(x, n - 1.0);
+                      }
+                      const double hn = hv - f / df;
+                      errp = fabs(hn - hv);
+                      hv = hv * (1.0 - omega) + hn * omega;
+                      if (++it > 100000) { atomicOr(err + inst, FSX_ERR_NEWTON); break; }
+                    }
+                    my_newton += it;
+                  }
+                  if (!gzero) lwv[off + v] = 0.0;
+                  if (hv < w_rcv) hv = w_rcv;
+                } else {
+                  hv = e;
+                  if constexpr (FAST) sl = -1;
+                  else sl = gzero ? -1 : (pin ? BLS(pb)[q] : LSl[off + bp + q]);
+                  if (!gzero && sl >= 0) atomicAdd(lwv + off + sl, w_rcv - hv);
+                }
+                wv = w_rcv > hv ? w_rcv : hv;
+              }
+            }
+            if (w <= vcap) {
+              BH(cb)[i] = hv;
+              BW(cb)[i] = wv;
+              if (first) BHW(cb)[i] = hw;
+              BLS(cb)[i] = sl;
+            }
+            // level-ordered copies only where a later reader needs them: wide levels (kept out of LDS)
+            // and the Gauss-Seidel iterations; grid-ordered outputs h, hwater, lake_depth always
+            const long long gp = off + b0 + i;
+            const bool wide = w > vcap;
+            if (!fill_only) {
+              if (wide || !gzero || !n_is_one) Hl[gp] = hv;
+              if (wide) Wl[gp] = wv;
+              if (wide || !gzero) LSl[gp] = sl;
+              h[off + v] = hv;
+              if (!gzero) { water[off + v] = wv; lsill[off + v] = sl; }
+            } else if (wide) {
+              Wl[gp] = wv;
+            }
+            if (first) {
+              if (wide) HWl[gp] = hw;
+              hwater[off + v] = hw;
+            }
+          }
+        };
+        if (staged && pin && gzero) body(std::true_type{});
+        else body(std::false_type{});
+        if (w > vcap) __syncthreads(); else lds_barrier();
+      }
+      cur = nxt_m;
+      nxt_m = after;
+    }
+    __syncthreads();
+    if (gzero || fill_only) {
       if (tid == 0) s_err = 0.0;
     } else {
       double ss = 0.0;
-      for (int v = tid; v < nn; v += FSX_WG) {
-        const double dd = H[v] - HP[v];
+      for (int p = tid; p < nn; p += FSX_WG) {
+        const double dd = Hl[off + p] - HPl[off + p];
         ss += dd * dd;
       }
       for (int o = 32; o > 0; o >>= 1) ss += __shfl_xor(ss, o, 64);
@@ -300,30 +681,91 @@ __global__ __launch_bounds__(FSX_WG) void k_spl_sfd(const InstParams* __restrict
       __syncthreads();
       if (tid == 0) {
         double x = 0.0;
-        for (int w = 0; w < FSX_WG / 64; w++) x += red[w];
+        for (int q = 0; q < FSX_WG / 64; q++) x += red[q];
         s_err = sqrt(x / (double)nn);
       }
     }
     __syncthreads();
   }
-  if (my_newton) atomicAdd((unsigned long long*)&s_newton, (unsigned long long)my_newton);
+#undef BH
+#undef BW
+#undef BHW
+#undef BLS
+  if (fill_only) return;
+  if (my_newton) atomicAdd(&s_newton, my_newton);
   __syncthreads();
   if (tid == 0) {
     ngs_out[inst] = ngs;
-    newton_out[inst] = s_newton;
-    scal[inst] = gzero ? 0.0 : 1.0;  // lake fill needed
+    newton_out[inst] = (long long)s_newton;
+    scal[inst] = gzero ? 0.0 : 1.0;
   }
 }
 
-void launch_levels_fill(DevState& S, hipStream_t st) {
-  k_levels_fill<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.rec, S.dmask, S.h, S.hwater, S.lake_depth, S.order, S.lstart, S.nlev,
-                                             S.err);
+// ---------------------------------------------------------------------------------------------------
+static size_t area_smem(int vcap) {
+  return 2 * vcap * sizeof(double) + GRP_STG * (sizeof(double) + 9 * sizeof(int)) + (GRP_BUCKET + 1) * sizeof(int);
 }
+static size_t spl_smem(int vcap) {
+  return (size_t)vcap * (6 * sizeof(double) + 2 * sizeof(int)) + GRP_STG * (3 * sizeof(double) + 2 * sizeof(int)) +
+         (GRP_BUCKET + 1) * sizeof(int);
+}
+static const int kAreaVcap = 2048;
+static const int kSplVcap = 1536;
+
+int sweep_set_lds(int smem_max) {
+  if ((int)area_smem(kAreaVcap) > smem_max || (int)spl_smem(kSplVcap) + 4096 > smem_max ||
+      LVL_HCAP * 4 > smem_max)
+    return -1;
+  if (hipFuncSetAttribute((const void*)k_area_sweep, hipFuncAttributeMaxDynamicSharedMemorySize, (int)area_smem(kAreaVcap)) != hipSuccess) return -1;
+  if (hipFuncSetAttribute((const void*)k_spl_sweep, hipFuncAttributeMaxDynamicSharedMemorySize, (int)spl_smem(kSplVcap)) != hipSuccess) return -1;
+  if (hipFuncSetAttribute((const void*)k_level_sort, hipFuncAttributeMaxDynamicSharedMemorySize, LVL_HCAP * 4) != hipSuccess) return -1;
+  return 0;
+}
+
+void launch_jump(DevState& S, hipStream_t st) {
+  k_jump_init<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.rec, S.J);
+  launch_fill_int(S.flags, 64, 0, st);
+  for (int it = 0; it < 26; it++) k_jump<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.J, S.flags, it);
+}
+
+void launch_levels(DevState& S, hipStream_t st) {
+  launch_jump(S, st);
+  k_level_sort<<<S.ninst, FSX_WG, LVL_HCAP * 4, st>>>(S.P, S.J, S.flags, S.lstart, S.lcur, S.nlev, S.gmeta, S.ngroups,
+                                                        S.grec, S.order, S.pos, S.err);
+}
+
+void launch_area_prep(DevState& S, hipStream_t st) {
+  k_area_prep<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.J, S.dmask, S.pos, S.lstart, S.precip, S.A0, S.dn);
+}
+
 void launch_area(DevState& S, hipStream_t st) {
-  k_area_sfd<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.rec, S.dmask, S.precip, S.order, S.lstart, S.nlev, S.area);
+  launch_area_prep(S, st);
+  launch_area_sweep(S, st);
 }
+
+void launch_area_sweep(DevState& S, hipStream_t st) {
+  k_area_sweep<<<S.ninst, FSX_WG, area_smem(kAreaVcap), st>>>(S.P, S.lstart, S.grec, S.ngroups, S.order, S.A0, S.dn,
+                                                                S.Al, S.area, kAreaVcap);
+}
+
+void launch_spl_fact(DevState& S, hipStream_t st) {
+  k_spl_fact<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.J, S.rec, S.pos, S.lstart, S.area, S.length, S.kfint,
+                                               S.ht, S.rp, S.F, S.HTl);
+}
+
+static void spl_sweep_launch(DevState& S, hipStream_t st, int fill_only) {
+  k_spl_sweep<<<S.ninst, FSX_WG, spl_smem(kSplVcap), st>>>(
+      S.P, S.lstart, S.nlev, S.grec, S.ngroups, S.order, S.rp, S.F, S.HTl, S.dn, S.bc, S.g, S.Al, S.h, S.hwater,
+      S.lake_depth, S.water, S.lsill, S.lwv, S.lsed, S.Hl, S.Wl, S.HWl, S.LSl, S.HPl, S.DHl, S.ELl, S.scal, S.ngs,
+      S.newton, S.err, kSplVcap, fill_only);
+}
+
+void launch_spl_sweep(DevState& S, hipStream_t st) { spl_sweep_launch(S, st, 0); }
 void launch_spl(DevState& S, hipStream_t st) {
-  k_spl_sfd<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.rec, S.dmask, S.bc, S.order, S.lstart, S.nlev, S.area, S.length, S.kfint,
-                                         S.g, S.ht, S.h, S.hp, S.dh, S.elev, S.water, S.lsill, S.lwv, S.lsed, S.scal,
-                                         S.ngs, S.newton, S.err);
+  launch_spl_fact(S, st);
+  spl_sweep_launch(S, st, 0);
+}
+void launch_fill(DevState& S, hipStream_t st) {
+  launch_spl_fact(S, st);
+  spl_sweep_launch(S, st, 1);
 }

@@ -173,6 +173,22 @@ class Batch:
         check(lib.fsx_get_stats(self._h, k, o), "get_stats")
         return dict(zip(["nbasins", "nconn", "ntree", "levels", "ngs", "newton_iters"], list(o)))
 
+    def profiling(self, on=True):
+        """Record a HIP event pair around every phase of each step (on the stream the step runs on)."""
+        check(lib.fsx_set_profiling(self._h, 1 if on else 0), "set_profiling")
+
+    def reset_timers(self): check(lib.fsx_reset_timers(self._h), "reset_timers")
+
+    def timers(self):
+        """{phase: (total ms, launches)} accumulated while profiling was on."""
+        n = len(_lib.PHASES)
+        ms = (C.c_double * n)()
+        cnt = (C.c_longlong * n)()
+        rc = lib.fsx_get_timers(self._h, ms, cnt, n)
+        if rc < 0:
+            check(rc, "get_timers")
+        return {p: (ms[i], cnt[i]) for i, p in enumerate(_lib.PHASES)}
+
     def memory(self):
         a, b = C.c_longlong(), C.c_longlong()
         check(lib.fsx_memory(self._h, C.byref(a), C.byref(b)), "memory")

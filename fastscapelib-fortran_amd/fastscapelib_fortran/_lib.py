@@ -17,6 +17,11 @@ FSX_H, FSX_B, FSX_U, FSX_PRECIP, FSX_ETOT, FSX_ERATE, FSX_AREA, FSX_LAKE_DEPTH, 
     FSX_HWATER, FSX_KF, FSX_KD = range(13)
 FSX_REC, FSX_NDON, FSX_DON, FSX_ORDER, FSX_MREC, FSX_MNREC = range(6)
 
+# phase timers, in the library's order (include/fastscape_hip.h, fsx_get_timers)
+PHASES = ["uplift", "receivers", "donors", "lm_labels", "lm_basins", "lm_walk", "lm_edges", "lm_sort", "lm_mst",
+          "lm_orient", "lm_correct", "levels", "area_prep", "area_sweep", "spl_prep", "spl_fact", "spl_sweep",
+          "spl_post", "diffusion"]
+
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 _vp = C.c_void_p
@@ -55,6 +60,9 @@ def _load():
         "fsx_indexx": [C.c_int, _dp, _ip],
         "fsx_get_stats": [_vp, C.c_int, C.POINTER(C.c_longlong)],
         "fsx_memory": [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)],
+        "fsx_set_profiling": [_vp, C.c_int],
+        "fsx_get_timers": [_vp, C.POINTER(C.c_double), C.POINTER(C.c_longlong), C.c_int],
+        "fsx_reset_timers": [_vp],
     }
     for name, args in sig.items():
         f = getattr(lib, name)

@@ -129,6 +129,17 @@ int fsx_indexx(int n, const double *keys, int *perm);
  * out[3]=levels, out[4]=Gauss-Seidel iterations, out[5]=Newton iterations (total). */
 int fsx_get_stats(fsx_ctx *ctx, int inst, long long *out6);
 
+/* Phase timers, the counterpart of the reference's per-phase cpu_time accumulators
+ * (timeUplift/timeSPL/timeDiffusion, src/FastScape_api.f90:207-257; printed by FastScape_Debug,
+ * src/FastScape_ctx.f90:553-559).  When on, every step records a HIP event pair around each phase
+ * on the stream it runs on.  fsx_get_timers fills up to n accumulated milliseconds and counts
+ * (phase order: uplift, receivers, donors, lm_labels, lm_basins, lm_walk, lm_edges, lm_sort, lm_mst,
+ * lm_orient, lm_correct, levels, area_prep, area_sweep, spl_prep, spl_fact, spl_sweep, spl_post,
+ * diffusion) and returns the number of phases. */
+int fsx_set_profiling(fsx_ctx *ctx, int on);
+int fsx_get_timers(fsx_ctx *ctx, double *ms, long long *counts, int n);
+int fsx_reset_timers(fsx_ctx *ctx);
+
 /* Device-resident memory of a context (bytes) and the bytes per node the layout costs. */
 int fsx_memory(fsx_ctx *ctx, long long *bytes, long long *bytes_per_node);
 
