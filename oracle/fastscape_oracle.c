@@ -549,13 +549,15 @@ static int find_mult_rec(fso_state *s) {
     int nr = s->mnrec[ij];
     double *w = s->mwrec + NDMAX * ij;
     if (s->p < 0.0) {
-      float slope = 0.0f;
+      /* slope is double precision but rounded through real(); 0.5 and 0.6 are default-real
+         literals promoted to double in the mixed-mode expression */
+      double slope = 0.0;
       if (nr != 0) {
         double sw = 0.0;
         for (int k = 0; k < nr; k++) sw += w[k];
-        slope = (float)(sw / (double)nr);
+        slope = (double)(float)(sw / (double)nr);
       }
-      s->p_mfd_exp[ij] = (double)(0.5f + 0.6f * slope); /* single-precision literals */
+      s->p_mfd_exp[ij] = (double)0.5f + (double)0.6f * slope;
     }
     for (int k = 0; k < nr; k++) w[k] = pow(w[k], s->p_mfd_exp[ij]);
     double sumw = 0.0;

@@ -1,0 +1,88 @@
+"""Writes tests/golden/*.npz from the reference Fortran compiled from its own sources
+(oracle/_ref/libfsref.so, built by `make -C oracle ref`).  Each fixture holds the seeded inputs and
+the reference's outputs, so tests can check the oracle and the GPU path without the reference.
+
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path[:0] = [os.path.join(ROOT, "oracle"), os.path.join(ROOT, "tests")]
+
+import oracle as O  # noqa: E402
+from conftest import landscape  # noqa: E402
+
+
+def run_ref(L, steps_to_save):
+    f = O.FortranRef()
+    f.setup(L["nx"], L["ny"])
+    f.set_xl_yl(L["xl"], L["yl"])
+    f.set_dt(L["dt"])
+    f.init_h(L["h"])
+    f.set_erosional_parameters(L["kf"], L["kfsed"], L["m"], L["n"], L["kd"], L["kdsed"], L["g1"], L["g2"], L["p"])
+    f.set_u(L["u"])
+    f.set_bc(L["ibc"])
+    out = {}
+    last = max(steps_to_save)
+    for s in range(1, last + 1):
+        f.execute_step()
+        if s in steps_to_save:
+            for k in ["h", "b", "etot", "erate", "area", "lake_depth"]:
+                out[f"{k}_{s}"] = f.field(k)
+    return out
+
+
+def inputs(L):
+    return {"in_" + k: np.asarray(v) for k, v in L.items()}
+
+
+def main():
+    # the reference's own CPU case: 101x101, dx = 1 km, D8, 200 steps
+    L = landscape(101, 101, seed=0)
+    np.savez_compressed(os.path.join(HERE, "plain_101_d8.npz"), **inputs(L), **run_ref(L, {1, 10, 50, 200}))
+
+    # short runs across boundary conditions, exponents and deposition (single flow direction)
+    cases = {
+        "sfd_1010_n1": dict(ibc=1010),
+        "sfd_0101_n15": dict(ibc=101, n=1.5, m=0.675),
+        "sfd_1000_n2": dict(ibc=1000, n=2.0, m=0.9),
+        "sfd_1111_g1": dict(ibc=1111, g=1.0),
+        "mfd_1111_p1_g1": dict(ibc=1111, g=1.0, p=1.0, kfsed=1e-5, kdsed=2e-2),
+        "mfd_1010_p1": dict(ibc=1010, p=1.0),
+        "mfd_1111_pneg": dict(ibc=1111, p=-1.0, n=1.5, m=0.675),
+    }
+    for name, kw in cases.items():
+        L = landscape(47, 31, seed=len(name), dx=250.0, kf=2e-6, dt=2e3, sinus=(10.0, 4000.0), kd=3e-2, **kw)
+        np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **inputs(L), **run_ref(L, {1, 5, 20}))
+
+    # routing snapshots: receivers, donors and stack after LocalMinima on fixed inputs
+    f = O.FortranRef()
+    snaps = {}
+    for ibc in (1111, 1010, 101, 1000, 11):
+        L = landscape(33, 27, seed=ibc, dx=500.0, sinus=(6.0, 5000.0), ibc=ibc)
+        dx = L["xl"] / 32
+        dy = L["yl"] / 26
+        rec, length, ndon, don, stack = f.routing(L["h"], 33, 27, dx, dy, ibc)
+        snaps.update({f"h_{ibc}": L["h"], f"rec_{ibc}": rec, f"length_{ibc}": length, f"ndon_{ibc}": ndon,
+                      f"don_{ibc}": don, f"stack_{ibc}": stack})
+    np.savez_compressed(os.path.join(HERE, "routing_33x27.npz"), **snaps)
+
+    # the Numerical Recipes edge sort on tie-heavy keys
+    rng = np.random.default_rng(5)
+    sorts = {}
+    for i, n in enumerate([1, 2, 7, 8, 9, 50, 300, 2500]):
+        a = rng.integers(0, max(2, n // 4), n).astype(float)
+        a[rng.random(n) < 0.1] = -1e10
+        sorts[f"keys_{i}"] = a
+        sorts[f"perm_{i}"] = f.indexx(a)
+    np.savez_compressed(os.path.join(HERE, "indexx.npz"), **sorts)
+
+
+if __name__ == "__main__":
+    main()

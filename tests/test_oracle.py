@@ -1,0 +1,279 @@
+"""Pins the oracle (oracle/fastscape_oracle.c, the CPU restatement the GPU path is checked against).
+
+1. Against the reference Fortran's own outputs: the committed fixtures in tests/golden/ (written by
+   tests/golden/make_golden.py from the unmodified reference build) and, when oracle/_ref is built,
+   live seeded runs.  Bit-exact.
+2. Against what the mathematics fixes: brute-force steepest descent, receiver inversion, DFS
+   preorder, subtree sums, the tridiagonal systems of the ADI step solved densely, the implicit
+   stream-power relation, the sort property of the edge order.
+"""
+from __future__ import annotations
+
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, apply, landscape
+
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def _golden(name):
+    return dict(np.load(os.path.join(GOLDEN, name), allow_pickle=False))
+
+
+def _L_from(g):
+    L = {k[3:]: g[k] for k in g if k.startswith("in_")}
+    for k in ["nx", "ny", "ibc"]:
+        L[k] = int(L[k])
+    for k in ["xl", "yl", "kfsed", "m", "n", "kdsed", "g1", "g2", "p", "dt"]:
+        L[k] = float(L[k])
+    return L
+
+
+# ---- 1. against the reference ---------------------------------------------------------------------
+@pytest.mark.parametrize("name", sorted(os.path.basename(p) for p in glob.glob(os.path.join(GOLDEN, "*.npz"))
+                                        if "routing" not in p and "indexx" not in p))
+def test_oracle_matches_reference_fixtures(oracle, name):
+    g = _golden(name)
+    L = _L_from(g)
+    o = oracle.COracle(L["nx"], L["ny"])
+    apply(o, L)
+    saved = sorted({int(k.split("_")[-1]) for k in g if not k.startswith("in_")})
+    for s in range(1, max(saved) + 1):
+        o.execute_step()
+        if s in saved:
+            for k in ["h", "b", "etot", "erate", "area", "lake_depth"]:
+                assert np.array_equal(o.field(k), g[f"{k}_{s}"]), f"{name}: {k} differs at step {s}"
+
+
+def test_oracle_routing_matches_reference_fixtures(oracle):
+    g = _golden("routing_33x27.npz")
+    for ibc in (1111, 1010, 101, 1000, 11):
+        h = g[f"h_{ibc}"]
+        dx, dy = 500.0, 500.0
+        rec, length, ndon, don, stack, _ = oracle.local_minima(h, 33, 27, dx, dy, ibc)
+        assert np.array_equal(rec, g[f"rec_{ibc}"])
+        assert np.array_equal(length, g[f"length_{ibc}"])
+        assert np.array_equal(ndon, g[f"ndon_{ibc}"])
+        assert np.array_equal(don, g[f"don_{ibc}"])
+        assert np.array_equal(stack, g[f"stack_{ibc}"])
+
+
+def test_oracle_indexx_matches_reference_fixtures(oracle):
+    g = _golden("indexx.npz")
+    for i in range(len([k for k in g if k.startswith("keys_")])):
+        assert np.array_equal(oracle.indexx(g[f"keys_{i}"]), g[f"perm_{i}"])
+
+
+def _ref():
+    import oracle as O
+    if not O.ref_available():
+        pytest.skip("reference build absent (make -C oracle ref needs the reference sources)")
+    return O.FortranRef()
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_oracle_indexx_matches_reference_live(oracle, seed):
+    f = _ref()
+    rng = np.random.default_rng(100 + seed)
+    for n in [3, 8, 15, 64, 257, 1000, 4000]:
+        a = rng.integers(0, max(2, n // (2 + seed)), n).astype(float)
+        a[rng.random(n) < 0.15] = -1e10
+        assert np.array_equal(oracle.indexx(a), f.indexx(a)), (seed, n)
+
+
+@pytest.mark.parametrize("ibc", [1111, 1010, 101, 1000, 11, 1, 10, 100])
+def test_oracle_routing_matches_reference_live(oracle, ibc):
+    f = _ref()
+    rng = np.random.default_rng(ibc)
+    for nx, ny in [(20, 13), (41, 37)]:
+        h = rng.random(nx * ny) * 3.0
+        h.reshape(ny, nx)[ny // 3:ny // 2, nx // 4:nx // 2] -= 2.0  # a closed depression
+        dx, dy = 120.0, 90.0
+        got = oracle.local_minima(h, nx, ny, dx, dy, ibc)
+        ref = f.routing(h, nx, ny, dx, dy, ibc)
+        for a, b, what in zip(got[:5], ref, ["rec", "length", "ndon", "don", "stack"]):
+            assert np.array_equal(a, b), (ibc, nx, what)
+
+
+# ---- 2. against the mathematics ---------------------------------------------------------------------
+def _brute_receivers(h, nx, ny, dx, dy, ibc):
+    (i1, i2, j1, j2, xc, yc), _ = __import__("oracle").bc_decode(ibc, nx, ny)
+    rec = np.arange(nx * ny)
+    ln = np.zeros(nx * ny)
+    for j in range(j1, j2 + 1):
+        for i in range(i1, i2 + 1):
+            ij = (i - 1) + (j - 1) * nx
+            best = np.finfo(float).tiny
+            for jj in (-1, 0, 1):
+                for ii in (-1, 0, 1):
+                    a, b = i + ii, j + jj
+                    a = a + nx if (a < 1 and xc) else a
+                    a = max(a, 1)
+                    a = a - nx if (a > nx and xc) else a
+                    a = min(a, nx)
+                    b = b + ny if (b < 1 and yc) else b
+                    b = max(b, 1)
+                    b = b - ny if (b > ny and yc) else b
+                    b = min(b, ny)
+                    k = (a - 1) + (b - 1) * nx
+                    if k == ij:
+                        continue
+                    d = np.sqrt((dx * ii) ** 2 + (dy * jj) ** 2)
+                    s = (h[ij] - h[k]) / d
+                    if s > best:
+                        best, rec[ij], ln[ij] = s, k, d
+    return rec, ln
+
+
+@pytest.mark.parametrize("ibc", [1111, 1010, 101, 1000])
+def test_receivers_are_first_steepest_descent(oracle, ibc):
+    rng = np.random.default_rng(ibc)
+    nx, ny = 17, 12
+    h = np.round(rng.random(nx * ny) * 4) / 4  # many exact slope ties
+    rec, ln = oracle.find_receiver(h, nx, ny, 30.0, 40.0, ibc)
+    br, bl = _brute_receivers(h, nx, ny, 30.0, 40.0, ibc)
+    assert np.array_equal(rec, br) and np.array_equal(ln, bl)
+
+
+def test_donors_invert_receivers_and_stack_is_preorder(oracle):
+    rng = np.random.default_rng(3)
+    nx, ny = 23, 19
+    h = rng.random(nx * ny)
+    rec, _ = oracle.find_receiver(h, nx, ny, 1.0, 1.0, 1111)
+    ndon, don = oracle.find_donor(rec)
+    for v in range(nx * ny):
+        d = [u for u in range(nx * ny) if rec[u] == v and u != v]
+        assert list(don[v, :ndon[v]]) == d  # ascending, exact inverse
+    stack = oracle.find_stack(rec, ndon, don)
+    # recursive preorder, roots ascending, children in donor order
+    want = []
+
+    def visit(v):
+        want.append(v)
+        for u in don[v, :ndon[v]]:
+            visit(u)
+    for v in range(nx * ny):
+        if rec[v] == v:
+            visit(v)
+    assert list(stack) == want
+
+
+def test_area_is_subtree_sum_with_root_doubling(oracle):
+    L = landscape(29, 21, seed=4)
+    o = oracle.COracle(29, 21)
+    apply(o, L)
+    o.execute_step()
+    rec = o.ifield("rec")
+    a = o.field("area")
+    dxdy = (L["xl"] / 28) * (L["yl"] / 20)
+    sub = np.full(rec.size, dxdy)
+    order = np.argsort(-_depth(rec), kind="stable")
+    for v in order:
+        if rec[v] != v:
+            sub[rec[v]] += sub[v]
+    roots = rec == np.arange(rec.size)
+    sub[roots] *= 2.0  # the reference's a(rec(ijk)) += a(ijk) with rec(ijk) == ijk
+    assert np.array_equal(a, sub)  # integral dx*dy: exact in any order
+
+
+def _depth(rec):
+    d = np.zeros(rec.size, np.int64)
+    for v in range(rec.size):
+        x, k = v, 0
+        while rec[x] != x:
+            x, k = rec[x], k + 1
+        d[v] = k
+    return d
+
+
+def test_local_minima_leaves_only_fixed_outlets(oracle):
+    for seed in range(4):
+        L = landscape(35, 28, seed=seed, sinus=(8.0, 6000.0), ibc=[1111, 1010, 101, 1000][seed])
+        dx = L["xl"] / 34
+        dy = L["yl"] / 27
+        rec, length, ndon, don, stack, st = oracle.local_minima(L["h"], 35, 28, dx, dy, L["ibc"])
+        _, bc = oracle.bc_decode(L["ibc"], 35, 28)
+        roots = np.flatnonzero(rec == np.arange(rec.size))
+        assert bc[roots].all(), "a closed basin kept its own outlet"
+        assert st["nlocmin_after"] == 0
+        assert st["ntree"] == st["nbasins"] - 1  # the basin graph is connected
+        # every receiver is a grid neighbour at the distance stored in length
+        i, j = np.arange(rec.size) % 35, np.arange(rec.size) // 35
+        di = np.abs(i - i[rec])
+        di = np.minimum(di, 35 - di)
+        dj = np.abs(j - j[rec])
+        dj = np.minimum(dj, 28 - dj)
+        nb = rec != np.arange(rec.size)
+        assert np.all(di[nb] <= 1) and np.all(dj[nb] <= 1)
+        assert np.allclose(length[nb], np.sqrt((di[nb] * dx) ** 2 + (dj[nb] * dy) ** 2), rtol=0, atol=1e-9)
+
+
+def test_indexx_sorts(oracle):
+    rng = np.random.default_rng(9)
+    for n in [1, 5, 8, 100, 3000]:
+        a = rng.integers(0, 10, n).astype(float)
+        p = oracle.indexx(a) - 1
+        assert np.array_equal(np.sort(p), np.arange(n))
+        assert np.all(np.diff(a[p]) >= 0)
+        b = rng.random(n)  # distinct keys: the unique sorted order
+        assert np.array_equal(oracle.indexx(b) - 1, np.argsort(b))
+
+
+def test_diffusion_solves_the_adi_systems(oracle):
+    """One step with erosion off: compare with the two tridiagonal systems solved densely."""
+    nx, ny, dt, kd = 9, 7, 50.0, 3.0
+    dx, dy = 10.0, 12.0
+    rng = np.random.default_rng(1)
+    h = rng.random(nx * ny)
+    o = oracle.COracle(nx, ny)
+    o.set_xl_yl(dx * (nx - 1), dy * (ny - 1))
+    o.set_dt(dt)
+    o.init_h(h)
+    o.set_erosional_parameters(np.zeros(nx * ny), -1.0, 0.4, 1.0, np.full(nx * ny, kd), -1.0, 0.0, 0.0, -2.0)
+    o.set_bc(1111)
+    o.execute_step()
+    z = h.reshape(ny, nx)
+    f = kd * dt / 2.0
+    ax, ay = f / dx ** 2, f / dy ** 2
+    zi = z.copy()
+    for j in range(1, ny - 1):  # x-implicit, y-explicit
+        A = np.eye(nx)
+        r = z[j].copy()
+        for i in range(1, nx - 1):
+            A[i, i - 1:i + 2] = [-ax, 1 + 2 * ax, -ax]
+            r[i] = z[j, i] + ay * (z[j + 1, i] - 2 * z[j, i] + z[j - 1, i])
+        zi[j] = np.linalg.solve(A, r)
+    zp = z.copy()
+    for i in range(1, nx - 1):  # y-implicit, x-explicit
+        A = np.eye(ny)
+        r = zi[:, i].copy()
+        for j in range(1, ny - 1):
+            A[j, j - 1:j + 2] = [-ay, 1 + 2 * ay, -ay]
+            r[j] = zi[j, i] + ax * (zi[j, i + 1] - 2 * zi[j, i] + zi[j, i - 1])
+        zp[:, i] = np.linalg.solve(A, r)
+    assert np.allclose(o.field("h"), zp.ravel(), rtol=0, atol=1e-13)
+
+
+def test_stream_power_satisfies_the_implicit_relation(oracle):
+    """n = 1, no deposition: every node above its receiver's water level obeys
+    h = (ht + F*h_rec)/(1+F), F = kf*dt*a**m/length (the implicit scheme of Braun & Willett)."""
+    L = landscape(31, 24, seed=6, u=0.0, kd=0.0)
+    o = oracle.COracle(31, 24)
+    apply(o, L)
+    ht = L["h"].copy()
+    o.execute_step()
+    h, a, rec, ln = o.field("h"), o.field("area"), o.ifield("rec"), o.field("length")
+    hw = o.field("lake_depth") + ht  # lake surface
+    ok = 0
+    for v in range(h.size):
+        r = rec[v]
+        if r == v or ht[v] <= hw[r] or h[v] <= h[r]:
+            continue
+        F = L["kf"][v] * L["dt"] * a[v] ** L["m"] / ln[v]
+        assert abs(h[v] - (ht[v] + F * h[r]) / (1 + F)) <= 1e-12 * max(1.0, abs(h[v]))
+        ok += 1
+    assert ok > h.size // 4
