@@ -49,6 +49,7 @@ struct fsx_ctx {
   std::vector<void*> allocs;
   hipStream_t stream = nullptr;
   bool dirty = true;
+  bool any_sfd = false, any_mfd = false;  // which flow-routing paths the active instances use
   long long bytes = 0;
   // phase timers: a ring of (start, stop) event pairs, accumulated every kRing steps
   bool prof = false;
@@ -154,6 +155,11 @@ static int check_inst(fsx_ctx* c, int inst) {
 
 static int upload_params(fsx_ctx* c) {
   if (!c->dirty) return 0;
+  c->any_sfd = c->any_mfd = false;
+  for (const InstParams& P : c->hp) {
+    if (!P.active || !P.run_spl) continue;
+    if (P.sfd) c->any_sfd = true; else c->any_mfd = true;
+  }
   HIPCHK(hipMemcpyAsync(c->S.P, c->hp.data(), sizeof(InstParams) * c->hp.size(), hipMemcpyHostToDevice, c->stream));
   c->dirty = false;
   return 0;
@@ -291,6 +297,20 @@ extern "C" int fsx_set_bc(fsx_ctx* c, int inst, int ibc) {
   return upload_bc(c, inst);
 }
 
+// Multiple-flow-direction buffers, allocated the first time an instance selects that routing.
+static int mfd_alloc(fsx_ctx* c) {
+  DevState& S = c->S;
+  if (S.mrec) return 0;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const long long N = S.ntot;
+  int rc = 0;
+#define A(p, n) if (!rc) rc = dalloc(c, &S.p, (n))
+  A(mrec, 8 * N); A(mdon, 8 * N); A(mw, 8 * N); A(mfact, 8 * N); A(mdir, 8 * N); A(mflag, N);
+  A(mnrec, N); A(mndon, N); A(mindeg, N); A(mstack, N); A(mls, N + S.ninst); A(mnlev, S.ninst);
+#undef A
+  return rc;
+}
+
 static int put(fsx_ctx* c, double* dst, const double* src, int n, int on_device) {
   HIPCHK(hipMemcpyAsync(dst, src, sizeof(double) * (size_t)n, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                         c->stream));
@@ -330,8 +350,9 @@ extern "C" int fsx_set_erosional_parameters(fsx_ctx* c, int inst, const double* 
   }
   const double tiny = 2.2250738585072014e-308;
   if (kdmax > tiny || kdsed > tiny) P.run_diff = 1;
-  if (!P.sfd) return fail(FSX_EINVAL, "multiple flow direction (p >= -1.5) is not available in this build");
   c->dirty = true;
+  if (!P.sfd)
+    if (int r = mfd_alloc(c)) return r;
   return 0;
 }
 
@@ -435,14 +456,18 @@ static void enqueue_routing(fsx_ctx* c, hipStream_t st) {
   PHASE(PH_LM_CORRECT, launch_lm_correct(S, st));
   PHASE(PH_DONORS, launch_donors(S, st));
   PHASE(PH_LEVELS, launch_levels(S, st));
-  PHASE(PH_AREA_PREP, launch_area_prep(S, st));
-  PHASE(PH_AREA_SWEEP, launch_area_sweep(S, st));
+  if (c->any_sfd) {
+    PHASE(PH_AREA_PREP, launch_area_prep(S, st));
+    PHASE(PH_AREA_SWEEP, launch_area_sweep(S, st));
+  }
+  // multiple flow direction: fill, receivers, Kahn layers, weighted accumulation
+  if (c->any_mfd) PHASE(PH_MFD_ROUTE, launch_mfd_route(S, st));
 }
 
 static void enqueue_route_only(fsx_ctx* c, hipStream_t st) {
   enqueue_routing(c, st);
   launch_spl_prep(c->S, st);
-  launch_fill(c->S, st);  // lake surface / depth without eroding
+  if (c->any_sfd) launch_fill(c->S, st);  // lake surface / depth without eroding
 }
 
 static void enqueue_step(fsx_ctx* c, hipStream_t st) {
@@ -450,8 +475,11 @@ static void enqueue_step(fsx_ctx* c, hipStream_t st) {
   PHASE(PH_UPLIFT, launch_uplift(S, st));
   enqueue_routing(c, st);
   PHASE(PH_SPL_PREP, launch_spl_prep(S, st));
-  PHASE(PH_SPL_FACT, launch_spl_fact(S, st));
-  PHASE(PH_SPL_SWEEP, launch_spl_sweep(S, st));  // also fills hwater / lake_depth
+  if (c->any_sfd) {
+    PHASE(PH_SPL_FACT, launch_spl_fact(S, st));
+    PHASE(PH_SPL_SWEEP, launch_spl_sweep(S, st));  // also fills hwater / lake_depth
+  }
+  if (c->any_mfd) PHASE(PH_MFD_SPL, launch_mfd_spl(S, st));
   PHASE(PH_SPL_POST, launch_spl_post(S, st));
   PHASE(PH_DIFFUSION, launch_diffusion(S, st));
 }
@@ -557,6 +585,16 @@ extern "C" int fsx_copy_ifield(fsx_ctx* c, int inst, int field, int* dst, int on
     case FSX_DON:
       k_expand_donors<<<(P.nn + 255) / 256, 256, 0, c->stream>>>(c->S.P, inst, c->S.dmask, nullptr, tmp);
       HIPCHK(hipMemcpyAsync(dst, tmp, sizeof(int) * 8 * (size_t)P.nn, kind, c->stream));
+      break;
+    case FSX_MREC:
+    case FSX_MNREC:
+    case FSX_MSTACK:
+      if (P.sfd || !c->S.mrec) return fail(FSX_EINVAL, "instance does not use multiple flow direction");
+      if (field == FSX_MREC)
+        HIPCHK(hipMemcpyAsync(dst, c->S.mrec + 8 * (size_t)P.off, sizeof(int) * 8 * (size_t)P.nn, kind, c->stream));
+      else
+        HIPCHK(hipMemcpyAsync(dst, (field == FSX_MNREC ? c->S.mnrec : c->S.mstack) + P.off, sizeof(int) * P.nn, kind,
+                              c->stream));
       break;
     default:
       return fail(FSX_EINVAL, "unknown integer field");

@@ -133,7 +133,8 @@ void launch_spl_fact(DevState& S, hipStream_t st);
 void launch_spl_sweep(DevState& S, hipStream_t st);
 void launch_spl(DevState& S, hipStream_t st);
 void launch_diffusion(DevState& S, hipStream_t st);
-void launch_mfd(DevState& S, hipStream_t st);
+void launch_mfd_route(DevState& S, hipStream_t st);
+void launch_mfd_spl(DevState& S, hipStream_t st);
 int sweep_set_lds(int smem_max);
 void launch_fill(DevState& S, hipStream_t st);
 
@@ -159,5 +160,7 @@ enum Phase {
   PH_SPL_SWEEP,    // * k_spl_sweep
   PH_SPL_POST,     // * k_spl_post
   PH_DIFFUSION,    //   k_adi_prep, k_adi_x, k_adi_y, k_adi_post
+  PH_MFD_ROUTE,    //   multiple flow direction: fill, receivers, donors, Kahn layers, accumulation
+  PH_MFD_SPL,      //   multiple flow direction: edge factors + Gauss-Seidel stream power
   PH_COUNT
 };

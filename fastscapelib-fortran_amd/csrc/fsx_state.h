@@ -41,10 +41,14 @@ struct DevState {
   int *sB = nullptr, *sp = nullptr, *sq = nullptr;
   double *sw = nullptr;
 
-  // ---- multiple flow direction (8 slots per node, allocated when any instance uses it) ----
-  int *mrec = nullptr, *mstack = nullptr, *mndon = nullptr, *mvis = nullptr;
-  uint8_t *mnrec = nullptr;
-  double *mwrec = nullptr, *mlrec = nullptr;
+  // ---- multiple flow direction (allocated when an instance first selects it) ----
+  // 8 slots per node in the reference's receiver order: target, weight, direction, stream-power
+  // factor; donor slots (u*8+k of the pointing slot); Kahn-layer order and layer starts
+  int *mrec = nullptr, *mdon = nullptr;
+  double *mw = nullptr, *mfact = nullptr;
+  uint8_t *mdir = nullptr, *mflag = nullptr;
+  int *mnrec = nullptr, *mndon = nullptr, *mindeg = nullptr, *mstack = nullptr, *mls = nullptr;
+  int *mnlev = nullptr;  // [ninst]
 
   // ---- per instance ----
   int *nbasins = nullptr, *basin0 = nullptr, *nconn = nullptr, *nlev = nullptr, *err = nullptr;

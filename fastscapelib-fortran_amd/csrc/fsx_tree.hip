@@ -185,6 +185,7 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_area_prep(const InstParams* __res
                                                          const int* __restrict__ lstart, const double* __restrict__ precip,
                                                          double* __restrict__ A0, int* __restrict__ dn) {
   NODE_PROLOGUE
+  if (!P.sfd) return;
   const int p = pos[g_];
   const int d = jdist(J[g_]);
   const long long gp = (long long)P.off + p;

@@ -163,6 +163,14 @@ class Batch:
     def donors(self, k): return self.ifield(k, _lib.FSX_NDON), self.ifield(k, _lib.FSX_DON)
     def order(self, k): return self.ifield(k, _lib.FSX_ORDER)
 
+    def mrec(self, k):
+        """Multiple-flow receivers of landscape k: (mnrec[nn], mrec[nn, 8]) in the reference's slot order."""
+        return self.ifield(k, _lib.FSX_MNREC), self.ifield(k, _lib.FSX_MREC)
+
+    def morder(self, k):
+        """Multiple-flow order of landscape k: every node before its receivers (the role of mstack)."""
+        return self.ifield(k, _lib.FSX_MSTACK)
+
     def get_step(self, k):
         s = C.c_int()
         check(lib.fsx_get_step(self._h, k, C.byref(s)), "get_step")

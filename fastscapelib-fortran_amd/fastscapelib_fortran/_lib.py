@@ -15,12 +15,12 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libfastscap
 # field ids (include/fastscape_hip.h)
 FSX_H, FSX_B, FSX_U, FSX_PRECIP, FSX_ETOT, FSX_ERATE, FSX_AREA, FSX_LAKE_DEPTH, FSX_SEDFLUX, FSX_LENGTH, \
     FSX_HWATER, FSX_KF, FSX_KD = range(13)
-FSX_REC, FSX_NDON, FSX_DON, FSX_ORDER, FSX_MREC, FSX_MNREC = range(6)
+FSX_REC, FSX_NDON, FSX_DON, FSX_ORDER, FSX_MREC, FSX_MNREC, FSX_MSTACK = range(7)
 
 # phase timers, in the library's order (include/fastscape_hip.h, fsx_get_timers)
 PHASES = ["uplift", "receivers", "donors", "lm_labels", "lm_basins", "lm_walk", "lm_edges", "lm_sort", "lm_mst",
           "lm_orient", "lm_correct", "levels", "area_prep", "area_sweep", "spl_prep", "spl_fact", "spl_sweep",
-          "spl_post", "diffusion"]
+          "spl_post", "diffusion", "mfd_route", "mfd_spl"]
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)

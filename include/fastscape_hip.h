@@ -54,8 +54,9 @@ enum {
   FSX_NDON = 1,   /* donor count (ndon)                                                  */
   FSX_DON = 2,    /* donors, 8 slots per node ascending, unused = -1 (don(8,nn))         */
   FSX_ORDER = 3,  /* base-level-first topological order of the nodes (replaces stack)    */
-  FSX_MREC = 4,   /* multiple-flow receivers, 8 slots per node, unused = -1 (mrec)       */
+  FSX_MREC = 4,   /* multiple-flow receivers, 8 slots per node in scan order, unused = -1 (mrec) */
   FSX_MNREC = 5,  /* multiple-flow receiver count (mnrec)                                */
+  FSX_MSTACK = 6, /* multiple-flow order, donors before receivers (replaces mstack)      */
 };
 
 /* Thread-local text of the last error. */
@@ -77,7 +78,8 @@ int fsx_set_dt(fsx_ctx *ctx, int inst, double dt);
 /* FastScape_Set_BC (src/FastScape_api.f90:602-614; SetBC src/FastScape_ctx.f90:565-593) */
 int fsx_set_bc(fsx_ctx *ctx, int inst, int ibc);
 /* FastScape_Set_Erosional_Parameters (src/FastScape_api.f90:523-536; SetErosionalParam
- * src/FastScape_ctx.f90:423-450): p < -1.5 selects single flow direction. */
+ * src/FastScape_ctx.f90:423-450): p < -1.5 selects single flow direction, otherwise multiple flow
+ * direction with weights slope**p (p < 0: exponent from the mean slope, src/FlowRouting.f90:232-243). */
 int fsx_set_erosional_parameters(fsx_ctx *ctx, int inst, const double *kf, double kfsed, double m, double n,
                                  const double *kd, double kdsed, double g1, double g2, double p, int on_device);
 /* Newton tolerance of the n != 1 stream-power solve.  No reference setter: the reference
@@ -107,7 +109,8 @@ int fsx_synchronize(fsx_ctx *ctx);
  * Copy_Drainage_Area / Copy_Lake_Depth (src/FastScape_api.f90:299-471) and internal fields. */
 int fsx_copy_field(fsx_ctx *ctx, int inst, int field, double *dst, int on_device);
 /* Integer state after the last step: receivers, donors, order (see FSX_REC...). dst holds nn
- * ints (8*nn for FSX_DON and FSX_MREC). */
+ * ints (8*nn for FSX_DON and FSX_MREC).  FSX_MREC/FSX_MNREC/FSX_MSTACK need an instance that uses
+ * multiple flow direction (p >= -1.5). */
 int fsx_copy_ifield(fsx_ctx *ctx, int inst, int field, int *dst, int on_device);
 
 /* FastScape_Get_Step (src/FastScape_api.f90:572-584) */
@@ -135,7 +138,7 @@ int fsx_get_stats(fsx_ctx *ctx, int inst, long long *out6);
  * on the stream it runs on.  fsx_get_timers fills up to n accumulated milliseconds and counts
  * (phase order: uplift, receivers, donors, lm_labels, lm_basins, lm_walk, lm_edges, lm_sort, lm_mst,
  * lm_orient, lm_correct, levels, area_prep, area_sweep, spl_prep, spl_fact, spl_sweep, spl_post,
- * diffusion) and returns the number of phases. */
+ * diffusion, mfd_route, mfd_spl) and returns the number of phases. */
 int fsx_set_profiling(fsx_ctx *ctx, int on);
 int fsx_get_timers(fsx_ctx *ctx, double *ms, long long *counts, int n);
 int fsx_reset_timers(fsx_ctx *ctx);

@@ -611,6 +611,129 @@ static void flow_acc_mfd(fso_state *s) {
   }
 }
 
+/* One node of the implicit stream-power solve (StreamPowerLawSingleFlowDirection :296-366 /
+ * StreamPowerLaw :94-176): reads its receivers' h and its single-flow receiver's water and lake sill. */
+typedef struct {
+  fso_state *s;
+  double *h, *a, *ht, *kfint, *elev, *water, *lwv;
+  int *lsill;
+  double n, m, dt;
+  int mfd, status;
+  long newton;
+} spl_ctx;
+
+static void spl_node(spl_ctx *x, int ijk) {
+  fso_state *s = x->s;
+  double *h = x->h, *a = x->a, *ht = x->ht, *kfint = x->kfint, *elev = x->elev, *water = x->water, *lwv = x->lwv;
+  int *lsill = x->lsill;
+  const double n = x->n, m = x->m, dt = x->dt;
+  const int mfd = x->mfd;
+  int status = 0;
+  long newton = 0;
+  int ijr = s->rec[ijk];
+  if (ijr == ijk) {
+    water[ijk] = ht[ijk];
+    lsill[ijk] = ijk;
+    lwv[ijk] = 0.0;
+    return;
+  }
+  double w_rcv = water[ijr];
+  if (elev[ijk] > w_rcv) {
+    if (fabs(n - 1.0) < DBL_MIN) {
+      if (!mfd) { /* :306-314 */
+        double f = elev[ijk], df = 1.0;
+        double fact = kfint[ijk] * dt * pow(a[ijk], m) / s->length[ijk];
+        f = f + fact * h[ijr];
+        df = df + fact;
+        h[ijk] = f / df;
+      } else if (s->mnrec[ijk] > 0) { /* :104-116 */
+        double f = elev[ijk], df = 1.0;
+        for (int k = 0; k < s->mnrec[ijk]; k++) {
+          int r = s->mrec[NDMAX * ijk + k];
+          if (ht[ijk] >= ht[r]) {
+            double fact = kfint[ijk] * dt * pow(a[ijk] * s->mwrec[NDMAX * ijk + k], m) / s->mlrec[NDMAX * ijk + k];
+            f = f + fact * h[r];
+            df = df + fact;
+          }
+        }
+        h[ijk] = f / df;
+      }
+    } else if (!mfd || s->mnrec[ijk] > 0) { /* Newton, :343-358 (SFD) / :145-162 (MFD) */
+      double omega = 0.875 / n, tolp = s->newton_tol, errp = 2.0 * tolp, h0 = elev[ijk];
+      long it = 0;
+      while (errp > tolp) {
+        double f = h[ijk] - h0, df = 1.0;
+        if (!mfd) {
+          if (ht[ijk] > ht[ijr]) {
+            double fact = kfint[ijk] * dt * pow(a[ijk], m) / pow(s->length[ijk], n);
+            double x = dmax(0.0, h[ijk] - h[ijr]);
+            f = f + fact * pow(x, n);
+            df = df + fact * n * pow(x, n - 1.0);
+          }
+        } else {
+          for (int k = 0; k < s->mnrec[ijk]; k++) {
+            int r = s->mrec[NDMAX * ijk + k];
+            if (ht[ijk] > ht[r]) {
+              double fact = kfint[ijk] * dt * pow(a[ijk] * s->mwrec[NDMAX * ijk + k], m) /
+                            pow(s->mlrec[NDMAX * ijk + k], n);
+              double x = dmax(0.0, h[ijk] - h[r]);
+              f = f + fact * pow(x, n);
+              df = df + fact * n * pow(x, n - 1.0);
+            }
+          }
+        }
+        double hn = h[ijk] - f / df;
+        errp = fabs(hn - h[ijk]);
+        h[ijk] = h[ijk] * (1.0 - omega) + hn * omega;
+        if (++it > 100000) { status = -6; break; }
+      }
+      newton += it;
+    }
+    lsill[ijk] = ijk;
+    lwv[ijk] = 0.0;
+    if (h[ijk] < w_rcv) h[ijk] = w_rcv;
+  } else {
+    h[ijk] = elev[ijk];
+    lsill[ijk] = lsill[ijr];
+    if (lsill[ijk] != -1) lwv[lsill[ijk]] = lwv[lsill[ijk]] + (w_rcv - h[ijk]);
+  }
+  water[ijk] = dmax(w_rcv, h[ijk]);
+  if (status) x->status = status;
+  x->newton += newton;
+}
+
+/* MFD solve order.  The reference walks mstack backwards, so a node's multiple-flow receivers are
+ * always solved first; its single-flow receiver ijr1, whose water level and lake sill it reads
+ * (:98-100), is among them except where h0(ijk) == h0(ijr1) exactly (flats left by a filled
+ * lake).  There the reference's read depends on where ijr1 happens to sit in mstack: the current
+ * value if later, else the previous Gauss-Seidel iteration's -- and in the first iteration the
+ * never-initialised array (water is allocated without a value, :22).  This restatement fixes the
+ * defined order: a node is solved once all its multiple-flow receivers and its single-flow receiver
+ * are, which is the reference's order everywhere the two agree. */
+static void mfd_solve_ready(spl_ctx *x, int v0, char *done, int *stk) {
+  fso_state *s = x->s;
+  if (done[v0]) return;
+  int top = 0;
+  stk[top++] = v0;
+  while (top > 0) {
+    int v = stk[top - 1], dep = -1;
+    for (int k = 0; k < s->mnrec[v] && dep < 0; k++) {
+      int r = s->mrec[NDMAX * v + k];
+      if (!done[r]) dep = r;
+    }
+    if (dep < 0 && s->rec[v] != v && !done[s->rec[v]]) dep = s->rec[v];
+    if (dep >= 0) {
+      stk[top++] = dep;
+      continue;
+    }
+    top--;
+    if (!done[v]) {
+      spl_node(x, v);
+      done[v] = 1;
+    }
+  }
+}
+
 /* ------------------------------------------------------------------------------------------ */
 /* StreamPowerLawSingleFlowDirection (mfd=0), src/StreamPowerLaw.f90:210-400, and StreamPowerLaw
  * (mfd=1), src/StreamPowerLaw.f90:3-206.  Gauss-Seidel loop on the deposition term; n=1 closed form
@@ -641,8 +764,9 @@ static int stream_power(fso_state *s, int mfd) {
   const double n = s->n, m = s->m, dt = s->dt;
   const int *ord = mfd ? s->mstack : s->stack;
   int ngs = 0;
-  long newton = 0;
-  int status = 0;
+  char *done = (char *)malloc((size_t)nn);
+  int *dfs = (int *)malloc(sizeof(int) * ((size_t)nn + 1));
+  spl_ctx X = {s, h, a, ht, kfint, elev, water, lwv, lsill, n, m, dt, mfd, 0, 0};
   while (err > tol && ngs < 99) {
     ngs++;
     for (int i = 0; i < nn; i++) { hp[i] = h[i]; dh[i] = ht[i] - hp[i]; }
@@ -669,76 +793,11 @@ static int stream_power(fso_state *s, int mfd) {
     for (int i = 0; i < nn; i++) /* :285-289 */
       elev[i] = s->bc[i] ? ht[i] : ht[i] + (dh[i] - (ht[i] - hp[i])) * s->g[i] * dx * dy / a[i];
     /* solve from base level up: forward stack (SFD) / reverse mstack (MFD) */
-    for (int q = 0; q < nn; q++) {
-      int ijk = mfd ? ord[nn - 1 - q] : ord[q];
-      int ijr = s->rec[ijk];
-      if (ijr == ijk) {
-        water[ijk] = ht[ijk];
-        lsill[ijk] = ijk;
-        lwv[ijk] = 0.0;
-        continue;
-      }
-      double w_rcv = water[ijr];
-      if (elev[ijk] > w_rcv) {
-        if (fabs(n - 1.0) < DBL_MIN) {
-          if (!mfd) { /* :306-314 */
-            double f = elev[ijk], df = 1.0;
-            double fact = kfint[ijk] * dt * pow(a[ijk], m) / s->length[ijk];
-            f = f + fact * h[ijr];
-            df = df + fact;
-            h[ijk] = f / df;
-          } else if (s->mnrec[ijk] > 0) { /* :104-116 */
-            double f = elev[ijk], df = 1.0;
-            for (int k = 0; k < s->mnrec[ijk]; k++) {
-              int r = s->mrec[NDMAX * ijk + k];
-              if (ht[ijk] >= ht[r]) {
-                double fact = kfint[ijk] * dt * pow(a[ijk] * s->mwrec[NDMAX * ijk + k], m) / s->mlrec[NDMAX * ijk + k];
-                f = f + fact * h[r];
-                df = df + fact;
-              }
-            }
-            h[ijk] = f / df;
-          }
-        } else if (!mfd || s->mnrec[ijk] > 0) { /* Newton, :343-358 (SFD) / :145-162 (MFD) */
-          double omega = 0.875 / n, tolp = s->newton_tol, errp = 2.0 * tolp, h0 = elev[ijk];
-          long it = 0;
-          while (errp > tolp) {
-            double f = h[ijk] - h0, df = 1.0;
-            if (!mfd) {
-              if (ht[ijk] > ht[ijr]) {
-                double fact = kfint[ijk] * dt * pow(a[ijk], m) / pow(s->length[ijk], n);
-                double x = dmax(0.0, h[ijk] - h[ijr]);
-                f = f + fact * pow(x, n);
-                df = df + fact * n * pow(x, n - 1.0);
-              }
-            } else {
-              for (int k = 0; k < s->mnrec[ijk]; k++) {
-                int r = s->mrec[NDMAX * ijk + k];
-                if (ht[ijk] > ht[r]) {
-                  double fact = kfint[ijk] * dt * pow(a[ijk] * s->mwrec[NDMAX * ijk + k], m) /
-                                pow(s->mlrec[NDMAX * ijk + k], n);
-                  double x = dmax(0.0, h[ijk] - h[r]);
-                  f = f + fact * pow(x, n);
-                  df = df + fact * n * pow(x, n - 1.0);
-                }
-              }
-            }
-            double hn = h[ijk] - f / df;
-            errp = fabs(hn - h[ijk]);
-            h[ijk] = h[ijk] * (1.0 - omega) + hn * omega;
-            if (++it > 100000) { status = -6; break; }
-          }
-          newton += it;
-        }
-        lsill[ijk] = ijk;
-        lwv[ijk] = 0.0;
-        if (h[ijk] < w_rcv) h[ijk] = w_rcv;
-      } else {
-        h[ijk] = elev[ijk];
-        lsill[ijk] = lsill[ijr];
-        if (lsill[ijk] != -1) lwv[lsill[ijk]] = lwv[lsill[ijk]] + (w_rcv - h[ijk]);
-      }
-      water[ijk] = dmax(w_rcv, h[ijk]);
+    if (!mfd) {
+      for (int q = 0; q < nn; q++) spl_node(&X, ord[q]);
+    } else {
+      for (int i = 0; i < nn; i++) done[i] = 0;
+      for (int q = 0; q < nn; q++) mfd_solve_ready(&X, ord[nn - 1 - q], done, dfs);
     }
     double ss = 0.0;
     for (int i = 0; i < nn; i++) ss += (h[i] - hp[i]) * (h[i] - hp[i]);
@@ -757,9 +816,10 @@ static int stream_power(fso_state *s, int mfd) {
     s->sedflux[i] = ht[i] - h[i];
   }
   s->st.ngs = ngs;
-  s->st.newton_iters = newton;
+  s->st.newton_iters = X.newton;
   free(ht); free(kfint); free(dh); free(hp); free(elev); free(water); free(lwv); free(lsed); free(lsill);
-  return status;
+  free(done); free(dfs);
+  return X.status;
 }
 
 /* ------------------------------------------------------------------------------------------ */

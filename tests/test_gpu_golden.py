@@ -42,8 +42,6 @@ def test_trajectory_matches_reference(name):
     import fastscapelib_fortran as fs
     g, L = _load(name)
     saved = sorted({int(k.split("_")[-1]) for k in g if not k.startswith("in_")})
-    if L["p"] >= -1.5:
-        pytest.skip("multiple flow direction is not built on the GPU yet")
     print(f"golden {name}: {L['nx']}x{L['ny']} ibc={L['ibc']} p={L['p']} n={L['n']} steps={saved}", flush=True)
     with fs.Batch([(L["nx"], L["ny"])]) as b:
         apply(b, L, 0)
