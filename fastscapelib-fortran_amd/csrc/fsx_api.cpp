@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -45,7 +46,9 @@ struct fsx_ctx {
   std::vector<int> steps;
   std::vector<double> xl, yl;
   std::vector<int> ibc;
-  std::vector<int> chunk_base;
+  std::vector<char> used;                    // slot holds a landscape
+  std::map<long long, long long> free_ranges;  // free node ranges of the arena (offset -> length)
+  long long capacity = 0, chunk_cap = 0;
   std::vector<void*> allocs;
   hipStream_t stream = nullptr;
   bool dirty = true;
@@ -149,7 +152,7 @@ static void geometry(InstParams& P, double xl, double yl) {
 
 static int check_inst(fsx_ctx* c, int inst) {
   if (!c) return fail(FSX_EINVAL, "null context");
-  if (inst < 0 || inst >= c->S.ninst) return fail(FSX_EINVAL, "instance id out of range");
+  if (inst < 0 || inst >= c->S.ninst || !c->used[inst]) return fail(FSX_EINVAL, "instance id out of range");
   return 0;
 }
 
@@ -178,56 +181,105 @@ static int upload_bc(fsx_ctx* c, int inst) {
   return 0;
 }
 
-extern "C" int fsx_create(int ninst, const int* nx, const int* ny, fsx_ctx** out) {
-  if (!out || ninst <= 0 || !nx || !ny) return fail(FSX_EINVAL, "fsx_create: bad arguments");
-  *out = nullptr;
+// ---- node arena and landscape slots ----------------------------------------------------------------
+// A context owns an arena of `capacity` nodes and `nslots` landscape slots.  A landscape occupies a
+// contiguous node range (first fit over the free ranges) and a slot; the chunk table, the chunk ranges
+// and the per-landscape auxiliary offsets are rebuilt whenever the set of landscapes changes.
+static long long arena_take(fsx_ctx* c, long long n) {
+  for (auto it = c->free_ranges.begin(); it != c->free_ranges.end(); ++it) {
+    if (it->second < n) continue;
+    const long long off = it->first, len = it->second;
+    c->free_ranges.erase(it);
+    if (len > n) c->free_ranges[off + n] = len - n;
+    return off;
+  }
+  return -1;
+}
+
+static void arena_give(fsx_ctx* c, long long off, long long n) {
+  auto it = c->free_ranges.emplace(off, n).first;
+  auto nx = std::next(it);
+  if (nx != c->free_ranges.end() && it->first + it->second == nx->first) {
+    it->second += nx->second;
+    c->free_ranges.erase(nx);
+  }
+  if (it != c->free_ranges.begin()) {
+    auto pv = std::prev(it);
+    if (pv->first + pv->second == it->first) {
+      pv->second += it->second;
+      c->free_ranges.erase(it);
+    }
+  }
+}
+
+static int relayout(fsx_ctx* c) {
+  DevState& S = c->S;
+  std::vector<Chunk> chunks;
+  std::vector<std::pair<int, int>> by_off;
+  for (int k = 0; k < S.ninst; k++)
+    if (c->used[k]) by_off.push_back({c->hp[k].off, k});
+  std::sort(by_off.begin(), by_off.end());
+  for (size_t r = 0; r < by_off.size(); r++) c->hp[by_off[r].second].aux = by_off[r].first + (int)r;
+  for (int k = 0; k < S.ninst; k++) {
+    InstParams& P = c->hp[k];
+    P.c0 = (int)chunks.size();
+    if (c->used[k])
+      for (int s = 0; s < P.nn; s += FSX_CHUNK) chunks.push_back(Chunk{k, s});
+    P.c1 = (int)chunks.size();
+  }
+  if ((long long)chunks.size() > c->chunk_cap) return fail(FSX_EINVAL, "chunk table overflow");
+  S.nchunks = (int)chunks.size();
+  if (!chunks.empty())
+    HIPCHK(hipMemcpyAsync(S.chunks, chunks.data(), sizeof(Chunk) * chunks.size(), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  S.max_nn = S.max_nx = S.max_ny = 0;
+  for (int k = 0; k < S.ninst; k++)
+    if (c->used[k]) {
+      S.max_nn = std::max(S.max_nn, c->hp[k].nn);
+      S.max_nx = std::max(S.max_nx, c->hp[k].nx);
+      S.max_ny = std::max(S.max_ny, c->hp[k].ny);
+    }
+  c->dirty = true;
+  return 0;
+}
+
+// State of a new landscape, SetUp defaults (src/FastScape_ctx.f90:81-97): h = b = u = etot = erate =
+// 0, precip = 1, kf = kd = 0; everything else is rewritten by each step.
+__global__ void k_slot_init(long long off, int nn, double* h, double* b, double* u, double* etot, double* erate,
+                            double* area, double* lake_depth, double* sedflux, double* hwater, double* kf,
+                            double* kd, double* g, double* precip) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += gridDim.x * blockDim.x) {
+    const long long q = off + i;
+    h[q] = 0.0; b[q] = 0.0; u[q] = 0.0; etot[q] = 0.0; erate[q] = 0.0; area[q] = 0.0; lake_depth[q] = 0.0;
+    sedflux[q] = 0.0; hwater[q] = 0.0; kf[q] = 0.0; kd[q] = 0.0; g[q] = 0.0; precip[q] = 1.0;
+  }
+}
+
+static int ctx_alloc(long long capacity, int nslots, fsx_ctx** out) {
+  if (capacity <= 0 || nslots <= 0) return fail(FSX_EINVAL, "empty context");
+  if (capacity > 300000000LL) return fail(FSX_EINVAL, "fsx_create: batch too large");
   fsx_ctx* c = new fsx_ctx();
   DevState& S = c->S;
-  S.ninst = ninst;
-  c->hp.resize(ninst);
-  c->steps.assign(ninst, 0);
-  c->xl.assign(ninst, 0.0);
-  c->yl.assign(ninst, 0.0);
-  c->ibc.assign(ninst, 1111);
-  long long off = 0;
-  std::vector<Chunk> chunks;
-  c->chunk_base.push_back(0);
-  for (int k = 0; k < ninst; k++) {
-    if (nx[k] < 3 || ny[k] < 3) {
-      delete c;
-      return fail(FSX_EINVAL, "fsx_create: nx and ny must be >= 3");
-    }
-    InstParams& P = c->hp[k];
-    memset(&P, 0, sizeof P);
-    P.nx = nx[k];
-    P.ny = ny[k];
-    P.nn = nx[k] * ny[k];
-    P.off = (int)off;
-    P.active = 1;
-    P.newton_tol = 1.0e-3;
-    P.p = 1.0;
-    decode_bc(P, 1111);
-    geometry(P, (double)(P.nx - 1), (double)(P.ny - 1));
-    for (int s = 0; s < P.nn; s += FSX_CHUNK) chunks.push_back(Chunk{k, s});
-    c->chunk_base.push_back((int)chunks.size());
-    off += P.nn;
-    S.max_nn = std::max(S.max_nn, P.nn);
-    S.max_nx = std::max(S.max_nx, P.nx);
-    S.max_ny = std::max(S.max_ny, P.ny);
-  }
-  if (off > 300000000LL) {
-    delete c;
-    return fail(FSX_EINVAL, "fsx_create: batch too large");
-  }
-  S.ntot = off;
-  S.nchunks = (int)chunks.size();
-  const long long N = off;
+  S.ninst = nslots;
+  c->hp.resize(nslots);
+  for (InstParams& P : c->hp) memset(&P, 0, sizeof P);
+  c->used.assign(nslots, 0);
+  c->steps.assign(nslots, 0);
+  c->xl.assign(nslots, 0.0);
+  c->yl.assign(nslots, 0.0);
+  c->ibc.assign(nslots, 1111);
+  c->capacity = capacity;
+  c->free_ranges[0] = capacity;
+  c->chunk_cap = capacity / FSX_CHUNK + nslots + 1;
+  S.ntot = capacity;
+  const long long N = capacity;
+  const int ninst = nslots;
   int rc = 0;
   if (configure_sweeps() != 0) { delete c; return fail(FSX_EHIP, "cannot configure LDS of the sweep kernels"); }
   hipError_t he = hipStreamCreate(&c->stream);
   if (he != hipSuccess) { delete c; return fail(FSX_EHIP, "hipStreamCreate failed"); }
 #define A(p, n) if (!rc) rc = dalloc(c, &S.p, (n))
-  A(P, ninst); A(chunks, S.nchunks);
+  A(P, ninst); A(chunks, c->chunk_cap);
   A(h, N); A(b, N); A(u, N); A(kf, N); A(kd, N); A(precip, N); A(etot, N); A(erate, N); A(area, N);
   A(length, N); A(hwater, N); A(lake_depth, N); A(sedflux, N); A(ht, N); A(water, N); A(zint, N); A(zintp, N);
   A(gam, N); A(dh, N); A(hp, N); A(lwv, N); A(lsed, N); A(kfint, N); A(kdint, N); A(g, N); A(pexp, N); A(elev, N);
@@ -240,25 +292,107 @@ extern "C" int fsx_create(int ninst, const int* nx, const int* ny, fsx_ctx** out
   A(segs, FSX_EDGE_CAP * N); A(cw, FSX_EDGE_CAP * N); A(skey, FSX_EDGE_CAP * N); A(intree, FSX_EDGE_CAP * N);
   A(sB, FSX_SLAB_CAP * N); A(sp, FSX_SLAB_CAP * N); A(sq, FSX_SLAB_CAP * N); A(sw, FSX_SLAB_CAP * N);
   A(nbasins, ninst); A(basin0, ninst); A(nconn, ninst); A(nlev, ninst); A(err, ninst); A(ngs, ninst);
-  A(ntree, ninst); A(chunkcnt, S.nchunks); A(newton, ninst); A(scal, ninst); A(flags, 64); A(cbase, ninst + 1);
+  A(ntree, ninst); A(chunkcnt, c->chunk_cap); A(newton, ninst); A(scal, ninst); A(flags, 64);
   A(nsegs, ninst); A(segctr, ninst); A(maxd, ninst);
   A(J, N); A(ngroups, ninst); A(grec, 2 * (N + ninst)); A(first, N); A(next, N); A(gmeta, N + ninst); A(pos, N); A(lcur, N + ninst); A(rp, N); A(LSl, N); A(dn, 8 * N);
   A(A0, N); A(Al, N); A(F, N); A(HTl, N); A(Hl, N); A(Wl, N); A(HWl, N); A(HPl, N); A(DHl, N); A(ELl, N);
 #undef A
   if (rc) { fsx_destroy(c); return rc; }
-  he = hipMemcpy(S.chunks, chunks.data(), sizeof(Chunk) * chunks.size(), hipMemcpyHostToDevice);
-  if (he == hipSuccess)
-    he = hipMemcpy(S.cbase, c->chunk_base.data(), sizeof(int) * c->chunk_base.size(), hipMemcpyHostToDevice);
-  if (he != hipSuccess) { fsx_destroy(c); return fail(FSX_EHIP, "upload of the chunk table failed"); }
-  // SetUp defaults (src/FastScape_ctx.f90:81-97): precip = 1, BC 1111
-  std::vector<double> ones((size_t)S.max_nn, 1.0);
-  for (int k = 0; k < ninst; k++) {
-    const InstParams& P = c->hp[k];
-    he = hipMemcpy(S.precip + P.off, ones.data(), sizeof(double) * P.nn, hipMemcpyHostToDevice);
-    if (he != hipSuccess) { fsx_destroy(c); return fail(FSX_EHIP, "precip upload failed"); }
-    if ((rc = upload_bc(c, k))) { fsx_destroy(c); return rc; }
-  }
   *out = c;
+  return 0;
+}
+
+static int slot_add(fsx_ctx* c, int nx, int ny, int* inst, bool relay) {
+  if (nx < 3 || ny < 3) return fail(FSX_EINVAL, "nx and ny must be >= 3");
+  const long long nn = (long long)nx * ny;
+  int k = -1;
+  for (int s = 0; s < c->S.ninst; s++)
+    if (!c->used[s]) { k = s; break; }
+  if (k < 0) return fail(FSX_ENOMEM, "no free landscape slot");
+  const long long off = arena_take(c, nn);
+  if (off < 0) return fail(FSX_ENOMEM, "no free node range of that size");
+  InstParams& P = c->hp[k];
+  memset(&P, 0, sizeof P);
+  P.nx = nx;
+  P.ny = ny;
+  P.nn = (int)nn;
+  P.off = (int)off;
+  P.active = 1;
+  P.newton_tol = 1.0e-3;
+  P.p = 1.0;
+  P.sfd = 1;
+  decode_bc(P, 1111);
+  geometry(P, (double)(P.nx - 1), (double)(P.ny - 1));
+  c->used[k] = 1;
+  c->steps[k] = 0;
+  c->xl[k] = c->yl[k] = 0.0;
+  c->ibc[k] = 1111;
+  DevState& S = c->S;
+  k_slot_init<<<(int)std::min<long long>(1024, (nn + 255) / 256), 256, 0, c->stream>>>(
+      off, (int)nn, S.h, S.b, S.u, S.etot, S.erate, S.area, S.lake_depth, S.sedflux, S.hwater, S.kf, S.kd, S.g,
+      S.precip);
+  HIPCHK(hipGetLastError());
+  if (int r = upload_bc(c, k)) return r;
+  if (relay)
+    if (int r = relayout(c)) return r;
+  *inst = k;
+  return 0;
+}
+
+extern "C" int fsx_create(int ninst, const int* nx, const int* ny, fsx_ctx** out) {
+  if (!out || ninst <= 0 || !nx || !ny) return fail(FSX_EINVAL, "fsx_create: bad arguments");
+  *out = nullptr;
+  long long total = 0;
+  for (int k = 0; k < ninst; k++) {
+    if (nx[k] < 3 || ny[k] < 3) return fail(FSX_EINVAL, "fsx_create: nx and ny must be >= 3");
+    total += (long long)nx[k] * ny[k];
+  }
+  fsx_ctx* c = nullptr;
+  if (int r = ctx_alloc(total, ninst, &c)) return r;
+  for (int k = 0; k < ninst; k++) {
+    int id = -1;
+    if (int r = slot_add(c, nx[k], ny[k], &id, false)) { fsx_destroy(c); return r; }
+  }
+  if (int r = relayout(c)) { fsx_destroy(c); return r; }
+  *out = c;
+  return 0;
+}
+
+extern "C" int fsx_create_pool(long long capacity_nodes, int nslots, fsx_ctx** out) {
+  if (!out) return fail(FSX_EINVAL, "fsx_create_pool: null output");
+  *out = nullptr;
+  fsx_ctx* c = nullptr;
+  if (int r = ctx_alloc(capacity_nodes, nslots, &c)) return r;
+  if (int r = relayout(c)) { fsx_destroy(c); return r; }
+  *out = c;
+  return 0;
+}
+
+extern "C" int fsx_add_instance(fsx_ctx* c, int nx, int ny, int* inst) {
+  if (!c || !inst) return fail(FSX_EINVAL, "fsx_add_instance: bad arguments");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return slot_add(c, nx, ny, inst, true);
+}
+
+extern "C" int fsx_remove_instance(fsx_ctx* c, int inst) {
+  if (int r = check_inst(c, inst)) return r;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  InstParams& P = c->hp[inst];
+  arena_give(c, P.off, P.nn);
+  memset(&P, 0, sizeof P);
+  c->used[inst] = 0;
+  return relayout(c);
+}
+
+extern "C" int fsx_pool_info(fsx_ctx* c, long long* free_nodes, long long* largest_free, int* free_slots) {
+  if (!c) return fail(FSX_EINVAL, "null context");
+  long long tot = 0, big = 0;
+  for (auto& fr : c->free_ranges) { tot += fr.second; big = std::max(big, fr.second); }
+  int fs = 0;
+  for (int s = 0; s < c->S.ninst; s++) fs += !c->used[s];
+  if (free_nodes) *free_nodes = tot;
+  if (largest_free) *largest_free = big;
+  if (free_slots) *free_slots = fs;
   return 0;
 }
 

@@ -49,13 +49,12 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_root_count(const InstParams* __re
 
 // exclusive scan of the per-chunk root counts of each landscape; nbasins; basin0 reset
 __global__ __launch_bounds__(FSX_WG) void k_root_scan(const InstParams* __restrict__ PP, int* __restrict__ chunkcnt,
-                                                      const int* __restrict__ cbase, int* __restrict__ nbasins,
-                                                      int* __restrict__ basin0) {
+                                                      int* __restrict__ nbasins, int* __restrict__ basin0) {
   __shared__ int lds[FSX_WG / 64];
   const int inst = blockIdx.x;
   const InstParams& P = PP[inst];
   if (!P.active || !P.run_spl) return;
-  const int c0 = cbase[inst], c1 = cbase[inst + 1];
+  const int c0 = P.c0, c1 = P.c1;
   int carry = 0;
   for (int b = c0; b < c1; b += FSX_WG) {
     const int i = b + threadIdx.x;
@@ -751,7 +750,7 @@ void launch_lm_labels(DevState& S, hipStream_t st) {
 void launch_lm_basins(DevState& S, hipStream_t st) {
   const int nb = S.nchunks;
   k_root_count<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.rec, S.chunkcnt);
-  k_root_scan<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.chunkcnt, S.cbase, S.nbasins, S.basin0);
+  k_root_scan<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.chunkcnt, S.nbasins, S.basin0);
   k_root_rank<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.rec, S.bc, S.chunkcnt, S.rootrank, S.outlet, S.basin0);
   launch_fill_int(S.bsize, S.ntot, 0, st);
   k_basin_label<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.J, S.rootrank, S.bc, S.basin, S.bsize);

@@ -35,7 +35,8 @@ struct InstParams {
   int fix_left, fix_right, fix_bottom, fix_top;  // BC digits == '1'
   int run_uplift, run_spl, run_diff, sfd;
   int active;
-  int pad0;
+  int aux;                   // offset of the nn+1-entry per-landscape arrays (level starts, groups)
+  int c0, c1;                // chunk-table range of the landscape
   double dx, dy, dt;
   double len[9];             // neighbour distance by direction (ii+1)+3*(jj+1), host-computed
   double dxdy;               // dx*dy

@@ -42,7 +42,7 @@ __global__ __launch_bounds__(FSX_WG) void k_mfd_fill(const InstParams* __restric
   const InstParams& P = PP[inst];
   if (!P.active || !P.run_spl || P.sfd) return;
   const long long off = P.off;
-  const int* ls = lstart + off + inst;
+  const int* ls = lstart + P.aux;
   const int L = nlev[inst];
   for (int l = 0; l < L; l++) {
     const int b0 = ls[l], b1 = ls[l + 1];
@@ -170,7 +170,7 @@ __global__ __launch_bounds__(FSX_WG) void k_mfd_levels(const InstParams* __restr
   if (!P.active || !P.run_spl || P.sfd) return;
   const int nn = P.nn, tid = threadIdx.x;
   const long long off = P.off;
-  int* ls = mls + off + inst;  // nn+1 entries
+  int* ls = mls + P.aux;  // nn+1 entries
   if (tid == 0) s_tail = 0;
   __syncthreads();
   for (int v = tid; v < nn; v += FSX_WG)
@@ -215,7 +215,7 @@ __global__ __launch_bounds__(FSX_WG) void k_mfd_area(const InstParams* __restric
   const InstParams& P = PP[inst];
   if (!P.active || !P.run_spl || P.sfd) return;
   const long long off = P.off;
-  const int* ls = mls + off + inst;
+  const int* ls = mls + P.aux;
   const int L = mnlev[inst];
   for (int l = 0; l < L; l++) {
     const int b0 = ls[l], b1 = ls[l + 1];
@@ -285,7 +285,7 @@ __global__ __launch_bounds__(FSX_WG) void k_mfd_spl(
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int nn = P.nn;
   const long long off = P.off;
-  const int* ls = mls + off + inst;
+  const int* ls = mls + P.aux;
   const int L = mnlev[inst];
   const double n = P.n;
   const bool n_is_one = fabs(n - 1.0) < 2.2250738585072014e-308;

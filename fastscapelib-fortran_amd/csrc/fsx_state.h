@@ -61,6 +61,5 @@ struct DevState {
   long long *newton = nullptr;
   double *scal = nullptr;      // per-instance scalar scratch (tolerances, GS error)
   int *flags = nullptr;        // convergence flags of iterative kernels (64)
-  int *cbase = nullptr;        // first chunk of each instance [ninst+1]
   int *nsegs = nullptr, *segctr = nullptr;  // edge-sort segment lists [ninst]
 };

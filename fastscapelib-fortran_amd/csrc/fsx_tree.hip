@@ -85,8 +85,8 @@ __global__ __launch_bounds__(FSX_WG) void k_level_sort(const InstParams* __restr
   if (!P.active || !P.run_spl) return;
   const int nn = P.nn, tid = threadIdx.x;
   const long long off = P.off;
-  int* ls = lstart + off + inst;  // nn+1 entries
-  int* lc = lcur + off + inst;
+  int* ls = lstart + P.aux;  // nn+1 entries
+  int* lc = lcur + P.aux;
   if (tid == 0) {
     s_max = 0;
     if (flags[25]) atomicOr(err + inst, FSX_ERR_LEVELS);  // jumping did not converge: receiver cycle
@@ -142,7 +142,7 @@ __global__ __launch_bounds__(FSX_WG) void k_level_sort(const InstParams* __restr
   __syncthreads();
   // level groups: a new group starts at level 0, at or after a wide level, and where the start
   // position crosses a multiple of GRP_HALF
-  int* gm = gmeta + off + inst;  // <= L+1 entries
+  int* gm = gmeta + P.aux;  // <= L+1 entries
   int gcarry = 0;
   for (int s0 = 0; s0 < L; s0 += FSX_WG) {
     const int l = s0 + tid;
@@ -167,7 +167,7 @@ __global__ __launch_bounds__(FSX_WG) void k_level_sort(const InstParams* __restr
   __syncthreads();
   // per-group records (lo, hi, p0, p1) + (pm, p2, -, -): level range, position range, start of the
   // level below and end of the level above
-  int4* gr = grec + 2 * (off + inst);
+  int4* gr = grec + 2 * (long long)P.aux;
   for (int g = tid; g < gcarry; g += FSX_WG) {
     const int lo = gm[g], hi = gm[g + 1];
     const int p0 = ls[lo], p1 = ls[hi];
@@ -192,7 +192,7 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_area_prep(const InstParams* __res
   A0[gp] = P.dxdy * precip[g_];
   int ds[8];
   const int nd = donors_sorted(P, v, dmask[g_], ds);
-  const int base = lstart[P.off + c.inst + d + 1];
+  const int base = lstart[P.aux + d + 1];
   int* out = dn + 8 * gp;
 #pragma unroll
   for (int k = 0; k < 8; k++) out[k] = k < nd ? pos[P.off + ds[nd - 1 - k]] - base : -1;
@@ -220,7 +220,7 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_spl_fact(const InstParams* __rest
     F[gp] = 0.0;
     return;
   }
-  rp[gp] = pos[P.off + r] - lstart[P.off + c.inst + d - 1];
+  rp[gp] = pos[P.off + r] - lstart[P.aux + d - 1];
   const double am = pow(area[g_], P.m);
   const double n = P.n;
   if (fabs(n - 1.0) < 2.2250738585072014e-308) {
@@ -278,8 +278,8 @@ __global__ __launch_bounds__(FSX_WG) void k_area_sweep(const InstParams* __restr
   if (!P.active || !P.run_spl || !P.sfd) return;
   const int tid = threadIdx.x;
   const long long off = P.off;
-  const int* ls = lstart + off + inst;
-  const int4* gr = grec + 2 * (off + inst);
+  const int* ls = lstart + P.aux;
+  const int4* gr = grec + 2 * (long long)P.aux;
   const int G = ngroups[inst];
   auto meta = [&](int g) {
     GroupMeta m;
@@ -428,8 +428,8 @@ __global__ __launch_bounds__(FSX_WG) void k_spl_sweep(
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int nn = P.nn;
   const long long off = P.off;
-  const int* ls = lstart + off + inst;
-  const int4* gr = grec + 2 * (off + inst);
+  const int* ls = lstart + P.aux;
+  const int4* gr = grec + 2 * (long long)P.aux;
   const int L = nlev[inst];
   const int G = ngroups[inst];
   const double n = P.n;

@@ -33,7 +33,7 @@ __all__ = [
     "fastscape_copy_erosion_rate", "fastscape_copy_drainage_area", "fastscape_copy_lake_depth",
     "fastscape_get_step", "fastscape_get_sizes", "fastscape_destroy", "fastscape_set_h", "fastscape_set_basement",
     "fastscape_set_precip", "fastscape_set_all_layers", "fastscape_reset_cumulative_erosion", "fastscape_view",
-    "fastscape_debug", "Batch", "FastscapeError", "indexx",
+    "fastscape_debug", "Batch", "Pool", "FastscapeError", "indexx",
 ]
 
 
@@ -55,8 +55,8 @@ class Batch:
         h = C.c_void_p()
         check(lib.fsx_create(n, nx, ny, C.byref(h)), "fsx_create")
         self._h = h
-        self.sizes = sizes
-        self.nn = [a * b for a, b in sizes]
+        self.sizes = dict(enumerate(sizes))
+        self.nn = {k: a * b for k, (a, b) in self.sizes.items()}
 
     # -- lifetime --
     def close(self):
@@ -201,6 +201,43 @@ class Batch:
         a, b = C.c_longlong(), C.c_longlong()
         check(lib.fsx_memory(self._h, C.byref(a), C.byref(b)), "memory")
         return a.value, b.value
+
+
+class Pool(Batch):
+    """A Batch whose landscapes come and go: an arena of ``capacity_nodes`` nodes and ``slots``
+    landscape slots in HBM.  ``add(nx, ny)`` places a landscape (SetUp defaults) and returns its id,
+    or None when no slot or contiguous node range is free; ``remove(id)`` frees it.  ``step`` advances
+    every landscape currently held."""
+
+    def __init__(self, capacity_nodes, slots=256):
+        h = C.c_void_p()
+        check(lib.fsx_create_pool(int(capacity_nodes), int(slots), C.byref(h)), "fsx_create_pool")
+        self._h = h
+        self.sizes = {}
+        self.nn = {}
+        self.capacity = int(capacity_nodes)
+        self.slots = int(slots)
+
+    def add(self, nx, ny):
+        k = C.c_int(-1)
+        rc = lib.fsx_add_instance(self._h, int(nx), int(ny), C.byref(k))
+        if rc == _lib.FSX_ENOMEM:
+            return None
+        check(rc, "add_instance")
+        self.sizes[k.value] = (int(nx), int(ny))
+        self.nn[k.value] = int(nx) * int(ny)
+        return k.value
+
+    def remove(self, k):
+        check(lib.fsx_remove_instance(self._h, int(k)), "remove_instance")
+        del self.sizes[k]
+        del self.nn[k]
+
+    def info(self):
+        """(free nodes, largest free contiguous range, free slots)."""
+        a, b, c = C.c_longlong(), C.c_longlong(), C.c_int()
+        check(lib.fsx_pool_info(self._h, C.byref(a), C.byref(b), C.byref(c)), "pool_info")
+        return a.value, b.value, c.value
 
 
 # ---- the f2py-style module state (one landscape, as the reference's FastScapeContext) ----------

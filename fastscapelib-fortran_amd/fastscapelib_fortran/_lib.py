@@ -27,6 +27,9 @@ _ip = C.POINTER(C.c_int)
 _vp = C.c_void_p
 
 
+FSX_OK, FSX_EINVAL, FSX_EHIP, FSX_ESTEP, FSX_ENOMEM = 0, -1, -2, -3, -4
+
+
 class FastscapeError(RuntimeError):
     """A C-ABI call failed; the message is the library's last error."""
 
@@ -63,6 +66,10 @@ def _load():
         "fsx_set_profiling": [_vp, C.c_int],
         "fsx_get_timers": [_vp, C.POINTER(C.c_double), C.POINTER(C.c_longlong), C.c_int],
         "fsx_reset_timers": [_vp],
+        "fsx_create_pool": [C.c_longlong, C.c_int, C.POINTER(_vp)],
+        "fsx_add_instance": [_vp, C.c_int, C.c_int, _ip],
+        "fsx_remove_instance": [_vp, C.c_int],
+        "fsx_pool_info": [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(C.c_int)],
     }
     for name, args in sig.items():
         f = getattr(lib, name)

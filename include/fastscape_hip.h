@@ -71,6 +71,18 @@ int fsx_create(int ninst, const int *nx, const int *ny, fsx_ctx **out);
 /* FastScape_Destroy (src/FastScape_api.f90:174-184) */
 int fsx_destroy(fsx_ctx *ctx);
 
+/* Landscape pools for serving.  The reference holds one landscape per process and allocates it in
+ * SetUp / frees it in Destroy (src/FastScape_ctx.f90:57-113,117-154); a pool generalises that
+ * lifecycle to landscapes that come and go while others step.  fsx_create_pool reserves an arena of
+ * capacity_nodes nodes and nslots landscape slots in HBM; fsx_add_instance places an nx x ny landscape
+ * in the first free node range that fits (SetUp defaults as fsx_create; FSX_ENOMEM when no slot or
+ * range is free) and returns its id; fsx_remove_instance frees it.  Ids of other landscapes do not
+ * change.  fsx_pool_info reports free nodes, the largest free range and free slots. */
+int fsx_create_pool(long long capacity_nodes, int nslots, fsx_ctx **out);
+int fsx_add_instance(fsx_ctx *ctx, int nx, int ny, int *inst);
+int fsx_remove_instance(fsx_ctx *ctx, int inst);
+int fsx_pool_info(fsx_ctx *ctx, long long *free_nodes, long long *largest_free, int *free_slots);
+
 /* FastScape_Set_XL_YL (src/FastScape_api.f90:491-503) */
 int fsx_set_xl_yl(fsx_ctx *ctx, int inst, double xl, double yl);
 /* FastScape_Set_DT (src/FastScape_api.f90:507-519) */

@@ -1,0 +1,104 @@
+"""Host logic of the continuous-batching server, on a stand-in pool (no GPU): admission by free
+slots and contiguous node ranges, backfill, quanta never overshooting a run's step count,
+retirement, rejection of runs that can never fit, latency bookkeeping."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+
+class FakePool:
+    """Arena of `capacity` nodes with first-fit ranges, like the library's pool."""
+
+    def __init__(self, capacity, slots):
+        self.capacity, self.slots = capacity, slots
+        self.nn, self.ranges, self.steps = {}, {}, {}
+        self.log = []
+
+    def add(self, nx, ny):
+        n = nx * ny
+        if len(self.nn) >= self.slots:
+            return None
+        used = sorted(self.ranges.values())
+        off = 0
+        for a, b in used:
+            if a - off >= n:
+                break
+            off = b
+        if off + n > self.capacity:
+            return None
+        k = next(s for s in range(self.slots) if s not in self.nn)
+        self.nn[k], self.ranges[k], self.steps[k] = n, (off, off + n), 0
+        return k
+
+    def remove(self, k):
+        del self.nn[k], self.ranges[k], self.steps[k]
+
+    def step(self, q):
+        assert q > 0
+        for k in self.nn:
+            self.steps[k] += q
+        self.log.append((q, sorted(self.nn)))
+
+    def field(self, k, f):
+        return np.full(self.nn[k], float(self.steps[k]))
+
+
+@pytest.fixture
+def serving():
+    from fastscapelib_fortran import serving
+    return serving
+
+
+def _runs(serving, spec):
+    return [serving.Run(rid=i, nx=nx, ny=ny, steps=st, setup=lambda p, k: None, arrival=t)
+            for i, (nx, ny, st, t) in enumerate(spec)]
+
+
+def test_every_run_gets_exactly_its_steps(serving):
+    pool = FakePool(capacity=10_000, slots=4)
+    srv = serving.Server(pool, quantum=5)
+    runs = _runs(serving, [(50, 50, 13, 0), (40, 40, 7, 0), (30, 30, 22, 0), (60, 60, 3, 0), (20, 20, 9, 0),
+                           (70, 70, 11, 0)])
+    rep = srv.serve(runs, realtime=False)
+    assert rep["runs"] == 6 and rep["rejected"] == 0
+    for r in runs:
+        assert r.result["h"][0] == r.steps, r.rid  # stepped exactly `steps` times while resident
+    assert rep["cell_steps"] == sum(r.nx * r.ny * r.steps for r in runs)
+    assert all(q <= 5 for q, _ in pool.log)
+
+
+def test_admission_respects_memory_and_backfills(serving):
+    pool = FakePool(capacity=5000, slots=8)
+    srv = serving.Server(pool, quantum=4)
+    # 60x60 = 3600 fits only alone; 30x30 = 900 runs fit beside it
+    runs = _runs(serving, [(60, 60, 8, 0), (60, 60, 8, 0), (30, 30, 4, 0)])
+    srv.serve(runs, realtime=False)
+    first = pool.log[0][1]
+    assert first == sorted([runs[0].slot, runs[2].slot])  # the small run went ahead of the second big one
+    assert runs[2].admitted < runs[1].admitted
+    assert all(sum(pool.nn.get(k, 0) for k in ks) <= 5000 for _, ks in pool.log)
+    assert runs[1].admitted >= runs[0].done
+
+
+def test_never_fitting_run_is_rejected(serving):
+    pool = FakePool(capacity=1000, slots=2)
+    srv = serving.Server(pool, quantum=4)
+    runs = _runs(serving, [(100, 100, 5, 0), (20, 20, 5, 0)])
+    rep = srv.serve(runs, realtime=False)
+    assert rep["rejected"] == 1 and rep["runs"] == 1
+
+
+def test_latency_is_completion_minus_arrival(serving):
+    t = [0.0]
+
+    def clock():
+        t[0] += 0.001
+        return t[0]
+    pool = FakePool(capacity=10_000, slots=4)
+    srv = serving.Server(pool, quantum=3, clock=clock)
+    runs = _runs(serving, [(10, 10, 6, 0), (10, 10, 9, 0)])
+    rep = srv.serve(runs, realtime=False)
+    for r in runs:
+        assert r.done > r.admitted >= r.arrival and r.latency == r.done - r.arrival
+    assert rep["p99_latency_s"] >= rep["p50_latency_s"] > 0
