@@ -8,6 +8,16 @@ The default workload is the uniform batch: 64 independent 513x513 landscapes (dx
 fixed boundary, m = 0.4, n = 1, kd = 1e-2, dt = 1e3 yr.  The same landscape run alone gives the plain
 one-at-a-time figure.  Synthetic, seeded inputs; FP64 throughout.
 
+Further legs (each reported beside the headline value):
+* ragged: 200 landscapes, nx and ny log-uniform in [65, 1025], dx = 250 m, h = U[0,1) + a 10 m 2-D
+  sinusoid of wavelength U[5, 50] km, n from {1, 1.5, 2}, m = 0.45 n, kf = 10**U(-6, -5),
+  kd ~ U(1e-3, 1e-1), uplift 1e-3 m/yr off the fixed boundary, BC 1010 (even ids) / 1111 (odd),
+  dt = 2e3 yr, Newton tolerance 1e-8; batched vs the same landscapes one at a time (a sample);
+* serving: a Poisson stream of runs (20 runs/s) with grids log-uniform in [65, 513], step counts
+  log-uniform in [20, 200], dt log-uniform in [1e3, 1e4], D8 or multiple flow direction (p = 1)
+  50/50, other parameters as the ragged leg; served by the continuous-batching loop on a landscape
+  pool; p50/p99 completion latency, and the same runs all submitted at once for the saturated rate.
+
 Prints ONE JSON line: metric, value (batched cell-steps/s), plain rate and the batched/plain ratio,
 the roofline of the dominant kernel (HIP events recorded around it on the stream it runs on during
 the timed region), and the reference Fortran timed on this host's cores on a bounded sample.
@@ -64,15 +74,91 @@ def uniform_batch(ninst=64, n=513, dx=200.0):
     return out
 
 
+def _fixed_mask(nx, ny, ibc):
+    """Nodes on a fixed (base-level) boundary for a BC code (digits bottom/right/top/left)."""
+    c = f"{ibc:4d}"
+    e = np.zeros((ny, nx), bool)
+    if c[0] == "1":
+        e[0, :] = True
+    if c[1] == "1":
+        e[:, -1] = True
+    if c[2] == "1":
+        e[-1, :] = True
+    if c[3] == "1":
+        e[:, 0] = True
+    return e.ravel()
+
+
+def _sinus_landscape(rng, nx, ny, dx, amp=10.0):
+    lam = rng.uniform(5e3, 5e4)
+    h = rng.random(nx * ny)
+    x = np.arange(nx) * dx
+    y = np.arange(ny) * dx
+    return h + amp * (np.sin(2 * np.pi * x[None, :] / lam) * np.sin(2 * np.pi * y[:, None] / lam)).ravel()
+
+
+def ragged_batch(ninst=200, lo=65, hi=1025, seed0=1000):
+    """Ragged batch recipe; per landscape from PCG64(seed0 + id): nx, ny, h noise + sinusoid, then
+    n, kf, kd."""
+    out = []
+    for i in range(ninst):
+        rng = np.random.default_rng(seed0 + i)
+        nx = int(round(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
+        ny = int(round(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
+        dx = 250.0
+        h = _sinus_landscape(rng, nx, ny, dx)
+        n = float(rng.choice([1.0, 1.5, 2.0]))
+        kf = 10.0 ** rng.uniform(-6.0, -5.0)
+        kd = rng.uniform(1e-3, 1e-1)
+        ibc = 1010 if i % 2 == 0 else 1111
+        u = np.full(nx * ny, 1e-3)
+        u[_fixed_mask(nx, ny, ibc)] = 0.0
+        out.append(dict(nx=nx, ny=ny, xl=dx * (nx - 1), yl=dx * (ny - 1), h=h, u=u, kf=kf, m=0.45 * n, n=n, kd=kd,
+                        dt=2e3, ibc=ibc, newton_tol=1e-8))
+    return out
+
+
+def serving_stream(rate=20.0, duration=8.0, lo=65, hi=513, smin=20, smax=200, seed=7):
+    """Arrivals of the serving leg: Poisson(rate) over `duration` seconds; per run grid, steps, dt,
+    D8 or multiple flow direction, parameters as the ragged recipe (from PCG64(seed), then per run
+    PCG64(seed*100000 + id))."""
+    rng = np.random.default_rng(seed)
+    t, runs = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > duration:
+            break
+        i = len(runs)
+        r = np.random.default_rng(seed * 100000 + i)
+        nx = int(round(np.exp(r.uniform(np.log(lo), np.log(hi)))))
+        ny = int(round(np.exp(r.uniform(np.log(lo), np.log(hi)))))
+        steps = int(round(np.exp(r.uniform(np.log(smin), np.log(smax)))))
+        dt = float(np.exp(r.uniform(np.log(1e3), np.log(1e4))))
+        dx = 250.0
+        h = _sinus_landscape(r, nx, ny, dx)
+        n = float(r.choice([1.0, 1.5, 2.0]))
+        kf = 10.0 ** r.uniform(-6.0, -5.0)
+        kd = r.uniform(1e-3, 1e-1)
+        mfd = bool(r.random() < 0.5)
+        ibc = 1010 if i % 2 == 0 else 1111
+        u = np.full(nx * ny, 1e-3)
+        u[_fixed_mask(nx, ny, ibc)] = 0.0
+        runs.append((t, steps, dict(nx=nx, ny=ny, xl=dx * (nx - 1), yl=dx * (ny - 1), h=h, u=u, kf=kf, m=0.45 * n,
+                                    n=n, kd=kd, dt=dt, ibc=ibc, newton_tol=1e-8, p=1.0 if mfd else -2.0)))
+    return runs
+
+
 def configure(obj, L, *k):
     obj.set_xl_yl(*k, L["xl"], L["yl"])
     obj.set_dt(*k, L["dt"])
     obj.init_h(*k, L["h"])
     nn = L["nx"] * L["ny"]
     obj.set_erosional_parameters(*k, np.full(nn, L["kf"]), -1.0, L["m"], L["n"], np.full(nn, L["kd"]), -1.0, 0.0,
-                                 0.0, -2.0)
+                                 0.0, L.get("p", -2.0))
     obj.set_u(*k, L["u"])
     obj.set_bc(*k, L["ibc"])
+    if "newton_tol" in L and hasattr(obj, "set_newton_tol"):
+        obj.set_newton_tol(*k, L["newton_tol"])
 
 
 def _sync():
@@ -102,6 +188,58 @@ def run_gpu(fs, Ls, steps, warmup, profile):
         stats = [b.stats(k) for k in range(len(Ls))]
         mem = b.memory()
     return dt, timers, stats, mem
+
+
+def plain_rate(fs, Ls, steps):
+    """One landscape at a time (each after one untimed step): cell-steps/s over the sample."""
+    cells, secs = 0, 0.0
+    for L in Ls:
+        dt, _, _, _ = run_gpu(fs, [L], steps, 1, profile=False)
+        cells += L["nx"] * L["ny"] * steps
+        secs += dt
+    return cells / secs
+
+
+def ragged_leg(fs, steps, sample):
+    Ls = ragged_batch()
+    ncells = sum(L["nx"] * L["ny"] for L in Ls)
+    dt, _, stats, mem = run_gpu(fs, Ls, steps, 1, profile=False)
+    batched = ncells * steps / dt
+    pick = Ls[::max(1, len(Ls) // sample)][:sample]
+    plain = plain_rate(fs, pick, max(2, steps // 2))
+    return {"workload": "ragged batch 200 x log-uniform[65,1025]^2, n in {1,1.5,2}, Newton tol 1e-8, BC 1010/1111",
+            "landscapes": len(Ls), "cells": ncells, "steps": steps, "ms_per_step": dt / steps * 1e3,
+            "value": batched, "plain": plain, "plain_sample": f"{len(pick)} landscapes of the batch, one at a time",
+            "batched_over_plain": batched / plain, "hbm_bytes_resident": mem[0],
+            "newton_iters_per_step": float(np.sum([s["newton_iters"] for s in stats]))}
+
+
+def serving_leg(fs, quantum, duration, sample):
+    from fastscapelib_fortran import serving
+    stream = serving_stream(duration=duration)
+    cap, per_node = serving.pool_capacity(0.5)
+    need = sum(L["nx"] * L["ny"] for _, _, L in stream)
+    cap = int(min(cap, max(need, 1)))
+
+    def make(realtime):
+        return [serving.Run(rid=i, nx=L["nx"], ny=L["ny"], steps=st, arrival=t,
+                            setup=(lambda p, k, L=L: configure(p, L, k))) for i, (t, st, L) in enumerate(stream)]
+    out = {"workload": f"Poisson 20 runs/s for {duration:g} s, grids log-uniform[65,513]^2, steps log-uniform[20,200],"
+                       " D8/MFD 50/50", "runs": len(stream), "quantum": quantum, "pool_nodes": cap,
+           "bytes_per_node": round(per_node, 1)}
+    offered = sum(L["nx"] * L["ny"] * st for _, st, L in stream) / duration
+    with fs.Pool(cap, slots=256) as p:
+        rep = serving.Server(p, quantum=quantum).serve(make(True), realtime=True)
+    out.update({"offered_cell_steps_per_s": offered, "achieved_cell_steps_per_s": rep["cell_steps_per_s"],
+                "p50_latency_s": rep["p50_latency_s"], "p99_latency_s": rep["p99_latency_s"],
+                "wall_s": rep["wall_s"], "rejected": rep["rejected"]})
+    with fs.Pool(cap, slots=256) as p:
+        sat = serving.Server(p, quantum=quantum).serve(make(False), realtime=False)
+    out["saturated_cell_steps_per_s"] = sat["cell_steps_per_s"]
+    pick = [L for _, _, L in stream][::max(1, len(stream) // sample)][:sample]
+    out["plain"] = plain_rate(fs, pick, 10)
+    out["served_over_plain"] = sat["cell_steps_per_s"] / out["plain"]
+    return out
 
 
 # ---- CPU baseline: the reference Fortran (oracle/_ref), one landscape per process -------------------
@@ -174,10 +312,16 @@ def main():
     ap.add_argument("--ninst", type=int, default=64)
     ap.add_argument("--n", type=int, default=513)
     ap.add_argument("--plain-steps", type=int, default=40)
-    ap.add_argument("--cpu-steps", type=int, default=40)
-    ap.add_argument("--cpu-procs", type=int, default=0, help="0: min(8, host cores)")
+    ap.add_argument("--cpu-steps", type=int, default=200)
+    ap.add_argument("--cpu-procs", type=int, default=0, help="0: min(16, host cores)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--legs", default="uniform,plain,ragged,serving",
+                    help="comma list of uniform, plain, ragged, serving")
+    ap.add_argument("--ragged-steps", type=int, default=10)
+    ap.add_argument("--serve-seconds", type=float, default=8.0)
+    ap.add_argument("--quantum", type=int, default=8)
     args = ap.parse_args()
+    legs = set(args.legs.split(","))
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (landscapes shard "
               "across GPUs as independent batches, see DESIGN.md)", file=sys.stderr)
@@ -191,8 +335,10 @@ def main():
     value = ncells * args.steps / dt
 
     # plain path: one landscape at a time (the first of the batch)
-    pdt, _, _, _ = run_gpu(fs, Ls[:1], args.plain_steps, 1, profile=False)
-    plain = Ls[0]["nx"] * Ls[0]["ny"] * args.plain_steps / pdt
+    plain, pdt = None, None
+    if "plain" in legs:
+        pdt, _, _, _ = run_gpu(fs, Ls[:1], args.plain_steps, 1, profile=False)
+        plain = Ls[0]["nx"] * Ls[0]["ny"] * args.plain_steps / pdt
 
     phases = {p: round(ms / max(1, args.steps), 4) for p, (ms, c) in timers.items() if c}
     out = {
@@ -201,9 +347,9 @@ def main():
         "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded uniform-noise landscapes)",
         "config": {"workload": f"uniform batch {args.ninst} x {args.n}x{args.n} D8 n=1 g=0 kd=1e-2 fixed BC",
                    "landscapes": args.ninst, "cells": ncells},
-        "plain": {"value": plain, "ms_per_step": pdt / args.plain_steps * 1e3,
-                  "workload": f"1 x {args.n}x{args.n}, same parameters"},
-        "batched_over_plain": value / plain,
+        "plain": ({"value": plain, "ms_per_step": pdt / args.plain_steps * 1e3,
+                   "workload": f"1 x {args.n}x{args.n}, same parameters"} if plain else None),
+        "batched_over_plain": value / plain if plain else None,
         "roofline": roofline(timers, stats, ncells),
         "phase_ms_per_step": phases,
         "step_model_frac": round(336.0 * ncells / (dt / args.steps) / 1e9 / HBM_PEAK_GBS, 5),
@@ -211,8 +357,12 @@ def main():
         "basins_per_landscape": int(np.mean([s["nbasins"] for s in stats])),
         "levels_per_landscape": int(np.mean([s["levels"] for s in stats])),
     }
+    if "ragged" in legs:
+        out["ragged"] = ragged_leg(fs, args.ragged_steps, sample=8)
+    if "serving" in legs:
+        out["serving"] = serving_leg(fs, args.quantum, args.serve_seconds, sample=8)
     if not args.no_cpu:
-        procs = args.cpu_procs or min(8, os.cpu_count() or 1)
+        procs = args.cpu_procs or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(Ls, args.cpu_steps, procs)
     print(json.dumps(out))
 

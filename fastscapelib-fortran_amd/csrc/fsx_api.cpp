@@ -384,6 +384,14 @@ extern "C" int fsx_remove_instance(fsx_ctx* c, int inst) {
   return relayout(c);
 }
 
+extern "C" int fsx_device_memory(long long* free_bytes, long long* total_bytes) {
+  size_t f = 0, t = 0;
+  HIPCHK(hipMemGetInfo(&f, &t));
+  if (free_bytes) *free_bytes = (long long)f;
+  if (total_bytes) *total_bytes = (long long)t;
+  return 0;
+}
+
 extern "C" int fsx_pool_info(fsx_ctx* c, long long* free_nodes, long long* largest_free, int* free_slots) {
   if (!c) return fail(FSX_EINVAL, "null context");
   long long tot = 0, big = 0;

@@ -70,6 +70,7 @@ def _load():
         "fsx_add_instance": [_vp, C.c_int, C.c_int, _ip],
         "fsx_remove_instance": [_vp, C.c_int],
         "fsx_pool_info": [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(C.c_int)],
+        "fsx_device_memory": [C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -118,3 +119,11 @@ def indexx(keys) -> np.ndarray:
     out = np.empty(k.size, np.int32)
     check(lib.fsx_indexx(k.size, k.ctypes.data_as(_dp), out.ctypes.data_as(_ip)), "fsx_indexx")
     return out
+
+
+def device_memory():
+    """(free, total) device memory in bytes."""
+    f, t = C.c_longlong(), C.c_longlong()
+    check(lib.fsx_device_memory(C.byref(f), C.byref(t)), "device_memory")
+    return f.value, t.value
+

@@ -59,9 +59,8 @@ _FIELDS = {"h": _lib.FSX_H, "b": _lib.FSX_B, "etot": _lib.FSX_ETOT, "erate": _li
 def pool_capacity(hbm_fraction=0.8, slots=256):
     """Nodes a pool may reserve: the given fraction of free HBM over the library's bytes per node
     (measured on a small pool, multiple-flow buffers included)."""
-    import torch
     from . import Pool
-    free, _ = torch.cuda.mem_get_info()
+    free, _ = _lib.device_memory()
     with Pool(4096, 1) as p:
         k = p.add(3, 3)
         p.set_erosional_parameters(k, np.zeros(9), -1.0, 0.4, 1.0, np.zeros(9), -1.0, 0.0, 0.0, 1.0)  # MFD buffers

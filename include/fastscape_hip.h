@@ -82,6 +82,9 @@ int fsx_create_pool(long long capacity_nodes, int nslots, fsx_ctx **out);
 int fsx_add_instance(fsx_ctx *ctx, int nx, int ny, int *inst);
 int fsx_remove_instance(fsx_ctx *ctx, int inst);
 int fsx_pool_info(fsx_ctx *ctx, long long *free_nodes, long long *largest_free, int *free_slots);
+/* Free and total device memory in bytes (hipMemGetInfo), for sizing pools against free HBM; the
+ * reference allocates host memory in SetUp without such a check (src/FastScape_ctx.f90:64-80). */
+int fsx_device_memory(long long *free_bytes, long long *total_bytes);
 
 /* FastScape_Set_XL_YL (src/FastScape_api.f90:491-503) */
 int fsx_set_xl_yl(fsx_ctx *ctx, int inst, double xl, double yl);
