@@ -101,6 +101,25 @@ __device__ __forceinline__ int donors_sorted(const InstParams& P, int v, unsigne
   return nd;
 }
 
+// x**n and x**(n-1) of the Newton step of the stream-power law (src/StreamPowerLaw.f90:352-353), x >= 0.
+// For the exponents the configurations use, the libm power is replaced by its exact-arithmetic
+// equivalent (n = 2: x*x and x; n = 1.5: x*sqrt(x) and sqrt(x)), within an ulp of pow and far
+// cheaper inside the serial Newton loop.  kind: 0 general, 1 n == 2, 2 n == 1.5 (npow_kind).
+__device__ __forceinline__ int npow_kind(double n) { return n == 2.0 ? 1 : (n == 1.5 ? 2 : 0); }
+__device__ __forceinline__ void npow(double x, double n, int kind, double& pn, double& pn1) {
+  if (kind == 1) {
+    pn = x * x;
+    pn1 = x;
+  } else if (kind == 2) {
+    const double r = sqrt(x);
+    pn = x * r;
+    pn1 = r;
+  } else {
+    pn = pow(x, n);
+    pn1 = pow(x, n - 1.0);
+  }
+}
+
 // order-preserving 64-bit image of a double (for atomic min on doubles)
 __device__ __forceinline__ unsigned long long dkey(double x) {
   unsigned long long u = (unsigned long long)__double_as_longlong(x);

@@ -289,6 +289,7 @@ __global__ __launch_bounds__(FSX_WG) void k_mfd_spl(
   const int L = mnlev[inst];
   const double n = P.n;
   const bool n_is_one = fabs(n - 1.0) < 2.2250738585072014e-308;
+  const int nkind = npow_kind(n);
 
   double hm = 0.0, gmx = -1.7976931348623157e308;
   for (int v = tid; v < nn; v += FSX_WG) {
@@ -399,8 +400,10 @@ __global__ __launch_bounds__(FSX_WG) void k_mfd_spl(
                   const double fk = mfact[8 * gv + k];
                   if (fk >= 0.0) {
                     const double x = fmax(0.0, hv - h[off + mrec[8 * gv + k]]);
-                    f = f + fk * pow(x, n);
-                    df = df + fk * n * pow(x, n - 1.0);
+                    double pn, pn1;
+                    npow(x, n, nkind, pn, pn1);
+                    f = f + fk * pn;
+                    df = df + fk * n * pn1;
                   }
                 }
                 const double hn = hv - f / df;

@@ -434,6 +434,7 @@ __global__ __launch_bounds__(FSX_WG) void k_spl_sweep(
   const int G = ngroups[inst];
   const double n = P.n;
   const bool n_is_one = fabs(n - 1.0) < 2.2250738585072014e-308;
+  const int nkind = npow_kind(n);
 
   // tol = 1e-4*(maxval(abs(h))+1), gmax = maxval(g); level-ordered working copy of h
   double hm = 0.0, gmx = -1.7976931348623157e308;
@@ -615,8 +616,10 @@ __global__ __launch_bounds__(FSX_WG) void k_spl_sweep(
                       double f = hv - h0, df = 1.0;
                       if (act) {
                         const double x = fmax(0.0, hv - hr);
-                        f = f + fact * pow(x, n);
-                        df = df + fact * n * pow(x, n - 1.0);
+                        double pn, pn1;
+                        npow(x, n, nkind, pn, pn1);
+                        f = f + fact * pn;
+                        df = df + fact * n * pn1;
                       }
                       const double hn = hv - f / df;
                       errp = fabs(hn - hv);
