@@ -286,7 +286,7 @@ static int ctx_alloc(long long capacity, int nslots, fsx_ctx** out) {
   A(rec, N); A(basin, N); A(rootrank, N); A(order, N); A(lstart, N + ninst); A(lsill, N);
   A(dmask, N); A(bc, N);
   A(outlet, N); A(bsize, N); A(soff, N); A(ecount, N); A(eoff, N); A(comp, N); A(bpar, N); A(bnew, N); A(best, N);
-  A(tdeg, N); A(toff, N); A(tadj, 2 * N); A(bfront, N);
+  A(tdeg, N); A(toff, N); A(tadj, 2 * N); A(bfront, N); A(trev, 2 * N); A(tjmp, 2 * N);
   A(cb0, FSX_EDGE_CAP * N); A(cb1, FSX_EDGE_CAP * N); A(cn0, FSX_EDGE_CAP * N); A(cn1, FSX_EDGE_CAP * N);
   A(sid, FSX_EDGE_CAP * N); A(npos, FSX_EDGE_CAP * N); A(Lasc, FSX_EDGE_CAP * N); A(Rasc, FSX_EDGE_CAP * N);
   A(segs, FSX_EDGE_CAP * N); A(cw, FSX_EDGE_CAP * N); A(skey, FSX_EDGE_CAP * N); A(intree, FSX_EDGE_CAP * N);

@@ -619,81 +619,152 @@ __global__ __launch_bounds__(FSX_WG) void k_boruvka(const InstParams* __restrict
 }
 
 // ---- 5. orientation from basin0 (order_tree) --------------------------------------------------------
-__global__ __launch_bounds__(FSX_WG) void k_orient(const InstParams* __restrict__ PP, const int* __restrict__ nbasins,
-                                                   const int* __restrict__ nconn, const int* __restrict__ basin0,
-                                                   int* __restrict__ cb0, int* __restrict__ cb1, int* __restrict__ cn0,
-                                                   int* __restrict__ cn1, const uint8_t* __restrict__ intree,
-                                                   int* __restrict__ tdeg, int* __restrict__ toff, int* __restrict__ cur,
-                                                   int* __restrict__ tadj, int* __restrict__ pe, int* __restrict__ front) {
+// order_tree (:389-533) walks the spanning tree from basin0 and makes every edge's first basin the
+// parent side.  Here: the Euler tour of the tree that starts with basin0's first edge traverses every
+// edge parent -> child before child -> parent, so after list-ranking the tour (distance of each
+// directed edge to the tour's end, by pointer jumping) the direction with the larger distance is the
+// downward one.  The result is the unique rooted tree; the adjacency order (filled with atomics) only
+// changes the tour, not the orientation.  Directed edges ("slots") are grouped by source basin.
+#define TOUR_GX 16
+#define TOUR_FLAG0 32  // flags[TOUR_FLAG0 + round]: the jumping round changed something
+#define TOUR_ROUNDS 30
+
+__device__ __forceinline__ unsigned long long tpack(unsigned dist, unsigned nxt) {
+  return ((unsigned long long)dist << 32) | nxt;
+}
+
+__global__ __launch_bounds__(FSX_CHUNK) void k_tour_deg(const InstParams* __restrict__ PP, const int* __restrict__ nconn,
+                                                        const int* __restrict__ cb0, const int* __restrict__ cb1,
+                                                        const uint8_t* __restrict__ intree, int* __restrict__ tdeg) {
+  const int inst = blockIdx.y;
+  const InstParams& P = PP[inst];
+  if (!P.active || !P.run_spl) return;
+  const int n = nconn[inst];
+  const long long eb = (long long)FSX_EDGE_CAP * P.off;
+  for (int e = blockIdx.x * FSX_CHUNK + threadIdx.x; e < n; e += gridDim.x * FSX_CHUNK)
+    if (intree[eb + e]) {
+      atomicAdd(tdeg + P.off + cb0[eb + e], 1);
+      atomicAdd(tdeg + P.off + cb1[eb + e], 1);
+    }
+}
+
+__global__ __launch_bounds__(FSX_WG) void k_tour_scan(const InstParams* __restrict__ PP, const int* __restrict__ nbasins,
+                                                      const int* __restrict__ tdeg, int* __restrict__ toff,
+                                                      int* __restrict__ tcur, int* __restrict__ nslot) {
   __shared__ int lds[FSX_WG / 64];
-  __shared__ int s_end;
   const int inst = blockIdx.x;
   const InstParams& P = PP[inst];
   if (!P.active || !P.run_spl) return;
-  const int nb = nbasins[inst], n = nconn[inst], tid = threadIdx.x;
-  if (n == 0) return;
-  const long long eb = (long long)FSX_EDGE_CAP * P.off;
-  int* E0 = cb0 + eb;
-  int* E1 = cb1 + eb;
-  int* N0 = cn0 + eb;
-  int* N1 = cn1 + eb;
-  const uint8_t* IT = intree + eb;
-  int* DG = tdeg + P.off;
-  int* OF = toff + P.off;
-  int* CU = cur + P.off;
-  int* ADJ = tadj + P.off * 2;  // 2*(nb-1) entries <= 2*nn
-  int* PE = pe + P.off;
-  int* FR = front + P.off;
-  for (int b = tid; b < nb; b += FSX_WG) { DG[b] = 0; CU[b] = 0; PE[b] = -1; }
-  __syncthreads();
-  for (int e = tid; e < n; e += FSX_WG)
-    if (IT[e]) { atomicAdd(DG + E0[e], 1); atomicAdd(DG + E1[e], 1); }
-  __syncthreads();
+  const int nb = nbasins[inst];
   int carry = 0;
   for (int s = 0; s < nb; s += FSX_WG) {
-    const int b = s + tid;
-    const int x = b < nb ? DG[b] : 0;
+    const int b = s + threadIdx.x;
+    const int x = b < nb ? tdeg[P.off + b] : 0;
     int tot;
     const int incl = block_incl_scan<FSX_WG>(x, lds, tot);
-    if (b < nb) OF[b] = carry + incl - x;
+    if (b < nb) {
+      toff[P.off + b] = carry + incl - x;
+      tcur[P.off + b] = 0;
+    }
     carry += tot;
   }
-  __syncthreads();
-  for (int e = tid; e < n; e += FSX_WG)
-    if (IT[e]) {
-      const int a = E0[e], c = E1[e];
-      ADJ[OF[a] + atomicAdd(CU + a, 1)] = e;
-      ADJ[OF[c] + atomicAdd(CU + c, 1)] = e;
-    }
-  __syncthreads();
-  const int root = basin0[inst];
-  if (tid == 0) {
-    FR[0] = root;
-    PE[root] = -2;
-    s_end = 1;
+  if (threadIdx.x == 0) nslot[inst] = carry;
+}
+
+__global__ __launch_bounds__(FSX_CHUNK) void k_tour_fill(const InstParams* __restrict__ PP, const int* __restrict__ nconn,
+                                                         const int* __restrict__ cb0, const int* __restrict__ cb1,
+                                                         const uint8_t* __restrict__ intree, const int* __restrict__ toff,
+                                                         int* __restrict__ tcur, int* __restrict__ ttgt,
+                                                         int* __restrict__ trev, int* __restrict__ sa,
+                                                         int* __restrict__ sc) {
+  const int inst = blockIdx.y;
+  const InstParams& P = PP[inst];
+  if (!P.active || !P.run_spl) return;
+  const int n = nconn[inst];
+  const long long eb = (long long)FSX_EDGE_CAP * P.off, sb = 2LL * P.off;
+  for (int e = blockIdx.x * FSX_CHUNK + threadIdx.x; e < n; e += gridDim.x * FSX_CHUNK) {
+    if (!intree[eb + e]) continue;
+    const int a = cb0[eb + e], c = cb1[eb + e];
+    const int x = toff[P.off + a] + atomicAdd(tcur + P.off + a, 1);
+    const int y = toff[P.off + c] + atomicAdd(tcur + P.off + c, 1);
+    ttgt[sb + x] = c;
+    ttgt[sb + y] = a;
+    trev[sb + x] = y;
+    trev[sb + y] = x;
+    sa[eb + e] = x;
+    sc[eb + e] = y;
   }
-  __syncthreads();
-  int fs = 0, fe = 1;
-  while (fs < fe) {
-    for (int idx = fs + tid; idx < fe; idx += FSX_WG) {
-      const int B = FR[idx];
-      for (int q = OF[B]; q < OF[B] + DG[B]; q++) {
-        const int e = ADJ[q];
-        const int o = E0[e] == B ? E1[e] : E0[e];
-        if (PE[o] == -1) {
-          PE[o] = e;
-          FR[atomicAdd(&s_end, 1)] = o;
-          if (E0[e] != B) {  // make (n1,p1) the parent side
-            E0[e] = B; E1[e] = o;
-            const int t = N0[e]; N0[e] = N1[e]; N1[e] = t;
-          }
-        }
-      }
+}
+
+// successor of slot s (u -> v): the slot after (v -> u) in v's list, circularly; the slot whose
+// successor is basin0's first slot ends the tour
+__global__ __launch_bounds__(FSX_CHUNK) void k_tour_init(const InstParams* __restrict__ PP, const int* __restrict__ nslot,
+                                                         const int* __restrict__ basin0, const int* __restrict__ tdeg,
+                                                         const int* __restrict__ toff, const int* __restrict__ ttgt,
+                                                         const int* __restrict__ trev,
+                                                         unsigned long long* __restrict__ tj) {
+  const int inst = blockIdx.y;
+  const InstParams& P = PP[inst];
+  if (!P.active || !P.run_spl) return;
+  const int ns = nslot[inst];
+  if (ns == 0) return;
+  const long long sb = 2LL * P.off;
+  const int s0 = toff[P.off + basin0[inst]];
+  for (int s = blockIdx.x * FSX_CHUNK + threadIdx.x; s < ns; s += gridDim.x * FSX_CHUNK) {
+    const int v = ttgt[sb + s], r = trev[sb + s];
+    const int o = toff[P.off + v];
+    const int nx = r + 1 < o + tdeg[P.off + v] ? r + 1 : o;
+    tj[sb + s] = nx == s0 ? tpack(0u, (unsigned)s) : tpack(1u, (unsigned)nx);
+  }
+}
+
+__global__ __launch_bounds__(FSX_CHUNK) void k_tour_jump(const InstParams* __restrict__ PP, const int* __restrict__ nslot,
+                                                         unsigned long long* __restrict__ tj, int* __restrict__ flags,
+                                                         int round) {
+  if (round > 0 && flags[TOUR_FLAG0 + round - 1] == 0) return;
+  const int inst = blockIdx.y;
+  const InstParams& P = PP[inst];
+  if (!P.active || !P.run_spl) return;
+  const int ns = nslot[inst];
+  unsigned long long* T = tj + 2LL * P.off;
+  for (int s = blockIdx.x * FSX_CHUNK + threadIdx.x; s < ns; s += gridDim.x * FSX_CHUNK) {
+    const unsigned long long w = T[s];
+    const int x = (int)(unsigned)(w & 0xffffffffu);
+    const unsigned long long wx = T[x];
+    const int xx = (int)(unsigned)(wx & 0xffffffffu);
+    if (xx != x) {
+      T[s] = tpack((unsigned)((w >> 32) + (wx >> 32)), (unsigned)xx);
+      flags[TOUR_FLAG0 + round] = 1;
     }
-    __syncthreads();
-    fs = fe;
-    fe = s_end;
-    __syncthreads();
+  }
+}
+
+// make (cb0, cn0) the parent side of every tree edge
+__global__ __launch_bounds__(FSX_CHUNK) void k_tour_orient(const InstParams* __restrict__ PP, const int* __restrict__ nconn,
+                                                           int* __restrict__ cb0, int* __restrict__ cb1,
+                                                           int* __restrict__ cn0, int* __restrict__ cn1,
+                                                           const uint8_t* __restrict__ intree, const int* __restrict__ sa,
+                                                           const int* __restrict__ sc,
+                                                           const unsigned long long* __restrict__ tj,
+                                                           const int* __restrict__ flags, int* __restrict__ err) {
+  const int inst = blockIdx.y;
+  const InstParams& P = PP[inst];
+  if (!P.active || !P.run_spl) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && flags[TOUR_FLAG0 + TOUR_ROUNDS - 1])
+    atomicOr(err + inst, FSX_ERR_LEVELS);  // tour longer than the rounds can rank
+  const int n = nconn[inst];
+  const long long eb = (long long)FSX_EDGE_CAP * P.off, sb = 2LL * P.off;
+  for (int e = blockIdx.x * FSX_CHUNK + threadIdx.x; e < n; e += gridDim.x * FSX_CHUNK) {
+    if (!intree[eb + e]) continue;
+    const unsigned da = (unsigned)(tj[sb + sa[eb + e]] >> 32), dc = (unsigned)(tj[sb + sc[eb + e]] >> 32);
+    if (da < dc) {  // c -> a comes first: c is the parent
+      const int t0 = cb0[eb + e];
+      cb0[eb + e] = cb1[eb + e];
+      cb1[eb + e] = t0;
+      const int t1 = cn0[eb + e];
+      cn0[eb + e] = cn1[eb + e];
+      cn1[eb + e] = t1;
+    }
   }
 }
 
@@ -775,8 +846,17 @@ void launch_lm_mst(DevState& S, hipStream_t st) {
 }
 
 void launch_lm_orient(DevState& S, hipStream_t st) {
-  k_orient<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.nbasins, S.nconn, S.basin0, S.cb0, S.cb1, S.cn0, S.cn1, S.intree, S.tdeg,
-                                        S.toff, S.bsize, S.tadj, S.bpar, S.bfront);
+  const dim3 g(TOUR_GX, S.ninst), ge(4 * TOUR_GX, S.ninst);
+  launch_fill_int(S.tdeg, S.ntot, 0, st);
+  launch_fill_int(S.flags + TOUR_FLAG0, 64 - TOUR_FLAG0, 0, st);
+  k_tour_deg<<<g, FSX_CHUNK, 0, st>>>(S.P, S.nconn, S.cb0, S.cb1, S.intree, S.tdeg);
+  k_tour_scan<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.nbasins, S.tdeg, S.toff, S.bpar, S.maxd);
+  k_tour_fill<<<g, FSX_CHUNK, 0, st>>>(S.P, S.nconn, S.cb0, S.cb1, S.intree, S.toff, S.bpar, S.tadj, S.trev, S.Lasc,
+                                        S.Rasc);
+  k_tour_init<<<ge, FSX_CHUNK, 0, st>>>(S.P, S.maxd, S.basin0, S.tdeg, S.toff, S.tadj, S.trev, S.tjmp);
+  for (int r = 0; r < TOUR_ROUNDS; r++) k_tour_jump<<<ge, FSX_CHUNK, 0, st>>>(S.P, S.maxd, S.tjmp, S.flags, r);
+  k_tour_orient<<<g, FSX_CHUNK, 0, st>>>(S.P, S.nconn, S.cb0, S.cb1, S.cn0, S.cn1, S.intree, S.Lasc, S.Rasc, S.tjmp,
+                                          S.flags, S.err);
 }
 
 void launch_lm_correct(DevState& S, hipStream_t st) {

@@ -30,6 +30,8 @@ struct DevState {
   int *outlet = nullptr, *bsize = nullptr, *soff = nullptr, *ecount = nullptr, *eoff = nullptr;
   int *comp = nullptr, *bpar = nullptr, *bnew = nullptr, *best = nullptr, *tdeg = nullptr, *toff = nullptr;
   int *tadj = nullptr, *bfront = nullptr;
+  int *trev = nullptr;                   // Euler tour: reverse slot of each directed tree edge (2 per node)
+  unsigned long long *tjmp = nullptr;    // Euler tour: (distance to end, next slot), 2 per node
 
   // ---- per edge (FSX_EDGE_CAP * node offsets) ----
   int *cb0 = nullptr, *cb1 = nullptr, *cn0 = nullptr, *cn1 = nullptr, *sid = nullptr, *npos = nullptr;
