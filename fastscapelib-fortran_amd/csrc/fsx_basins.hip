@@ -334,7 +334,7 @@ __device__ __forceinline__ void kswap(double* K, int* I, int p, int q) {
   const int ti = I[p]; I[p] = I[q]; I[q] = ti;
 }
 
-__device__ void nr_insertion(double* K, int* I, int l, int ir) {
+__device__ __forceinline__ void nr_insertion(double* K, int* I, int l, int ir) {
   for (int j = l + 1; j <= ir; j++) {
     const double a = K[j];
     const int ia = I[j];
@@ -351,7 +351,7 @@ __device__ void nr_insertion(double* K, int* I, int l, int ir) {
 
 // returns (istar, jstar) in every thread of the group; sh must be per-group shared scratch (>= 4 ints)
 template <int NT>
-__device__ void nr_partition(double* K, int* I, int* LA, int* RA, int l, int ir, int* lds, int* sh, double* shd,
+__device__ __forceinline__ void nr_partition(double* K, int* I, int* LA, int* RA, int l, int ir, int* lds, int* sh, double* shd,
                              int& istar, int& jstar) {
   const int rk = grp_rank<NT>();
   if (rk == 0) {
@@ -477,63 +477,140 @@ __global__ __launch_bounds__(FSX_WG) void k_nr_big(const InstParams* __restrict_
   if (threadIdx.x == 0) nsegs[inst] = s_ns;
 }
 
-// Segments of <= NR_BIG keys: each wave takes a segment and finishes it depth first.
-#define NR_WSTACK 64
+// Segments of <= NR_BIG keys, one 256-thread block per segment, entirely in LDS.  Sub-ranges are
+// disjoint, so their order of processing does not matter: each round either partitions one range of
+// more than NR_MID keys with the whole block, or up to four ranges of NR_TINY+1..NR_MID keys with one
+// wave each; ranges of at most NR_TINY keys are finished at the end, one per thread, by the
+// sequential routine itself (median of three, the scanning partition, insertion sort below 7).
+#define NR_MID 512
+#define NR_TINY 32
+#define NR_QCAP 512
+
+// loc_min_3_indexx on K/I[l..ir] (LDS), one thread, as the reference's loop (:955-1044)
+__device__ __forceinline__ void nr_serial(double* K, int* I, int l, int ir) {
+  int stk[16];
+  int js = 0;
+  for (;;) {
+    if (ir - l < 7) {
+      nr_insertion(K, I, l, ir);
+      if (js == 0) return;
+      ir = stk[--js];
+      l = stk[--js];
+      continue;
+    }
+    const int k = (l + ir) >> 1;
+    kswap(K, I, k, l + 1);
+    if (K[l + 1] > K[ir]) kswap(K, I, l + 1, ir);
+    if (K[l] > K[ir]) kswap(K, I, l, ir);
+    if (K[l + 1] > K[l]) kswap(K, I, l + 1, l);
+    int i = l + 1, j = ir;
+    const double a = K[l];
+    const int ia = I[l];
+    for (;;) {
+      do i++; while (K[i] < a);
+      do j--; while (K[j] > a);
+      if (j < i) break;
+      kswap(K, I, i, j);
+    }
+    K[l] = K[j]; I[l] = I[j];
+    K[j] = a; I[j] = ia;
+    // push the larger sub-range, continue with the smaller (:1033-1042): at most log2(32) pending
+    if (ir - i + 1 >= j - l) {
+      stk[js++] = i;
+      stk[js++] = ir;
+      ir = j - 1;
+    } else {
+      stk[js++] = l;
+      stk[js++] = j - 1;
+      l = i;
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void k_nr_small(const InstParams* __restrict__ PP, const int* __restrict__ nsegs,
                                                   int* __restrict__ segctr, double* __restrict__ skey,
-                                                  int* __restrict__ sid, int* __restrict__ Lasc, int* __restrict__ Rasc,
-                                                  const int* __restrict__ segs, int* __restrict__ err) {
-  __shared__ int sh[4][4];
-  __shared__ double shd[4][1];
-  __shared__ int wst[4][2 * NR_WSTACK];
+                                                  int* __restrict__ sid, const int* __restrict__ segs,
+                                                  int* __restrict__ err) {
+  __shared__ double sK[NR_BIG];
+  __shared__ int sI[NR_BIG];
+  __shared__ int sLA[NR_BIG];
+  __shared__ int sRA[NR_BIG];
+  __shared__ int q[2 * NR_QCAP];      // pending ranges (more than NR_TINY keys)
+  __shared__ int tq[2 * NR_BIG / 2];  // tiny ranges
+  __shared__ int lds[4];
+  __shared__ int sh[5][4];
+  __shared__ double shd[5][1];
+  __shared__ int s_seg, s_nq, s_nt, s_take, s_ovf;
   const int inst = blockIdx.y;
   const InstParams& P = PP[inst];
   if (!P.active || !P.run_spl) return;
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int tid = threadIdx.x, w = tid >> 6;
   const long long eb = (long long)FSX_EDGE_CAP * P.off;
   double* K = skey + eb;
   int* I = sid + eb;
-  int* LA = Lasc + eb;
-  int* RA = Rasc + eb;
   const int* SG = segs + eb;
   const int ns = nsegs[inst];
   for (;;) {
-    int s = 0;
-    if (lane == 0) s = atomicAdd(segctr + inst, 1);
-    s = __shfl(s, 0, 64);
-    if (s >= ns) break;
-    int top = 0;
-    if (lane == 0) { wst[w][0] = SG[2 * s]; wst[w][1] = SG[2 * s + 1]; }
-    top = 1;
-    grp_sync<64>();
-    while (top > 0) {
-      top--;
-      const int l = wst[w][2 * top], ir = wst[w][2 * top + 1];
-      grp_sync<64>();
-      if (ir - l < 7) {
-        if (lane == 0) nr_insertion(K, I, l, ir);
-        grp_sync<64>();
-        continue;
-      }
-      int is, js;
-      nr_partition<64>(K, I, LA, RA, l, ir, nullptr, sh[w], shd[w], is, js);
-      if (lane == 0) {
-        if (js - 1 - l + 1 >= 2) {
-          if (top < NR_WSTACK) { wst[w][2 * top] = l; wst[w][2 * top + 1] = js - 1; }
-          else atomicOr(err + inst, FSX_ERR_SORTSTACK);
-        }
-      }
-      if (js - l >= 2) top++;
-      if (lane == 0) {
-        if (ir - is + 1 >= 2) {
-          if (top < NR_WSTACK) { wst[w][2 * top] = is; wst[w][2 * top + 1] = ir; }
-          else atomicOr(err + inst, FSX_ERR_SORTSTACK);
-        }
-      }
-      if (ir - is + 1 >= 2) top++;
-      if (top > NR_WSTACK) top = NR_WSTACK;
-      grp_sync<64>();
+    if (tid == 0) s_seg = atomicAdd(segctr + inst, 1);
+    __syncthreads();
+    const int sg = s_seg;
+    if (sg >= ns) break;
+    const int g0 = SG[2 * sg], g1 = SG[2 * sg + 1], n = g1 - g0 + 1;
+    for (int p = tid; p < n; p += 256) { sK[p] = K[g0 + p]; sI[p] = I[g0 + p]; }
+    if (tid == 0) {
+      s_nq = 0; s_nt = 0; s_ovf = 0;
+      if (n > NR_TINY) { q[0] = 0; q[1] = n - 1; s_nq = 1; }
+      else if (n >= 2) { tq[0] = 0; tq[1] = n - 1; s_nt = 1; }
     }
+    __syncthreads();
+    while (s_nq > 0) {
+      // a round: the top range alone if it is big, else up to four mid ranges
+      const int top = s_nq - 1;
+      const bool big = q[2 * top + 1] - q[2 * top] + 1 > NR_MID;
+      int is = 0, js = 0, l = 0, ir = -1;
+      if (big) {
+        l = q[2 * top]; ir = q[2 * top + 1];
+        __syncthreads();
+        if (tid == 0) { s_nq = top; s_take = 1; }
+        nr_partition<256>(sK, sI, sLA, sRA, l, ir, lds, sh[4], shd[4], is, js);
+      } else {
+        int take = 0;
+        for (int t = top; t >= 0 && take < 4; t--) {
+          if (q[2 * t + 1] - q[2 * t] + 1 > NR_MID) break;
+          take++;
+        }
+        if (w < take) { l = q[2 * (top - w)]; ir = q[2 * (top - w) + 1]; }
+        __syncthreads();
+        if (tid == 0) { s_nq = top + 1 - take; s_take = take; }
+        if (w < take) nr_partition<64>(sK, sI, sLA, sRA, l, ir, nullptr, sh[w], shd[w], is, js);
+      }
+      __syncthreads();
+      const bool lead = big ? tid == 0 : ((tid & 63) == 0 && w < s_take);
+      if (lead) {
+        const int r[2][2] = {{l, js - 1}, {is, ir}};
+        for (int c = 0; c < 2; c++) {
+          const int a0 = r[c][0], a1 = r[c][1], sz = a1 - a0 + 1;
+          if (sz > NR_TINY) {
+            const int slot = atomicAdd(&s_nq, 1);
+            if (slot < NR_QCAP) { q[2 * slot] = a0; q[2 * slot + 1] = a1; } else s_ovf = 1;
+          } else if (sz >= 2) {
+            const int slot = atomicAdd(&s_nt, 1);
+            tq[2 * slot] = a0; tq[2 * slot + 1] = a1;
+          }
+        }
+      }
+      __syncthreads();
+      if (s_ovf) break;
+    }
+    if (s_ovf) {
+      if (tid == 0) atomicOr(err + inst, FSX_ERR_SORTSTACK);
+      __syncthreads();
+      continue;
+    }
+    for (int t = tid; t < s_nt; t += 256) nr_serial(sK, sI, tq[2 * t], tq[2 * t + 1]);
+    __syncthreads();
+    for (int p = tid; p < n; p += 256) { K[g0 + p] = sK[p]; I[g0 + p] = sI[p]; }
+    __syncthreads();
   }
 }
 
@@ -810,7 +887,7 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_correct(const InstParams* __restr
 void launch_nr_sort(DevState& S, hipStream_t st) {
   launch_fill_int(S.segctr, S.ninst, 0, st);
   k_nr_big<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.nconn, S.cw, S.skey, S.sid, S.Lasc, S.Rasc, S.segs, S.nsegs, S.err);
-  k_nr_small<<<dim3(8, S.ninst), 256, 0, st>>>(S.P, S.nsegs, S.segctr, S.skey, S.sid, S.Lasc, S.Rasc, S.segs, S.err);
+  k_nr_small<<<dim3(16, S.ninst), 256, 0, st>>>(S.P, S.nsegs, S.segctr, S.skey, S.sid, S.segs, S.err);
   k_nr_pos<<<dim3(16, S.ninst), FSX_CHUNK, 0, st>>>(S.P, S.nconn, S.sid, S.npos);
 }
 
