@@ -25,6 +25,7 @@ static int configure_sweeps() {
   if (hipGetDevice(&dev) != hipSuccess) return -1;
   if (hipDeviceGetAttribute(&smem, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return -1;
   if (sweep_set_lds(smem) != 0) return -1;
+  if (basins_set_lds(smem) != 0) return -1;
   g_sweep_cap = smem;
   return 0;
 }
@@ -283,6 +284,7 @@ static int ctx_alloc(long long capacity, int nslots, fsx_ctx** out) {
   A(h, N); A(b, N); A(u, N); A(kf, N); A(kd, N); A(precip, N); A(etot, N); A(erate, N); A(area, N);
   A(length, N); A(hwater, N); A(lake_depth, N); A(sedflux, N); A(ht, N); A(water, N); A(zint, N); A(zintp, N);
   A(gam, N); A(dh, N); A(hp, N); A(lwv, N); A(lsed, N); A(kfint, N); A(kdint, N); A(g, N); A(pexp, N); A(elev, N);
+  A(adx, N); A(ady, N); A(adg, N); A(adf, N);
   A(rec, N); A(basin, N); A(rootrank, N); A(order, N); A(lstart, N + ninst); A(lsill, N);
   A(dmask, N); A(bc, N);
   A(outlet, N); A(bsize, N); A(soff, N); A(ecount, N); A(eoff, N); A(comp, N); A(bpar, N); A(bnew, N); A(best, N);
@@ -292,7 +294,7 @@ static int ctx_alloc(long long capacity, int nslots, fsx_ctx** out) {
   A(segs, FSX_EDGE_CAP * N); A(cw, FSX_EDGE_CAP * N); A(skey, FSX_EDGE_CAP * N); A(intree, FSX_EDGE_CAP * N);
   A(sB, FSX_SLAB_CAP * N); A(sp, FSX_SLAB_CAP * N); A(sq, FSX_SLAB_CAP * N); A(sw, FSX_SLAB_CAP * N);
   A(nbasins, ninst); A(basin0, ninst); A(nconn, ninst); A(nlev, ninst); A(err, ninst); A(ngs, ninst);
-  A(ntree, ninst); A(chunkcnt, c->chunk_cap); A(newton, ninst); A(scal, ninst); A(flags, 64);
+  A(ntree, ninst); A(chunkcnt, c->chunk_cap); A(newton, ninst); A(scal, ninst); A(redk, 2 * ninst); A(flags, 64);
   A(nsegs, ninst); A(segctr, ninst); A(maxd, ninst);
   A(J, N); A(ngroups, ninst); A(grec, 2 * (N + ninst)); A(first, N); A(next, N); A(gmeta, N + ninst); A(pos, N); A(lcur, N + ninst); A(rp, N); A(LSl, N); A(dn, 8 * N);
   A(A0, N); A(Al, N); A(F, N); A(HTl, N); A(Hl, N); A(Wl, N); A(HWl, N); A(HPl, N); A(DHl, N); A(ELl, N);

@@ -625,74 +625,114 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_nr_pos(const InstParams* __restri
 }
 
 // ---- 4. Boruvka -----------------------------------------------------------------------------------
+// Minimum spanning forest keyed by the edge's rank in the sorted order (npos): every component
+// picks its lowest-ranked outgoing edge, components merge along them, repeat.  With a strict
+// total order the tree is unique, so it is Kruskal's in that order (mst_kruskal, :333-385).  The
+// component id of every basin lives in LDS when the landscape's basins fit (BORUVKA_LDS), and the
+// edge scan handles BORUVKA_UNROLL edges per thread at a time so their loads overlap.
+#define BORUVKA_UNROLL 4
+static int g_boruvka_lds = 0;  // component ids held in LDS (ints), set from the device's LDS size
+
+template <bool INLDS>
+__device__ __forceinline__ void boruvka_body(const int nb, const int n, const int* E0, const int* E1, const int* NP,
+                                             const int* SI, int* C, int* BST, int* BN, uint8_t* IT, int* s_any,
+                                             int* s_chg) {
+  const int tid = threadIdx.x;
+  for (int round = 0; round < 64; round++) {
+    for (int b = tid; b < nb; b += FSX_WG) BST[b] = INT_MAX;
+    if (tid == 0) *s_any = 0;
+    __syncthreads();
+    for (int e0 = tid; e0 < n; e0 += FSX_WG * BORUVKA_UNROLL) {
+      int a[BORUVKA_UNROLL], c[BORUVKA_UNROLL], k[BORUVKA_UNROLL];
+#pragma unroll
+      for (int u = 0; u < BORUVKA_UNROLL; u++) {
+        const int e = e0 + u * FSX_WG;
+        a[u] = e < n ? E0[e] : 0;
+        c[u] = e < n ? E1[e] : 0;
+        k[u] = e < n ? NP[e] : INT_MAX;
+      }
+#pragma unroll
+      for (int u = 0; u < BORUVKA_UNROLL; u++) {
+        a[u] = C[a[u]];
+        c[u] = C[c[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < BORUVKA_UNROLL; u++)
+        if (k[u] != INT_MAX && a[u] != c[u]) {
+          atomicMin(BST + a[u], k[u]);
+          atomicMin(BST + c[u], k[u]);
+        }
+    }
+    __syncthreads();
+    for (int cc = tid; cc < nb; cc += FSX_WG) {
+      if (C[cc] != cc) continue;
+      BN[cc] = cc;
+      const int kk = BST[cc];
+      if (kk == INT_MAX) continue;
+      const int e = SI[kk];
+      IT[e] = 1;
+      const int c0 = C[E0[e]], c1 = C[E1[e]];
+      const int o = c0 == cc ? c1 : c0;
+      BN[cc] = (BST[o] == kk) ? (cc < o ? cc : o) : o;
+      *s_any = 1;
+    }
+    __syncthreads();
+    if (!*s_any) break;
+    for (;;) {
+      if (tid == 0) *s_chg = 0;
+      __syncthreads();
+      for (int cc = tid; cc < nb; cc += FSX_WG) {
+        if (C[cc] != cc) continue;
+        const int x = BN[cc], y = BN[x];
+        if (x != y) { BN[cc] = y; *s_chg = 1; }
+      }
+      __syncthreads();
+      if (!*s_chg) break;
+    }
+    for (int b = tid; b < nb; b += FSX_WG) C[b] = BN[C[b]];
+    __syncthreads();
+  }
+}
+
 __global__ __launch_bounds__(FSX_WG) void k_boruvka(const InstParams* __restrict__ PP, const int* __restrict__ nbasins,
                                                     const int* __restrict__ nconn, const int* __restrict__ cb0,
                                                     const int* __restrict__ cb1, const int* __restrict__ npos,
                                                     const int* __restrict__ sid, int* __restrict__ comp,
                                                     int* __restrict__ best, int* __restrict__ bnew,
-                                                    uint8_t* __restrict__ intree, int* __restrict__ ntree) {
+                                                    uint8_t* __restrict__ intree, int* __restrict__ ntree, int ldscap) {
+  extern __shared__ int sC[];
   __shared__ int s_any, s_chg;
   const int inst = blockIdx.x;
   const InstParams& P = PP[inst];
   if (!P.active || !P.run_spl) return;
   const int nb = nbasins[inst], n = nconn[inst], tid = threadIdx.x;
   const long long eb = (long long)FSX_EDGE_CAP * P.off;
-  const int* E0 = cb0 + eb;
-  const int* E1 = cb1 + eb;
-  const int* NP = npos + eb;
-  const int* SI = sid + eb;
   uint8_t* IT = intree + eb;
-  int* C = comp + P.off;
-  int* BST = best + P.off;
-  int* BN = bnew + P.off;
+  const bool inlds = nb <= ldscap;
+  int* C = inlds ? sC : comp + P.off;
   for (int b = tid; b < nb; b += FSX_WG) C[b] = b;
   for (int e = tid; e < n; e += FSX_WG) IT[e] = 0;
   __syncthreads();
-  if (n == 0) return;
-  for (int round = 0; round < 64; round++) {
-    for (int b = tid; b < nb; b += FSX_WG) BST[b] = INT_MAX;
-    if (tid == 0) s_any = 0;
-    __syncthreads();
-    for (int e = tid; e < n; e += FSX_WG) {
-      const int c0 = C[E0[e]], c1 = C[E1[e]];
-      if (c0 != c1) {
-        const int k = NP[e];
-        atomicMin(BST + c0, k);
-        atomicMin(BST + c1, k);
-      }
-    }
-    __syncthreads();
-    for (int cc = tid; cc < nb; cc += FSX_WG) {
-      if (C[cc] != cc) continue;
-      BN[cc] = cc;
-      const int k = BST[cc];
-      if (k == INT_MAX) continue;
-      const int e = SI[k];
-      IT[e] = 1;
-      const int c0 = C[E0[e]], c1 = C[E1[e]];
-      const int o = c0 == cc ? c1 : c0;
-      BN[cc] = (BST[o] == k) ? (cc < o ? cc : o) : o;
-      s_any = 1;
-    }
-    __syncthreads();
-    if (!s_any) break;
-    for (;;) {
-      if (tid == 0) s_chg = 0;
-      __syncthreads();
-      for (int cc = tid; cc < nb; cc += FSX_WG) {
-        if (C[cc] != cc) continue;
-        const int x = BN[cc], y = BN[x];
-        if (x != y) { BN[cc] = y; s_chg = 1; }
-      }
-      __syncthreads();
-      if (!s_chg) break;
-    }
-    for (int b = tid; b < nb; b += FSX_WG) C[b] = BN[C[b]];
-    __syncthreads();
+  if (n > 0) {
+    if (inlds)
+      boruvka_body<true>(nb, n, cb0 + eb, cb1 + eb, npos + eb, sid + eb, sC, best + P.off, bnew + P.off, IT, &s_any,
+                         &s_chg);
+    else
+      boruvka_body<false>(nb, n, cb0 + eb, cb1 + eb, npos + eb, sid + eb, comp + P.off, best + P.off, bnew + P.off, IT,
+                          &s_any, &s_chg);
   }
   int cnt = 0;
   for (int e = tid; e < n; e += FSX_WG) cnt += IT[e];
   if (cnt) atomicAdd(ntree + inst, cnt);
+}
+
+int basins_set_lds(int smem_max) {
+  const int bytes = smem_max - 1024;  // room for the static shared words
+  if (bytes < 4096) return -1;
+  if (hipFuncSetAttribute((const void*)k_boruvka, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess)
+    return -1;
+  g_boruvka_lds = bytes / (int)sizeof(int);
+  return 0;
 }
 
 // ---- 5. orientation from basin0 (order_tree) --------------------------------------------------------
@@ -918,8 +958,8 @@ void launch_lm_edges(DevState& S, hipStream_t st) {
 
 void launch_lm_mst(DevState& S, hipStream_t st) {
   launch_fill_int(S.ntree, S.ninst, 0, st);
-  k_boruvka<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.nbasins, S.nconn, S.cb0, S.cb1, S.npos, S.sid, S.comp, S.best, S.bnew,
-                                         S.intree, S.ntree);
+  k_boruvka<<<S.ninst, FSX_WG, (size_t)g_boruvka_lds * sizeof(int), st>>>(
+      S.P, S.nbasins, S.nconn, S.cb0, S.cb1, S.npos, S.sid, S.comp, S.best, S.bnew, S.intree, S.ntree, g_boruvka_lds);
 }
 
 void launch_lm_orient(DevState& S, hipStream_t st) {

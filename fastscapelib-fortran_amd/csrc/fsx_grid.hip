@@ -96,17 +96,30 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_spl_prep(const InstParams* __rest
 }
 
 // Stream-power bookkeeping, src/StreamPowerLaw.f90:380-393: b=min(h,b), lake sediment fill,
-// etot=(etot+ht)-h, erate=(ht-h)/dt, Sedflux=ht-h.
+// etot=(etot+ht)-h, erate=(ht-h)/dt, Sedflux=ht-h.  For single flow direction the sweep left h and
+// the lake surface in level order (Hl, HWl); they are scattered back to grid order here, with
+// lake_depth = hwater - h of the routing (FlowRoutingSingleFlowDirection :87-89).
 __global__ __launch_bounds__(FSX_CHUNK) void k_spl_post(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
                                                         double* __restrict__ h, double* __restrict__ b,
                                                         const double* __restrict__ ht, double* __restrict__ etot,
                                                         double* __restrict__ erate, double* __restrict__ sedflux,
                                                         const int* __restrict__ lsill, const double* __restrict__ lwv,
                                                         const double* __restrict__ lsed, const double* __restrict__ water,
-                                                        const double* __restrict__ gmax_flag) {
+                                                        const double* __restrict__ gmax_flag, const int* __restrict__ pos,
+                                                        const double* __restrict__ Hl, const double* __restrict__ HWl,
+                                                        double* __restrict__ hwater, double* __restrict__ lake_depth) {
   NODE_KERNEL_PROLOGUE
   if (!P.run_spl) return;
-  double hv = h[g_];
+  double hv;
+  if (P.sfd) {
+    const long long gp = (long long)P.off + pos[g_];
+    hv = Hl[gp];
+    const double hw = HWl[gp];
+    hwater[g_] = hw;
+    lake_depth[g_] = hw - ht[g_];
+  } else {
+    hv = h[g_];
+  }
   b[g_] = fmin(hv, b[g_]) == hv ? hv : b[g_];
   if (gmax_flag[c.inst] != 0.0) {  // the fill adds exactly zero when g == 0 everywhere
     const int sl = lsill[g_];
@@ -119,8 +132,8 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_spl_post(const InstParams* __rest
         hv = hv + s / vol * (water[g_] - hv);
       }
     }
-    h[g_] = hv;
   }
+  h[g_] = hv;
   const double t = ht[g_];
   etot[g_] = etot[g_] + t - hv;
   erate[g_] = (t - hv) / P.dt;
@@ -144,137 +157,195 @@ __device__ __forceinline__ double coef(double ka, double kb, double hdt, double 
   return (ka + kb) / 2.0 * hdt / d2;
 }
 
-// x pass: rows j = 1..ny-2 (0-based); reads h (the reference's zintp == h at this point),
-// writes zint for those rows.
-__global__ __launch_bounds__(64) void k_adi_x(const InstParams* __restrict__ PP, const double* __restrict__ h,
-                                              const double* __restrict__ kdint, double* __restrict__ zint,
-                                              double* __restrict__ gam, int* __restrict__ err) {
+// The coefficients ((kd_a+kd_b)/2)*(dt/2)/d**2 depend only on the pair of nodes, and a+b == b+a
+// exactly, so each is computed once per node pair: cx(i,j) between (i,j) and (i+1,j), cy(i,j)
+// between (i,j) and (i,j+1) (factxm(i,j) == cx(i-1,j), factym(i,j) == cy(i,j-1)).  The diagonal and
+// right-hand side of each line system are then node-parallel; only the Thomas recurrence (tridag,
+// :164-202) runs one thread per line, reading its inputs in prefetched chunks.
+__global__ __launch_bounds__(FSX_CHUNK) void k_adi_coef(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
+                                                        const double* __restrict__ kdint, double* __restrict__ cx,
+                                                        double* __restrict__ cy) {
+  NODE_KERNEL_PROLOGUE
+  if (!P.run_diff) return;
+  const int i = v % P.nx, j = v / P.nx;
+  const double hdt = P.dt / 2.0;
+  const double* K = kdint + P.off;
+  cx[g_] = i < P.nx - 1 ? coef(K[v + 1], K[v], hdt, P.dx * P.dx) : 0.0;
+  cy[g_] = j < P.ny - 1 ? coef(K[v + P.nx], K[v], hdt, P.dy * P.dy) : 0.0;
+}
+
+// x pass, rows j = 1..ny-2 (:45-82): diagonal and right-hand side; zintp == h at this point
+__global__ __launch_bounds__(FSX_CHUNK) void k_adi_xrhs(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
+                                                        const double* __restrict__ h, const double* __restrict__ cx,
+                                                        const double* __restrict__ cy, double* __restrict__ dg,
+                                                        double* __restrict__ rf) {
+  NODE_KERNEL_PROLOGUE
+  if (!P.run_diff) return;
+  const int nx = P.nx, i = v % nx, j = v / nx;
+  if (j == 0 || j == P.ny - 1) return;
+  const double* Z = h + P.off;
+  const double* CX = cx + P.off;
+  const double* CY = cy + P.off;
+  const double fyp = CY[v], fym = CY[v - nx];
+  double diag, f;
+  const double frhs = Z[v] + fyp * Z[v + nx] - (fyp + fym) * Z[v] + fym * Z[v - nx];
+  if (i == 0) {
+    if (P.fix_left) { diag = 1.0; f = Z[v]; }
+    else { diag = 1.0 + CX[v]; f = frhs; }
+  } else if (i == nx - 1) {
+    if (P.fix_right) { diag = 1.0; f = Z[v]; }
+    else { diag = 1.0 + CX[v - 1]; f = frhs; }
+  } else {
+    diag = 1.0 + CX[v] + CX[v - 1];
+    f = frhs;
+  }
+  dg[g_] = diag;
+  rf[g_] = f;
+}
+
+// y pass, columns i = 1..nx-2 (:97-134): zint rows 1..ny-2 from the x pass, rows 0 and ny-1 are h.
+// The reflective-bottom coefficient uses kdint(i,j) with j left at ny by the x loop (:114).
+__global__ __launch_bounds__(FSX_CHUNK) void k_adi_yrhs(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
+                                                        const double* __restrict__ h, const double* __restrict__ zint,
+                                                        const double* __restrict__ kdint, const double* __restrict__ cx,
+                                                        const double* __restrict__ cy, double* __restrict__ dg,
+                                                        double* __restrict__ rf) {
+  NODE_KERNEL_PROLOGUE
+  if (!P.run_diff) return;
+  const int nx = P.nx, ny = P.ny, i = v % nx, j = v / nx;
+  if (i == 0 || i == nx - 1) return;
+  const double* ZR = (j == 0 || j == ny - 1) ? h + P.off : zint + P.off;
+  const double* CX = cx + P.off;
+  const double* CY = cy + P.off;
+  double fxp = CX[v];
+  const double fxm = CX[v - 1];
+  double diag, f;
+  if (j == 0) {
+    if (P.fix_bottom) { diag = 1.0; f = ZR[v]; }
+    else {
+      fxp = coef(kdint[P.off + i + 1], kdint[P.off + (ny - 1) * nx + i], P.dt / 2.0, P.dx * P.dx);
+      diag = 1.0 + CY[v];
+      f = ZR[v] + fxp * ZR[v + 1] - (fxp + fxm) * ZR[v] + fxm * ZR[v - 1];
+    }
+  } else if (j == ny - 1) {
+    if (P.fix_top) { diag = 1.0; f = ZR[v]; }
+    else {
+      diag = 1.0 + CY[v - nx];
+      f = ZR[v] + fxp * ZR[v + 1] - (fxp + fxm) * ZR[v] + fxm * ZR[v - 1];
+    }
+  } else {
+    diag = 1.0 + CY[v] + CY[v - nx];
+    f = ZR[v] + fxp * ZR[v + 1] - (fxp + fxm) * ZR[v] + fxm * ZR[v - 1];
+  }
+  dg[g_] = diag;
+  rf[g_] = f;
+}
+
+// Thomas recurrence along one line of n nodes at stride st from base o (tridag, :178-198):
+//   bet = b(1); u(1) = r(1)/bet; gam(k) = c(k-1)/bet; bet = b(k) - a(k)*gam(k); u(k) = (r(k) - a(k)*u(k-1))/bet;
+//   u(k) = u(k) - gam(k+1)*u(k+1) backwards.
+// a(k) = -cc(k-1), c(k-1) = -cc(k-1) for the coupling cc between k-1 and k, except c(1) = 0 when the
+// first node is fixed and a(n) = 0 when the last one is; inputs are read in chunks of ADI_CH.
+#define ADI_CH 8
+__device__ __forceinline__ bool thomas_line(const double* __restrict__ dg, const double* __restrict__ rf,
+                                            const double* __restrict__ cc, double* __restrict__ U,
+                                            double* __restrict__ G, long long o, int st, int n, bool fix_first,
+                                            bool fix_last) {
+  double bet = dg[o];
+  if (bet == 0.0) return false;
+  double uprev = rf[o] / bet;
+  U[o] = uprev;
+  double nd[ADI_CH], nr[ADI_CH], nc[ADI_CH];
+#pragma unroll
+  for (int e = 0; e < ADI_CH; e++) {
+    const int k = 1 + e;
+    const long long q = o + (long long)k * st;
+    nd[e] = k < n ? dg[q] : 0.0;
+    nr[e] = k < n ? rf[q] : 0.0;
+    nc[e] = k < n ? cc[q - st] : 0.0;
+  }
+  for (int k0 = 1; k0 < n; k0 += ADI_CH) {
+    double d[ADI_CH], r[ADI_CH], c[ADI_CH];
+#pragma unroll
+    for (int e = 0; e < ADI_CH; e++) { d[e] = nd[e]; r[e] = nr[e]; c[e] = nc[e]; }
+#pragma unroll
+    for (int e = 0; e < ADI_CH; e++) {  // next chunk in flight while this one is solved
+      const int k = k0 + ADI_CH + e;
+      const long long q = o + (long long)k * st;
+      nd[e] = k < n ? dg[q] : 0.0;
+      nr[e] = k < n ? rf[q] : 0.0;
+      nc[e] = k < n ? cc[q - st] : 0.0;
+    }
+#pragma unroll
+    for (int e = 0; e < ADI_CH; e++) {
+      const int k = k0 + e;
+      if (k < n) {
+        const double cprev = (k == 1 && fix_first) ? 0.0 : -c[e];
+        const double inf = (k == n - 1 && fix_last) ? 0.0 : -c[e];
+        const double gm = cprev / bet;
+        bet = d[e] - inf * gm;
+        if (bet == 0.0) return false;
+        uprev = (r[e] - inf * uprev) / bet;
+        const long long q = o + (long long)k * st;
+        G[q] = gm;
+        U[q] = uprev;
+      }
+    }
+  }
+  // back substitution, chunks from the end
+  double nu[ADI_CH], ng[ADI_CH];
+#pragma unroll
+  for (int e = 0; e < ADI_CH; e++) {
+    const int k = n - 2 - e;
+    const long long q = o + (long long)k * st;
+    nu[e] = k >= 0 ? U[q] : 0.0;
+    ng[e] = k >= 0 ? G[q + st] : 0.0;
+  }
+  for (int k0 = n - 2; k0 >= 0; k0 -= ADI_CH) {
+    double u[ADI_CH], g[ADI_CH];
+#pragma unroll
+    for (int e = 0; e < ADI_CH; e++) { u[e] = nu[e]; g[e] = ng[e]; }
+#pragma unroll
+    for (int e = 0; e < ADI_CH; e++) {
+      const int k = k0 - ADI_CH - e;
+      const long long q = o + (long long)k * st;
+      nu[e] = k >= 0 ? U[q] : 0.0;
+      ng[e] = k >= 0 ? G[q + st] : 0.0;
+    }
+#pragma unroll
+    for (int e = 0; e < ADI_CH; e++) {
+      const int k = k0 - e;
+      if (k >= 0) {
+        uprev = u[e] - g[e] * uprev;
+        U[o + (long long)k * st] = uprev;
+      }
+    }
+  }
+  return true;
+}
+
+// x pass: one thread per row j = 1..ny-2, writes zint (all columns) for those rows
+__global__ __launch_bounds__(64) void k_adi_x(const InstParams* __restrict__ PP, const double* __restrict__ dg,
+                                              const double* __restrict__ rf, const double* __restrict__ cx,
+                                              double* __restrict__ zint, double* __restrict__ gam, int* __restrict__ err) {
   const int inst = blockIdx.y;
   const InstParams& P = PP[inst];
   const int j = 1 + blockIdx.x * 64 + threadIdx.x;
   if (!P.active || !P.run_diff || j > P.ny - 2) return;
-  const int nx = P.nx;
-  const double* Z = h + P.off;
-  const double* K = kdint + P.off;
-  double* U = zint + P.off;
-  double* G = gam + P.off;
-  const double hdt = P.dt / 2.0, dx2 = P.dx * P.dx, dy2 = P.dy * P.dy;
-  const int r0 = j * nx, rm = (j - 1) * nx, rp = (j + 1) * nx;
-  double diag, sup, inf, f, fxp, fxm, fyp, fym;
-  // i = 0 (left boundary)
-  if (P.fix_left) { diag = 1.0; sup = 0.0; f = Z[r0]; }
-  else {
-    fxp = coef(K[r0 + 1], K[r0], hdt, dx2);
-    fyp = coef(K[rp], K[r0], hdt, dy2);
-    fym = coef(K[rm], K[r0], hdt, dy2);
-    diag = 1.0 + fxp;
-    sup = -fxp;
-    f = Z[r0] + fyp * Z[rp] - (fyp + fym) * Z[r0] + fym * Z[rm];
-  }
-  if (diag == 0.0) { atomicOr(err + inst, FSX_ERR_TRIDAG); return; }
-  double bet = diag, uprev = f / bet, cprev = sup;
-  U[r0] = uprev;
-  for (int i = 1; i < nx; i++) {
-    if (i < nx - 1) {
-      fxp = coef(K[r0 + i + 1], K[r0 + i], hdt, dx2);
-      fxm = coef(K[r0 + i - 1], K[r0 + i], hdt, dx2);
-      fyp = coef(K[rp + i], K[r0 + i], hdt, dy2);
-      fym = coef(K[rm + i], K[r0 + i], hdt, dy2);
-      diag = 1.0 + fxp + fxm;
-      sup = -fxp;
-      inf = -fxm;
-      f = Z[r0 + i] + fyp * Z[rp + i] - (fyp + fym) * Z[r0 + i] + fym * Z[rm + i];
-    } else if (P.fix_right) {
-      diag = 1.0; inf = 0.0; f = Z[r0 + i]; sup = 0.0;
-    } else {
-      fxm = coef(K[r0 + i - 1], K[r0 + i], hdt, dx2);
-      fyp = coef(K[rp + i], K[r0 + i], hdt, dy2);
-      fym = coef(K[rm + i], K[r0 + i], hdt, dy2);
-      diag = 1.0 + fxm;
-      inf = -fxm;
-      f = Z[r0 + i] + fyp * Z[rp + i] - (fyp + fym) * Z[r0 + i] + fym * Z[rm + i];
-      sup = 0.0;
-    }
-    const double gm = cprev / bet;
-    bet = diag - inf * gm;
-    if (bet == 0.0) { atomicOr(err + inst, FSX_ERR_TRIDAG); return; }
-    uprev = (f - inf * uprev) / bet;
-    G[r0 + i] = gm;
-    U[r0 + i] = uprev;
-    cprev = sup;
-  }
-  for (int i = nx - 2; i >= 0; i--) {
-    uprev = U[r0 + i] - G[r0 + i + 1] * uprev;
-    U[r0 + i] = uprev;
-  }
+  const long long o = (long long)P.off + (long long)j * P.nx;
+  if (!thomas_line(dg, rf, cx, zint, gam, o, 1, P.nx, P.fix_left, P.fix_right)) atomicOr(err + inst, FSX_ERR_TRIDAG);
 }
 
-// y pass: columns i = 1..nx-2; reads zint (rows 1..ny-2) and h (rows 0 and ny-1, untouched by
-// the x pass), writes zintp for those columns.
-__global__ __launch_bounds__(64) void k_adi_y(const InstParams* __restrict__ PP, const double* __restrict__ h,
-                                              const double* __restrict__ zint, const double* __restrict__ kdint,
+// y pass: one thread per column i = 1..nx-2, writes zintp for those columns
+__global__ __launch_bounds__(64) void k_adi_y(const InstParams* __restrict__ PP, const double* __restrict__ dg,
+                                              const double* __restrict__ rf, const double* __restrict__ cy,
                                               double* __restrict__ zintp, double* __restrict__ gam, int* __restrict__ err) {
   const int inst = blockIdx.y;
   const InstParams& P = PP[inst];
   const int i = 1 + blockIdx.x * 64 + threadIdx.x;
   if (!P.active || !P.run_diff || i > P.nx - 2) return;
-  const int nx = P.nx, ny = P.ny;
-  const double* H = h + P.off;
-  const double* ZI = zint + P.off;
-  const double* K = kdint + P.off;
-  double* U = zintp + P.off;
-  double* G = gam + P.off;
-  const double hdt = P.dt / 2.0, dx2 = P.dx * P.dx, dy2 = P.dy * P.dy;
-  auto Z = [&](int ii, int jj) -> double { return (jj == 0 || jj == ny - 1) ? H[jj * nx + ii] : ZI[jj * nx + ii]; };
-  const int jst = ny - 1;  // 0-based value of the row index left over by "do j=2,ny-1" (ny >= 3)
-  double diag, sup, inf, f, fxp, fxm, fyp, fym;
-  if (P.fix_bottom) { diag = 1.0; sup = 0.0; f = Z(i, 0); }
-  else {
-    fxp = coef(K[i + 1], K[jst * nx + i], hdt, dx2);  // stale j, src/Diffusion.f90:114
-    fxm = coef(K[i - 1], K[i], hdt, dx2);
-    fyp = coef(K[nx + i], K[i], hdt, dy2);
-    diag = 1.0 + fyp;
-    sup = -fyp;
-    f = Z(i, 0) + fxp * Z(i + 1, 0) - (fxp + fxm) * Z(i, 0) + fxm * Z(i - 1, 0);
-  }
-  if (diag == 0.0) { atomicOr(err + inst, FSX_ERR_TRIDAG); return; }
-  double bet = diag, uprev = f / bet, cprev = sup;
-  U[i] = uprev;
-  for (int j = 1; j < ny; j++) {
-    const int o = j * nx + i;
-    if (j < ny - 1) {
-      fxp = coef(K[o + 1], K[o], hdt, dx2);
-      fxm = coef(K[o - 1], K[o], hdt, dx2);
-      fyp = coef(K[o + nx], K[o], hdt, dy2);
-      fym = coef(K[o - nx], K[o], hdt, dy2);
-      diag = 1.0 + fyp + fym;
-      sup = -fyp;
-      inf = -fym;
-      f = Z(i, j) + fxp * Z(i + 1, j) - (fxp + fxm) * Z(i, j) + fxm * Z(i - 1, j);
-    } else if (P.fix_top) {
-      diag = 1.0; inf = 0.0; f = Z(i, j); sup = 0.0;
-    } else {
-      fxp = coef(K[o + 1], K[o], hdt, dx2);
-      fxm = coef(K[o - 1], K[o], hdt, dx2);
-      fym = coef(K[o - nx], K[o], hdt, dy2);
-      diag = 1.0 + fym;
-      inf = -fym;
-      f = Z(i, j) + fxp * Z(i + 1, j) - (fxp + fxm) * Z(i, j) + fxm * Z(i - 1, j);
-      sup = 0.0;
-    }
-    const double gm = cprev / bet;
-    bet = diag - inf * gm;
-    if (bet == 0.0) { atomicOr(err + inst, FSX_ERR_TRIDAG); return; }
-    uprev = (f - inf * uprev) / bet;
-    G[o] = gm;
-    U[o] = uprev;
-    cprev = sup;
-  }
-  for (int j = ny - 2; j >= 0; j--) {
-    const int o = j * nx + i;
-    uprev = U[o] - G[o + nx] * uprev;
-    U[o] = uprev;
-  }
+  const long long o = (long long)P.off + i;
+  if (!thomas_line(dg, rf, cy, zintp, gam, o, P.nx, P.ny, P.fix_bottom, P.fix_top)) atomicOr(err + inst, FSX_ERR_TRIDAG);
 }
 
 // Diffusion output, src/Diffusion.f90:143-152 (columns 0 and nx-1 keep h: zintp==h there).
@@ -309,12 +380,16 @@ void launch_spl_prep(DevState& S, hipStream_t st) {
 }
 void launch_spl_post(DevState& S, hipStream_t st) {
   k_spl_post<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.h, S.b, S.ht, S.etot, S.erate, S.sedflux, S.lsill,
-                                               S.lwv, S.lsed, S.water, S.scal);
+                                               S.lwv, S.lsed, S.water, S.scal, S.pos, S.Hl, S.HWl, S.hwater,
+                                               S.lake_depth);
 }
 void launch_diffusion(DevState& S, hipStream_t st) {
   k_adi_prep<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.h, S.b, S.kd, S.kdint);
+  k_adi_coef<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.kdint, S.adx, S.ady);
+  k_adi_xrhs<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.h, S.adx, S.ady, S.adg, S.adf);
   dim3 gx((S.max_ny + 63) / 64, S.ninst), gy((S.max_nx + 63) / 64, S.ninst);
-  k_adi_x<<<gx, 64, 0, st>>>(S.P, S.h, S.kdint, S.zint, S.gam, S.err);
-  k_adi_y<<<gy, 64, 0, st>>>(S.P, S.h, S.zint, S.kdint, S.zintp, S.gam, S.err);
+  k_adi_x<<<gx, 64, 0, st>>>(S.P, S.adg, S.adf, S.adx, S.zint, S.gam, S.err);
+  k_adi_yrhs<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.h, S.zint, S.kdint, S.adx, S.ady, S.adg, S.adf);
+  k_adi_y<<<gy, 64, 0, st>>>(S.P, S.adg, S.adf, S.ady, S.zintp, S.gam, S.err);
   k_adi_post<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.h, S.b, S.etot, S.erate, S.zintp);
 }

@@ -69,6 +69,16 @@ __device__ __forceinline__ int nbr_wc(const InstParams& P, int i, int j, int ii,
 // donors of v are exactly the q in this set with rec[q] == v.  Returns the count; q[] and dir[].
 __device__ __forceinline__ int neighbours_wc(const InstParams& P, int v, int* q, int* dir) {
   int i = v % P.nx, j = v / P.nx, c = 0;
+  if (i > 0 && j > 0 && i < P.nx - 1 && j < P.ny - 1) {  // interior: no wrap, no clamp, no duplicates
+#pragma unroll
+    for (int d = 0; d < 9; d++) {
+      if (d == 4) continue;
+      q[c] = v + (d % 3 - 1) + (d / 3 - 1) * P.nx;
+      dir[c] = d;
+      c++;
+    }
+    return c;
+  }
   for (int jj = -1; jj <= 1; jj++)
     for (int ii = -1; ii <= 1; ii++) {
       if (ii == 0 && jj == 0) continue;
@@ -90,6 +100,17 @@ __device__ __forceinline__ int dbit(int d) { return d < 4 ? d : d - 1; }
 // Donors of v in ascending local index (the reference's donor order, src/FlowRouting.f90:379-385).
 __device__ __forceinline__ int donors_sorted(const InstParams& P, int v, unsigned mask, int* d) {
   if (!mask) return 0;
+  const int i = v % P.nx, j = v / P.nx;
+  if (i > 0 && j > 0 && i < P.nx - 1 && j < P.ny - 1) {  // interior: bit order is ascending index order
+    int nd = 0;
+#pragma unroll
+    for (int b = 0; b < 8; b++)
+      if (mask & (1u << b)) {
+        const int dd = b < 4 ? b : b + 1;
+        d[nd++] = v + (dd % 3 - 1) + (dd / 3 - 1) * P.nx;
+      }
+    return nd;
+  }
   int q[8], dir[8];
   int c = neighbours_wc(P, v, q, dir), nd = 0;
   for (int t = 0; t < c; t++)
@@ -120,10 +141,14 @@ __device__ __forceinline__ void npow(double x, double n, int kind, double& pn, d
   }
 }
 
-// order-preserving 64-bit image of a double (for atomic min on doubles)
+// order-preserving 64-bit image of a double (for atomic min/max on doubles); 0 is below every image
 __device__ __forceinline__ unsigned long long dkey(double x) {
   unsigned long long u = (unsigned long long)__double_as_longlong(x);
   return (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double dval(unsigned long long k) {
+  const unsigned long long u = (k & 0x8000000000000000ull) ? (k & ~0x8000000000000000ull) : ~k;
+  return __longlong_as_double((long long)u);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -156,6 +181,7 @@ void launch_diffusion(DevState& S, hipStream_t st);
 void launch_mfd_route(DevState& S, hipStream_t st);
 void launch_mfd_spl(DevState& S, hipStream_t st);
 int sweep_set_lds(int smem_max);
+int basins_set_lds(int smem_max);
 void launch_fill(DevState& S, hipStream_t st);
 
 // Phases timed with HIP events when profiling is on (the reference times Uplift, SPL and Diffusion

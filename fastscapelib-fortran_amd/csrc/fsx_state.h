@@ -17,6 +17,7 @@ struct DevState {
   double *lake_depth = nullptr, *sedflux = nullptr, *ht = nullptr, *water = nullptr, *zint = nullptr;
   double *zintp = nullptr, *gam = nullptr, *dh = nullptr, *hp = nullptr, *lwv = nullptr, *lsed = nullptr;
   double *kfint = nullptr, *kdint = nullptr, *g = nullptr, *pexp = nullptr, *elev = nullptr;
+  double *adx = nullptr, *ady = nullptr, *adg = nullptr, *adf = nullptr;  // ADI couplings, diagonal, rhs
   int *rec = nullptr, *basin = nullptr, *rootrank = nullptr, *order = nullptr;
   int *lstart = nullptr, *lsill = nullptr;
   uint8_t *dmask = nullptr, *bc = nullptr;
@@ -62,6 +63,7 @@ struct DevState {
   int* nclosed = nullptr;  // level groups of the sweeps (node offsets + instance, nn+1 per landscape)
   long long *newton = nullptr;
   double *scal = nullptr;      // per-instance scalar scratch (tolerances, GS error)
+  unsigned long long *redk = nullptr;  // per-instance order-preserving maxima: maxval(abs(h)), maxval(g)
   int *flags = nullptr;        // convergence flags of iterative kernels (64)
   int *nsegs = nullptr, *segctr = nullptr;  // edge-sort segment lists [ninst]
 };

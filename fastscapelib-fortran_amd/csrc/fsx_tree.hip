@@ -60,6 +60,7 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_jump(const InstParams* __restrict
 // (so a group has at most GRP_BUCKET levels and GRP_STG nodes; a level wider than GRP_HALF forms its
 // own group); the sweeps stage one group at a time into LDS.
 #define LVL_HCAP 32768
+#define LVL_UNROLL 8
 #define GRP_HALF 1024                // a level wider than this is a group of its own
 #define GRP_BUCKET 1024              // levels starting in one bucket of positions share a group
 #define GRP_STG (GRP_HALF + GRP_BUCKET)  // positions of a group (staging capacity)
@@ -92,11 +93,20 @@ __global__ __launch_bounds__(FSX_WG) void k_level_sort(const InstParams* __restr
     if (flags[25]) atomicOr(err + inst, FSX_ERR_LEVELS);  // jumping did not converge: receiver cycle
   }
   __syncthreads();
+  // the node loops take LVL_UNROLL nodes per thread at a time so their loads overlap
   int mx = 0;
-  for (int v = tid; v < nn; v += FSX_WG) {
-    int d = jdist(J[off + v]);
-    d = d < 0 ? 0 : (d >= nn ? nn - 1 : d);
-    mx = d > mx ? d : mx;
+  for (int v0 = tid; v0 < nn; v0 += LVL_UNROLL * FSX_WG) {
+    int dd[LVL_UNROLL];
+#pragma unroll
+    for (int u = 0; u < LVL_UNROLL; u++) {
+      const int v = v0 + u * FSX_WG;
+      dd[u] = v < nn ? jdist(J[off + v]) : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < LVL_UNROLL; u++) {
+      const int d = dd[u] < 0 ? 0 : (dd[u] >= nn ? nn - 1 : dd[u]);
+      mx = d > mx ? d : mx;
+    }
   }
   for (int o = 32; o > 0; o >>= 1) { const int y = __shfl_xor(mx, o, 64); mx = y > mx ? y : mx; }
   if ((tid & 63) == 0) atomicMax(&s_max, mx);
@@ -106,10 +116,19 @@ __global__ __launch_bounds__(FSX_WG) void k_level_sort(const InstParams* __restr
   for (int l = tid; l < LH; l += FSX_WG) hist[l] = 0;
   for (int l = LVL_HCAP + tid; l < L; l += FSX_WG) lc[l] = 0;
   __syncthreads();
-  for (int v = tid; v < nn; v += FSX_WG) {
-    int d = jdist(J[off + v]);
-    d = d < 0 ? 0 : (d >= nn ? nn - 1 : d);
-    if (d < LVL_HCAP) atomicAdd(hist + d, 1); else atomicAdd(lc + d, 1);
+  for (int v0 = tid; v0 < nn; v0 += LVL_UNROLL * FSX_WG) {
+    int dd[LVL_UNROLL];
+#pragma unroll
+    for (int u = 0; u < LVL_UNROLL; u++) {
+      const int v = v0 + u * FSX_WG;
+      dd[u] = v < nn ? jdist(J[off + v]) : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < LVL_UNROLL; u++) {
+      if (v0 + u * FSX_WG >= nn) continue;
+      const int d = dd[u] < 0 ? 0 : (dd[u] >= nn ? nn - 1 : dd[u]);
+      if (d < LVL_HCAP) atomicAdd(hist + d, 1); else atomicAdd(lc + d, 1);
+    }
   }
   __syncthreads();
   int carry = 0;
@@ -130,13 +149,23 @@ __global__ __launch_bounds__(FSX_WG) void k_level_sort(const InstParams* __restr
     nlev[inst] = L;
   }
   __syncthreads();
-  for (int v = tid; v < nn; v += FSX_WG) {
-    int d = jdist(J[off + v]);
-    d = d < 0 ? 0 : (d >= nn ? nn - 1 : d);
-    const int p = d < LVL_HCAP ? atomicAdd(hist + d, 1) : atomicAdd(lc + d, 1);
-    if (p < nn) {
-      order[off + p] = v;
-      pos[off + v] = p;
+  for (int v0 = tid; v0 < nn; v0 += LVL_UNROLL * FSX_WG) {
+    int dd[LVL_UNROLL];
+#pragma unroll
+    for (int u = 0; u < LVL_UNROLL; u++) {
+      const int v = v0 + u * FSX_WG;
+      dd[u] = v < nn ? jdist(J[off + v]) : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < LVL_UNROLL; u++) {
+      const int v = v0 + u * FSX_WG;
+      if (v >= nn) continue;
+      const int d = dd[u] < 0 ? 0 : (dd[u] >= nn ? nn - 1 : dd[u]);
+      const int p = d < LVL_HCAP ? atomicAdd(hist + d, 1) : atomicAdd(lc + d, 1);
+      if (p < nn) {
+        order[off + p] = v;
+        pos[off + v] = p;
+      }
     }
   }
   __syncthreads();
@@ -204,24 +233,58 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_area_prep(const InstParams* __res
 __global__ __launch_bounds__(FSX_CHUNK) void k_spl_fact(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
                                                         const unsigned long long* __restrict__ J, const int* __restrict__ rec,
                                                         const int* __restrict__ pos, const int* __restrict__ lstart,
-                                                        const double* __restrict__ area, const double* __restrict__ length,
+                                                        const double* __restrict__ Al, double* __restrict__ area,
+                                                        const double* __restrict__ length,
                                                         const double* __restrict__ kfint, const double* __restrict__ ht,
-                                                        int* __restrict__ rp, double* __restrict__ F, double* __restrict__ HTl) {
-  NODE_PROLOGUE
-  if (!P.sfd) return;
+                                                        int* __restrict__ rp, double* __restrict__ F, double* __restrict__ HTl,
+                                                        double* __restrict__ Hl, int* __restrict__ LSl,
+                                                        double* __restrict__ lsed, const double* __restrict__ g,
+                                                        unsigned long long* __restrict__ redk) {
+  __shared__ unsigned long long smax[2][FSX_CHUNK / 64];
+  const Chunk c = chunks[blockIdx.x];
+  const InstParams& P = PP[c.inst];
+  const int v = c.start + (int)threadIdx.x;
+  if (!P.active || !P.run_spl || !P.sfd) return;  // uniform over the block: one chunk, one landscape
+  const bool live = v < P.nn;
+  const long long g_ = (long long)P.off + v;
+  // the landscape's maxval(abs(h)) and maxval(g) (src/StreamPowerLaw.f90:246,376), h == ht here:
+  // reduced over the block, then one atomic per block
+  unsigned long long kh = live ? dkey(fabs(ht[g_])) : 0ull, kg = live ? dkey(g[g_]) : 0ull;
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long a = __shfl_xor(kh, o, 64), b = __shfl_xor(kg, o, 64);
+    kh = a > kh ? a : kh;
+    kg = b > kg ? b : kg;
+  }
+  if ((threadIdx.x & 63) == 0) { smax[0][threadIdx.x >> 6] = kh; smax[1][threadIdx.x >> 6] = kg; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < FSX_CHUNK / 64; w++) {
+      kh = smax[0][w] > kh ? smax[0][w] : kh;
+      kg = smax[1][w] > kg ? smax[1][w] : kg;
+    }
+    atomicMax(redk + 2 * c.inst, kh);
+    atomicMax(redk + 2 * c.inst + 1, kg);
+  }
+  if (!live) return;
   const int p = pos[g_];
   const int d = jdist(J[g_]);
   const long long gp = (long long)P.off + p;
   const int r = rec[g_];
   const double t = ht[g_];
+  const double av = Al[gp];  // drainage area, level order -> grid order
+  area[g_] = av;
   HTl[gp] = t;
+  // Gauss-Seidel state: h in level order, no lake sill, no lake sediment
+  Hl[gp] = t;
+  LSl[gp] = -1;
+  lsed[g_] = 0.0;
   if (d == 0) {
     rp[gp] = -1;
     F[gp] = 0.0;
     return;
   }
   rp[gp] = pos[P.off + r] - lstart[P.aux + d - 1];
-  const double am = pow(area[g_], P.m);
+  const double am = pow(av, P.m);
   const double n = P.n;
   if (fabs(n - 1.0) < 2.2250738585072014e-308) {
     F[gp] = kfint[g_] * P.dt * am / length[g_];
@@ -249,7 +312,6 @@ struct GroupMeta {
 struct AreaRec {
   double a;
   int d[8];
-  int o;
 };
 
 __device__ __forceinline__ void area_load(AreaRec& r, long long gp, const double* A0, const int* dn, const int* order) {
@@ -258,7 +320,6 @@ __device__ __forceinline__ void area_load(AreaRec& r, long long gp, const double
   const int4 x = q[0], y = q[1];
   r.d[0] = x.x; r.d[1] = x.y; r.d[2] = x.z; r.d[3] = x.w;
   r.d[4] = y.x; r.d[5] = y.y; r.d[6] = y.z; r.d[7] = y.w;
-  r.o = order[gp];
 }
 
 __global__ __launch_bounds__(FSX_WG) void k_area_sweep(const InstParams* __restrict__ PP, const int* __restrict__ lstart,
@@ -308,14 +369,12 @@ __global__ __launch_bounds__(FSX_WG) void k_area_sweep(const InstParams* __restr
         sA[tid] = r0.a;
 #pragma unroll
         for (int k = 0; k < 8; k++) sD[8 * tid + k] = r0.d[k];
-        sO[tid] = r0.o;
       }
       if (tid + FSX_WG < np) {
         const int t = tid + FSX_WG;
         sA[t] = r1.a;
 #pragma unroll
         for (int k = 0; k < 8; k++) sD[8 * t + k] = r1.d[k];
-        sO[t] = r1.o;
       }
       if (tid < cur.hi - cur.lo) sL[tid] = rl;
       if (tid == 0) sL[cur.hi - cur.lo] = cur.p1;
@@ -337,20 +396,18 @@ __global__ __launch_bounds__(FSX_WG) void k_area_sweep(const InstParams* __restr
         constexpr bool FAST = decltype(fast_tag)::value;
         for (int i = tid; i < w; i += FSX_WG) {
           double a;
-          int q[8], o;
+          int q[8];
           if constexpr (FAST) {
             const int si = b0 - cur.p0 + i;
             a = sA[si];
 #pragma unroll
             for (int k = 0; k < 8; k++) q[k] = sD[8 * si + k];
-            o = sO[si];
           } else {
             AreaRec r;
             area_load(r, off + b0 + i, A0, dn, order);
             a = r.a;
 #pragma unroll
             for (int k = 0; k < 8; k++) q[k] = r.d[k];
-            o = r.o;
           }
           // unused slots add an exact +0.0, so the sum keeps the reference's order
           double dv[8];
@@ -364,7 +421,6 @@ __global__ __launch_bounds__(FSX_WG) void k_area_sweep(const InstParams* __restr
           if (l == 0) a = a + a;
           if (w <= vcap) cv[i] = a;
           Al[off + b0 + i] = a;
-          area[off + o] = a;
         }
       };
       if (staged && nin) body(std::true_type{});
@@ -406,7 +462,8 @@ __global__ __launch_bounds__(FSX_WG) void k_spl_sweep(
     double* __restrict__ water, int* __restrict__ lsill, double* __restrict__ lwv, double* __restrict__ lsed,
     double* __restrict__ Hl, double* __restrict__ Wl, double* __restrict__ HWl, int* __restrict__ LSl,
     double* __restrict__ HPl, double* __restrict__ DHl, double* __restrict__ ELl, double* __restrict__ scal,
-    int* __restrict__ ngs_out, long long* __restrict__ newton_out, int* __restrict__ err, int vcap, int fill_only) {
+    int* __restrict__ ngs_out, long long* __restrict__ newton_out, int* __restrict__ err, int vcap, int fill_only,
+    const unsigned long long* __restrict__ redk) {
   extern __shared__ double sm[];
   __shared__ double red[FSX_WG / 64];
   __shared__ double s_tol, s_err, s_gmax;
@@ -436,37 +493,13 @@ __global__ __launch_bounds__(FSX_WG) void k_spl_sweep(
   const bool n_is_one = fabs(n - 1.0) < 2.2250738585072014e-308;
   const int nkind = npow_kind(n);
 
-  // tol = 1e-4*(maxval(abs(h))+1), gmax = maxval(g); level-ordered working copy of h
-  double hm = 0.0, gmx = -1.7976931348623157e308;
-  for (int p = tid; p < nn; p += FSX_WG) {
-    const int v = order[off + p];
-    const double hv = h[off + v];
-    hm = fmax(hm, fabs(hv));
-    gmx = fmax(gmx, g[off + v]);
-    Hl[off + p] = hv;
-    LSl[off + p] = -1;
-    lsed[off + v] = 0.0;
-  }
-  for (int o = 32; o > 0; o >>= 1) {
-    hm = fmax(hm, __shfl_xor(hm, o, 64));
-    gmx = fmax(gmx, __shfl_xor(gmx, o, 64));
-  }
-  if (lane == 0) red[wid] = hm;
-  __syncthreads();
+  // tol = 1e-4*(maxval(abs(h))+1), gmax = maxval(g), reduced by k_spl_fact, which also set up the
+  // level-ordered working copy of h
   if (tid == 0) {
-    double x = red[0];
-    for (int q = 1; q < FSX_WG / 64; q++) x = fmax(x, red[q]);
-    s_tol = 1.0e-4 * (x + 1.0);
-    s_newton = 0;
-  }
-  __syncthreads();
-  if (lane == 0) red[wid] = gmx;
-  __syncthreads();
-  if (tid == 0) {
-    double x = red[0];
-    for (int q = 1; q < FSX_WG / 64; q++) x = fmax(x, red[q]);
-    s_gmax = x;
+    s_tol = 1.0e-4 * (dval(redk[2 * inst]) + 1.0);
+    s_gmax = dval(redk[2 * inst + 1]);
     s_err = 2.0 * s_tol;
+    s_newton = 0;
   }
   __syncthreads();
   const bool gzero = s_gmax < 2.2250738585072014e-308;
@@ -580,7 +613,6 @@ __global__ __launch_bounds__(FSX_WG) void k_spl_sweep(
             if (l == 0) {
               wv = t;
               if (!gzero && !fill_only) lwv[off + v] = 0.0;
-              if (first) lake_depth[off + v] = 0.0;
             } else {
               const int q = rr.q;
               double hr, w_rcv;
@@ -591,7 +623,6 @@ __global__ __launch_bounds__(FSX_WG) void k_spl_sweep(
                 if constexpr (FAST) hwr = BHW(pb)[q];
                 else hwr = pin ? BHW(pb)[q] : HWl[off + bp + q];
                 hw = hwr > t ? hwr + 1.0e-8 : t;
-                lake_depth[off + v] = hw - t;
               }
               if (fill_only) {
                 wv = w_rcv;
@@ -649,19 +680,17 @@ __global__ __launch_bounds__(FSX_WG) void k_spl_sweep(
             // and the Gauss-Seidel iterations; grid-ordered outputs h, hwater, lake_depth always
             const long long gp = off + b0 + i;
             const bool wide = w > vcap;
+            // results stay in level order (coalesced stores); k_spl_post scatters h, hwater and
+            // lake_depth to grid order
             if (!fill_only) {
-              if (wide || !gzero || !n_is_one) Hl[gp] = hv;
+              Hl[gp] = hv;
               if (wide) Wl[gp] = wv;
               if (wide || !gzero) LSl[gp] = sl;
-              h[off + v] = hv;
               if (!gzero) { water[off + v] = wv; lsill[off + v] = sl; }
             } else if (wide) {
               Wl[gp] = wv;
             }
-            if (first) {
-              if (wide) HWl[gp] = hw;
-              hwater[off + v] = hw;
-            }
+            if (first) HWl[gp] = hw;
           }
         };
         if (staged && pin && gzero) body(std::true_type{});
@@ -753,15 +782,16 @@ void launch_area_sweep(DevState& S, hipStream_t st) {
 }
 
 void launch_spl_fact(DevState& S, hipStream_t st) {
-  k_spl_fact<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.J, S.rec, S.pos, S.lstart, S.area, S.length, S.kfint,
-                                               S.ht, S.rp, S.F, S.HTl);
+  launch_fill_int(reinterpret_cast<int*>(S.redk), 4LL * S.ninst, 0, st);
+  k_spl_fact<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.J, S.rec, S.pos, S.lstart, S.Al, S.area, S.length,
+                                               S.kfint, S.ht, S.rp, S.F, S.HTl, S.Hl, S.LSl, S.lsed, S.g, S.redk);
 }
 
 static void spl_sweep_launch(DevState& S, hipStream_t st, int fill_only) {
   k_spl_sweep<<<S.ninst, FSX_WG, spl_smem(kSplVcap), st>>>(
       S.P, S.lstart, S.nlev, S.grec, S.ngroups, S.order, S.rp, S.F, S.HTl, S.dn, S.bc, S.g, S.Al, S.h, S.hwater,
       S.lake_depth, S.water, S.lsill, S.lwv, S.lsed, S.Hl, S.Wl, S.HWl, S.LSl, S.HPl, S.DHl, S.ELl, S.scal, S.ngs,
-      S.newton, S.err, kSplVcap, fill_only);
+      S.newton, S.err, kSplVcap, fill_only, S.redk);
 }
 
 void launch_spl_sweep(DevState& S, hipStream_t st) { spl_sweep_launch(S, st, 0); }
@@ -769,7 +799,20 @@ void launch_spl(DevState& S, hipStream_t st) {
   launch_spl_fact(S, st);
   spl_sweep_launch(S, st, 0);
 }
+// lake surface and depth of a routing-only pass, back to grid order
+__global__ __launch_bounds__(FSX_CHUNK) void k_fill_post(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
+                                                         const int* __restrict__ pos, const double* __restrict__ HWl,
+                                                         const double* __restrict__ ht, double* __restrict__ hwater,
+                                                         double* __restrict__ lake_depth) {
+  NODE_PROLOGUE
+  if (!P.sfd) return;
+  const double hw = HWl[(long long)P.off + pos[g_]];
+  hwater[g_] = hw;
+  lake_depth[g_] = hw - ht[g_];
+}
+
 void launch_fill(DevState& S, hipStream_t st) {
   launch_spl_fact(S, st);
   spl_sweep_launch(S, st, 1);
+  k_fill_post<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.pos, S.HWl, S.ht, S.hwater, S.lake_depth);
 }
