@@ -190,6 +190,38 @@ def run_gpu(fs, Ls, steps, warmup, profile):
     return dt, timers, stats, mem
 
 
+def run_headline(fs, Ls, steps, warmup, prof_steps):
+    """The headline measurement.  A profiled window (every phase bracketed by HIP events, kernels
+    launched one by one) gives the per-phase breakdown and names the dominant single-kernel phase;
+    the timed region then replays each step as captured HIP graphs, with that one phase launched
+    between its own HIP events on the step's stream, so its mean duration is measured live."""
+    with fs.Batch([(L["nx"], L["ny"]) for L in Ls]) as b:
+        for k, L in enumerate(Ls):
+            configure(b, L, k)
+        if warmup:
+            b.step(warmup)
+        b.profiling(True)
+        b.reset_timers()
+        b.step(prof_steps)
+        prof = b.timers()
+        b.profiling(False)
+        stats = [b.stats(k) for k in range(len(Ls))]
+        dom = dominant_phase(prof, stats, sum(L["nx"] * L["ny"] for L in Ls))
+        b.time_phase(dom)
+        b.step(1)  # captures the graphs
+        b.reset_timers()
+        _sync()
+        t0 = time.perf_counter()
+        b.step(steps)
+        _sync()
+        dt = time.perf_counter() - t0
+        live = b.timers()
+        b.time_phase(None)
+        stats = [b.stats(k) for k in range(len(Ls))]
+        mem = b.memory()
+    return dt, prof, live, dom, stats, mem
+
+
 def plain_rate(fs, Ls, steps):
     """One landscape at a time (each after one untimed step): cell-steps/s over the sample."""
     cells, secs = 0, 0.0
@@ -281,23 +313,29 @@ def cpu_baseline(Ls, steps, procs):
                       f"makespan {makespan:.1f} s, per-process {min(times):.1f}-{max(times):.1f} s"}
 
 
-def roofline(timers, stats, ncells):
-    """Dominant single-kernel phase by time; achieved = algorithmic bytes per launch / mean launch time."""
+def _alg_bytes(ph, stats, ncells):
+    if ph in ALG_BYTES_PER_NODE:
+        return ALG_BYTES_PER_NODE[ph] * ncells
+    if ph in ALG_BYTES_PER_EDGE:
+        return ALG_BYTES_PER_EDGE[ph] * sum(s["nconn"] for s in stats)
+    return None
+
+
+def dominant_phase(timers, stats, ncells):
+    """The single-kernel phase with the largest time (phases with an algorithmic-bytes figure)."""
     best = None
     for ph, (ms, cnt) in timers.items():
-        if cnt == 0:
-            continue
-        if ph in ALG_BYTES_PER_NODE:
-            bytes_per_launch = ALG_BYTES_PER_NODE[ph] * ncells
-        elif ph in ALG_BYTES_PER_EDGE:
-            bytes_per_launch = ALG_BYTES_PER_EDGE[ph] * sum(s["nconn"] for s in stats)
-        else:
-            continue
-        launches = cnt * (2 if ph == "donors" else 1)
-        avg_ms = ms / launches
-        if best is None or ms > best[1]:
-            best = (ph, ms, avg_ms, bytes_per_launch)
-    ph, _, avg_ms, nbytes = best
+        if cnt and _alg_bytes(ph, stats, ncells) is not None and (best is None or ms > best[1]):
+            best = (ph, ms)
+    return best[0]
+
+
+def roofline(ph, timers, stats, ncells):
+    """achieved = algorithmic bytes per launch / mean launch time of phase ph."""
+    ms, cnt = timers[ph]
+    launches = cnt * (2 if ph == "donors" else 1)  # the donor phase launches its kernel twice a step
+    avg_ms = ms / launches
+    nbytes = _alg_bytes(ph, stats, ncells)
     achieved = nbytes / (avg_ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": ph,
@@ -309,6 +347,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--prof-steps", type=int, default=10)
     ap.add_argument("--ninst", type=int, default=64)
     ap.add_argument("--n", type=int, default=513)
     ap.add_argument("--plain-steps", type=int, default=40)
@@ -331,7 +370,7 @@ def main():
 
     Ls = uniform_batch(args.ninst, args.n)
     ncells = sum(L["nx"] * L["ny"] for L in Ls)
-    dt, timers, stats, mem = run_gpu(fs, Ls, args.steps, args.warmup, profile=True)
+    dt, timers, live, dom, stats, mem = run_headline(fs, Ls, args.steps, args.warmup, args.prof_steps)
     value = ncells * args.steps / dt
 
     # plain path: one landscape at a time (the first of the batch)
@@ -340,7 +379,7 @@ def main():
         pdt, _, _, _ = run_gpu(fs, Ls[:1], args.plain_steps, 1, profile=False)
         plain = Ls[0]["nx"] * Ls[0]["ny"] * args.plain_steps / pdt
 
-    phases = {p: round(ms / max(1, args.steps), 4) for p, (ms, c) in timers.items() if c}
+    phases = {p: round(ms / max(1, args.prof_steps), 4) for p, (ms, c) in timers.items() if c}
     out = {
         "metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
@@ -350,8 +389,9 @@ def main():
         "plain": ({"value": plain, "ms_per_step": pdt / args.plain_steps * 1e3,
                    "workload": f"1 x {args.n}x{args.n}, same parameters"} if plain else None),
         "batched_over_plain": value / plain if plain else None,
-        "roofline": roofline(timers, stats, ncells),
+        "roofline": roofline(dom, live, stats, ncells),
         "phase_ms_per_step": phases,
+        "phase_ms_per_step_note": f"profiled window of {args.prof_steps} steps, kernels launched one by one",
         "step_model_frac": round(336.0 * ncells / (dt / args.steps) / 1e9 / HBM_PEAK_GBS, 5),
         "hbm_bytes_resident": mem[0],
         "basins_per_landscape": int(np.mean([s["nbasins"] for s in stats])),

@@ -54,6 +54,16 @@ struct fsx_ctx {
   hipStream_t stream = nullptr;
   bool dirty = true;
   bool any_sfd = false, any_mfd = false;  // which flow-routing paths the active instances use
+  // one step captured as a HIP graph and replayed; rebuilt when the launch sequence changes
+  bool use_graph = true, graph_valid = false, g_sfd = false, g_mfd = false;
+  hipGraphExec_t gexec = nullptr;
+  // live timing of one phase inside graph replays: graph (phases before it), the phase launched
+  // directly between two events, graph (phases after it)
+  int timed_phase = -1, g_timed = -1, timed_seq = -1;
+  hipGraphExec_t gpre = nullptr, gpost = nullptr;
+  // emission window over the step's phase sequence (see PHASE)
+  int emit_seq = 0, emit_lo = 0, emit_hi = 1 << 30;
+  std::vector<int>* seq_log = nullptr;
   long long bytes = 0;
   // phase timers: a ring of (start, stop) event pairs, accumulated every kRing steps
   bool prof = false;
@@ -65,8 +75,8 @@ struct fsx_ctx {
 };
 static const int kRing = 64;
 
-static int flush_timers(fsx_ctx* c, hipStream_t st) {
-  if (!c->prof || c->ring_used == 0) return 0;
+static int flush_timers_any(fsx_ctx* c, hipStream_t st) {
+  if (c->ring_used == 0) return 0;
   HIPCHK(hipStreamSynchronize(st));
   for (int s = 0; s < c->ring_used; s++)
     for (int p = 0; p < PH_COUNT; p++) {
@@ -80,6 +90,11 @@ static int flush_timers(fsx_ctx* c, hipStream_t st) {
     }
   c->ring_used = 0;
   return 0;
+}
+
+static int flush_timers(fsx_ctx* c, hipStream_t st) {
+  if (!c->prof && c->timed_phase < 0) return 0;
+  return flush_timers_any(c, st);
 }
 
 struct PhaseMark {
@@ -97,10 +112,15 @@ struct PhaseMark {
     c->rec_mask[k] = 1;
   }
 };
-#define PHASE(ph, call)                    \
-  do {                                     \
-    PhaseMark pm_(c, st, ph);              \
-    call;                                  \
+// A step is a sequence of phases; only those whose sequence number lies in [emit_lo, emit_hi) are
+// launched (all, by default), and with seq_log set the sequence is only recorded.
+#define PHASE(ph, call)                                             \
+  do {                                                              \
+    const int s_ = c->emit_seq++;                                   \
+    if (c->seq_log) { c->seq_log->push_back(ph); break; }           \
+    if (s_ < c->emit_lo || s_ >= c->emit_hi) break;                 \
+    PhaseMark pm_(c, st, ph);                                       \
+    call;                                                           \
   } while (0)
 
 extern "C" const char* fsx_last_error(void) { return g_err.c_str(); }
@@ -241,6 +261,7 @@ static int relayout(fsx_ctx* c) {
       S.max_ny = std::max(S.max_ny, c->hp[k].ny);
     }
   c->dirty = true;
+  c->graph_valid = false;
   return 0;
 }
 
@@ -409,6 +430,9 @@ extern "C" int fsx_pool_info(fsx_ctx* c, long long* free_nodes, long long* large
 extern "C" int fsx_destroy(fsx_ctx* c) {
   if (!c) return 0;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->gpre) (void)hipGraphExecDestroy(c->gpre);
+  if (c->gpost) (void)hipGraphExecDestroy(c->gpost);
   for (auto& e : c->ev) (void)hipEventDestroy(e);
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -448,6 +472,7 @@ static int mfd_alloc(fsx_ctx* c) {
   HIPCHK(hipStreamSynchronize(c->stream));
   const long long N = S.ntot;
   int rc = 0;
+  c->graph_valid = false;
 #define A(p, n) if (!rc) rc = dalloc(c, &S.p, (n))
   A(mrec, 8 * N); A(mdon, 8 * N); A(mw, 8 * N); A(mfact, 8 * N); A(mdir, 8 * N); A(mflag, N);
   A(mnrec, N); A(mndon, N); A(mindeg, N); A(mstack, N); A(mls, N + S.ninst); A(mnlev, S.ninst);
@@ -628,6 +653,62 @@ static void enqueue_step(fsx_ctx* c, hipStream_t st) {
   PHASE(PH_DIFFUSION, launch_diffusion(S, st));
 }
 
+static int capture(fsx_ctx* c, hipStream_t st, int lo, int hi, hipGraphExec_t* out) {
+  hipGraph_t g = nullptr;
+  HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  c->emit_seq = 0;
+  c->emit_lo = lo;
+  c->emit_hi = hi;
+  enqueue_step(c, st);
+  c->emit_lo = 0;
+  c->emit_hi = 1 << 30;
+  const hipError_t ec = hipStreamEndCapture(st, &g);
+  if (ec != hipSuccess) return fail(FSX_EHIP, std::string("graph capture: ") + hipGetErrorString(ec));
+  const hipError_t ei = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ei != hipSuccess) return fail(FSX_EHIP, std::string("graph instantiate: ") + hipGetErrorString(ei));
+  return 0;
+}
+
+// One graph for the whole step, or (a phase timed live) the graphs before and after that phase.
+static int build_graphs(fsx_ctx* c, hipStream_t st) {
+  for (hipGraphExec_t* g : {&c->gexec, &c->gpre, &c->gpost})
+    if (*g) {
+      HIPCHK(hipGraphExecDestroy(*g));
+      *g = nullptr;
+    }
+  int sx = -1, nseq = 0;
+  if (c->timed_phase >= 0) {
+    std::vector<int> seq;
+    c->seq_log = &seq;
+    c->emit_seq = 0;
+    enqueue_step(c, st);
+    c->seq_log = nullptr;
+    nseq = (int)seq.size();
+    for (int i = 0; i < nseq && sx < 0; i++)
+      if (seq[i] == c->timed_phase) sx = i;
+  }
+  if (sx < 0) {
+    if (int r = capture(c, st, 0, 1 << 30, &c->gexec)) return r;
+  } else {
+    if (c->ev.empty()) {
+      c->ev.resize((size_t)kRing * PH_COUNT * 2);
+      for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
+      c->rec_mask.assign((size_t)kRing * PH_COUNT, 0);
+    }
+    if (sx > 0)
+      if (int r = capture(c, st, 0, sx, &c->gpre)) return r;
+    if (sx + 1 < nseq)
+      if (int r = capture(c, st, sx + 1, 1 << 30, &c->gpost)) return r;
+    c->timed_seq = sx;
+  }
+  c->graph_valid = true;
+  c->g_sfd = c->any_sfd;
+  c->g_mfd = c->any_mfd;
+  c->g_timed = c->timed_phase;
+  return 0;
+}
+
 extern "C" int fsx_execute_steps(fsx_ctx* c, int nsteps, void* stream, int sync) {
   if (!c) return fail(FSX_EINVAL, "null context");
   if (nsteps < 0) return fail(FSX_EINVAL, "nsteps < 0");
@@ -635,10 +716,39 @@ extern "C" int fsx_execute_steps(fsx_ctx* c, int nsteps, void* stream, int sync)
   if (stream && c->dirty) HIPCHK(hipStreamSynchronize(c->stream));
   if (int r = upload_params(c)) return r;
   if (stream) HIPCHK(hipStreamSynchronize(c->stream));
-  for (int s = 0; s < nsteps; s++) {
-    enqueue_step(c, st);
-    if (c->prof && ++c->ring_used == kRing)
-      if (int r = flush_timers(c, st)) return r;
+  if (c->use_graph && !c->prof && nsteps > 0) {
+    // the step's launch sequence depends on the layout and on which routing paths are in use; the
+    // kernels read every other parameter from device memory, so one capture serves until those change
+    if (!c->graph_valid || c->g_sfd != c->any_sfd || c->g_mfd != c->any_mfd || c->g_timed != c->timed_phase) {
+      if (int r = build_graphs(c, st)) return r;
+    }
+    if (c->gexec) {
+      for (int s = 0; s < nsteps; s++) HIPCHK(hipGraphLaunch(c->gexec, st));
+    } else {
+      for (int s = 0; s < nsteps; s++) {
+        if (c->gpre) HIPCHK(hipGraphLaunch(c->gpre, st));
+        const int k = c->ring_used * PH_COUNT + c->timed_phase;
+        HIPCHK(hipEventRecord(c->ev[2 * k], st));
+        c->emit_seq = 0;
+        c->emit_lo = c->timed_seq;
+        c->emit_hi = c->timed_seq + 1;
+        enqueue_step(c, st);
+        c->emit_lo = 0;
+        c->emit_hi = 1 << 30;
+        HIPCHK(hipEventRecord(c->ev[2 * k + 1], st));
+        c->rec_mask[k] = 1;
+        if (c->gpost) HIPCHK(hipGraphLaunch(c->gpost, st));
+        if (++c->ring_used == kRing)
+          if (int r = flush_timers_any(c, st)) return r;
+      }
+    }
+  } else {
+    for (int s = 0; s < nsteps; s++) {
+      c->emit_seq = 0;
+      enqueue_step(c, st);
+      if (c->prof && ++c->ring_used == kRing)
+        if (int r = flush_timers(c, st)) return r;
+    }
   }
   HIPCHK(hipGetLastError());
   for (int k = 0; k < c->S.ninst; k++)
@@ -647,6 +757,20 @@ extern "C" int fsx_execute_steps(fsx_ctx* c, int nsteps, void* stream, int sync)
     HIPCHK(hipStreamSynchronize(st));
     return check_errors(c);
   }
+  return 0;
+}
+
+extern "C" int fsx_time_phase(fsx_ctx* c, int phase) {
+  if (!c) return fail(FSX_EINVAL, "null context");
+  if (phase < -1 || phase >= PH_COUNT) return fail(FSX_EINVAL, "unknown phase");
+  if (int r = flush_timers_any(c, c->stream)) return r;
+  c->timed_phase = phase;
+  return 0;
+}
+
+extern "C" int fsx_set_graphs(fsx_ctx* c, int on) {
+  if (!c) return fail(FSX_EINVAL, "null context");
+  c->use_graph = on != 0;
   return 0;
 }
 

@@ -137,6 +137,15 @@ class Batch:
 
     def synchronize(self): check(lib.fsx_synchronize(self._h), "synchronize")
 
+    def time_phase(self, name):
+        """Time one phase (a name of ``_lib.PHASES``, or None) live while steps replay HIP graphs."""
+        k = -1 if name is None else _lib.PHASES.index(name)
+        check(lib.fsx_time_phase(self._h, k), "time_phase")
+
+    def graphs(self, on=True):
+        """Replay each step as one captured HIP graph (default) or launch its kernels one by one."""
+        check(lib.fsx_set_graphs(self._h, 1 if on else 0), "set_graphs")
+
     def route(self, sync=True):
         check(lib.fsx_route(self._h, C.c_void_p(), 1 if sync else 0), "route")
 

@@ -71,6 +71,8 @@ def _load():
         "fsx_remove_instance": [_vp, C.c_int],
         "fsx_pool_info": [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(C.c_int)],
         "fsx_device_memory": [C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)],
+        "fsx_set_graphs": [_vp, C.c_int],
+        "fsx_time_phase": [_vp, C.c_int],
     }
     for name, args in sig.items():
         f = getattr(lib, name)

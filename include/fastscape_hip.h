@@ -119,6 +119,10 @@ int fsx_set_active(fsx_ctx *ctx, int inst, int active);
 int fsx_execute_steps(fsx_ctx *ctx, int nsteps, void *stream, int sync);
 /* Wait for queued steps and report device-side failures. */
 int fsx_synchronize(fsx_ctx *ctx);
+/* Run each step as a replay of one captured HIP graph (1, the default) or as individual kernel
+ * launches (0).  Results are the same; graphs are not used while phase profiling is on.  No
+ * reference counterpart: an execution option of FastScape_Execute_Step's device implementation. */
+int fsx_set_graphs(fsx_ctx *ctx, int on);
 
 /* FastScape_Copy_H / Copy_Basement / Copy_Total_Erosion / Copy_Erosion_Rate /
  * Copy_Drainage_Area / Copy_Lake_Depth (src/FastScape_api.f90:299-471) and internal fields. */
@@ -155,6 +159,10 @@ int fsx_get_stats(fsx_ctx *ctx, int inst, long long *out6);
  * lm_orient, lm_correct, levels, area_prep, area_sweep, spl_prep, spl_fact, spl_sweep, spl_post,
  * diffusion, mfd_route, mfd_spl) and returns the number of phases. */
 int fsx_set_profiling(fsx_ctx *ctx, int on);
+/* Time one phase (0..fsx_get_timers' count-1; -1: none) live while steps replay HIP graphs: the
+ * phases before it and after it run as two graphs and the phase itself is launched directly between
+ * an event pair on the step's stream.  Its time accumulates in fsx_get_timers. */
+int fsx_time_phase(fsx_ctx *ctx, int phase);
 int fsx_get_timers(fsx_ctx *ctx, double *ms, long long *counts, int n);
 int fsx_reset_timers(fsx_ctx *ctx);
 
