@@ -476,6 +476,7 @@ static int mfd_alloc(fsx_ctx* c) {
 #define A(p, n) if (!rc) rc = dalloc(c, &S.p, (n))
   A(mrec, 8 * N); A(mdon, 8 * N); A(mw, 8 * N); A(mfact, 8 * N); A(mdir, 8 * N); A(mflag, N);
   A(mnrec, N); A(mndon, N); A(mindeg, N); A(mstack, N); A(mls, N + S.ninst); A(mnlev, S.ninst);
+  A(mpos, N); A(mdp, 8 * N); A(mdw, 8 * N);
 #undef A
   return rc;
 }

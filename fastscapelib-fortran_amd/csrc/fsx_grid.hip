@@ -107,7 +107,8 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_spl_post(const InstParams* __rest
                                                         const double* __restrict__ lsed, const double* __restrict__ water,
                                                         const double* __restrict__ gmax_flag, const int* __restrict__ pos,
                                                         const double* __restrict__ Hl, const double* __restrict__ HWl,
-                                                        double* __restrict__ hwater, double* __restrict__ lake_depth) {
+                                                        double* __restrict__ hwater, double* __restrict__ lake_depth,
+                                                        const int* __restrict__ mpos) {
   NODE_KERNEL_PROLOGUE
   if (!P.run_spl) return;
   double hv;
@@ -118,7 +119,7 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_spl_post(const InstParams* __rest
     hwater[g_] = hw;
     lake_depth[g_] = hw - ht[g_];
   } else {
-    hv = h[g_];
+    hv = Hl[(long long)P.off + mpos[g_]];  // multiple flow direction: Kahn-order positions
   }
   b[g_] = fmin(hv, b[g_]) == hv ? hv : b[g_];
   if (gmax_flag[c.inst] != 0.0) {  // the fill adds exactly zero when g == 0 everywhere
@@ -381,7 +382,7 @@ void launch_spl_prep(DevState& S, hipStream_t st) {
 void launch_spl_post(DevState& S, hipStream_t st) {
   k_spl_post<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.h, S.b, S.ht, S.etot, S.erate, S.sedflux, S.lsill,
                                                S.lwv, S.lsed, S.water, S.scal, S.pos, S.Hl, S.HWl, S.hwater,
-                                               S.lake_depth);
+                                               S.lake_depth, S.mpos);
 }
 void launch_diffusion(DevState& S, hipStream_t st) {
   k_adi_prep<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.h, S.b, S.kd, S.kdint);

@@ -205,52 +205,153 @@ __global__ __launch_bounds__(FSX_WG) void k_mfd_levels(const InstParams* __restr
   }
 }
 
+// ---- per-position records --------------------------------------------------------------------
+// The layer sweeps run over positions of the Kahn order (mstack): mpos is a node's position; every
+// per-node input a sweep needs is copied to its position by node-parallel kernels, so a sweep's only
+// dependent reads are its neighbours' values of earlier layers.
+__global__ __launch_bounds__(FSX_CHUNK) void k_mfd_pos(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
+                                                       const int* __restrict__ morder, int* __restrict__ mpos) {
+  MFD_PROLOGUE
+  mpos[P.off + morder[g_]] = v;  // here v runs over positions
+}
+
+// donors (position, weight) and receivers (positions) of each position; own area dx*dy*precip
+__global__ __launch_bounds__(FSX_CHUNK) void k_mfd_prep(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
+                                                        const int* __restrict__ mpos, const int* __restrict__ rec,
+                                                        const int* __restrict__ mrec, const int* __restrict__ mnrec,
+                                                        const double* __restrict__ mw, const int* __restrict__ mdon,
+                                                        const int* __restrict__ mndon, const double* __restrict__ precip,
+                                                        double* __restrict__ A0, int* __restrict__ mdp,
+                                                        double* __restrict__ mdw, int* __restrict__ dn,
+                                                        int* __restrict__ rp) {
+  MFD_PROLOGUE
+  const long long off = P.off, gp = off + mpos[g_];
+  A0[gp] = P.dxdy * precip[g_];
+  const int nd = mndon[g_], nr = mnrec[g_];
+  for (int j = 0; j < 8; j++) {
+    int dpos = -1;
+    double w = 0.0;
+    if (j < nd) {
+      const int e = mdon[8 * g_ + j];
+      dpos = mpos[off + (e >> 3)];
+      w = mw[8 * off + e];
+    }
+    mdp[8 * gp + j] = dpos;
+    mdw[8 * gp + j] = w;
+    dn[8 * gp + j] = j < nr ? mpos[off + mrec[8 * g_ + j]] : -1;
+  }
+  const int r = rec[g_];
+  rp[gp] = r == v ? -1 : mpos[off + r];
+}
+
 // ---- accumulation (FlowAccumulation :97-119), layer by layer from the sources ----------------------
+// Each position pulls its donors (earlier layers); the next layer's records are loaded while the
+// current one is summed.
+struct MfdAreaRec {
+  double a0;
+  int d[8];
+  double w[8];
+};
+__device__ __forceinline__ void mfd_area_load(MfdAreaRec& r, long long gp, const double* A0, const int* mdp,
+                                              const double* mdw) {
+  r.a0 = A0[gp];
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    r.d[j] = mdp[8 * gp + j];
+    r.w[j] = mdw[8 * gp + j];
+  }
+}
+
 __global__ __launch_bounds__(FSX_WG) void k_mfd_area(const InstParams* __restrict__ PP, const int* __restrict__ mls,
-                                                     const int* __restrict__ mnlev, const int* __restrict__ morder,
-                                                     const int* __restrict__ mdon, const int* __restrict__ mndon,
-                                                     const double* __restrict__ mw, const double* __restrict__ precip,
-                                                     double* __restrict__ area) {
+                                                     const int* __restrict__ mnlev, const double* __restrict__ A0,
+                                                     const int* __restrict__ mdp, const double* __restrict__ mdw,
+                                                     double* __restrict__ Al) {
   const int inst = blockIdx.x;
   const InstParams& P = PP[inst];
   if (!P.active || !P.run_spl || P.sfd) return;
+  const int tid = threadIdx.x;
   const long long off = P.off;
   const int* ls = mls + P.aux;
   const int L = mnlev[inst];
+  MfdAreaRec nx;
+  int b0 = ls[0], b1 = L > 0 ? ls[1] : 0;
+  if (L > 0 && tid < b1 - b0) mfd_area_load(nx, off + b0 + tid, A0, mdp, mdw);
   for (int l = 0; l < L; l++) {
-    const int b0 = ls[l], b1 = ls[l + 1];
-    for (int p = b0 + (int)threadIdx.x; p < b1; p += FSX_WG) {
-      const int v = morder[off + p];
-      const long long gv = off + v;
-      double a = P.dxdy * precip[gv];
-      const int nd = mndon[gv];
-      for (int k = 0; k < nd; k++) {
-        const int e = mdon[8 * gv + k];
-        a = a + area[off + (e >> 3)] * mw[8 * off + e];
-      }
-      area[gv] = a;
+    const int c0 = b0, c1 = b1;
+    MfdAreaRec cur = nx;
+    if (l + 1 < L) {
+      b0 = c1;
+      b1 = ls[l + 2];
+      if (tid < b1 - b0) mfd_area_load(nx, off + b0 + tid, A0, mdp, mdw);
+    }
+    for (int p = c0 + tid; p < c1; p += FSX_WG) {
+      MfdAreaRec r;
+      if (p == c0 + tid) r = cur;
+      else mfd_area_load(r, off + p, A0, mdp, mdw);
+      double a = r.a0;
+#pragma unroll
+      for (int j = 0; j < 8; j++)
+        if (r.d[j] >= 0) a = a + Al[off + r.d[j]] * r.w[j];
+      Al[off + p] = a;
     }
     mfd_barrier();
   }
 }
 
-// ---- per-edge stream-power factors (StreamPowerLaw :104-116 and :145-162) -------------------------
+// ---- per-position stream-power inputs (StreamPowerLaw :104-116 and :145-162) ----------------------
 // n == 1: kf*dt*(a*w)**m/l where ht(ijk) >= ht(rcv); otherwise kf*dt*(a*w)**m/l**n where
-// ht(ijk) > ht(rcv).  -1 marks an edge that does not take part.  Constant over the Gauss-Seidel loop.
+// ht(ijk) > ht(rcv); -1 marks an edge that does not take part (constant over the Gauss-Seidel loop).
+// Also: area back to grid order, ht / h / lake sill in position order, the elevation coefficient
+// (g, NaN on fixed nodes: elev = ht there), and the landscape's maxval(abs(h)) and maxval(g).
 __global__ __launch_bounds__(FSX_CHUNK) void k_mfd_fact(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
-                                                        const int* __restrict__ mrec, const int* __restrict__ mnrec,
-                                                        const double* __restrict__ mw, const uint8_t* __restrict__ mdir,
-                                                        const double* __restrict__ area, const double* __restrict__ kfint,
-                                                        const double* __restrict__ ht, double* __restrict__ mfact) {
-  MFD_PROLOGUE
+                                                        const int* __restrict__ mpos, const int* __restrict__ mrec,
+                                                        const int* __restrict__ mnrec, const double* __restrict__ mw,
+                                                        const uint8_t* __restrict__ mdir, const double* __restrict__ Al,
+                                                        double* __restrict__ area, const double* __restrict__ kfint,
+                                                        const double* __restrict__ ht, const double* __restrict__ g,
+                                                        const uint8_t* __restrict__ bc, double* __restrict__ mfact,
+                                                        double* __restrict__ HTl, double* __restrict__ Hl,
+                                                        int* __restrict__ LSl, double* __restrict__ EC,
+                                                        double* __restrict__ lsed, unsigned long long* __restrict__ redk) {
+  __shared__ unsigned long long smax[2][FSX_CHUNK / 64];
+  const Chunk c = chunks[blockIdx.x];
+  const InstParams& P = PP[c.inst];
+  const int v = c.start + (int)threadIdx.x;
+  if (!P.active || !P.run_spl || P.sfd) return;  // uniform over the block
+  const bool live = v < P.nn;
+  const long long g_ = (long long)P.off + v;
+  unsigned long long kh = live ? dkey(fabs(ht[g_])) : 0ull, kg = live ? dkey(g[g_]) : 0ull;
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long x = __shfl_xor(kh, o, 64), y = __shfl_xor(kg, o, 64);
+    kh = x > kh ? x : kh;
+    kg = y > kg ? y : kg;
+  }
+  if ((threadIdx.x & 63) == 0) { smax[0][threadIdx.x >> 6] = kh; smax[1][threadIdx.x >> 6] = kg; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < FSX_CHUNK / 64; w++) {
+      kh = smax[0][w] > kh ? smax[0][w] : kh;
+      kg = smax[1][w] > kg ? smax[1][w] : kg;
+    }
+    atomicMax(redk + 2 * c.inst, kh);
+    atomicMax(redk + 2 * c.inst + 1, kg);
+  }
+  if (!live) return;
+  const long long gp = (long long)P.off + mpos[g_];
+  const double a = Al[gp], t = ht[g_];
+  area[g_] = a;
+  HTl[gp] = t;
+  Hl[gp] = t;
+  LSl[gp] = -1;
+  lsed[g_] = 0.0;
+  EC[gp] = bc[g_] ? __longlong_as_double(0x7ff8000000000000ll) : g[g_];
   const int nr = mnrec[g_];
-  const double t = ht[g_], a = area[g_], kd = kfint[g_] * P.dt;
+  const double kd = kfint[g_] * P.dt;
   const bool n_is_one = fabs(P.n - 1.0) < 2.2250738585072014e-308;
   for (int k = 0; k < 8; k++) {
     double f = -1.0;
     if (k < nr) {
-      const int r = mrec[8 * g_ + k];
-      const double tr = ht[P.off + r];
+      const double tr = ht[P.off + mrec[8 * g_ + k]];
       const double l = P.len[mdir[8 * g_ + k]];
       if (n_is_one) {
         if (t >= tr) f = kd * pow(a * mw[8 * g_ + k], P.m) / l;
@@ -258,7 +359,7 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_mfd_fact(const InstParams* __rest
         if (t > tr) f = kd * pow(a * mw[8 * g_ + k], P.m) / pow(l, P.n);
       }
     }
-    mfact[8 * g_ + k] = f;
+    mfact[8 * gp + k] = f;
   }
 }
 
@@ -267,14 +368,38 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_mfd_fact(const InstParams* __rest
 // lake-sediment update of lake sills; elev (:84-90); solve from the last layer back (:94-176):
 // closed form over the receivers for n == 1, relaxed Newton (omega = 0.875/n) otherwise; lakes keep
 // elev and collect their water volume at their sill; err = rms(h-hp), zero when g == 0 everywhere.
+// Everything runs on positions; node-indexed water, lake sill and sediment are kept for the
+// lake fill of k_spl_post.
+struct MfdSolveRec {
+  int v, rp;
+  double t, e, h;
+  int q[8];
+  double f[8];
+};
+__device__ __forceinline__ void mfd_solve_load(MfdSolveRec& r, long long off, int p, const int* morder, const int* rp,
+                                               const double* HTl, const double* ELl, const double* Hl, const int* dn,
+                                               const double* mfact, bool gzero) {
+  const long long gp = off + p;
+  r.v = morder[gp];
+  r.rp = rp[gp];
+  r.t = HTl[gp];
+  r.e = gzero ? r.t : ELl[gp];
+  r.h = Hl[gp];
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    r.q[k] = dn[8 * gp + k];
+    r.f[k] = mfact[8 * gp + k];
+  }
+}
+
 __global__ __launch_bounds__(FSX_WG) void k_mfd_spl(
     const InstParams* __restrict__ PP, const int* __restrict__ mls, const int* __restrict__ mnlev,
-    const int* __restrict__ morder, const int* __restrict__ rec, const uint8_t* __restrict__ bc,
-    const double* __restrict__ g, const double* __restrict__ area, const double* __restrict__ ht,
-    const int* __restrict__ mrec, const int* __restrict__ mnrec, const double* __restrict__ mw,
-    const double* __restrict__ mfact, const int* __restrict__ mdon, const int* __restrict__ mndon,
-    double* __restrict__ h, double* __restrict__ hp, double* __restrict__ dh, double* __restrict__ elev,
-    double* __restrict__ water, int* __restrict__ lsill, double* __restrict__ lwv, double* __restrict__ lsed,
+    const int* __restrict__ morder, const int* __restrict__ rp, const int* __restrict__ dn,
+    const double* __restrict__ mfact, const int* __restrict__ mdp, const double* __restrict__ mdw,
+    const double* __restrict__ EC, const double* __restrict__ Al, const double* __restrict__ HTl,
+    double* __restrict__ Hl, double* __restrict__ HPl, double* __restrict__ DHl, double* __restrict__ ELl,
+    double* __restrict__ Wl, int* __restrict__ LSl, double* __restrict__ water, int* __restrict__ lsill,
+    double* __restrict__ lwv, double* __restrict__ lsed, const unsigned long long* __restrict__ redk,
     double* __restrict__ scal, int* __restrict__ ngs_out, long long* __restrict__ newton_out, int* __restrict__ err) {
   __shared__ double red[FSX_WG / 64];
   __shared__ double s_tol, s_err, s_gmax;
@@ -290,34 +415,11 @@ __global__ __launch_bounds__(FSX_WG) void k_mfd_spl(
   const double n = P.n;
   const bool n_is_one = fabs(n - 1.0) < 2.2250738585072014e-308;
   const int nkind = npow_kind(n);
-
-  double hm = 0.0, gmx = -1.7976931348623157e308;
-  for (int v = tid; v < nn; v += FSX_WG) {
-    hm = fmax(hm, fabs(h[off + v]));
-    gmx = fmax(gmx, g[off + v]);
-    lsill[off + v] = -1;
-    lsed[off + v] = 0.0;
-  }
-  for (int o = 32; o > 0; o >>= 1) {
-    hm = fmax(hm, __shfl_xor(hm, o, 64));
-    gmx = fmax(gmx, __shfl_xor(gmx, o, 64));
-  }
-  if (lane == 0) red[wid] = hm;
-  __syncthreads();
   if (tid == 0) {
-    double x = red[0];
-    for (int q = 1; q < FSX_WG / 64; q++) x = fmax(x, red[q]);
-    s_tol = 1.0e-4 * (x + 1.0);
-    s_newton = 0;
-  }
-  __syncthreads();
-  if (lane == 0) red[wid] = gmx;
-  __syncthreads();
-  if (tid == 0) {
-    double x = red[0];
-    for (int q = 1; q < FSX_WG / 64; q++) x = fmax(x, red[q]);
-    s_gmax = x;
+    s_tol = 1.0e-4 * (dval(redk[2 * inst]) + 1.0);
+    s_gmax = dval(redk[2 * inst + 1]);
     s_err = 2.0 * s_tol;
+    s_newton = 0;
   }
   __syncthreads();
   const bool gzero = s_gmax < 2.2250738585072014e-308;
@@ -325,87 +427,101 @@ __global__ __launch_bounds__(FSX_WG) void k_mfd_spl(
   unsigned long long my_newton = 0;
   while (s_err > s_tol && ngs < 99) {
     ngs++;
-    for (int v = tid; v < nn; v += FSX_WG) hp[off + v] = h[off + v];
+    for (int p = tid; p < nn; p += FSX_WG) HPl[off + p] = Hl[off + p];
     __syncthreads();
     if (!gzero) {
       for (int l = 0; l < L; l++) {
         const int b0 = ls[l], b1 = ls[l + 1];
         for (int p = b0 + tid; p < b1; p += FSX_WG) {
-          const int v = morder[off + p];
-          const long long gv = off + v;
-          const double own = ht[gv] - hp[gv];
+          const long long gp = off + p;
+          const int v = morder[gp];
+          const double own = HTl[gp] - HPl[gp];
           double s = own;
-          const int nd = mndon[gv];
-          for (int k = 0; k < nd; k++) {
-            const int e = mdon[8 * gv + k];
-            s = s + dh[off + (e >> 3)] * mw[8 * off + e];
+#pragma unroll
+          for (int j = 0; j < 8; j++) {
+            const int dpos = mdp[8 * gp + j];
+            if (dpos >= 0) s = s + DHl[off + dpos] * mdw[8 * gp + j];
           }
           double dv = s;
-          if (rec[gv] != v) {
+          if (rp[gp] >= 0) {
             const double tt = s - own;
-            if (lsill[gv] == v) lsed[gv] = tt <= 0.0 ? 0.0 : tt;
+            if (LSl[gp] == v) lsed[off + v] = tt <= 0.0 ? 0.0 : tt;
             dv = tt + own;
           } else {
-            lsed[gv] = s;
+            lsed[off + v] = s;
           }
-          dh[gv] = dv;
+          DHl[gp] = dv;
         }
         mfd_barrier();
       }
-      for (int v = tid; v < nn; v += FSX_WG) {
-        const long long gv = off + v;
-        const double t = ht[gv];
-        elev[gv] = bc[gv] ? t : t + (dh[gv] - (t - hp[gv])) * g[gv] * P.dx * P.dy / area[gv];
+      for (int p = tid; p < nn; p += FSX_WG) {
+        const long long gp = off + p;
+        const double t = HTl[gp], gc = EC[gp];
+        ELl[gp] = isnan(gc) ? t : t + (DHl[gp] - (t - HPl[gp])) * gc * P.dx * P.dy / Al[gp];
       }
       __syncthreads();
     }
+    // solve, last layer first; the next layer's records are loaded while this one is solved
+    MfdSolveRec nxr;
+    int b0 = L > 0 ? ls[L - 1] : 0, b1 = L > 0 ? ls[L] : 0;
+    if (L > 0 && tid < b1 - b0) mfd_solve_load(nxr, off, b0 + tid, morder, rp, HTl, ELl, Hl, dn, mfact, gzero);
     for (int l = L - 1; l >= 0; l--) {
-      const int b0 = ls[l], b1 = ls[l + 1];
-      for (int p = b0 + tid; p < b1; p += FSX_WG) {
-        const int v = morder[off + p];
-        const long long gv = off + v;
-        const int r = rec[gv];
-        const double t = ht[gv];
-        if (r == v) {
-          water[gv] = t;
-          lsill[gv] = v;
-          lwv[gv] = 0.0;
+      const int c0 = b0, c1 = b1;
+      MfdSolveRec cur = nxr;
+      if (l > 0) {
+        b1 = c0;
+        b0 = ls[l - 1];
+        if (tid < b1 - b0) mfd_solve_load(nxr, off, b0 + tid, morder, rp, HTl, ELl, Hl, dn, mfact, gzero);
+      }
+      for (int p = c0 + tid; p < c1; p += FSX_WG) {
+        MfdSolveRec r;
+        if (p == c0 + tid) r = cur;
+        else mfd_solve_load(r, off, p, morder, rp, HTl, ELl, Hl, dn, mfact, gzero);
+        const long long gp = off + p;
+        const int v = r.v;
+        if (r.rp < 0) {
+          Wl[gp] = r.t;
+          LSl[gp] = v;
+          if (!gzero) {
+            lwv[off + v] = 0.0;
+            water[off + v] = r.t;
+            lsill[off + v] = v;
+          }
           continue;
         }
-        const double w_rcv = water[off + r];
-        const double e = gzero ? t : elev[gv];
-        double hv = h[gv];
+        const double w_rcv = Wl[off + r.rp];
+        double hv = r.h;
         int sl;
-        if (e > w_rcv) {
-          const int nr = mnrec[gv];
-          if (nr > 0) {
+        if (r.e > w_rcv) {
+          if (r.q[0] >= 0) {
+            double hr[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) hr[k] = r.q[k] >= 0 ? Hl[off + r.q[k]] : 0.0;
             if (n_is_one) {
-              double f = e, df = 1.0;
-              for (int k = 0; k < nr; k++) {
-                const double fk = mfact[8 * gv + k];
-                if (fk >= 0.0) {
-                  f = f + fk * h[off + mrec[8 * gv + k]];
-                  df = df + fk;
+              double f = r.e, df = 1.0;
+#pragma unroll
+              for (int k = 0; k < 8; k++)
+                if (r.q[k] >= 0 && r.f[k] >= 0.0) {
+                  f = f + r.f[k] * hr[k];
+                  df = df + r.f[k];
                 }
-              }
               hv = f / df;
             } else {
               const double omega = 0.875 / n, tolp = P.newton_tol;
               double errp = 2.0 * tolp;
-              const double h0 = e;
+              const double h0 = r.e;
               int it = 0;
               while (errp > tolp) {
                 double f = hv - h0, df = 1.0;
-                for (int k = 0; k < nr; k++) {
-                  const double fk = mfact[8 * gv + k];
-                  if (fk >= 0.0) {
-                    const double x = fmax(0.0, hv - h[off + mrec[8 * gv + k]]);
+#pragma unroll
+                for (int k = 0; k < 8; k++)
+                  if (r.q[k] >= 0 && r.f[k] >= 0.0) {
+                    const double x = fmax(0.0, hv - hr[k]);
                     double pn, pn1;
                     npow(x, n, nkind, pn, pn1);
-                    f = f + fk * pn;
-                    df = df + fk * n * pn1;
+                    f = f + r.f[k] * pn;
+                    df = df + r.f[k] * n * pn1;
                   }
-                }
                 const double hn = hv - f / df;
                 errp = fabs(hn - hv);
                 hv = hv * (1.0 - omega) + hn * omega;
@@ -415,16 +531,21 @@ __global__ __launch_bounds__(FSX_WG) void k_mfd_spl(
             }
           }
           sl = v;
-          lwv[gv] = 0.0;
+          if (!gzero) lwv[off + v] = 0.0;
           if (hv < w_rcv) hv = w_rcv;
         } else {
-          hv = e;
-          sl = lsill[off + r];
+          hv = r.e;
+          sl = LSl[off + r.rp];
           if (!gzero && sl >= 0) atomicAdd(lwv + off + sl, w_rcv - hv);
         }
-        h[gv] = hv;
-        water[gv] = w_rcv > hv ? w_rcv : hv;
-        lsill[gv] = sl;
+        const double wv = w_rcv > hv ? w_rcv : hv;
+        Hl[gp] = hv;
+        Wl[gp] = wv;
+        LSl[gp] = sl;
+        if (!gzero) {
+          water[off + v] = wv;
+          lsill[off + v] = sl;
+        }
       }
       mfd_barrier();
     }
@@ -432,8 +553,8 @@ __global__ __launch_bounds__(FSX_WG) void k_mfd_spl(
       if (tid == 0) s_err = 0.0;
     } else {
       double ss = 0.0;
-      for (int v = tid; v < nn; v += FSX_WG) {
-        const double d = h[off + v] - hp[off + v];
+      for (int p = tid; p < nn; p += FSX_WG) {
+        const double d = Hl[off + p] - HPl[off + p];
         ss += d * d;
       }
       for (int o = 32; o > 0; o >>= 1) ss += __shfl_xor(ss, o, 64);
@@ -465,13 +586,17 @@ void launch_mfd_route(DevState& S, hipStream_t st) {
   k_mfd_don<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.mrec, S.mnrec, S.mdon, S.mndon, S.err);
   k_mfd_levels<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.mindeg, S.mrec, S.mnrec, S.mflag, S.rec, S.mstack, S.mls, S.mnlev,
                                             S.err);
-  k_mfd_area<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.mls, S.mnlev, S.mstack, S.mdon, S.mndon, S.mw, S.precip, S.area);
+  k_mfd_pos<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.mstack, S.mpos);
+  k_mfd_prep<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.mpos, S.rec, S.mrec, S.mnrec, S.mw, S.mdon, S.mndon,
+                                               S.precip, S.A0, S.mdp, S.mdw, S.dn, S.rp);
+  k_mfd_area<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.mls, S.mnlev, S.A0, S.mdp, S.mdw, S.Al);
 }
 
 void launch_mfd_spl(DevState& S, hipStream_t st) {
-  k_mfd_fact<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.mrec, S.mnrec, S.mw, S.mdir, S.area, S.kfint, S.ht,
-                                               S.mfact);
-  k_mfd_spl<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.mls, S.mnlev, S.mstack, S.rec, S.bc, S.g, S.area, S.ht, S.mrec,
-                                         S.mnrec, S.mw, S.mfact, S.mdon, S.mndon, S.h, S.hp, S.dh, S.elev, S.water,
-                                         S.lsill, S.lwv, S.lsed, S.scal, S.ngs, S.newton, S.err);
+  launch_fill_int(reinterpret_cast<int*>(S.redk), 4LL * S.ninst, 0, st);
+  k_mfd_fact<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.mpos, S.mrec, S.mnrec, S.mw, S.mdir, S.Al, S.area,
+                                               S.kfint, S.ht, S.g, S.bc, S.mfact, S.HTl, S.Hl, S.LSl, S.F, S.lsed, S.redk);
+  k_mfd_spl<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.mls, S.mnlev, S.mstack, S.rp, S.dn, S.mfact, S.mdp, S.mdw, S.F, S.Al,
+                                         S.HTl, S.Hl, S.HPl, S.DHl, S.ELl, S.Wl, S.LSl, S.water, S.lsill, S.lwv, S.lsed,
+                                         S.redk, S.scal, S.ngs, S.newton, S.err);
 }

@@ -52,6 +52,8 @@ struct DevState {
   uint8_t *mdir = nullptr, *mflag = nullptr;
   int *mnrec = nullptr, *mndon = nullptr, *mindeg = nullptr, *mstack = nullptr, *mls = nullptr;
   int *mnlev = nullptr;  // [ninst]
+  int *mpos = nullptr, *mdp = nullptr;  // position of each node in the Kahn order; donor positions (8/node)
+  double *mdw = nullptr;                // donor weights by position (8/node)
 
   // ---- per instance ----
   int *nbasins = nullptr, *basin0 = nullptr, *nconn = nullptr, *nlev = nullptr, *err = nullptr;
