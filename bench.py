@@ -170,8 +170,12 @@ def _sync():
         pass
 
 
+GRAPHS = True  # replay captured HIP graphs (off: launch every kernel directly, for kernel-trace profilers)
+
+
 def run_gpu(fs, Ls, steps, warmup, profile):
     with fs.Batch([(L["nx"], L["ny"]) for L in Ls]) as b:
+        b.graphs(GRAPHS)
         for k, L in enumerate(Ls):
             configure(b, L, k)
         if warmup:
@@ -196,6 +200,7 @@ def run_headline(fs, Ls, steps, warmup, prof_steps):
     the timed region then replays each step as captured HIP graphs, with that one phase launched
     between its own HIP events on the step's stream, so its mean duration is measured live."""
     with fs.Batch([(L["nx"], L["ny"]) for L in Ls]) as b:
+        b.graphs(GRAPHS)
         for k, L in enumerate(Ls):
             configure(b, L, k)
         if warmup:
@@ -261,11 +266,13 @@ def serving_leg(fs, quantum, duration, sample):
            "bytes_per_node": round(per_node, 1)}
     offered = sum(L["nx"] * L["ny"] * st for _, st, L in stream) / duration
     with fs.Pool(cap, slots=256) as p:
+        p.graphs(GRAPHS)
         rep = serving.Server(p, quantum=quantum).serve(make(True), realtime=True)
     out.update({"offered_cell_steps_per_s": offered, "achieved_cell_steps_per_s": rep["cell_steps_per_s"],
                 "p50_latency_s": rep["p50_latency_s"], "p99_latency_s": rep["p99_latency_s"],
                 "wall_s": rep["wall_s"], "rejected": rep["rejected"]})
     with fs.Pool(cap, slots=256) as p:
+        p.graphs(GRAPHS)
         sat = serving.Server(p, quantum=quantum).serve(make(False), realtime=False)
     out["saturated_cell_steps_per_s"] = sat["cell_steps_per_s"]
     pick = [L for _, _, L in stream][::max(1, len(stream) // sample)][:sample]
@@ -333,8 +340,7 @@ def dominant_phase(timers, stats, ncells):
 def roofline(ph, timers, stats, ncells):
     """achieved = algorithmic bytes per launch / mean launch time of phase ph."""
     ms, cnt = timers[ph]
-    launches = cnt * (2 if ph == "donors" else 1)  # the donor phase launches its kernel twice a step
-    avg_ms = ms / launches
+    avg_ms = ms / cnt  # one timed launch per step (the donor phase's second launch when it runs twice)
     nbytes = _alg_bytes(ph, stats, ncells)
     achieved = nbytes / (avg_ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -359,7 +365,11 @@ def main():
     ap.add_argument("--ragged-steps", type=int, default=10)
     ap.add_argument("--serve-seconds", type=float, default=8.0)
     ap.add_argument("--quantum", type=int, default=8)
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch every kernel directly instead of replaying HIP graphs (for kernel-trace profiling)")
     args = ap.parse_args()
+    global GRAPHS
+    GRAPHS = not args.no_graphs
     legs = set(args.legs.split(","))
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (landscapes shard "

@@ -60,6 +60,7 @@ struct fsx_ctx {
   // live timing of one phase inside graph replays: graph (phases before it), the phase launched
   // directly between two events, graph (phases after it)
   int timed_phase = -1, g_timed = -1, timed_seq = -1;
+  int eager_timed = -1;  // the timed phase while steps are launched directly
   hipGraphExec_t gpre = nullptr, gpost = nullptr;
   // emission window over the step's phase sequence (see PHASE)
   int emit_seq = 0, emit_lo = 0, emit_hi = 1 << 30;
@@ -102,7 +103,7 @@ struct PhaseMark {
   hipStream_t st;
   int k;
   PhaseMark(fsx_ctx* c_, hipStream_t st_, int ph) : c(c_), st(st_), k(-1) {
-    if (!c->prof) return;
+    if (!c->prof && ph != c->eager_timed) return;
     k = c->ring_used * PH_COUNT + ph;
     (void)hipEventRecord(c->ev[2 * k], st);
   }
@@ -744,12 +745,14 @@ extern "C" int fsx_execute_steps(fsx_ctx* c, int nsteps, void* stream, int sync)
       }
     }
   } else {
+    c->eager_timed = c->prof ? -1 : c->timed_phase;
     for (int s = 0; s < nsteps; s++) {
       c->emit_seq = 0;
       enqueue_step(c, st);
-      if (c->prof && ++c->ring_used == kRing)
+      if ((c->prof || c->eager_timed >= 0) && ++c->ring_used == kRing)
         if (int r = flush_timers(c, st)) return r;
     }
+    c->eager_timed = -1;
   }
   HIPCHK(hipGetLastError());
   for (int k = 0; k < c->S.ninst; k++)
@@ -765,6 +768,11 @@ extern "C" int fsx_time_phase(fsx_ctx* c, int phase) {
   if (!c) return fail(FSX_EINVAL, "null context");
   if (phase < -1 || phase >= PH_COUNT) return fail(FSX_EINVAL, "unknown phase");
   if (int r = flush_timers_any(c, c->stream)) return r;
+  if (phase >= 0 && c->ev.empty()) {
+    c->ev.resize((size_t)kRing * PH_COUNT * 2);
+    for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
+    c->rec_mask.assign((size_t)kRing * PH_COUNT, 0);
+  }
   c->timed_phase = phase;
   return 0;
 }
