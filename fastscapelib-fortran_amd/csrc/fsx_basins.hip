@@ -133,19 +133,21 @@ __global__ __launch_bounds__(64) void k_basin_walk(const InstParams* __restrict_
                                                    const int* __restrict__ basin, const double* __restrict__ h,
                                                    const int* __restrict__ soff, const int* __restrict__ bsize,
                                                    int* __restrict__ ecount, int* __restrict__ sB, int* __restrict__ sp,
-                                                   int* __restrict__ sq, double* __restrict__ sw, int* __restrict__ err) {
+                                                   int* __restrict__ sq, double* __restrict__ sw, int* __restrict__ err,
+                                                   int ninst) {
   __shared__ int lB[WALK_TCAP * 64];
   __shared__ double lW[WALK_TCAP * 64];
   int* LB = lB + threadIdx.x;
   double* LW = lW + threadIdx.x;
   constexpr int II[8] = {-1, 1, 0, 0, -1, 1, -1, 1};
   constexpr int JJ[8] = {0, 0, -1, 1, -1, -1, 1, 1};
-  const int inst = blockIdx.y;
+  int inst, sub;
+  if (!xcd_split(ninst, WALK_GX, inst, sub)) return;
   const InstParams& P = PP[inst];
   if (!P.active || !P.run_spl) return;
   const int nb = nbasins[inst];
   const int2 it = make_int2(inst, 0);
-  for (int b = blockIdx.x * 64 + threadIdx.x; b < nb; b += WALK_GX * 64) {
+  for (int b = sub * 64 + threadIdx.x; b < nb; b += WALK_GX * 64) {
     const int off = P.off, nx = P.nx, ny = P.ny;
     if (bc[off + outlet[off + b]]) continue;  // open basin: no scan
     const int r = outlet[off + b];
@@ -946,8 +948,9 @@ void launch_lm_basins(DevState& S, hipStream_t st) {
 }
 
 void launch_lm_walk(DevState& S, hipStream_t st) {
-  k_basin_walk<<<dim3(WALK_GX, S.ninst), 64, 0, st>>>(S.P, S.nbasins, S.bc, S.outlet, S.rec, S.first, S.next, S.basin, S.h,
-                                                       S.soff, S.bsize, S.ecount, S.sB, S.sp, S.sq, S.sw, S.err);
+  k_basin_walk<<<xcd_grid(S.ninst, WALK_GX), 64, 0, st>>>(S.P, S.nbasins, S.bc, S.outlet, S.rec, S.first, S.next,
+                                                           S.basin, S.h, S.soff, S.bsize, S.ecount, S.sB, S.sp, S.sq,
+                                                           S.sw, S.err, S.ninst);
 }
 
 void launch_lm_edges(DevState& S, hipStream_t st) {

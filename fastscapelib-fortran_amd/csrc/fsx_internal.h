@@ -141,6 +141,23 @@ __device__ __forceinline__ void npow(double x, double n, int kind, double& pn, d
   }
 }
 
+// XCD-aware block decomposition.  The dispatcher hands workgroups to the 8 XCDs of the device in
+// round-robin order of their linear id, and each XCD has its own L2; a kernel that runs per_inst
+// blocks for every landscape renumbers its blocks so that all blocks of a landscape land on one XCD
+// (its scattered reads then share that L2).  Launch with xcd_grid(ninst, per_inst) 1-D blocks;
+// returns false for the padding blocks.
+#define FSX_NXCD 8
+__host__ __device__ __forceinline__ int xcd_grid(int ninst, int per_inst) {
+  return FSX_NXCD * ((ninst + FSX_NXCD - 1) / FSX_NXCD) * per_inst;
+}
+__device__ __forceinline__ bool xcd_split(int ninst, int per_inst, int& inst, int& sub) {
+  const int lin = (int)blockIdx.x, x = lin % FSX_NXCD, r = lin / FSX_NXCD;
+  const int i = r / per_inst;
+  sub = r - i * per_inst;
+  inst = i * FSX_NXCD + x;
+  return inst < ninst;
+}
+
 // order-preserving 64-bit image of a double (for atomic min/max on doubles); 0 is below every image
 __device__ __forceinline__ unsigned long long dkey(double x) {
   unsigned long long u = (unsigned long long)__double_as_longlong(x);
