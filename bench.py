@@ -40,7 +40,7 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "cell-steps/s batched+served vs plain on 1 GPU; p50/p99 run latency"
 
 # Algorithmic (compulsory) bytes of the single-kernel phases: every input read once and every
-# output written once, per unit (node of the batch unless stated).  See DESIGN.md.
+# output written once, per unit (node of the batch unless stated).
 ALG_BYTES_PER_NODE = {
     "uplift": 40,        # R h,u,b  W h,b
     "receivers": 20,     # R h  W rec(i32), length
@@ -337,6 +337,21 @@ def dominant_phase(timers, stats, ncells):
     return best[0]
 
 
+def measured_traffic(ph, workload):
+    """HBM bytes per launch of phase ph's kernel from the committed counter summary of this workload
+    (FETCH_SIZE and WRITE_SIZE runs scaled by a copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_uniform_batch_64x513.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = d.get("kernels", {}).get(ph)
+    if d.get("workload") != workload or not k:
+        return None
+    return int(k["fetch_bytes"] + k["write_bytes"])
+
+
 def roofline(ph, timers, stats, ncells):
     """achieved = algorithmic bytes per launch / mean launch time of phase ph."""
     ms, cnt = timers[ph]
@@ -346,6 +361,21 @@ def roofline(ph, timers, stats, ncells):
     return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": ph,
             "avg_launch_ms": round(avg_ms, 4), "alg_bytes_per_launch": int(nbytes)}
+
+
+def copy_leg(nbytes=1 << 30, reps=5):
+    """A plain device copy of known size (the HBM-counter calibration for the roofline's traffic)."""
+    import torch
+    n = nbytes // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    return {"bytes_read": nbytes, "bytes_written": nbytes, "ms": dt * 1e3, "GBps": 2 * nbytes / dt / 1e9}
 
 
 def main():
@@ -361,7 +391,7 @@ def main():
     ap.add_argument("--cpu-procs", type=int, default=0, help="0: min(16, host cores)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--legs", default="uniform,plain,ragged,serving",
-                    help="comma list of uniform, plain, ragged, serving")
+                    help="comma list of uniform, plain, ragged, serving, copy (copy alone: HBM-counter calibration)")
     ap.add_argument("--ragged-steps", type=int, default=10)
     ap.add_argument("--serve-seconds", type=float, default=8.0)
     ap.add_argument("--quantum", type=int, default=8)
@@ -373,12 +403,16 @@ def main():
     legs = set(args.legs.split(","))
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (landscapes shard "
-              "across GPUs as independent batches, see DESIGN.md)", file=sys.stderr)
+              "across GPUs as independent batches, one process per GPU)", file=sys.stderr)
         sys.exit(2)
 
+    if legs == {"copy"}:
+        print(json.dumps({"copy": copy_leg()}))
+        return
     import fastscapelib_fortran as fs
 
     Ls = uniform_batch(args.ninst, args.n)
+    workload = f"uniform batch {args.ninst} x {args.n}x{args.n} D8 n=1 g=0 kd=1e-2 fixed BC"
     ncells = sum(L["nx"] * L["ny"] for L in Ls)
     dt, timers, live, dom, stats, mem = run_headline(fs, Ls, args.steps, args.warmup, args.prof_steps)
     value = ncells * args.steps / dt
@@ -394,12 +428,12 @@ def main():
         "metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded uniform-noise landscapes)",
-        "config": {"workload": f"uniform batch {args.ninst} x {args.n}x{args.n} D8 n=1 g=0 kd=1e-2 fixed BC",
+        "config": {"workload": workload,
                    "landscapes": args.ninst, "cells": ncells},
         "plain": ({"value": plain, "ms_per_step": pdt / args.plain_steps * 1e3,
                    "workload": f"1 x {args.n}x{args.n}, same parameters"} if plain else None),
         "batched_over_plain": value / plain if plain else None,
-        "roofline": roofline(dom, live, stats, ncells),
+        "roofline": dict(roofline(dom, live, stats, ncells), traffic=measured_traffic(dom, workload)),
         "phase_ms_per_step": phases,
         "phase_ms_per_step_note": f"profiled window of {args.prof_steps} steps, kernels launched one by one",
         "step_model_frac": round(336.0 * ncells / (dt / args.steps) / 1e9 / HBM_PEAK_GBS, 5),
