@@ -47,6 +47,19 @@ struct Chunk {
   int inst, start;
 };
 
+// Per-position records of the single-flow sweeps, one aligned block per position so that a prep
+// kernel writes whole segments and a sweep reads one contiguous record.
+struct alignas(16) AreaRec64 {  // drainage area: donor positions (relative, -1 unused), own term
+  int d[8];
+  double a0;
+  double pad[3];
+};
+struct alignas(16) SplRec32 {  // stream power: node, receiver position (relative), factor, ht
+  int o, q;
+  double f, t;
+  double pad;
+};
+
 // ------------------------------------------------------------------------------------------
 // Neighbour of (i,j) in direction (ii,jj) with cyclic wrap else clamp, as find_receiver
 // (src/FlowRouting.f90:336-345).  0-based.

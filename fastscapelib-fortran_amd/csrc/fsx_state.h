@@ -26,6 +26,8 @@ struct DevState {
   // per-position (level order) working arrays of the sweeps
   double *A0 = nullptr, *Al = nullptr, *F = nullptr, *HTl = nullptr, *Hl = nullptr, *Wl = nullptr, *HWl = nullptr;
   double *HPl = nullptr, *DHl = nullptr, *ELl = nullptr;
+  AreaRec64* arec = nullptr;  // per-position drainage-area records
+  SplRec32* srec = nullptr;   // per-position stream-power records
 
   // ---- per basin (node offsets) ----
   int *outlet = nullptr, *bsize = nullptr, *soff = nullptr, *ecount = nullptr, *eoff = nullptr;

@@ -212,19 +212,23 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_area_prep(const InstParams* __res
                                                          const unsigned long long* __restrict__ J,
                                                          const uint8_t* __restrict__ dmask, const int* __restrict__ pos,
                                                          const int* __restrict__ lstart, const double* __restrict__ precip,
-                                                         double* __restrict__ A0, int* __restrict__ dn) {
+                                                         AreaRec64* __restrict__ arec) {
   NODE_PROLOGUE
   if (!P.sfd) return;
   const int p = pos[g_];
   const int d = jdist(J[g_]);
   const long long gp = (long long)P.off + p;
-  A0[gp] = P.dxdy * precip[g_];
   int ds[8];
   const int nd = donors_sorted(P, v, dmask[g_], ds);
   const int base = lstart[P.aux + d + 1];
-  int* out = dn + 8 * gp;
+  int o[8];
 #pragma unroll
-  for (int k = 0; k < 8; k++) out[k] = k < nd ? pos[P.off + ds[nd - 1 - k]] - base : -1;
+  for (int k = 0; k < 8; k++) o[k] = k < nd ? pos[P.off + ds[nd - 1 - k]] - base : -1;
+  int4* w = reinterpret_cast<int4*>(arec + gp);
+  w[0] = make_int4(o[0], o[1], o[2], o[3]);
+  w[1] = make_int4(o[4], o[5], o[6], o[7]);
+  reinterpret_cast<double2*>(arec + gp)[2] = make_double2(P.dxdy * precip[g_], 0.0);
+  reinterpret_cast<double2*>(arec + gp)[3] = make_double2(0.0, 0.0);
 }
 
 // Stream-power inputs per position: receiver position in the previous level, ht, and
@@ -236,10 +240,8 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_spl_fact(const InstParams* __rest
                                                         const double* __restrict__ Al, double* __restrict__ area,
                                                         const double* __restrict__ length,
                                                         const double* __restrict__ kfint, const double* __restrict__ ht,
-                                                        int* __restrict__ rp, double* __restrict__ F, double* __restrict__ HTl,
-                                                        double* __restrict__ Hl, int* __restrict__ LSl,
-                                                        double* __restrict__ lsed, const double* __restrict__ g,
-                                                        unsigned long long* __restrict__ redk) {
+                                                        SplRec32* __restrict__ srec, double* __restrict__ lsed,
+                                                        const double* __restrict__ g, unsigned long long* __restrict__ redk) {
   __shared__ unsigned long long smax[2][FSX_CHUNK / 64];
   const Chunk c = chunks[blockIdx.x];
   const InstParams& P = PP[c.inst];
@@ -273,24 +275,23 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_spl_fact(const InstParams* __rest
   const double t = ht[g_];
   const double av = Al[gp];  // drainage area, level order -> grid order
   area[g_] = av;
-  HTl[gp] = t;
-  // Gauss-Seidel state: h in level order, no lake sill, no lake sediment
-  Hl[gp] = t;
-  LSl[gp] = -1;
-  lsed[g_] = 0.0;
-  if (d == 0) {
-    rp[gp] = -1;
-    F[gp] = 0.0;
-    return;
+  lsed[g_] = 0.0;  // no lake sediment before the first Gauss-Seidel pass
+  int q = -1;
+  double f = 0.0;
+  if (d > 0) {
+    q = pos[P.off + r] - lstart[P.aux + d - 1];
+    const double am = pow(av, P.m);
+    const double n = P.n;
+    if (fabs(n - 1.0) < 2.2250738585072014e-308) {
+      f = kfint[g_] * P.dt * am / length[g_];
+    } else {
+      f = (t > ht[P.off + r]) ? kfint[g_] * P.dt * am / pow(length[g_], n) : -1.0;
+    }
   }
-  rp[gp] = pos[P.off + r] - lstart[P.aux + d - 1];
-  const double am = pow(av, P.m);
-  const double n = P.n;
-  if (fabs(n - 1.0) < 2.2250738585072014e-308) {
-    F[gp] = kfint[g_] * P.dt * am / length[g_];
-  } else {
-    F[gp] = (t > ht[P.off + r]) ? kfint[g_] * P.dt * am / pow(length[g_], n) : -1.0;
-  }
+  ulonglong2* w = reinterpret_cast<ulonglong2*>(srec + gp);
+  w[0] = make_ulonglong2((unsigned long long)(unsigned)v | ((unsigned long long)(unsigned)q << 32),
+                         (unsigned long long)__double_as_longlong(f));
+  w[1] = make_ulonglong2((unsigned long long)__double_as_longlong(t), 0ull);
 }
 
 // ---- staged level groups ---------------------------------------------------------------------------
@@ -314,9 +315,9 @@ struct AreaRec {
   int d[8];
 };
 
-__device__ __forceinline__ void area_load(AreaRec& r, long long gp, const double* A0, const int* dn, const int* order) {
-  r.a = A0[gp];
-  const int4* q = reinterpret_cast<const int4*>(dn + 8 * gp);
+__device__ __forceinline__ void area_load(AreaRec& r, long long gp, const AreaRec64* arec) {
+  r.a = reinterpret_cast<const double2*>(arec + gp)[2].x;
+  const int4* q = reinterpret_cast<const int4*>(arec + gp);
   const int4 x = q[0], y = q[1];
   r.d[0] = x.x; r.d[1] = x.y; r.d[2] = x.z; r.d[3] = x.w;
   r.d[4] = y.x; r.d[5] = y.y; r.d[6] = y.z; r.d[7] = y.w;
@@ -324,8 +325,7 @@ __device__ __forceinline__ void area_load(AreaRec& r, long long gp, const double
 
 __global__ __launch_bounds__(FSX_WG) void k_area_sweep(const InstParams* __restrict__ PP, const int* __restrict__ lstart,
                                                        const int4* __restrict__ grec, const int* __restrict__ ngroups,
-                                                       const int* __restrict__ order, const double* __restrict__ A0,
-                                                       const int* __restrict__ dn, double* __restrict__ Al,
+                                                       const AreaRec64* __restrict__ arec, double* __restrict__ Al,
                                                        double* __restrict__ area, int vcap) {
   extern __shared__ double sm[];
   // level values double-buffered at sm[0..vcap) and sm[vcap..2*vcap); offsets, not a pointer array,
@@ -354,8 +354,8 @@ __global__ __launch_bounds__(FSX_WG) void k_area_sweep(const InstParams* __restr
   auto prefetch = [&](const GroupMeta& m) {
     const int np = m.p1 - m.p0;
     if (np > GRP_STG) return;
-    if (tid < np) area_load(r0, off + m.p0 + tid, A0, dn, order);
-    if (tid + FSX_WG < np) area_load(r1, off + m.p0 + tid + FSX_WG, A0, dn, order);
+    if (tid < np) area_load(r0, off + m.p0 + tid, arec);
+    if (tid + FSX_WG < np) area_load(r1, off + m.p0 + tid + FSX_WG, arec);
     if (tid < m.hi - m.lo) rl = ls[m.lo + tid];
   };
   GroupMeta cur = meta(G - 1);
@@ -404,7 +404,7 @@ __global__ __launch_bounds__(FSX_WG) void k_area_sweep(const InstParams* __restr
             for (int k = 0; k < 8; k++) q[k] = sD[8 * si + k];
           } else {
             AreaRec r;
-            area_load(r, off + b0 + i, A0, dn, order);
+            area_load(r, off + b0 + i, arec);
             a = r.a;
 #pragma unroll
             for (int k = 0; k < 8; k++) q[k] = r.d[k];
@@ -444,20 +444,27 @@ struct SplRec {
   double f, t, h;
 };
 
-__device__ __forceinline__ void spl_load(SplRec& r, long long gp, const int* order, const int* rp, const double* F,
-                                         const double* HTl, const double* Hl) {
-  r.o = order[gp];
-  r.q = rp[gp];
-  r.f = F[gp];
-  r.t = HTl[gp];
-  r.h = Hl[gp];
+// h of the first Gauss-Seidel pass is ht; later passes start from the previous pass's h
+__device__ __forceinline__ void spl_load(SplRec& r, long long gp, const SplRec32* srec, const double* Hl, bool first) {
+  const ulonglong2* w = reinterpret_cast<const ulonglong2*>(srec + gp);
+  const ulonglong2 a = w[0], b = w[1];
+  r.o = (int)(unsigned)(a.x & 0xffffffffull);
+  r.q = (int)(unsigned)(a.x >> 32);
+  r.f = __longlong_as_double((long long)a.y);
+  r.t = __longlong_as_double((long long)b.x);
+  r.h = first ? r.t : Hl[gp];
+}
+__device__ __forceinline__ double srec_t(const SplRec32* srec, long long gp) {
+  return __longlong_as_double((long long)reinterpret_cast<const ulonglong2*>(srec + gp)[1].x);
+}
+__device__ __forceinline__ int srec_o(const SplRec32* srec, long long gp) {
+  return (int)(unsigned)(reinterpret_cast<const ulonglong2*>(srec + gp)[0].x & 0xffffffffull);
 }
 
 __global__ __launch_bounds__(FSX_WG) void k_spl_sweep(
     const InstParams* __restrict__ PP, const int* __restrict__ lstart, const int* __restrict__ nlev,
-    const int4* __restrict__ grec, const int* __restrict__ ngroups, const int* __restrict__ order,
-    const int* __restrict__ rp, const double* __restrict__ F, const double* __restrict__ HTl,
-    const int* __restrict__ dn, const uint8_t* __restrict__ bc, const double* __restrict__ g,
+    const int4* __restrict__ grec, const int* __restrict__ ngroups, const SplRec32* __restrict__ srec,
+    const AreaRec64* __restrict__ arec, const uint8_t* __restrict__ bc, const double* __restrict__ g,
     const double* __restrict__ Al, double* __restrict__ h, double* __restrict__ hwater, double* __restrict__ lake_depth,
     double* __restrict__ water, int* __restrict__ lsill, double* __restrict__ lwv, double* __restrict__ lsed,
     double* __restrict__ Hl, double* __restrict__ Wl, double* __restrict__ HWl, int* __restrict__ LSl,
@@ -516,7 +523,7 @@ __global__ __launch_bounds__(FSX_WG) void k_spl_sweep(
     ngs++;
     const bool first = ngs == 1;
     if (!gzero && !fill_only) {
-      for (int p = tid; p < nn; p += FSX_WG) HPl[off + p] = Hl[off + p];
+      for (int p = tid; p < nn; p += FSX_WG) HPl[off + p] = first ? srec_t(srec, off + p) : Hl[off + p];
       __syncthreads();
       // erosion push, bottom-up (donors in descending index); lake sediment of lake sills
       for (int l = L - 1; l >= 0; l--) {
@@ -527,10 +534,10 @@ __global__ __launch_bounds__(FSX_WG) void k_spl_sweep(
         const bool nin = wn <= vcap;
         for (int i = tid; i < w; i += FSX_WG) {
           const long long gp = off + b0 + i;
-          const int v = order[gp];
-          const double own = HTl[gp] - HPl[gp];
+          const int v = srec_o(srec, gp);
+          const double own = srec_t(srec, gp) - HPl[gp];
           double s = own;
-          const int* dd = dn + 8 * gp;
+          const int* dd = arec[gp].d;
           for (int k = 0; k < 8; k++) {
             const int q = dd[k];
             if (q < 0) break;
@@ -539,7 +546,7 @@ __global__ __launch_bounds__(FSX_WG) void k_spl_sweep(
           double dv;
           if (l > 0) {
             const double t = s - own;
-            if (LSl[gp] == v) lsed[off + v] = t <= 0.0 ? 0.0 : t;
+            if (!first && LSl[gp] == v) lsed[off + v] = t <= 0.0 ? 0.0 : t;
             dv = t + own;
           } else {
             dv = s;
@@ -552,8 +559,8 @@ __global__ __launch_bounds__(FSX_WG) void k_spl_sweep(
       }
       for (int p = tid; p < nn; p += FSX_WG) {
         const long long gp = off + p;
-        const int v = order[gp];
-        const double t = HTl[gp];
+        const int v = srec_o(srec, gp);
+        const double t = srec_t(srec, gp);
         ELl[gp] = bc[off + v] ? t : t + (DHl[gp] - (t - HPl[gp])) * g[off + v] * P.dx * P.dy / Al[gp];
       }
       __syncthreads();
@@ -564,8 +571,8 @@ __global__ __launch_bounds__(FSX_WG) void k_spl_sweep(
     auto prefetch = [&](const GroupMeta& m) {
       const int np = m.p1 - m.p0;
       if (np > GRP_STG) return;
-      if (tid < np) spl_load(r0, off + m.p0 + tid, order, rp, F, HTl, Hl);
-      if (tid + FSX_WG < np) spl_load(r1, off + m.p0 + tid + FSX_WG, order, rp, F, HTl, Hl);
+      if (tid < np) spl_load(r0, off + m.p0 + tid, srec, Hl, first);
+      if (tid + FSX_WG < np) spl_load(r1, off + m.p0 + tid + FSX_WG, srec, Hl, first);
       if (tid < m.hi - m.lo) rl = ls[m.lo + tid];
     };
     GroupMeta cur = meta(0);
@@ -604,7 +611,7 @@ __global__ __launch_bounds__(FSX_WG) void k_spl_sweep(
               const int si = b0 - cur.p0 + i;
               rr.o = sO[si]; rr.q = sQ[si]; rr.f = sF[si]; rr.t = sT[si]; rr.h = sH[si];
             } else {
-              spl_load(rr, off + b0 + i, order, rp, F, HTl, Hl);
+              spl_load(rr, off + b0 + i, srec, Hl, first);
             }
             const int v = rr.o;
             const double t = rr.t;
@@ -768,7 +775,7 @@ void launch_levels(DevState& S, hipStream_t st) {
 }
 
 void launch_area_prep(DevState& S, hipStream_t st) {
-  k_area_prep<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.J, S.dmask, S.pos, S.lstart, S.precip, S.A0, S.dn);
+  k_area_prep<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.J, S.dmask, S.pos, S.lstart, S.precip, S.arec);
 }
 
 void launch_area(DevState& S, hipStream_t st) {
@@ -777,19 +784,19 @@ void launch_area(DevState& S, hipStream_t st) {
 }
 
 void launch_area_sweep(DevState& S, hipStream_t st) {
-  k_area_sweep<<<S.ninst, FSX_WG, area_smem(kAreaVcap), st>>>(S.P, S.lstart, S.grec, S.ngroups, S.order, S.A0, S.dn,
+  k_area_sweep<<<S.ninst, FSX_WG, area_smem(kAreaVcap), st>>>(S.P, S.lstart, S.grec, S.ngroups, S.arec,
                                                                 S.Al, S.area, kAreaVcap);
 }
 
 void launch_spl_fact(DevState& S, hipStream_t st) {
   launch_fill_int(reinterpret_cast<int*>(S.redk), 4LL * S.ninst, 0, st);
   k_spl_fact<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.J, S.rec, S.pos, S.lstart, S.Al, S.area, S.length,
-                                               S.kfint, S.ht, S.rp, S.F, S.HTl, S.Hl, S.LSl, S.lsed, S.g, S.redk);
+                                               S.kfint, S.ht, S.srec, S.lsed, S.g, S.redk);
 }
 
 static void spl_sweep_launch(DevState& S, hipStream_t st, int fill_only) {
   k_spl_sweep<<<S.ninst, FSX_WG, spl_smem(kSplVcap), st>>>(
-      S.P, S.lstart, S.nlev, S.grec, S.ngroups, S.order, S.rp, S.F, S.HTl, S.dn, S.bc, S.g, S.Al, S.h, S.hwater,
+      S.P, S.lstart, S.nlev, S.grec, S.ngroups, S.srec, S.arec, S.bc, S.g, S.Al, S.h, S.hwater,
       S.lake_depth, S.water, S.lsill, S.lwv, S.lsed, S.Hl, S.Wl, S.HWl, S.LSl, S.HPl, S.DHl, S.ELl, S.scal, S.ngs,
       S.newton, S.err, kSplVcap, fill_only, S.redk);
 }
