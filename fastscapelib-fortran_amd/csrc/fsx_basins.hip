@@ -21,7 +21,7 @@
 #include "fsx_state.h"
 
 #define NODE_PROLOGUE                        \
-  const Chunk c = chunks[blockIdx.x];        \
+  const Chunk c = chunks[chunk_id()];        \
   const InstParams& P = PP[c.inst];          \
   const int v = c.start + (int)threadIdx.x;  \
   const bool live = P.active && P.run_spl && v < P.nn;
@@ -44,7 +44,7 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_root_count(const InstParams* __re
   NODE_PROLOGUE
   const int f = (live && rec[P.off + v] == v) ? 1 : 0;
   const int t = block_sum<FSX_CHUNK>(f, lds);
-  if (threadIdx.x == 0) chunkcnt[blockIdx.x] = t;
+  if (threadIdx.x == 0) chunkcnt[chunk_id()] = t;
 }
 
 // exclusive scan of the per-chunk root counts of each landscape; nbasins; basin0 reset
@@ -80,7 +80,7 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_root_rank(const InstParams* __res
   int tot;
   const int incl = block_incl_scan<FSX_CHUNK>(f, lds, tot);
   if (f) {
-    const int r = chunkcnt[blockIdx.x] + incl - 1;
+    const int r = chunkcnt[chunk_id()] + incl - 1;
     rootrank[P.off + v] = r;
     outlet[P.off + r] = v;
     if (bc[P.off + v]) atomicMin(basin0 + c.inst, r);
@@ -269,7 +269,7 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_edge_fill(const InstParams* __res
                                                          const int* __restrict__ sq, const double* __restrict__ sw,
                                                          int* __restrict__ cb0, int* __restrict__ cb1, int* __restrict__ cn0,
                                                          int* __restrict__ cn1, double* __restrict__ cw) {
-  const Chunk c = chunks[blockIdx.x];
+  const Chunk c = chunks[chunk_id()];
   const InstParams& P = PP[c.inst];
   const int b = c.start + (int)threadIdx.x;
   if (!P.active || !P.run_spl || b >= nbasins[c.inst] || nconn[c.inst] == 0) return;

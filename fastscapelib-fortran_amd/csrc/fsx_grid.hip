@@ -3,11 +3,13 @@
 //
 // Every kernel keeps the reference's floating-point operation order (the library is built with
 // -ffp-contract=off), so results are bit-identical to the reference's FP64 Fortran.
+#include <climits>
+
 #include "fsx_scan.h"
 #include "fsx_state.h"
 
 #define NODE_KERNEL_PROLOGUE                          \
-  const Chunk c = chunks[blockIdx.x];                 \
+  const Chunk c = chunks[chunk_id()];                 \
   const InstParams& P = PP[c.inst];                   \
   const int v = c.start + (int)threadIdx.x;           \
   if (!P.active || v >= P.nn) return;                 \
@@ -57,8 +59,9 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_receivers(const InstParams* __res
 
 // Donor masks: bit dbit(d) set when the neighbour of v in direction d drains to v.  Listing the
 // set bits in ascending neighbour index reproduces find_donor (src/FlowRouting.f90:366-389).
-// Also threads the donors as first[v] (smallest donor) and next[d] (next larger donor of the same
-// receiver), which lets a depth-first preorder walk advance with one load per step.
+// Also threads the donors as first[v] (smallest donor) and next[v] (next larger donor of rec(v)),
+// which lets a depth-first preorder walk advance with one load per step.  Every thread writes only
+// its own node: next[v] is gathered from the receivers around rec(v).
 __global__ __launch_bounds__(FSX_CHUNK) void k_donors(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
                                                       const int* __restrict__ rec, uint8_t* __restrict__ dmask,
                                                       int* __restrict__ first, int* __restrict__ next) {
@@ -68,18 +71,24 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_donors(const InstParams* __restri
   int q[8], dir[8];
   const int c8 = neighbours_wc(P, v, q, dir);
   unsigned m = 0;
-  int d[8], nd = 0;
-  for (int t = 0; t < c8; t++)
-    if (ri[q[t]] == v) {
+  int f = INT_MAX;
+#pragma unroll
+  for (int t = 0; t < 8; t++)
+    if (t < c8 && ri[q[t]] == v) {
       m |= 1u << dbit(dir[t]);
-      int x = q[t], s = nd++;
-      while (s > 0 && d[s - 1] > x) { d[s] = d[s - 1]; s--; }
-      d[s] = x;
+      f = q[t] < f ? q[t] : f;
     }
   dmask[g_] = (uint8_t)m;
-  first[g_] = nd ? d[0] : -1;
-  for (int k = 0; k < nd; k++) next[P.off + d[k]] = k + 1 < nd ? d[k + 1] : -1;
-  if (ri[v] == v) next[g_] = -1;
+  first[g_] = m ? f : -1;
+  const int r = ri[v];
+  int nxt = INT_MAX;
+  if (r != v) {
+    const int cr = neighbours_wc(P, r, q, dir);
+#pragma unroll
+    for (int t = 0; t < 8; t++)
+      if (t < cr && q[t] > v && q[t] < nxt && ri[q[t]] == r) nxt = q[t];
+  }
+  next[g_] = nxt == INT_MAX ? -1 : nxt;
 }
 
 // Stream-power set-up, src/StreamPowerLaw.f90:236-250: g, kfint, ht.

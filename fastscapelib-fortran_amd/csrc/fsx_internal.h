@@ -171,6 +171,16 @@ __device__ __forceinline__ bool xcd_split(int ninst, int per_inst, int& inst, in
   return inst < ninst;
 }
 
+// Chunk handled by this block of a node-parallel launch (one block per chunk, gridDim.x == number
+// of chunks).  Block b runs on XCD b % 8; chunk ids are handed out so that each XCD works through one
+// contiguous run of chunks: neighbouring rows of a stencil, and the nodes of a landscape, share that
+// XCD's L2 instead of being fetched by all eight.
+__device__ __forceinline__ int chunk_id() {
+  const int n = (int)gridDim.x, b = (int)blockIdx.x, x = b % FSX_NXCD, k = b / FSX_NXCD;
+  const int q = n / FSX_NXCD, r = n % FSX_NXCD;
+  return x * q + (x < r ? x : r) + k;
+}
+
 // order-preserving 64-bit image of a double (for atomic min/max on doubles); 0 is below every image
 __device__ __forceinline__ unsigned long long dkey(double x) {
   unsigned long long u = (unsigned long long)__double_as_longlong(x);

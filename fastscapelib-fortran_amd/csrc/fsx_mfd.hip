@@ -23,7 +23,7 @@
 #include "fsx_state.h"
 
 #define MFD_PROLOGUE                                        \
-  const Chunk c = chunks[blockIdx.x];                       \
+  const Chunk c = chunks[chunk_id()];                       \
   const InstParams& P = PP[c.inst];                         \
   const int v = c.start + (int)threadIdx.x;                 \
   if (!P.active || !P.run_spl || P.sfd || v >= P.nn) return; \
@@ -314,7 +314,7 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_mfd_fact(const InstParams* __rest
                                                         int* __restrict__ LSl, double* __restrict__ EC,
                                                         double* __restrict__ lsed, unsigned long long* __restrict__ redk) {
   __shared__ unsigned long long smax[2][FSX_CHUNK / 64];
-  const Chunk c = chunks[blockIdx.x];
+  const Chunk c = chunks[chunk_id()];
   const InstParams& P = PP[c.inst];
   const int v = c.start + (int)threadIdx.x;
   if (!P.active || !P.run_spl || P.sfd) return;  // uniform over the block

@@ -12,13 +12,14 @@
 //    reference's summation order, src/FlowRouting.f90:136-139) and top-down for the lake fill and
 //    the stream-power solve (each node needs only its receiver).
 // Nodes of one level are independent, so their order inside the level does not affect any value.
+#include <algorithm>
 #include <type_traits>
 
 #include "fsx_scan.h"
 #include "fsx_state.h"
 
 #define NODE_PROLOGUE                         \
-  const Chunk c = chunks[blockIdx.x];         \
+  const Chunk c = chunks[chunk_id()];         \
   const InstParams& P = PP[c.inst];           \
   const int v = c.start + (int)threadIdx.x;   \
   if (!P.active || !P.run_spl || v >= P.nn) return; \
@@ -40,18 +41,28 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_jump_init(const InstParams* __res
 
 // J[v] = (distance to ancestor, ancestor).  One jump replaces the ancestor by its own ancestor;
 // a 64-bit load/store keeps the pair consistent, so jumping in place is safe.
+// One pointer-jumping round.  A round after the one that changed nothing returns at once; the grid
+// strides over the chunks so that such a launch costs few workgroups.
 __global__ __launch_bounds__(FSX_CHUNK) void k_jump(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
-                                                    unsigned long long* __restrict__ J, int* __restrict__ flags, int iter) {
+                                                    int nchunks, unsigned long long* __restrict__ J,
+                                                    int* __restrict__ flags, int iter) {
   if (iter > 0 && flags[iter - 1] == 0) return;
-  NODE_PROLOGUE
-  unsigned long long* Ji = J + P.off;
-  const unsigned long long w = Ji[v];
-  const int x = jnext(w);
-  const unsigned long long wx = Ji[x];
-  if (jnext(wx) != x) {
-    Ji[v] = jpack((unsigned)(jdist(w) + jdist(wx)), (unsigned)jnext(wx));
-    flags[iter] = 1;
+  bool changed = false;
+  for (int cb = blockIdx.x; cb < nchunks; cb += gridDim.x) {
+    const Chunk c = chunks[cb];
+    const InstParams& P = PP[c.inst];
+    const int v = c.start + (int)threadIdx.x;
+    if (!P.active || !P.run_spl || v >= P.nn) continue;
+    unsigned long long* Ji = J + P.off;
+    const unsigned long long w = Ji[v];
+    const int x = jnext(w);
+    const unsigned long long wx = Ji[x];
+    if (jnext(wx) != x) {
+      Ji[v] = jpack((unsigned)(jdist(w) + jdist(wx)), (unsigned)jnext(wx));
+      changed = true;
+    }
   }
+  if (changed) flags[iter] = 1;
 }
 
 // ---- level order (counting sort by depth, one workgroup per landscape) ---------------------------
@@ -243,7 +254,7 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_spl_fact(const InstParams* __rest
                                                         SplRec32* __restrict__ srec, double* __restrict__ lsed,
                                                         const double* __restrict__ g, unsigned long long* __restrict__ redk) {
   __shared__ unsigned long long smax[2][FSX_CHUNK / 64];
-  const Chunk c = chunks[blockIdx.x];
+  const Chunk c = chunks[chunk_id()];
   const InstParams& P = PP[c.inst];
   const int v = c.start + (int)threadIdx.x;
   if (!P.active || !P.run_spl || !P.sfd) return;  // uniform over the block: one chunk, one landscape
@@ -762,10 +773,13 @@ int sweep_set_lds(int smem_max) {
   return 0;
 }
 
+static const int kJumpBlocks = 8192;  // 8 workgroups of 256 per CU
+
 void launch_jump(DevState& S, hipStream_t st) {
   k_jump_init<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.rec, S.J);
   launch_fill_int(S.flags, 64, 0, st);
-  for (int it = 0; it < 26; it++) k_jump<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.J, S.flags, it);
+  const int nb = std::min(S.nchunks, kJumpBlocks);
+  for (int it = 0; it < 26; it++) k_jump<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.nchunks, S.J, S.flags, it);
 }
 
 void launch_levels(DevState& S, hipStream_t st) {
