@@ -334,16 +334,113 @@ __device__ __forceinline__ bool thomas_line(const double* __restrict__ dg, const
   return true;
 }
 
-// x pass: one thread per row j = 1..ny-2, writes zint (all columns) for those rows
+// x pass (rows j = 1..ny-2): a band of up to 64 consecutive rows per one-wave workgroup, one row
+// per lane.  Rows are contiguous in memory, so a lane walking its own row would touch a new cache
+// line every few steps while its neighbours touch others; instead the band's diagonal, right-hand
+// side and couplings are staged through LDS in tiles of ADX_K columns, read as whole row segments
+// (the next tile in flight while the current one is solved).  The forward elimination writes gam
+// and the intermediate solution to band-transposed scratch (column k of lane t at band start +
+// k*R + t, so a wave's stores are contiguous), and the back substitution returns the solution to
+// row-major zint through LDS.  Arithmetic is the Thomas recurrence of thomas_line.
+#define ADX_K 16
+#define ADX_RS (64 / ADX_K)  // rows covered by one staging load of the wave
 __global__ __launch_bounds__(64) void k_adi_x(const InstParams* __restrict__ PP, const double* __restrict__ dg,
                                               const double* __restrict__ rf, const double* __restrict__ cx,
-                                              double* __restrict__ zint, double* __restrict__ gam, int* __restrict__ err) {
+                                              double* __restrict__ zint, double* __restrict__ GT,
+                                              double* __restrict__ UT, int* __restrict__ err) {
+  constexpr int LD = ADX_K + 1;  // padded LDS row: lanes reading column c of their rows spread over the banks
+  constexpr int NM = 64 / ADX_RS;
+  __shared__ double sD[64 * LD], sR[64 * LD], sC[64 * LD];
   const int inst = blockIdx.y;
   const InstParams& P = PP[inst];
-  const int j = 1 + blockIdx.x * 64 + threadIdx.x;
-  if (!P.active || !P.run_diff || j > P.ny - 2) return;
-  const long long o = (long long)P.off + (long long)j * P.nx;
-  if (!thomas_line(dg, rf, cx, zint, gam, o, 1, P.nx, P.fix_left, P.fix_right)) atomicOr(err + inst, FSX_ERR_TRIDAG);
+  const int j0 = 1 + blockIdx.x * 64;
+  if (!P.active || !P.run_diff || j0 > P.ny - 2) return;
+  const int nx = P.nx, R = min(64, P.ny - 1 - j0), lane = threadIdx.x;
+  const bool own = lane < R;
+  const long long base = (long long)P.off + (long long)j0 * nx;
+  const int lc = lane % ADX_K, lr = lane / ADX_K;  // staging: column lc of rows lr, lr+ADX_RS, ...
+  double pd[NM], pr[NM], pc[NM];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int m = 0; m < NM; m++) {
+      const int rr = lr + ADX_RS * m, k = k0 + lc;
+      const bool in = rr < R && k < nx;
+      const long long q = base + (long long)rr * nx + k;
+      pd[m] = in ? dg[q] : 0.0;
+      pr[m] = in ? rf[q] : 0.0;
+      pc[m] = (in && k >= 1) ? cx[q - 1] : 0.0;
+    }
+  };
+  fetch(0);
+  double bet = 1.0, uprev = 0.0;
+  bool ok = true;
+  for (int k0 = 0; k0 < nx; k0 += ADX_K) {
+#pragma unroll
+    for (int m = 0; m < NM; m++) {
+      const int a = (lr + ADX_RS * m) * LD + lc;
+      sD[a] = pd[m];
+      sR[a] = pr[m];
+      sC[a] = pc[m];
+    }
+    __syncthreads();
+    if (k0 + ADX_K < nx) fetch(k0 + ADX_K);
+    const int kn = min(ADX_K, nx - k0);
+    if (own && ok) {
+      for (int c = 0; c < kn; c++) {
+        const int k = k0 + c;
+        const double d = sD[lane * LD + c], r = sR[lane * LD + c];
+        if (k == 0) {
+          bet = d;
+          if (bet == 0.0) { ok = false; break; }
+          uprev = r / bet;
+        } else {
+          const double cc = sC[lane * LD + c];
+          const double cprev = (k == 1 && P.fix_left) ? 0.0 : -cc;
+          const double inf = (k == nx - 1 && P.fix_right) ? 0.0 : -cc;
+          const double gm = cprev / bet;
+          bet = d - inf * gm;
+          if (bet == 0.0) { ok = false; break; }
+          uprev = (r - inf * uprev) / bet;
+          GT[base + (long long)k * R + lane] = gm;
+        }
+        UT[base + (long long)k * R + lane] = uprev;
+      }
+    }
+    __syncthreads();
+  }
+  if (own && !ok) atomicOr(err + inst, FSX_ERR_TRIDAG);
+  // back substitution u(k) = u(k) - gam(k+1)*u(k+1), k = nx-2..0, tiles from the last; uprev == u(nx-1)
+  double nu[ADX_K], ng[ADX_K];
+  auto fetch_back = [&](int k0) {
+#pragma unroll
+    for (int c = 0; c < ADX_K; c++) {
+      const int k = k0 + c;
+      const bool in = own && k < nx - 1;
+      nu[c] = in ? UT[base + (long long)k * R + lane] : 0.0;
+      ng[c] = in ? GT[base + (long long)(k + 1) * R + lane] : 0.0;
+    }
+  };
+  const int last0 = ((nx - 1) / ADX_K) * ADX_K;
+  fetch_back(last0);
+  for (int k0 = last0; k0 >= 0; k0 -= ADX_K) {
+    double u[ADX_K], g[ADX_K];
+#pragma unroll
+    for (int c = 0; c < ADX_K; c++) { u[c] = nu[c]; g[c] = ng[c]; }
+    if (k0 > 0) fetch_back(k0 - ADX_K);
+#pragma unroll
+    for (int c = ADX_K - 1; c >= 0; c--) {
+      const int k = k0 + c;
+      if (k < nx - 1) uprev = u[c] - g[c] * uprev;
+      if (k < nx) sD[lane * LD + c] = uprev;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < NM; m++) {
+      const int rr = lr + ADX_RS * m, k = k0 + lc;
+      if (rr < R && k < nx) zint[base + (long long)rr * nx + k] = sD[(lr + ADX_RS * m) * LD + lc];
+    }
+    __syncthreads();
+  }
 }
 
 // y pass: one thread per column i = 1..nx-2, writes zintp for those columns
@@ -398,7 +495,7 @@ void launch_diffusion(DevState& S, hipStream_t st) {
   k_adi_coef<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.kdint, S.adx, S.ady);
   k_adi_xrhs<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.h, S.adx, S.ady, S.adg, S.adf);
   dim3 gx((S.max_ny + 63) / 64, S.ninst), gy((S.max_nx + 63) / 64, S.ninst);
-  k_adi_x<<<gx, 64, 0, st>>>(S.P, S.adg, S.adf, S.adx, S.zint, S.gam, S.err);
+  k_adi_x<<<gx, 64, 0, st>>>(S.P, S.adg, S.adf, S.adx, S.zint, S.gam, S.zintp, S.err);  // zintp: scratch until the y pass
   k_adi_yrhs<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.h, S.zint, S.kdint, S.adx, S.ady, S.adg, S.adf);
   k_adi_y<<<gy, 64, 0, st>>>(S.P, S.adg, S.adf, S.ady, S.zintp, S.gam, S.err);
   k_adi_post<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.h, S.b, S.etot, S.erate, S.zintp);
