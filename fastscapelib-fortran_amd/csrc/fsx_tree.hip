@@ -99,10 +99,15 @@ __global__ __launch_bounds__(FSX_WG) void k_level_sort(const InstParams* __restr
   const long long off = P.off;
   int* ls = lstart + P.aux;  // nn+1 entries
   int* lc = lcur + P.aux;
-  if (tid == 0) {
-    s_max = 0;
-    if (flags[25]) atomicOr(err + inst, FSX_ERR_LEVELS);  // jumping did not converge: receiver cycle
+  if (flags[25]) {  // jumping did not converge: a receiver cycle, so depths are meaningless
+    if (tid == 0) {
+      atomicOr(err + inst, FSX_ERR_LEVELS);
+      nlev[inst] = 0;  // the sweeps then visit no level
+      ngroups[inst] = 0;
+    }
+    return;
   }
+  if (tid == 0) s_max = 0;
   __syncthreads();
   // the node loops take LVL_UNROLL nodes per thread at a time so their loads overlap
   int mx = 0;
@@ -223,9 +228,9 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_area_prep(const InstParams* __res
                                                          const unsigned long long* __restrict__ J,
                                                          const uint8_t* __restrict__ dmask, const int* __restrict__ pos,
                                                          const int* __restrict__ lstart, const double* __restrict__ precip,
-                                                         AreaRec64* __restrict__ arec) {
+                                                         const int* __restrict__ flags, AreaRec64* __restrict__ arec) {
   NODE_PROLOGUE
-  if (!P.sfd) return;
+  if (!P.sfd || flags[25]) return;  // flags[25]: no level order this step (receiver cycle)
   const int p = pos[g_];
   const int d = jdist(J[g_]);
   const long long gp = (long long)P.off + p;
@@ -252,7 +257,8 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_spl_fact(const InstParams* __rest
                                                         const double* __restrict__ length,
                                                         const double* __restrict__ kfint, const double* __restrict__ ht,
                                                         SplRec32* __restrict__ srec, double* __restrict__ lsed,
-                                                        const double* __restrict__ g, unsigned long long* __restrict__ redk) {
+                                                        const double* __restrict__ g, unsigned long long* __restrict__ redk,
+                                                        const int* __restrict__ flags) {
   __shared__ unsigned long long smax[2][FSX_CHUNK / 64];
   const Chunk c = chunks[chunk_id()];
   const InstParams& P = PP[c.inst];
@@ -278,7 +284,7 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_spl_fact(const InstParams* __rest
     atomicMax(redk + 2 * c.inst, kh);
     atomicMax(redk + 2 * c.inst + 1, kg);
   }
-  if (!live) return;
+  if (!live || flags[25]) return;  // flags[25]: no level order this step (receiver cycle)
   const int p = pos[g_];
   const int d = jdist(J[g_]);
   const long long gp = (long long)P.off + p;
@@ -317,6 +323,24 @@ struct GroupMeta {
   int p2;  // end of level hi (== p1 when hi is the last level)
 };
 
+// Group metadata (two int4 per group) is read by lanes 0..5 of every wave as a vector load and
+// broadcast with readlane when the group starts.  A scalar load would share its wait counter with
+// the LDS accesses, so while one is in flight every LDS wait of the level loop waits for it too.
+__device__ __forceinline__ int meta_lane(const int4* gr, int G, int g) {
+  const int lane = threadIdx.x & 63;
+  return (g >= 0 && g < G && lane < 6) ? reinterpret_cast<const int*>(gr)[8 * g + lane] : 0;
+}
+__device__ __forceinline__ GroupMeta meta_bcast(int x) {
+  GroupMeta m;
+  m.lo = __builtin_amdgcn_readlane(x, 0);
+  m.hi = __builtin_amdgcn_readlane(x, 1);
+  m.p0 = __builtin_amdgcn_readlane(x, 2);
+  m.p1 = __builtin_amdgcn_readlane(x, 3);
+  m.pm = __builtin_amdgcn_readlane(x, 4);
+  m.p2 = __builtin_amdgcn_readlane(x, 5);
+  return m;
+}
+
 // ---- bottom-up: drainage area -------------------------------------------------------------------
 // FlowAccumulationSingleFlowDirection, src/FlowRouting.f90:123-143: a = dx*dy*precip, donors added
 // from the highest index down; a base-level node finally adds its own area to itself (:138 with
@@ -353,13 +377,6 @@ __global__ __launch_bounds__(FSX_WG) void k_area_sweep(const InstParams* __restr
   const int* ls = lstart + P.aux;
   const int4* gr = grec + 2 * (long long)P.aux;
   const int G = ngroups[inst];
-  auto meta = [&](int g) {
-    GroupMeta m;
-    if (g < 0 || g >= G) { m.lo = m.hi = m.p0 = m.p1 = m.pm = m.p2 = 0; return m; }
-    const int4 a = gr[2 * g], b = gr[2 * g + 1];
-    m.lo = a.x; m.hi = a.y; m.p0 = a.z; m.p1 = a.w; m.pm = b.x; m.p2 = b.y;
-    return m;
-  };
   AreaRec r0, r1;
   int rl = 0;
   auto prefetch = [&](const GroupMeta& m) {
@@ -369,8 +386,8 @@ __global__ __launch_bounds__(FSX_WG) void k_area_sweep(const InstParams* __restr
     if (tid + FSX_WG < np) area_load(r1, off + m.p0 + tid + FSX_WG, arec);
     if (tid < m.hi - m.lo) rl = ls[m.lo + tid];
   };
-  GroupMeta cur = meta(G - 1);
-  GroupMeta nxt_m = meta(G - 2);
+  GroupMeta cur = meta_bcast(meta_lane(gr, G, G - 1));
+  GroupMeta nxt_m = meta_bcast(meta_lane(gr, G, G - 2));
   prefetch(cur);
   for (int g = G - 1; g >= 0; g--) {
     const int np = cur.p1 - cur.p0;
@@ -391,41 +408,38 @@ __global__ __launch_bounds__(FSX_WG) void k_area_sweep(const InstParams* __restr
       if (tid == 0) sL[cur.hi - cur.lo] = cur.p1;
     }
     lds_barrier();
-    const GroupMeta after = meta(g - 2);  // two groups ahead: in flight while this group runs
+    const int after = meta_lane(gr, G, g - 2);  // two groups ahead: in flight while this group runs
     if (g > 0) prefetch(nxt_m);
-    for (int l = cur.hi - 1; l >= cur.lo; l--) {
-      const int b0 = staged ? sL[l - cur.lo] : cur.p0;
-      const int w = (staged ? sL[l - cur.lo + 1] : cur.p1) - b0;
-      const int bn = b0 + w;
-      const int wn = (l + 1 < cur.hi) ? sL[l - cur.lo + 2] - bn : cur.p2 - bn;
+    auto stg = [&](AreaRec& r, int si) {
+      r.a = sA[si];
+#pragma unroll
+      for (int k = 0; k < 8; k++) r.d[k] = sD[8 * si + k];
+    };
+    // one level: w positions from b0, record x0 of position tid when staged (position tid + FSX_WG,
+    // rare, is read from the staging area); the level above (donors) has wn positions from b0 + w
+    auto level = [&](int l, int b0, int w, int wn, const AreaRec& x0) {
       double* cv = sm + ((l & 1) ? vcap : 0);
       const double* nv = sm + ((l & 1) ? 0 : vcap);
+      const int bn = b0 + w;
       const bool nin = wn <= vcap;
-      // FAST: records staged and the level above in LDS -- the body touches no global loads, so no
-      // wait on in-flight prefetches or stores is needed between levels
-      auto body = [&](auto fast_tag) {
+      // FAST: records staged and the level above in LDS -- no global loads, and the eight donor
+      // reads are issued together (an unused slot reads entry 0 and adds an exact +0.0, keeping
+      // the reference's summation order)
+      auto one = [&](int i, const AreaRec& r, auto fast_tag) {
         constexpr bool FAST = decltype(fast_tag)::value;
-        for (int i = tid; i < w; i += FSX_WG) {
-          double a;
-          int q[8];
-          if constexpr (FAST) {
-            const int si = b0 - cur.p0 + i;
-            a = sA[si];
-#pragma unroll
-            for (int k = 0; k < 8; k++) q[k] = sD[8 * si + k];
-          } else {
-            AreaRec r;
-            area_load(r, off + b0 + i, arec);
-            a = r.a;
-#pragma unroll
-            for (int k = 0; k < 8; k++) q[k] = r.d[k];
-          }
-          // unused slots add an exact +0.0, so the sum keeps the reference's order
+        {
+          double a = r.a;
           double dv[8];
 #pragma unroll
           for (int k = 0; k < 8; k++) {
-            if constexpr (FAST) dv[k] = q[k] < 0 ? 0.0 : nv[q[k]];
-            else dv[k] = q[k] < 0 ? 0.0 : (nin ? nv[q[k]] : Al[off + bn + q[k]]);
+            const int q = r.d[k];
+            if constexpr (FAST) dv[k] = nv[q < 0 ? 0 : q];
+            else dv[k] = q < 0 ? 0.0 : (nin ? nv[q] : Al[off + bn + q]);
+          }
+          if constexpr (FAST) {
+            __builtin_amdgcn_sched_barrier(0);  // all eight reads in flight before any is used
+#pragma unroll
+            for (int k = 0; k < 8; k++) dv[k] = r.d[k] < 0 ? 0.0 : dv[k];
           }
 #pragma unroll
           for (int k = 0; k < 8; k++) a = a + dv[k];
@@ -434,12 +448,49 @@ __global__ __launch_bounds__(FSX_WG) void k_area_sweep(const InstParams* __restr
           Al[off + b0 + i] = a;
         }
       };
-      if (staged && nin) body(std::true_type{});
-      else body(std::false_type{});
+      AreaRec x1;
+      if (staged && tid + FSX_WG < w) stg(x1, b0 - cur.p0 + tid + FSX_WG);
+      if (staged && nin) {
+        if (tid < w) one(tid, x0, std::true_type{});
+        if (tid + FSX_WG < w) one(tid + FSX_WG, x1, std::true_type{});
+      } else if (staged) {
+        if (tid < w) one(tid, x0, std::false_type{});
+        if (tid + FSX_WG < w) one(tid + FSX_WG, x1, std::false_type{});
+      } else {
+        for (int i = tid; i < w; i += FSX_WG) {
+          AreaRec r;
+          area_load(r, off + b0 + i, arec);
+          one(i, r, std::false_type{});
+        }
+      }
       if (w > vcap) __syncthreads(); else lds_barrier();
+    };
+    if (staged) {
+      // software-pipelined over the group's levels: the bounds of level l-1 and its records are read
+      // from LDS while level l is solved, so a level's critical path is its donor reads alone
+      // level starts of the group: lane k of every wave holds that of level lo+k (k < 64), so a
+      // level's bounds are a register read (readlane) rather than an LDS round trip
+      const int lsv = (tid & 63) <= cur.hi - cur.lo ? sL[tid & 63] : 0;
+      auto st = [&](int k) { return k < 64 ? __builtin_amdgcn_readlane(lsv, k) : sL[k]; };
+      int l = cur.hi - 1;
+      int A = st(l - cur.lo), B = cur.p1, C = cur.p2;   // level l = [A, B), level l+1 = [B, C)
+      int Pm = l - 1 >= cur.lo ? st(l - 1 - cur.lo) : 0;  // start of level l-1
+      AreaRec x0;
+      if (tid < B - A) stg(x0, A - cur.p0 + tid);
+      for (; l >= cur.lo; l--) {
+        const int Pm2 = l - 2 >= cur.lo ? st(l - 2 - cur.lo) : 0;
+        AreaRec y0;
+        if (l - 1 >= cur.lo && tid < A - Pm) stg(y0, Pm - cur.p0 + tid);
+        level(l, A, B - A, C - B, x0);
+        C = B; B = A; A = Pm; Pm = Pm2;
+        x0 = y0;
+      }
+    } else {
+      AreaRec none;
+      level(cur.lo, cur.p0, cur.p1 - cur.p0, cur.p2 - cur.p1, none);
     }
     cur = nxt_m;
-    nxt_m = after;
+    nxt_m = meta_bcast(after);
   }
 }
 
@@ -523,13 +574,6 @@ __global__ __launch_bounds__(FSX_WG) void k_spl_sweep(
   const bool gzero = s_gmax < 2.2250738585072014e-308;
   int ngs = 0;
   unsigned long long my_newton = 0;
-  auto meta = [&](int gg) {
-    GroupMeta m;
-    if (gg < 0 || gg >= G) { m.lo = m.hi = m.p0 = m.p1 = m.pm = m.p2 = 0; return m; }
-    const int4 a = gr[2 * gg], b = gr[2 * gg + 1];
-    m.lo = a.x; m.hi = a.y; m.p0 = a.z; m.p1 = a.w; m.pm = b.x; m.p2 = b.y;
-    return m;
-  };
   while (s_err > s_tol && ngs < 99) {
     ngs++;
     const bool first = ngs == 1;
@@ -586,8 +630,8 @@ __global__ __launch_bounds__(FSX_WG) void k_spl_sweep(
       if (tid + FSX_WG < np) spl_load(r1, off + m.p0 + tid + FSX_WG, srec, Hl, first);
       if (tid < m.hi - m.lo) rl = ls[m.lo + tid];
     };
-    GroupMeta cur = meta(0);
-    GroupMeta nxt_m = meta(1);
+    GroupMeta cur = meta_bcast(meta_lane(gr, G, 0));
+    GroupMeta nxt_m = meta_bcast(meta_lane(gr, G, 1));
     prefetch(cur);
     for (int gg = 0; gg < G; gg++) {
       const int np = cur.p1 - cur.p0;
@@ -602,121 +646,151 @@ __global__ __launch_bounds__(FSX_WG) void k_spl_sweep(
         if (tid == 0) sL[cur.hi - cur.lo] = cur.p1;
       }
       lds_barrier();
-      const GroupMeta after = meta(gg + 2);
+      const int after = meta_lane(gr, G, gg + 2);
       if (gg + 1 < G) prefetch(nxt_m);
-      for (int l = cur.lo; l < cur.hi; l++) {
-        const int b0 = staged ? sL[l - cur.lo] : cur.p0;
-        const int w = (staged ? sL[l - cur.lo + 1] : cur.p1) - b0;
-        const int bp = l == cur.lo ? cur.pm : sL[l - 1 - cur.lo], wp = l > 0 ? b0 - bp : 0;
+      auto stg = [&](SplRec& r, int si) {
+        r.o = sO[si]; r.q = sQ[si]; r.f = sF[si]; r.t = sT[si]; r.h = sH[si];
+      };
+      // one level: w positions from b0, receivers in the level from bp; record x0 of position tid
+      // when staged (position tid + FSX_WG, rare, is read from the staging area)
+      auto level = [&](int l, int b0, int w, int bp, const SplRec& x0) {
+        const int wp = l > 0 ? b0 - bp : 0;
         const int cb = l & 1, pb = cb ^ 1;
         const bool pin = wp <= vcap;
         // FAST: staged records, receiver level in LDS, g == 0 -- no global loads in the body
-        auto body = [&](auto fast_tag) {
+        auto one = [&](int i, const SplRec& rr, auto fast_tag) {
           constexpr bool FAST = decltype(fast_tag)::value;
+          const int v = rr.o;
+          const double t = rr.t;
+          double hv = rr.h, wv, hw = t;
+          int sl = v;
+          if (l == 0) {
+            wv = t;
+            if (!gzero && !fill_only) lwv[off + v] = 0.0;
+          } else {
+            const int q = rr.q;
+            double hr, w_rcv;
+            if constexpr (FAST) { hr = BH(pb)[q]; w_rcv = BW(pb)[q]; }
+            else { hr = pin ? BH(pb)[q] : Hl[off + bp + q]; w_rcv = pin ? BW(pb)[q] : Wl[off + bp + q]; }
+            if (first) {
+              double hwr;
+              if constexpr (FAST) hwr = BHW(pb)[q];
+              else hwr = pin ? BHW(pb)[q] : HWl[off + bp + q];
+              hw = hwr > t ? hwr + 1.0e-8 : t;
+            }
+            if (fill_only) {
+              wv = w_rcv;
+            } else {
+              double e;
+              if constexpr (FAST) e = t;
+              else e = gzero ? t : ELl[off + b0 + i];
+              if (e > w_rcv) {
+                const double fact = rr.f;
+                if (n_is_one) {
+                  double f = e, df = 1.0;
+                  f = f + fact * hr;
+                  df = df + fact;
+                  hv = f / df;
+                } else {
+                  const double omega = 0.875 / n, tolp = P.newton_tol;
+                  double errp = 2.0 * tolp;
+                  const double h0 = e;
+                  const bool act = fact >= 0.0;
+                  int it = 0;
+                  while (errp > tolp) {
+                    double f = hv - h0, df = 1.0;
+                    if (act) {
+                      const double x = fmax(0.0, hv - hr);
+                      double pn, pn1;
+                      npow(x, n, nkind, pn, pn1);
+                      f = f + fact * pn;
+                      df = df + fact * n * pn1;
+                    }
+                    const double hn = hv - f / df;
+                    errp = fabs(hn - hv);
+                    hv = hv * (1.0 - omega) + hn * omega;
+                    if (++it > 100000) { atomicOr(err + inst, FSX_ERR_NEWTON); break; }
+                  }
+                  my_newton += it;
+                }
+                if (!gzero) lwv[off + v] = 0.0;
+                if (hv < w_rcv) hv = w_rcv;
+              } else {
+                hv = e;
+                if constexpr (FAST) sl = -1;
+                else sl = gzero ? -1 : (pin ? BLS(pb)[q] : LSl[off + bp + q]);
+                if (!gzero && sl >= 0) atomicAdd(lwv + off + sl, w_rcv - hv);
+              }
+              wv = w_rcv > hv ? w_rcv : hv;
+            }
+          }
+          if (w <= vcap) {
+            BH(cb)[i] = hv;
+            BW(cb)[i] = wv;
+            if (first) BHW(cb)[i] = hw;
+            BLS(cb)[i] = sl;
+          }
+          // level-ordered copies only where a later reader needs them: wide levels (kept out of LDS)
+          // and the Gauss-Seidel iterations; grid-ordered outputs h, hwater, lake_depth always
+          const long long gp = off + b0 + i;
+          const bool wide = w > vcap;
+          // results stay in level order (coalesced stores); k_spl_post scatters h, hwater and
+          // lake_depth to grid order
+          if (!fill_only) {
+            Hl[gp] = hv;
+            if (wide) Wl[gp] = wv;
+            if (wide || !gzero) LSl[gp] = sl;
+            if (!gzero) { water[off + v] = wv; lsill[off + v] = sl; }
+          } else if (wide) {
+            Wl[gp] = wv;
+          }
+          if (first) HWl[gp] = hw;
+        };
+        if (staged) {
+          SplRec x1;
+          if (tid + FSX_WG < w) stg(x1, b0 - cur.p0 + tid + FSX_WG);
+          if (pin && gzero) {
+            if (tid < w) one(tid, x0, std::true_type{});
+            if (tid + FSX_WG < w) one(tid + FSX_WG, x1, std::true_type{});
+          } else {
+            if (tid < w) one(tid, x0, std::false_type{});
+            if (tid + FSX_WG < w) one(tid + FSX_WG, x1, std::false_type{});
+          }
+        } else {
           for (int i = tid; i < w; i += FSX_WG) {
             SplRec rr;
-            if constexpr (FAST) {
-              const int si = b0 - cur.p0 + i;
-              rr.o = sO[si]; rr.q = sQ[si]; rr.f = sF[si]; rr.t = sT[si]; rr.h = sH[si];
-            } else if (staged) {
-              const int si = b0 - cur.p0 + i;
-              rr.o = sO[si]; rr.q = sQ[si]; rr.f = sF[si]; rr.t = sT[si]; rr.h = sH[si];
-            } else {
-              spl_load(rr, off + b0 + i, srec, Hl, first);
-            }
-            const int v = rr.o;
-            const double t = rr.t;
-            double hv = rr.h, wv, hw = t;
-            int sl = v;
-            if (l == 0) {
-              wv = t;
-              if (!gzero && !fill_only) lwv[off + v] = 0.0;
-            } else {
-              const int q = rr.q;
-              double hr, w_rcv;
-              if constexpr (FAST) { hr = BH(pb)[q]; w_rcv = BW(pb)[q]; }
-              else { hr = pin ? BH(pb)[q] : Hl[off + bp + q]; w_rcv = pin ? BW(pb)[q] : Wl[off + bp + q]; }
-              if (first) {
-                double hwr;
-                if constexpr (FAST) hwr = BHW(pb)[q];
-                else hwr = pin ? BHW(pb)[q] : HWl[off + bp + q];
-                hw = hwr > t ? hwr + 1.0e-8 : t;
-              }
-              if (fill_only) {
-                wv = w_rcv;
-              } else {
-                double e;
-                if constexpr (FAST) e = t;
-                else e = gzero ? t : ELl[off + b0 + i];
-                if (e > w_rcv) {
-                  const double fact = rr.f;
-                  if (n_is_one) {
-                    double f = e, df = 1.0;
-                    f = f + fact * hr;
-                    df = df + fact;
-                    hv = f / df;
-                  } else {
-                    const double omega = 0.875 / n, tolp = P.newton_tol;
-                    double errp = 2.0 * tolp;
-                    const double h0 = e;
-                    const bool act = fact >= 0.0;
-                    int it = 0;
-                    while (errp > tolp) {
-                      double f = hv - h0, df = 1.0;
-                      if (act) {
-                        const double x = fmax(0.0, hv - hr);
-                        double pn, pn1;
-                        npow(x, n, nkind, pn, pn1);
-                        f = f + fact * pn;
-                        df = df + fact * n * pn1;
-                      }
-                      const double hn = hv - f / df;
-                      errp = fabs(hn - hv);
-                      hv = hv * (1.0 - omega) + hn * omega;
-                      if (++it > 100000) { atomicOr(err + inst, FSX_ERR_NEWTON); break; }
-                    }
-                    my_newton += it;
-                  }
-                  if (!gzero) lwv[off + v] = 0.0;
-                  if (hv < w_rcv) hv = w_rcv;
-                } else {
-                  hv = e;
-                  if constexpr (FAST) sl = -1;
-                  else sl = gzero ? -1 : (pin ? BLS(pb)[q] : LSl[off + bp + q]);
-                  if (!gzero && sl >= 0) atomicAdd(lwv + off + sl, w_rcv - hv);
-                }
-                wv = w_rcv > hv ? w_rcv : hv;
-              }
-            }
-            if (w <= vcap) {
-              BH(cb)[i] = hv;
-              BW(cb)[i] = wv;
-              if (first) BHW(cb)[i] = hw;
-              BLS(cb)[i] = sl;
-            }
-            // level-ordered copies only where a later reader needs them: wide levels (kept out of LDS)
-            // and the Gauss-Seidel iterations; grid-ordered outputs h, hwater, lake_depth always
-            const long long gp = off + b0 + i;
-            const bool wide = w > vcap;
-            // results stay in level order (coalesced stores); k_spl_post scatters h, hwater and
-            // lake_depth to grid order
-            if (!fill_only) {
-              Hl[gp] = hv;
-              if (wide) Wl[gp] = wv;
-              if (wide || !gzero) LSl[gp] = sl;
-              if (!gzero) { water[off + v] = wv; lsill[off + v] = sl; }
-            } else if (wide) {
-              Wl[gp] = wv;
-            }
-            if (first) HWl[gp] = hw;
+            spl_load(rr, off + b0 + i, srec, Hl, first);
+            one(i, rr, std::false_type{});
           }
-        };
-        if (staged && pin && gzero) body(std::true_type{});
-        else body(std::false_type{});
+        }
         if (w > vcap) __syncthreads(); else lds_barrier();
+      };
+      if (staged) {
+        // software-pipelined over the group's levels: the bounds of level l+1 and its records are
+        // read from LDS while level l is solved, so a level's critical path is its receiver reads
+        // level starts of the group: lane k of every wave holds that of level lo+k (k < 64), so a
+        // level's bounds are a register read (readlane) rather than an LDS round trip
+        const int lsv = (tid & 63) <= cur.hi - cur.lo ? sL[tid & 63] : 0;
+        auto st = [&](int k) { return k < 64 ? __builtin_amdgcn_readlane(lsv, k) : sL[k]; };
+        int l = cur.lo;
+        int A = st(0), B = st(1), Bp = cur.pm;                  // level l = [A, B), level l-1 from Bp
+        int Nn = l + 2 <= cur.hi ? st(l + 2 - cur.lo) : 0;     // level l+1 = [B, Nn)
+        SplRec x0;
+        if (tid < B - A) stg(x0, A - cur.p0 + tid);
+        for (; l < cur.hi; l++) {
+          const int Nn2 = l + 3 <= cur.hi ? st(l + 3 - cur.lo) : 0;
+          SplRec y0;
+          if (l + 1 < cur.hi && tid < Nn - B) stg(y0, B - cur.p0 + tid);
+          level(l, A, B - A, Bp, x0);
+          Bp = A; A = B; B = Nn; Nn = Nn2;
+          x0 = y0;
+        }
+      } else {
+        SplRec none;
+        level(cur.lo, cur.p0, cur.p1 - cur.p0, cur.pm, none);
       }
       cur = nxt_m;
-      nxt_m = after;
+      nxt_m = meta_bcast(after);
     }
     __syncthreads();
     if (gzero || fill_only) {
@@ -789,7 +863,8 @@ void launch_levels(DevState& S, hipStream_t st) {
 }
 
 void launch_area_prep(DevState& S, hipStream_t st) {
-  k_area_prep<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.J, S.dmask, S.pos, S.lstart, S.precip, S.arec);
+  k_area_prep<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.J, S.dmask, S.pos, S.lstart, S.precip, S.flags,
+                                               S.arec);
 }
 
 void launch_area(DevState& S, hipStream_t st) {
@@ -805,7 +880,7 @@ void launch_area_sweep(DevState& S, hipStream_t st) {
 void launch_spl_fact(DevState& S, hipStream_t st) {
   launch_fill_int(reinterpret_cast<int*>(S.redk), 4LL * S.ninst, 0, st);
   k_spl_fact<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.J, S.rec, S.pos, S.lstart, S.Al, S.area, S.length,
-                                               S.kfint, S.ht, S.srec, S.lsed, S.g, S.redk);
+                                               S.kfint, S.ht, S.srec, S.lsed, S.g, S.redk, S.flags);
 }
 
 static void spl_sweep_launch(DevState& S, hipStream_t st, int fill_only) {
