@@ -635,58 +635,107 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_nr_pos(const InstParams* __restri
 #define BORUVKA_UNROLL 4
 static int g_boruvka_lds = 0;  // component ids held in LDS (ints), set from the device's LDS size
 
+// The edge scan of a round reads the edges still joining two components at the start of the
+// previous round and writes those still joining two now (components only merge, so an edge inside a
+// component never matters again); the lists shrink with the components.  List k is (a, c, rank)
+// triples; list 0 is the sorted edge set itself.
+struct EdgeList {
+  int *a, *c, *k;
+};
+
 template <bool INLDS>
 __device__ __forceinline__ void boruvka_body(const int nb, const int n, const int* E0, const int* E1, const int* NP,
-                                             const int* SI, int* C, int* BST, int* BN, uint8_t* IT, int* s_any,
-                                             int* s_chg) {
-  const int tid = threadIdx.x;
+                                             const int* SI, int* C, unsigned long long* BST, int* BN, uint8_t* IT,
+                                             EdgeList LA, EdgeList LB, int* s_any, int* s_chg, int* s_cnt) {
+  constexpr int U = BORUVKA_UNROLL;
+  constexpr unsigned long long NONE = ~0ull;
+  const int tid = threadIdx.x, lane = tid & 63;
+  int ne = n;
   for (int round = 0; round < 64; round++) {
-    for (int b = tid; b < nb; b += FSX_WG) BST[b] = INT_MAX;
-    if (tid == 0) *s_any = 0;
+    const EdgeList out = (round & 1) ? LB : LA;
+    const EdgeList in = (round & 1) ? LA : LB;  // read from round 1 on
+    for (int b = tid; b < nb; b += FSX_WG) BST[b] = NONE;
+    if (tid == 0) { *s_any = 0; *s_cnt = 0; }
     __syncthreads();
-    for (int e0 = tid; e0 < n; e0 += FSX_WG * BORUVKA_UNROLL) {
-      int a[BORUVKA_UNROLL], c[BORUVKA_UNROLL], k[BORUVKA_UNROLL];
+    // lowest-ranked live edge of every component, with the component at its other end:
+    // (rank << 32 | other) -- ranks are distinct, so the minimum names one edge
+    for (int e0 = tid; e0 < ne; e0 += FSX_WG * U) {
+      int a[U], c[U], k[U];
 #pragma unroll
-      for (int u = 0; u < BORUVKA_UNROLL; u++) {
+      for (int u = 0; u < U; u++) {
         const int e = e0 + u * FSX_WG;
-        a[u] = e < n ? E0[e] : 0;
-        c[u] = e < n ? E1[e] : 0;
-        k[u] = e < n ? NP[e] : INT_MAX;
+        const bool in_range = e < ne;
+        a[u] = in_range ? (round ? in.a[e] : E0[e]) : 0;
+        c[u] = in_range ? (round ? in.c[e] : E1[e]) : 0;
+        k[u] = in_range ? (round ? in.k[e] : NP[e]) : INT_MAX;
       }
 #pragma unroll
-      for (int u = 0; u < BORUVKA_UNROLL; u++) {
+      for (int u = 0; u < U; u++) {
         a[u] = C[a[u]];
         c[u] = C[c[u]];
       }
 #pragma unroll
-      for (int u = 0; u < BORUVKA_UNROLL; u++)
-        if (k[u] != INT_MAX && a[u] != c[u]) {
-          atomicMin(BST + a[u], k[u]);
-          atomicMin(BST + c[u], k[u]);
+      for (int u = 0; u < U; u++) {
+        const bool live = k[u] != INT_MAX && a[u] != c[u];
+        if (live) {
+          const unsigned long long kr = (unsigned long long)(unsigned)k[u] << 32;
+          atomicMin(BST + a[u], kr | (unsigned)c[u]);
+          atomicMin(BST + c[u], kr | (unsigned)a[u]);
         }
+        // keep the live edge (as component ids) for the next round: one LDS atomic per wave
+        const unsigned long long m = __ballot(live);
+        const int cnt = __popcll(m);
+        int base = 0;
+        if (lane == 0 && cnt) base = atomicAdd(s_cnt, cnt);
+        base = __shfl(base, 0, 64);
+        if (live) {
+          const int r = base + __popcll(m & ((1ull << lane) - 1ull));
+          out.a[r] = a[u];
+          out.c[r] = c[u];
+          out.k[r] = k[u];
+        }
+      }
     }
     __syncthreads();
-    for (int cc = tid; cc < nb; cc += FSX_WG) {
-      if (C[cc] != cc) continue;
-      BN[cc] = cc;
-      const int kk = BST[cc];
-      if (kk == INT_MAX) continue;
-      const int e = SI[kk];
-      IT[e] = 1;
-      const int c0 = C[E0[e]], c1 = C[E1[e]];
-      const int o = c0 == cc ? c1 : c0;
-      BN[cc] = (BST[o] == kk) ? (cc < o ? cc : o) : o;
-      *s_any = 1;
+    ne = *s_cnt;
+    // every component joins the other end of its edge (the smaller id of a mutual pair stays root)
+    for (int c0 = tid; c0 < nb; c0 += FSX_WG * U) {
+      unsigned long long v[U], w[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int cc = c0 + u * FSX_WG;
+        v[u] = (cc < nb && C[cc] == cc) ? BST[cc] : NONE;
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) w[u] = v[u] != NONE ? BST[(int)(v[u] & 0xffffffffu)] : NONE;
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int cc = c0 + u * FSX_WG;
+        if (cc >= nb || C[cc] != cc) continue;
+        if (v[u] == NONE) { BN[cc] = cc; continue; }
+        const int kk = (int)(v[u] >> 32), o = (int)(v[u] & 0xffffffffu);
+        IT[SI[kk]] = 1;
+        BN[cc] = (int)(w[u] >> 32) == kk ? (cc < o ? cc : o) : o;
+        *s_any = 1;
+      }
     }
     __syncthreads();
     if (!*s_any) break;
     for (;;) {
       if (tid == 0) *s_chg = 0;
       __syncthreads();
-      for (int cc = tid; cc < nb; cc += FSX_WG) {
-        if (C[cc] != cc) continue;
-        const int x = BN[cc], y = BN[x];
-        if (x != y) { BN[cc] = y; *s_chg = 1; }
+      for (int c0 = tid; c0 < nb; c0 += FSX_WG * U) {
+        int x[U], y[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          const int cc = c0 + u * FSX_WG;
+          x[u] = (cc < nb && C[cc] == cc) ? BN[cc] : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) y[u] = x[u] >= 0 ? BN[x[u]] : -1;
+#pragma unroll
+        for (int u = 0; u < U; u++)
+          if (x[u] != y[u]) { BN[c0 + u * FSX_WG] = y[u]; *s_chg = 1; }
       }
       __syncthreads();
       if (!*s_chg) break;
@@ -696,14 +745,19 @@ __device__ __forceinline__ void boruvka_body(const int nb, const int n, const in
   }
 }
 
+// Edge lists of the rounds reuse the edge sort's buffers, free once the ranks are known: list A in
+// Lasc/Rasc/segs, list B in the int views of cw (a: first half, c: second half) and skey.
 __global__ __launch_bounds__(FSX_WG) void k_boruvka(const InstParams* __restrict__ PP, const int* __restrict__ nbasins,
                                                     const int* __restrict__ nconn, const int* __restrict__ cb0,
                                                     const int* __restrict__ cb1, const int* __restrict__ npos,
                                                     const int* __restrict__ sid, int* __restrict__ comp,
-                                                    int* __restrict__ best, int* __restrict__ bnew,
-                                                    uint8_t* __restrict__ intree, int* __restrict__ ntree, int ldscap) {
+                                                    unsigned long long* __restrict__ best, int* __restrict__ bnew,
+                                                    uint8_t* __restrict__ intree, int* __restrict__ ntree,
+                                                    int* __restrict__ Lasc, int* __restrict__ Rasc,
+                                                    int* __restrict__ segs, double* __restrict__ cw,
+                                                    double* __restrict__ skey, int ldscap) {
   extern __shared__ int sC[];
-  __shared__ int s_any, s_chg;
+  __shared__ int s_any, s_chg, s_cnt;
   const int inst = blockIdx.x;
   const InstParams& P = PP[inst];
   if (!P.active || !P.run_spl) return;
@@ -715,13 +769,17 @@ __global__ __launch_bounds__(FSX_WG) void k_boruvka(const InstParams* __restrict
   for (int b = tid; b < nb; b += FSX_WG) C[b] = b;
   for (int e = tid; e < n; e += FSX_WG) IT[e] = 0;
   __syncthreads();
+  const long long ecap = (long long)FSX_EDGE_CAP * P.nn;
+  const EdgeList LA = {Lasc + eb, Rasc + eb, segs + eb};
+  int* cwi = reinterpret_cast<int*>(cw) + 2 * eb;
+  const EdgeList LB = {cwi, cwi + ecap, reinterpret_cast<int*>(skey) + 2 * eb};
   if (n > 0) {
     if (inlds)
-      boruvka_body<true>(nb, n, cb0 + eb, cb1 + eb, npos + eb, sid + eb, sC, best + P.off, bnew + P.off, IT, &s_any,
-                         &s_chg);
+      boruvka_body<true>(nb, n, cb0 + eb, cb1 + eb, npos + eb, sid + eb, sC, best + P.off, bnew + P.off, IT, LA, LB,
+                         &s_any, &s_chg, &s_cnt);
     else
       boruvka_body<false>(nb, n, cb0 + eb, cb1 + eb, npos + eb, sid + eb, comp + P.off, best + P.off, bnew + P.off, IT,
-                          &s_any, &s_chg);
+                          LA, LB, &s_any, &s_chg, &s_cnt);
   }
   int cnt = 0;
   for (int e = tid; e < n; e += FSX_WG) cnt += IT[e];
@@ -962,7 +1020,8 @@ void launch_lm_edges(DevState& S, hipStream_t st) {
 void launch_lm_mst(DevState& S, hipStream_t st) {
   launch_fill_int(S.ntree, S.ninst, 0, st);
   k_boruvka<<<S.ninst, FSX_WG, (size_t)g_boruvka_lds * sizeof(int), st>>>(
-      S.P, S.nbasins, S.nconn, S.cb0, S.cb1, S.npos, S.sid, S.comp, S.best, S.bnew, S.intree, S.ntree, g_boruvka_lds);
+      S.P, S.nbasins, S.nconn, S.cb0, S.cb1, S.npos, S.sid, S.comp, S.best, S.bnew, S.intree, S.ntree, S.Lasc, S.Rasc,
+      S.segs, S.cw, S.skey, g_boruvka_lds);
 }
 
 void launch_lm_orient(DevState& S, hipStream_t st) {

@@ -31,7 +31,8 @@ struct DevState {
 
   // ---- per basin (node offsets) ----
   int *outlet = nullptr, *bsize = nullptr, *soff = nullptr, *ecount = nullptr, *eoff = nullptr;
-  int *comp = nullptr, *bpar = nullptr, *bnew = nullptr, *best = nullptr, *tdeg = nullptr, *toff = nullptr;
+  int *comp = nullptr, *bpar = nullptr, *bnew = nullptr, *tdeg = nullptr, *toff = nullptr;
+  unsigned long long* best = nullptr;  // spanning forest: (rank << 32 | other component) of a component's edge
   int *tadj = nullptr, *bfront = nullptr;
   int *trev = nullptr;                   // Euler tour: reverse slot of each directed tree edge (2 per node)
   unsigned long long *tjmp = nullptr;    // Euler tour: (distance to end, next slot), 2 per node
