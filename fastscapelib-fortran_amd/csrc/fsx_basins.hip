@@ -90,13 +90,19 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_root_rank(const InstParams* __res
 __global__ __launch_bounds__(FSX_CHUNK) void k_basin_label(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
                                                            const unsigned long long* __restrict__ J,
                                                            const int* __restrict__ rootrank, const uint8_t* __restrict__ bc,
-                                                           int* __restrict__ basin, int* __restrict__ bsize) {
+                                                           const double* __restrict__ h, const int* __restrict__ first,
+                                                           int* __restrict__ basin, int* __restrict__ bsize,
+                                                           WalkNode* __restrict__ wnode) {
   NODE_PROLOGUE
   if (!live) return;
   const int r = (int)(unsigned)(J[P.off + v] & 0xffffffffu);
   const int b = rootrank[P.off + r];
   const bool closed = !bc[P.off + r];
-  basin[P.off + v] = closed ? b : -(b + 1);  // signed: open basins negative
+  const int bv = closed ? b : -(b + 1);  // signed: open basins negative
+  basin[P.off + v] = bv;
+  reinterpret_cast<ulonglong2*>(wnode)[P.off + v] =
+      make_ulonglong2((unsigned long long)__double_as_longlong(h[P.off + v]),
+                      (unsigned long long)(unsigned)bv | ((unsigned long long)(unsigned)first[P.off + v] << 32));
   if (closed) atomicAdd(bsize + P.off + b, 1);
 }
 
@@ -128,9 +134,8 @@ __global__ __launch_bounds__(FSX_WG) void k_bsize_scan(const InstParams* __restr
 #define WALK_GX 64  // blocks per landscape; each thread walks basins b = x, x + WALK_GX*64, ...
 __global__ __launch_bounds__(64) void k_basin_walk(const InstParams* __restrict__ PP, const int* __restrict__ nbasins,
                                                    const uint8_t* __restrict__ bc, const int* __restrict__ outlet,
-                                                   const int* __restrict__ rec, const int* __restrict__ first,
-                                                   const int* __restrict__ next,
-                                                   const int* __restrict__ basin, const double* __restrict__ h,
+                                                   const int* __restrict__ rec, const int* __restrict__ next,
+                                                   const WalkNode* __restrict__ wnode,
                                                    const int* __restrict__ soff, const int* __restrict__ bsize,
                                                    int* __restrict__ ecount, int* __restrict__ sB, int* __restrict__ sp,
                                                    int* __restrict__ sq, double* __restrict__ sw, int* __restrict__ err,
@@ -146,81 +151,107 @@ __global__ __launch_bounds__(64) void k_basin_walk(const InstParams* __restrict_
   const InstParams& P = PP[inst];
   if (!P.active || !P.run_spl) return;
   const int nb = nbasins[inst];
-  const int2 it = make_int2(inst, 0);
-  for (int b = sub * 64 + threadIdx.x; b < nb; b += WALK_GX * 64) {
-    const int off = P.off, nx = P.nx, ny = P.ny;
-    if (bc[off + outlet[off + b]]) continue;  // open basin: no scan
-    const int r = outlet[off + b];
-    const int* R = rec + off;
-    const int* FI = first + off;
-    const int* NX = next + off;
-    const int* BS = basin + off;
-    const double* H = h + off;
-    const long long sbase = (long long)FSX_SLAB_CAP * (off + soff[off + b]);
-    int* TB = sB + sbase;
-    int* TP = sp + sbase;
-    int* TQ = sq + sbase;
-    double* TW = sw + sbase;
-    const int cap = FSX_SLAB_CAP * bsize[off + b];
-    int ne = 0;
-    int v = r;
-    for (int visits = 0;; visits++) {
-      if (visits > P.nn) { atomicOr(err + it.x, FSX_ERR_LEVELS); break; }
-      // scan the 8 neighbours of v (no wrap), src/LocalMinima.f90:281-320
-      const int i = v % nx, j = v / nx;
-      const double hv = H[v];
-      const int fv = FI[v];
-      int qs[8], bs[8];
-      double hq[8];
-#pragma unroll
-      for (int k = 0; k < 8; k++) {
-        const int ki = i + II[k], kj = j + JJ[k];
-        const bool in = !(ki < 0 || ki >= nx || kj < 0 || kj >= ny);
-        qs[k] = in ? kj * nx + ki : -1;
-        bs[k] = in ? BS[qs[k]] : 0;
-        hq[k] = in ? H[qs[k]] : 0.0;
-      }
-#pragma unroll
-      for (int k = 0; k < 8; k++) {
-        if (qs[k] < 0) continue;
-        const int Bs = bs[k];
-        const bool closedB = Bs >= 0;
-        const int B = closedB ? Bs : -(Bs + 1);
-        if (b >= B && closedB) continue;
-        const double w = hv > hq[k] ? hv : hq[k];
-        int t = 0;
-        const int nl = ne < WALK_TCAP ? ne : WALK_TCAP;
-        while (t < nl && LB[64 * t] != B) t++;
-        if (t == nl) {
-          while (t < ne && TB[t] != B) t++;
-        }
-        if (t == ne) {
-          if (ne >= cap) { atomicOr(err + it.x, FSX_ERR_NCONN); continue; }
-          if (t < WALK_TCAP) { LB[64 * t] = B; LW[64 * t] = w; }
-          else { TB[t] = B; TW[t] = w; }
-          TP[t] = v; TQ[t] = qs[k];
-          ne++;
-        } else {
-          const double tw = t < WALK_TCAP ? LW[64 * t] : TW[t];
-          if (w < tw) {
-            if (t < WALK_TCAP) LW[64 * t] = w; else TW[t] = w;
-            TP[t] = v; TQ[t] = qs[k];
-          }
-        }
-      }
-      // next node in depth-first preorder: first donor, else next sibling up the receiver chain
-      if (fv >= 0) { v = fv; continue; }
-      bool found = false;
-      while (v != r) {
-        const int nx_ = NX[v];
-        if (nx_ >= 0) { v = nx_; found = true; break; }
-        v = R[v];
-      }
-      if (!found) break;
+  const int off = P.off, nx = P.nx, ny = P.ny;
+  const int* R = rec + off;
+  const int* NX = next + off;
+  const ulonglong2* WN = reinterpret_cast<const ulonglong2*>(wnode) + off;
+  // One flat loop of visits: a lane that finishes a basin starts its next one at once, so the lanes
+  // of a wave do not wait for the largest basin of every round (basins differ widely in size).
+  constexpr int STRIDE = WALK_GX * 64;
+  int b = sub * 64 + threadIdx.x - STRIDE;
+  int r = 0, v = 0, ne = 0, cap = 0, visits = 0;
+  int *TB = nullptr, *TP = nullptr, *TQ = nullptr;
+  double* TW = nullptr;
+  auto next_basin = [&]() -> bool {  // the lane's next closed basin; false when none is left
+    for (b += STRIDE; b < nb; b += STRIDE) {
+      const int o = outlet[off + b];
+      if (bc[off + o]) continue;  // open basin: no scan
+      r = o;
+      v = o;
+      ne = 0;
+      visits = 0;
+      const long long sbase = (long long)FSX_SLAB_CAP * (off + soff[off + b]);
+      TB = sB + sbase;
+      TP = sp + sbase;
+      TQ = sq + sbase;
+      TW = sw + sbase;
+      cap = FSX_SLAB_CAP * bsize[off + b];
+      return true;
     }
+    return false;
+  };
+  auto finish = [&]() {
     const int nl = ne < WALK_TCAP ? ne : WALK_TCAP;
     for (int t = 0; t < nl; t++) { TB[t] = LB[64 * t]; TW[t] = LW[64 * t]; }
     ecount[off + b] = ne;
+  };
+  bool live = next_basin();
+  while (live) {
+    if (++visits > P.nn) {  // not a tree: give up on this basin
+      atomicOr(err + inst, FSX_ERR_LEVELS);
+      finish();
+      live = next_basin();
+      continue;
+    }
+    // scan the 8 neighbours of v (no wrap), src/LocalMinima.f90:281-320
+    const int i = v % nx, j = v / nx;
+    const ulonglong2 wv = WN[v];
+    const double hv = __longlong_as_double((long long)wv.x);
+    const int fv = (int)(unsigned)(wv.y >> 32);
+    int qs[8], bs[8];
+    double hq[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const int ki = i + II[k], kj = j + JJ[k];
+      const bool in = !(ki < 0 || ki >= nx || kj < 0 || kj >= ny);
+      qs[k] = in ? kj * nx + ki : -1;
+      const ulonglong2 wq = in ? WN[qs[k]] : make_ulonglong2(0ull, 0ull);
+      bs[k] = (int)(unsigned)(wq.y & 0xffffffffull);
+      hq[k] = __longlong_as_double((long long)wq.x);
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      if (qs[k] < 0) continue;
+      const int Bs = bs[k];
+      const bool closedB = Bs >= 0;
+      const int B = closedB ? Bs : -(Bs + 1);
+      if (b >= B && closedB) continue;
+      const double w = hv > hq[k] ? hv : hq[k];
+      int t = 0;
+      const int nl = ne < WALK_TCAP ? ne : WALK_TCAP;
+      while (t < nl && LB[64 * t] != B) t++;
+      if (t == nl) {
+        while (t < ne && TB[t] != B) t++;
+      }
+      if (t == ne) {
+        if (ne >= cap) { atomicOr(err + inst, FSX_ERR_NCONN); continue; }
+        if (t < WALK_TCAP) { LB[64 * t] = B; LW[64 * t] = w; }
+        else { TB[t] = B; TW[t] = w; }
+        TP[t] = v; TQ[t] = qs[k];
+        ne++;
+      } else {
+        const double tw = t < WALK_TCAP ? LW[64 * t] : TW[t];
+        if (w < tw) {
+          if (t < WALK_TCAP) LW[64 * t] = w; else TW[t] = w;
+          TP[t] = v; TQ[t] = qs[k];
+        }
+      }
+    }
+    // next node in depth-first preorder: first donor, else next sibling up the receiver chain
+    if (fv >= 0) {
+      v = fv;
+      continue;
+    }
+    bool found = false;
+    while (v != r) {
+      const int nx_ = NX[v];
+      if (nx_ >= 0) { v = nx_; found = true; break; }
+      v = R[v];
+    }
+    if (!found) {
+      finish();
+      live = next_basin();
+    }
   }
 }
 
@@ -1001,14 +1032,15 @@ void launch_lm_basins(DevState& S, hipStream_t st) {
   k_root_scan<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.chunkcnt, S.nbasins, S.basin0);
   k_root_rank<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.rec, S.bc, S.chunkcnt, S.rootrank, S.outlet, S.basin0);
   launch_fill_int(S.bsize, S.ntot, 0, st);
-  k_basin_label<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.J, S.rootrank, S.bc, S.basin, S.bsize);
+  k_basin_label<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.J, S.rootrank, S.bc, S.h, S.first, S.basin, S.bsize,
+                                          S.wnode);
   k_bsize_scan<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.nbasins, S.bsize, S.soff, S.ecount);
 }
 
 void launch_lm_walk(DevState& S, hipStream_t st) {
-  k_basin_walk<<<xcd_grid(S.ninst, WALK_GX), 64, 0, st>>>(S.P, S.nbasins, S.bc, S.outlet, S.rec, S.first, S.next,
-                                                           S.basin, S.h, S.soff, S.bsize, S.ecount, S.sB, S.sp, S.sq,
-                                                           S.sw, S.err, S.ninst);
+  k_basin_walk<<<xcd_grid(S.ninst, WALK_GX), 64, 0, st>>>(S.P, S.nbasins, S.bc, S.outlet, S.rec, S.next, S.wnode,
+                                                           S.soff, S.bsize, S.ecount, S.sB, S.sp, S.sq, S.sw, S.err,
+                                                           S.ninst);
 }
 
 void launch_lm_edges(DevState& S, hipStream_t st) {

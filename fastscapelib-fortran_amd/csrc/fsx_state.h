@@ -28,6 +28,7 @@ struct DevState {
   double *HPl = nullptr, *DHl = nullptr, *ELl = nullptr;
   AreaRec64* arec = nullptr;  // per-position drainage-area records
   SplRec32* srec = nullptr;   // per-position stream-power records
+  WalkNode* wnode = nullptr;  // basin walk: (h, basin, first donor) per node
 
   // ---- per basin (node offsets) ----
   int *outlet = nullptr, *bsize = nullptr, *soff = nullptr, *ecount = nullptr, *eoff = nullptr;
