@@ -315,6 +315,7 @@ static int ctx_alloc(long long capacity, int nslots, fsx_ctx** out) {
   A(sid, FSX_EDGE_CAP * N); A(npos, FSX_EDGE_CAP * N); A(Lasc, FSX_EDGE_CAP * N); A(Rasc, FSX_EDGE_CAP * N);
   A(segs, FSX_EDGE_CAP * N); A(cw, FSX_EDGE_CAP * N); A(skey, FSX_EDGE_CAP * N); A(intree, FSX_EDGE_CAP * N);
   A(sB, FSX_SLAB_CAP * N); A(sp, FSX_SLAB_CAP * N); A(sq, FSX_SLAB_CAP * N); A(sw, FSX_SLAB_CAP * N);
+  A(sh, FSX_SLAB_CAP * N);
   A(nbasins, ninst); A(basin0, ninst); A(nconn, ninst); A(nlev, ninst); A(err, ninst); A(ngs, ninst);
   A(ntree, ninst); A(chunkcnt, c->chunk_cap); A(newton, ninst); A(scal, ninst); A(redk, 2 * ninst); A(flags, 64);
   A(nsegs, ninst); A(segctr, ninst); A(maxd, ninst);

@@ -138,8 +138,8 @@ __global__ __launch_bounds__(64) void k_basin_walk(const InstParams* __restrict_
                                                    const WalkNode* __restrict__ wnode,
                                                    const int* __restrict__ soff, const int* __restrict__ bsize,
                                                    int* __restrict__ ecount, int* __restrict__ sB, int* __restrict__ sp,
-                                                   int* __restrict__ sq, double* __restrict__ sw, int* __restrict__ err,
-                                                   int ninst) {
+                                                   int* __restrict__ sq, double* __restrict__ sw, int* __restrict__ sh,
+                                                   int* __restrict__ err, int ninst) {
   __shared__ int lB[WALK_TCAP * 64];
   __shared__ double lW[WALK_TCAP * 64];
   int* LB = lB + threadIdx.x;
@@ -160,8 +160,13 @@ __global__ __launch_bounds__(64) void k_basin_walk(const InstParams* __restrict_
   constexpr int STRIDE = WALK_GX * 64;
   int b = sub * 64 + threadIdx.x - STRIDE;
   int r = 0, v = 0, ne = 0, cap = 0, visits = 0;
-  int *TB = nullptr, *TP = nullptr, *TQ = nullptr;
+  int *TB = nullptr, *TP = nullptr, *TQ = nullptr, *TH = nullptr;
   double* TW = nullptr;
+  // entries beyond WALK_TCAP (large basins) are found through an open-addressing hash over TB:
+  // TH[slot] = entry index, -1 empty; hshift = 0 until the first such entry; hfull once an insert
+  // found no free slot (then a hash miss is confirmed by a scan of those entries)
+  int hshift = 0, hbits = 0;
+  bool hfull = false;
   auto next_basin = [&]() -> bool {  // the lane's next closed basin; false when none is left
     for (b += STRIDE; b < nb; b += STRIDE) {
       const int o = outlet[off + b];
@@ -175,7 +180,10 @@ __global__ __launch_bounds__(64) void k_basin_walk(const InstParams* __restrict_
       TP = sp + sbase;
       TQ = sq + sbase;
       TW = sw + sbase;
+      TH = sh + sbase;
       cap = FSX_SLAB_CAP * bsize[off + b];
+      hshift = 0;
+      hfull = false;
       return true;
     }
     return false;
@@ -220,13 +228,38 @@ __global__ __launch_bounds__(64) void k_basin_walk(const InstParams* __restrict_
       int t = 0;
       const int nl = ne < WALK_TCAP ? ne : WALK_TCAP;
       while (t < nl && LB[64 * t] != B) t++;
-      if (t == nl) {
-        while (t < ne && TB[t] != B) t++;
+      int hslot = -1;  // free hash slot for B when it is new
+      if (t == nl && ne > WALK_TCAP) {
+        const unsigned hmask = (1u << hbits) - 1u;
+        unsigned s0 = ((unsigned)B * 2654435761u) >> hshift;
+        t = ne;
+        for (int pr = 0; pr <= (int)hmask; pr++, s0 = (s0 + 1u) & hmask) {
+          const int x = TH[s0];
+          if (x < 0) { hslot = (int)s0; break; }
+          if (TB[x] == B) { t = x; break; }
+        }
+        if (t == ne && hfull) {
+          t = WALK_TCAP;
+          while (t < ne && TB[t] != B) t++;
+        }
       }
       if (t == ne) {
         if (ne >= cap) { atomicOr(err + inst, FSX_ERR_NCONN); continue; }
-        if (t < WALK_TCAP) { LB[64 * t] = B; LW[64 * t] = w; }
-        else { TB[t] = B; TW[t] = w; }
+        if (t < WALK_TCAP) {
+          LB[64 * t] = B;
+          LW[64 * t] = w;
+        } else {
+          if (t == WALK_TCAP) {  // first entry past LDS: a hash over the slab, capacity 2^hbits <= cap
+            hbits = 31 - __clz(cap);
+            hshift = 32 - hbits;
+            for (int s = 0; s < (1 << hbits); s++) TH[s] = -1;
+            hslot = (int)(((unsigned)B * 2654435761u) >> hshift);
+          }
+          if (hslot >= 0) TH[hslot] = t;
+          else hfull = true;
+          TB[t] = B;
+          TW[t] = w;
+        }
         TP[t] = v; TQ[t] = qs[k];
         ne++;
       } else {
@@ -1039,8 +1072,8 @@ void launch_lm_basins(DevState& S, hipStream_t st) {
 
 void launch_lm_walk(DevState& S, hipStream_t st) {
   k_basin_walk<<<xcd_grid(S.ninst, WALK_GX), 64, 0, st>>>(S.P, S.nbasins, S.bc, S.outlet, S.rec, S.next, S.wnode,
-                                                           S.soff, S.bsize, S.ecount, S.sB, S.sp, S.sq, S.sw, S.err,
-                                                           S.ninst);
+                                                           S.soff, S.bsize, S.ecount, S.sB, S.sp, S.sq, S.sw, S.sh,
+                                                           S.err, S.ninst);
 }
 
 void launch_lm_edges(DevState& S, hipStream_t st) {

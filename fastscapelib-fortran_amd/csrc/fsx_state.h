@@ -46,6 +46,7 @@ struct DevState {
 
   // ---- candidate-pass slab (FSX_SLAB_CAP * node offsets) ----
   int *sB = nullptr, *sp = nullptr, *sq = nullptr;
+  int* sh = nullptr;  // basin walk: hash index over a basin's pass-table entries beyond those kept in LDS
   double *sw = nullptr;
 
   // ---- multiple flow direction (allocated when an instance first selects it) ----
