@@ -968,16 +968,21 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_tour_jump(const InstParams* __res
   if (!P.active || !P.run_spl) return;
   const int ns = nslot[inst];
   unsigned long long* T = tj + 2LL * P.off;
+  bool changed = false;
   for (int s = blockIdx.x * FSX_CHUNK + threadIdx.x; s < ns; s += gridDim.x * FSX_CHUNK) {
-    const unsigned long long w = T[s];
-    const int x = (int)(unsigned)(w & 0xffffffffu);
-    const unsigned long long wx = T[x];
-    const int xx = (int)(unsigned)(wx & 0xffffffffu);
-    if (xx != x) {
-      T[s] = tpack((unsigned)((w >> 32) + (wx >> 32)), (unsigned)xx);
-      flags[TOUR_FLAG0 + round] = 1;
+    unsigned long long w = T[s];
+#pragma unroll
+    for (int hop = 0; hop < 2; hop++) {  // two hops per round (any pair read is valid)
+      const int x = (int)(unsigned)(w & 0xffffffffu);
+      const unsigned long long wx = T[x];
+      const int xx = (int)(unsigned)(wx & 0xffffffffu);
+      if (xx == x) break;
+      w = tpack((unsigned)((w >> 32) + (wx >> 32)), (unsigned)xx);
+      T[s] = w;
+      changed = true;
     }
   }
+  if (changed) flags[TOUR_FLAG0 + round] = 1;
 }
 
 // make (cb0, cn0) the parent side of every tree edge

@@ -41,7 +41,7 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_jump_init(const InstParams* __res
 
 // J[v] = (distance to ancestor, ancestor).  One jump replaces the ancestor by its own ancestor;
 // a 64-bit load/store keeps the pair consistent, so jumping in place is safe.
-// One pointer-jumping round.  A round after the one that changed nothing returns at once; the grid
+// One pointer-jumping round (two hops).  A round after the one that changed nothing returns at once; the grid
 // strides over the chunks so that such a launch costs few workgroups.
 __global__ __launch_bounds__(FSX_CHUNK) void k_jump(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
                                                     int nchunks, unsigned long long* __restrict__ J,
@@ -54,11 +54,16 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_jump(const InstParams* __restrict
     const int v = c.start + (int)threadIdx.x;
     if (!P.active || !P.run_spl || v >= P.nn) continue;
     unsigned long long* Ji = J + P.off;
-    const unsigned long long w = Ji[v];
-    const int x = jnext(w);
-    const unsigned long long wx = Ji[x];
-    if (jnext(wx) != x) {
-      Ji[v] = jpack((unsigned)(jdist(w) + jdist(wx)), (unsigned)jnext(wx));
+    // two hops per round: any (distance, ancestor) pair read is valid, whoever wrote it when, so a
+    // thread may jump again from its own new pair
+    unsigned long long w = Ji[v];
+#pragma unroll
+    for (int hop = 0; hop < 2; hop++) {
+      const int x = jnext(w);
+      const unsigned long long wx = Ji[x];
+      if (jnext(wx) == x) break;
+      w = jpack((unsigned)(jdist(w) + jdist(wx)), (unsigned)jnext(wx));
+      Ji[v] = w;
       changed = true;
     }
   }
