@@ -45,13 +45,13 @@ ALG_BYTES_PER_NODE = {
     "uplift": 40,        # R h,u,b  W h,b
     "receivers": 20,     # R h  W rec(i32), length
     "donors": 13,        # R rec  W dmask(u8), first, next
-    "lm_walk": 24,       # R h, basin, first, next, rec (nodes of closed basins ~ all nodes)
-    "area_prep": 61,     # R J, dmask, pos, precip  W A0, donor slots(8 x i32)
-    "area_sweep": 60,    # R A0, donor slots, order  W Al, area
+    "lm_walk": 24,       # R (h, basin, first) record, next, rec (nodes of closed basins ~ all nodes)
+    "area_prep": 65,     # R J, dmask, pos, donor pos, precip  W record (a0, 8 donor slots)
+    "area_sweep": 48,    # R record  W Al
     "spl_prep": 48,      # R h, b, kf  W kfint, g, ht
-    "spl_fact": 72,      # R J, rec, pos(x2), area, length, kfint, ht  W rp, F, HTl
-    "spl_sweep": 56,     # R order, rp, F, HTl, Hl  W h, hwater, lake_depth
-    "spl_post": 64,      # R h, b, ht, etot  W b, etot, erate, sedflux
+    "spl_fact": 92,      # R J, rec, pos(x2), Al, length, kfint, ht(x2)  W area, lsed, record (node, rcv, f, ht)
+    "spl_sweep": 40,     # R record  W Hl, HWl (level order)
+    "spl_post": 100,     # R pos, Hl, HWl, ht, b, etot  W h, b, etot, erate, sedflux, hwater, lake_depth
 }
 ALG_BYTES_PER_EDGE = {"lm_mst": 13, "lm_orient": 33}  # per basin-graph edge
 

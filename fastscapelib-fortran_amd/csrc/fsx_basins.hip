@@ -131,7 +131,7 @@ __global__ __launch_bounds__(FSX_WG) void k_bsize_scan(const InstParams* __restr
 
 // ---- 2. passes (connect_basins) -------------------------------------------------------------------
 #define WALK_TCAP 16  // pass-table entries per thread kept in LDS; further entries live in the slab
-#define WALK_GX 64  // blocks per landscape; each thread walks basins b = x, x + WALK_GX*64, ...
+#define WALK_GX 32  // blocks per landscape; each thread walks basins b = x, x + WALK_GX*64, ...
 __global__ __launch_bounds__(64) void k_basin_walk(const InstParams* __restrict__ PP, const int* __restrict__ nbasins,
                                                    const uint8_t* __restrict__ bc, const int* __restrict__ outlet,
                                                    const int* __restrict__ rec, const int* __restrict__ next,

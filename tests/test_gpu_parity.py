@@ -93,6 +93,34 @@ def test_routing_matches_oracle(fs, oracle, ibc, shape):
     assert rel_err(area, a) <= A_TOL
 
 
+def _bowl(n, seed, depth, rad, noise):
+    """A closed bowl (Gaussian depression of the given depth and radius, in grid fractions) on a
+    plateau of uniform noise: the bowl is one large basin ringed by many small pits."""
+    rng = np.random.default_rng(seed)
+    x = np.arange(n) - (n - 1) / 2
+    r2 = (x[None, :] ** 2 + x[:, None] ** 2) / (rad * n) ** 2
+    return (100.0 - depth * np.exp(-r2) + noise * rng.random((n, n))).ravel()
+
+
+@pytest.mark.parametrize("n,depth,rad,noise,min_edges", [(65, 20.0, 0.3, 0.5, 100), (97, 30.0, 0.25, 1.0, 20)])
+def test_routing_basin_with_many_neighbours(fs, oracle, n, depth, rad, noise, min_edges):
+    """One closed basin borders dozens of others, so its pass table outgrows the walk's per-thread
+    table and is found through the hash index: receivers, lengths and donors stay bit-exact."""
+    L = landscape(n, n, seed=5, ibc=1111)
+    L["h"] = _bowl(n, 5, depth, rad, noise)
+    dx = L["xl"] / (n - 1)
+    _, cb, cn, _ = oracle.connect_basins(L["h"], n, n, dx, dx, 1111)
+    real = cn[:, 0] >= 0
+    assert np.bincount(cb[real, 0]).max() >= min_edges  # the case the test is for
+    rec, length, ndon, don, area, hw, order = _route_gpu(fs, L)
+    r_rec, r_len, r_ndon, r_don, _, _ = oracle.local_minima(L["h"], n, n, dx, dx, 1111)
+    assert np.array_equal(rec, r_rec), f"receivers differ at {np.flatnonzero(rec != r_rec)[:10]}"
+    assert np.array_equal(length, r_len)
+    assert np.array_equal(ndon, r_ndon) and np.array_equal(don, r_don)
+    got, ref = _run_both(fs, oracle, L, 5)
+    _check(got, ref, 5)
+
+
 def _run_both(fs, oracle, L, steps):
     o = oracle.COracle(L["nx"], L["ny"])
     apply(o, L)
