@@ -171,15 +171,18 @@ def _sync():
 
 
 GRAPHS = True  # replay captured HIP graphs (off: launch every kernel directly, for kernel-trace profilers)
+TUNE = True    # run the launch-shape autotuner after the warm-up steps of a batch
 
 
-def run_gpu(fs, Ls, steps, warmup, profile):
+def run_gpu(fs, Ls, steps, warmup, profile, tune=False):
     with fs.Batch([(L["nx"], L["ny"]) for L in Ls]) as b:
         b.graphs(GRAPHS)
         for k, L in enumerate(Ls):
             configure(b, L, k)
         if warmup:
             b.step(warmup)
+        if tune and TUNE:
+            b.autotune()
         if profile:
             b.profiling(True)
             b.reset_timers()
@@ -205,6 +208,7 @@ def run_headline(fs, Ls, steps, warmup, prof_steps):
             configure(b, L, k)
         if warmup:
             b.step(warmup)
+        tuned = b.autotune() if TUNE else b.tuning()
         b.profiling(True)
         b.reset_timers()
         b.step(prof_steps)
@@ -224,7 +228,7 @@ def run_headline(fs, Ls, steps, warmup, prof_steps):
         b.time_phase(None)
         stats = [b.stats(k) for k in range(len(Ls))]
         mem = b.memory()
-    return dt, prof, live, dom, stats, mem
+    return dt, prof, live, dom, stats, mem, tuned
 
 
 def plain_rate(fs, Ls, steps):
@@ -240,7 +244,7 @@ def plain_rate(fs, Ls, steps):
 def ragged_leg(fs, steps, sample):
     Ls = ragged_batch()
     ncells = sum(L["nx"] * L["ny"] for L in Ls)
-    dt, _, stats, mem = run_gpu(fs, Ls, steps, 1, profile=False)
+    dt, _, stats, mem = run_gpu(fs, Ls, steps, 1, profile=False, tune=True)
     batched = ncells * steps / dt
     pick = Ls[::max(1, len(Ls) // sample)][:sample]
     plain = plain_rate(fs, pick, max(2, steps // 2))
@@ -397,9 +401,11 @@ def main():
     ap.add_argument("--quantum", type=int, default=8)
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every kernel directly instead of replaying HIP graphs (for kernel-trace profiling)")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (no autotuner)")
     args = ap.parse_args()
-    global GRAPHS
+    global GRAPHS, TUNE
     GRAPHS = not args.no_graphs
+    TUNE = not args.no_tune
     legs = set(args.legs.split(","))
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (landscapes shard "
@@ -414,7 +420,7 @@ def main():
     Ls = uniform_batch(args.ninst, args.n)
     workload = f"uniform batch {args.ninst} x {args.n}x{args.n} D8 n=1 g=0 kd=1e-2 fixed BC"
     ncells = sum(L["nx"] * L["ny"] for L in Ls)
-    dt, timers, live, dom, stats, mem = run_headline(fs, Ls, args.steps, args.warmup, args.prof_steps)
+    dt, timers, live, dom, stats, mem, tuned = run_headline(fs, Ls, args.steps, args.warmup, args.prof_steps)
     value = ncells * args.steps / dt
 
     # plain path: one landscape at a time (the first of the batch)
@@ -434,6 +440,7 @@ def main():
                    "workload": f"1 x {args.n}x{args.n}, same parameters"} if plain else None),
         "batched_over_plain": value / plain if plain else None,
         "roofline": dict(roofline(dom, live, stats, ncells), traffic=measured_traffic(dom, workload)),
+        "launch_shapes": dict(tuned, chosen_by="autotuner" if TUNE else "defaults"),
         "phase_ms_per_step": phases,
         "phase_ms_per_step_note": f"profiled window of {args.prof_steps} steps, kernels launched one by one",
         "step_model_frac": round(336.0 * ncells / (dt / args.steps) / 1e9 / HBM_PEAK_GBS, 5),

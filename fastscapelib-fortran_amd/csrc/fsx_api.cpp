@@ -942,6 +942,72 @@ extern "C" int fsx_reset_timers(fsx_ctx* c) {
   return 0;
 }
 
+// The candidates of each launch shape, the phase(s) that time it, and the shape's field.
+struct TuneKnob {
+  int DevState::*field;
+  int cand[4];
+  int ncand;
+  int ph0, ph1;  // timed phases (ph1 = -1: one)
+};
+
+extern "C" int fsx_autotune(fsx_ctx* c, int reps, int* chosen) {
+  if (!c) return fail(FSX_EINVAL, "null context");
+  if (reps < 1) return fail(FSX_EINVAL, "fsx_autotune: reps must be >= 1");
+  if (int r = upload_params(c)) return r;
+  static const TuneKnob knobs[] = {
+      {&DevState::walk_gx, {32, 64, 128, 256}, 4, PH_LM_WALK, -1},
+      {&DevState::jump_blocks, {1024, 2048, 4096, 8192}, 4, PH_LM_LABELS, PH_LEVELS},
+      {&DevState::tour_gx, {8, 16, 32, 0}, 3, PH_LM_ORIENT, -1},
+      {&DevState::nr_small_gx, {16, 32, 64, 128}, 4, PH_LM_SORT, -1},
+  };
+  // keep the caller's profiling state and accumulated timers
+  if (int r = flush_timers_any(c, c->stream)) return r;
+  const bool prof0 = c->prof;
+  double ms0[PH_COUNT];
+  long long n0[PH_COUNT];
+  for (int p = 0; p < PH_COUNT; p++) { ms0[p] = c->acc_ms[p]; n0[p] = c->acc_n[p]; }
+  int rc = fsx_set_profiling(c, 1);
+  for (const TuneKnob& k : knobs) {
+    if (rc) break;
+    int best = c->S.*k.field;
+    double best_ms = 1e300;
+    for (int i = 0; i < k.ncand && !rc; i++) {
+      c->S.*k.field = k.cand[i];
+      enqueue_routing(c, c->stream);  // untimed pass: caches and code warm for this shape
+      ++c->ring_used;
+      if ((rc = fsx_get_timers(c, nullptr, nullptr, 0)) < 0) break;
+      rc = 0;
+      fsx_reset_timers(c);
+      for (int r = 0; r < reps && !rc; r++) {
+        enqueue_routing(c, c->stream);
+        if (++c->ring_used == kRing) rc = flush_timers_any(c, c->stream);  // one ring slot per pass
+      }
+      if ((rc = fsx_get_timers(c, nullptr, nullptr, 0)) < 0) break;
+      rc = 0;
+      const double t = c->acc_ms[k.ph0] + (k.ph1 >= 0 ? c->acc_ms[k.ph1] : 0.0);
+      if (t < best_ms) { best_ms = t; best = k.cand[i]; }
+    }
+    c->S.*k.field = best;
+  }
+  if (!rc) rc = check_errors(c);
+  const int rp = fsx_set_profiling(c, prof0 ? 1 : 0);
+  for (int p = 0; p < PH_COUNT; p++) { c->acc_ms[p] = ms0[p]; c->acc_n[p] = n0[p]; }
+  c->graph_valid = false;  // captured launches carry the old shapes
+  if (rc) return rc;
+  if (rp) return rp;
+  if (chosen) fsx_get_tuning(c, chosen);
+  return 0;
+}
+
+extern "C" int fsx_get_tuning(fsx_ctx* c, int* out4) {
+  if (!c || !out4) return fail(FSX_EINVAL, "fsx_get_tuning: null argument");
+  out4[0] = c->S.walk_gx;
+  out4[1] = c->S.jump_blocks;
+  out4[2] = c->S.tour_gx;
+  out4[3] = c->S.nr_small_gx;
+  return 0;
+}
+
 extern "C" int fsx_memory(fsx_ctx* c, long long* bytes, long long* per_node) {
   if (!c) return fail(FSX_EINVAL, "null context");
   if (bytes) *bytes = c->bytes;

@@ -131,7 +131,7 @@ __global__ __launch_bounds__(FSX_WG) void k_bsize_scan(const InstParams* __restr
 
 // ---- 2. passes (connect_basins) -------------------------------------------------------------------
 #define WALK_TCAP 16  // pass-table entries per thread kept in LDS; further entries live in the slab
-#define WALK_GX 32  // blocks per landscape; each thread walks basins b = x, x + WALK_GX*64, ...
+// launched with gx blocks per landscape (DevState::walk_gx); thread x walks basins x, x + gx*64, ...
 __global__ __launch_bounds__(64) void k_basin_walk(const InstParams* __restrict__ PP, const int* __restrict__ nbasins,
                                                    const uint8_t* __restrict__ bc, const int* __restrict__ outlet,
                                                    const int* __restrict__ rec, const int* __restrict__ next,
@@ -139,7 +139,7 @@ __global__ __launch_bounds__(64) void k_basin_walk(const InstParams* __restrict_
                                                    const int* __restrict__ soff, const int* __restrict__ bsize,
                                                    int* __restrict__ ecount, int* __restrict__ sB, int* __restrict__ sp,
                                                    int* __restrict__ sq, double* __restrict__ sw, int* __restrict__ sh,
-                                                   int* __restrict__ err, int ninst) {
+                                                   int* __restrict__ err, int ninst, int gx) {
   __shared__ int lB[WALK_TCAP * 64];
   __shared__ double lW[WALK_TCAP * 64];
   int* LB = lB + threadIdx.x;
@@ -147,7 +147,7 @@ __global__ __launch_bounds__(64) void k_basin_walk(const InstParams* __restrict_
   constexpr int II[8] = {-1, 1, 0, 0, -1, 1, -1, 1};
   constexpr int JJ[8] = {0, 0, -1, 1, -1, -1, 1, 1};
   int inst, sub;
-  if (!xcd_split(ninst, WALK_GX, inst, sub)) return;
+  if (!xcd_split(ninst, gx, inst, sub)) return;
   const InstParams& P = PP[inst];
   if (!P.active || !P.run_spl) return;
   const int nb = nbasins[inst];
@@ -157,7 +157,7 @@ __global__ __launch_bounds__(64) void k_basin_walk(const InstParams* __restrict_
   const ulonglong2* WN = reinterpret_cast<const ulonglong2*>(wnode) + off;
   // One flat loop of visits: a lane that finishes a basin starts its next one at once, so the lanes
   // of a wave do not wait for the largest basin of every round (basins differ widely in size).
-  constexpr int STRIDE = WALK_GX * 64;
+  const int STRIDE = gx * 64;
   int b = sub * 64 + threadIdx.x - STRIDE;
   int r = 0, v = 0, ne = 0, cap = 0, visits = 0;
   int *TB = nullptr, *TP = nullptr, *TQ = nullptr, *TH = nullptr;
@@ -866,7 +866,6 @@ int basins_set_lds(int smem_max) {
 // directed edge to the tour's end, by pointer jumping) the direction with the larger distance is the
 // downward one.  The result is the unique rooted tree; the adjacency order (filled with atomics) only
 // changes the tour, not the orientation.  Directed edges ("slots") are grouped by source basin.
-#define TOUR_GX 16
 #define TOUR_FLAG0 32  // flags[TOUR_FLAG0 + round]: the jumping round changed something
 #define TOUR_ROUNDS 30
 
@@ -1056,7 +1055,7 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_correct(const InstParams* __restr
 void launch_nr_sort(DevState& S, hipStream_t st) {
   launch_fill_int(S.segctr, S.ninst, 0, st);
   k_nr_big<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.nconn, S.cw, S.skey, S.sid, S.Lasc, S.Rasc, S.segs, S.nsegs, S.err);
-  k_nr_small<<<dim3(16, S.ninst), 256, 0, st>>>(S.P, S.nsegs, S.segctr, S.skey, S.sid, S.segs, S.err);
+  k_nr_small<<<dim3(S.nr_small_gx, S.ninst), 256, 0, st>>>(S.P, S.nsegs, S.segctr, S.skey, S.sid, S.segs, S.err);
   k_nr_pos<<<dim3(16, S.ninst), FSX_CHUNK, 0, st>>>(S.P, S.nconn, S.sid, S.npos);
 }
 
@@ -1076,9 +1075,9 @@ void launch_lm_basins(DevState& S, hipStream_t st) {
 }
 
 void launch_lm_walk(DevState& S, hipStream_t st) {
-  k_basin_walk<<<xcd_grid(S.ninst, WALK_GX), 64, 0, st>>>(S.P, S.nbasins, S.bc, S.outlet, S.rec, S.next, S.wnode,
+  k_basin_walk<<<xcd_grid(S.ninst, S.walk_gx), 64, 0, st>>>(S.P, S.nbasins, S.bc, S.outlet, S.rec, S.next, S.wnode,
                                                            S.soff, S.bsize, S.ecount, S.sB, S.sp, S.sq, S.sw, S.sh,
-                                                           S.err, S.ninst);
+                                                           S.err, S.ninst, S.walk_gx);
 }
 
 void launch_lm_edges(DevState& S, hipStream_t st) {
@@ -1095,7 +1094,7 @@ void launch_lm_mst(DevState& S, hipStream_t st) {
 }
 
 void launch_lm_orient(DevState& S, hipStream_t st) {
-  const dim3 g(TOUR_GX, S.ninst), ge(4 * TOUR_GX, S.ninst);
+  const dim3 g(S.tour_gx, S.ninst), ge(4 * S.tour_gx, S.ninst);
   launch_fill_int(S.tdeg, S.ntot, 0, st);
   launch_fill_int(S.flags + TOUR_FLAG0, 64 - TOUR_FLAG0, 0, st);
   k_tour_deg<<<g, FSX_CHUNK, 0, st>>>(S.P, S.nconn, S.cb0, S.cb1, S.intree, S.tdeg);

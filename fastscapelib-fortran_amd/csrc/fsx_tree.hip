@@ -852,12 +852,11 @@ int sweep_set_lds(int smem_max) {
   return 0;
 }
 
-static const int kJumpBlocks = 8192;  // 8 workgroups of 256 per CU
 
 void launch_jump(DevState& S, hipStream_t st) {
   k_jump_init<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.rec, S.J);
   launch_fill_int(S.flags, 64, 0, st);
-  const int nb = std::min(S.nchunks, kJumpBlocks);
+  const int nb = std::min(S.nchunks, S.jump_blocks);
   for (int it = 0; it < 26; it++) k_jump<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.nchunks, S.J, S.flags, it);
 }
 

@@ -142,6 +142,20 @@ class Batch:
         k = -1 if name is None else _lib.PHASES.index(name)
         check(lib.fsx_time_phase(self._h, k), "time_phase")
 
+    TUNING = ("walk_gx", "jump_blocks", "tour_gx", "nr_small_gx")
+
+    def autotune(self, reps=2):
+        """Choose the launch shapes of the routing kernels by timing candidates on the current
+        landscapes (which it leaves unchanged); returns the chosen shapes."""
+        out = (C.c_int * 4)()
+        check(lib.fsx_autotune(self._h, int(reps), out), "autotune")
+        return dict(zip(self.TUNING, list(out)))
+
+    def tuning(self):
+        out = (C.c_int * 4)()
+        check(lib.fsx_get_tuning(self._h, out), "get_tuning")
+        return dict(zip(self.TUNING, list(out)))
+
     def graphs(self, on=True):
         """Replay each step as one captured HIP graph (default) or launch its kernels one by one."""
         check(lib.fsx_set_graphs(self._h, 1 if on else 0), "set_graphs")

@@ -73,6 +73,8 @@ def _load():
         "fsx_device_memory": [C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)],
         "fsx_set_graphs": [_vp, C.c_int],
         "fsx_time_phase": [_vp, C.c_int],
+        "fsx_autotune": [_vp, C.c_int, _ip],
+        "fsx_get_tuning": [_vp, _ip],
     }
     for name, args in sig.items():
         f = getattr(lib, name)

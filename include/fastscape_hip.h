@@ -166,6 +166,16 @@ int fsx_time_phase(fsx_ctx *ctx, int phase);
 int fsx_get_timers(fsx_ctx *ctx, double *ms, long long *counts, int n);
 int fsx_reset_timers(fsx_ctx *ctx);
 
+/* Launch-shape autotuner.  Times the routing phases (on the current landscapes, which it does not
+ * change) under candidate launch shapes -- basin-walk blocks per landscape, the pointer-jumping grid
+ * cap, Euler-tour blocks per landscape, edge-sort blocks per landscape -- with reps timed passes per
+ * candidate, and keeps the fastest of each.  No result depends on these shapes.  chosen (4 ints, or
+ * NULL) receives walk_gx, jump_blocks, tour_gx, nr_small_gx.  No reference counterpart: the
+ * reference has no launch shapes. */
+int fsx_autotune(fsx_ctx *ctx, int reps, int *chosen);
+/* The launch shapes in use (same order as fsx_autotune's chosen). */
+int fsx_get_tuning(fsx_ctx *ctx, int *out4);
+
 /* Device-resident memory of a context (bytes) and the bytes per node the layout costs. */
 int fsx_memory(fsx_ctx *ctx, long long *bytes, long long *bytes_per_node);
 

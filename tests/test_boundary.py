@@ -56,6 +56,8 @@ def test_host_side_argument_checks(lib):
     assert L.fsx_indexx(-1, None, None) == -1
     assert L.fsx_indexx(0, None, None) == 0
     assert L.fsx_destroy(None) == 0
+    assert L.fsx_autotune(None, 1, None) == -1
+    assert L.fsx_get_tuning(None, None) == -1
     assert L.fsx_version().startswith(b"fastscape")
 
 

@@ -1,0 +1,40 @@
+"""The launch-shape autotuner picks shapes from its candidate sets, leaves the landscapes as they were,
+and steps taken with the tuned shapes (graph-replayed) equal steps with the default ones bitwise."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import apply, landscape
+
+pytestmark = pytest.mark.gpu
+
+CANDIDATES = {"walk_gx": {32, 64, 128, 256}, "jump_blocks": {1024, 2048, 4096, 8192}, "tour_gx": {8, 16, 32},
+              "nr_small_gx": {16, 32, 64, 128}}
+
+
+def test_autotune_changes_shapes_not_results():
+    import fastscapelib_fortran as fs
+    Ls = [landscape(97, 81, seed=31, sinus=(5.0, 8000.0)), landscape(64, 90, seed=32, ibc=1010, n=1.5, m=0.675),
+          landscape(50, 50, seed=33, p=1.0)]
+    print("autotune: 3 landscapes, 4 steps", flush=True)
+    with fs.Batch([(L["nx"], L["ny"]) for L in Ls]) as a, fs.Batch([(L["nx"], L["ny"]) for L in Ls]) as b:
+        for k, L in enumerate(Ls):
+            apply(a, L, k)
+            apply(b, L, k)
+        a.step(2)
+        b.step(2)
+        h_before = [b.h(k) for k in range(len(Ls))]
+        chosen = b.autotune(reps=1)
+        assert set(chosen) == set(CANDIDATES)
+        for name, v in chosen.items():
+            assert v in CANDIDATES[name], (name, v)
+        assert b.tuning() == chosen
+        for k in range(len(Ls)):
+            assert np.array_equal(b.h(k), h_before[k])  # tuning runs routing only
+        a.step(2)
+        b.step(2)
+        for k in range(len(Ls)):
+            assert np.array_equal(a.h(k), b.h(k)), k
+            assert np.array_equal(a.rec(k), b.rec(k)), k
+            assert np.array_equal(a.field(k, 6), b.field(k, 6)), k
