@@ -955,7 +955,7 @@ extern "C" int fsx_autotune(fsx_ctx* c, int reps, int* chosen) {
   if (reps < 1) return fail(FSX_EINVAL, "fsx_autotune: reps must be >= 1");
   if (int r = upload_params(c)) return r;
   static const TuneKnob knobs[] = {
-      {&DevState::walk_gx, {32, 64, 128, 256}, 4, PH_LM_WALK, -1},
+      {&DevState::walk_gx, {64, 128, 256, 512}, 4, PH_LM_WALK, -1},
       {&DevState::jump_blocks, {1024, 2048, 4096, 8192}, 4, PH_LM_LABELS, PH_LEVELS},
       {&DevState::tour_gx, {8, 16, 32, 0}, 3, PH_LM_ORIENT, -1},
       {&DevState::nr_small_gx, {16, 32, 64, 128}, 4, PH_LM_SORT, -1},

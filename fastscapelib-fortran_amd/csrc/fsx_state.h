@@ -11,7 +11,7 @@ struct DevState {
   InstParams* P = nullptr;  // [ninst]
   Chunk* chunks = nullptr;  // [nchunks]
   // launch shapes, chosen by fsx_autotune (no result depends on them)
-  int walk_gx = 128;       // basin walk: blocks per landscape
+  int walk_gx = 256;       // basin walk: blocks per landscape
   int jump_blocks = 2048;  // pointer jumping: grid cap
   int tour_gx = 16;        // Euler tour kernels: blocks per landscape (x4 for the ranking rounds)
   int nr_small_gx = 64;    // edge sort: blocks per landscape for the segments in LDS

@@ -9,7 +9,7 @@ from conftest import apply, landscape
 
 pytestmark = pytest.mark.gpu
 
-CANDIDATES = {"walk_gx": {32, 64, 128, 256}, "jump_blocks": {1024, 2048, 4096, 8192}, "tour_gx": {8, 16, 32},
+CANDIDATES = {"walk_gx": {64, 128, 256, 512}, "jump_blocks": {1024, 2048, 4096, 8192}, "tour_gx": {8, 16, 32},
               "nr_small_gx": {16, 32, 64, 128}}
 
 
