@@ -155,59 +155,51 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_spl_post(const InstParams* __rest
 // x-implicit system with the Thomas algorithm (tridag, :164-202) and one thread per column the
 // y-implicit one.  Coefficients follow the reference's left-to-right order, e.g.
 // ((kd(i+1,j)+kd(i,j))/2)*(dt/2)/dx**2 and ((z + fyp*zp) - (fyp+fym)*z) + fym*zm.
-__global__ __launch_bounds__(FSX_CHUNK) void k_adi_prep(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
-                                                        const double* __restrict__ h, const double* __restrict__ b,
-                                                        const double* __restrict__ kd, double* __restrict__ kdint) {
-  NODE_KERNEL_PROLOGUE
-  if (!P.run_diff) return;
-  kdint[g_] = (P.kdsed > 0.0 && (h[g_] - b[g_]) > 1.0e-6) ? P.kdsed : kd[g_];
-}
-
 __device__ __forceinline__ double coef(double ka, double kb, double hdt, double d2) {
   return (ka + kb) / 2.0 * hdt / d2;
 }
 
-// The coefficients ((kd_a+kd_b)/2)*(dt/2)/d**2 depend only on the pair of nodes, and a+b == b+a
-// exactly, so each is computed once per node pair: cx(i,j) between (i,j) and (i+1,j), cy(i,j)
-// between (i,j) and (i,j+1) (factxm(i,j) == cx(i-1,j), factym(i,j) == cy(i,j-1)).  The diagonal and
-// right-hand side of each line system are then node-parallel; only the Thomas recurrence (tridag,
-// :164-202) runs one thread per line, reading its inputs in prefetched chunks.
-__global__ __launch_bounds__(FSX_CHUNK) void k_adi_coef(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
-                                                        const double* __restrict__ kdint, double* __restrict__ cx,
-                                                        double* __restrict__ cy) {
-  NODE_KERNEL_PROLOGUE
-  if (!P.run_diff) return;
-  const int i = v % P.nx, j = v / P.nx;
-  const double hdt = P.dt / 2.0;
-  const double* K = kdint + P.off;
-  cx[g_] = i < P.nx - 1 ? coef(K[v + 1], K[v], hdt, P.dx * P.dx) : 0.0;
-  cy[g_] = j < P.ny - 1 ? coef(K[v + P.nx], K[v], hdt, P.dy * P.dy) : 0.0;
+// kdint (:26-33), the couplings cx, cy and the x pass's diagonal and right-hand side (rows
+// j = 1..ny-2, :45-82; zintp == h at this point) in one pass: a thread recomputes the kdint of the
+// four nodes around it, which is the expression their own threads evaluate, so every coupling is
+// bit-identical to the one stored by its owner.
+__device__ __forceinline__ double kdint_of(const InstParams& P, const double* h, const double* b, const double* kd,
+                                          long long g) {
+  return (P.kdsed > 0.0 && (h[g] - b[g]) > 1.0e-6) ? P.kdsed : kd[g];
 }
 
-// x pass, rows j = 1..ny-2 (:45-82): diagonal and right-hand side; zintp == h at this point
-__global__ __launch_bounds__(FSX_CHUNK) void k_adi_xrhs(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
-                                                        const double* __restrict__ h, const double* __restrict__ cx,
-                                                        const double* __restrict__ cy, double* __restrict__ dg,
-                                                        double* __restrict__ rf) {
+__global__ __launch_bounds__(FSX_CHUNK) void k_adi_xprep(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
+                                                         const double* __restrict__ h, const double* __restrict__ b,
+                                                         const double* __restrict__ kd, double* __restrict__ kdint,
+                                                         double* __restrict__ cx, double* __restrict__ cy,
+                                                         double* __restrict__ dg, double* __restrict__ rf) {
   NODE_KERNEL_PROLOGUE
   if (!P.run_diff) return;
-  const int nx = P.nx, i = v % nx, j = v / nx;
-  if (j == 0 || j == P.ny - 1) return;
+  const int nx = P.nx, ny = P.ny, i = v % nx, j = v / nx;
+  const double hdt = P.dt / 2.0, dx2 = P.dx * P.dx, dy2 = P.dy * P.dy;
+  const double kv = kdint_of(P, h, b, kd, g_);
+  kdint[g_] = kv;
+  const double cxv = i < nx - 1 ? coef(kdint_of(P, h, b, kd, g_ + 1), kv, hdt, dx2) : 0.0;
+  const double cyv = j < ny - 1 ? coef(kdint_of(P, h, b, kd, g_ + nx), kv, hdt, dy2) : 0.0;
+  cx[g_] = cxv;
+  cy[g_] = cyv;
+  if (j == 0 || j == ny - 1) return;
   const double* Z = h + P.off;
-  const double* CX = cx + P.off;
-  const double* CY = cy + P.off;
-  const double fyp = CY[v], fym = CY[v - nx];
+  const double fyp = cyv, fym = coef(kv, kdint_of(P, h, b, kd, g_ - nx), hdt, dy2);  // cy(i, j-1)
   double diag, f;
   const double frhs = Z[v] + fyp * Z[v + nx] - (fyp + fym) * Z[v] + fym * Z[v - nx];
   if (i == 0) {
     if (P.fix_left) { diag = 1.0; f = Z[v]; }
-    else { diag = 1.0 + CX[v]; f = frhs; }
-  } else if (i == nx - 1) {
-    if (P.fix_right) { diag = 1.0; f = Z[v]; }
-    else { diag = 1.0 + CX[v - 1]; f = frhs; }
+    else { diag = 1.0 + cxv; f = frhs; }
   } else {
-    diag = 1.0 + CX[v] + CX[v - 1];
-    f = frhs;
+    const double cxm = coef(kv, kdint_of(P, h, b, kd, g_ - 1), hdt, dx2);  // cx(i-1, j)
+    if (i == nx - 1) {
+      if (P.fix_right) { diag = 1.0; f = Z[v]; }
+      else { diag = 1.0 + cxm; f = frhs; }
+    } else {
+      diag = 1.0 + cxv + cxm;
+      f = frhs;
+    }
   }
   dg[g_] = diag;
   rf[g_] = f;
@@ -491,9 +483,7 @@ void launch_spl_post(DevState& S, hipStream_t st) {
                                                S.lake_depth, S.mpos);
 }
 void launch_diffusion(DevState& S, hipStream_t st) {
-  k_adi_prep<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.h, S.b, S.kd, S.kdint);
-  k_adi_coef<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.kdint, S.adx, S.ady);
-  k_adi_xrhs<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.h, S.adx, S.ady, S.adg, S.adf);
+  k_adi_xprep<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.h, S.b, S.kd, S.kdint, S.adx, S.ady, S.adg, S.adf);
   dim3 gx((S.max_ny + 63) / 64, S.ninst), gy((S.max_nx + 63) / 64, S.ninst);
   k_adi_x<<<gx, 64, 0, st>>>(S.P, S.adg, S.adf, S.adx, S.zint, S.gam, S.zintp, S.err);  // zintp: scratch until the y pass
   k_adi_yrhs<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.h, S.zint, S.kdint, S.adx, S.ady, S.adg, S.adf);

@@ -371,7 +371,7 @@ __global__ __launch_bounds__(FSX_WG) void k_area_sweep(const InstParams* __restr
   // level values double-buffered at sm[0..vcap) and sm[vcap..2*vcap); offsets, not a pointer array,
   // so every access stays an LDS (ds_*) instruction
   double* sA = sm + 2 * vcap;
-  int* sD = reinterpret_cast<int*>(sA + GRP_STG);
+  int* sD = reinterpret_cast<int*>(sA + GRP_STG);  // donor slot k of staged position t at k*GRP_STG + t
   int* sO = sD + 8 * GRP_STG;
   int* sL = sO + GRP_STG;  // level starts of the group, GRP_BUCKET+1
   const int inst = blockIdx.x;
@@ -401,13 +401,13 @@ __global__ __launch_bounds__(FSX_WG) void k_area_sweep(const InstParams* __restr
       if (tid < np) {
         sA[tid] = r0.a;
 #pragma unroll
-        for (int k = 0; k < 8; k++) sD[8 * tid + k] = r0.d[k];
+        for (int k = 0; k < 8; k++) sD[k * GRP_STG + tid] = r0.d[k];
       }
       if (tid + FSX_WG < np) {
         const int t = tid + FSX_WG;
         sA[t] = r1.a;
 #pragma unroll
-        for (int k = 0; k < 8; k++) sD[8 * t + k] = r1.d[k];
+        for (int k = 0; k < 8; k++) sD[k * GRP_STG + t] = r1.d[k];
       }
       if (tid < cur.hi - cur.lo) sL[tid] = rl;
       if (tid == 0) sL[cur.hi - cur.lo] = cur.p1;
@@ -418,7 +418,7 @@ __global__ __launch_bounds__(FSX_WG) void k_area_sweep(const InstParams* __restr
     auto stg = [&](AreaRec& r, int si) {
       r.a = sA[si];
 #pragma unroll
-      for (int k = 0; k < 8; k++) r.d[k] = sD[8 * si + k];
+      for (int k = 0; k < 8; k++) r.d[k] = sD[k * GRP_STG + si];
     };
     // one level: w positions from b0, record x0 of position tid when staged (position tid + FSX_WG,
     // rare, is read from the staging area); the level above (donors) has wn positions from b0 + w
