@@ -94,16 +94,26 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_basin_label(const InstParams* __r
                                                            int* __restrict__ basin, int* __restrict__ bsize,
                                                            WalkNode* __restrict__ wnode) {
   NODE_PROLOGUE
-  if (!live) return;
-  const int r = (int)(unsigned)(J[P.off + v] & 0xffffffffu);
-  const int b = rootrank[P.off + r];
-  const bool closed = !bc[P.off + r];
-  const int bv = closed ? b : -(b + 1);  // signed: open basins negative
-  basin[P.off + v] = bv;
-  reinterpret_cast<ulonglong2*>(wnode)[P.off + v] =
-      make_ulonglong2((unsigned long long)__double_as_longlong(h[P.off + v]),
-                      (unsigned long long)(unsigned)bv | ((unsigned long long)(unsigned)first[P.off + v] << 32));
-  if (closed) atomicAdd(bsize + P.off + b, 1);
+  int bv = INT_MIN;  // lanes past the landscape take part in the run detection below as a non-basin
+  if (live) {
+    const int r = (int)(unsigned)(J[P.off + v] & 0xffffffffu);
+    const int b = rootrank[P.off + r];
+    bv = !bc[P.off + r] ? b : -(b + 1);  // signed: open basins negative
+    basin[P.off + v] = bv;
+    reinterpret_cast<ulonglong2*>(wnode)[P.off + v] =
+        make_ulonglong2((unsigned long long)__double_as_longlong(h[P.off + v]),
+                        (unsigned long long)(unsigned)bv | ((unsigned long long)(unsigned)first[P.off + v] << 32));
+  }
+  // basin sizes: consecutive nodes of a row mostly share a basin, so each run of equal ids within the
+  // wave adds its length with one atomic
+  const int lane = threadIdx.x & 63;
+  const int prev = __shfl_up(bv, 1, 64);
+  const unsigned long long starts = __ballot(lane == 0 || prev != bv);
+  if (live && bv >= 0 && ((starts >> lane) & 1ull)) {
+    const unsigned long long after = lane == 63 ? 0ull : starts >> (lane + 1);
+    const int len = after ? __ffsll((long long)after) : 64 - lane;
+    atomicAdd(bsize + P.off + bv, len);
+  }
 }
 
 // exclusive scan of closed-basin sizes -> candidate slab offsets; clears edge counts
