@@ -79,12 +79,19 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_root_rank(const InstParams* __res
   const int f = (live && rec[P.off + v] == v) ? 1 : 0;
   int tot;
   const int incl = block_incl_scan<FSX_CHUNK>(f, lds, tot);
+  int open_rank = INT_MAX;  // basin0: the smallest rank of an open basin, one atomic per wave
   if (f) {
     const int r = chunkcnt[chunk_id()] + incl - 1;
     rootrank[P.off + v] = r;
     outlet[P.off + r] = v;
-    if (bc[P.off + v]) atomicMin(basin0 + c.inst, r);
+    if (bc[P.off + v]) open_rank = r;
   }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const int y = __shfl_xor(open_rank, o, 64);
+    open_rank = y < open_rank ? y : open_rank;
+  }
+  if ((threadIdx.x & 63) == 0 && open_rank != INT_MAX) atomicMin(basin0 + c.inst, open_rank);
 }
 
 __global__ __launch_bounds__(FSX_CHUNK) void k_basin_label(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
