@@ -617,7 +617,7 @@ static int check_errors(fsx_ctx* c) {
 static void enqueue_routing(fsx_ctx* c, hipStream_t st) {
   DevState& S = c->S;
   PHASE(PH_RECEIVERS, launch_receivers(S, st));
-  PHASE(PH_DONORS, launch_donors(S, st));
+  PHASE(PH_DONORS, launch_donors(S, st, 1));
   // LocalMinima, src/LocalMinima.f90:1-132
   PHASE(PH_LM_LABELS, launch_lm_labels(S, st));
   PHASE(PH_LM_BASINS, launch_lm_basins(S, st));
@@ -627,7 +627,7 @@ static void enqueue_routing(fsx_ctx* c, hipStream_t st) {
   PHASE(PH_LM_MST, launch_lm_mst(S, st));
   PHASE(PH_LM_ORIENT, launch_lm_orient(S, st));
   PHASE(PH_LM_CORRECT, launch_lm_correct(S, st));
-  PHASE(PH_DONORS, launch_donors(S, st));
+  PHASE(PH_DONORS, launch_donors(S, st, 0));
   PHASE(PH_LEVELS, launch_levels(S, st));
   if (c->any_sfd) {
     PHASE(PH_AREA_PREP, launch_area_prep(S, st));

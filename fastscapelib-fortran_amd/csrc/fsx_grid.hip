@@ -64,7 +64,7 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_receivers(const InstParams* __res
 // its own node: next[v] is gathered from the receivers around rec(v).
 __global__ __launch_bounds__(FSX_CHUNK) void k_donors(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
                                                       const int* __restrict__ rec, uint8_t* __restrict__ dmask,
-                                                      int* __restrict__ first, int* __restrict__ next) {
+                                                      int* __restrict__ first, int* __restrict__ next, int thread) {
   NODE_KERNEL_PROLOGUE
   if (!P.run_spl) return;
   const int* ri = rec + P.off;
@@ -79,6 +79,7 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_donors(const InstParams* __restri
       f = q[t] < f ? q[t] : f;
     }
   dmask[g_] = (uint8_t)m;
+  if (!thread) return;  // the mask alone (the walk's threading is only needed before LocalMinima)
   first[g_] = m ? f : -1;
   const int r = ri[v];
   int nxt = INT_MAX;
@@ -471,8 +472,8 @@ void launch_uplift(DevState& S, hipStream_t st) {
 void launch_receivers(DevState& S, hipStream_t st) {
   k_receivers<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.h, S.rec, S.length);
 }
-void launch_donors(DevState& S, hipStream_t st) {
-  k_donors<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.rec, S.dmask, S.first, S.next);
+void launch_donors(DevState& S, hipStream_t st, int thread) {
+  k_donors<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.rec, S.dmask, S.first, S.next, thread);
 }
 void launch_spl_prep(DevState& S, hipStream_t st) {
   k_spl_prep<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.h, S.b, S.kf, S.kfint, S.g, S.ht);
