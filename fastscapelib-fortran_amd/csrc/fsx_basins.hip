@@ -560,14 +560,15 @@ __global__ __launch_bounds__(FSX_WG) void k_nr_big(const InstParams* __restrict_
   if (threadIdx.x == 0) nsegs[inst] = s_ns;
 }
 
-// Segments of <= NR_BIG keys, one 256-thread block per segment, entirely in LDS.  Sub-ranges are
+// Segments of <= NR_BIG keys, one NR_SMALL_WG-thread block per segment, entirely in LDS.  Sub-ranges are
 // disjoint, so their order of processing does not matter: each round either partitions one range of
-// more than NR_MID keys with the whole block, or up to four ranges of NR_TINY+1..NR_MID keys with one
-// wave each; ranges of at most NR_TINY keys are finished at the end, one per thread, by the
+// more than NR_MID keys with the whole block, or up to one range of NR_TINY+1..NR_MID keys per
+// wave; ranges of at most NR_TINY keys are finished at the end, one per thread, by the
 // sequential routine itself (median of three, the scanning partition, insertion sort below 7).
 #define NR_MID 512
 #define NR_TINY 32
 #define NR_QCAP 512
+#define NR_SMALL_WG 512  // threads per segment block: its waves partition mid ranges side by side
 
 // loc_min_3_indexx on K/I[l..ir] (LDS), one thread, as the reference's loop (:955-1044)
 __device__ __forceinline__ void nr_serial(double* K, int* I, int l, int ir) {
@@ -610,7 +611,7 @@ __device__ __forceinline__ void nr_serial(double* K, int* I, int l, int ir) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_nr_small(const InstParams* __restrict__ PP, const int* __restrict__ nsegs,
+__global__ __launch_bounds__(NR_SMALL_WG) void k_nr_small(const InstParams* __restrict__ PP, const int* __restrict__ nsegs,
                                                   int* __restrict__ segctr, double* __restrict__ skey,
                                                   int* __restrict__ sid, const int* __restrict__ segs,
                                                   int* __restrict__ err) {
@@ -620,9 +621,10 @@ __global__ __launch_bounds__(256) void k_nr_small(const InstParams* __restrict__
   __shared__ int sRA[NR_BIG];
   __shared__ int q[2 * NR_QCAP];      // pending ranges (more than NR_TINY keys)
   __shared__ int tq[2 * NR_BIG / 2];  // tiny ranges
-  __shared__ int lds[4];
-  __shared__ int sh[5][4];
-  __shared__ double shd[5][1];
+  constexpr int NWV = NR_SMALL_WG / 64;
+  __shared__ int lds[NWV];
+  __shared__ int sh[NWV + 1][4];
+  __shared__ double shd[NWV + 1][1];
   __shared__ int s_seg, s_nq, s_nt, s_take, s_ovf;
   const int inst = blockIdx.y;
   const InstParams& P = PP[inst];
@@ -639,7 +641,7 @@ __global__ __launch_bounds__(256) void k_nr_small(const InstParams* __restrict__
     const int sg = s_seg;
     if (sg >= ns) break;
     const int g0 = SG[2 * sg], g1 = SG[2 * sg + 1], n = g1 - g0 + 1;
-    for (int p = tid; p < n; p += 256) { sK[p] = K[g0 + p]; sI[p] = I[g0 + p]; }
+    for (int p = tid; p < n; p += NR_SMALL_WG) { sK[p] = K[g0 + p]; sI[p] = I[g0 + p]; }
     if (tid == 0) {
       s_nq = 0; s_nt = 0; s_ovf = 0;
       if (n > NR_TINY) { q[0] = 0; q[1] = n - 1; s_nq = 1; }
@@ -655,10 +657,10 @@ __global__ __launch_bounds__(256) void k_nr_small(const InstParams* __restrict__
         l = q[2 * top]; ir = q[2 * top + 1];
         __syncthreads();
         if (tid == 0) { s_nq = top; s_take = 1; }
-        nr_partition<256>(sK, sI, sLA, sRA, l, ir, lds, sh[4], shd[4], is, js);
+        nr_partition<NR_SMALL_WG>(sK, sI, sLA, sRA, l, ir, lds, sh[NWV], shd[NWV], is, js);
       } else {
         int take = 0;
-        for (int t = top; t >= 0 && take < 4; t--) {
+        for (int t = top; t >= 0 && take < NWV; t--) {
           if (q[2 * t + 1] - q[2 * t] + 1 > NR_MID) break;
           take++;
         }
@@ -690,9 +692,9 @@ __global__ __launch_bounds__(256) void k_nr_small(const InstParams* __restrict__
       __syncthreads();
       continue;
     }
-    for (int t = tid; t < s_nt; t += 256) nr_serial(sK, sI, tq[2 * t], tq[2 * t + 1]);
+    for (int t = tid; t < s_nt; t += NR_SMALL_WG) nr_serial(sK, sI, tq[2 * t], tq[2 * t + 1]);
     __syncthreads();
-    for (int p = tid; p < n; p += 256) { K[g0 + p] = sK[p]; I[g0 + p] = sI[p]; }
+    for (int p = tid; p < n; p += NR_SMALL_WG) { K[g0 + p] = sK[p]; I[g0 + p] = sI[p]; }
     __syncthreads();
   }
 }
@@ -1072,7 +1074,7 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_correct(const InstParams* __restr
 void launch_nr_sort(DevState& S, hipStream_t st) {
   launch_fill_int(S.segctr, S.ninst, 0, st);
   k_nr_big<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.nconn, S.cw, S.skey, S.sid, S.Lasc, S.Rasc, S.segs, S.nsegs, S.err);
-  k_nr_small<<<dim3(S.nr_small_gx, S.ninst), 256, 0, st>>>(S.P, S.nsegs, S.segctr, S.skey, S.sid, S.segs, S.err);
+  k_nr_small<<<dim3(S.nr_small_gx, S.ninst), NR_SMALL_WG, 0, st>>>(S.P, S.nsegs, S.segctr, S.skey, S.sid, S.segs, S.err);
   k_nr_pos<<<dim3(16, S.ninst), FSX_CHUNK, 0, st>>>(S.P, S.nconn, S.sid, S.npos);
 }
 
