@@ -172,6 +172,7 @@ def _sync():
 
 GRAPHS = True  # replay captured HIP graphs (off: launch every kernel directly, for kernel-trace profilers)
 TUNE = True    # run the launch-shape autotuner after the warm-up steps of a batch
+SHAPES = None  # or the headline batch's launch shapes, given (walk_gx, jump_blocks, tour_gx, nr_small_gx)
 
 
 def run_gpu(fs, Ls, steps, warmup, profile, tune=False):
@@ -208,6 +209,8 @@ def run_headline(fs, Ls, steps, warmup, prof_steps):
             configure(b, L, k)
         if warmup:
             b.step(warmup)
+        if SHAPES:
+            b.set_tuning(dict(zip(b.TUNING, SHAPES)))
         tuned = b.autotune() if TUNE else b.tuning()
         b.profiling(True)
         b.reset_timers()
@@ -402,10 +405,14 @@ def main():
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every kernel directly instead of replaying HIP graphs (for kernel-trace profiling)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (no autotuner)")
+    ap.add_argument("--shapes", default=None,
+                    help="walk_gx,jump_blocks,tour_gx,nr_small_gx for the headline batch instead of the autotuner")
     args = ap.parse_args()
-    global GRAPHS, TUNE
+    global GRAPHS, TUNE, SHAPES
     GRAPHS = not args.no_graphs
-    TUNE = not args.no_tune
+    TUNE = not args.no_tune and not args.shapes
+    if args.shapes:
+        SHAPES = [int(v) for v in args.shapes.split(",")]
     legs = set(args.legs.split(","))
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (landscapes shard "
@@ -440,7 +447,7 @@ def main():
                    "workload": f"1 x {args.n}x{args.n}, same parameters"} if plain else None),
         "batched_over_plain": value / plain if plain else None,
         "roofline": dict(roofline(dom, live, stats, ncells), traffic=measured_traffic(dom, workload)),
-        "launch_shapes": dict(tuned, chosen_by="autotuner" if TUNE else "defaults"),
+        "launch_shapes": dict(tuned, chosen_by="autotuner" if TUNE else ("given" if SHAPES else "defaults")),
         "phase_ms_per_step": phases,
         "phase_ms_per_step_note": f"profiled window of {args.prof_steps} steps, kernels launched one by one",
         "step_model_frac": round(336.0 * ncells / (dt / args.steps) / 1e9 / HBM_PEAK_GBS, 5),

@@ -1008,6 +1008,19 @@ extern "C" int fsx_get_tuning(fsx_ctx* c, int* out4) {
   return 0;
 }
 
+extern "C" int fsx_set_tuning(fsx_ctx* c, const int* in4) {
+  if (!c || !in4) return fail(FSX_EINVAL, "fsx_set_tuning: null argument");
+  static const int lo[4] = {1, 1, 1, 1}, hi[4] = {4096, 1 << 20, 1024, 4096};
+  for (int i = 0; i < 4; i++)
+    if (in4[i] < lo[i] || in4[i] > hi[i]) return fail(FSX_EINVAL, "fsx_set_tuning: launch shape out of range");
+  c->S.walk_gx = in4[0];
+  c->S.jump_blocks = in4[1];
+  c->S.tour_gx = in4[2];
+  c->S.nr_small_gx = in4[3];
+  c->graph_valid = false;  // captured launches carry the old shapes
+  return 0;
+}
+
 extern "C" int fsx_memory(fsx_ctx* c, long long* bytes, long long* per_node) {
   if (!c) return fail(FSX_EINVAL, "null context");
   if (bytes) *bytes = c->bytes;

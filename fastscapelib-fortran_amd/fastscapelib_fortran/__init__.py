@@ -156,6 +156,14 @@ class Batch:
         check(lib.fsx_get_tuning(self._h, out), "get_tuning")
         return dict(zip(self.TUNING, list(out)))
 
+    def set_tuning(self, shapes):
+        """Set the launch shapes (a dict as returned by ``autotune``/``tuning``; missing keys keep
+        their current value)."""
+        cur = self.tuning()
+        cur.update({k: int(v) for k, v in dict(shapes).items() if k in self.TUNING})
+        arr = (C.c_int * 4)(*[cur[k] for k in self.TUNING])
+        check(lib.fsx_set_tuning(self._h, arr), "set_tuning")
+
     def graphs(self, on=True):
         """Replay each step as one captured HIP graph (default) or launch its kernels one by one."""
         check(lib.fsx_set_graphs(self._h, 1 if on else 0), "set_graphs")

@@ -75,6 +75,7 @@ def _load():
         "fsx_time_phase": [_vp, C.c_int],
         "fsx_autotune": [_vp, C.c_int, _ip],
         "fsx_get_tuning": [_vp, _ip],
+        "fsx_set_tuning": [_vp, _ip],
     }
     for name, args in sig.items():
         f = getattr(lib, name)

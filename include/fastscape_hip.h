@@ -175,6 +175,10 @@ int fsx_reset_timers(fsx_ctx *ctx);
 int fsx_autotune(fsx_ctx *ctx, int reps, int *chosen);
 /* The launch shapes in use (same order as fsx_autotune's chosen). */
 int fsx_get_tuning(fsx_ctx *ctx, int *out4);
+/* Set the launch shapes (same order; e.g. ones an earlier fsx_autotune chose for this workload).
+   Each must be >= 1; walk blocks <= 4096, jump grid <= 2^20, tour blocks <= 1024, sort blocks <= 4096.
+   No counterpart in the reference. */
+int fsx_set_tuning(fsx_ctx *ctx, const int *in4);
 
 /* Device-resident memory of a context (bytes) and the bytes per node the layout costs. */
 int fsx_memory(fsx_ctx *ctx, long long *bytes, long long *bytes_per_node);

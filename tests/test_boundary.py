@@ -58,6 +58,7 @@ def test_host_side_argument_checks(lib):
     assert L.fsx_destroy(None) == 0
     assert L.fsx_autotune(None, 1, None) == -1
     assert L.fsx_get_tuning(None, None) == -1
+    assert L.fsx_set_tuning(None, None) == -1
     assert L.fsx_version().startswith(b"fastscape")
 
 

@@ -38,3 +38,27 @@ def test_autotune_changes_shapes_not_results():
             assert np.array_equal(a.h(k), b.h(k)), k
             assert np.array_equal(a.rec(k), b.rec(k)), k
             assert np.array_equal(a.field(k, 6), b.field(k, 6)), k
+
+
+def test_set_tuning_roundtrip_and_bounds():
+    import fastscapelib_fortran as fs
+    from fastscapelib_fortran import _lib
+    Ls = [landscape(70, 66, seed=41), landscape(45, 52, seed=42, ibc=1010)]
+    print("set_tuning: 2 landscapes, 3 steps", flush=True)
+    with fs.Batch([(L["nx"], L["ny"]) for L in Ls]) as a, fs.Batch([(L["nx"], L["ny"]) for L in Ls]) as b:
+        for k, L in enumerate(Ls):
+            apply(a, L, k)
+            apply(b, L, k)
+        a.step(1)
+        b.step(1)  # b's default-shape graphs are captured here
+        shapes = {"walk_gx": 64, "jump_blocks": 1024, "tour_gx": 32, "nr_small_gx": 16}
+        b.set_tuning(shapes)
+        assert b.tuning() == shapes
+        with pytest.raises(_lib.FastscapeError):
+            b.set_tuning({"walk_gx": 0})
+        assert b.tuning() == shapes  # a rejected call changes nothing
+        a.step(2)
+        b.step(2)
+        for k in range(len(Ls)):
+            assert np.array_equal(a.h(k), b.h(k)), k
+            assert np.array_equal(a.rec(k), b.rec(k)), k
