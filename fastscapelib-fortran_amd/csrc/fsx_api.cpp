@@ -959,6 +959,7 @@ extern "C" int fsx_autotune(fsx_ctx* c, int reps, int* chosen) {
       {&DevState::jump_blocks, {1024, 2048, 4096, 8192}, 4, PH_LM_LABELS, PH_LEVELS},
       {&DevState::tour_gx, {8, 16, 32, 0}, 3, PH_LM_ORIENT, -1},
       {&DevState::nr_small_gx, {16, 32, 64, 128}, 4, PH_LM_SORT, -1},
+      {&DevState::area_wg, {256, 512, 1024, 0}, 3, PH_AREA_SWEEP, -1},
   };
   // keep the caller's profiling state and accumulated timers
   if (int r = flush_timers_any(c, c->stream)) return r;
@@ -999,24 +1000,28 @@ extern "C" int fsx_autotune(fsx_ctx* c, int reps, int* chosen) {
   return 0;
 }
 
-extern "C" int fsx_get_tuning(fsx_ctx* c, int* out4) {
-  if (!c || !out4) return fail(FSX_EINVAL, "fsx_get_tuning: null argument");
-  out4[0] = c->S.walk_gx;
-  out4[1] = c->S.jump_blocks;
-  out4[2] = c->S.tour_gx;
-  out4[3] = c->S.nr_small_gx;
+extern "C" int fsx_get_tuning(fsx_ctx* c, int* out) {
+  if (!c || !out) return fail(FSX_EINVAL, "fsx_get_tuning: null argument");
+  out[0] = c->S.walk_gx;
+  out[1] = c->S.jump_blocks;
+  out[2] = c->S.tour_gx;
+  out[3] = c->S.nr_small_gx;
+  out[4] = c->S.area_wg;
   return 0;
 }
 
-extern "C" int fsx_set_tuning(fsx_ctx* c, const int* in4) {
-  if (!c || !in4) return fail(FSX_EINVAL, "fsx_set_tuning: null argument");
+extern "C" int fsx_set_tuning(fsx_ctx* c, const int* in) {
+  if (!c || !in) return fail(FSX_EINVAL, "fsx_set_tuning: null argument");
   static const int lo[4] = {1, 1, 1, 1}, hi[4] = {4096, 1 << 20, 1024, 4096};
   for (int i = 0; i < 4; i++)
-    if (in4[i] < lo[i] || in4[i] > hi[i]) return fail(FSX_EINVAL, "fsx_set_tuning: launch shape out of range");
-  c->S.walk_gx = in4[0];
-  c->S.jump_blocks = in4[1];
-  c->S.tour_gx = in4[2];
-  c->S.nr_small_gx = in4[3];
+    if (in[i] < lo[i] || in[i] > hi[i]) return fail(FSX_EINVAL, "fsx_set_tuning: launch shape out of range");
+  if (in[4] != 256 && in[4] != 512 && in[4] != 1024)
+    return fail(FSX_EINVAL, "fsx_set_tuning: area_wg must be 256, 512 or 1024");
+  c->S.walk_gx = in[0];
+  c->S.jump_blocks = in[1];
+  c->S.tour_gx = in[2];
+  c->S.nr_small_gx = in[3];
+  c->S.area_wg = in[4];
   c->graph_valid = false;  // captured launches carry the old shapes
   return 0;
 }

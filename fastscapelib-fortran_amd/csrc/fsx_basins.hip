@@ -98,6 +98,7 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_basin_label(const InstParams* __r
                                                            const unsigned long long* __restrict__ J,
                                                            const int* __restrict__ rootrank, const uint8_t* __restrict__ bc,
                                                            const double* __restrict__ h, const int* __restrict__ first,
+                                                           const int* __restrict__ rec, const int* __restrict__ next,
                                                            int* __restrict__ basin, int* __restrict__ bsize,
                                                            WalkNode* __restrict__ wnode) {
   NODE_PROLOGUE
@@ -107,9 +108,27 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_basin_label(const InstParams* __r
     const int b = rootrank[P.off + r];
     bv = !bc[P.off + r] ? b : -(b + 1);  // signed: open basins negative
     basin[P.off + v] = bv;
+    // successor of v in the depth-first preorder of its tree (the order of src/FlowRouting.f90
+    // find_stack): the first donor, else the next sibling of the nearest ancestor-or-self that has
+    // one; -1 when the traversal is back at the root.  Each receiver edge is climbed by one leaf
+    // only (the last of its subtree), so the climbs cost O(nodes) in total.
+    int sv = first[P.off + v];
+    if (sv < 0) {
+      int u = v;
+      for (int guard = 0; guard < P.nn; guard++) {  // the receivers form a forest; bounded anyway
+        const int ru = rec[P.off + u];
+        if (ru == u) break;
+        const int nu = next[P.off + u];
+        if (nu >= 0) {
+          sv = nu;
+          break;
+        }
+        u = ru;
+      }
+    }
     reinterpret_cast<ulonglong2*>(wnode)[P.off + v] =
         make_ulonglong2((unsigned long long)__double_as_longlong(h[P.off + v]),
-                        (unsigned long long)(unsigned)bv | ((unsigned long long)(unsigned)first[P.off + v] << 32));
+                        (unsigned long long)(unsigned)bv | ((unsigned long long)(unsigned)sv << 32));
   }
   // basin sizes: consecutive nodes of a row mostly share a basin, so each run of equal ids within the
   // wave adds its length with one atomic
@@ -147,12 +166,15 @@ __global__ __launch_bounds__(FSX_WG) void k_bsize_scan(const InstParams* __restr
 }
 
 // ---- 2. passes (connect_basins) -------------------------------------------------------------------
-#define WALK_TCAP 16  // pass-table entries per thread kept in LDS; further entries live in the slab
+// Pass-table entries per thread kept in LDS; further entries live in the slab.  8 covers the
+// neighbouring basins of nearly every basin while keeping LDS per workgroup at 6 KB, so that the
+// walk, a chain of dependent loads per lane, runs at 8 waves per SIMD.
+#define WALK_TCAP 8
 // launched with gx blocks per landscape (DevState::walk_gx); thread x walks basins x, x + gx*64, ...
-__global__ __launch_bounds__(64) void k_basin_walk(const InstParams* __restrict__ PP, const int* __restrict__ nbasins,
-                                                   const uint8_t* __restrict__ bc, const int* __restrict__ outlet,
-                                                   const int* __restrict__ rec, const int* __restrict__ next,
-                                                   const WalkNode* __restrict__ wnode,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8)))
+void k_basin_walk(const InstParams* __restrict__ PP, const int* __restrict__ nbasins,
+                  const uint8_t* __restrict__ bc, const int* __restrict__ outlet,
+                  const WalkNode* __restrict__ wnode,
                                                    const int* __restrict__ soff, const int* __restrict__ bsize,
                                                    int* __restrict__ ecount, int* __restrict__ sB, int* __restrict__ sp,
                                                    int* __restrict__ sq, double* __restrict__ sw, int* __restrict__ sh,
@@ -169,14 +191,12 @@ __global__ __launch_bounds__(64) void k_basin_walk(const InstParams* __restrict_
   if (!P.active || !P.run_spl) return;
   const int nb = nbasins[inst];
   const int off = P.off, nx = P.nx, ny = P.ny;
-  const int* R = rec + off;
-  const int* NX = next + off;
   const ulonglong2* WN = reinterpret_cast<const ulonglong2*>(wnode) + off;
   // One flat loop of visits: a lane that finishes a basin starts its next one at once, so the lanes
   // of a wave do not wait for the largest basin of every round (basins differ widely in size).
   const int STRIDE = gx * 64;
   int b = sub * 64 + threadIdx.x - STRIDE;
-  int r = 0, v = 0, ne = 0, cap = 0, visits = 0;
+  int v = 0, ne = 0, cap = 0, visits = 0;
   int *TB = nullptr, *TP = nullptr, *TQ = nullptr, *TH = nullptr;
   double* TW = nullptr;
   // entries beyond WALK_TCAP (large basins) are found through an open-addressing hash over TB:
@@ -188,7 +208,6 @@ __global__ __launch_bounds__(64) void k_basin_walk(const InstParams* __restrict_
     for (b += STRIDE; b < nb; b += STRIDE) {
       const int o = outlet[off + b];
       if (bc[off + o]) continue;  // open basin: no scan
-      r = o;
       v = o;
       ne = 0;
       visits = 0;
@@ -287,18 +306,10 @@ __global__ __launch_bounds__(64) void k_basin_walk(const InstParams* __restrict_
         }
       }
     }
-    // next node in depth-first preorder: first donor, else next sibling up the receiver chain
+    // next node in depth-first preorder (k_basin_label's successor), -1 past the basin's last node
     if (fv >= 0) {
       v = fv;
-      continue;
-    }
-    bool found = false;
-    while (v != r) {
-      const int nx_ = NX[v];
-      if (nx_ >= 0) { v = nx_; found = true; break; }
-      v = R[v];
-    }
-    if (!found) {
+    } else {
       finish();
       live = next_basin();
     }
@@ -1088,13 +1099,13 @@ void launch_lm_basins(DevState& S, hipStream_t st) {
   k_root_scan<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.chunkcnt, S.nbasins, S.basin0);
   k_root_rank<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.rec, S.bc, S.chunkcnt, S.rootrank, S.outlet, S.basin0);
   launch_fill_int(S.bsize, S.ntot, 0, st);
-  k_basin_label<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.J, S.rootrank, S.bc, S.h, S.first, S.basin, S.bsize,
-                                          S.wnode);
+  k_basin_label<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.J, S.rootrank, S.bc, S.h, S.first, S.rec, S.next, S.basin,
+                                          S.bsize, S.wnode);
   k_bsize_scan<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.nbasins, S.bsize, S.soff, S.ecount);
 }
 
 void launch_lm_walk(DevState& S, hipStream_t st) {
-  k_basin_walk<<<xcd_grid(S.ninst, S.walk_gx), 64, 0, st>>>(S.P, S.nbasins, S.bc, S.outlet, S.rec, S.next, S.wnode,
+  k_basin_walk<<<xcd_grid(S.ninst, S.walk_gx), 64, 0, st>>>(S.P, S.nbasins, S.bc, S.outlet, S.wnode,
                                                            S.soff, S.bsize, S.ecount, S.sB, S.sp, S.sq, S.sw, S.sh,
                                                            S.err, S.ninst, S.walk_gx);
 }

@@ -49,10 +49,11 @@ struct Chunk {
 
 // Per-position records of the single-flow sweeps, one aligned block per position so that a prep
 // kernel writes whole segments and a sweep reads one contiguous record.
-// What the basin walk reads of a node, in one 16-byte load: elevation, signed basin id, first donor.
+// What the basin walk reads of a node, in one 16-byte load: elevation, signed basin id, and the next
+// node of its tree's depth-first preorder (-1 after the last).
 struct alignas(16) WalkNode {
   double h;
-  int basin, first;
+  int basin, succ;
 };
 
 struct alignas(16) AreaRec64 {  // drainage area: donor positions (relative, -1 unused), own term

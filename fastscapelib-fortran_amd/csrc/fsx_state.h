@@ -15,6 +15,7 @@ struct DevState {
   int jump_blocks = 2048;  // pointer jumping: grid cap
   int tour_gx = 16;        // Euler tour kernels: blocks per landscape (x4 for the ranking rounds)
   int nr_small_gx = 64;    // edge sort: blocks per landscape for the segments in LDS
+  int area_wg = 1024;      // drainage-area sweep: threads per landscape (256, 512 or 1024)
 
   // ---- per node (ntot) ----
   double *h = nullptr, *b = nullptr, *u = nullptr, *kf = nullptr, *kd = nullptr, *precip = nullptr;
@@ -33,7 +34,7 @@ struct DevState {
   double *HPl = nullptr, *DHl = nullptr, *ELl = nullptr;
   AreaRec64* arec = nullptr;  // per-position drainage-area records
   SplRec32* srec = nullptr;   // per-position stream-power records
-  WalkNode* wnode = nullptr;  // basin walk: (h, basin, first donor) per node
+  WalkNode* wnode = nullptr;  // basin walk: (h, basin, preorder successor) per node
 
   // ---- per basin (node offsets) ----
   int *outlet = nullptr, *bsize = nullptr, *soff = nullptr, *ecount = nullptr, *eoff = nullptr;

@@ -363,17 +363,24 @@ __device__ __forceinline__ void area_load(AreaRec& r, long long gp, const AreaRe
   r.d[4] = y.x; r.d[5] = y.y; r.d[6] = y.z; r.d[7] = y.w;
 }
 
-__global__ __launch_bounds__(FSX_WG) void k_area_sweep(const InstParams* __restrict__ PP, const int* __restrict__ lstart,
-                                                       const int4* __restrict__ grec, const int* __restrict__ ngroups,
-                                                       const AreaRec64* __restrict__ arec, double* __restrict__ Al,
-                                                       double* __restrict__ area, int vcap) {
+// WG threads per landscape: NS record slots per thread cover a group's GRP_STG positions, NL
+// level-start slots its <= GRP_BUCKET levels.  Fewer threads shorten a level's critical path (fewer
+// idle waves sharing each SIMD's issue and the barrier); wide levels then take several positions per
+// thread.
+template <int WG>
+__global__ __launch_bounds__(WG) void k_area_sweep(const InstParams* __restrict__ PP, const int* __restrict__ lstart,
+                                                   const int4* __restrict__ grec, const int* __restrict__ ngroups,
+                                                   const AreaRec64* __restrict__ arec, double* __restrict__ Al,
+                                                   int vcap) {
+  constexpr int NS = GRP_STG / WG;
+  constexpr int NL = (GRP_BUCKET + WG - 1) / WG;
+  static_assert(GRP_STG % WG == 0 && WG >= 64 && WG % 64 == 0, "workgroup size");
   extern __shared__ double sm[];
   // level values double-buffered at sm[0..vcap) and sm[vcap..2*vcap); offsets, not a pointer array,
   // so every access stays an LDS (ds_*) instruction
   double* sA = sm + 2 * vcap;
   int* sD = reinterpret_cast<int*>(sA + GRP_STG);  // donor slot k of staged position t at k*GRP_STG + t
-  int* sO = sD + 8 * GRP_STG;
-  int* sL = sO + GRP_STG;  // level starts of the group, GRP_BUCKET+1
+  int* sL = sD + 8 * GRP_STG;  // level starts of the group, GRP_BUCKET+1
   const int inst = blockIdx.x;
   const InstParams& P = PP[inst];
   if (!P.active || !P.run_spl || !P.sfd) return;
@@ -382,14 +389,17 @@ __global__ __launch_bounds__(FSX_WG) void k_area_sweep(const InstParams* __restr
   const int* ls = lstart + P.aux;
   const int4* gr = grec + 2 * (long long)P.aux;
   const int G = ngroups[inst];
-  AreaRec r0, r1;
-  int rl = 0;
+  AreaRec rr[NS];
+  int rl[NL];
   auto prefetch = [&](const GroupMeta& m) {
     const int np = m.p1 - m.p0;
     if (np > GRP_STG) return;
-    if (tid < np) area_load(r0, off + m.p0 + tid, arec);
-    if (tid + FSX_WG < np) area_load(r1, off + m.p0 + tid + FSX_WG, arec);
-    if (tid < m.hi - m.lo) rl = ls[m.lo + tid];
+#pragma unroll
+    for (int u = 0; u < NS; u++)
+      if (tid + u * WG < np) area_load(rr[u], off + m.p0 + tid + u * WG, arec);
+#pragma unroll
+    for (int u = 0; u < NL; u++)
+      if (tid + u * WG < m.hi - m.lo) rl[u] = ls[m.lo + tid + u * WG];
   };
   GroupMeta cur = meta_bcast(meta_lane(gr, G, G - 1));
   GroupMeta nxt_m = meta_bcast(meta_lane(gr, G, G - 2));
@@ -398,18 +408,18 @@ __global__ __launch_bounds__(FSX_WG) void k_area_sweep(const InstParams* __restr
     const int np = cur.p1 - cur.p0;
     const bool staged = np <= GRP_STG;
     if (staged) {
-      if (tid < np) {
-        sA[tid] = r0.a;
 #pragma unroll
-        for (int k = 0; k < 8; k++) sD[k * GRP_STG + tid] = r0.d[k];
-      }
-      if (tid + FSX_WG < np) {
-        const int t = tid + FSX_WG;
-        sA[t] = r1.a;
+      for (int u = 0; u < NS; u++) {
+        const int t = tid + u * WG;
+        if (t < np) {
+          sA[t] = rr[u].a;
 #pragma unroll
-        for (int k = 0; k < 8; k++) sD[k * GRP_STG + t] = r1.d[k];
+          for (int k = 0; k < 8; k++) sD[k * GRP_STG + t] = rr[u].d[k];
+        }
       }
-      if (tid < cur.hi - cur.lo) sL[tid] = rl;
+#pragma unroll
+      for (int u = 0; u < NL; u++)
+        if (tid + u * WG < cur.hi - cur.lo) sL[tid + u * WG] = rl[u];
       if (tid == 0) sL[cur.hi - cur.lo] = cur.p1;
     }
     lds_barrier();
@@ -420,8 +430,8 @@ __global__ __launch_bounds__(FSX_WG) void k_area_sweep(const InstParams* __restr
 #pragma unroll
       for (int k = 0; k < 8; k++) r.d[k] = sD[k * GRP_STG + si];
     };
-    // one level: w positions from b0, record x0 of position tid when staged (position tid + FSX_WG,
-    // rare, is read from the staging area); the level above (donors) has wn positions from b0 + w
+    // one level: w positions from b0, record x0 of position tid when staged (positions tid + WG, ...
+    // are read from the staging area); the level above (donors) has wn positions from b0 + w
     auto level = [&](int l, int b0, int w, int wn, const AreaRec& x0) {
       double* cv = sm + ((l & 1) ? vcap : 0);
       const double* nv = sm + ((l & 1) ? 0 : vcap);
@@ -432,37 +442,41 @@ __global__ __launch_bounds__(FSX_WG) void k_area_sweep(const InstParams* __restr
       // the reference's summation order)
       auto one = [&](int i, const AreaRec& r, auto fast_tag) {
         constexpr bool FAST = decltype(fast_tag)::value;
-        {
-          double a = r.a;
-          double dv[8];
+        double a = r.a;
+        double dv[8];
 #pragma unroll
-          for (int k = 0; k < 8; k++) {
-            const int q = r.d[k];
-            if constexpr (FAST) dv[k] = nv[q < 0 ? 0 : q];
-            else dv[k] = q < 0 ? 0.0 : (nin ? nv[q] : Al[off + bn + q]);
-          }
-          if constexpr (FAST) {
-            __builtin_amdgcn_sched_barrier(0);  // all eight reads in flight before any is used
-#pragma unroll
-            for (int k = 0; k < 8; k++) dv[k] = r.d[k] < 0 ? 0.0 : dv[k];
-          }
-#pragma unroll
-          for (int k = 0; k < 8; k++) a = a + dv[k];
-          if (l == 0) a = a + a;
-          if (w <= vcap) cv[i] = a;
-          Al[off + b0 + i] = a;
+        for (int k = 0; k < 8; k++) {
+          const int q = r.d[k];
+          if constexpr (FAST) dv[k] = nv[q < 0 ? 0 : q];
+          else dv[k] = q < 0 ? 0.0 : (nin ? nv[q] : Al[off + bn + q]);
         }
+        if constexpr (FAST) {
+          __builtin_amdgcn_sched_barrier(0);  // all eight reads in flight before any is used
+#pragma unroll
+          for (int k = 0; k < 8; k++) dv[k] = r.d[k] < 0 ? 0.0 : dv[k];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) a = a + dv[k];
+        if (l == 0) a = a + a;
+        if (w <= vcap) cv[i] = a;
+        Al[off + b0 + i] = a;
       };
-      AreaRec x1;
-      if (staged && tid + FSX_WG < w) stg(x1, b0 - cur.p0 + tid + FSX_WG);
       if (staged && nin) {
         if (tid < w) one(tid, x0, std::true_type{});
-        if (tid + FSX_WG < w) one(tid + FSX_WG, x1, std::true_type{});
+        for (int i = tid + WG; i < w; i += WG) {
+          AreaRec x;
+          stg(x, b0 - cur.p0 + i);
+          one(i, x, std::true_type{});
+        }
       } else if (staged) {
         if (tid < w) one(tid, x0, std::false_type{});
-        if (tid + FSX_WG < w) one(tid + FSX_WG, x1, std::false_type{});
+        for (int i = tid + WG; i < w; i += WG) {
+          AreaRec x;
+          stg(x, b0 - cur.p0 + i);
+          one(i, x, std::false_type{});
+        }
       } else {
-        for (int i = tid; i < w; i += FSX_WG) {
+        for (int i = tid; i < w; i += WG) {
           AreaRec r;
           area_load(r, off + b0 + i, arec);
           one(i, r, std::false_type{});
@@ -833,7 +847,7 @@ __global__ __launch_bounds__(FSX_WG) void k_spl_sweep(
 
 // ---------------------------------------------------------------------------------------------------
 static size_t area_smem(int vcap) {
-  return 2 * vcap * sizeof(double) + GRP_STG * (sizeof(double) + 9 * sizeof(int)) + (GRP_BUCKET + 1) * sizeof(int);
+  return 2 * vcap * sizeof(double) + GRP_STG * (sizeof(double) + 8 * sizeof(int)) + (GRP_BUCKET + 1) * sizeof(int);
 }
 static size_t spl_smem(int vcap) {
   return (size_t)vcap * (6 * sizeof(double) + 2 * sizeof(int)) + GRP_STG * (3 * sizeof(double) + 2 * sizeof(int)) +
@@ -846,7 +860,8 @@ int sweep_set_lds(int smem_max) {
   if ((int)area_smem(kAreaVcap) > smem_max || (int)spl_smem(kSplVcap) + 4096 > smem_max ||
       LVL_HCAP * 4 > smem_max)
     return -1;
-  if (hipFuncSetAttribute((const void*)k_area_sweep, hipFuncAttributeMaxDynamicSharedMemorySize, (int)area_smem(kAreaVcap)) != hipSuccess) return -1;
+  for (const void* f : {(const void*)k_area_sweep<256>, (const void*)k_area_sweep<512>, (const void*)k_area_sweep<1024>})
+    if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)area_smem(kAreaVcap)) != hipSuccess) return -1;
   if (hipFuncSetAttribute((const void*)k_spl_sweep, hipFuncAttributeMaxDynamicSharedMemorySize, (int)spl_smem(kSplVcap)) != hipSuccess) return -1;
   if (hipFuncSetAttribute((const void*)k_level_sort, hipFuncAttributeMaxDynamicSharedMemorySize, LVL_HCAP * 4) != hipSuccess) return -1;
   return 0;
@@ -877,8 +892,13 @@ void launch_area(DevState& S, hipStream_t st) {
 }
 
 void launch_area_sweep(DevState& S, hipStream_t st) {
-  k_area_sweep<<<S.ninst, FSX_WG, area_smem(kAreaVcap), st>>>(S.P, S.lstart, S.grec, S.ngroups, S.arec,
-                                                                S.Al, S.area, kAreaVcap);
+  const size_t sm = area_smem(kAreaVcap);
+  if (S.area_wg == 256)
+    k_area_sweep<256><<<S.ninst, 256, sm, st>>>(S.P, S.lstart, S.grec, S.ngroups, S.arec, S.Al, kAreaVcap);
+  else if (S.area_wg == 512)
+    k_area_sweep<512><<<S.ninst, 512, sm, st>>>(S.P, S.lstart, S.grec, S.ngroups, S.arec, S.Al, kAreaVcap);
+  else
+    k_area_sweep<1024><<<S.ninst, 1024, sm, st>>>(S.P, S.lstart, S.grec, S.ngroups, S.arec, S.Al, kAreaVcap);
 }
 
 void launch_spl_fact(DevState& S, hipStream_t st) {

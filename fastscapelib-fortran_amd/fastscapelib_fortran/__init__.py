@@ -142,17 +142,17 @@ class Batch:
         k = -1 if name is None else _lib.PHASES.index(name)
         check(lib.fsx_time_phase(self._h, k), "time_phase")
 
-    TUNING = ("walk_gx", "jump_blocks", "tour_gx", "nr_small_gx")
+    TUNING = ("walk_gx", "jump_blocks", "tour_gx", "nr_small_gx", "area_wg")
 
     def autotune(self, reps=2):
         """Choose the launch shapes of the routing kernels by timing candidates on the current
         landscapes (which it leaves unchanged); returns the chosen shapes."""
-        out = (C.c_int * 4)()
+        out = (C.c_int * len(self.TUNING))()
         check(lib.fsx_autotune(self._h, int(reps), out), "autotune")
         return dict(zip(self.TUNING, list(out)))
 
     def tuning(self):
-        out = (C.c_int * 4)()
+        out = (C.c_int * len(self.TUNING))()
         check(lib.fsx_get_tuning(self._h, out), "get_tuning")
         return dict(zip(self.TUNING, list(out)))
 
@@ -161,7 +161,7 @@ class Batch:
         their current value)."""
         cur = self.tuning()
         cur.update({k: int(v) for k, v in dict(shapes).items() if k in self.TUNING})
-        arr = (C.c_int * 4)(*[cur[k] for k in self.TUNING])
+        arr = (C.c_int * len(self.TUNING))(*[cur[k] for k in self.TUNING])
         check(lib.fsx_set_tuning(self._h, arr), "set_tuning")
 
     def graphs(self, on=True):
