@@ -45,7 +45,7 @@ ALG_BYTES_PER_NODE = {
     "uplift": 40,        # R h,u,b  W h,b
     "receivers": 20,     # R h  W rec(i32), length
     "donors": 13,        # R rec  W dmask(u8), first, next
-    "lm_walk": 24,       # R (h, basin, first) record, next, rec (nodes of closed basins ~ all nodes)
+    "lm_walk": 17,       # R (h, basin, preorder successor) record, candidate mask (closed-basin nodes ~ all)
     "area_prep": 65,     # R J, dmask, pos, donor pos, precip  W record (a0, 8 donor slots)
     "area_sweep": 48,    # R record  W Al
     "spl_prep": 48,      # R h, b, kf  W kfint, g, ht

@@ -321,7 +321,7 @@ static int ctx_alloc(long long capacity, int nslots, fsx_ctx** out) {
   A(nsegs, ninst); A(segctr, ninst); A(maxd, ninst);
   A(J, N); A(ngroups, ninst); A(grec, 2 * (N + ninst)); A(first, N); A(next, N); A(gmeta, N + ninst); A(pos, N); A(lcur, N + ninst); A(rp, N); A(LSl, N); A(dn, 8 * N);
   A(A0, N); A(Al, N); A(F, N); A(HTl, N); A(Hl, N); A(Wl, N); A(HWl, N); A(HPl, N); A(DHl, N); A(ELl, N);
-  A(arec, N); A(srec, N); A(wnode, N);
+  A(arec, N); A(srec, N); A(wnode, N); A(wmask, N);
 #undef A
   if (rc) { fsx_destroy(c); return rc; }
   *out = c;

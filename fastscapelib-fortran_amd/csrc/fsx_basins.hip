@@ -166,6 +166,32 @@ __global__ __launch_bounds__(FSX_WG) void k_bsize_scan(const InstParams* __restr
 }
 
 // ---- 2. passes (connect_basins) -------------------------------------------------------------------
+// Which neighbours the pass search of a node examines (src/LocalMinima.f90:281-297): those inside the
+// grid whose basin is open or has a larger id than the node's closed basin.  Node-parallel over the
+// labels, so that the walk loads only these neighbours' records (about two of eight).
+__global__ __launch_bounds__(FSX_CHUNK) void k_walk_mask(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
+                                                         const int* __restrict__ basin, uint8_t* __restrict__ wmask) {
+  NODE_PROLOGUE
+  if (!live) return;
+  const int* Bn = basin + P.off;
+  const int b = Bn[v];
+  unsigned m = 0;
+  if (b >= 0) {
+    constexpr int II[8] = {-1, 1, 0, 0, -1, 1, -1, 1};
+    constexpr int JJ[8] = {0, 0, -1, 1, -1, -1, 1, 1};
+    const int i = v % P.nx, j = v / P.nx;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const int ki = i + II[k], kj = j + JJ[k];
+      if (ki < 0 || ki >= P.nx || kj < 0 || kj >= P.ny) continue;
+      const int Bs = Bn[kj * P.nx + ki];
+      if (Bs < 0 || Bs > b) m |= 1u << k;
+    }
+  }
+  wmask[P.off + v] = (uint8_t)m;
+}
+
+
 // Pass-table entries per thread kept in LDS; further entries live in the slab.  8 covers the
 // neighbouring basins of nearly every basin while keeping LDS per workgroup at 6 KB, so that the
 // walk, a chain of dependent loads per lane, runs at 8 waves per SIMD.
@@ -174,7 +200,7 @@ __global__ __launch_bounds__(FSX_WG) void k_bsize_scan(const InstParams* __restr
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8)))
 void k_basin_walk(const InstParams* __restrict__ PP, const int* __restrict__ nbasins,
                   const uint8_t* __restrict__ bc, const int* __restrict__ outlet,
-                  const WalkNode* __restrict__ wnode,
+                  const WalkNode* __restrict__ wnode, const uint8_t* __restrict__ wmask,
                                                    const int* __restrict__ soff, const int* __restrict__ bsize,
                                                    int* __restrict__ ecount, int* __restrict__ sB, int* __restrict__ sp,
                                                    int* __restrict__ sq, double* __restrict__ sw, int* __restrict__ sh,
@@ -190,8 +216,9 @@ void k_basin_walk(const InstParams* __restrict__ PP, const int* __restrict__ nba
   const InstParams& P = PP[inst];
   if (!P.active || !P.run_spl) return;
   const int nb = nbasins[inst];
-  const int off = P.off, nx = P.nx, ny = P.ny;
+  const int off = P.off, nx = P.nx;
   const ulonglong2* WN = reinterpret_cast<const ulonglong2*>(wnode) + off;
+  const uint8_t* MK = wmask + off;
   // One flat loop of visits: a lane that finishes a basin starts its next one at once, so the lanes
   // of a wave do not wait for the largest basin of every round (basins differ widely in size).
   const int STRIDE = gx * 64;
@@ -237,29 +264,28 @@ void k_basin_walk(const InstParams* __restrict__ PP, const int* __restrict__ nba
       live = next_basin();
       continue;
     }
-    // scan the 8 neighbours of v (no wrap), src/LocalMinima.f90:281-320
+    // scan the 8 neighbours of v (no wrap), src/LocalMinima.f90:281-320: only those k_walk_mask
+    // marked (inside the grid, open basin or larger id) are loaded
     const int i = v % nx, j = v / nx;
     const ulonglong2 wv = WN[v];
+    const unsigned mv = MK[v];
     const double hv = __longlong_as_double((long long)wv.x);
     const int fv = (int)(unsigned)(wv.y >> 32);
     int qs[8], bs[8];
     double hq[8];
 #pragma unroll
     for (int k = 0; k < 8; k++) {
-      const int ki = i + II[k], kj = j + JJ[k];
-      const bool in = !(ki < 0 || ki >= nx || kj < 0 || kj >= ny);
-      qs[k] = in ? kj * nx + ki : -1;
-      const ulonglong2 wq = in ? WN[qs[k]] : make_ulonglong2(0ull, 0ull);
+      qs[k] = (j + JJ[k]) * nx + (i + II[k]);
+      const ulonglong2 wq = ((mv >> k) & 1u) ? WN[qs[k]] : make_ulonglong2(0ull, 0ull);
       bs[k] = (int)(unsigned)(wq.y & 0xffffffffull);
       hq[k] = __longlong_as_double((long long)wq.x);
     }
 #pragma unroll
     for (int k = 0; k < 8; k++) {
-      if (qs[k] < 0) continue;
+      if (!((mv >> k) & 1u)) continue;
       const int Bs = bs[k];
       const bool closedB = Bs >= 0;
       const int B = closedB ? Bs : -(Bs + 1);
-      if (b >= B && closedB) continue;
       const double w = hv > hq[k] ? hv : hq[k];
       int t = 0;
       const int nl = ne < WALK_TCAP ? ne : WALK_TCAP;
@@ -1102,10 +1128,11 @@ void launch_lm_basins(DevState& S, hipStream_t st) {
   k_basin_label<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.J, S.rootrank, S.bc, S.h, S.first, S.rec, S.next, S.basin,
                                           S.bsize, S.wnode);
   k_bsize_scan<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.nbasins, S.bsize, S.soff, S.ecount);
+  k_walk_mask<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.basin, S.wmask);
 }
 
 void launch_lm_walk(DevState& S, hipStream_t st) {
-  k_basin_walk<<<xcd_grid(S.ninst, S.walk_gx), 64, 0, st>>>(S.P, S.nbasins, S.bc, S.outlet, S.wnode,
+  k_basin_walk<<<xcd_grid(S.ninst, S.walk_gx), 64, 0, st>>>(S.P, S.nbasins, S.bc, S.outlet, S.wnode, S.wmask,
                                                            S.soff, S.bsize, S.ecount, S.sB, S.sp, S.sq, S.sw, S.sh,
                                                            S.err, S.ninst, S.walk_gx);
 }

@@ -35,6 +35,7 @@ struct DevState {
   AreaRec64* arec = nullptr;  // per-position drainage-area records
   SplRec32* srec = nullptr;   // per-position stream-power records
   WalkNode* wnode = nullptr;  // basin walk: (h, basin, preorder successor) per node
+  uint8_t* wmask = nullptr;   // basin walk: neighbours k (bit k) that the pass search of a node examines
 
   // ---- per basin (node offsets) ----
   int *outlet = nullptr, *bsize = nullptr, *soff = nullptr, *ecount = nullptr, *eoff = nullptr;
