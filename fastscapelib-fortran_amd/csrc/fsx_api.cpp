@@ -318,7 +318,7 @@ static int ctx_alloc(long long capacity, int nslots, fsx_ctx** out) {
   A(sh, FSX_SLAB_CAP * N);
   A(nbasins, ninst); A(basin0, ninst); A(nconn, ninst); A(nlev, ninst); A(err, ninst); A(ngs, ninst);
   A(ntree, ninst); A(chunkcnt, c->chunk_cap); A(newton, ninst); A(scal, ninst); A(redk, 2 * ninst); A(flags, 64);
-  A(nsegs, ninst); A(segctr, ninst); A(maxd, ninst);
+  A(nsegs, ninst); A(segctr, ninst); A(maxd, ninst); A(bstate, 4 * ninst);
   A(J, N); A(ngroups, ninst); A(grec, 2 * (N + ninst)); A(first, N); A(next, N); A(gmeta, N + ninst); A(pos, N); A(lcur, N + ninst); A(rp, N); A(LSl, N); A(dn, 8 * N);
   A(A0, N); A(Al, N); A(F, N); A(HTl, N); A(Hl, N); A(Wl, N); A(HWl, N); A(HPl, N); A(DHl, N); A(ELl, N);
   A(arec, N); A(srec, N); A(wnode, N); A(wmask, N);
@@ -960,6 +960,7 @@ extern "C" int fsx_autotune(fsx_ctx* c, int reps, int* chosen) {
       {&DevState::tour_gx, {8, 16, 32, 0}, 3, PH_LM_ORIENT, -1},
       {&DevState::nr_small_gx, {16, 32, 64, 128}, 4, PH_LM_SORT, -1},
       {&DevState::area_wg, {256, 512, 1024, 0}, 3, PH_AREA_SWEEP, -1},
+      {&DevState::bor_gx, {2, 4, 8, 16}, 4, PH_LM_MST, -1},
   };
   // keep the caller's profiling state and accumulated timers
   if (int r = flush_timers_any(c, c->stream)) return r;
@@ -1007,6 +1008,7 @@ extern "C" int fsx_get_tuning(fsx_ctx* c, int* out) {
   out[2] = c->S.tour_gx;
   out[3] = c->S.nr_small_gx;
   out[4] = c->S.area_wg;
+  out[5] = c->S.bor_gx;
   return 0;
 }
 
@@ -1017,11 +1019,13 @@ extern "C" int fsx_set_tuning(fsx_ctx* c, const int* in) {
     if (in[i] < lo[i] || in[i] > hi[i]) return fail(FSX_EINVAL, "fsx_set_tuning: launch shape out of range");
   if (in[4] != 256 && in[4] != 512 && in[4] != 1024)
     return fail(FSX_EINVAL, "fsx_set_tuning: area_wg must be 256, 512 or 1024");
+  if (in[5] < 1 || in[5] > 1024) return fail(FSX_EINVAL, "fsx_set_tuning: bor_gx must be in [1, 1024]");
   c->S.walk_gx = in[0];
   c->S.jump_blocks = in[1];
   c->S.tour_gx = in[2];
   c->S.nr_small_gx = in[3];
   c->S.area_wg = in[4];
+  c->S.bor_gx = in[5];
   c->graph_valid = false;  // captured launches carry the old shapes
   return 0;
 }

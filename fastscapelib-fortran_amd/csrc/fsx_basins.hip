@@ -763,147 +763,314 @@ struct EdgeList {
   int *a, *c, *k;
 };
 
-template <bool INLDS>
-__device__ __forceinline__ void boruvka_body(const int nb, const int n, const int* E0, const int* E1, const int* NP,
-                                             const int* SI, int* C, unsigned long long* BST, int* BN, uint8_t* IT,
-                                             EdgeList LA, EdgeList LB, int* s_any, int* s_chg, int* s_cnt) {
+// The components still roots at the start of a round are kept as a list too (ping-pong between
+// LC0 and LC1; round 0 is every basin): a round's per-component passes (clearing the minima, the
+// merge, the pointer jumping over the merge pointers and the id update) touch only those, and only
+// their ids can appear in the next round's edge list.
+//
+// The first BOR_SPLIT rounds, whose edge lists are long, run as separate launches: the edge scan
+// with DevState::bor_gx workgroups per landscape (component ids in global memory), the rest of the
+// round with one.  k_boruvka then finishes the remaining rounds in one workgroup with the component
+// ids in LDS.  Per-landscape state between the launches: bstate[4*inst + 0..3] = live edges, live
+// components, done, edges kept by the running scan.
+#define BOR_SPLIT 4
+
+// One workgroup's share of a round's edge scan: blocks of FSX_WG*U edges starting at block g,
+// every G-th.  cnt: counter of the edges kept (LDS or global).
+__device__ __forceinline__ void bor_scan(int round, int ne, int g, int G, const int* E0, const int* E1, const int* NP,
+                                         const int* C, unsigned long long* BST, EdgeList in, EdgeList out, int* cnt) {
+  constexpr int U = BORUVKA_UNROLL;
+  const int tid = threadIdx.x, lane = tid & 63;
+  // lowest-ranked live edge of every component, with the component at its other end:
+  // (rank << 32 | other) -- ranks are distinct, so the minimum names one edge.  The U edges of a
+  // thread are loaded unconditionally (index clamped) so that their loads issue together.
+  const int* pa = round ? in.a : E0;
+  const int* pc = round ? in.c : E1;
+  const int* pk = round ? in.k : NP;
+  for (int e0 = g * FSX_WG * U + tid; e0 < ne; e0 += G * FSX_WG * U) {
+    int a[U], c[U], k[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int e = e0 + u * FSX_WG;
+      const int ee = e < ne ? e : e0;
+      a[u] = pa[ee];
+      c[u] = pc[ee];
+      k[u] = pk[ee];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      a[u] = C[a[u]];
+      c[u] = C[c[u]];
+    }
+    bool live[U];
+    unsigned long long m[U];
+    int tot = 0;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      live[u] = e0 + u * FSX_WG < ne && a[u] != c[u];
+      const unsigned long long kr = (unsigned long long)(unsigned)k[u] << 32;
+      // the list keeps a component's edges together (round 0: a basin's passes are consecutive), so
+      // lanes of one run of equal a first take the minimum over windows of 8 lanes of the run, and one
+      // lane per window issues the atomic for that side
+      const int key = live[u] ? a[u] : -1 - lane;
+      unsigned long long va = live[u] ? (kr | (unsigned)c[u]) : ~0ull;
+#pragma unroll
+      for (int off = 1; off < 8; off <<= 1) {
+        const unsigned long long o = __shfl_down(va, off, 64);
+        const int ok = __shfl_down(key, off, 64);
+        if (lane + off < 64 && ok == key && o < va) va = o;
+      }
+      const int kprev = __shfl_up(key, 1, 64);
+      const unsigned long long heads = __ballot(lane == 0 || kprev != key);
+      const int rs = 63 - __clzll((long long)(heads & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull))));
+      if (live[u]) {
+        if (((lane - rs) & 7) == 0) atomicMin(BST + a[u], va);
+        atomicMin(BST + c[u], kr | (unsigned)a[u]);
+      }
+      m[u] = __ballot(live[u]);
+      tot += __popcll(m[u]);
+    }
+    // keep the live edges (as component ids) for the next round: one atomic per wave
+    int base = 0;
+    if (lane == 0 && tot) base = atomicAdd(cnt, tot);
+    base = __shfl(base, 0, 64);
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (live[u]) {
+        const int r = base + __popcll(m[u] & ((1ull << lane) - 1ull));
+        out.a[r] = a[u];
+        out.c[r] = c[u];
+        out.k[r] = k[u];
+      }
+      base += __popcll(m[u]);
+    }
+  }
+}
+
+// The rest of a round, one workgroup: every component joins the other end of its edge (the smaller
+// id of a mutual pair stays root), the merge pointers are jumped to their roots, the round's
+// components take their new ids and those still roots form the next list (their minima cleared
+// when clear_next).  Returns false when no component had an edge (the forest is complete).
+__device__ __forceinline__ bool bor_merge(int round, int nl, const int* SI, int* C, unsigned long long* BST, int* BN,
+                                          uint8_t* IT, const int* Lin, int* Lout, bool clear_next, int* s_any,
+                                          int* s_chg, int* s_ncomp) {
   constexpr int U = BORUVKA_UNROLL;
   constexpr unsigned long long NONE = ~0ull;
   const int tid = threadIdx.x, lane = tid & 63;
-  int ne = n;
-  for (int round = 0; round < 64; round++) {
+  auto comp_at = [&](int i) { return round ? Lin[i] : i; };
+  if (tid == 0) { *s_any = 0; *s_ncomp = 0; }
+  __syncthreads();
+  for (int i0 = tid; i0 < nl; i0 += FSX_WG * U) {
+    int cc[U];
+    unsigned long long v[U], w[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int i = i0 + u * FSX_WG;
+      cc[u] = i < nl ? comp_at(i) : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) v[u] = cc[u] >= 0 ? BST[cc[u]] : NONE;
+#pragma unroll
+    for (int u = 0; u < U; u++) w[u] = v[u] != NONE ? BST[(int)(v[u] & 0xffffffffu)] : NONE;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (cc[u] < 0) continue;
+      if (v[u] == NONE) { BN[cc[u]] = cc[u]; continue; }
+      const int kk = (int)(v[u] >> 32), o = (int)(v[u] & 0xffffffffu);
+      IT[SI[kk]] = 1;
+      BN[cc[u]] = (int)(w[u] >> 32) == kk ? (cc[u] < o ? cc[u] : o) : o;
+      *s_any = 1;
+    }
+  }
+  __syncthreads();
+  if (!*s_any) return false;
+  for (;;) {
+    if (tid == 0) *s_chg = 0;
+    __syncthreads();
+    for (int i0 = tid; i0 < nl; i0 += FSX_WG * U) {
+      int cc[U], x[U], y[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int i = i0 + u * FSX_WG;
+        cc[u] = i < nl ? comp_at(i) : -1;
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) x[u] = cc[u] >= 0 ? BN[cc[u]] : -1;
+#pragma unroll
+      for (int u = 0; u < U; u++) y[u] = x[u] >= 0 ? BN[x[u]] : -1;
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        if (x[u] != y[u]) { BN[cc[u]] = y[u]; *s_chg = 1; }
+    }
+    __syncthreads();
+    if (!*s_chg) break;
+  }
+  for (int i0 = tid; i0 < nl; i0 += FSX_WG) {
+    const int cc = comp_at(i0);
+    const int r = BN[cc];
+    C[cc] = r;
+    const bool root = r == cc;
+    const unsigned long long m = __ballot(root);
+    const int cnt = __popcll(m);
+    int base = 0;
+    if (lane == 0 && cnt) base = atomicAdd(s_ncomp, cnt);
+    base = __shfl(base, 0, 64);
+    if (root) {
+      Lout[base + __popcll(m & ((1ull << lane) - 1ull))] = cc;
+      if (clear_next) BST[cc] = NONE;
+    }
+  }
+  __syncthreads();
+  return true;
+}
+
+// Rounds round0.. of one landscape in one workgroup (edges ne, components nl at round0).
+template <bool INLDS>
+__device__ __forceinline__ void boruvka_body(int round0, int ne, int nl, const int* E0, const int* E1, const int* NP,
+                                             const int* SI, int* C, unsigned long long* BST, int* BN, uint8_t* IT,
+                                             EdgeList LA, EdgeList LB, int* LC0, int* LC1, int* s_any, int* s_chg,
+                                             int* s_cnt, int* s_ncomp) {
+  constexpr unsigned long long NONE = ~0ull;
+  const int tid = threadIdx.x;
+  for (int round = round0; round < 64; round++) {
     const EdgeList out = (round & 1) ? LB : LA;
     const EdgeList in = (round & 1) ? LA : LB;  // read from round 1 on
-    for (int b = tid; b < nb; b += FSX_WG) BST[b] = NONE;
-    if (tid == 0) { *s_any = 0; *s_cnt = 0; }
+    const int* Lin = (round & 1) ? LC1 : LC0;    // live components (read from round 1 on)
+    int* Lout = (round & 1) ? LC0 : LC1;
+    for (int i = tid; i < nl; i += FSX_WG) BST[round ? Lin[i] : i] = NONE;
+    if (tid == 0) *s_cnt = 0;
     __syncthreads();
-    // lowest-ranked live edge of every component, with the component at its other end:
-    // (rank << 32 | other) -- ranks are distinct, so the minimum names one edge
-    for (int e0 = tid; e0 < ne; e0 += FSX_WG * U) {
-      int a[U], c[U], k[U];
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        const int e = e0 + u * FSX_WG;
-        const bool in_range = e < ne;
-        a[u] = in_range ? (round ? in.a[e] : E0[e]) : 0;
-        c[u] = in_range ? (round ? in.c[e] : E1[e]) : 0;
-        k[u] = in_range ? (round ? in.k[e] : NP[e]) : INT_MAX;
-      }
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        a[u] = C[a[u]];
-        c[u] = C[c[u]];
-      }
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        const bool live = k[u] != INT_MAX && a[u] != c[u];
-        if (live) {
-          const unsigned long long kr = (unsigned long long)(unsigned)k[u] << 32;
-          atomicMin(BST + a[u], kr | (unsigned)c[u]);
-          atomicMin(BST + c[u], kr | (unsigned)a[u]);
-        }
-        // keep the live edge (as component ids) for the next round: one LDS atomic per wave
-        const unsigned long long m = __ballot(live);
-        const int cnt = __popcll(m);
-        int base = 0;
-        if (lane == 0 && cnt) base = atomicAdd(s_cnt, cnt);
-        base = __shfl(base, 0, 64);
-        if (live) {
-          const int r = base + __popcll(m & ((1ull << lane) - 1ull));
-          out.a[r] = a[u];
-          out.c[r] = c[u];
-          out.k[r] = k[u];
-        }
-      }
-    }
+    bor_scan(round, ne, 0, 1, E0, E1, NP, C, BST, in, out, s_cnt);
     __syncthreads();
     ne = *s_cnt;
-    // every component joins the other end of its edge (the smaller id of a mutual pair stays root)
-    for (int c0 = tid; c0 < nb; c0 += FSX_WG * U) {
-      unsigned long long v[U], w[U];
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        const int cc = c0 + u * FSX_WG;
-        v[u] = (cc < nb && C[cc] == cc) ? BST[cc] : NONE;
-      }
-#pragma unroll
-      for (int u = 0; u < U; u++) w[u] = v[u] != NONE ? BST[(int)(v[u] & 0xffffffffu)] : NONE;
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        const int cc = c0 + u * FSX_WG;
-        if (cc >= nb || C[cc] != cc) continue;
-        if (v[u] == NONE) { BN[cc] = cc; continue; }
-        const int kk = (int)(v[u] >> 32), o = (int)(v[u] & 0xffffffffu);
-        IT[SI[kk]] = 1;
-        BN[cc] = (int)(w[u] >> 32) == kk ? (cc < o ? cc : o) : o;
-        *s_any = 1;
-      }
-    }
-    __syncthreads();
-    if (!*s_any) break;
-    for (;;) {
-      if (tid == 0) *s_chg = 0;
-      __syncthreads();
-      for (int c0 = tid; c0 < nb; c0 += FSX_WG * U) {
-        int x[U], y[U];
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-          const int cc = c0 + u * FSX_WG;
-          x[u] = (cc < nb && C[cc] == cc) ? BN[cc] : -1;
-        }
-#pragma unroll
-        for (int u = 0; u < U; u++) y[u] = x[u] >= 0 ? BN[x[u]] : -1;
-#pragma unroll
-        for (int u = 0; u < U; u++)
-          if (x[u] != y[u]) { BN[c0 + u * FSX_WG] = y[u]; *s_chg = 1; }
-      }
-      __syncthreads();
-      if (!*s_chg) break;
-    }
-    for (int b = tid; b < nb; b += FSX_WG) C[b] = BN[C[b]];
-    __syncthreads();
+    if (!bor_merge(round, nl, SI, C, BST, BN, IT, Lin, Lout, false, s_any, s_chg, s_ncomp)) break;
+    nl = *s_ncomp;
+  }
+}
+
+struct BorArgs {
+  const InstParams* PP;
+  const int *nbasins, *nconn, *cb0, *cb1, *npos, *sid;
+  int* comp;
+  unsigned long long* best;
+  int* bnew;
+  uint8_t* intree;
+  int *ntree, *Lasc, *Rasc, *segs;
+  double *cw, *skey;
+  int *lc0, *lc1, *bstate;
+};
+
+struct BorView {  // one landscape's slices
+  long long eb;
+  const int *E0, *E1, *NP, *SI;
+  int *C, *BN, *LC0, *LC1;
+  unsigned long long* BST;
+  uint8_t* IT;
+  EdgeList LA, LB;
+};
+
+__device__ __forceinline__ BorView bor_view(const BorArgs& A, const InstParams& P) {
+  BorView V;
+  V.eb = (long long)FSX_EDGE_CAP * P.off;
+  V.E0 = A.cb0 + V.eb;
+  V.E1 = A.cb1 + V.eb;
+  V.NP = A.npos + V.eb;
+  V.SI = A.sid + V.eb;
+  V.C = A.comp + P.off;
+  V.BN = A.bnew + P.off;
+  V.LC0 = A.lc0 + P.off;
+  V.LC1 = A.lc1 + P.off;
+  V.BST = A.best + P.off;
+  V.IT = A.intree + V.eb;
+  const long long ecap = (long long)FSX_EDGE_CAP * P.nn;
+  V.LA = {A.Lasc + V.eb, A.Rasc + V.eb, A.segs + V.eb};
+  int* cwi = reinterpret_cast<int*>(A.cw) + 2 * V.eb;
+  V.LB = {cwi, cwi + ecap, reinterpret_cast<int*>(A.skey) + 2 * V.eb};
+  return V;
+}
+
+// set-up: component ids, cleared minima and tree flags, the state of round 0
+__global__ __launch_bounds__(FSX_WG) void k_bor_start(BorArgs A) {
+  const int inst = blockIdx.x;
+  const InstParams& P = A.PP[inst];
+  if (!P.active || !P.run_spl) return;
+  const int nb = A.nbasins[inst], n = A.nconn[inst], tid = threadIdx.x;
+  const BorView V = bor_view(A, P);
+  for (int b = tid; b < nb; b += FSX_WG) {
+    V.C[b] = b;
+    V.BST[b] = ~0ull;
+  }
+  for (int e = tid; e < n; e += FSX_WG) V.IT[e] = 0;
+  if (tid == 0) {
+    int* st = A.bstate + 4 * inst;
+    st[0] = n;
+    st[1] = nb;
+    st[2] = n > 0 ? 0 : 1;
+    st[3] = 0;
+  }
+}
+
+// a split round's edge scan: gridDim.x workgroups per landscape
+__global__ __launch_bounds__(FSX_WG) void k_bor_scan(BorArgs A, int round) {
+  const int inst = blockIdx.y;
+  const InstParams& P = A.PP[inst];
+  if (!P.active || !P.run_spl) return;
+  int* st = A.bstate + 4 * inst;
+  if (st[2]) return;
+  const BorView V = bor_view(A, P);
+  const EdgeList out = (round & 1) ? V.LB : V.LA;
+  const EdgeList in = (round & 1) ? V.LA : V.LB;
+  bor_scan(round, st[0], blockIdx.x, gridDim.x, V.E0, V.E1, V.NP, V.C, V.BST, in, out, st + 3);
+}
+
+// the rest of a split round, one workgroup per landscape
+__global__ __launch_bounds__(FSX_WG) void k_bor_merge(BorArgs A, int round) {
+  __shared__ int s_any, s_chg, s_ncomp;
+  const int inst = blockIdx.x;
+  const InstParams& P = A.PP[inst];
+  if (!P.active || !P.run_spl) return;
+  int* st = A.bstate + 4 * inst;
+  if (st[2]) return;
+  const BorView V = bor_view(A, P);
+  const int ne = st[3], nl = st[1];
+  const int* Lin = (round & 1) ? V.LC1 : V.LC0;
+  int* Lout = (round & 1) ? V.LC0 : V.LC1;
+  __syncthreads();  // every thread has read the state before thread 0 rewrites it
+  const bool more = bor_merge(round, nl, V.SI, V.C, V.BST, V.BN, V.IT, Lin, Lout, true, &s_any, &s_chg, &s_ncomp);
+  if (threadIdx.x == 0) {
+    st[0] = ne;
+    st[3] = 0;
+    if (more) st[1] = s_ncomp; else st[2] = 1;
   }
 }
 
 // Edge lists of the rounds reuse the edge sort's buffers, free once the ranks are known: list A in
-// Lasc/Rasc/segs, list B in the int views of cw (a: first half, c: second half) and skey.
-__global__ __launch_bounds__(FSX_WG) void k_boruvka(const InstParams* __restrict__ PP, const int* __restrict__ nbasins,
-                                                    const int* __restrict__ nconn, const int* __restrict__ cb0,
-                                                    const int* __restrict__ cb1, const int* __restrict__ npos,
-                                                    const int* __restrict__ sid, int* __restrict__ comp,
-                                                    unsigned long long* __restrict__ best, int* __restrict__ bnew,
-                                                    uint8_t* __restrict__ intree, int* __restrict__ ntree,
-                                                    int* __restrict__ Lasc, int* __restrict__ Rasc,
-                                                    int* __restrict__ segs, double* __restrict__ cw,
-                                                    double* __restrict__ skey, int ldscap) {
+// Lasc/Rasc/segs, list B in the int views of cw (a: first half, c: second half) and skey.  Finishes
+// the rounds from BOR_SPLIT on and counts the tree edges.
+__global__ __launch_bounds__(FSX_WG) void k_boruvka(BorArgs A, int ldscap) {
   extern __shared__ int sC[];
-  __shared__ int s_any, s_chg, s_cnt;
+  __shared__ int s_any, s_chg, s_cnt, s_ncomp;
   const int inst = blockIdx.x;
-  const InstParams& P = PP[inst];
+  const InstParams& P = A.PP[inst];
   if (!P.active || !P.run_spl) return;
-  const int nb = nbasins[inst], n = nconn[inst], tid = threadIdx.x;
-  const long long eb = (long long)FSX_EDGE_CAP * P.off;
-  uint8_t* IT = intree + eb;
-  const bool inlds = nb <= ldscap;
-  int* C = inlds ? sC : comp + P.off;
-  for (int b = tid; b < nb; b += FSX_WG) C[b] = b;
-  for (int e = tid; e < n; e += FSX_WG) IT[e] = 0;
-  __syncthreads();
-  const long long ecap = (long long)FSX_EDGE_CAP * P.nn;
-  const EdgeList LA = {Lasc + eb, Rasc + eb, segs + eb};
-  int* cwi = reinterpret_cast<int*>(cw) + 2 * eb;
-  const EdgeList LB = {cwi, cwi + ecap, reinterpret_cast<int*>(skey) + 2 * eb};
-  if (n > 0) {
-    if (inlds)
-      boruvka_body<true>(nb, n, cb0 + eb, cb1 + eb, npos + eb, sid + eb, sC, best + P.off, bnew + P.off, IT, LA, LB,
-                         &s_any, &s_chg, &s_cnt);
-    else
-      boruvka_body<false>(nb, n, cb0 + eb, cb1 + eb, npos + eb, sid + eb, comp + P.off, best + P.off, bnew + P.off, IT,
-                          LA, LB, &s_any, &s_chg, &s_cnt);
+  const int nb = A.nbasins[inst], n = A.nconn[inst], tid = threadIdx.x;
+  const BorView V = bor_view(A, P);
+  const int* st = A.bstate + 4 * inst;
+  if (!st[2]) {
+    const int ne = st[0], nl = st[1];
+    if (nb <= ldscap) {
+      for (int b = tid; b < nb; b += FSX_WG) sC[b] = V.C[b];
+      __syncthreads();
+      boruvka_body<true>(BOR_SPLIT, ne, nl, V.E0, V.E1, V.NP, V.SI, sC, V.BST, V.BN, V.IT, V.LA, V.LB, V.LC0, V.LC1,
+                         &s_any, &s_chg, &s_cnt, &s_ncomp);
+    } else {
+      boruvka_body<false>(BOR_SPLIT, ne, nl, V.E0, V.E1, V.NP, V.SI, V.C, V.BST, V.BN, V.IT, V.LA, V.LB, V.LC0,
+                          V.LC1, &s_any, &s_chg, &s_cnt, &s_ncomp);
+    }
   }
   int cnt = 0;
-  for (int e = tid; e < n; e += FSX_WG) cnt += IT[e];
-  if (cnt) atomicAdd(ntree + inst, cnt);
+  for (int e = tid; e < n; e += FSX_WG) cnt += V.IT[e];
+  if (cnt) atomicAdd(A.ntree + inst, cnt);
 }
 
 int basins_set_lds(int smem_max) {
@@ -1145,9 +1312,15 @@ void launch_lm_edges(DevState& S, hipStream_t st) {
 
 void launch_lm_mst(DevState& S, hipStream_t st) {
   launch_fill_int(S.ntree, S.ninst, 0, st);
-  k_boruvka<<<S.ninst, FSX_WG, (size_t)g_boruvka_lds * sizeof(int), st>>>(
-      S.P, S.nbasins, S.nconn, S.cb0, S.cb1, S.npos, S.sid, S.comp, S.best, S.bnew, S.intree, S.ntree, S.Lasc, S.Rasc,
-      S.segs, S.cw, S.skey, g_boruvka_lds);
+  // bpar / bfront: the live-component lists (bpar is free until the Euler tour)
+  const BorArgs A = {S.P,    S.nbasins, S.nconn, S.cb0,  S.cb1, S.npos, S.sid,  S.comp,  S.best, S.bnew,
+                     S.intree, S.ntree, S.Lasc,  S.Rasc, S.segs, S.cw,  S.skey, S.bpar, S.bfront, S.bstate};
+  k_bor_start<<<S.ninst, FSX_WG, 0, st>>>(A);
+  for (int r = 0; r < BOR_SPLIT; r++) {
+    k_bor_scan<<<dim3(S.bor_gx, S.ninst), FSX_WG, 0, st>>>(A, r);
+    k_bor_merge<<<S.ninst, FSX_WG, 0, st>>>(A, r);
+  }
+  k_boruvka<<<S.ninst, FSX_WG, (size_t)g_boruvka_lds * sizeof(int), st>>>(A, g_boruvka_lds);
 }
 
 void launch_lm_orient(DevState& S, hipStream_t st) {

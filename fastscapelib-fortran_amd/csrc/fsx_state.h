@@ -16,6 +16,7 @@ struct DevState {
   int tour_gx = 16;        // Euler tour kernels: blocks per landscape (x4 for the ranking rounds)
   int nr_small_gx = 64;    // edge sort: blocks per landscape for the segments in LDS
   int area_wg = 1024;      // drainage-area sweep: threads per landscape (256, 512 or 1024)
+  int bor_gx = 4;          // spanning forest: workgroups per landscape for the first rounds' edge scans
 
   // ---- per node (ntot) ----
   double *h = nullptr, *b = nullptr, *u = nullptr, *kf = nullptr, *kd = nullptr, *precip = nullptr;
@@ -41,7 +42,8 @@ struct DevState {
   int *outlet = nullptr, *bsize = nullptr, *soff = nullptr, *ecount = nullptr, *eoff = nullptr;
   int *comp = nullptr, *bpar = nullptr, *bnew = nullptr, *tdeg = nullptr, *toff = nullptr;
   unsigned long long* best = nullptr;  // spanning forest: (rank << 32 | other component) of a component's edge
-  int *tadj = nullptr, *bfront = nullptr;
+  int *tadj = nullptr;
+  int *bfront = nullptr;  // spanning forest: list of the components still roots (with bpar, ping-pong)
   int *trev = nullptr;                   // Euler tour: reverse slot of each directed tree edge (2 per node)
   unsigned long long *tjmp = nullptr;    // Euler tour: (distance to end, next slot), 2 per node
 
@@ -80,4 +82,5 @@ struct DevState {
   unsigned long long *redk = nullptr;  // per-instance order-preserving maxima: maxval(abs(h)), maxval(g)
   int *flags = nullptr;        // convergence flags of iterative kernels (64)
   int *nsegs = nullptr, *segctr = nullptr;  // edge-sort segment lists [ninst]
+  int* bstate = nullptr;       // spanning forest between launches: live edges, components, done, kept [4*ninst]
 };
