@@ -389,24 +389,48 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_edge_fill(const InstParams* __res
                                                          int* __restrict__ cn1, double* __restrict__ cw) {
   const Chunk c = chunks[chunk_id()];
   const InstParams& P = PP[c.inst];
-  const int b = c.start + (int)threadIdx.x;
-  if (!P.active || !P.run_spl || b >= nbasins[c.inst] || nconn[c.inst] == 0) return;
+  const int b = c.start + (int)threadIdx.x, lane = (int)threadIdx.x & 63;
+  // every lane of the wave stays for the cooperative copy below (its shuffles need them all)
+  const bool valid = P.active && P.run_spl && b < nbasins[c.inst] && nconn[c.inst] != 0;
   const long long eb = (long long)FSX_EDGE_CAP * P.off;
-  const int e0 = eoff[P.off + b];
-  if (bc[P.off + outlet[P.off + b]]) {
-    if (b != basin0[c.inst]) {
-      cb0[eb + e0] = basin0[c.inst]; cb1[eb + e0] = b;
-      cn0[eb + e0] = -1; cn1[eb + e0] = -1;
-      cw[eb + e0] = -1.0e10;
+  int ne = 0, e0 = 0;
+  long long sbase = 0;
+  if (valid) {
+    e0 = eoff[P.off + b];
+    if (bc[P.off + outlet[P.off + b]]) {
+      if (b != basin0[c.inst]) {
+        cb0[eb + e0] = basin0[c.inst]; cb1[eb + e0] = b;
+        cn0[eb + e0] = -1; cn1[eb + e0] = -1;
+        cw[eb + e0] = -1.0e10;
+      }
+    } else {
+      ne = ecount[P.off + b];
+      sbase = (long long)FSX_SLAB_CAP * (P.off + soff[P.off + b]);
     }
-    return;
   }
-  const long long sbase = (long long)FSX_SLAB_CAP * (P.off + soff[P.off + b]);
-  const int ne = ecount[P.off + b];
-  for (int k = 0; k < ne; k++) {
-    cb0[eb + e0 + k] = b; cb1[eb + e0 + k] = sB[sbase + k];
-    cn0[eb + e0 + k] = sp[sbase + k]; cn1[eb + e0 + k] = sq[sbase + k];
-    cw[eb + e0 + k] = sw[sbase + k];
+  // The wave's closed basins' passes, copied from their slabs one edge per lane: the wave's edges are
+  // consecutive in the output (basin order), so the stores are coalesced.  Edge t of the wave belongs
+  // to the last lane whose exclusive prefix of counts is <= t.
+  const int incl = wave_incl_scan(ne);
+  const int total = __shfl(incl, 63, 64), excl = incl - ne;
+  const int sb_lo = (int)(sbase & 0xffffffffll), sb_hi = (int)(sbase >> 32);
+  for (int t0 = 0; t0 < total; t0 += 64) {
+    const int t = t0 + lane;
+    int o = 0;
+#pragma unroll
+    for (int step = 32; step >= 1; step >>= 1)
+      if (__shfl(excl, o + step, 64) <= t) o += step;
+    const int k = t - __shfl(excl, o, 64);
+    const int oe0 = __shfl(e0, o, 64);
+    const long long osb = ((long long)__shfl(sb_hi, o, 64) << 32) | (unsigned)__shfl(sb_lo, o, 64);
+    if (t < total) {
+      const long long d = eb + oe0 + k, q = osb + k;
+      cb0[d] = c.start + (int)(threadIdx.x & ~63u) + o;  // the owner lane's basin
+      cb1[d] = sB[q];
+      cn0[d] = sp[q];
+      cn1[d] = sq[q];
+      cw[d] = sw[q];
+    }
   }
 }
 
