@@ -44,7 +44,7 @@ METRIC = "cell-steps/s batched+served vs plain on 1 GPU; p50/p99 run latency"
 ALG_BYTES_PER_NODE = {
     "uplift": 40,        # R h,u,b  W h,b
     "receivers": 20,     # R h  W rec(i32), length
-    "donors": 13,        # R rec  W dmask(u8), first, next
+    "donors": 5,         # R rec  W dmask(u8)
     "lm_walk": 17,       # R (h, basin, preorder successor) record, candidate mask (closed-basin nodes ~ all)
     "area_prep": 65,     # R J, dmask, pos, donor pos, precip  W record (a0, 8 donor slots)
     "area_sweep": 48,    # R record  W Al

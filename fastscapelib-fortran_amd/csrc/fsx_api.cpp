@@ -318,8 +318,8 @@ static int ctx_alloc(long long capacity, int nslots, fsx_ctx** out) {
   A(sh, FSX_SLAB_CAP * N);
   A(nbasins, ninst); A(basin0, ninst); A(nconn, ninst); A(nlev, ninst); A(err, ninst); A(ngs, ninst);
   A(ntree, ninst); A(chunkcnt, c->chunk_cap); A(newton, ninst); A(scal, ninst); A(redk, 2 * ninst); A(flags, 64);
-  A(nsegs, ninst); A(segctr, ninst); A(maxd, ninst); A(bstate, 4 * ninst);
-  A(J, N); A(ngroups, ninst); A(grec, 2 * (N + ninst)); A(first, N); A(next, N); A(gmeta, N + ninst); A(pos, N); A(lcur, N + ninst); A(rp, N); A(LSl, N); A(dn, 8 * N);
+  A(nsegs, ninst); A(segctr, ninst); A(maxd, ninst); A(bstate, 4 * ninst); A(nrst, 16 * ninst);
+  A(J, N); A(ngroups, ninst); A(grec, 2 * (N + ninst)); A(gmeta, N + ninst); A(pos, N); A(lcur, N + ninst); A(rp, N); A(LSl, N); A(dn, 8 * N);
   A(A0, N); A(Al, N); A(F, N); A(HTl, N); A(Hl, N); A(Wl, N); A(HWl, N); A(HPl, N); A(DHl, N); A(ELl, N);
   A(arec, N); A(srec, N); A(wnode, N); A(wmask, N);
 #undef A
@@ -617,7 +617,7 @@ static int check_errors(fsx_ctx* c) {
 static void enqueue_routing(fsx_ctx* c, hipStream_t st) {
   DevState& S = c->S;
   PHASE(PH_RECEIVERS, launch_receivers(S, st));
-  PHASE(PH_DONORS, launch_donors(S, st, 1));
+  PHASE(PH_DONORS, launch_donors(S, st));
   // LocalMinima, src/LocalMinima.f90:1-132
   PHASE(PH_LM_LABELS, launch_lm_labels(S, st));
   PHASE(PH_LM_BASINS, launch_lm_basins(S, st));
@@ -627,7 +627,7 @@ static void enqueue_routing(fsx_ctx* c, hipStream_t st) {
   PHASE(PH_LM_MST, launch_lm_mst(S, st));
   PHASE(PH_LM_ORIENT, launch_lm_orient(S, st));
   PHASE(PH_LM_CORRECT, launch_lm_correct(S, st));
-  PHASE(PH_DONORS, launch_donors(S, st, 0));
+  PHASE(PH_DONORS, launch_donors(S, st));
   PHASE(PH_LEVELS, launch_levels(S, st));
   if (c->any_sfd) {
     PHASE(PH_AREA_PREP, launch_area_prep(S, st));

@@ -91,14 +91,39 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_root_rank(const InstParams* __res
     const int y = __shfl_xor(open_rank, o, 64);
     open_rank = y < open_rank ? y : open_rank;
   }
-  if ((threadIdx.x & 63) == 0 && open_rank != INT_MAX) atomicMin(basin0 + c.inst, open_rank);
+  // most waves holding an open root see a smaller one already stored (ranks follow node order and
+  // the first open root comes early), so they skip the atomic on the one contended word
+  if ((threadIdx.x & 63) == 0 && open_rank != INT_MAX && open_rank < __atomic_load_n(basin0 + c.inst, __ATOMIC_RELAXED))
+    atomicMin(basin0 + c.inst, open_rank);
+}
+
+// The smallest donor of r (donor mask m) larger than u, or -1.  Interior nodes: the mask's bit order
+// is ascending index order, so the set bits are visited in order without decoding all donors.
+__device__ __forceinline__ int donor_after(const InstParams& P, int r, unsigned m, int u) {
+  if (!m) return -1;
+  const int i = r % P.nx, j = r / P.nx;
+  if (i > 0 && j > 0 && i < P.nx - 1 && j < P.ny - 1) {
+    while (m) {
+      const int bit = __ffs(m) - 1;
+      m &= m - 1;
+      const int dd = bit < 4 ? bit : bit + 1;
+      const int q = r + (dd % 3 - 1) + (dd / 3 - 1) * P.nx;
+      if (q > u) return q;
+    }
+    return -1;
+  }
+  int d[8];
+  const int nd = donors_sorted(P, r, m, d);
+  for (int t = 0; t < nd; t++)
+    if (d[t] > u) return d[t];
+  return -1;
 }
 
 __global__ __launch_bounds__(FSX_CHUNK) void k_basin_label(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
                                                            const unsigned long long* __restrict__ J,
                                                            const int* __restrict__ rootrank, const uint8_t* __restrict__ bc,
-                                                           const double* __restrict__ h, const int* __restrict__ first,
-                                                           const int* __restrict__ rec, const int* __restrict__ next,
+                                                           const double* __restrict__ h, const uint8_t* __restrict__ dmask,
+                                                           const int* __restrict__ rec,
                                                            int* __restrict__ basin, int* __restrict__ bsize,
                                                            WalkNode* __restrict__ wnode) {
   NODE_PROLOGUE
@@ -109,16 +134,18 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_basin_label(const InstParams* __r
     bv = !bc[P.off + r] ? b : -(b + 1);  // signed: open basins negative
     basin[P.off + v] = bv;
     // successor of v in the depth-first preorder of its tree (the order of src/FlowRouting.f90
-    // find_stack): the first donor, else the next sibling of the nearest ancestor-or-self that has
-    // one; -1 when the traversal is back at the root.  Each receiver edge is climbed by one leaf
-    // only (the last of its subtree), so the climbs cost O(nodes) in total.
-    int sv = first[P.off + v];
+    // find_stack, donors ascending): the first donor, else the next larger sibling of the nearest
+    // ancestor-or-self that has one; -1 when the traversal is back at the root.  Donors come from
+    // the donor masks.  Each receiver edge is climbed by one leaf only (the last of its subtree), so
+    // the climbs cost O(nodes) in total.
+    const uint8_t* DM = dmask + P.off;
+    int sv = donor_after(P, v, DM[v], -1);
     if (sv < 0) {
       int u = v;
       for (int guard = 0; guard < P.nn; guard++) {  // the receivers form a forest; bounded anyway
         const int ru = rec[P.off + u];
         if (ru == u) break;
-        const int nu = next[P.off + u];
+        const int nu = donor_after(P, ru, DM[ru], u);
         if (nu >= 0) {
           sv = nu;
           break;
@@ -570,10 +597,97 @@ __device__ __forceinline__ void nr_partition(double* K, int* I, int* LA, int* RA
 }
 
 // Large segments, one workgroup per landscape; segments of <= NR_BIG keys go to a list.
+// Ranges of more than NR_BIG keys are partitioned level by level: launch r partitions every big
+// range of level r, DevState::nr_gx workgroups per landscape taking ranges in turn (each range by one
+// workgroup; ranges are disjoint, so the order of processing does not matter), and appends the
+// children: big ones to level r+1's list, the others to the segment list of k_nr_small.  Level lists
+// alternate between two halves of the int view of cw (free once k_nr_init has copied the keys);
+// nrst[NR_ST * inst + r] counts level r's ranges, nrst[NR_ST * inst + NR_ST - 1] the segments.  Ranges
+// still big after NR_LEVELS levels are finished by k_nr_big with a stack in LDS.
+#define NR_LEVELS 12
+#define NR_ST 16
+
+__device__ __forceinline__ int* nr_list(double* cw, long long eb, int n, int level) {
+  return reinterpret_cast<int*>(cw) + 2 * eb + ((level & 1) ? n + 2 : 0);
+}
+
+__global__ __launch_bounds__(FSX_WG) void k_nr_init(const InstParams* __restrict__ PP, const int* __restrict__ nconn,
+                                                    double* __restrict__ cw, double* __restrict__ skey,
+                                                    int* __restrict__ sid, int* __restrict__ segs,
+                                                    int* __restrict__ nrst) {
+  const int inst = blockIdx.x;
+  const InstParams& P = PP[inst];
+  if (!P.active || !P.run_spl) return;
+  const int n = nconn[inst];
+  const long long eb = (long long)FSX_EDGE_CAP * P.off;
+  for (int p = threadIdx.x; p < n; p += FSX_WG) {
+    skey[eb + p] = cw[eb + p];
+    sid[eb + p] = p;
+  }
+  __syncthreads();  // every key copied before the level lists overwrite cw
+  if (threadIdx.x == 0) {
+    int* st = nrst + NR_ST * inst;
+    for (int r = 0; r < NR_ST; r++) st[r] = 0;
+    if (n > NR_BIG) {
+      int* L0 = nr_list(cw, eb, n, 0);
+      L0[0] = 0;
+      L0[1] = n - 1;
+      st[0] = 1;
+    } else if (n >= 2) {
+      segs[eb] = 0;
+      segs[eb + 1] = n - 1;
+      st[NR_ST - 1] = 1;
+    }
+  }
+}
+
+__global__ __launch_bounds__(FSX_WG) void k_nr_level(const InstParams* __restrict__ PP, const int* __restrict__ nconn,
+                                                     double* __restrict__ cw, double* __restrict__ skey,
+                                                     int* __restrict__ sid, int* __restrict__ Lasc,
+                                                     int* __restrict__ Rasc, int* __restrict__ segs,
+                                                     int* __restrict__ nrst, int level) {
+  __shared__ int lds[FSX_WG / 64];
+  __shared__ int sh[4];
+  __shared__ double shd[1];
+  const int inst = blockIdx.y;
+  const InstParams& P = PP[inst];
+  if (!P.active || !P.run_spl) return;
+  int* st = nrst + NR_ST * inst;
+  const int nr = st[level];
+  if ((int)blockIdx.x >= nr) return;
+  const int n = nconn[inst];
+  const long long eb = (long long)FSX_EDGE_CAP * P.off;
+  const int* Lc = nr_list(cw, eb, n, level);
+  int* Ln = nr_list(cw, eb, n, level + 1);
+  for (int t = blockIdx.x; t < nr; t += gridDim.x) {
+    const int l = Lc[2 * t], ir = Lc[2 * t + 1];
+    int is, js;
+    nr_partition<FSX_WG>(skey + eb, sid + eb, Lasc + eb, Rasc + eb, l, ir, lds, sh, shd, is, js);
+    if (threadIdx.x == 0) {
+      const int segl[2][2] = {{l, js - 1}, {is, ir}};
+      for (int c = 0; c < 2; c++) {
+        const int sl = segl[c][0], sr = segl[c][1], sz = sr - sl + 1;
+        if (sz > NR_BIG) {
+          const int k = atomicAdd(st + level + 1, 1);
+          Ln[2 * k] = sl;
+          Ln[2 * k + 1] = sr;
+        } else if (sz >= 2) {
+          const int k = atomicAdd(st + NR_ST - 1, 1);
+          segs[eb + 2 * k] = sl;
+          segs[eb + 2 * k + 1] = sr;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// big ranges left after NR_LEVELS levels (none for keys of ordinary spread): one workgroup, stack in LDS
 __global__ __launch_bounds__(FSX_WG) void k_nr_big(const InstParams* __restrict__ PP, const int* __restrict__ nconn,
-                                                   const double* __restrict__ cw, double* __restrict__ skey,
+                                                   double* __restrict__ cw, double* __restrict__ skey,
                                                    int* __restrict__ sid, int* __restrict__ Lasc, int* __restrict__ Rasc,
-                                                   int* __restrict__ segs, int* __restrict__ nsegs, int* __restrict__ err) {
+                                                   int* __restrict__ segs, int* __restrict__ nrst, int* __restrict__ nsegs,
+                                                   int* __restrict__ err) {
   __shared__ int lds[FSX_WG / 64];
   __shared__ int sh[4];
   __shared__ double shd[1];
@@ -584,17 +698,21 @@ __global__ __launch_bounds__(FSX_WG) void k_nr_big(const InstParams* __restrict_
   if (!P.active || !P.run_spl) return;
   const int n = nconn[inst];
   const long long eb = (long long)FSX_EDGE_CAP * P.off;
+  int* st = nrst + NR_ST * inst;
   double* K = skey + eb;
   int* I = sid + eb;
   int* LA = Lasc + eb;
   int* RA = Rasc + eb;
   int* SG = segs + eb;
-  for (int p = threadIdx.x; p < n; p += FSX_WG) { K[p] = cw[eb + p]; I[p] = p; }
   if (threadIdx.x == 0) {
+    const int left = st[NR_LEVELS];
+    const int* L = nr_list(cw, eb, n, NR_LEVELS);
     s_nb = 0;
-    s_ns = 0;
-    if (n > NR_BIG) { bst[0] = 0; bst[1] = n - 1; s_nb = 1; }
-    else if (n >= 2) { SG[0] = 0; SG[1] = n - 1; s_ns = 1; }
+    s_ns = st[NR_ST - 1];
+    for (int t = 0; t < left; t++) {
+      if (s_nb < 128) { bst[2 * s_nb] = L[2 * t]; bst[2 * s_nb + 1] = L[2 * t + 1]; s_nb++; }
+      else atomicOr(err + inst, FSX_ERR_SORTSTACK);
+    }
   }
   __syncthreads();
   while (s_nb > 0) {
@@ -1301,7 +1419,12 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_correct(const InstParams* __restr
 // ---------------------------------------------------------------------------------------------------
 void launch_nr_sort(DevState& S, hipStream_t st) {
   launch_fill_int(S.segctr, S.ninst, 0, st);
-  k_nr_big<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.nconn, S.cw, S.skey, S.sid, S.Lasc, S.Rasc, S.segs, S.nsegs, S.err);
+  k_nr_init<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.nconn, S.cw, S.skey, S.sid, S.segs, S.nrst);
+  for (int r = 0; r < NR_LEVELS; r++)
+    k_nr_level<<<dim3(std::min(S.nr_gx, 1 << r), S.ninst), FSX_WG, 0, st>>>(S.P, S.nconn, S.cw, S.skey, S.sid, S.Lasc,
+                                                                              S.Rasc, S.segs, S.nrst, r);
+  k_nr_big<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.nconn, S.cw, S.skey, S.sid, S.Lasc, S.Rasc, S.segs, S.nrst, S.nsegs,
+                                       S.err);
   k_nr_small<<<dim3(S.nr_small_gx, S.ninst), NR_SMALL_WG, 0, st>>>(S.P, S.nsegs, S.segctr, S.skey, S.sid, S.segs, S.err);
   k_nr_pos<<<dim3(16, S.ninst), FSX_CHUNK, 0, st>>>(S.P, S.nconn, S.sid, S.npos);
 }
@@ -1316,7 +1439,7 @@ void launch_lm_basins(DevState& S, hipStream_t st) {
   k_root_scan<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.chunkcnt, S.nbasins, S.basin0);
   k_root_rank<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.rec, S.bc, S.chunkcnt, S.rootrank, S.outlet, S.basin0);
   launch_fill_int(S.bsize, S.ntot, 0, st);
-  k_basin_label<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.J, S.rootrank, S.bc, S.h, S.first, S.rec, S.next, S.basin,
+  k_basin_label<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.J, S.rootrank, S.bc, S.h, S.dmask, S.rec, S.basin,
                                           S.bsize, S.wnode);
   k_bsize_scan<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.nbasins, S.bsize, S.soff, S.ecount);
   k_walk_mask<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.basin, S.wmask);

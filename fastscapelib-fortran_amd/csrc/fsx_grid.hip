@@ -63,33 +63,17 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_receivers(const InstParams* __res
 // which lets a depth-first preorder walk advance with one load per step.  Every thread writes only
 // its own node: next[v] is gathered from the receivers around rec(v).
 __global__ __launch_bounds__(FSX_CHUNK) void k_donors(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
-                                                      const int* __restrict__ rec, uint8_t* __restrict__ dmask,
-                                                      int* __restrict__ first, int* __restrict__ next, int thread) {
+                                                      const int* __restrict__ rec, uint8_t* __restrict__ dmask) {
   NODE_KERNEL_PROLOGUE
   if (!P.run_spl) return;
   const int* ri = rec + P.off;
   int q[8], dir[8];
   const int c8 = neighbours_wc(P, v, q, dir);
   unsigned m = 0;
-  int f = INT_MAX;
 #pragma unroll
   for (int t = 0; t < 8; t++)
-    if (t < c8 && ri[q[t]] == v) {
-      m |= 1u << dbit(dir[t]);
-      f = q[t] < f ? q[t] : f;
-    }
+    if (t < c8 && ri[q[t]] == v) m |= 1u << dbit(dir[t]);
   dmask[g_] = (uint8_t)m;
-  if (!thread) return;  // the mask alone (the walk's threading is only needed before LocalMinima)
-  first[g_] = m ? f : -1;
-  const int r = ri[v];
-  int nxt = INT_MAX;
-  if (r != v) {
-    const int cr = neighbours_wc(P, r, q, dir);
-#pragma unroll
-    for (int t = 0; t < 8; t++)
-      if (t < cr && q[t] > v && q[t] < nxt && ri[q[t]] == r) nxt = q[t];
-  }
-  next[g_] = nxt == INT_MAX ? -1 : nxt;
 }
 
 // Stream-power set-up, src/StreamPowerLaw.f90:236-250: g, kfint, ht.
@@ -472,8 +456,8 @@ void launch_uplift(DevState& S, hipStream_t st) {
 void launch_receivers(DevState& S, hipStream_t st) {
   k_receivers<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.h, S.rec, S.length);
 }
-void launch_donors(DevState& S, hipStream_t st, int thread) {
-  k_donors<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.rec, S.dmask, S.first, S.next, thread);
+void launch_donors(DevState& S, hipStream_t st) {
+  k_donors<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.rec, S.dmask);
 }
 void launch_spl_prep(DevState& S, hipStream_t st) {
   k_spl_prep<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.h, S.b, S.kf, S.kfint, S.g, S.ht);

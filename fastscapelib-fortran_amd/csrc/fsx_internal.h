@@ -203,7 +203,7 @@ __device__ __forceinline__ double dval(unsigned long long k) {
 struct DevState;
 void launch_uplift(DevState& S, hipStream_t st);
 void launch_receivers(DevState& S, hipStream_t st);
-void launch_donors(DevState& S, hipStream_t st, int thread);  // thread: also first/next
+void launch_donors(DevState& S, hipStream_t st);  // donor mask of every node
 void launch_lm_labels(DevState& S, hipStream_t st);
 void launch_lm_basins(DevState& S, hipStream_t st);
 void launch_lm_walk(DevState& S, hipStream_t st);

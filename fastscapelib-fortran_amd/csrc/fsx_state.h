@@ -17,6 +17,7 @@ struct DevState {
   int nr_small_gx = 64;    // edge sort: blocks per landscape for the segments in LDS
   int area_wg = 1024;      // drainage-area sweep: threads per landscape (256, 512 or 1024)
   int bor_gx = 4;          // spanning forest: workgroups per landscape for the first rounds' edge scans
+  int nr_gx = 8;           // edge sort: workgroups per landscape for the big ranges of a level
 
   // ---- per node (ntot) ----
   double *h = nullptr, *b = nullptr, *u = nullptr, *kf = nullptr, *kd = nullptr, *precip = nullptr;
@@ -74,7 +75,6 @@ struct DevState {
   int *ngs = nullptr, *ntree = nullptr, *chunkcnt = nullptr, *maxd = nullptr, *ngroups = nullptr;
   int *gmeta = nullptr;
   int4* grec = nullptr;  // per-group records of the sweeps, 2 per group
-  int *first = nullptr, *next = nullptr;  // smallest donor / next larger sibling (-1: none)
   int2* closed_list = nullptr;  // (instance, basin) of every closed basin
   int* nclosed = nullptr;  // level groups of the sweeps (node offsets + instance, nn+1 per landscape)
   long long *newton = nullptr;
@@ -83,4 +83,5 @@ struct DevState {
   int *flags = nullptr;        // convergence flags of iterative kernels (64)
   int *nsegs = nullptr, *segctr = nullptr;  // edge-sort segment lists [ninst]
   int* bstate = nullptr;       // spanning forest between launches: live edges, components, done, kept [4*ninst]
+  int* nrst = nullptr;         // edge sort between launches: big ranges per level, segments [16*ninst]
 };
