@@ -175,8 +175,15 @@ TUNE = True    # run the launch-shape autotuner after the warm-up steps of a bat
 SHAPES = None  # or the headline batch's launch shapes, given (walk_gx, jump_blocks, tour_gx, nr_small_gx)
 
 
-def run_gpu(fs, Ls, steps, warmup, profile, tune=False):
-    with fs.Batch([(L["nx"], L["ny"]) for L in Ls]) as b:
+def open_batch(fs, Ls, lanes=1):
+    """One context for all landscapes, or (lanes > 1) fs.Lanes: several contexts stepping on their own
+    streams, so one lane's per-landscape kernels overlap another's node-parallel ones."""
+    sizes = [(L["nx"], L["ny"]) for L in Ls]
+    return fs.Lanes(sizes, lanes) if lanes > 1 else fs.Batch(sizes)
+
+
+def run_gpu(fs, Ls, steps, warmup, profile, tune=False, lanes=1):
+    with open_batch(fs, Ls, lanes) as b:
         b.graphs(GRAPHS)
         for k, L in enumerate(Ls):
             configure(b, L, k)
@@ -198,12 +205,15 @@ def run_gpu(fs, Ls, steps, warmup, profile, tune=False):
     return dt, timers, stats, mem
 
 
-def run_headline(fs, Ls, steps, warmup, prof_steps):
+def run_headline(fs, Ls, steps, warmup, prof_steps, lanes=1):
     """The headline measurement.  A profiled window (every phase bracketed by HIP events, kernels
     launched one by one) gives the per-phase breakdown and names the dominant single-kernel phase;
     the timed region then replays each step as captured HIP graphs, with that one phase launched
-    between its own HIP events on the step's stream, so its mean duration is measured live."""
-    with fs.Batch([(L["nx"], L["ny"]) for L in Ls]) as b:
+    between its own HIP events on the step's stream, so its mean duration is measured live.  With
+    lanes, the timers are lane 0's (its landscapes: ``sel``) while the other lanes run beside it."""
+    with open_batch(fs, Ls, lanes) as b:
+        sel = b.members[0] if lanes > 1 else list(range(len(Ls)))
+        lane_timers = (lambda: b.timers(lane=0)) if lanes > 1 else b.timers
         b.graphs(GRAPHS)
         for k, L in enumerate(Ls):
             configure(b, L, k)
@@ -212,13 +222,14 @@ def run_headline(fs, Ls, steps, warmup, prof_steps):
         if SHAPES:
             b.set_tuning(dict(zip(b.TUNING, SHAPES)))
         tuned = b.autotune() if TUNE else b.tuning()
+        tuned = tuned if isinstance(tuned, list) else [tuned]
         b.profiling(True)
         b.reset_timers()
         b.step(prof_steps)
-        prof = b.timers()
+        prof = lane_timers()
         b.profiling(False)
-        stats = [b.stats(k) for k in range(len(Ls))]
-        dom = dominant_phase(prof, stats, sum(L["nx"] * L["ny"] for L in Ls))
+        stats = [b.stats(k) for k in sel]
+        dom = dominant_phase(prof, stats, sum(Ls[k]["nx"] * Ls[k]["ny"] for k in sel))
         b.time_phase(dom)
         b.step(1)  # captures the graphs
         b.reset_timers()
@@ -227,11 +238,11 @@ def run_headline(fs, Ls, steps, warmup, prof_steps):
         b.step(steps)
         _sync()
         dt = time.perf_counter() - t0
-        live = b.timers()
+        live = lane_timers()
         b.time_phase(None)
         stats = [b.stats(k) for k in range(len(Ls))]
         mem = b.memory()
-    return dt, prof, live, dom, stats, mem, tuned
+    return dt, prof, live, dom, stats, mem, tuned, sel
 
 
 def plain_rate(fs, Ls, steps):
@@ -244,15 +255,15 @@ def plain_rate(fs, Ls, steps):
     return cells / secs
 
 
-def ragged_leg(fs, steps, sample):
+def ragged_leg(fs, steps, sample, lanes=1):
     Ls = ragged_batch()
     ncells = sum(L["nx"] * L["ny"] for L in Ls)
-    dt, _, stats, mem = run_gpu(fs, Ls, steps, 1, profile=False, tune=True)
+    dt, _, stats, mem = run_gpu(fs, Ls, steps, 1, profile=False, tune=True, lanes=lanes)
     batched = ncells * steps / dt
     pick = Ls[::max(1, len(Ls) // sample)][:sample]
     plain = plain_rate(fs, pick, max(2, steps // 2))
     return {"workload": "ragged batch 200 x log-uniform[65,1025]^2, n in {1,1.5,2}, Newton tol 1e-8, BC 1010/1111",
-            "landscapes": len(Ls), "cells": ncells, "steps": steps, "ms_per_step": dt / steps * 1e3,
+            "landscapes": len(Ls), "cells": ncells, "lanes": lanes, "steps": steps, "ms_per_step": dt / steps * 1e3,
             "value": batched, "plain": plain, "plain_sample": f"{len(pick)} landscapes of the batch, one at a time",
             "batched_over_plain": batched / plain, "hbm_bytes_resident": mem[0],
             "newton_iters_per_step": float(np.sum([s["newton_iters"] for s in stats]))}
@@ -344,9 +355,9 @@ def dominant_phase(timers, stats, ncells):
     return best[0]
 
 
-def measured_traffic(ph, workload):
+def measured_traffic(ph, workload, lanes=1):
     """HBM bytes per launch of phase ph's kernel from the committed counter summary of this workload
-    (FETCH_SIZE and WRITE_SIZE runs scaled by a copy of known size), or None."""
+    and lane count (FETCH_SIZE and WRITE_SIZE runs scaled by a copy of known size), or None."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic_uniform_batch_64x513.json")
     try:
         with open(path) as f:
@@ -354,7 +365,7 @@ def measured_traffic(ph, workload):
     except (OSError, ValueError):
         return None
     k = d.get("kernels", {}).get(ph)
-    if d.get("workload") != workload or not k:
+    if d.get("workload") != workload or d.get("lanes", 1) != lanes or not k:
         return None
     return int(k["fetch_bytes"] + k["write_bytes"])
 
@@ -405,6 +416,8 @@ def main():
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every kernel directly instead of replaying HIP graphs (for kernel-trace profiling)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (no autotuner)")
+    ap.add_argument("--lanes", type=int, default=3,
+                    help="contexts stepping the batch side by side on their own streams (1: one context)")
     ap.add_argument("--shapes", default=None,
                     help="walk_gx,jump_blocks,tour_gx,nr_small_gx for the headline batch instead of the autotuner")
     args = ap.parse_args()
@@ -427,8 +440,10 @@ def main():
     Ls = uniform_batch(args.ninst, args.n)
     workload = f"uniform batch {args.ninst} x {args.n}x{args.n} D8 n=1 g=0 kd=1e-2 fixed BC"
     ncells = sum(L["nx"] * L["ny"] for L in Ls)
-    dt, timers, live, dom, stats, mem, tuned = run_headline(fs, Ls, args.steps, args.warmup, args.prof_steps)
+    dt, timers, live, dom, stats, mem, tuned, sel = run_headline(fs, Ls, args.steps, args.warmup, args.prof_steps,
+                                                                 args.lanes)
     value = ncells * args.steps / dt
+    sel_cells = sum(Ls[k]["nx"] * Ls[k]["ny"] for k in sel)
 
     # plain path: one landscape at a time (the first of the batch)
     plain, pdt = None, None
@@ -442,21 +457,26 @@ def main():
         "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded uniform-noise landscapes)",
         "config": {"workload": workload,
-                   "landscapes": args.ninst, "cells": ncells},
+                   "landscapes": args.ninst, "cells": ncells, "lanes": args.lanes},
         "plain": ({"value": plain, "ms_per_step": pdt / args.plain_steps * 1e3,
                    "workload": f"1 x {args.n}x{args.n}, same parameters"} if plain else None),
         "batched_over_plain": value / plain if plain else None,
-        "roofline": dict(roofline(dom, live, stats, ncells), traffic=measured_traffic(dom, workload)),
-        "launch_shapes": dict(tuned, chosen_by="autotuner" if TUNE else ("given" if SHAPES else "defaults")),
+        "roofline": dict(roofline(dom, live, [stats[k] for k in sel], sel_cells),
+                         traffic=measured_traffic(dom, workload, args.lanes),
+                         measured_on=f"lane 0 of {args.lanes}: {len(sel)} landscapes" if args.lanes > 1 else "all"),
+        "launch_shapes": {"per_lane": tuned,
+                          "chosen_by": "autotuner" if TUNE else ("given" if SHAPES else "defaults")},
         "phase_ms_per_step": phases,
-        "phase_ms_per_step_note": f"profiled window of {args.prof_steps} steps, kernels launched one by one",
+        "phase_ms_per_step_note": f"profiled window of {args.prof_steps} steps, kernels launched one by one"
+                                  + (f"; lane 0 of {args.lanes} ({len(sel)} landscapes), the others beside it"
+                                     if args.lanes > 1 else ""),
         "step_model_frac": round(336.0 * ncells / (dt / args.steps) / 1e9 / HBM_PEAK_GBS, 5),
         "hbm_bytes_resident": mem[0],
         "basins_per_landscape": int(np.mean([s["nbasins"] for s in stats])),
         "levels_per_landscape": int(np.mean([s["levels"] for s in stats])),
     }
     if "ragged" in legs:
-        out["ragged"] = ragged_leg(fs, args.ragged_steps, sample=8)
+        out["ragged"] = ragged_leg(fs, args.ragged_steps, sample=8, lanes=args.lanes)
     if "serving" in legs:
         out["serving"] = serving_leg(fs, args.quantum, args.serve_seconds, sample=8)
     if not args.no_cpu:

@@ -33,7 +33,7 @@ __all__ = [
     "fastscape_copy_erosion_rate", "fastscape_copy_drainage_area", "fastscape_copy_lake_depth",
     "fastscape_get_step", "fastscape_get_sizes", "fastscape_destroy", "fastscape_set_h", "fastscape_set_basement",
     "fastscape_set_precip", "fastscape_set_all_layers", "fastscape_reset_cumulative_erosion", "fastscape_view",
-    "fastscape_debug", "Batch", "Pool", "FastscapeError", "indexx",
+    "fastscape_debug", "Batch", "Lanes", "Pool", "FastscapeError", "indexx",
 ]
 
 
@@ -232,6 +232,114 @@ class Batch:
         a, b = C.c_longlong(), C.c_longlong()
         check(lib.fsx_memory(self._h, C.byref(a), C.byref(b)), "memory")
         return a.value, b.value
+
+
+class Lanes:
+    """Landscapes split over ``lanes`` Batch contexts, each stepping on its own HIP stream.
+
+    Per-landscape kernels (level sweeps, edge sort, basin walk, spanning forest) run one workgroup
+    per landscape, so a batch of a few dozen landscapes leaves most compute units idle while they
+    run.  ``step`` enqueues every lane's steps before waiting for any, so one lane's per-landscape
+    kernels overlap the other lanes' node-parallel ones.  Landscapes are dealt to lanes by size
+    (largest first, to the lane with the fewest cells); results do not depend on the split.  The
+    interface is Batch's: per-landscape calls take the landscape id of the whole set.
+    """
+
+    _PER = {"set_xl_yl", "set_dt", "set_bc", "set_newton_tol", "set_active", "set_erosional_parameters",
+            "init_h", "set_field", "set_u", "set_h", "set_basement", "set_precip", "set_all_layers",
+            "reset_cumulative_erosion", "field", "ifield", "h", "rec", "donors", "order", "mrec", "morder",
+            "get_step", "stats"}
+    TUNING = Batch.TUNING
+
+    def __init__(self, sizes, lanes=2):
+        sizes = [(int(a), int(b)) for a, b in sizes]
+        if not sizes:
+            raise ValueError("Lanes needs at least one landscape")
+        nl = max(1, min(int(lanes), len(sizes)))
+        load, members = [0] * nl, [[] for _ in range(nl)]
+        for k in sorted(range(len(sizes)), key=lambda k: (-sizes[k][0] * sizes[k][1], k)):
+            j = min(range(nl), key=lambda j: (load[j], j))
+            members[j].append(k)
+            load[j] += sizes[k][0] * sizes[k][1]
+        self.sizes = dict(enumerate(sizes))
+        self.nn = {k: a * b for k, (a, b) in self.sizes.items()}
+        self.members = [sorted(m) for m in members]
+        self.where = {k: (j, i) for j, m in enumerate(self.members) for i, k in enumerate(m)}
+        self.lanes = []
+        try:
+            for m in self.members:
+                self.lanes.append(Batch([sizes[k] for k in m]))
+        except Exception:
+            self.close()
+            raise
+
+    def __getattr__(self, name):
+        if name in Lanes._PER:
+            def per(k, *a, **kw):
+                j, i = self.where[k]
+                return getattr(self.lanes[j], name)(i, *a, **kw)
+            return per
+        raise AttributeError(name)
+
+    def __len__(self):
+        return len(self.sizes)
+
+    def close(self):
+        for b in getattr(self, "lanes", []):
+            b.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def step(self, nsteps=1, sync=True):
+        # one step per lane in turn: a lane whose host side waits (its phase-timer ring filling up)
+        # still has the other lanes' steps queued on the device
+        for _ in range(int(nsteps)):
+            for b in self.lanes:
+                b.step(1, sync=False)
+        if sync:
+            for b in self.lanes:
+                b.step(0, sync=True)  # waits for the lane's stream and reports its errors
+
+    def synchronize(self):
+        for b in self.lanes:
+            b.step(0, sync=True)
+
+    def _all(self, name, *a):
+        return [getattr(b, name)(*a) for b in self.lanes]
+
+    def graphs(self, on=True): self._all("graphs", on)
+    def profiling(self, on=True): self._all("profiling", on)
+    def reset_timers(self): self._all("reset_timers")
+    def time_phase(self, name): self._all("time_phase", name)
+    def set_tuning(self, shapes): self._all("set_tuning", shapes)
+    def route(self, sync=True): self._all("route", sync)
+
+    def autotune(self, reps=2):
+        """Tunes each lane on its own landscapes (lanes run one after another while tuning)."""
+        return self._all("autotune", reps)
+
+    def tuning(self):
+        return self._all("tuning")
+
+    def timers(self, lane=None):
+        """{phase: (ms, launches)}: of one lane, or summed over lanes (lanes run concurrently, so the
+        sum over-counts wall time)."""
+        if lane is not None:
+            return self.lanes[lane].timers()
+        out = {}
+        for b in self.lanes:
+            for p, (ms, c) in b.timers().items():
+                m0, c0 = out.get(p, (0.0, 0))
+                out[p] = (m0 + ms, c0 + c)
+        return out
+
+    def memory(self):
+        tot = [b.memory() for b in self.lanes]
+        return sum(t[0] for t in tot), tot[0][1]
 
 
 class Pool(Batch):
