@@ -305,7 +305,7 @@ static int ctx_alloc(long long capacity, int nslots, fsx_ctx** out) {
   A(P, ninst); A(chunks, c->chunk_cap);
   A(h, N); A(b, N); A(u, N); A(kf, N); A(kd, N); A(precip, N); A(etot, N); A(erate, N); A(area, N);
   A(length, N); A(hwater, N); A(lake_depth, N); A(sedflux, N); A(ht, N); A(water, N); A(zint, N); A(zintp, N);
-  A(gam, N); A(dh, N); A(hp, N); A(lwv, 3 * N); A(lsed, N); A(kfint, N); A(kdint, N); A(g, N); A(pexp, N); A(elev, N);
+  A(gam, N); A(dh, N); A(hp, N); A(lwv, N); A(lsed, N); A(kfint, N); A(kdint, N); A(g, N); A(pexp, N); A(elev, N);
   A(adx, N); A(ady, N); A(adg, N); A(adf, N);
   A(rec, N); A(basin, N); A(rootrank, N); A(order, N); A(lstart, N + ninst); A(lsill, N);
   A(dmask, N); A(bc, N);

@@ -1232,7 +1232,7 @@ int basins_set_lds(int smem_max) {
 // downward one.  The result is the unique rooted tree; the adjacency order (filled with atomics) only
 // changes the tour, not the orientation.  Directed edges ("slots") are grouped by source basin.
 #define TOUR_FLAG0 32  // flags[TOUR_FLAG0 + round]: the jumping round changed something
-#define TOUR_ROUNDS 30
+#define TOUR_ROUNDS FSX_JUMP_ROUNDS  // the tour has at most 2 slots per basin: the same bound
 
 __device__ __forceinline__ unsigned long long tpack(unsigned dist, unsigned nxt) {
   return ((unsigned long long)dist << 32) | nxt;

@@ -97,7 +97,7 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_spl_post(const InstParams* __rest
                                                         double* __restrict__ h, double* __restrict__ b,
                                                         const double* __restrict__ ht, double* __restrict__ etot,
                                                         double* __restrict__ erate, double* __restrict__ sedflux,
-                                                        const int* __restrict__ lsill, const unsigned long long* __restrict__ lwv,
+                                                        const int* __restrict__ lsill, const double* __restrict__ lwv,
                                                         const double* __restrict__ lsed, const double* __restrict__ water,
                                                         const double* __restrict__ gmax_flag, const int* __restrict__ pos,
                                                         const double* __restrict__ Hl, const double* __restrict__ HWl,
@@ -119,7 +119,7 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_spl_post(const InstParams* __rest
   if (gmax_flag[c.inst] != 0.0) {  // the fill adds exactly zero when g == 0 everywhere
     const int sl = lsill[g_];
     if (sl >= 0) {
-      const double vol = lakevol_get(lwv, P.off + sl);
+      const double vol = lwv[P.off + sl];
       if (vol > 0.0) {
         double s = lsed[P.off + sl];
         s = s < vol ? s : vol;

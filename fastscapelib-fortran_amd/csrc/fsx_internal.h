@@ -142,34 +142,6 @@ __device__ __forceinline__ int donors_sorted(const InstParams& P, int v, unsigne
   return nd;
 }
 
-// Lake water volume of a sill (src/StreamPowerLaw.f90:319-320): a sum of non-negative depths added
-// by the lake's nodes, which the sweeps visit concurrently.  Kept as three 64-bit integer words
-// holding the sums of each contribution's 32-bit fixed-point digits (units 1, 2^-32 and 2^-64 m), so
-// that the atomic additions are exact and the total does not depend on their order: the result is
-// the same whatever the schedule or the other landscapes of the batch.  Digits are extracted
-// exactly (the integer part is subtracted, then scaled by 2^32); a contribution keeps its value down
-// to 2^-64 m and is clamped below 2^32 m.
-__device__ __forceinline__ void lakevol_clear(unsigned long long* a, long long i) {
-  a[3 * i] = 0ull;
-  a[3 * i + 1] = 0ull;
-  a[3 * i + 2] = 0ull;
-}
-__device__ __forceinline__ void lakevol_add(unsigned long long* a, long long i, double x) {
-  if (!(x > 0.0)) return;
-  if (x > 4294967295.0) x = 4294967295.0;
-  const double d2 = floor(x);
-  const double r1 = (x - d2) * 4294967296.0;
-  const double d1 = floor(r1);
-  const double d0 = floor((r1 - d1) * 4294967296.0);
-  atomicAdd(a + 3 * i, (unsigned long long)d2);
-  atomicAdd(a + 3 * i + 1, (unsigned long long)d1);
-  atomicAdd(a + 3 * i + 2, (unsigned long long)d0);
-}
-__device__ __forceinline__ double lakevol_get(const unsigned long long* a, long long i) {
-  return ((double)a[3 * i + 2] * 5.421010862427522e-20 + (double)a[3 * i + 1] * 2.3283064365386963e-10) +
-         (double)a[3 * i];
-}
-
 // x**n and x**(n-1) of the Newton step of the stream-power law (src/StreamPowerLaw.f90:352-353), x >= 0.
 // For the exponents the configurations use, the libm power is replaced by its exact-arithmetic
 // equivalent (n = 2: x*x and x; n = 1.5: x*sqrt(x) and sqrt(x)), within an ulp of pow and far
@@ -194,6 +166,11 @@ __device__ __forceinline__ void npow(double x, double n, int kind, double& pn, d
 // blocks for every landscape renumbers its blocks so that all blocks of a landscape land on one XCD
 // (its scattered reads then share that L2).  Launch with xcd_grid(ninst, per_inst) 1-D blocks;
 // returns false for the padding blocks.
+// Pointer-jumping launches (two hops each).  A hop adds the pointed-to pair's distance, which at the
+// start of the launch is at least the launch's guaranteed reach D, so one launch multiplies D by at
+// least 3 (2D after the first hop, + D after the second): 20 launches reach 3^20 > 2^31 nodes and a
+// 21st changes nothing unless the pointers form a cycle (flags[FSX_JUMP_ROUNDS - 1] then reports it).
+#define FSX_JUMP_ROUNDS 21
 #define FSX_NXCD 8
 __host__ __device__ __forceinline__ int xcd_grid(int ninst, int per_inst) {
   return FSX_NXCD * ((ninst + FSX_NXCD - 1) / FSX_NXCD) * per_inst;

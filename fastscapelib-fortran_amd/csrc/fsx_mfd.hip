@@ -399,7 +399,7 @@ __global__ __launch_bounds__(FSX_WG) void k_mfd_spl(
     const double* __restrict__ EC, const double* __restrict__ Al, const double* __restrict__ HTl,
     double* __restrict__ Hl, double* __restrict__ HPl, double* __restrict__ DHl, double* __restrict__ ELl,
     double* __restrict__ Wl, int* __restrict__ LSl, double* __restrict__ water, int* __restrict__ lsill,
-    unsigned long long* __restrict__ lwv, double* __restrict__ lsed, const unsigned long long* __restrict__ redk,
+    double* __restrict__ lwv, double* __restrict__ lsed, const unsigned long long* __restrict__ redk,
     double* __restrict__ scal, int* __restrict__ ngs_out, long long* __restrict__ newton_out, int* __restrict__ err) {
   __shared__ double red[FSX_WG / 64];
   __shared__ double s_tol, s_err, s_gmax;
@@ -483,7 +483,7 @@ __global__ __launch_bounds__(FSX_WG) void k_mfd_spl(
           Wl[gp] = r.t;
           LSl[gp] = v;
           if (!gzero) {
-            lakevol_clear(lwv, off + v);
+            lwv[off + v] = 0.0;
             water[off + v] = r.t;
             lsill[off + v] = v;
           }
@@ -531,12 +531,12 @@ __global__ __launch_bounds__(FSX_WG) void k_mfd_spl(
             }
           }
           sl = v;
-          if (!gzero) lakevol_clear(lwv, off + v);
+          if (!gzero) lwv[off + v] = 0.0;
           if (hv < w_rcv) hv = w_rcv;
         } else {
           hv = r.e;
           sl = LSl[off + r.rp];
-          if (!gzero && sl >= 0) lakevol_add(lwv, off + sl, w_rcv - hv);
+          if (!gzero && sl >= 0) atomicAdd(lwv + off + sl, w_rcv - hv);
         }
         const double wv = w_rcv > hv ? w_rcv : hv;
         Hl[gp] = hv;

@@ -23,8 +23,7 @@ struct DevState {
   double *h = nullptr, *b = nullptr, *u = nullptr, *kf = nullptr, *kd = nullptr, *precip = nullptr;
   double *etot = nullptr, *erate = nullptr, *area = nullptr, *length = nullptr, *hwater = nullptr;
   double *lake_depth = nullptr, *sedflux = nullptr, *ht = nullptr, *water = nullptr, *zint = nullptr;
-  double *zintp = nullptr, *gam = nullptr, *dh = nullptr, *hp = nullptr, *lsed = nullptr;
-  unsigned long long* lwv = nullptr;  // lake water volume per sill, three integer words per node (lakevol_*)
+  double *zintp = nullptr, *gam = nullptr, *dh = nullptr, *hp = nullptr, *lwv = nullptr, *lsed = nullptr;
   double *kfint = nullptr, *kdint = nullptr, *g = nullptr, *pexp = nullptr, *elev = nullptr;
   double *adx = nullptr, *ady = nullptr, *adg = nullptr, *adf = nullptr;  // ADI couplings, diagonal, rhs
   int *rec = nullptr, *basin = nullptr, *rootrank = nullptr, *order = nullptr;

@@ -104,7 +104,7 @@ __global__ __launch_bounds__(FSX_WG) void k_level_sort(const InstParams* __restr
   const long long off = P.off;
   int* ls = lstart + P.aux;  // nn+1 entries
   int* lc = lcur + P.aux;
-  if (flags[25]) {  // jumping did not converge: a receiver cycle, so depths are meaningless
+  if (flags[FSX_JUMP_ROUNDS - 1]) {  // jumping did not converge: a receiver cycle, so depths are meaningless
     if (tid == 0) {
       atomicOr(err + inst, FSX_ERR_LEVELS);
       nlev[inst] = 0;  // the sweeps then visit no level
@@ -235,7 +235,7 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_area_prep(const InstParams* __res
                                                          const int* __restrict__ lstart, const double* __restrict__ precip,
                                                          const int* __restrict__ flags, AreaRec64* __restrict__ arec) {
   NODE_PROLOGUE
-  if (!P.sfd || flags[25]) return;  // flags[25]: no level order this step (receiver cycle)
+  if (!P.sfd || flags[FSX_JUMP_ROUNDS - 1]) return;  // no level order this step (receiver cycle)
   const int p = pos[g_];
   const int d = jdist(J[g_]);
   const long long gp = (long long)P.off + p;
@@ -289,7 +289,7 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_spl_fact(const InstParams* __rest
     atomicMax(redk + 2 * c.inst, kh);
     atomicMax(redk + 2 * c.inst + 1, kg);
   }
-  if (!live || flags[25]) return;  // flags[25]: no level order this step (receiver cycle)
+  if (!live || flags[FSX_JUMP_ROUNDS - 1]) return;  // no level order this step (receiver cycle)
   const int p = pos[g_];
   const int d = jdist(J[g_]);
   const long long gp = (long long)P.off + p;
@@ -547,7 +547,7 @@ __global__ __launch_bounds__(FSX_WG) void k_spl_sweep(
     const int4* __restrict__ grec, const int* __restrict__ ngroups, const SplRec32* __restrict__ srec,
     const AreaRec64* __restrict__ arec, const uint8_t* __restrict__ bc, const double* __restrict__ g,
     const double* __restrict__ Al, double* __restrict__ h, double* __restrict__ hwater, double* __restrict__ lake_depth,
-    double* __restrict__ water, int* __restrict__ lsill, unsigned long long* __restrict__ lwv, double* __restrict__ lsed,
+    double* __restrict__ water, int* __restrict__ lsill, double* __restrict__ lwv, double* __restrict__ lsed,
     double* __restrict__ Hl, double* __restrict__ Wl, double* __restrict__ HWl, int* __restrict__ LSl,
     double* __restrict__ HPl, double* __restrict__ DHl, double* __restrict__ ELl, double* __restrict__ scal,
     int* __restrict__ ngs_out, long long* __restrict__ newton_out, int* __restrict__ err, int vcap, int fill_only,
@@ -685,7 +685,7 @@ __global__ __launch_bounds__(FSX_WG) void k_spl_sweep(
           int sl = v;
           if (l == 0) {
             wv = t;
-            if (!gzero && !fill_only) lakevol_clear(lwv, off + v);
+            if (!gzero && !fill_only) lwv[off + v] = 0.0;
           } else {
             const int q = rr.q;
             double hr, w_rcv;
@@ -732,13 +732,13 @@ __global__ __launch_bounds__(FSX_WG) void k_spl_sweep(
                   }
                   my_newton += it;
                 }
-                if (!gzero) lakevol_clear(lwv, off + v);
+                if (!gzero) lwv[off + v] = 0.0;
                 if (hv < w_rcv) hv = w_rcv;
               } else {
                 hv = e;
                 if constexpr (FAST) sl = -1;
                 else sl = gzero ? -1 : (pin ? BLS(pb)[q] : LSl[off + bp + q]);
-                if (!gzero && sl >= 0) lakevol_add(lwv, off + sl, w_rcv - hv);
+                if (!gzero && sl >= 0) atomicAdd(lwv + off + sl, w_rcv - hv);
               }
               wv = w_rcv > hv ? w_rcv : hv;
             }
@@ -872,7 +872,7 @@ void launch_jump(DevState& S, hipStream_t st) {
   k_jump_init<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.rec, S.J);
   launch_fill_int(S.flags, 64, 0, st);
   const int nb = std::min(S.nchunks, S.jump_blocks);
-  for (int it = 0; it < 26; it++) k_jump<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.nchunks, S.J, S.flags, it);
+  for (int it = 0; it < FSX_JUMP_ROUNDS; it++) k_jump<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.nchunks, S.J, S.flags, it);
 }
 
 void launch_levels(DevState& S, hipStream_t st) {
