@@ -318,7 +318,7 @@ static int ctx_alloc(long long capacity, int nslots, fsx_ctx** out) {
   A(sh, FSX_SLAB_CAP * N);
   A(nbasins, ninst); A(basin0, ninst); A(nconn, ninst); A(nlev, ninst); A(err, ninst); A(ngs, ninst);
   A(ntree, ninst); A(chunkcnt, c->chunk_cap); A(newton, ninst); A(scal, ninst); A(redk, 2 * ninst); A(flags, 64);
-  A(nsegs, ninst); A(segctr, ninst); A(maxd, ninst); A(bstate, 4 * ninst); A(nrst, 16 * ninst);
+  A(nsegs, ninst); A(segctr, ninst); A(maxd, ninst); A(bstate, 4 * ninst); A(nrst, 32 * ninst);
   A(J, N); A(ngroups, ninst); A(grec, 2 * (N + ninst)); A(gmeta, N + ninst); A(pos, N); A(lcur, N + ninst); A(rp, N); A(LSl, N); A(dn, 8 * N);
   A(A0, N); A(Al, N); A(F, N); A(HTl, N); A(Hl, N); A(Wl, N); A(HWl, N); A(HPl, N); A(DHl, N); A(ELl, N);
   A(arec, N); A(srec, N); A(wnode, N); A(wmask, N);

@@ -602,10 +602,10 @@ __device__ __forceinline__ void nr_partition(double* K, int* I, int* LA, int* RA
 // workgroup; ranges are disjoint, so the order of processing does not matter), and appends the
 // children: big ones to level r+1's list, the others to the segment list of k_nr_small.  Level lists
 // alternate between two halves of the int view of cw (free once k_nr_init has copied the keys);
-// nrst[NR_ST * inst + r] counts level r's ranges, nrst[NR_ST * inst + NR_ST - 1] the segments.  Ranges
-// still big after NR_LEVELS levels are finished by k_nr_big with a stack in LDS.
-#define NR_LEVELS 12
-#define NR_ST 16
+// nrst[NR_ST * inst + r] counts level r's ranges, nrst[NR_ST * inst + NR_ST - 1] the segments.  The
+// number of levels launched grows with the largest landscape (nr_levels); ranges still big after them
+// are finished by k_nr_big, one at a time, depth first with a stack in LDS.
+#define NR_ST 32  // per-landscape counters: levels 0..NR_ST-2, segments at NR_ST-1
 
 __device__ __forceinline__ int* nr_list(double* cw, long long eb, int n, int level) {
   return reinterpret_cast<int*>(cw) + 2 * eb + ((level & 1) ? n + 2 : 0);
@@ -682,12 +682,13 @@ __global__ __launch_bounds__(FSX_WG) void k_nr_level(const InstParams* __restric
   }
 }
 
-// big ranges left after NR_LEVELS levels (none for keys of ordinary spread): one workgroup, stack in LDS
+// big ranges left after the level launches (none for keys of ordinary spread): one workgroup, each
+// range depth first with a stack in LDS (the stack holds at most two ranges per partition depth)
 __global__ __launch_bounds__(FSX_WG) void k_nr_big(const InstParams* __restrict__ PP, const int* __restrict__ nconn,
                                                    double* __restrict__ cw, double* __restrict__ skey,
                                                    int* __restrict__ sid, int* __restrict__ Lasc, int* __restrict__ Rasc,
                                                    int* __restrict__ segs, int* __restrict__ nrst, int* __restrict__ nsegs,
-                                                   int* __restrict__ err) {
+                                                   int* __restrict__ err, int nlevels) {
   __shared__ int lds[FSX_WG / 64];
   __shared__ int sh[4];
   __shared__ double shd[1];
@@ -704,15 +705,14 @@ __global__ __launch_bounds__(FSX_WG) void k_nr_big(const InstParams* __restrict_
   int* LA = Lasc + eb;
   int* RA = Rasc + eb;
   int* SG = segs + eb;
+  const int left = st[nlevels];
+  const int* L = nr_list(cw, eb, n, nlevels);
+  if (threadIdx.x == 0) s_ns = st[NR_ST - 1];
+  for (int t = 0; t < left; t++) {
   if (threadIdx.x == 0) {
-    const int left = st[NR_LEVELS];
-    const int* L = nr_list(cw, eb, n, NR_LEVELS);
-    s_nb = 0;
-    s_ns = st[NR_ST - 1];
-    for (int t = 0; t < left; t++) {
-      if (s_nb < 128) { bst[2 * s_nb] = L[2 * t]; bst[2 * s_nb + 1] = L[2 * t + 1]; s_nb++; }
-      else atomicOr(err + inst, FSX_ERR_SORTSTACK);
-    }
+    bst[0] = L[2 * t];
+    bst[1] = L[2 * t + 1];
+    s_nb = 1;
   }
   __syncthreads();
   while (s_nb > 0) {
@@ -735,6 +735,7 @@ __global__ __launch_bounds__(FSX_WG) void k_nr_big(const InstParams* __restrict_
       }
     }
     __syncthreads();
+  }
   }
   if (threadIdx.x == 0) nsegs[inst] = s_ns;
 }
@@ -1420,11 +1421,15 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_correct(const InstParams* __restr
 void launch_nr_sort(DevState& S, hipStream_t st) {
   launch_fill_int(S.segctr, S.ninst, 0, st);
   k_nr_init<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.nconn, S.cw, S.skey, S.sid, S.segs, S.nrst);
-  for (int r = 0; r < NR_LEVELS; r++)
-    k_nr_level<<<dim3(std::min(S.nr_gx, 1 << r), S.ninst), FSX_WG, 0, st>>>(S.P, S.nconn, S.cw, S.skey, S.sid, S.Lasc,
-                                                                              S.Rasc, S.segs, S.nrst, r);
+  // levels for the largest landscape's edge count (6 per node at most): log2(edges / NR_BIG) plus slack
+  // for unbalanced partitions, within the per-landscape counters
+  int nlev = 4;
+  for (long long e = 6LL * S.max_nn; e > NR_BIG && nlev < NR_ST - 2; e >>= 1) nlev++;
+  for (int r = 0; r < nlev; r++)
+    k_nr_level<<<dim3(std::min(S.nr_gx, 1 << std::min(r, 20)), S.ninst), FSX_WG, 0, st>>>(
+        S.P, S.nconn, S.cw, S.skey, S.sid, S.Lasc, S.Rasc, S.segs, S.nrst, r);
   k_nr_big<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.nconn, S.cw, S.skey, S.sid, S.Lasc, S.Rasc, S.segs, S.nrst, S.nsegs,
-                                       S.err);
+                                       S.err, nlev);
   k_nr_small<<<dim3(S.nr_small_gx, S.ninst), NR_SMALL_WG, 0, st>>>(S.P, S.nsegs, S.segctr, S.skey, S.sid, S.segs, S.err);
   k_nr_pos<<<dim3(16, S.ninst), FSX_CHUNK, 0, st>>>(S.P, S.nconn, S.sid, S.npos);
 }

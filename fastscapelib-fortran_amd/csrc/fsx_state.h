@@ -83,5 +83,5 @@ struct DevState {
   int *flags = nullptr;        // convergence flags of iterative kernels (64)
   int *nsegs = nullptr, *segctr = nullptr;  // edge-sort segment lists [ninst]
   int* bstate = nullptr;       // spanning forest between launches: live edges, components, done, kept [4*ninst]
-  int* nrst = nullptr;         // edge sort between launches: big ranges per level, segments [16*ninst]
+  int* nrst = nullptr;         // edge sort between launches: big ranges per level, segments [32*ninst]
 };
