@@ -416,7 +416,7 @@ def main():
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every kernel directly instead of replaying HIP graphs (for kernel-trace profiling)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (no autotuner)")
-    ap.add_argument("--lanes", type=int, default=3,
+    ap.add_argument("--lanes", type=int, default=2,
                     help="contexts stepping the batch side by side on their own streams (1: one context)")
     ap.add_argument("--shapes", default=None,
                     help="walk_gx,jump_blocks,tour_gx,nr_small_gx for the headline batch instead of the autotuner")
