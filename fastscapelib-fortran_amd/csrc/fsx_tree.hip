@@ -112,31 +112,13 @@ __global__ __launch_bounds__(FSX_WG) void k_level_sort(const InstParams* __restr
     }
     return;
   }
+  // one pass over the nodes: the histogram of depths below LVL_HCAP (in LDS) and the largest depth;
+  // depths from LVL_HCAP on (very deep trees only) are counted in global memory by a second pass.
+  // The node loops take LVL_UNROLL nodes per thread at a time so their loads overlap.
+  for (int l = tid; l < LVL_HCAP; l += FSX_WG) hist[l] = 0;
   if (tid == 0) s_max = 0;
   __syncthreads();
-  // the node loops take LVL_UNROLL nodes per thread at a time so their loads overlap
   int mx = 0;
-  for (int v0 = tid; v0 < nn; v0 += LVL_UNROLL * FSX_WG) {
-    int dd[LVL_UNROLL];
-#pragma unroll
-    for (int u = 0; u < LVL_UNROLL; u++) {
-      const int v = v0 + u * FSX_WG;
-      dd[u] = v < nn ? jdist(J[off + v]) : 0;
-    }
-#pragma unroll
-    for (int u = 0; u < LVL_UNROLL; u++) {
-      const int d = dd[u] < 0 ? 0 : (dd[u] >= nn ? nn - 1 : dd[u]);
-      mx = d > mx ? d : mx;
-    }
-  }
-  for (int o = 32; o > 0; o >>= 1) { const int y = __shfl_xor(mx, o, 64); mx = y > mx ? y : mx; }
-  if ((tid & 63) == 0) atomicMax(&s_max, mx);
-  __syncthreads();
-  const int L = s_max + 1;
-  const int LH = L < LVL_HCAP ? L : LVL_HCAP;
-  for (int l = tid; l < LH; l += FSX_WG) hist[l] = 0;
-  for (int l = LVL_HCAP + tid; l < L; l += FSX_WG) lc[l] = 0;
-  __syncthreads();
   for (int v0 = tid; v0 < nn; v0 += LVL_UNROLL * FSX_WG) {
     int dd[LVL_UNROLL];
 #pragma unroll
@@ -148,7 +130,30 @@ __global__ __launch_bounds__(FSX_WG) void k_level_sort(const InstParams* __restr
     for (int u = 0; u < LVL_UNROLL; u++) {
       if (v0 + u * FSX_WG >= nn) continue;
       const int d = dd[u] < 0 ? 0 : (dd[u] >= nn ? nn - 1 : dd[u]);
-      if (d < LVL_HCAP) atomicAdd(hist + d, 1); else atomicAdd(lc + d, 1);
+      mx = d > mx ? d : mx;
+      if (d < LVL_HCAP) atomicAdd(hist + d, 1);
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) { const int y = __shfl_xor(mx, o, 64); mx = y > mx ? y : mx; }
+  if ((tid & 63) == 0) atomicMax(&s_max, mx);
+  __syncthreads();
+  const int L = s_max + 1;
+  if (L > LVL_HCAP) {
+    for (int l = LVL_HCAP + tid; l < L; l += FSX_WG) lc[l] = 0;
+    __syncthreads();
+    for (int v0 = tid; v0 < nn; v0 += LVL_UNROLL * FSX_WG) {
+      int dd[LVL_UNROLL];
+#pragma unroll
+      for (int u = 0; u < LVL_UNROLL; u++) {
+        const int v = v0 + u * FSX_WG;
+        dd[u] = v < nn ? jdist(J[off + v]) : -1;
+      }
+#pragma unroll
+      for (int u = 0; u < LVL_UNROLL; u++) {
+        if (v0 + u * FSX_WG >= nn) continue;
+        const int d = dd[u] < 0 ? 0 : (dd[u] >= nn ? nn - 1 : dd[u]);
+        if (d >= LVL_HCAP) atomicAdd(lc + d, 1);
+      }
     }
   }
   __syncthreads();

@@ -168,6 +168,24 @@ def test_nonlinear_and_bc_variants(fs, oracle, n, ibc, g):
     _check(got, ref, 25)
 
 
+def test_deeper_than_lds_level_histogram(fs, oracle):
+    """A 40001x3 strip fixed on its left edge only, rising to the right: receiver chains of ~40000
+    nodes, so the level order holds more levels than the LDS histogram (32768) and the deep levels
+    go through the global-memory counts."""
+    nx, ny = 40001, 3
+    L = landscape(nx, ny, seed=77, dx=100.0, ibc=1, kf=1e-6, kd=1e-3)
+    L["h"] = (np.tile(np.arange(nx, dtype=np.float64), ny) * 0.5 + L["h"] * 0.1)
+    L["u"] = np.full(nx * ny, 1e-4)
+    L["u"][np.arange(nx * ny) % nx == 0] = 0.0
+    print("deep strip 40001x3, ibc=0001, 2 steps", flush=True)
+    got, ref = _run_both(fs, oracle, L, 2)
+    _check(got, ref, 2)
+    with fs.Batch([(nx, ny)]) as b:
+        apply(b, L, 0)
+        b.step(1)
+        assert b.stats(0)["levels"] > 32768
+
+
 def test_batch_equals_alone(fs, oracle):
     """Ragged batch: each landscape's result equals its run alone, bit for bit."""
     Ls = [landscape(65, 90, seed=1, dx=250.0, ibc=1010, sinus=(10.0, 5000.0)),
