@@ -53,7 +53,7 @@ ALG_BYTES_PER_NODE = {
     "spl_sweep": 40,     # R record  W Hl, HWl (level order)
     "spl_post": 100,     # R pos, Hl, HWl, ht, b, etot  W h, b, etot, erate, sedflux, hwater, lake_depth
 }
-ALG_BYTES_PER_EDGE = {"lm_mst": 13, "lm_orient": 33}  # per basin-graph edge
+ALG_BYTES_PER_EDGE = {}  # per basin-graph edge (the spanning-forest and tour phases are multi-kernel)
 
 
 def uniform_batch(ninst=64, n=513, dx=200.0):
