@@ -251,19 +251,26 @@ class Lanes:
             "get_step", "stats"}
     TUNING = Batch.TUNING
 
-    def __init__(self, sizes, lanes=2):
+    @staticmethod
+    def deal(sizes, lanes):
+        """Landscape ids per lane: largest landscape first, each to the lane with the fewest cells so
+        far (ties: the lower lane); ids ascending within a lane.  At most one lane per landscape."""
         sizes = [(int(a), int(b)) for a, b in sizes]
-        if not sizes:
-            raise ValueError("Lanes needs at least one landscape")
         nl = max(1, min(int(lanes), len(sizes)))
         load, members = [0] * nl, [[] for _ in range(nl)]
         for k in sorted(range(len(sizes)), key=lambda k: (-sizes[k][0] * sizes[k][1], k)):
             j = min(range(nl), key=lambda j: (load[j], j))
             members[j].append(k)
             load[j] += sizes[k][0] * sizes[k][1]
+        return [sorted(m) for m in members]
+
+    def __init__(self, sizes, lanes=2):
+        sizes = [(int(a), int(b)) for a, b in sizes]
+        if not sizes:
+            raise ValueError("Lanes needs at least one landscape")
         self.sizes = dict(enumerate(sizes))
         self.nn = {k: a * b for k, (a, b) in self.sizes.items()}
-        self.members = [sorted(m) for m in members]
+        self.members = Lanes.deal(sizes, lanes)
         self.where = {k: (j, i) for j, m in enumerate(self.members) for i, k in enumerate(m)}
         self.lanes = []
         try:

@@ -102,3 +102,16 @@ def test_latency_is_completion_minus_arrival(serving):
     for r in runs:
         assert r.done > r.admitted >= r.arrival and r.latency == r.done - r.arrival
     assert rep["p99_latency_s"] >= rep["p50_latency_s"] > 0
+
+
+def test_lanes_deal_balances_cells_and_covers_every_landscape():
+    """Lanes.deal (host logic only): largest first to the least-loaded lane, every id exactly once."""
+    from fastscapelib_fortran import Lanes
+    sizes = [(200, 200), (100, 100), (100, 100), (100, 100), (100, 100), (50, 50)]
+    m = Lanes.deal(sizes, 2)
+    assert sorted(k for lane in m for k in lane) == list(range(len(sizes)))
+    loads = [sum(sizes[k][0] * sizes[k][1] for k in lane) for lane in m]
+    assert max(loads) - min(loads) <= 50 * 50
+    assert all(lane == sorted(lane) for lane in m)
+    assert Lanes.deal(sizes, 10) == [[0], [1], [2], [3], [4], [5]]  # no empty lanes
+    assert Lanes.deal([(513, 513)] * 64, 2) == [list(range(0, 64, 2)), list(range(1, 64, 2))]
