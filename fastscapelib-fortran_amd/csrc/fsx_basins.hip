@@ -800,7 +800,10 @@ __global__ __launch_bounds__(NR_SMALL_WG) void k_nr_small(const InstParams* __re
   __shared__ int sLA[NR_BIG];
   __shared__ int sRA[NR_BIG];
   __shared__ int q[2 * NR_QCAP];      // pending ranges (more than NR_TINY keys)
-  __shared__ int tq[2 * NR_BIG / 2];  // tiny ranges
+  // tiny ranges as 16-bit positions (segments hold at most NR_BIG = 2048 keys): with them the block
+  // stays under a third of the 160 KB LDS, so three blocks share a compute unit instead of two
+  __shared__ short tq[2 * NR_BIG / 2];
+  static_assert(NR_BIG <= 32767, "tiny-range positions are 16-bit");
   constexpr int NWV = NR_SMALL_WG / 64;
   __shared__ int lds[NWV];
   __shared__ int sh[NWV + 1][4];
@@ -825,7 +828,7 @@ __global__ __launch_bounds__(NR_SMALL_WG) void k_nr_small(const InstParams* __re
     if (tid == 0) {
       s_nq = 0; s_nt = 0; s_ovf = 0;
       if (n > NR_TINY) { q[0] = 0; q[1] = n - 1; s_nq = 1; }
-      else if (n >= 2) { tq[0] = 0; tq[1] = n - 1; s_nt = 1; }
+      else if (n >= 2) { tq[0] = 0; tq[1] = (short)(n - 1); s_nt = 1; }
     }
     __syncthreads();
     while (s_nq > 0) {
@@ -860,7 +863,7 @@ __global__ __launch_bounds__(NR_SMALL_WG) void k_nr_small(const InstParams* __re
             if (slot < NR_QCAP) { q[2 * slot] = a0; q[2 * slot + 1] = a1; } else s_ovf = 1;
           } else if (sz >= 2) {
             const int slot = atomicAdd(&s_nt, 1);
-            tq[2 * slot] = a0; tq[2 * slot + 1] = a1;
+            tq[2 * slot] = (short)a0; tq[2 * slot + 1] = (short)a1;
           }
         }
       }
