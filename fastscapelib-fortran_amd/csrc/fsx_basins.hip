@@ -536,24 +536,39 @@ __device__ __forceinline__ void nr_partition(double* K, int* I, int* LA, int* RA
   grp_sync<NT>();
   const double a = shd[0];
   int cL = 0, cR = 0;
+  const int ln = rk & 63;
+  const unsigned long long below = (1ull << ln) - 1;
+  // a chunk of NT * NR_ELT positions: wave w takes NR_ELT runs of 64 consecutive positions, so every
+  // load and list store of a wave is contiguous; position order inside the wave is (run, lane), the
+  // ballots give each flagged lane its rank, and one scan over the waves places them in the chunk
   for (int cb = l + 1; cb <= ir; cb += NT * NR_ELT) {
-    const int p0 = cb + rk * NR_ELT;
-    int fl = 0, fr = 0;  // bit masks over the NR_ELT positions of this thread
+    const int p0 = cb + (rk >> 6) * (64 * NR_ELT) + ln;
+    unsigned long long ml[NR_ELT], mr[NR_ELT];
+    int nl = 0, nr = 0;
+#pragma unroll
     for (int e = 0; e < NR_ELT; e++) {
-      const int p = p0 + e;
+      const int p = p0 + 64 * e;
+      bool fl = false, fr = false;
       if (p <= ir) {
         const double kp = K[p];
-        if (p >= l + 2 && kp >= a) fl |= 1 << e;
-        if (p <= ir - 1 && kp <= a) fr |= 1 << e;
+        fl = p >= l + 2 && kp >= a;
+        fr = p <= ir - 1 && kp <= a;
       }
+      ml[e] = __ballot(fl);
+      mr[e] = __ballot(fr);
+      nl += __popcll(ml[e]);
+      nr += __popcll(mr[e]);
     }
-    const int nl = __popc(fl), nr = __popc(fr);
+    const int own = nl | (nr << 16);
     int tot;
-    const int incl = grp_scan<NT>(nl | (nr << 16), lds, tot);
+    const int incl = grp_scan<NT>(ln == 0 ? own : 0, lds, tot);
     int rl = cL + (incl & 0xffff) - nl, rr = cR + (incl >> 16) - nr;
+#pragma unroll
     for (int e = 0; e < NR_ELT; e++) {
-      if (fl & (1 << e)) LA[l + rl++] = p0 + e;
-      if (fr & (1 << e)) RA[l + rr++] = p0 + e;
+      if ((ml[e] >> ln) & 1) LA[l + rl + __popcll(ml[e] & below)] = p0 + 64 * e;
+      if ((mr[e] >> ln) & 1) RA[l + rr + __popcll(mr[e] & below)] = p0 + 64 * e;
+      rl += __popcll(ml[e]);
+      rr += __popcll(mr[e]);
     }
     cL += tot & 0xffff;
     cR += tot >> 16;
