@@ -45,6 +45,7 @@ struct fsx_ctx {
   DevState S;
   std::vector<InstParams> hp;
   std::vector<int> steps;
+  std::vector<char> routed;                  // receivers, levels and area of a step are in place
   std::vector<double> xl, yl;
   std::vector<int> ibc;
   std::vector<char> used;                    // slot holds a landscape
@@ -288,6 +289,7 @@ static int ctx_alloc(long long capacity, int nslots, fsx_ctx** out) {
   for (InstParams& P : c->hp) memset(&P, 0, sizeof P);
   c->used.assign(nslots, 0);
   c->steps.assign(nslots, 0);
+  c->routed.assign(nslots, 0);
   c->xl.assign(nslots, 0.0);
   c->yl.assign(nslots, 0.0);
   c->ibc.assign(nslots, 1111);
@@ -351,6 +353,7 @@ static int slot_add(fsx_ctx* c, int nx, int ny, int* inst, bool relay) {
   geometry(P, (double)(P.nx - 1), (double)(P.ny - 1));
   c->used[k] = 1;
   c->steps[k] = 0;
+  c->routed[k] = 0;
   c->xl[k] = c->yl[k] = 0.0;
   c->ibc[k] = 1111;
   DevState& S = c->S;
@@ -758,7 +761,10 @@ extern "C" int fsx_execute_steps(fsx_ctx* c, int nsteps, void* stream, int sync)
   }
   HIPCHK(hipGetLastError());
   for (int k = 0; k < c->S.ninst; k++)
-    if (c->hp[k].active) c->steps[k] += nsteps;
+    if (c->hp[k].active) {
+      c->steps[k] += nsteps;
+      if (c->hp[k].run_spl && nsteps > 0) c->routed[k] = 1;
+    }
   if (sync) {
     HIPCHK(hipStreamSynchronize(st));
     return check_errors(c);
@@ -798,6 +804,8 @@ extern "C" int fsx_route(fsx_ctx* c, void* stream, int sync) {
   if (stream) HIPCHK(hipStreamSynchronize(c->stream));
   enqueue_route_only(c, st);
   HIPCHK(hipGetLastError());
+  for (int k = 0; k < c->S.ninst; k++)
+    if (c->hp[k].active && c->hp[k].run_spl) c->routed[k] = 1;
   if (sync) {
     HIPCHK(hipStreamSynchronize(st));
     return check_errors(c);
@@ -829,6 +837,43 @@ extern "C" int fsx_copy_field(fsx_ctx* c, int inst, int field, double* dst, int 
   HIPCHK(hipMemcpyAsync(dst, src + P.off, sizeof(double) * P.nn, on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
                         c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+extern "C" int fsx_copy_derived(fsx_ctx* c, int inst, int which, double* dst, int on_device) {
+  if (int r = check_inst(c, inst)) return r;
+  if (!dst) return fail(FSX_EINVAL, "null destination");
+  if (which != FSX_SLOPE && which != FSX_CURVATURE && which != FSX_CHI) return fail(FSX_EINVAL, "unknown derived field");
+  if (which == FSX_CHI && !c->routed[inst]) return fail(FSX_EINVAL, "chi needs the routing of a step first");
+  if (int r = upload_params(c)) return r;
+  const InstParams& P = c->hp[inst];
+  double* tmp = c->S.zint + P.off;  // scratch outside a step
+  if (which == FSX_CHI)
+    launch_chi(c->S, inst, P.dx * P.dy * 10.0, P.m / P.n, tmp, c->stream);
+  else
+    launch_terrain(c->S, inst, P.nn, which == FSX_SLOPE ? 0 : 1, 45.0 / std::atan(1.0), tmp, c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(dst, tmp, sizeof(double) * P.nn, on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+extern "C" int fsx_get_fluxes(fsx_ctx* c, int inst, double* out3) {
+  if (int r = check_inst(c, inst)) return r;
+  if (!out3) return fail(FSX_EINVAL, "null destination");
+  if (!c->routed[inst]) return fail(FSX_EINVAL, "fluxes need the routing of a step first");
+  const InstParams& P = c->hp[inst];
+  if (!P.sfd && !c->S.mrec) return fail(FSX_EINVAL, "no multiple-flow routing in place");
+  if (int r = upload_params(c)) return r;
+  double* sums = c->S.zintp + P.off;  // scratch outside a step (nn >= 9 entries)
+  launch_fluxes(c->S, inst, c->S.zint + P.off, sums, c->stream);
+  HIPCHK(hipGetLastError());
+  double s3[3];
+  HIPCHK(hipMemcpyAsync(s3, sums, sizeof s3, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const double surf = c->xl[inst] * c->yl[inst] / (double)(P.nx - 1) / (double)(P.ny - 1);
+  for (int k = 0; k < 3; k++) out3[k] = s3[k] * surf;
   return 0;
 }
 

@@ -232,6 +232,10 @@ void launch_spl(DevState& S, hipStream_t st);
 void launch_diffusion(DevState& S, hipStream_t st);
 void launch_mfd_route(DevState& S, hipStream_t st);
 void launch_mfd_spl(DevState& S, hipStream_t st);
+// read-outs of one landscape (outside a step)
+void launch_terrain(DevState& S, int inst, int nn, int which, double con, double* out, hipStream_t st);
+void launch_chi(DevState& S, int inst, double a0, double mn, double* out, hipStream_t st);
+void launch_fluxes(DevState& S, int inst, double* flux, double* sums, hipStream_t st);
 int sweep_set_lds(int smem_max);
 int basins_set_lds(int smem_max);
 void launch_fill(DevState& S, hipStream_t st);

@@ -12,8 +12,9 @@ The free functions keep the reference's call surface and f2py argument conventio
 Documented divergences: call-order and size errors raise ``RuntimeError``/``ValueError`` where the
 reference STOPs the process; ``fastscape_setup`` starts from h = 0 rather than Fortran random
 numbers (every run sets h with ``fastscape_init_h``); routines outside the per-step update
-(advection, marine, stratigraphy, VTK, flexure, chi/slope/curvature read-outs) raise
-``NotImplementedError``.
+(advection, marine, stratigraphy, VTK, flexure, the catchment read-out) raise
+``NotImplementedError``.  The slope, curvature and chi read-outs and the three fluxes are computed on
+the GPU from the resident state.
 
 ``Batch`` runs many independent landscapes of different sizes in one set of kernel launches.
 """
@@ -31,6 +32,7 @@ __all__ = [
     "fastscape_init_h", "fastscape_set_erosional_parameters", "fastscape_set_u", "fastscape_set_bc",
     "fastscape_execute_step", "fastscape_copy_h", "fastscape_copy_basement", "fastscape_copy_total_erosion",
     "fastscape_copy_erosion_rate", "fastscape_copy_drainage_area", "fastscape_copy_lake_depth",
+    "fastscape_copy_slope", "fastscape_copy_curvature", "fastscape_copy_chi", "fastscape_get_fluxes",
     "fastscape_get_step", "fastscape_get_sizes", "fastscape_destroy", "fastscape_set_h", "fastscape_set_basement",
     "fastscape_set_precip", "fastscape_set_all_layers", "fastscape_reset_cumulative_erosion", "fastscape_view",
     "fastscape_debug", "Batch", "Lanes", "Pool", "FastscapeError", "indexx",
@@ -183,6 +185,29 @@ class Batch:
         check(lib.fsx_copy_field(self._h, k, field, ptr(out), 0), "copy_field")
         return out
 
+    def derived(self, k, which, out=None):
+        """Slope (FSX_SLOPE), curvature (FSX_CURVATURE) or chi (FSX_CHI) of landscape k, computed on the
+        GPU from its current state (FastScape_Copy_Slope / _Curvature / _Chi)."""
+        if out is None:
+            out = np.empty(self.nn[k], np.float64)
+        if hasattr(out, "data_ptr") and getattr(out, "is_cuda", False):
+            check(lib.fsx_copy_derived(self._h, k, which, C.c_void_p(out.data_ptr()), 1), "copy_derived")
+            return out
+        if out.dtype != np.float64 or not out.flags.c_contiguous or out.size < self.nn[k]:
+            raise ValueError(f"copy target must be a contiguous float64 array of >= {self.nn[k]} values")
+        check(lib.fsx_copy_derived(self._h, k, which, ptr(out), 0), "copy_derived")
+        return out
+
+    def slope(self, k): return self.derived(k, _lib.FSX_SLOPE)
+    def curvature(self, k): return self.derived(k, _lib.FSX_CURVATURE)
+    def chi(self, k): return self.derived(k, _lib.FSX_CHI)
+
+    def fluxes(self, k):
+        """(tectonic, erosion, boundary) flux of landscape k (FastScape_Get_Fluxes)."""
+        o = (C.c_double * 3)()
+        check(lib.fsx_get_fluxes(self._h, k, o), "get_fluxes")
+        return o[0], o[1], o[2]
+
     def ifield(self, k, field):
         n = self.nn[k] * (8 if field in (_lib.FSX_DON, _lib.FSX_MREC) else 1)
         out = np.empty(n, np.int32)
@@ -248,7 +273,7 @@ class Lanes:
     _PER = {"set_xl_yl", "set_dt", "set_bc", "set_newton_tol", "set_active", "set_erosional_parameters",
             "init_h", "set_field", "set_u", "set_h", "set_basement", "set_precip", "set_all_layers",
             "reset_cumulative_erosion", "field", "ifield", "h", "rec", "donors", "order", "mrec", "morder",
-            "get_step", "stats"}
+            "get_step", "stats", "derived", "slope", "curvature", "chi", "fluxes"}
     TUNING = Batch.TUNING
 
     @staticmethod
@@ -472,6 +497,36 @@ def fastscape_copy_drainage_area(a): _copy_into(_lib.FSX_AREA, a)
 def fastscape_copy_lake_depth(d): _copy_into(_lib.FSX_LAKE_DEPTH, d)
 
 
+def _derived_into(which, out):
+    b = _need()
+    if not isinstance(out, np.ndarray) or out.dtype != np.float64 or not out.flags.c_contiguous:
+        raise ValueError("copy target must be a contiguous float64 numpy array (f2py intent(inout))")
+    if out.size < b.nn[0]:
+        raise ValueError(f"copy target holds {out.size} values, need {b.nn[0]}")
+    b.derived(0, which, out.reshape(-1))
+
+
+def fastscape_copy_slope(s):
+    """FastScape_Copy_Slope (src/FastScape_api.f90:395-407): slope in degrees, outer ring 0."""
+    _derived_into(_lib.FSX_SLOPE, s)
+
+
+def fastscape_copy_curvature(c):
+    """FastScape_Copy_Curvature (src/FastScape_api.f90:411-423), outer ring 0."""
+    _derived_into(_lib.FSX_CURVATURE, c)
+
+
+def fastscape_copy_chi(c):
+    """FastScape_Copy_Chi (src/FastScape_api.f90:379-391), along the last step's receivers."""
+    _derived_into(_lib.FSX_CHI, c)
+
+
+def fastscape_get_fluxes():
+    """FastScape_Get_Fluxes (src/FastScape_api.f90:762-774): (tectonic, erosion, boundary) flux, as
+    f2py returns the three intent(out) scalars."""
+    return _need().fluxes(0)
+
+
 def fastscape_get_step():
     """FastScape_Get_Step (src/FastScape_api.f90:572-584)."""
     return _need().get_step(0)
@@ -514,6 +569,5 @@ def _outside(name):
 
 
 for _n in ["fastscape_set_v", "fastscape_set_marine_parameters", "fastscape_strati", "fastscape_vtk",
-           "fastscape_get_fluxes", "fastscape_init_f", "fastscape_copy_f", "fastscape_copy_chi",
-           "fastscape_copy_slope", "fastscape_copy_curvature", "fastscape_copy_catchment"]:
+           "fastscape_init_f", "fastscape_copy_f", "fastscape_copy_catchment"]:
     globals()[_n] = _outside(_n)

@@ -48,6 +48,12 @@ enum {
   FSX_KF = 11,        /* kf (write through fsx_set_erosional_parameters)             */
   FSX_KD = 12,        /* kd (idem)                                                    */
 };
+/* derived field ids of fsx_copy_derived */
+enum {
+  FSX_SLOPE = 0,     /* slope in degrees (slope, src/TerrainDerivatives.f90:3-36)     Copy_Slope     */
+  FSX_CURVATURE = 1, /* curvature (curvature, src/TerrainDerivatives.f90:40-74)        Copy_Curvature */
+  FSX_CHI = 2,       /* chi (CopyChi, src/FastScape_ctx.f90:226-247)                   Copy_Chi       */
+};
 /* integer field ids of fsx_copy_ifield */
 enum {
   FSX_REC = 0,    /* receiver after closed-basin rerouting, 0-based (rec)               */
@@ -131,6 +137,15 @@ int fsx_copy_field(fsx_ctx *ctx, int inst, int field, double *dst, int on_device
  * ints (8*nn for FSX_DON and FSX_MREC).  FSX_MREC/FSX_MNREC/FSX_MSTACK need an instance that uses
  * multiple flow direction (p >= -1.5). */
 int fsx_copy_ifield(fsx_ctx *ctx, int inst, int field, int *dst, int on_device);
+
+/* FastScape_Copy_Slope / Copy_Curvature / Copy_Chi (src/FastScape_api.f90:379-423), computed on the
+ * device from the current state of one landscape into dst (nn doubles).  Chi follows the receivers,
+ * drainage area and flow lengths of the last step (FSX_EINVAL before the first step or fsx_route). */
+int fsx_copy_derived(fsx_ctx *ctx, int inst, int which, double *dst, int on_device);
+/* FastScape_Get_Fluxes (src/FastScape_api.f90:762-774; compute_fluxes, src/FastScape_ctx.f90:843-886):
+ * out3 = tectonic, erosion and boundary flux of one landscape's current state, computed on the device
+ * (FSX_EINVAL before the first step or fsx_route). */
+int fsx_get_fluxes(fsx_ctx *ctx, int inst, double *out3);
 
 /* FastScape_Get_Step (src/FastScape_api.f90:572-584) */
 int fsx_get_step(fsx_ctx *ctx, int inst, int *step);

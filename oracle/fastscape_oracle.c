@@ -1078,3 +1078,126 @@ void fso_copy_mrec(const fso_state *s, int *mrec, double *mwrec, double *mlrec) 
   memcpy(mwrec, s->mwrec, sizeof(double) * NDMAX * (size_t)s->nn);
   memcpy(mlrec, s->mlrec, sizeof(double) * NDMAX * (size_t)s->nn);
 }
+
+/* ------------------------------------------------------------------------------------------ */
+/* Read-outs on the state after a step (FastScape_Copy_Slope / _Curvature / _Chi and
+ * FastScape_Get_Fluxes, src/FastScape_api.f90:379-423,762-774).                                */
+
+/* slope, src/TerrainDerivatives.f90:3-36: Sobel-weighted gradient of the 3x3 block, in degrees;
+ * the outer ring stays 0.  Neighbours: a,b,c the row above (j+1), d,f the row, g,h,i below.    */
+void fso_slope(const double *h, int nx, int ny, double dx, double dy, double *out) {
+  const double con = 45.0 / atan(1.0);
+  for (int ij = 0; ij < nx * ny; ij++) out[ij] = 0.0;
+  for (int j = 1; j < ny - 1; j++)
+    for (int i = 1; i < nx - 1; i++) {
+      const int ij = i + j * nx;
+      const int ia = ij + nx - 1, ib = ia + 1, ic = ib + 1, id = ij - 1, jf = ij + 1, ig = ij - nx - 1,
+                ih = ig + 1, ii = ih + 1;
+      const double dzdx = ((h[ic] + 2.0 * h[jf] + h[ii]) - (h[ia] + 2.0 * h[id] + h[ig])) / 8.0 / dx;
+      const double dzdy = ((h[ig] + 2.0 * h[ih] + h[ii]) - (h[ia] + 2.0 * h[ib] + h[ic])) / 8.0 / dy;
+      double v = dzdx * dzdx + dzdy * dzdy;
+      if (v > DBL_MIN) v = atan(sqrt(v)) * con;
+      out[ij] = v;
+    }
+}
+
+/* curvature, src/TerrainDerivatives.f90:40-74: coefficients of the quadratic fitted to the 3x3
+ * block (1..3 the row above, 4..6 the row, 7..9 below), profile-type curvature
+ * (a(1+e^2)+b(1+d^2)-cde)/(1+d^2+e^2)^(3/2); the outer ring stays 0.                           */
+void fso_curvature(const double *h, int nx, int ny, double dx, double dy, double *out) {
+  for (int ij = 0; ij < nx * ny; ij++) out[ij] = 0.0;
+  for (int j = 1; j < ny - 1; j++)
+    for (int i = 1; i < nx - 1; i++) {
+      const int ij = i + j * nx;
+      const int i1 = ij + nx - 1, i2 = i1 + 1, i3 = i2 + 1, i4 = ij - 1, i5 = ij, i6 = ij + 1, i7 = ij - nx - 1,
+                i8 = i7 + 1, i9 = i8 + 1;
+      const double a = (h[i1] + h[i3] + h[i4] + h[i6] + h[i7] + h[i9]) / dx / dx / 12.0 -
+                       (h[i2] + h[i5] + h[i8]) / dx / dx / 6.0;
+      const double b = (h[i1] + h[i2] + h[i3] + h[i7] + h[i8] + h[i9]) / dy / dy / 12.0 -
+                       (h[i4] + h[i5] + h[i6]) / dy / dy / 6.0;
+      const double c = (h[i3] + h[i7] - h[i1] - h[i9]) / dx / dy / 4.0;
+      const double d = (h[i3] + h[i6] + h[i9] - h[i1] - h[i4] - h[i7]) / dx / 6.0;
+      const double e = (h[i1] + h[i2] + h[i3] - h[i7] - h[i8] - h[i9]) / dy / 6.0;
+      double v = 1.0 + d * d + e * e;
+      if (v > DBL_MIN) v = (a * (1.0 + e * e) + b * (1.0 + d * d) - c * d * e) / pow(v, 3.0 / 2.0);
+      out[ij] = v;
+    }
+}
+
+/* CopyChi, src/FastScape_ctx.f90:226-247: along the stack (any order that visits a receiver before
+ * its donors), chi = chi(rec) + (a0/a)^(m/n) * length where a > a0 = 10 dx dy, else 0.            */
+void fso_chi(const int *order, const int *rec, const double *a, const double *length, int nn, double dx,
+             double dy, double m, double n, double *out) {
+  const double a0 = dx * dy * 10.0;
+  for (int ij = 0; ij < nn; ij++) out[ij] = 0.0;
+  for (int ij = 0; ij < nn; ij++) {
+    const int k = order[ij];
+    if (a[k] > a0) out[k] = out[rec[k]] + pow(a0 / a[k], m / n) * length[k];
+  }
+}
+
+/* compute_fluxes, src/FastScape_ctx.f90:843-886.  surf = xl yl/(nx-1)/(ny-1); tectonic = sum(u)
+ * surf, erosion = sum(erate) surf (index order); the erosion rate, clamped at 0 at every node once
+ * its donors have delivered, is pushed to the receivers -- single flow: along the reversed stack, so
+ * a base-level node finally adds its own flux to itself; multiple flow: along mstack, weighted --
+ * and boundary = sum over the fixed boundary nodes, index order, times surf.                      */
+static void fluxes_sums(const double *u, const double *erate, const double *flux, const unsigned char *bc, int nn,
+                        double surf, double out[3]) {
+  double su = 0.0, se = 0.0, sb = 0.0;
+  for (int ij = 0; ij < nn; ij++) su += u[ij];
+  for (int ij = 0; ij < nn; ij++) se += erate[ij];
+  for (int ij = 0; ij < nn; ij++)
+    if (bc[ij]) sb += flux[ij];
+  out[0] = su * surf;
+  out[1] = se * surf;
+  out[2] = sb * surf;
+}
+
+void fso_fluxes_sfd(const int *order, const int *rec, const double *u, const double *erate,
+                    const unsigned char *bc, int nn, double surf, double out[3]) {
+  double *flux = malloc(sizeof(double) * (size_t)nn);
+  memcpy(flux, erate, sizeof(double) * (size_t)nn);
+  for (int ij = nn - 1; ij >= 0; ij--) {
+    const int k = order[ij];
+    flux[k] = dmax(0.0, flux[k]);
+    flux[rec[k]] = flux[rec[k]] + flux[k];
+  }
+  fluxes_sums(u, erate, flux, bc, nn, surf, out);
+  free(flux);
+}
+
+void fso_fluxes_mfd(const int *mstack, const int *mnrec, const int *mrec, const double *mwrec, const double *u,
+                    const double *erate, const unsigned char *bc, int nn, double surf, double out[3]) {
+  double *flux = malloc(sizeof(double) * (size_t)nn);
+  memcpy(flux, erate, sizeof(double) * (size_t)nn);
+  for (int ij = 0; ij < nn; ij++) {
+    const int k = mstack[ij];
+    flux[k] = dmax(0.0, flux[k]);
+    for (int q = 0; q < mnrec[k]; q++) {
+      const int r = mrec[NDMAX * k + q];
+      flux[r] = flux[r] + flux[k] * mwrec[NDMAX * k + q];
+    }
+  }
+  fluxes_sums(u, erate, flux, bc, nn, surf, out);
+  free(flux);
+}
+
+void fso_copy_slope(const fso_state *s, double *out) {
+  fso_slope(s->h, s->nx, s->ny, s->xl / (s->nx - 1), s->yl / (s->ny - 1), out);
+}
+
+void fso_copy_curvature(const fso_state *s, double *out) {
+  fso_curvature(s->h, s->nx, s->ny, s->xl / (s->nx - 1), s->yl / (s->ny - 1), out);
+}
+
+void fso_copy_chi(const fso_state *s, double *out) {
+  fso_chi(s->stack, s->rec, s->a, s->length, s->nn, s->xl / (s->nx - 1), s->yl / (s->ny - 1), s->m, s->n, out);
+}
+
+void fso_get_fluxes(const fso_state *s, double out[3]) {
+  const double surf = s->xl * s->yl / (s->nx - 1) / (s->ny - 1);
+  if (s->sfd)
+    fso_fluxes_sfd(s->stack, s->rec, s->u, s->erate, s->bc, s->nn, surf, out);
+  else
+    fso_fluxes_mfd(s->mstack, s->mnrec, s->mrec, s->mwrec, s->u, s->erate, s->bc, s->nn, surf, out);
+}

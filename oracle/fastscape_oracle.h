@@ -64,6 +64,22 @@ void fso_copy_ifield(const fso_state *s, int field, int *dst);
 void fso_copy_don(const fso_state *s, int *dst);   /* (nn,8) row-major: dst[8*ij+k] */
 void fso_copy_mrec(const fso_state *s, int *mrec, double *mwrec, double *mlrec);
 
+/* read-outs after a step: FastScape_Copy_Slope / _Curvature / _Chi, FastScape_Get_Fluxes
+ * (out = tectonic, erosion, boundary flux) */
+void fso_copy_slope(const fso_state *s, double *out);
+void fso_copy_curvature(const fso_state *s, double *out);
+void fso_copy_chi(const fso_state *s, double *out);
+void fso_get_fluxes(const fso_state *s, double out[3]);
+/* the same on caller-owned arrays (order: receivers before donors; mrec/mwrec 8 slots per node) */
+void fso_slope(const double *h, int nx, int ny, double dx, double dy, double *out);
+void fso_curvature(const double *h, int nx, int ny, double dx, double dy, double *out);
+void fso_chi(const int *order, const int *rec, const double *a, const double *length, int nn, double dx,
+             double dy, double m, double n, double *out);
+void fso_fluxes_sfd(const int *order, const int *rec, const double *u, const double *erate,
+                    const unsigned char *bc, int nn, double surf, double out[3]);
+void fso_fluxes_mfd(const int *mstack, const int *mnrec, const int *mrec, const double *mwrec, const double *u,
+                    const double *erate, const unsigned char *bc, int nn, double surf, double out[3]);
+
 /* ---- individual routines, stateless, for unit tests ---- */
 void fso_bc_decode(int ibc, int nx, int ny, int *i1, int *i2, int *j1, int *j2, int *xcyc, int *ycyc,
                    unsigned char *bc);

@@ -70,7 +70,8 @@ def _c():
                                                              C.c_double, C.c_double, C.c_double, C.c_double]),
                            ("fso_copy_field", [C.c_int, _dp]), ("fso_copy_ifield", [C.c_int, _ip]),
                            ("fso_copy_don", [_ip]), ("fso_copy_mrec", [_ip, _dp, _dp]),
-                           ("fso_reset_cumulative_erosion", [])]:
+                           ("fso_reset_cumulative_erosion", []), ("fso_copy_slope", [_dp]),
+                           ("fso_copy_curvature", [_dp]), ("fso_copy_chi", [_dp]), ("fso_get_fluxes", [_dp])]:
             f = getattr(lib, name)
             f.argtypes = [C.c_void_p] + args
             f.restype = None
@@ -89,6 +90,13 @@ def _c():
         lib.fso_connect_basins_dump.argtypes = [_ip, _ip, C.c_void_p, _dp, C.c_int, C.c_int, C.c_int, _ip,
                                                 _ip, _ip, _dp]
         lib.fso_connect_basins_dump.restype = C.c_int
+        for name in ["fso_slope", "fso_curvature"]:
+            getattr(lib, name).argtypes = [_dp, C.c_int, C.c_int, C.c_double, C.c_double, _dp]
+            getattr(lib, name).restype = None
+        lib.fso_chi.argtypes = [_ip, _ip, _dp, _dp, C.c_int] + [C.c_double] * 4 + [_dp]
+        lib.fso_chi.restype = None
+        lib.fso_fluxes_sfd.argtypes = [_ip, _ip, _dp, _dp, C.c_void_p, C.c_int, C.c_double, _dp]
+        lib.fso_fluxes_sfd.restype = None
         _clib = lib
     return _clib
 
@@ -153,6 +161,18 @@ class COracle:
         out = np.empty(self.nn, np.int32)
         self.lib.fso_copy_ifield(self.h_, IFIELDS[name], _i(out))
         return out
+
+    def derived(self, name):
+        """FastScape_Copy_Slope / _Curvature / _Chi on the current state ("slope", "curvature", "chi")."""
+        out = np.empty(self.nn, np.float64)
+        getattr(self.lib, "fso_copy_" + name)(self.h_, _d(out))
+        return out
+
+    def fluxes(self):
+        """FastScape_Get_Fluxes: (tectonic, erosion, boundary)."""
+        out = np.empty(3, np.float64)
+        self.lib.fso_get_fluxes(self.h_, _d(out))
+        return tuple(out)
 
     def donors(self):
         """(ndon, don[nn,8]) with unused slots set to -1."""
@@ -246,6 +266,37 @@ def connect_basins(h, nx, ny, dx, dy, ibc=1111):
     return b0.value, cb[:nc].copy(), cn[:nc].copy(), cw[:nc].copy()
 
 
+def slope(h, nx, ny, dx, dy):
+    out = np.empty(nx * ny)
+    _c().fso_slope(_d(np.ascontiguousarray(h, np.float64)), nx, ny, dx, dy, _d(out))
+    return out
+
+
+def curvature(h, nx, ny, dx, dy):
+    out = np.empty(nx * ny)
+    _c().fso_curvature(_d(np.ascontiguousarray(h, np.float64)), nx, ny, dx, dy, _d(out))
+    return out
+
+
+def chi(order, rec, a, length, dx, dy, m, n):
+    """CopyChi's pass over any order that puts receivers before donors."""
+    out = np.empty(len(rec))
+    _c().fso_chi(_i(np.ascontiguousarray(order, np.int32)), _i(np.ascontiguousarray(rec, np.int32)),
+                 _d(np.ascontiguousarray(a, np.float64)), _d(np.ascontiguousarray(length, np.float64)), len(rec),
+                 dx, dy, m, n, _d(out))
+    return out
+
+
+def fluxes_sfd(order, rec, u, erate, bc, surf):
+    """compute_fluxes (single flow) over any order that puts receivers before donors."""
+    out = np.empty(3)
+    bc = np.ascontiguousarray(bc, np.uint8)
+    _c().fso_fluxes_sfd(_i(np.ascontiguousarray(order, np.int32)), _i(np.ascontiguousarray(rec, np.int32)),
+                        _d(np.ascontiguousarray(u, np.float64)), _d(np.ascontiguousarray(erate, np.float64)),
+                        bc.ctypes.data_as(C.c_void_p), len(rec), surf, _d(out))
+    return tuple(out)
+
+
 # ---- the reference Fortran -------------------------------------------------------------------
 
 class FortranRef:
@@ -304,6 +355,17 @@ class FortranRef:
         out = np.empty(self.nn, np.float64)
         getattr(self.lib, self._COPY[name])(_d(out))
         return out
+
+    def derived(self, name):
+        """fastscape_copy_slope / _curvature / _chi."""
+        out = np.empty(self.nn, np.float64)
+        getattr(self.lib, "fastscape_copy_" + name + "_")(_d(out))
+        return out
+
+    def fluxes(self):
+        t, e, b = C.c_double(), C.c_double(), C.c_double()
+        self.lib.fastscape_get_fluxes_(C.byref(t), C.byref(e), C.byref(b))
+        return t.value, e.value, b.value
 
     # -- internal routines, called on caller-owned arrays --
     def find_receiver(self, h, nx, ny, dx, dy, ibc=1111):

@@ -59,6 +59,8 @@ def test_host_side_argument_checks(lib):
     assert L.fsx_autotune(None, 1, None) == -1
     assert L.fsx_get_tuning(None, None) == -1
     assert L.fsx_set_tuning(None, None) == -1
+    assert L.fsx_copy_derived(None, 0, 0, None, 0) == -1
+    assert L.fsx_get_fluxes(None, 0, None) == -1
     assert L.fsx_version().startswith(b"fastscape")
 
 
@@ -81,8 +83,10 @@ def test_facade_call_order_errors():
         fs.fastscape_setup()
     with pytest.raises(RuntimeError, match="SetUp"):
         fs.fastscape_copy_h(np.zeros(4))
+    with pytest.raises(RuntimeError, match="SetUp"):
+        fs.fastscape_get_fluxes()
     for name in ["fastscape_set_advect_every_step", "fastscape_strati", "fastscape_set_marine_parameters",
-                 "fastscape_vtk", "fastscape_get_fluxes"]:
+                 "fastscape_vtk", "fastscape_copy_catchment"]:
         if hasattr(fs, name):
             with pytest.raises(NotImplementedError):
                 getattr(fs, name)()
@@ -97,5 +101,6 @@ def test_bench_refuses_several_gpus():
 def test_oracle_library_exports(oracle):
     so = C.CDLL(oracle.build_c())
     for n in ["fso_create", "fso_execute_step", "fso_local_minima", "fso_indexx", "fso_find_receiver",
-              "fso_find_donor", "fso_find_stack", "fso_copy_field"]:
+              "fso_find_donor", "fso_find_stack", "fso_copy_field", "fso_slope", "fso_curvature", "fso_chi",
+              "fso_fluxes_sfd", "fso_fluxes_mfd", "fso_get_fluxes"]:
         assert hasattr(so, n), n

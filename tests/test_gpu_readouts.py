@@ -1,0 +1,125 @@
+"""GPU read-outs (fsx_copy_derived: slope, curvature, chi; fsx_get_fluxes) against the oracle.
+
+Each case steps a batch of landscapes on the GPU and the same landscapes in the oracle, then checks
+
+* the kernels, on the GPU's own state: the oracle's slope / curvature of the GPU's h, its chi over
+  the GPU's receivers, order, area and lengths, its single-flow fluxes over the GPU's receivers,
+  order, u and erosion rate.  Slope, curvature and chi within 1e-12 relative to the field's largest
+  value (the GPU's atan / pow may differ from glibc's in the last place, chi sums them along a
+  path); fluxes within 1e-12 + 2 nn eps of the sum of the magnitudes of their terms (the GPU adds in
+  another order -- a tree reduction, and donors' shares by atomics -- and a sum of nn terms in index
+  order is itself only good to (nn-1) eps of that magnitude);
+* end to end, against the oracle's own state (which the GPU's matches to 1e-10 in h): slope,
+  curvature and chi within 1e-7 of the field's largest value, fluxes within 1e-8 of the sum of the
+  magnitudes of their terms (multiple flow is checked this way only);
+* that chi and the fluxes refuse a landscape that no step has routed.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import apply, landscape
+
+pytestmark = pytest.mark.gpu
+
+CASES = [  # nx, ny, ibc, p, g
+    (41, 37, 1111, -2.0, 0.0),
+    (64, 45, 101, -2.0, 0.0),
+    (57, 33, 1010, -2.0, 0.0),
+    (50, 61, 1111, 1.0, 1.0),
+    (47, 40, 1010, -1.0, 0.0),
+]
+
+
+def _sum_tol(nn):
+    return 1e-12 + 2.0 * nn * np.finfo(np.float64).eps
+
+
+def _close(got, want, tol):
+    scale = max(float(np.abs(want).max()), 1e-300)
+    return float(np.abs(np.asarray(got) - np.asarray(want)).max()) <= tol * scale
+
+
+def test_readouts_match_oracle(oracle):
+    import fastscapelib_fortran as fs
+    from fastscapelib_fortran import _lib
+    Ls = [landscape(nx, ny, seed=nx * ny, dx=250.0, ibc=ibc, p=p, g=g, kfsed=1e-5 if g else -1.0,
+                    sinus=(5.0, 4000.0)) for nx, ny, ibc, p, g in CASES]
+    steps = 3
+    print(f"read-outs: {len(Ls)} landscapes, {steps} steps", flush=True)
+    with fs.Batch([(L["nx"], L["ny"]) for L in Ls]) as b:
+        for k, L in enumerate(Ls):
+            apply(b, L, k)
+        b.step(steps)
+        for k, (L, (nx, ny, ibc, p, g)) in enumerate(zip(Ls, CASES)):
+            dx, dy = L["xl"] / (nx - 1), L["yl"] / (ny - 1)
+            o = oracle.COracle(nx, ny)
+            apply(o, L)
+            for _ in range(steps):
+                o.execute_step()
+            h = b.h(k)
+            sl, cu, ch = b.slope(k), b.curvature(k), b.chi(k)
+            # kernels on the GPU's own state
+            assert _close(sl, oracle.slope(h, nx, ny, dx, dy), 1e-12), k
+            assert _close(cu, oracle.curvature(h, nx, ny, dx, dy), 1e-12), k
+            rec, order, area, length = b.rec(k), b.order(k), b.field(k, _lib.FSX_AREA), b.field(k, _lib.FSX_LENGTH)
+            assert _close(ch, oracle.chi(order, rec, area, length, dx, dy, L["m"], L["n"]), 1e-12), k
+            fl = b.fluxes(k)
+            u, er = b.field(k, _lib.FSX_U), b.field(k, _lib.FSX_ERATE)
+            surf = L["xl"] * L["yl"] / (nx - 1) / (ny - 1)
+            mag = (np.abs(u).sum() * surf, np.abs(er).sum() * surf, 2.0 * np.abs(er).sum() * surf)
+            if p < -1.5:
+                _, bc = oracle.bc_decode(ibc, nx, ny)
+                want = oracle.fluxes_sfd(order, rec, u, er, bc, surf)
+                for q in range(3):
+                    assert abs(fl[q] - want[q]) <= _sum_tol(nx * ny) * mag[q], (k, q, fl, want)
+            # end to end against the oracle's own state
+            assert _close(sl, o.derived("slope"), 1e-7), k
+            assert _close(cu, o.derived("curvature"), 1e-7), k
+            assert _close(ch, o.derived("chi"), 1e-7), k
+            want = o.fluxes()
+            for q in range(3):
+                assert abs(fl[q] - want[q]) <= 1e-8 * mag[q], (k, q, fl, want)
+
+
+def test_readouts_need_a_routed_step():
+    import fastscapelib_fortran as fs
+    L = landscape(20, 20, seed=1)
+    with fs.Batch([(20, 20)]) as b:
+        apply(b, L, 0)
+        assert np.isfinite(b.slope(0)).all()  # slope and curvature need only h
+        with pytest.raises(fs.FastscapeError):
+            b.chi(0)
+        with pytest.raises(fs.FastscapeError):
+            b.fluxes(0)
+        b.step(1)
+        assert np.isfinite(b.chi(0)).all() and all(np.isfinite(b.fluxes(0)))
+
+
+def test_readouts_uniform_batch_member_full_size(oracle):
+    """A 513x513 member of the uniform batch after two steps, on the GPU's state (as above)."""
+    import fastscapelib_fortran as fs
+    from fastscapelib_fortran import _lib
+    from test_gpu_fullsize import _apply, _uniform
+    Ls = [_uniform(s) for s in range(4)]
+    print("read-outs: 4 x 513x513, 2 steps", flush=True)
+    with fs.Batch([(L["nx"], L["ny"]) for L in Ls]) as b:
+        for k, L in enumerate(Ls):
+            _apply(b, L, k)
+        b.step(2)
+        k, L, n = 3, Ls[3], 513
+        dx = L["xl"] / (n - 1)
+        h, rec, order = b.h(k), b.rec(k), b.order(k)
+        area, length = b.field(k, _lib.FSX_AREA), b.field(k, _lib.FSX_LENGTH)
+        assert _close(b.slope(k), oracle.slope(h, n, n, dx, dx), 1e-12)
+        assert _close(b.curvature(k), oracle.curvature(h, n, n, dx, dx), 1e-12)
+        assert _close(b.chi(k), oracle.chi(order, rec, area, length, dx, dx, L["m"], L["n"]), 1e-12)
+        u, er = b.field(k, _lib.FSX_U), b.field(k, _lib.FSX_ERATE)
+        _, bc = oracle.bc_decode(L["ibc"], n, n)
+        surf = dx * dx
+        want = oracle.fluxes_sfd(order, rec, u, er, bc, surf)
+        got = b.fluxes(k)
+        mag = (np.abs(u).sum() * surf, np.abs(er).sum() * surf, 2.0 * np.abs(er).sum() * surf)
+        for q in range(3):
+            assert abs(got[q] - want[q]) <= _sum_tol(n * n) * mag[q], (q, got, want)
