@@ -843,13 +843,15 @@ extern "C" int fsx_copy_field(fsx_ctx* c, int inst, int field, double* dst, int 
 extern "C" int fsx_copy_derived(fsx_ctx* c, int inst, int which, double* dst, int on_device) {
   if (int r = check_inst(c, inst)) return r;
   if (!dst) return fail(FSX_EINVAL, "null destination");
-  if (which != FSX_SLOPE && which != FSX_CURVATURE && which != FSX_CHI) return fail(FSX_EINVAL, "unknown derived field");
-  if (which == FSX_CHI && !c->routed[inst]) return fail(FSX_EINVAL, "chi needs the routing of a step first");
+  if (which < FSX_SLOPE || which > FSX_CATCHMENT) return fail(FSX_EINVAL, "unknown derived field");
+  if (which >= FSX_CHI && !c->routed[inst]) return fail(FSX_EINVAL, "chi and catchment need the routing of a step first");
   if (int r = upload_params(c)) return r;
   const InstParams& P = c->hp[inst];
   double* tmp = c->S.zint + P.off;  // scratch outside a step
   if (which == FSX_CHI)
     launch_chi(c->S, inst, P.dx * P.dy * 10.0, P.m / P.n, tmp, c->stream);
+  else if (which == FSX_CATCHMENT)
+    launch_catchment(c->S, inst, P.nn, tmp, c->stream);
   else
     launch_terrain(c->S, inst, P.nn, which == FSX_SLOPE ? 0 : 1, 45.0 / std::atan(1.0), tmp, c->stream);
   HIPCHK(hipGetLastError());

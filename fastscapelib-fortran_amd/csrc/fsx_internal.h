@@ -235,6 +235,7 @@ void launch_mfd_spl(DevState& S, hipStream_t st);
 // read-outs of one landscape (outside a step)
 void launch_terrain(DevState& S, int inst, int nn, int which, double con, double* out, hipStream_t st);
 void launch_chi(DevState& S, int inst, double a0, double mn, double* out, hipStream_t st);
+void launch_catchment(DevState& S, int inst, int nn, double* out, hipStream_t st);
 void launch_fluxes(DevState& S, int inst, double* flux, double* sums, hipStream_t st);
 int sweep_set_lds(int smem_max);
 int basins_set_lds(int smem_max);

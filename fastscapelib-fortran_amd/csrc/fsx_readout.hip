@@ -1,6 +1,6 @@
-// fsx_readout.hip -- read-outs on the state a step leaves behind: slope, curvature, chi and the three
-// fluxes (FastScape_Copy_Slope / _Curvature / _Chi and FastScape_Get_Fluxes,
-// src/FastScape_api.f90:379-423,762-774).  They run on one landscape of a batch, on the context's
+// fsx_readout.hip -- read-outs on the state a step leaves behind: slope, curvature, chi, catchment and
+// the three fluxes (FastScape_Copy_Slope / _Curvature / _Chi / _Catchment and FastScape_Get_Fluxes,
+// src/FastScape_api.f90:379-439,762-774).  They run on one landscape of a batch, on the context's
 // stream, outside any step.
 #include <cfloat>
 
@@ -64,6 +64,23 @@ __global__ __launch_bounds__(FSX_WG) void k_chi(const InstParams* __restrict__ P
     }
     __syncthreads();
   }
+}
+
+// CopyCatchment (src/FastScape_ctx.f90:297-307): the reference's catch is catch0 of the root of the
+// node's tree in the receivers before closed-basin rerouting (find_stack propagates it,
+// src/FlowRouting.f90:408,437), with catch0 from random_number at SetUp (:94).  Here catch0 of a node
+// is (node + 1) / nn, so a catchment reads as its outlet; which nodes share a value is the reference's.
+// The roots come from the basin labels of the step's rerouting (signed: open basins -(b+1)) and the
+// outlet of each basin.
+__global__ __launch_bounds__(256) void k_catchment(const InstParams* __restrict__ PP, int inst,
+                                                   const int* __restrict__ basin, const int* __restrict__ outlet,
+                                                   double* __restrict__ out) {
+  const InstParams& P = PP[inst];
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= P.nn) return;
+  const int bv = basin[P.off + v];
+  const int b = bv >= 0 ? bv : -bv - 1;
+  out[v] = (double)(outlet[P.off + b] + 1) / (double)P.nn;
 }
 
 __device__ __forceinline__ double block_sum(double x, double* red) {
@@ -147,6 +164,10 @@ __global__ __launch_bounds__(FSX_WG) void k_fluxes(const InstParams* __restrict_
 
 void launch_terrain(DevState& S, int inst, int nn, int which, double con, double* out, hipStream_t st) {
   k_terrain<<<(nn + 255) / 256, 256, 0, st>>>(S.P, inst, S.h, which, con, out);
+}
+
+void launch_catchment(DevState& S, int inst, int nn, double* out, hipStream_t st) {
+  k_catchment<<<(nn + 255) / 256, 256, 0, st>>>(S.P, inst, S.basin, S.outlet, out);
 }
 
 void launch_chi(DevState& S, int inst, double a0, double mn, double* out, hipStream_t st) {

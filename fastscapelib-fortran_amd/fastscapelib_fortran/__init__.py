@@ -12,9 +12,9 @@ The free functions keep the reference's call surface and f2py argument conventio
 Documented divergences: call-order and size errors raise ``RuntimeError``/``ValueError`` where the
 reference STOPs the process; ``fastscape_setup`` starts from h = 0 rather than Fortran random
 numbers (every run sets h with ``fastscape_init_h``); routines outside the per-step update
-(advection, marine, stratigraphy, VTK, flexure, the catchment read-out) raise
-``NotImplementedError``.  The slope, curvature and chi read-outs and the three fluxes are computed on
-the GPU from the resident state.
+(advection, marine, stratigraphy, VTK, flexure) raise ``NotImplementedError``.  The slope,
+curvature, chi and catchment read-outs and the three fluxes are computed on the GPU from the resident
+state; the catchment labels a catchment (outlet + 1) / nn where the reference draws a random number.
 
 ``Batch`` runs many independent landscapes of different sizes in one set of kernel launches.
 """
@@ -32,7 +32,8 @@ __all__ = [
     "fastscape_init_h", "fastscape_set_erosional_parameters", "fastscape_set_u", "fastscape_set_bc",
     "fastscape_execute_step", "fastscape_copy_h", "fastscape_copy_basement", "fastscape_copy_total_erosion",
     "fastscape_copy_erosion_rate", "fastscape_copy_drainage_area", "fastscape_copy_lake_depth",
-    "fastscape_copy_slope", "fastscape_copy_curvature", "fastscape_copy_chi", "fastscape_get_fluxes",
+    "fastscape_copy_slope", "fastscape_copy_curvature", "fastscape_copy_chi", "fastscape_copy_catchment",
+    "fastscape_get_fluxes",
     "fastscape_get_step", "fastscape_get_sizes", "fastscape_destroy", "fastscape_set_h", "fastscape_set_basement",
     "fastscape_set_precip", "fastscape_set_all_layers", "fastscape_reset_cumulative_erosion", "fastscape_view",
     "fastscape_debug", "Batch", "Lanes", "Pool", "FastscapeError", "indexx",
@@ -201,6 +202,7 @@ class Batch:
     def slope(self, k): return self.derived(k, _lib.FSX_SLOPE)
     def curvature(self, k): return self.derived(k, _lib.FSX_CURVATURE)
     def chi(self, k): return self.derived(k, _lib.FSX_CHI)
+    def catchment(self, k): return self.derived(k, _lib.FSX_CATCHMENT)
 
     def fluxes(self, k):
         """(tectonic, erosion, boundary) flux of landscape k (FastScape_Get_Fluxes)."""
@@ -273,7 +275,7 @@ class Lanes:
     _PER = {"set_xl_yl", "set_dt", "set_bc", "set_newton_tol", "set_active", "set_erosional_parameters",
             "init_h", "set_field", "set_u", "set_h", "set_basement", "set_precip", "set_all_layers",
             "reset_cumulative_erosion", "field", "ifield", "h", "rec", "donors", "order", "mrec", "morder",
-            "get_step", "stats", "derived", "slope", "curvature", "chi", "fluxes"}
+            "get_step", "stats", "derived", "slope", "curvature", "chi", "catchment", "fluxes"}
     TUNING = Batch.TUNING
 
     @staticmethod
@@ -521,6 +523,12 @@ def fastscape_copy_chi(c):
     _derived_into(_lib.FSX_CHI, c)
 
 
+def fastscape_copy_catchment(c):
+    """FastScape_Copy_Catchment (src/FastScape_api.f90:427-439): (outlet + 1) / nn of each node's tree
+    before closed-basin rerouting; the reference gives that tree a random number instead."""
+    _derived_into(_lib.FSX_CATCHMENT, c)
+
+
 def fastscape_get_fluxes():
     """FastScape_Get_Fluxes (src/FastScape_api.f90:762-774): (tectonic, erosion, boundary) flux, as
     f2py returns the three intent(out) scalars."""
@@ -569,5 +577,5 @@ def _outside(name):
 
 
 for _n in ["fastscape_set_v", "fastscape_set_marine_parameters", "fastscape_strati", "fastscape_vtk",
-           "fastscape_init_f", "fastscape_copy_f", "fastscape_copy_catchment"]:
+           "fastscape_init_f", "fastscape_copy_f"]:
     globals()[_n] = _outside(_n)

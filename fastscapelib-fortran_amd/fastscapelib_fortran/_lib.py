@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libfastscap
 FSX_H, FSX_B, FSX_U, FSX_PRECIP, FSX_ETOT, FSX_ERATE, FSX_AREA, FSX_LAKE_DEPTH, FSX_SEDFLUX, FSX_LENGTH, \
     FSX_HWATER, FSX_KF, FSX_KD = range(13)
 FSX_REC, FSX_NDON, FSX_DON, FSX_ORDER, FSX_MREC, FSX_MNREC, FSX_MSTACK = range(7)
-FSX_SLOPE, FSX_CURVATURE, FSX_CHI = range(3)  # fsx_copy_derived
+FSX_SLOPE, FSX_CURVATURE, FSX_CHI, FSX_CATCHMENT = range(4)  # fsx_copy_derived
 
 # phase timers, in the library's order (include/fastscape_hip.h, fsx_get_timers)
 PHASES = ["uplift", "receivers", "donors", "lm_labels", "lm_basins", "lm_walk", "lm_edges", "lm_sort", "lm_mst",

@@ -53,6 +53,9 @@ enum {
   FSX_SLOPE = 0,     /* slope in degrees (slope, src/TerrainDerivatives.f90:3-36)     Copy_Slope     */
   FSX_CURVATURE = 1, /* curvature (curvature, src/TerrainDerivatives.f90:40-74)        Copy_Curvature */
   FSX_CHI = 2,       /* chi (CopyChi, src/FastScape_ctx.f90:226-247)                   Copy_Chi       */
+  FSX_CATCHMENT = 3, /* catchment label (CopyCatchment, :297-307): (outlet + 1) / nn of the node's tree
+                        before closed-basin rerouting, where the reference takes a random catch0 of
+                        that outlet; nodes share a value exactly as in the reference  Copy_Catchment */
 };
 /* integer field ids of fsx_copy_ifield */
 enum {
@@ -138,9 +141,9 @@ int fsx_copy_field(fsx_ctx *ctx, int inst, int field, double *dst, int on_device
  * multiple flow direction (p >= -1.5). */
 int fsx_copy_ifield(fsx_ctx *ctx, int inst, int field, int *dst, int on_device);
 
-/* FastScape_Copy_Slope / Copy_Curvature / Copy_Chi (src/FastScape_api.f90:379-423), computed on the
- * device from the current state of one landscape into dst (nn doubles).  Chi follows the receivers,
- * drainage area and flow lengths of the last step (FSX_EINVAL before the first step or fsx_route). */
+/* FastScape_Copy_Slope / Copy_Curvature / Copy_Chi / Copy_Catchment (src/FastScape_api.f90:379-439),
+ * computed on the device from the current state of one landscape into dst (nn doubles).  Chi and the
+ * catchment follow the routing of the last step (FSX_EINVAL before the first step or fsx_route). */
 int fsx_copy_derived(fsx_ctx *ctx, int inst, int which, double *dst, int on_device);
 /* FastScape_Get_Fluxes (src/FastScape_api.f90:762-774; compute_fluxes, src/FastScape_ctx.f90:843-886):
  * out3 = tectonic, erosion and boundary flux of one landscape's current state, computed on the device

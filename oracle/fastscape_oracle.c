@@ -1182,6 +1182,24 @@ void fso_fluxes_mfd(const int *mstack, const int *mnrec, const int *mrec, const 
   free(flux);
 }
 
+/* CopyCatchment, src/FastScape_ctx.f90:297-307: catch0 of the root of each node's tree, find_stack
+ * propagating it from the root (src/FlowRouting.f90:408,437) over the receivers before LocalMinima;
+ * catch0 comes from random_number at SetUp (src/FastScape_ctx.f90:94).  Here catch0 of a node is
+ * (node + 1) / nn, so only which nodes share a value is comparable with the reference.            */
+void fso_catchment(const int *rec, int nn, double *out) {
+  int *root = malloc(sizeof(int) * (size_t)nn);
+  for (int v = 0; v < nn; v++) root[v] = -1;
+  for (int v = 0; v < nn; v++) {
+    int r = v;
+    while (root[r] < 0 && rec[r] != r) r = rec[r];
+    const int top = root[r] >= 0 ? root[r] : r;
+    for (int w = v; root[w] < 0 && w != r; w = rec[w]) root[w] = top;
+    root[r] = top;
+  }
+  for (int v = 0; v < nn; v++) out[v] = (double)(root[v] + 1) / (double)nn;
+  free(root);
+}
+
 void fso_copy_slope(const fso_state *s, double *out) {
   fso_slope(s->h, s->nx, s->ny, s->xl / (s->nx - 1), s->yl / (s->ny - 1), out);
 }

@@ -75,6 +75,7 @@ void fso_slope(const double *h, int nx, int ny, double dx, double dy, double *ou
 void fso_curvature(const double *h, int nx, int ny, double dx, double dy, double *out);
 void fso_chi(const int *order, const int *rec, const double *a, const double *length, int nn, double dx,
              double dy, double m, double n, double *out);
+void fso_catchment(const int *rec, int nn, double *out); /* rec: receivers before LocalMinima */
 void fso_fluxes_sfd(const int *order, const int *rec, const double *u, const double *erate,
                     const unsigned char *bc, int nn, double surf, double out[3]);
 void fso_fluxes_mfd(const int *mstack, const int *mnrec, const int *mrec, const double *mwrec, const double *u,

@@ -95,6 +95,8 @@ def _c():
             getattr(lib, name).restype = None
         lib.fso_chi.argtypes = [_ip, _ip, _dp, _dp, C.c_int] + [C.c_double] * 4 + [_dp]
         lib.fso_chi.restype = None
+        lib.fso_catchment.argtypes = [_ip, C.c_int, _dp]
+        lib.fso_catchment.restype = None
         lib.fso_fluxes_sfd.argtypes = [_ip, _ip, _dp, _dp, C.c_void_p, C.c_int, C.c_double, _dp]
         lib.fso_fluxes_sfd.restype = None
         _clib = lib
@@ -284,6 +286,13 @@ def chi(order, rec, a, length, dx, dy, m, n):
     _c().fso_chi(_i(np.ascontiguousarray(order, np.int32)), _i(np.ascontiguousarray(rec, np.int32)),
                  _d(np.ascontiguousarray(a, np.float64)), _d(np.ascontiguousarray(length, np.float64)), len(rec),
                  dx, dy, m, n, _d(out))
+    return out
+
+
+def catchment(rec):
+    """CopyCatchment's labels from the receivers before closed-basin rerouting: (root + 1) / nn."""
+    out = np.empty(len(rec))
+    _c().fso_catchment(_i(np.ascontiguousarray(rec, np.int32)), len(rec), _d(out))
     return out
 
 

@@ -86,7 +86,7 @@ def test_facade_call_order_errors():
     with pytest.raises(RuntimeError, match="SetUp"):
         fs.fastscape_get_fluxes()
     for name in ["fastscape_set_advect_every_step", "fastscape_strati", "fastscape_set_marine_parameters",
-                 "fastscape_vtk", "fastscape_copy_catchment"]:
+                 "fastscape_vtk", "fastscape_copy_f"]:
         if hasattr(fs, name):
             with pytest.raises(NotImplementedError):
                 getattr(fs, name)()
@@ -102,5 +102,5 @@ def test_oracle_library_exports(oracle):
     so = C.CDLL(oracle.build_c())
     for n in ["fso_create", "fso_execute_step", "fso_local_minima", "fso_indexx", "fso_find_receiver",
               "fso_find_donor", "fso_find_stack", "fso_copy_field", "fso_slope", "fso_curvature", "fso_chi",
-              "fso_fluxes_sfd", "fso_fluxes_mfd", "fso_get_fluxes"]:
+              "fso_fluxes_sfd", "fso_fluxes_mfd", "fso_get_fluxes", "fso_catchment"]:
         assert hasattr(so, n), n

@@ -6,13 +6,14 @@ Each case steps a batch of landscapes on the GPU and the same landscapes in the 
   the GPU's receivers, order, area and lengths, its single-flow fluxes over the GPU's receivers,
   order, u and erosion rate.  Slope, curvature and chi within 1e-12 relative to the field's largest
   value (the GPU's atan / pow may differ from glibc's in the last place, chi sums them along a
-  path); fluxes within 1e-12 + 2 nn eps of the sum of the magnitudes of their terms (the GPU adds in
+  path); the catchment exactly, against the oracle's labels of the receivers that the last step
+  found before closed-basin rerouting (find_receiver on the h it started from plus u dt); fluxes within 1e-12 + 2 nn eps of the sum of the magnitudes of their terms (the GPU adds in
   another order -- a tree reduction, and donors' shares by atomics -- and a sum of nn terms in index
   order is itself only good to (nn-1) eps of that magnitude);
 * end to end, against the oracle's own state (which the GPU's matches to 1e-10 in h): slope,
   curvature and chi within 1e-7 of the field's largest value, fluxes within 1e-8 of the sum of the
   magnitudes of their terms (multiple flow is checked this way only);
-* that chi and the fluxes refuse a landscape that no step has routed.
+* that chi, the catchment and the fluxes refuse a landscape that no step has routed.
 """
 from __future__ import annotations
 
@@ -51,7 +52,9 @@ def test_readouts_match_oracle(oracle):
     with fs.Batch([(L["nx"], L["ny"]) for L in Ls]) as b:
         for k, L in enumerate(Ls):
             apply(b, L, k)
-        b.step(steps)
+        b.step(steps - 1)
+        h_prev = [b.h(k) for k in range(len(Ls))]
+        b.step(1)
         for k, (L, (nx, ny, ibc, p, g)) in enumerate(zip(Ls, CASES)):
             dx, dy = L["xl"] / (nx - 1), L["yl"] / (ny - 1)
             o = oracle.COracle(nx, ny)
@@ -65,6 +68,9 @@ def test_readouts_match_oracle(oracle):
             assert _close(cu, oracle.curvature(h, nx, ny, dx, dy), 1e-12), k
             rec, order, area, length = b.rec(k), b.order(k), b.field(k, _lib.FSX_AREA), b.field(k, _lib.FSX_LENGTH)
             assert _close(ch, oracle.chi(order, rec, area, length, dx, dy, L["m"], L["n"]), 1e-12), k
+            # catchment: the trees of the receivers the last step found before rerouting (on h + u dt)
+            rec_pre, _ = oracle.find_receiver(h_prev[k] + L["u"] * L["dt"], nx, ny, dx, dy, ibc)
+            assert np.array_equal(b.catchment(k), oracle.catchment(rec_pre)), k
             fl = b.fluxes(k)
             u, er = b.field(k, _lib.FSX_U), b.field(k, _lib.FSX_ERATE)
             surf = L["xl"] * L["yl"] / (nx - 1) / (ny - 1)
@@ -93,6 +99,8 @@ def test_readouts_need_a_routed_step():
             b.chi(0)
         with pytest.raises(fs.FastscapeError):
             b.fluxes(0)
+        with pytest.raises(fs.FastscapeError):
+            b.catchment(0)
         b.step(1)
         assert np.isfinite(b.chi(0)).all() and all(np.isfinite(b.fluxes(0)))
 

@@ -3,7 +3,8 @@ fso_copy_field) against
 
 1. the unmodified reference compiled from its own sources (oracle/_ref), after steps of the SFD and
    MFD paths under several boundary conditions: bit for bit (skipped when the reference build is
-   absent); and
+   absent), and the catchment's grouping (the reference labels each tree of the receivers found
+   before closed-basin rerouting with a random number, the oracle with (outlet + 1) / nn); and
 2. what the mathematics fixes: the slope of a plane (atan of the gradient's norm, in degrees), the
    curvature of a plane (0) and of a parabola in x (c/2 / (1 + (2cx)^2)^(3/2)), chi on a tree with a
    constant area (half the depth when (a0/a)^(m/n) = 1/2 and lengths are 1, restarting from 0 below
@@ -53,6 +54,33 @@ def test_readouts_match_reference_live(oracle, nx, ny, p, ibc, g):
     for name in ["slope", "curvature", "chi"]:
         assert np.array_equal(o.derived(name), f.derived(name)), name
     assert o.fluxes() == f.fluxes()
+
+
+@pytest.mark.parametrize("ibc,p", [(1111, -2.0), (1010, -2.0), (101, 1.0)])
+def test_catchment_groups_match_reference_live(oracle, ibc, p):
+    f = _ref()
+    nx, ny, dt = 37, 29, 1e3
+    f.setup(nx, ny)
+    rng = np.random.default_rng(ibc + 7)
+    h = rng.random(nx * ny) * 10.0
+    u = np.full(nx * ny, 1e-3)
+    f.set_xl_yl(100.0 * (nx - 1), 90.0 * (ny - 1))
+    f.set_dt(dt)
+    f.init_h(h)
+    f.set_erosional_parameters(np.full(nx * ny, 2e-5), -1.0, 0.4, 1.0, np.full(nx * ny, 1e-2), -1.0, 0.0, 0.0, p)
+    f.set_u(u)
+    f.set_bc(ibc)
+    f.execute_step()
+    h1 = f.field("h")
+    f.execute_step()
+    cat = f.derived("catchment")
+    rec_pre, _ = oracle.find_receiver(h1 + u * dt, nx, ny, 100.0, 90.0, ibc)
+    lab = oracle.catchment(rec_pre)
+    _, ia = np.unique(lab, return_inverse=True)
+    _, ib = np.unique(cat, return_inverse=True)
+    assert len(set(zip(ia.tolist(), ib.tolist()))) == ia.max() + 1 == ib.max() + 1  # the same partition
+    roots = rec_pre == np.arange(nx * ny)
+    assert np.array_equal(lab[roots], (np.flatnonzero(roots) + 1) / (nx * ny))
 
 
 def test_slope_and_curvature_of_a_plane(oracle):
