@@ -55,6 +55,7 @@ struct fsx_ctx {
   hipStream_t stream = nullptr;
   bool dirty = true;
   bool any_sfd = false, any_mfd = false;  // which flow-routing paths the active instances use
+  bool any_advect = false, g_adv = false;  // some active instance advects; as captured
   // one step captured as a HIP graph and replayed; rebuilt when the launch sequence changes
   bool use_graph = true, graph_valid = false, g_sfd = false, g_mfd = false;
   hipGraphExec_t gexec = nullptr;
@@ -182,7 +183,9 @@ static int check_inst(fsx_ctx* c, int inst) {
 static int upload_params(fsx_ctx* c) {
   if (!c->dirty) return 0;
   c->any_sfd = c->any_mfd = false;
+  c->any_advect = false;
   for (const InstParams& P : c->hp) {
+    if (P.active && P.run_advect) c->any_advect = true;
     if (!P.active || !P.run_spl) continue;
     if (P.sfd) c->any_sfd = true; else c->any_mfd = true;
   }
@@ -487,6 +490,16 @@ static int mfd_alloc(fsx_ctx* c) {
   return rc;
 }
 
+static int advect_alloc(fsx_ctx* c) {
+  DevState& S = c->S;
+  if (S.vx) return 0;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  int rc = dalloc(c, &S.vx, S.ntot);
+  if (!rc) rc = dalloc(c, &S.vy, S.ntot);
+  c->graph_valid = false;
+  return rc;
+}
+
 static int put(fsx_ctx* c, double* dst, const double* src, int n, int on_device) {
   HIPCHK(hipMemcpyAsync(dst, src, sizeof(double) * (size_t)n, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                         c->stream));
@@ -561,6 +574,18 @@ extern "C" int fsx_set_field(fsx_ctx* c, int inst, int field, const double* src,
     default: return fail(FSX_EINVAL, "field cannot be set");
   }
   return put(c, dst + P.off, src, P.nn, on_device);
+}
+
+extern "C" int fsx_set_v(fsx_ctx* c, int inst, const double* vx, const double* vy, int on_device) {
+  if (int r = check_inst(c, inst)) return r;
+  if (!vx || !vy) return fail(FSX_EINVAL, "vx and vy are required");
+  if (int r = advect_alloc(c)) return r;
+  InstParams& P = c->hp[inst];
+  if (int r = put(c, c->S.vx + P.off, vx, P.nn, on_device)) return r;
+  if (int r = put(c, c->S.vy + P.off, vy, P.nn, on_device)) return r;
+  P.run_advect = 1;
+  c->dirty = true;
+  return 0;
 }
 
 __global__ void k_add_layers(double* __restrict__ h, double* __restrict__ b, const double* __restrict__ dh, int n) {
@@ -648,6 +673,7 @@ static void enqueue_route_only(fsx_ctx* c, hipStream_t st) {
 
 static void enqueue_step(fsx_ctx* c, hipStream_t st) {
   DevState& S = c->S;
+  if (c->any_advect) PHASE(PH_UPLIFT, launch_advect(S, st));  // Advect precedes Uplift (api.f90:209-221)
   PHASE(PH_UPLIFT, launch_uplift(S, st));
   enqueue_routing(c, st);
   PHASE(PH_SPL_PREP, launch_spl_prep(S, st));
@@ -712,6 +738,7 @@ static int build_graphs(fsx_ctx* c, hipStream_t st) {
   c->graph_valid = true;
   c->g_sfd = c->any_sfd;
   c->g_mfd = c->any_mfd;
+  c->g_adv = c->any_advect;
   c->g_timed = c->timed_phase;
   return 0;
 }
@@ -726,7 +753,8 @@ extern "C" int fsx_execute_steps(fsx_ctx* c, int nsteps, void* stream, int sync)
   if (c->use_graph && !c->prof && nsteps > 0) {
     // the step's launch sequence depends on the layout and on which routing paths are in use; the
     // kernels read every other parameter from device memory, so one capture serves until those change
-    if (!c->graph_valid || c->g_sfd != c->any_sfd || c->g_mfd != c->any_mfd || c->g_timed != c->timed_phase) {
+    if (!c->graph_valid || c->g_sfd != c->any_sfd || c->g_mfd != c->any_mfd || c->g_adv != c->any_advect ||
+        c->g_timed != c->timed_phase) {
       if (int r = build_graphs(c, st)) return r;
     }
     if (c->gexec) {

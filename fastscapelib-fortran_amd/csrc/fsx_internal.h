@@ -34,6 +34,7 @@ struct InstParams {
   int xcyc, ycyc;            // cyclic flags (SetBC)
   int fix_left, fix_right, fix_bottom, fix_top;  // BC digits == '1'
   int run_uplift, run_spl, run_diff, sfd;
+  int run_advect;             // a velocity field was set (SetV)
   int active;
   int aux;                   // offset of the nn+1-entry per-landscape arrays (level starts, groups)
   int c0, c1;                // chunk-table range of the landscape
@@ -206,6 +207,7 @@ __device__ __forceinline__ double dval(unsigned long long k) {
 // ------------------------------------------------------------------------------------------
 // Host-side launch wrappers (defined in the .hip files).
 struct DevState;
+void launch_advect(DevState& S, hipStream_t st);  // needs S.vx/S.vy
 void launch_uplift(DevState& S, hipStream_t st);
 void launch_receivers(DevState& S, hipStream_t st);
 void launch_donors(DevState& S, hipStream_t st);  // donor mask of every node

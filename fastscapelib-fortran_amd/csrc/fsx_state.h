@@ -26,6 +26,7 @@ struct DevState {
   double *zintp = nullptr, *gam = nullptr, *dh = nullptr, *hp = nullptr, *lwv = nullptr, *lsed = nullptr;
   double *kfint = nullptr, *kdint = nullptr, *g = nullptr, *pexp = nullptr, *elev = nullptr;
   double *adx = nullptr, *ady = nullptr, *adg = nullptr, *adf = nullptr;  // ADI couplings, diagonal, rhs
+  double *vx = nullptr, *vy = nullptr;  // velocity field of the advection (allocated when first set)
   int *rec = nullptr, *basin = nullptr, *rootrank = nullptr, *order = nullptr;
   int *lstart = nullptr, *lsill = nullptr;
   uint8_t *dmask = nullptr, *bc = nullptr;

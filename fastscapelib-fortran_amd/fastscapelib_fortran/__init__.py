@@ -12,7 +12,8 @@ The free functions keep the reference's call surface and f2py argument conventio
 Documented divergences: call-order and size errors raise ``RuntimeError``/``ValueError`` where the
 reference STOPs the process; ``fastscape_setup`` starts from h = 0 rather than Fortran random
 numbers (every run sets h with ``fastscape_init_h``); routines outside the per-step update
-(advection, marine, stratigraphy, VTK, flexure) raise ``NotImplementedError``.  The slope,
+(marine, stratigraphy, VTK, flexure) raise ``NotImplementedError``; ``fastscape_set_v`` turns on the
+advection of h, b and etot (Advect) at the start of every step, on the GPU.  The slope,
 curvature, chi and catchment read-outs and the three fluxes are computed on the GPU from the resident
 state; the catchment labels a catchment (outlet + 1) / nn where the reference draws a random number.
 
@@ -33,7 +34,7 @@ __all__ = [
     "fastscape_execute_step", "fastscape_copy_h", "fastscape_copy_basement", "fastscape_copy_total_erosion",
     "fastscape_copy_erosion_rate", "fastscape_copy_drainage_area", "fastscape_copy_lake_depth",
     "fastscape_copy_slope", "fastscape_copy_curvature", "fastscape_copy_chi", "fastscape_copy_catchment",
-    "fastscape_get_fluxes",
+    "fastscape_get_fluxes", "fastscape_set_v",
     "fastscape_get_step", "fastscape_get_sizes", "fastscape_destroy", "fastscape_set_h", "fastscape_set_basement",
     "fastscape_set_precip", "fastscape_set_all_layers", "fastscape_reset_cumulative_erosion", "fastscape_view",
     "fastscape_debug", "Batch", "Lanes", "Pool", "FastscapeError", "indexx",
@@ -117,6 +118,16 @@ class Batch:
     def init_h(self, k, h):
         p, d, _a = self._src(k, h, "h")
         check(lib.fsx_init_h(self._h, k, p, d), "init_h")
+
+    def set_v(self, k, vx, vy):
+        """Velocity field of landscape k (FastScape_Set_V): advection of h, b and etot starts each step."""
+        p1, d1, a1 = self._src(k, vx, "vx")
+        p2, d2, a2 = self._src(k, vy, "vy")
+        if d1 != d2:
+            a1 = host_f64(vx.cpu().numpy() if d1 else vx, self.nn[k], "vx")
+            a2 = host_f64(vy.cpu().numpy() if d2 else vy, self.nn[k], "vy")
+            p1, p2, d1 = ptr(a1), ptr(a2), 0
+        check(lib.fsx_set_v(self._h, k, p1, p2, d1), "set_v")
 
     def set_field(self, k, field, x, name="field"):
         p, d, _a = self._src(k, x, name)
@@ -273,7 +284,7 @@ class Lanes:
     """
 
     _PER = {"set_xl_yl", "set_dt", "set_bc", "set_newton_tol", "set_active", "set_erosional_parameters",
-            "init_h", "set_field", "set_u", "set_h", "set_basement", "set_precip", "set_all_layers",
+            "init_h", "set_field", "set_u", "set_v", "set_h", "set_basement", "set_precip", "set_all_layers",
             "reset_cumulative_erosion", "field", "ifield", "h", "rec", "donors", "order", "mrec", "morder",
             "get_step", "stats", "derived", "slope", "curvature", "chi", "catchment", "fluxes"}
     TUNING = Batch.TUNING
@@ -465,6 +476,7 @@ def fastscape_set_dt(dt): _need().set_dt(0, dt)
 def fastscape_set_bc(ibc): _need().set_bc(0, ibc)
 def fastscape_init_h(h): _need().init_h(0, h)
 def fastscape_set_u(u): _need().set_u(0, u)
+def fastscape_set_v(ux, uy): _need().set_v(0, ux, uy)
 def fastscape_set_h(h): _need().set_h(0, h)
 def fastscape_set_basement(b): _need().set_basement(0, b)
 def fastscape_set_precip(p): _need().set_precip(0, p)
@@ -576,6 +588,6 @@ def _outside(name):
     return f
 
 
-for _n in ["fastscape_set_v", "fastscape_set_marine_parameters", "fastscape_strati", "fastscape_vtk",
+for _n in ["fastscape_set_marine_parameters", "fastscape_strati", "fastscape_vtk",
            "fastscape_init_f", "fastscape_copy_f"]:
     globals()[_n] = _outside(_n)

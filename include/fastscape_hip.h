@@ -114,6 +114,10 @@ int fsx_init_h(fsx_ctx *ctx, int inst, const double *h, int on_device);
 /* FastScape_Set_U / Set_H / Set_Basement / Set_Precip (src/FastScape_api.f90:618-630,664-676,
  * 696-724): field one of FSX_U, FSX_H, FSX_B, FSX_PRECIP.  Setting FSX_U switches uplift on. */
 int fsx_set_field(fsx_ctx *ctx, int inst, int field, const double *src, int on_device);
+/* FastScape_Set_V (src/FastScape_api.f90:634-646; SetV src/FastScape_ctx.f90:616-632): the velocity
+ * field (vx, vy: nn values each); switches on the advection of h, b and etot at the start of every
+ * step (Advect, src/Advect.f90:1-103). */
+int fsx_set_v(fsx_ctx *ctx, int inst, const double *vx, const double *vy, int on_device);
 /* FastScape_Set_All_Layers (src/FastScape_api.f90:680-692): h += dh, b += dh */
 int fsx_set_all_layers(fsx_ctx *ctx, int inst, const double *dh, int on_device);
 /* FastScape_Reset_Cumulative_Erosion (src/FastScape_api.f90:650-660) */

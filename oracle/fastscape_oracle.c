@@ -29,12 +29,12 @@ struct fso_state {
   int ibc, i1, i2, j1, j2, xcyc, ycyc;
   unsigned char *bc;
   double *h, *u, *b, *etot, *erate, *a, *length, *precip, *kf, *kd, *sedflux, *lake_depth, *hwater,
-      *g, *p_mfd_exp;
+      *g, *p_mfd_exp, *vx, *vy;
   int *rec, *ndon, *don, *stack;
   int *mrec, *mnrec, *mstack;
   double *mwrec, *mlrec;
   double kfsed, m, n, kdsed, g1, g2, p, sealevel, newton_tol;
-  int sfd, run_spl, run_uplift, run_diffusion;
+  int sfd, run_spl, run_uplift, run_diffusion, run_advect;
   int step;
   fso_stats st;
 };
@@ -839,6 +839,50 @@ static int tridag(const double *a, const double *b, const double *c, const doubl
   return 0;
 }
 
+/* Advect, src/Advect.f90:1-103: implicit upwind advection of h, b and etot, every row (x) and then
+ * every column (y) solved with tridag; diagonal 1 + |v| dt/dx, the upwind neighbour's coefficient
+ * -|v| dt/dx (sub-diagonal for v > 0, super-diagonal for v < 0), end rows of the system fixed;
+ * finally b = min(b, h).                                                                        */
+static int advect_lines(fso_state *s, int xdir) {
+  const int nx = s->nx, ny = s->ny;
+  const int n = xdir ? nx : ny, nl = xdir ? ny : nx, st = xdir ? 1 : nx;
+  const double d = xdir ? s->xl / (nx - 1) : s->yl / (ny - 1);
+  const double *v = xdir ? s->vx : s->vy;
+  double *diag = malloc(sizeof(double) * n), *sup = malloc(sizeof(double) * n), *inf = malloc(sizeof(double) * n);
+  double *rhs = malloc(sizeof(double) * n), *res = malloc(sizeof(double) * n), *gam = malloc(sizeof(double) * n);
+  double *fld[3] = {s->h, s->b, s->etot};
+  int rc = 0;
+  for (int l = 0; l < nl && !rc; l++) {
+    const int base = xdir ? l * nx : l;
+    for (int k = 0; k < n; k++) {
+      const double vk = v[base + k * st];
+      diag[k] = 1.0; sup[k] = 0.0; inf[k] = 0.0;
+      if (vk > 0.0) {
+        diag[k] = 1.0 + vk * s->dt / d;
+        inf[k] = -(vk * s->dt / d);
+      } else if (vk < 0.0) {
+        diag[k] = 1.0 - vk * s->dt / d;
+        sup[k] = vk * s->dt / d;
+      }
+    }
+    sup[0] = 0.0; diag[0] = 1.0; diag[n - 1] = 1.0; inf[n - 1] = 0.0;
+    for (int f = 0; f < 3 && !rc; f++) {
+      for (int k = 0; k < n; k++) rhs[k] = fld[f][base + k * st];
+      rc = tridag(inf, diag, sup, rhs, res, gam, n);
+      for (int k = 0; k < n; k++) fld[f][base + k * st] = res[k];
+    }
+  }
+  free(diag); free(sup); free(inf); free(rhs); free(res); free(gam);
+  return rc;
+}
+
+static int advect(fso_state *s) {
+  int rc = advect_lines(s, 1);
+  if (!rc) rc = advect_lines(s, 0);
+  for (int i = 0; i < s->nn; i++) s->b[i] = dmin(s->b[i], s->h[i]);
+  return rc;
+}
+
 /* Diffusion, src/Diffusion.f90:1-158: ADI, x-implicit rows j=2..ny-1, then y-implicit columns
  * i=2..nx-1.  Columns i=1,nx are never rewritten; the reflective-bottom coefficient (:114) uses
  * the loop index left over from the row loop (j=ny).                                           */
@@ -958,7 +1002,7 @@ fso_state *fso_create(int nx, int ny) {
 #define DA(p) s->p = (double *)calloc(nn, sizeof(double))
 #define IA(p) s->p = (int *)calloc(nn, sizeof(int))
   DA(h); DA(u); DA(b); DA(etot); DA(erate); DA(a); DA(length); DA(precip); DA(kf); DA(kd); DA(sedflux);
-  DA(lake_depth); DA(hwater); DA(g); DA(p_mfd_exp);
+  DA(lake_depth); DA(hwater); DA(g); DA(p_mfd_exp); DA(vx); DA(vy);
   IA(rec); IA(ndon); IA(stack); IA(mnrec); IA(mstack);
 #undef DA
 #undef IA
@@ -980,6 +1024,7 @@ void fso_destroy(fso_state *s) {
   free(s->precip); free(s->kf); free(s->kd); free(s->sedflux); free(s->lake_depth); free(s->hwater);
   free(s->g); free(s->p_mfd_exp); free(s->rec); free(s->ndon); free(s->stack); free(s->mnrec);
   free(s->mstack); free(s->don); free(s->mrec); free(s->mwrec); free(s->mlrec); free(s->bc);
+  free(s->vx); free(s->vy);
   free(s);
 }
 
@@ -989,6 +1034,13 @@ void fso_set_bc(fso_state *s, int ibc) {
   s->ibc = ibc;
   fso_bc_decode(ibc, s->nx, s->ny, &s->i1, &s->i2, &s->j1, &s->j2, &s->xcyc, &s->ycyc, s->bc);
 }
+/* SetV, src/FastScape_ctx.f90:616-632 */
+void fso_set_v(fso_state *s, const double *vx, const double *vy) {
+  memcpy(s->vx, vx, sizeof(double) * (size_t)s->nn);
+  memcpy(s->vy, vy, sizeof(double) * (size_t)s->nn);
+  s->run_advect = 1;
+}
+
 void fso_set_u(fso_state *s, const double *u) { memcpy(s->u, u, sizeof(double) * (size_t)s->nn); s->run_uplift = 1; }
 void fso_set_precip(fso_state *s, const double *p) { memcpy(s->precip, p, sizeof(double) * (size_t)s->nn); }
 void fso_init_h(fso_state *s, const double *h) {
@@ -1018,6 +1070,10 @@ void fso_set_erosional_parameters(fso_state *s, const double *kf, double kfsed, 
 /* FastScape_Execute_Step, src/FastScape_api.f90:201-263 (Advect/Marine/Strati not on this path). */
 int fso_execute_step(fso_state *s) {
   int rc = 0;
+  if (s->run_advect) {
+    rc = advect(s);
+    if (rc) return rc;
+  }
   if (s->run_uplift) { /* Uplift, src/Uplift.f90:9-10 */
     for (int i = 0; i < s->nn; i++) {
       s->h[i] = s->h[i] + s->u[i] * s->dt;

@@ -40,6 +40,7 @@ void fso_set_xl_yl(fso_state *s, double xl, double yl);
 void fso_set_dt(fso_state *s, double dt);
 void fso_set_bc(fso_state *s, int ibc);
 void fso_set_u(fso_state *s, const double *u);
+void fso_set_v(fso_state *s, const double *vx, const double *vy);  /* SetV: turns on Advect */
 void fso_set_precip(fso_state *s, const double *precip);
 void fso_init_h(fso_state *s, const double *h);   /* also sets b = h */
 void fso_set_h(fso_state *s, const double *h);

@@ -63,7 +63,8 @@ def _c():
         lib.fso_create.argtypes = [C.c_int, C.c_int]
         lib.fso_destroy.argtypes = [C.c_void_p]
         for name, args in [("fso_set_xl_yl", [C.c_double, C.c_double]), ("fso_set_dt", [C.c_double]),
-                           ("fso_set_bc", [C.c_int]), ("fso_set_u", [_dp]), ("fso_set_precip", [_dp]),
+                           ("fso_set_bc", [C.c_int]), ("fso_set_u", [_dp]), ("fso_set_v", [_dp, _dp]),
+                           ("fso_set_precip", [_dp]),
                            ("fso_init_h", [_dp]), ("fso_set_h", [_dp]), ("fso_set_basement", [_dp]),
                            ("fso_set_newton_tol", [C.c_double]),
                            ("fso_set_erosional_parameters", [_dp, C.c_double, C.c_double, C.c_double, _dp,
@@ -131,6 +132,7 @@ class COracle:
     def set_dt(self, dt): self.lib.fso_set_dt(self.h_, dt)
     def set_bc(self, ibc): self.lib.fso_set_bc(self.h_, int(ibc))
     def set_u(self, u): self.lib.fso_set_u(self.h_, _d(self._arr(u)))
+    def set_v(self, vx, vy): self.lib.fso_set_v(self.h_, _d(self._arr(vx)), _d(self._arr(vy)))
     def set_precip(self, p): self.lib.fso_set_precip(self.h_, _d(self._arr(p)))
     def init_h(self, h): self.lib.fso_init_h(self.h_, _d(self._arr(h)))
     def set_h(self, h): self.lib.fso_set_h(self.h_, _d(self._arr(h)))
@@ -341,6 +343,7 @@ class FortranRef:
     def set_dt(self, dt): self.lib.fastscape_set_dt_(self._rd(dt))
     def set_bc(self, ibc): self.lib.fastscape_set_bc_(self._ri(ibc))
     def set_u(self, u): self.lib.fastscape_set_u_(_d(self._arr(u)))
+    def set_v(self, vx, vy): self.lib.fastscape_set_v_(_d(self._arr(vx)), _d(self._arr(vy)))
     def set_precip(self, p): self.lib.fastscape_set_precip_(_d(self._arr(p)))
     def init_h(self, h): self.lib.fastscape_init_h_(_d(self._arr(h)))
 
