@@ -16,7 +16,9 @@ Further legs (each reported beside the headline value):
 * serving: a Poisson stream of runs (20 runs/s) with grids log-uniform in [65, 513], step counts
   log-uniform in [20, 200], dt log-uniform in [1e3, 1e4], D8 or multiple flow direction (p = 1)
   50/50, other parameters as the ragged leg; served by the continuous-batching loop on a landscape
-  pool; p50/p99 completion latency, and the same runs all submitted at once for the saturated rate.
+  pool; p50/p99 completion latency, and the same runs all submitted at once for the saturated rate;
+* readouts: slope, curvature, chi, catchment and fluxes of every landscape of the uniform batch
+  (ms per landscape), beside the reference Fortran on one of them.
 
 Prints ONE JSON line: metric, value (batched cell-steps/s), plain rate and the batched/plain ratio,
 the roofline of the dominant kernel (HIP events recorded around it on the stream it runs on during
@@ -269,6 +271,69 @@ def ragged_leg(fs, steps, sample, lanes=1):
             "newton_iters_per_step": float(np.sum([s["newton_iters"] for s in stats]))}
 
 
+def readout_leg(fs, Ls):
+    """The read-outs (slope, curvature, chi, catchment, fluxes) of every landscape of the batch after two
+    steps, one host call per landscape, each copying its nn values to host memory (PCIe included)."""
+    from fastscapelib_fortran import _lib
+    nn = Ls[0]["nx"] * Ls[0]["ny"]
+    res = {"workload": f"{len(Ls)} x {Ls[0]['nx']}x{Ls[0]['ny']} after 2 steps, ms per landscape, "
+                       "including the copy of the field to host memory"}
+    with fs.Batch([(L["nx"], L["ny"]) for L in Ls]) as b:
+        for k, L in enumerate(Ls):
+            configure(b, L, k)
+        b.step(2)
+        out = np.empty(nn, np.float64)
+        calls = [("slope", lambda k: b.derived(k, _lib.FSX_SLOPE, out)),
+                 ("curvature", lambda k: b.derived(k, _lib.FSX_CURVATURE, out)),
+                 ("chi", lambda k: b.derived(k, _lib.FSX_CHI, out)),
+                 ("catchment", lambda k: b.derived(k, _lib.FSX_CATCHMENT, out)),
+                 ("fluxes", lambda k: b.fluxes(k))]
+        for name, fn in calls:
+            fn(0)
+            t0 = time.perf_counter()
+            for k in range(len(Ls)):
+                fn(k)
+            res[name] = round((time.perf_counter() - t0) / len(Ls) * 1e3, 4)
+        batched = {}
+        for name, fn in [("slope", lambda: b.derived_all(_lib.FSX_SLOPE)),
+                         ("curvature", lambda: b.derived_all(_lib.FSX_CURVATURE)),
+                         ("chi", lambda: b.derived_all(_lib.FSX_CHI)),
+                         ("catchment", lambda: b.derived_all(_lib.FSX_CATCHMENT)),
+                         ("fluxes", lambda: b.fluxes_all())]:
+            fn()
+            t0 = time.perf_counter()
+            fn()
+            batched[name] = round((time.perf_counter() - t0) / len(Ls) * 1e3, 4)
+        res["batched"] = batched
+        res["batched_note"] = "all landscapes in one launch, ms per landscape (the same copies to host)"
+    return res
+
+
+def readout_cpu_baseline(L):
+    """The same read-outs on the CPU, one landscape after one step, one core: the reference Fortran when
+    its build is present (else the oracle's C restatement)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    use_ref = O.ref_available()
+    if use_ref:
+        o = O.FortranRef()
+        o.setup(L["nx"], L["ny"])
+    else:
+        o = O.COracle(L["nx"], L["ny"])
+    configure(o, L)
+    o.execute_step()
+    cpu = {"unit": "ms per landscape", "cores": 1, "kind": "reference" if use_ref else "port",
+           "sample": f"1 x {L['nx']}x{L['ny']} after one step"}
+    for name in ["slope", "curvature", "chi"] + (["catchment"] if use_ref else []):
+        t0 = time.perf_counter()
+        o.derived(name)
+        cpu[name] = round((time.perf_counter() - t0) * 1e3, 4)
+    t0 = time.perf_counter()
+    o.fluxes()
+    cpu["fluxes"] = round((time.perf_counter() - t0) * 1e3, 4)
+    return cpu
+
+
 def serving_leg(fs, quantum, duration, sample):
     from fastscapelib_fortran import serving
     stream = serving_stream(duration=duration)
@@ -408,7 +473,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=200)
     ap.add_argument("--cpu-procs", type=int, default=0, help="0: min(16, host cores)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--legs", default="uniform,plain,ragged,serving",
+    ap.add_argument("--legs", default="uniform,plain,ragged,serving,readouts",
                     help="comma list of uniform, plain, ragged, serving, copy (copy alone: HBM-counter calibration)")
     ap.add_argument("--ragged-steps", type=int, default=10)
     ap.add_argument("--serve-seconds", type=float, default=8.0)
@@ -479,6 +544,10 @@ def main():
         out["ragged"] = ragged_leg(fs, args.ragged_steps, sample=8, lanes=args.lanes)
     if "serving" in legs:
         out["serving"] = serving_leg(fs, args.quantum, args.serve_seconds, sample=8)
+    if "readouts" in legs:
+        out["readouts"] = readout_leg(fs, Ls)
+        if not args.no_cpu:
+            out["readouts"]["cpu_baseline"] = readout_cpu_baseline(Ls[0])
     if not args.no_cpu:
         procs = args.cpu_procs or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(Ls, args.cpu_steps, procs)

@@ -868,6 +868,78 @@ extern "C" int fsx_copy_field(fsx_ctx* c, int inst, int field, double* dst, int 
   return 0;
 }
 
+static void launch_derived(fsx_ctx* c, const int* ids, int n, int inst1, int which, double* out) {
+  if (which == FSX_CHI)
+    launch_chi(c->S, ids, n, inst1, out, c->stream);
+  else if (which == FSX_CATCHMENT)
+    launch_catchment(c->S, ids, n, inst1, out, c->stream);
+  else
+    launch_terrain(c->S, ids, n, inst1, which == FSX_SLOPE ? 0 : 1, 45.0 / std::atan(1.0), out, c->stream);
+}
+
+// the landscapes selected by sel[k] != 0 (every landscape when sel is null), checked, as a device list
+static int readout_list(fsx_ctx* c, const std::vector<char>& sel, bool need_routing, int* n) {
+  std::vector<int> ids;
+  for (int k = 0; k < c->S.ninst; k++) {
+    if (!sel[k]) continue;
+    if (!c->used[k]) return fail(FSX_EINVAL, "selected slot holds no landscape");
+    if (need_routing && !c->routed[k]) return fail(FSX_EINVAL, "a selected landscape has no routed step");
+    ids.push_back(k);
+  }
+  *n = (int)ids.size();
+  if (ids.empty()) return 0;
+  if (int r = upload_params(c)) return r;
+  HIPCHK(hipMemcpyAsync(c->S.sB, ids.data(), sizeof(int) * ids.size(), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));  // ids is a host temporary
+  return 0;
+}
+
+extern "C" int fsx_copy_derived_batch(fsx_ctx* c, int which, double* const* dst, int on_device) {
+  if (!c) return fail(FSX_EINVAL, "null context");
+  if (!dst) return fail(FSX_EINVAL, "null destination list");
+  if (which < FSX_SLOPE || which > FSX_CATCHMENT) return fail(FSX_EINVAL, "unknown derived field");
+  std::vector<char> sel(c->S.ninst);
+  for (int k = 0; k < c->S.ninst; k++) sel[k] = dst[k] != nullptr;
+  int n = 0;
+  if (int r = readout_list(c, sel, which >= FSX_CHI, &n)) return r;
+  if (!n) return 0;
+  launch_derived(c, c->S.sB, n, -1, which, c->S.zint);  // S.sB: 8 ints per node of scratch outside a step
+  HIPCHK(hipGetLastError());
+  const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  for (int k = 0; k < c->S.ninst; k++)
+    if (sel[k])
+      HIPCHK(hipMemcpyAsync(dst[k], c->S.zint + c->hp[k].off, sizeof(double) * c->hp[k].nn, kind, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+extern "C" int fsx_get_fluxes_batch(fsx_ctx* c, double* out) {
+  if (!c) return fail(FSX_EINVAL, "null context");
+  if (!out) return fail(FSX_EINVAL, "null destination");
+  std::vector<char> sel(c->S.ninst);
+  for (int k = 0; k < c->S.ninst; k++) {
+    sel[k] = c->used[k] && c->routed[k] && (c->hp[k].sfd || c->S.mrec);
+    for (int q = 0; q < 3; q++) out[3 * k + q] = std::nan("");
+  }
+  int n = 0;
+  if (int r = readout_list(c, sel, true, &n)) return r;
+  if (!n) return 0;
+  double* sums = c->S.zintp;  // 3 per listed landscape (nn >= 9 nodes each)
+  launch_fluxes(c->S, c->S.sB, n, -1, c->S.zint, sums, c->stream);
+  HIPCHK(hipGetLastError());
+  std::vector<double> s3(3 * (size_t)n);
+  HIPCHK(hipMemcpyAsync(s3.data(), sums, sizeof(double) * s3.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int k = 0, i = 0; k < c->S.ninst; k++) {
+    if (!sel[k]) continue;
+    const InstParams& P = c->hp[k];
+    const double surf = c->xl[k] * c->yl[k] / (double)(P.nx - 1) / (double)(P.ny - 1);
+    for (int q = 0; q < 3; q++) out[3 * k + q] = s3[3 * i + q] * surf;
+    i++;
+  }
+  return 0;
+}
+
 extern "C" int fsx_copy_derived(fsx_ctx* c, int inst, int which, double* dst, int on_device) {
   if (int r = check_inst(c, inst)) return r;
   if (!dst) return fail(FSX_EINVAL, "null destination");
@@ -876,12 +948,7 @@ extern "C" int fsx_copy_derived(fsx_ctx* c, int inst, int which, double* dst, in
   if (int r = upload_params(c)) return r;
   const InstParams& P = c->hp[inst];
   double* tmp = c->S.zint + P.off;  // scratch outside a step
-  if (which == FSX_CHI)
-    launch_chi(c->S, inst, P.dx * P.dy * 10.0, P.m / P.n, tmp, c->stream);
-  else if (which == FSX_CATCHMENT)
-    launch_catchment(c->S, inst, P.nn, tmp, c->stream);
-  else
-    launch_terrain(c->S, inst, P.nn, which == FSX_SLOPE ? 0 : 1, 45.0 / std::atan(1.0), tmp, c->stream);
+  launch_derived(c, nullptr, P.nn, inst, which, tmp);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(dst, tmp, sizeof(double) * P.nn, on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
                         c->stream));
@@ -897,7 +964,7 @@ extern "C" int fsx_get_fluxes(fsx_ctx* c, int inst, double* out3) {
   if (!P.sfd && !c->S.mrec) return fail(FSX_EINVAL, "no multiple-flow routing in place");
   if (int r = upload_params(c)) return r;
   double* sums = c->S.zintp + P.off;  // scratch outside a step (nn >= 9 entries)
-  launch_fluxes(c->S, inst, c->S.zint + P.off, sums, c->stream);
+  launch_fluxes(c->S, nullptr, P.nn, inst, c->S.zint + P.off, sums, c->stream);
   HIPCHK(hipGetLastError());
   double s3[3];
   HIPCHK(hipMemcpyAsync(s3, sums, sizeof s3, hipMemcpyDeviceToHost, c->stream));

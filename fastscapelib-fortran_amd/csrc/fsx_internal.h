@@ -234,11 +234,12 @@ void launch_spl(DevState& S, hipStream_t st);
 void launch_diffusion(DevState& S, hipStream_t st);
 void launch_mfd_route(DevState& S, hipStream_t st);
 void launch_mfd_spl(DevState& S, hipStream_t st);
-// read-outs of one landscape (outside a step)
-void launch_terrain(DevState& S, int inst, int nn, int which, double con, double* out, hipStream_t st);
-void launch_chi(DevState& S, int inst, double a0, double mn, double* out, hipStream_t st);
-void launch_catchment(DevState& S, int inst, int nn, double* out, hipStream_t st);
-void launch_fluxes(DevState& S, int inst, double* flux, double* sums, hipStream_t st);
+// read-outs (outside a step) of the landscapes listed in ids (device memory, n entries; outputs at each
+// landscape's node offset), or of the one landscape inst1 of n nodes when ids is null (outputs from 0)
+void launch_terrain(DevState& S, const int* ids, int n, int inst1, int which, double con, double* out, hipStream_t st);
+void launch_chi(DevState& S, const int* ids, int n, int inst1, double* out, hipStream_t st);
+void launch_catchment(DevState& S, const int* ids, int n, int inst1, double* out, hipStream_t st);
+void launch_fluxes(DevState& S, const int* ids, int n, int inst1, double* flux, double* sums, hipStream_t st);
 int sweep_set_lds(int smem_max);
 int basins_set_lds(int smem_max);
 void launch_fill(DevState& S, hipStream_t st);

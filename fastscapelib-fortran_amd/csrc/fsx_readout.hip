@@ -10,9 +10,12 @@
 // one thread per node; the outer ring is 0.  The 3x3 block is numbered as the reference's curvature
 // does it: 1..3 the row above (j+1), 4..6 the node's row, 7..9 the row below, each left to right; the
 // sums keep the reference's left-to-right order.  con = 45/atan(1) comes from the host.
-__global__ __launch_bounds__(256) void k_terrain(const InstParams* __restrict__ PP, int inst, const double* __restrict__ h,
-                                                 int which, double con, double* __restrict__ out) {
+__global__ __launch_bounds__(256) void k_terrain(const InstParams* __restrict__ PP, const int* __restrict__ ids, int inst1,
+                                                 const double* __restrict__ h, int which, double con,
+                                                 double* __restrict__ outb) {
+  const int inst = ids ? ids[blockIdx.y] : inst1;
   const InstParams& P = PP[inst];
+  double* out = outb + (ids ? P.off : 0);
   const int ij = blockIdx.x * blockDim.x + threadIdx.x;
   if (ij >= P.nn) return;
   const int nx = P.nx, i = ij % nx, j = ij / nx;
@@ -45,12 +48,15 @@ __global__ __launch_bounds__(256) void k_terrain(const InstParams* __restrict__ 
 // The reference walks the stack; chi of a node needs only its receiver's, so the levels of the
 // single-flow forest (base level first) give the same values, one workgroup per landscape.  A
 // base-level node has length 0 and its own chi (still 0) as receiver term, hence 0.
-__global__ __launch_bounds__(FSX_WG) void k_chi(const InstParams* __restrict__ PP, int inst, const int* __restrict__ lstart,
-                                                const int* __restrict__ nlev, const int* __restrict__ order,
-                                                const int* __restrict__ rec, const double* __restrict__ area,
-                                                const double* __restrict__ length, double a0, double mn,
-                                                double* __restrict__ out) {
+__global__ __launch_bounds__(FSX_WG) void k_chi(const InstParams* __restrict__ PP, const int* __restrict__ ids, int inst1,
+                                                const int* __restrict__ lstart, const int* __restrict__ nlev,
+                                                const int* __restrict__ order, const int* __restrict__ rec,
+                                                const double* __restrict__ area, const double* __restrict__ length,
+                                                double* __restrict__ outb) {
+  const int inst = ids ? ids[blockIdx.x] : inst1;
   const InstParams& P = PP[inst];
+  double* out = outb + (ids ? P.off : 0);
+  const double a0 = P.dx * P.dy * 10.0, mn = P.m / P.n;
   const long long off = P.off;
   const int* ls = lstart + P.aux;
   const int L = nlev[inst];
@@ -72,10 +78,12 @@ __global__ __launch_bounds__(FSX_WG) void k_chi(const InstParams* __restrict__ P
 // is (node + 1) / nn, so a catchment reads as its outlet; which nodes share a value is the reference's.
 // The roots come from the basin labels of the step's rerouting (signed: open basins -(b+1)) and the
 // outlet of each basin.
-__global__ __launch_bounds__(256) void k_catchment(const InstParams* __restrict__ PP, int inst,
-                                                   const int* __restrict__ basin, const int* __restrict__ outlet,
-                                                   double* __restrict__ out) {
+__global__ __launch_bounds__(256) void k_catchment(const InstParams* __restrict__ PP, const int* __restrict__ ids,
+                                                   int inst1, const int* __restrict__ basin,
+                                                   const int* __restrict__ outlet, double* __restrict__ outb) {
+  const int inst = ids ? ids[blockIdx.y] : inst1;
   const InstParams& P = PP[inst];
+  double* out = outb + (ids ? P.off : 0);
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= P.nn) return;
   const int bv = basin[P.off + v];
@@ -102,17 +110,20 @@ __device__ __forceinline__ double block_sum(double x, double* red) {
 // shares with double atomics, so the order of those additions (and of the three sums below, a tree
 // reduction where the reference adds in index order) differs from the reference's: the results
 // agree to rounding.  sums[0..2] = sum(u), sum(erate), sum(flux over fixed boundary nodes).
-__global__ __launch_bounds__(FSX_WG) void k_fluxes(const InstParams* __restrict__ PP, int inst,
-                                                   const int* __restrict__ lstart, const int* __restrict__ nlev,
+__global__ __launch_bounds__(FSX_WG) void k_fluxes(const InstParams* __restrict__ PP, const int* __restrict__ ids,
+                                                   int inst1, const int* __restrict__ lstart, const int* __restrict__ nlev,
                                                    const int* __restrict__ order, const int* __restrict__ rec,
                                                    const int* __restrict__ mls, const int* __restrict__ mnlev,
                                                    const int* __restrict__ mstack, const int* __restrict__ mrec,
                                                    const int* __restrict__ mnrec, const double* __restrict__ mw,
                                                    const double* __restrict__ u, const double* __restrict__ erate,
-                                                   const uint8_t* __restrict__ bc, double* __restrict__ flux,
-                                                   double* __restrict__ sums) {
+                                                   const uint8_t* __restrict__ bc, double* __restrict__ fluxb,
+                                                   double* __restrict__ sumsb) {
   __shared__ double red[FSX_WG / 64];
+  const int inst = ids ? ids[blockIdx.x] : inst1;
   const InstParams& P = PP[inst];
+  double* flux = fluxb + (ids ? P.off : 0);
+  double* sums = sumsb + (ids ? 3 * blockIdx.x : 0);
   const long long off = P.off;
   const int nn = P.nn;
   for (int v = threadIdx.x; v < nn; v += FSX_WG) flux[v] = erate[off + v];
@@ -162,19 +173,24 @@ __global__ __launch_bounds__(FSX_WG) void k_fluxes(const InstParams* __restrict_
   }
 }
 
-void launch_terrain(DevState& S, int inst, int nn, int which, double con, double* out, hipStream_t st) {
-  k_terrain<<<(nn + 255) / 256, 256, 0, st>>>(S.P, inst, S.h, which, con, out);
+// ids: a device list of n landscapes (outputs at each landscape's node offset, fluxes 3 per list entry),
+// or nullptr for the one landscape inst1 (outputs from index 0)
+void launch_terrain(DevState& S, const int* ids, int n, int inst1, int which, double con, double* out,
+                    hipStream_t st) {
+  const int nn = ids ? S.max_nn : n;
+  k_terrain<<<dim3((nn + 255) / 256, ids ? n : 1), 256, 0, st>>>(S.P, ids, inst1, S.h, which, con, out);
 }
 
-void launch_catchment(DevState& S, int inst, int nn, double* out, hipStream_t st) {
-  k_catchment<<<(nn + 255) / 256, 256, 0, st>>>(S.P, inst, S.basin, S.outlet, out);
+void launch_catchment(DevState& S, const int* ids, int n, int inst1, double* out, hipStream_t st) {
+  const int nn = ids ? S.max_nn : n;
+  k_catchment<<<dim3((nn + 255) / 256, ids ? n : 1), 256, 0, st>>>(S.P, ids, inst1, S.basin, S.outlet, out);
 }
 
-void launch_chi(DevState& S, int inst, double a0, double mn, double* out, hipStream_t st) {
-  k_chi<<<1, FSX_WG, 0, st>>>(S.P, inst, S.lstart, S.nlev, S.order, S.rec, S.area, S.length, a0, mn, out);
+void launch_chi(DevState& S, const int* ids, int n, int inst1, double* out, hipStream_t st) {
+  k_chi<<<ids ? n : 1, FSX_WG, 0, st>>>(S.P, ids, inst1, S.lstart, S.nlev, S.order, S.rec, S.area, S.length, out);
 }
 
-void launch_fluxes(DevState& S, int inst, double* flux, double* sums, hipStream_t st) {
-  k_fluxes<<<1, FSX_WG, 0, st>>>(S.P, inst, S.lstart, S.nlev, S.order, S.rec, S.mls, S.mnlev, S.mstack, S.mrec,
-                                 S.mnrec, S.mw, S.u, S.erate, S.bc, flux, sums);
+void launch_fluxes(DevState& S, const int* ids, int n, int inst1, double* flux, double* sums, hipStream_t st) {
+  k_fluxes<<<ids ? n : 1, FSX_WG, 0, st>>>(S.P, ids, inst1, S.lstart, S.nlev, S.order, S.rec, S.mls, S.mnlev,
+                                           S.mstack, S.mrec, S.mnrec, S.mw, S.u, S.erate, S.bc, flux, sums);
 }

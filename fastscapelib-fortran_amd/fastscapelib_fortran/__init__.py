@@ -215,6 +215,23 @@ class Batch:
     def chi(self, k): return self.derived(k, _lib.FSX_CHI)
     def catchment(self, k): return self.derived(k, _lib.FSX_CATCHMENT)
 
+    def derived_all(self, which, ks=None):
+        """One read-out (FSX_SLOPE, FSX_CURVATURE, FSX_CHI or FSX_CATCHMENT) of the landscapes ks (all by
+        default) in one launch: a list of arrays in the order of ks."""
+        ks = list(range(len(self.nn))) if ks is None else list(ks)
+        outs = [np.empty(self.nn[k], np.float64) for k in ks]
+        ptrs = (C.c_void_p * len(self.nn))()
+        for k, o in zip(ks, outs):
+            ptrs[k] = o.ctypes.data
+        check(lib.fsx_copy_derived_batch(self._h, which, ptrs, 0), "copy_derived_batch")
+        return outs
+
+    def fluxes_all(self):
+        """(tectonic, erosion, boundary) flux of every landscape in one launch: an array [n, 3]."""
+        out = np.empty((len(self.nn), 3), np.float64)
+        check(lib.fsx_get_fluxes_batch(self._h, ptr(out)), "get_fluxes_batch")
+        return out
+
     def fluxes(self, k):
         """(tectonic, erosion, boundary) flux of landscape k (FastScape_Get_Fluxes)."""
         o = (C.c_double * 3)()

@@ -60,6 +60,8 @@ def _load():
         "fsx_copy_ifield": [_vp, C.c_int, C.c_int, _vp, C.c_int],
         "fsx_copy_derived": [_vp, C.c_int, C.c_int, _vp, C.c_int],
         "fsx_set_v": [_vp, C.c_int, _vp, _vp, C.c_int],
+        "fsx_copy_derived_batch": [_vp, C.c_int, C.POINTER(C.c_void_p), C.c_int],
+        "fsx_get_fluxes_batch": [_vp, _vp],
         "fsx_get_fluxes": [_vp, C.c_int, _dp],
         "fsx_get_step": [_vp, C.c_int, _ip],
         "fsx_get_sizes": [_vp, C.c_int, _ip, _ip],

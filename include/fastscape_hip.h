@@ -153,6 +153,11 @@ int fsx_copy_derived(fsx_ctx *ctx, int inst, int which, double *dst, int on_devi
  * out3 = tectonic, erosion and boundary flux of one landscape's current state, computed on the device
  * (FSX_EINVAL before the first step or fsx_route). */
 int fsx_get_fluxes(fsx_ctx *ctx, int inst, double *out3);
+/* The same for many landscapes in one launch (no reference counterpart: the reference holds one
+ * landscape).  fsx_copy_derived_batch: dst has one pointer per landscape slot (nn doubles each, NULL to
+ * skip); fsx_get_fluxes_batch: out has 3 doubles per slot, NaN for a slot without a routed landscape. */
+int fsx_copy_derived_batch(fsx_ctx *ctx, int which, double *const *dst, int on_device);
+int fsx_get_fluxes_batch(fsx_ctx *ctx, double *out);
 
 /* FastScape_Get_Step (src/FastScape_api.f90:572-584) */
 int fsx_get_step(fsx_ctx *ctx, int inst, int *step);

@@ -13,7 +13,9 @@ Each case steps a batch of landscapes on the GPU and the same landscapes in the 
 * end to end, against the oracle's own state (which the GPU's matches to 1e-10 in h): slope,
   curvature and chi within 1e-7 of the field's largest value, fluxes within 1e-8 of the sum of the
   magnitudes of their terms (multiple flow is checked this way only);
-* that chi, the catchment and the fluxes refuse a landscape that no step has routed.
+* that chi, the catchment and the fluxes refuse a landscape that no step has routed;
+* that the batched entry points (all landscapes in one launch) give the per-landscape results: bit for
+  bit for the fields, to rounding for the fluxes (their atomics add in any order).
 """
 from __future__ import annotations
 
@@ -87,10 +89,22 @@ def test_readouts_match_oracle(oracle):
             want = o.fluxes()
             for q in range(3):
                 assert abs(fl[q] - want[q]) <= 1e-8 * mag[q], (k, q, fl, want)
+        # the batched entry points
+        for which in [_lib.FSX_SLOPE, _lib.FSX_CURVATURE, _lib.FSX_CHI, _lib.FSX_CATCHMENT]:
+            outs = b.derived_all(which)
+            for k in range(len(Ls)):
+                assert np.array_equal(outs[k], b.derived(k, which)), (which, k)
+        part = b.derived_all(_lib.FSX_CHI, [3, 1])
+        assert np.array_equal(part[0], b.chi(3)) and np.array_equal(part[1], b.chi(1))
+        fa = b.fluxes_all()
+        for k, (nx, ny, *_r) in enumerate(CASES):
+            one = b.fluxes(k)
+            assert all(abs(fa[k, q] - one[q]) <= _sum_tol(nx * ny) * max(abs(one[q]), 1.0) for q in range(3)), k
 
 
 def test_readouts_need_a_routed_step():
     import fastscapelib_fortran as fs
+    from fastscapelib_fortran import _lib
     L = landscape(20, 20, seed=1)
     with fs.Batch([(20, 20)]) as b:
         apply(b, L, 0)
@@ -101,6 +115,9 @@ def test_readouts_need_a_routed_step():
             b.fluxes(0)
         with pytest.raises(fs.FastscapeError):
             b.catchment(0)
+        with pytest.raises(fs.FastscapeError):
+            b.derived_all(_lib.FSX_CHI)
+        assert np.isnan(b.fluxes_all()).all()
         b.step(1)
         assert np.isfinite(b.chi(0)).all() and all(np.isfinite(b.fluxes(0)))
 
