@@ -17,6 +17,7 @@ Further legs (each reported beside the headline value):
   log-uniform in [20, 200], dt log-uniform in [1e3, 1e4], D8 or multiple flow direction (p = 1)
   50/50, other parameters as the ragged leg; served by the continuous-batching loop on a landscape
   pool; p50/p99 completion latency, and the same runs all submitted at once for the saturated rate;
+* advect: the uniform batch with a velocity field, advection of h, b and etot first in every step;
 * readouts: slope, curvature, chi, catchment and fluxes of every landscape of the uniform batch
   (ms per landscape), beside the reference Fortran on one of them.
 
@@ -271,6 +272,27 @@ def ragged_leg(fs, steps, sample, lanes=1):
             "newton_iters_per_step": float(np.sum([s["newton_iters"] for s in stats]))}
 
 
+def advect_leg(fs, Ls, steps, lanes):
+    """The uniform batch with a velocity field (vx = 2 cm/yr, vy = -1 cm/yr, +-20 % noise), so every step
+    starts with the advection of h, b and etot (fsx_set_v)."""
+    with open_batch(fs, Ls, lanes) as b:
+        b.graphs(GRAPHS)
+        for k, L in enumerate(Ls):
+            configure(b, L, k)
+            rng = np.random.default_rng(500 + k)
+            nn = L["nx"] * L["ny"]
+            b.set_v(k, 0.02 * (0.8 + 0.4 * rng.random(nn)), -0.01 * (0.8 + 0.4 * rng.random(nn)))
+        b.step(2)
+        _sync()
+        t0 = time.perf_counter()
+        b.step(steps)
+        _sync()
+        dt = time.perf_counter() - t0
+    ncells = sum(L["nx"] * L["ny"] for L in Ls)
+    return {"workload": f"uniform batch {len(Ls)} x {Ls[0]['nx']}x{Ls[0]['ny']} with advection first (set_v)",
+            "lanes": lanes, "steps": steps, "ms_per_step": dt / steps * 1e3, "value": ncells * steps / dt}
+
+
 def readout_leg(fs, Ls):
     """The read-outs (slope, curvature, chi, catchment, fluxes) of every landscape of the batch after two
     steps, one host call per landscape, each copying its nn values to host memory (PCIe included)."""
@@ -473,7 +495,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=200)
     ap.add_argument("--cpu-procs", type=int, default=0, help="0: min(16, host cores)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--legs", default="uniform,plain,ragged,serving,readouts",
+    ap.add_argument("--legs", default="uniform,plain,ragged,serving,advect,readouts",
                     help="comma list of uniform, plain, ragged, serving, copy (copy alone: HBM-counter calibration)")
     ap.add_argument("--ragged-steps", type=int, default=10)
     ap.add_argument("--serve-seconds", type=float, default=8.0)
@@ -544,6 +566,8 @@ def main():
         out["ragged"] = ragged_leg(fs, args.ragged_steps, sample=8, lanes=args.lanes)
     if "serving" in legs:
         out["serving"] = serving_leg(fs, args.quantum, args.serve_seconds, sample=8)
+    if "advect" in legs:
+        out["advect"] = advect_leg(fs, Ls, 20, args.lanes)
     if "readouts" in legs:
         out["readouts"] = readout_leg(fs, Ls)
         if not args.no_cpu:
