@@ -12,7 +12,8 @@ The free functions keep the reference's call surface and f2py argument conventio
 Documented divergences: call-order and size errors raise ``RuntimeError``/``ValueError`` where the
 reference STOPs the process; ``fastscape_setup`` starts from h = 0 rather than Fortran random
 numbers (every run sets h with ``fastscape_init_h``); routines outside the per-step update
-(marine, stratigraphy, VTK, flexure) raise ``NotImplementedError``; ``fastscape_set_v`` turns on the
+(marine, stratigraphy, flexure) raise ``NotImplementedError``; ``fastscape_vtk`` writes the
+reference's legacy-VTK files on the host from the fields copied off the GPU; ``fastscape_set_v`` turns on the
 advection of h, b and etot (Advect) at the start of every step, on the GPU.  The slope,
 curvature, chi and catchment read-outs and the three fluxes are computed on the GPU from the resident
 state; the catchment labels a catchment (outlet + 1) / nn where the reference draws a random number.
@@ -34,7 +35,7 @@ __all__ = [
     "fastscape_execute_step", "fastscape_copy_h", "fastscape_copy_basement", "fastscape_copy_total_erosion",
     "fastscape_copy_erosion_rate", "fastscape_copy_drainage_area", "fastscape_copy_lake_depth",
     "fastscape_copy_slope", "fastscape_copy_curvature", "fastscape_copy_chi", "fastscape_copy_catchment",
-    "fastscape_get_fluxes", "fastscape_set_v",
+    "fastscape_get_fluxes", "fastscape_set_v", "fastscape_vtk",
     "fastscape_get_step", "fastscape_get_sizes", "fastscape_destroy", "fastscape_set_h", "fastscape_set_basement",
     "fastscape_set_precip", "fastscape_set_all_layers", "fastscape_reset_cumulative_erosion", "fastscape_view",
     "fastscape_debug", "Batch", "Lanes", "Pool", "FastscapeError", "indexx",
@@ -99,7 +100,9 @@ class Batch:
         return ptr(a), 0, a
 
     # -- setters --
-    def set_xl_yl(self, k, xl, yl): check(lib.fsx_set_xl_yl(self._h, k, float(xl), float(yl)), "set_xl_yl")
+    def set_xl_yl(self, k, xl, yl):
+        check(lib.fsx_set_xl_yl(self._h, k, float(xl), float(yl)), "set_xl_yl")
+        self.__dict__.setdefault("_xlyl", {})[k] = (float(xl), float(yl))  # Pool has its own __init__
     def set_dt(self, k, dt): check(lib.fsx_set_dt(self._h, k, float(dt)), "set_dt")
     def set_bc(self, k, ibc): check(lib.fsx_set_bc(self._h, k, int(ibc)), "set_bc")
     def set_newton_tol(self, k, tol): check(lib.fsx_set_newton_tol(self._h, k, float(tol)), "set_newton_tol")
@@ -232,6 +235,15 @@ class Batch:
         check(lib.fsx_get_fluxes_batch(self._h, ptr(out)), "get_fluxes_batch")
         return out
 
+    def vtk(self, k, f, vex, directory="VTK"):
+        """FastScape_VTK for landscape k: legacy-VTK files of h (and b, sea level for vex < 0) with the
+        caller's field f, written on the host from the fields copied off the GPU."""
+        from .vtk import write_vtk
+        nx, ny = self.sizes[k]
+        xl, yl = self.__dict__.get("_xlyl", {}).get(k, (0.0, 0.0))
+        return write_vtk(self.get_step(k), nx, ny, xl, yl, self.h(k), self.field(k, _lib.FSX_B), f, float(vex),
+                         0.0, directory)
+
     def fluxes(self, k):
         """(tectonic, erosion, boundary) flux of landscape k (FastScape_Get_Fluxes)."""
         o = (C.c_double * 3)()
@@ -303,7 +315,7 @@ class Lanes:
     _PER = {"set_xl_yl", "set_dt", "set_bc", "set_newton_tol", "set_active", "set_erosional_parameters",
             "init_h", "set_field", "set_u", "set_v", "set_h", "set_basement", "set_precip", "set_all_layers",
             "reset_cumulative_erosion", "field", "ifield", "h", "rec", "donors", "order", "mrec", "morder",
-            "get_step", "stats", "derived", "slope", "curvature", "chi", "catchment", "fluxes"}
+            "get_step", "stats", "vtk", "derived", "slope", "curvature", "chi", "catchment", "fluxes"}
     TUNING = Batch.TUNING
 
     @staticmethod
@@ -552,6 +564,12 @@ def fastscape_copy_chi(c):
     _derived_into(_lib.FSX_CHI, c)
 
 
+def fastscape_vtk(fp, vexp):
+    """FastScape_VTK (src/FastScape_api.f90:728-741): VTK/Topography<step>.vtk (and Basement, SeaLevel
+    for vexp < 0) in the working directory, byte for byte as Make_VTK writes them."""
+    _need().vtk(0, fp, vexp)
+
+
 def fastscape_copy_catchment(c):
     """FastScape_Copy_Catchment (src/FastScape_api.f90:427-439): (outlet + 1) / nn of each node's tree
     before closed-basin rerouting; the reference gives that tree a random number instead."""
@@ -605,6 +623,6 @@ def _outside(name):
     return f
 
 
-for _n in ["fastscape_set_marine_parameters", "fastscape_strati", "fastscape_vtk",
+for _n in ["fastscape_set_marine_parameters", "fastscape_strati",
            "fastscape_init_f", "fastscape_copy_f"]:
     globals()[_n] = _outside(_n)

@@ -374,6 +374,10 @@ class FortranRef:
         getattr(self.lib, "fastscape_copy_" + name + "_")(_d(out))
         return out
 
+    def vtk(self, f, vex):
+        """fastscape_vtk: writes VTK/*.vtk into the working directory."""
+        self.lib.fastscape_vtk_(_d(np.ascontiguousarray(f, np.float64).copy()), self._rd(vex))
+
     def fluxes(self):
         t, e, b = C.c_double(), C.c_double(), C.c_double()
         self.lib.fastscape_get_fluxes_(C.byref(t), C.byref(e), C.byref(b))

@@ -86,7 +86,7 @@ def test_facade_call_order_errors():
     with pytest.raises(RuntimeError, match="SetUp"):
         fs.fastscape_get_fluxes()
     for name in ["fastscape_set_advect_every_step", "fastscape_strati", "fastscape_set_marine_parameters",
-                 "fastscape_vtk", "fastscape_copy_f"]:
+                 "fastscape_copy_f"]:
         if hasattr(fs, name):
             with pytest.raises(NotImplementedError):
                 getattr(fs, name)()

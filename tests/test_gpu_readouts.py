@@ -148,3 +148,23 @@ def test_readouts_uniform_batch_member_full_size(oracle):
         mag = (np.abs(u).sum() * surf, np.abs(er).sum() * surf, 2.0 * np.abs(er).sum() * surf)
         for q in range(3):
             assert abs(got[q] - want[q]) <= _sum_tol(n * n) * mag[q], (q, got, want)
+
+
+def test_vtk_snapshot_of_a_batch_member(tmp_path):
+    """Batch.vtk writes the member's own fields, grid spacing and step into the reference's file layout."""
+    import fastscapelib_fortran as fs
+    from fastscapelib_fortran import _lib
+    from fastscapelib_fortran.vtk import write_vtk
+    Ls = [landscape(30, 20, seed=1), landscape(41, 27, seed=2, dx=150.0)]
+    with fs.Batch([(L["nx"], L["ny"]) for L in Ls]) as b:
+        for k, L in enumerate(Ls):
+            apply(b, L, k)
+        b.step(3)
+        f = b.field(1, _lib.FSX_ERATE)
+        paths = b.vtk(1, f, -2.0, directory=str(tmp_path / "gpu"))
+        want = write_vtk(3, 41, 27, Ls[1]["xl"], Ls[1]["yl"], b.h(1), b.field(1, _lib.FSX_B), f, -2.0,
+                         directory=str(tmp_path / "want"))
+    assert [p.rsplit("/", 1)[1] for p in paths] == ["Topography0000003.vtk", "Basement0000003.vtk",
+                                                    "SeaLevel0000003.vtk"]
+    for p, q in zip(paths, want):
+        assert open(p, "rb").read() == open(q, "rb").read()
