@@ -355,6 +355,26 @@ class Lanes:
             return per
         raise AttributeError(name)
 
+    def derived_all(self, which, ks=None):
+        """Batch.derived_all over the lanes: one launch per lane, results in the order of ks."""
+        ks = list(range(len(self.sizes))) if ks is None else list(ks)
+        got = {}
+        for j, b in enumerate(self.lanes):
+            mine = [k for k in ks if self.where[k][0] == j]
+            if mine:
+                for k, o in zip(mine, b.derived_all(which, [self.where[k][1] for k in mine])):
+                    got[k] = o
+        return [got[k] for k in ks]
+
+    def fluxes_all(self):
+        """Batch.fluxes_all over the lanes: an array [n, 3] in landscape order."""
+        out = np.empty((len(self.sizes), 3), np.float64)
+        for j, b in enumerate(self.lanes):
+            fl = b.fluxes_all()
+            for i, k in enumerate(self.members[j]):
+                out[k] = fl[i]
+        return out
+
     def __len__(self):
         return len(self.sizes)
 

@@ -168,3 +168,20 @@ def test_vtk_snapshot_of_a_batch_member(tmp_path):
                                                     "SeaLevel0000003.vtk"]
     for p, q in zip(paths, want):
         assert open(p, "rb").read() == open(q, "rb").read()
+
+
+def test_batched_readouts_over_lanes():
+    """Lanes.derived_all / fluxes_all give each landscape's own read-outs, in landscape order."""
+    import fastscapelib_fortran as fs
+    from fastscapelib_fortran import _lib
+    Ls = [landscape(30 + 3 * k, 25 + k, seed=k) for k in range(5)]
+    with fs.Lanes([(L["nx"], L["ny"]) for L in Ls], 2) as b:
+        for k, L in enumerate(Ls):
+            apply(b, L, k)
+        b.step(2)
+        chi = b.derived_all(_lib.FSX_CHI, [4, 0, 3])
+        assert all(np.array_equal(c, b.chi(k)) for c, k in zip(chi, [4, 0, 3]))
+        fa = b.fluxes_all()
+        for k, L in enumerate(Ls):
+            one = b.fluxes(k)
+            assert all(abs(fa[k, q] - one[q]) <= _sum_tol(L["nx"] * L["ny"]) * max(abs(one[q]), 1.0) for q in range(3))
