@@ -1,29 +1,31 @@
 #!/usr/bin/env python3
 """Throughput of the MI355X landscape step (FastScape_Execute_Step) on one GPU.
 
-A "step" is one FastScape_Execute_Step over every landscape of a batch resident in HBM:
-uplift, D8 receivers, closed-basin rerouting, drainage area, implicit stream power, ADI diffusion.
-The default workload is the uniform batch: 64 independent 513x513 landscapes (dx = 200 m), seeds
-0..63, h ~ U[0,1), kf log-uniform in [2e-6, 2e-5], uplift U ~ U[5e-4, 2e-3] m/yr inside and 0 on the
-fixed boundary, m = 0.4, n = 1, kd = 1e-2, dt = 1e3 yr.  The same landscape run alone gives the plain
-one-at-a-time figure.  Synthetic, seeded inputs; FP64 throughout.
+A "step" is one FastScape_Execute_Step over every landscape of a batch resident in HBM: uplift,
+D8 receivers, closed-basin rerouting, drainage area, implicit stream power (Newton for n != 1), ADI
+diffusion.  Synthetic, seeded inputs (fastscapelib_fortran/workloads.py); FP64 throughout.
 
-Further legs (each reported beside the headline value):
-* ragged: 200 landscapes, nx and ny log-uniform in [65, 1025], dx = 250 m, h = U[0,1) + a 10 m 2-D
-  sinusoid of wavelength U[5, 50] km, n from {1, 1.5, 2}, m = 0.45 n, kf = 10**U(-6, -5),
-  kd ~ U(1e-3, 1e-1), uplift 1e-3 m/yr off the fixed boundary, BC 1010 (even ids) / 1111 (odd),
-  dt = 2e3 yr, Newton tolerance 1e-8; batched vs the same landscapes one at a time (a sample);
-* serving: a Poisson stream of runs (20 runs/s) with grids log-uniform in [65, 513], step counts
-  log-uniform in [20, 200], dt log-uniform in [1e3, 1e4], D8 or multiple flow direction (p = 1)
-  50/50, other parameters as the ragged leg; served by the continuous-batching loop on a landscape
-  pool; p50/p99 completion latency, and the same runs all submitted at once for the saturated rate;
-* advect: the uniform batch with a velocity field, advection of h, b and etot first in every step;
-* readouts: slope, curvature, chi, catchment and fluxes of every landscape of the uniform batch
-  (ms per landscape), beside the reference Fortran on one of them.
+Headline (``value``): the ragged batch -- 200 landscapes, nx and ny log-uniform in [65, 1025],
+dx = 250 m, h = U[0,1) + a 10 m 2-D sinusoid of wavelength U[5, 50] km, n from {1, 1.5, 2},
+m = 0.45 n, kf = 10**U(-6, -5), kd ~ U(1e-3, 1e-1), Newton tolerance 1e-8, BC 1010 / 1111
+alternating, dt = 2e3 yr -- stepped as a batch, cell-steps/s over K timed steps of its 300 (after W
+warm-up steps and the launch-shape autotuner), beside the same landscapes one at a time (``plain``, a
+sample, each tuned the same way).
 
-Prints ONE JSON line: metric, value (batched cell-steps/s), plain rate and the batched/plain ratio,
-the roofline of the dominant kernel (HIP events recorded around it on the stream it runs on during
-the timed region), and the reference Fortran timed on this host's cores on a bounded sample.
+Further legs:
+* uniform: 64 x 513 x 513 (dx = 200 m, seeds 0..63, n = 1), batched;
+* mixed: one 4097 x 4097 multiple-flow landscape with deposition (g = 1) and 64 D8 landscapes of
+  257 x 257 in one context;
+* serving: Poisson arrivals (20 runs/s) over a window, grids log-uniform in [65, 2049], step counts
+  log-uniform in [50, 2000], dt log-uniform in [1e3, 1e4] yr, D8 or multiple flow direction 50/50,
+  parameters as the ragged batch, served by the continuous-batching loop on a landscape pool; p50/p99
+  completion latency and cell-steps/s until a deadline;
+* advect (uniform batch with advection first) and readouts (slope, curvature, chi, catchment, fluxes).
+
+Prints ONE JSON line: metric, value, the plain rate and the batched/plain ratio, the roofline of the
+dominant kernel (HIP events recorded around it on the stream it runs on, inside the timed region),
+and the reference Fortran timed on this host's cores (one landscape per process) on a bounded sample
+of the same batch.
 """
 from __future__ import annotations
 
@@ -38,6 +40,8 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "fastscapelib-fortran_amd"))
+
+from fastscapelib_fortran import workloads as W  # noqa: E402  (numpy recipes; does not touch the GPU)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "cell-steps/s batched+served vs plain on 1 GPU; p50/p99 run latency"
@@ -58,110 +62,8 @@ ALG_BYTES_PER_NODE = {
 }
 ALG_BYTES_PER_EDGE = {}  # per basin-graph edge (the spanning-forest and tour phases are multi-kernel)
 
-
-def uniform_batch(ninst=64, n=513, dx=200.0):
-    """Uniform batch recipe; per landscape: h noise, then kf, then U, from PCG64(seed)."""
-    out = []
-    for seed in range(ninst):
-        rng = np.random.default_rng(seed)
-        nn = n * n
-        h = rng.random(nn)
-        kf = 10.0 ** rng.uniform(np.log10(2e-6), np.log10(2e-5))
-        U = rng.uniform(5e-4, 2e-3)
-        u = np.full(nn, U)
-        e = np.zeros((n, n), bool)
-        e[0, :] = e[-1, :] = e[:, 0] = e[:, -1] = True
-        u[e.ravel()] = 0.0
-        out.append(dict(nx=n, ny=n, xl=dx * (n - 1), yl=dx * (n - 1), h=h, u=u, kf=kf, m=0.4, n=1.0, kd=1e-2,
-                        dt=1e3, ibc=1111))
-    return out
-
-
-def _fixed_mask(nx, ny, ibc):
-    """Nodes on a fixed (base-level) boundary for a BC code (digits bottom/right/top/left)."""
-    c = f"{ibc:4d}"
-    e = np.zeros((ny, nx), bool)
-    if c[0] == "1":
-        e[0, :] = True
-    if c[1] == "1":
-        e[:, -1] = True
-    if c[2] == "1":
-        e[-1, :] = True
-    if c[3] == "1":
-        e[:, 0] = True
-    return e.ravel()
-
-
-def _sinus_landscape(rng, nx, ny, dx, amp=10.0):
-    lam = rng.uniform(5e3, 5e4)
-    h = rng.random(nx * ny)
-    x = np.arange(nx) * dx
-    y = np.arange(ny) * dx
-    return h + amp * (np.sin(2 * np.pi * x[None, :] / lam) * np.sin(2 * np.pi * y[:, None] / lam)).ravel()
-
-
-def ragged_batch(ninst=200, lo=65, hi=1025, seed0=1000):
-    """Ragged batch recipe; per landscape from PCG64(seed0 + id): nx, ny, h noise + sinusoid, then
-    n, kf, kd."""
-    out = []
-    for i in range(ninst):
-        rng = np.random.default_rng(seed0 + i)
-        nx = int(round(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
-        ny = int(round(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
-        dx = 250.0
-        h = _sinus_landscape(rng, nx, ny, dx)
-        n = float(rng.choice([1.0, 1.5, 2.0]))
-        kf = 10.0 ** rng.uniform(-6.0, -5.0)
-        kd = rng.uniform(1e-3, 1e-1)
-        ibc = 1010 if i % 2 == 0 else 1111
-        u = np.full(nx * ny, 1e-3)
-        u[_fixed_mask(nx, ny, ibc)] = 0.0
-        out.append(dict(nx=nx, ny=ny, xl=dx * (nx - 1), yl=dx * (ny - 1), h=h, u=u, kf=kf, m=0.45 * n, n=n, kd=kd,
-                        dt=2e3, ibc=ibc, newton_tol=1e-8))
-    return out
-
-
-def serving_stream(rate=20.0, duration=8.0, lo=65, hi=513, smin=20, smax=200, seed=7):
-    """Arrivals of the serving leg: Poisson(rate) over `duration` seconds; per run grid, steps, dt,
-    D8 or multiple flow direction, parameters as the ragged recipe (from PCG64(seed), then per run
-    PCG64(seed*100000 + id))."""
-    rng = np.random.default_rng(seed)
-    t, runs = 0.0, []
-    while True:
-        t += rng.exponential(1.0 / rate)
-        if t > duration:
-            break
-        i = len(runs)
-        r = np.random.default_rng(seed * 100000 + i)
-        nx = int(round(np.exp(r.uniform(np.log(lo), np.log(hi)))))
-        ny = int(round(np.exp(r.uniform(np.log(lo), np.log(hi)))))
-        steps = int(round(np.exp(r.uniform(np.log(smin), np.log(smax)))))
-        dt = float(np.exp(r.uniform(np.log(1e3), np.log(1e4))))
-        dx = 250.0
-        h = _sinus_landscape(r, nx, ny, dx)
-        n = float(r.choice([1.0, 1.5, 2.0]))
-        kf = 10.0 ** r.uniform(-6.0, -5.0)
-        kd = r.uniform(1e-3, 1e-1)
-        mfd = bool(r.random() < 0.5)
-        ibc = 1010 if i % 2 == 0 else 1111
-        u = np.full(nx * ny, 1e-3)
-        u[_fixed_mask(nx, ny, ibc)] = 0.0
-        runs.append((t, steps, dict(nx=nx, ny=ny, xl=dx * (nx - 1), yl=dx * (ny - 1), h=h, u=u, kf=kf, m=0.45 * n,
-                                    n=n, kd=kd, dt=dt, ibc=ibc, newton_tol=1e-8, p=1.0 if mfd else -2.0)))
-    return runs
-
-
-def configure(obj, L, *k):
-    obj.set_xl_yl(*k, L["xl"], L["yl"])
-    obj.set_dt(*k, L["dt"])
-    obj.init_h(*k, L["h"])
-    nn = L["nx"] * L["ny"]
-    obj.set_erosional_parameters(*k, np.full(nn, L["kf"]), -1.0, L["m"], L["n"], np.full(nn, L["kd"]), -1.0, 0.0,
-                                 0.0, L.get("p", -2.0))
-    obj.set_u(*k, L["u"])
-    obj.set_bc(*k, L["ibc"])
-    if "newton_tol" in L and hasattr(obj, "set_newton_tol"):
-        obj.set_newton_tol(*k, L["newton_tol"])
+RAGGED_WORKLOAD = ("ragged batch 200 x log-uniform[65,1025]^2 (dx 250 m, noise + 10 m sinusoid), n in {1,1.5,2}, "
+                   "Newton tol 1e-8, BC 1010/1111, dt 2e3")
 
 
 def _sync():
@@ -175,37 +77,41 @@ def _sync():
 
 GRAPHS = True  # replay captured HIP graphs (off: launch every kernel directly, for kernel-trace profilers)
 TUNE = True    # run the launch-shape autotuner after the warm-up steps of a batch
-SHAPES = None  # or the headline batch's launch shapes, given (walk_gx, jump_blocks, tour_gx, nr_small_gx)
+SHAPES = None  # or launch shapes given on the command line (walk_gx, jump_blocks, tour_gx, nr_small_gx, area_wg, bor_gx)
 
 
 def open_batch(fs, Ls, lanes=1):
     """One context for all landscapes, or (lanes > 1) fs.Lanes: several contexts stepping on their own
     streams, so one lane's per-landscape kernels overlap another's node-parallel ones."""
     sizes = [(L["nx"], L["ny"]) for L in Ls]
-    return fs.Lanes(sizes, lanes) if lanes > 1 else fs.Batch(sizes)
+    return fs.Lanes(sizes, lanes) if lanes > 1 and len(Ls) > 1 else fs.Batch(sizes)
 
 
-def run_gpu(fs, Ls, steps, warmup, profile, tune=False, lanes=1):
+def _tune(b):
+    if SHAPES:
+        b.set_tuning(dict(zip(b.TUNING, SHAPES)))
+        return b.tuning()
+    return b.autotune() if TUNE else b.tuning()
+
+
+def run_gpu(fs, Ls, steps, warmup, tune=True, lanes=1):
     with open_batch(fs, Ls, lanes) as b:
         b.graphs(GRAPHS)
         for k, L in enumerate(Ls):
-            configure(b, L, k)
+            W.configure(b, L, k)
         if warmup:
             b.step(warmup)
-        if tune and TUNE:
-            b.autotune()
-        if profile:
-            b.profiling(True)
-            b.reset_timers()
+        if tune:
+            _tune(b)
+            b.step(1)  # captures the graphs of the tuned shapes
         _sync()
         t0 = time.perf_counter()
         b.step(steps)
         _sync()
         dt = time.perf_counter() - t0
-        timers = b.timers() if profile else None
         stats = [b.stats(k) for k in range(len(Ls))]
         mem = b.memory()
-    return dt, timers, stats, mem
+    return dt, stats, mem
 
 
 def run_headline(fs, Ls, steps, warmup, prof_steps, lanes=1):
@@ -215,16 +121,15 @@ def run_headline(fs, Ls, steps, warmup, prof_steps, lanes=1):
     between its own HIP events on the step's stream, so its mean duration is measured live.  With
     lanes, the timers are lane 0's (its landscapes: ``sel``) while the other lanes run beside it."""
     with open_batch(fs, Ls, lanes) as b:
-        sel = b.members[0] if lanes > 1 else list(range(len(Ls)))
-        lane_timers = (lambda: b.timers(lane=0)) if lanes > 1 else b.timers
+        multi = isinstance(b, fs.Lanes)
+        sel = b.members[0] if multi else list(range(len(Ls)))
+        lane_timers = (lambda: b.timers(lane=0)) if multi else b.timers
         b.graphs(GRAPHS)
         for k, L in enumerate(Ls):
-            configure(b, L, k)
+            W.configure(b, L, k)
         if warmup:
             b.step(warmup)
-        if SHAPES:
-            b.set_tuning(dict(zip(b.TUNING, SHAPES)))
-        tuned = b.autotune() if TUNE else b.tuning()
+        tuned = _tune(b)
         tuned = tuned if isinstance(tuned, list) else [tuned]
         b.profiling(True)
         b.reset_timers()
@@ -243,42 +148,59 @@ def run_headline(fs, Ls, steps, warmup, prof_steps, lanes=1):
         dt = time.perf_counter() - t0
         live = lane_timers()
         b.time_phase(None)
-        stats = [b.stats(k) for k in range(len(Ls))]
+        stats_all = [b.stats(k) for k in range(len(Ls))]
         mem = b.memory()
-    return dt, prof, live, dom, stats, mem, tuned, sel
+    return dict(dt=dt, prof=prof, live=live, dom=dom, stats=stats_all, mem=mem, tuned=tuned, sel=sel,
+                lanes=len(b.lanes) if multi else 1)
 
 
 def plain_rate(fs, Ls, steps):
-    """One landscape at a time (each after one untimed step): cell-steps/s over the sample."""
+    """One landscape at a time (each alone in a context, after one untimed step and the same launch-shape
+    tuning as the batch): cell-steps/s over the sample."""
     cells, secs = 0, 0.0
     for L in Ls:
-        dt, _, _, _ = run_gpu(fs, [L], steps, 1, profile=False)
+        dt, _, _ = run_gpu(fs, [L], steps, 1, tune=True)
         cells += L["nx"] * L["ny"] * steps
         secs += dt
     return cells / secs
 
 
-def ragged_leg(fs, steps, sample, lanes=1):
-    Ls = ragged_batch()
-    ncells = sum(L["nx"] * L["ny"] for L in Ls)
-    dt, _, stats, mem = run_gpu(fs, Ls, steps, 1, profile=False, tune=True, lanes=lanes)
-    batched = ncells * steps / dt
-    pick = Ls[::max(1, len(Ls) // sample)][:sample]
-    plain = plain_rate(fs, pick, max(2, steps // 2))
-    return {"workload": "ragged batch 200 x log-uniform[65,1025]^2, n in {1,1.5,2}, Newton tol 1e-8, BC 1010/1111",
-            "landscapes": len(Ls), "cells": ncells, "lanes": lanes, "steps": steps, "ms_per_step": dt / steps * 1e3,
-            "value": batched, "plain": plain, "plain_sample": f"{len(pick)} landscapes of the batch, one at a time",
-            "batched_over_plain": batched / plain, "hbm_bytes_resident": mem[0],
-            "newton_iters_per_step": float(np.sum([s["newton_iters"] for s in stats]))}
+def spread(Ls, n):
+    """n members spread over the batch sorted by size (smallest to largest), in batch order."""
+    order = sorted(range(len(Ls)), key=lambda i: (Ls[i]["nx"] * Ls[i]["ny"], i))
+    pick = {order[int(round(j * (len(order) - 1) / max(1, n - 1)))] for j in range(n)}
+    return sorted(pick)
 
 
-def advect_leg(fs, Ls, steps, lanes):
+def uniform_leg(fs, steps, lanes):
+    Ls = W.uniform_batch()
+    dt, stats, mem = run_gpu(fs, Ls, steps, 2, tune=True, lanes=lanes)
+    ncells = W.cells(Ls)
+    return {"workload": "uniform batch 64 x 513x513 D8 n=1 g=0 kd=1e-2 fixed BC (dx 200 m, dt 1e3)",
+            "landscapes": len(Ls), "cells": ncells, "lanes": lanes, "steps": steps,
+            "ms_per_step": dt / steps * 1e3, "value": ncells * steps / dt, "hbm_bytes_resident": mem[0],
+            "levels_per_landscape": int(np.mean([s["levels"] for s in stats])),
+            "basins_per_landscape": int(np.mean([s["nbasins"] for s in stats]))}
+
+
+def mixed_leg(fs, steps):
+    """The large multiple-flow landscape and 64 small D8 landscapes, one context (one launch per phase)."""
+    Ls = W.mixed_batch()
+    dt, stats, mem = run_gpu(fs, Ls, steps, 1, tune=True)
+    ncells = W.cells(Ls)
+    return {"workload": "mixed batch: 1 x 4097^2 MFD p=1 g=1 (dx 100 m) + 64 x 257^2 D8 n=1, one context, dt 1e3",
+            "cells": ncells, "steps": steps, "ms_per_step": dt / steps * 1e3, "value": ncells * steps / dt,
+            "hbm_bytes_resident": mem[0], "gauss_seidel_iters_large": stats[0]["ngs"]}
+
+
+def advect_leg(fs, steps, lanes):
     """The uniform batch with a velocity field (vx = 2 cm/yr, vy = -1 cm/yr, +-20 % noise), so every step
     starts with the advection of h, b and etot (fsx_set_v)."""
+    Ls = W.uniform_batch()
     with open_batch(fs, Ls, lanes) as b:
         b.graphs(GRAPHS)
         for k, L in enumerate(Ls):
-            configure(b, L, k)
+            W.configure(b, L, k)
             rng = np.random.default_rng(500 + k)
             nn = L["nx"] * L["ny"]
             b.set_v(k, 0.02 * (0.8 + 0.4 * rng.random(nn)), -0.01 * (0.8 + 0.4 * rng.random(nn)))
@@ -288,21 +210,21 @@ def advect_leg(fs, Ls, steps, lanes):
         b.step(steps)
         _sync()
         dt = time.perf_counter() - t0
-    ncells = sum(L["nx"] * L["ny"] for L in Ls)
-    return {"workload": f"uniform batch {len(Ls)} x {Ls[0]['nx']}x{Ls[0]['ny']} with advection first (set_v)",
-            "lanes": lanes, "steps": steps, "ms_per_step": dt / steps * 1e3, "value": ncells * steps / dt}
+    return {"workload": "uniform batch 64 x 513x513 with advection first (set_v)",
+            "lanes": lanes, "steps": steps, "ms_per_step": dt / steps * 1e3, "value": W.cells(Ls) * steps / dt}
 
 
-def readout_leg(fs, Ls):
-    """The read-outs (slope, curvature, chi, catchment, fluxes) of every landscape of the batch after two
-    steps, one host call per landscape, each copying its nn values to host memory (PCIe included)."""
+def readout_leg(fs):
+    """The read-outs (slope, curvature, chi, catchment, fluxes) of every landscape of the uniform batch
+    after two steps, one host call per landscape, each copying its nn values to host memory (PCIe
+    included); then all landscapes in one launch."""
     from fastscapelib_fortran import _lib
+    Ls = W.uniform_batch()
     nn = Ls[0]["nx"] * Ls[0]["ny"]
-    res = {"workload": f"{len(Ls)} x {Ls[0]['nx']}x{Ls[0]['ny']} after 2 steps, ms per landscape, "
-                       "including the copy of the field to host memory"}
+    res = {"workload": "64 x 513x513 after 2 steps, ms per landscape, including the copy of the field to host memory"}
     with fs.Batch([(L["nx"], L["ny"]) for L in Ls]) as b:
         for k, L in enumerate(Ls):
-            configure(b, L, k)
+            W.configure(b, L, k)
         b.step(2)
         out = np.empty(nn, np.float64)
         calls = [("slope", lambda k: b.derived(k, _lib.FSX_SLOPE, out)),
@@ -328,7 +250,7 @@ def readout_leg(fs, Ls):
             batched[name] = round((time.perf_counter() - t0) / len(Ls) * 1e3, 4)
         res["batched"] = batched
         res["batched_note"] = "all landscapes in one launch, ms per landscape (the same copies to host)"
-    return res
+    return res, Ls[0]
 
 
 def readout_cpu_baseline(L):
@@ -342,7 +264,7 @@ def readout_cpu_baseline(L):
         o.setup(L["nx"], L["ny"])
     else:
         o = O.COracle(L["nx"], L["ny"])
-    configure(o, L)
+    W.configure(o, L)
     o.execute_step()
     cpu = {"unit": "ms per landscape", "cores": 1, "kind": "reference" if use_ref else "port",
            "sample": f"1 x {L['nx']}x{L['ny']} after one step"}
@@ -356,73 +278,77 @@ def readout_cpu_baseline(L):
     return cpu
 
 
-def serving_leg(fs, quantum, duration, sample):
+def serving_leg(fs, window, deadline, quantum, sample):
+    """Arrivals of the serving stream during `window` seconds, served in real time; the loop stops at
+    `deadline` seconds (runs still resident or waiting then are reported as unfinished)."""
     from fastscapelib_fortran import serving
-    stream = serving_stream(duration=duration)
+    stream = [x for x in W.serving_stream(duration=window)]
     cap, per_node = serving.pool_capacity(0.5)
     need = sum(L["nx"] * L["ny"] for _, _, L in stream)
-    cap = int(min(cap, max(need, 1)))
-
-    def make(realtime):
-        return [serving.Run(rid=i, nx=L["nx"], ny=L["ny"], steps=st, arrival=t,
-                            setup=(lambda p, k, L=L: configure(p, L, k))) for i, (t, st, L) in enumerate(stream)]
-    out = {"workload": f"Poisson 20 runs/s for {duration:g} s, grids log-uniform[65,513]^2, steps log-uniform[20,200],"
-                       " D8/MFD 50/50", "runs": len(stream), "quantum": quantum, "pool_nodes": cap,
-           "bytes_per_node": round(per_node, 1)}
-    offered = sum(L["nx"] * L["ny"] * st for _, st, L in stream) / duration
+    cap = int(min(cap, max(need, max(L["nx"] * L["ny"] for _, _, L in stream))))
+    runs = [serving.Run(rid=i, nx=L["nx"], ny=L["ny"], steps=st, arrival=t,
+                        setup=(lambda p, k, L=L: W.configure(p, L, k))) for i, (t, st, L) in enumerate(stream)]
+    offered = sum(L["nx"] * L["ny"] * st for _, st, L in stream)
     with fs.Pool(cap, slots=256) as p:
         p.graphs(GRAPHS)
-        rep = serving.Server(p, quantum=quantum).serve(make(True), realtime=True)
-    out.update({"offered_cell_steps_per_s": offered, "achieved_cell_steps_per_s": rep["cell_steps_per_s"],
-                "p50_latency_s": rep["p50_latency_s"], "p99_latency_s": rep["p99_latency_s"],
-                "wall_s": rep["wall_s"], "rejected": rep["rejected"]})
-    with fs.Pool(cap, slots=256) as p:
-        p.graphs(GRAPHS)
-        sat = serving.Server(p, quantum=quantum).serve(make(False), realtime=False)
-    out["saturated_cell_steps_per_s"] = sat["cell_steps_per_s"]
-    pick = [L for _, _, L in stream][::max(1, len(stream) // sample)][:sample]
-    out["plain"] = plain_rate(fs, pick, 10)
-    out["served_over_plain"] = sat["cell_steps_per_s"] / out["plain"]
+        srv = serving.Server(p, quantum=quantum)
+        rep = srv.serve(runs, realtime=True, deadline=deadline)
+    out = {"workload": f"Poisson 20 runs/s for {window:g} s; grids log-uniform[65,2049]^2, steps log-uniform[50,2000], "
+                       "dt log-uniform[1e3,1e4], D8/MFD 50/50, n in {1,1.5,2}, Newton tol 1e-8",
+           "runs": len(stream), "quantum": quantum, "pool_nodes": cap, "bytes_per_node": round(per_node, 1),
+           "offered_cell_steps": offered, "deadline_s": deadline}
+    out.update({k: rep[k] for k in ("runs_finished", "runs_unfinished", "cell_steps", "wall_s", "cell_steps_per_s",
+                                    "p50_latency_s", "p99_latency_s", "busy_cell_steps_per_s")})
+    out["latency_note"] = "completion latency (arrival to copy-out) of the runs finished by the deadline"
+    pick = [stream[i][2] for i in spread([L for _, _, L in stream], sample)]
+    out["plain"] = plain_rate(fs, pick, 5)
+    out["plain_sample"] = f"{len(pick)} runs of the stream spread over its sizes, one at a time, 5 steps each"
+    out["served_over_plain"] = out["busy_cell_steps_per_s"] / out["plain"]
     return out
 
 
 # ---- CPU baseline: the reference Fortran (oracle/_ref), one landscape per process -------------------
 def _cpu_worker(args):
-    L, steps, use_ref = args
+    L, steps, lib = args
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    if use_ref:
-        o = O.FortranRef()
+    if lib:
+        o = O.FortranRef(lib)
         o.setup(L["nx"], L["ny"])
     else:
         o = O.COracle(L["nx"], L["ny"])
     t0 = time.perf_counter()
-    configure(o, L)
+    W.configure(o, L)
     for _ in range(steps):
         o.execute_step()
     o.field("h")
     return time.perf_counter() - t0
 
 
-def cpu_baseline(Ls, steps, procs):
-    ref_lib = os.path.join(ROOT, "oracle", "_ref", "libfsref.so")
-    use_ref = os.path.exists(ref_lib)
-    if not use_ref:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle as O
+def cpu_baseline(Ls, procs, budget=6e6):
+    """The ragged batch's landscapes on the host's cores, one landscape per process (the reference keeps
+    its state in module variables), each for about `budget` cell-steps (its own step count)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    lib = O.REF8_LIB if O.ref_available(O.REF8_LIB) else None  # the build with Newton tolerance 1e-8
+    if not lib:
         O.build_c()
-    jobs = [(Ls[i % len(Ls)], steps, use_ref) for i in range(procs)]
+    members = spread(Ls, procs)
+    jobs = [(Ls[i], int(min(W.RAGGED_STEPS, max(2, round(budget / (Ls[i]["nx"] * Ls[i]["ny"]))))), lib)
+            for i in members]
     ctx = mp.get_context("spawn")
     t0 = time.perf_counter()
     with ctx.Pool(procs) as pool:
         times = pool.map(_cpu_worker, jobs)
     makespan = time.perf_counter() - t0
-    cells = sum(L["nx"] * L["ny"] for L, _, _ in jobs)
-    return {"value": cells * steps / makespan, "unit": "cell-steps/s", "cores": procs,
-            "kind": "reference" if use_ref else "port",
-            "sample": f"{procs} x {Ls[0]['nx']}x{Ls[0]['ny']} landscapes of the batch, {steps} steps each, one "
-                      f"process per core ({'unmodified reference Fortran, flang -O2' if use_ref else 'C port'}); "
-                      f"makespan {makespan:.1f} s, per-process {min(times):.1f}-{max(times):.1f} s"}
+    cs = sum(L["nx"] * L["ny"] * st for L, st, _ in jobs)
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None
+    return {"value": cs / makespan, "unit": "cell-steps/s", "cores": procs,
+            "kind": "reference" if lib else "port", "host_cpu_count": os.cpu_count(), "host_cpus_usable": aff,
+            "sample": f"{procs} landscapes of the ragged batch spread over its sizes, one process per core, each "
+                      f"~{budget:.0e} cell-steps ({min(s for _, s, _ in jobs)}-{max(s for _, s, _ in jobs)} steps; "
+                      + ("reference Fortran with Newton tolerance 1e-8, flang -O2" if lib else "C port") +
+                      f"); makespan {makespan:.1f} s, per-process {min(times):.1f}-{max(times):.1f} s"}
 
 
 def _alg_bytes(ph, stats, ncells):
@@ -445,16 +371,18 @@ def dominant_phase(timers, stats, ncells):
 def measured_traffic(ph, workload, lanes=1):
     """HBM bytes per launch of phase ph's kernel from the committed counter summary of this workload
     and lane count (FETCH_SIZE and WRITE_SIZE runs scaled by a copy of known size), or None."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_uniform_batch_64x513.json")
-    try:
-        with open(path) as f:
-            d = json.load(f)
-    except (OSError, ValueError):
-        return None
-    k = d.get("kernels", {}).get(ph)
-    if d.get("workload") != workload or d.get("lanes", 1) != lanes or not k:
-        return None
-    return int(k["fetch_bytes"] + k["write_bytes"])
+    for name in sorted(os.listdir(os.path.join(ROOT, "profiles"))):
+        if not (name.startswith("hbm_traffic_") and name.endswith(".json")):
+            continue
+        try:
+            with open(os.path.join(ROOT, "profiles", name)) as f:
+                d = json.load(f)
+        except (OSError, ValueError):
+            continue
+        k = d.get("kernels", {}).get(ph)
+        if d.get("workload") == workload and d.get("lanes", 1) == lanes and k:
+            return int(k["fetch_bytes"] + k["write_bytes"])
+    return None
 
 
 def roofline(ph, timers, stats, ncells):
@@ -486,19 +414,20 @@ def copy_leg(nbytes=1 << 30, reps=5):
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--prof-steps", type=int, default=10)
-    ap.add_argument("--ninst", type=int, default=64)
-    ap.add_argument("--n", type=int, default=513)
-    ap.add_argument("--plain-steps", type=int, default=40)
-    ap.add_argument("--cpu-steps", type=int, default=200)
-    ap.add_argument("--cpu-procs", type=int, default=0, help="0: min(16, host cores)")
+    ap.add_argument("--prof-steps", type=int, default=5)
+    ap.add_argument("--plain-steps", type=int, default=6)
+    ap.add_argument("--plain-sample", type=int, default=8)
+    ap.add_argument("--cpu-procs", type=int, default=0, help="0: the host's usable cores, at most 16")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--legs", default="uniform,plain,ragged,serving,advect,readouts",
-                    help="comma list of uniform, plain, ragged, serving, copy (copy alone: HBM-counter calibration)")
-    ap.add_argument("--ragged-steps", type=int, default=10)
-    ap.add_argument("--serve-seconds", type=float, default=8.0)
+    ap.add_argument("--legs", default="ragged,plain,uniform,mixed,serving,advect,readouts",
+                    help="comma list of ragged (the headline), plain, uniform, mixed, serving, advect, readouts; "
+                         "copy alone: HBM-counter calibration")
+    ap.add_argument("--uniform-steps", type=int, default=20)
+    ap.add_argument("--mixed-steps", type=int, default=3)
+    ap.add_argument("--serve-window", type=float, default=5.0)
+    ap.add_argument("--serve-deadline", type=float, default=60.0)
     ap.add_argument("--quantum", type=int, default=8)
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every kernel directly instead of replaying HIP graphs (for kernel-trace profiling)")
@@ -506,7 +435,7 @@ def main():
     ap.add_argument("--lanes", type=int, default=2,
                     help="contexts stepping the batch side by side on their own streams (1: one context)")
     ap.add_argument("--shapes", default=None,
-                    help="walk_gx,jump_blocks,tour_gx,nr_small_gx for the headline batch instead of the autotuner")
+                    help="walk_gx,jump_blocks,tour_gx,nr_small_gx,area_wg,bor_gx instead of the autotuner")
     args = ap.parse_args()
     global GRAPHS, TUNE, SHAPES
     GRAPHS = not args.no_graphs
@@ -524,57 +453,68 @@ def main():
         return
     import fastscapelib_fortran as fs
 
-    Ls = uniform_batch(args.ninst, args.n)
-    workload = f"uniform batch {args.ninst} x {args.n}x{args.n} D8 n=1 g=0 kd=1e-2 fixed BC"
-    ncells = sum(L["nx"] * L["ny"] for L in Ls)
-    dt, timers, live, dom, stats, mem, tuned, sel = run_headline(fs, Ls, args.steps, args.warmup, args.prof_steps,
-                                                                 args.lanes)
-    value = ncells * args.steps / dt
-    sel_cells = sum(Ls[k]["nx"] * Ls[k]["ny"] for k in sel)
+    def note(msg):
+        print(f"bench: {msg}", file=sys.stderr, flush=True)
 
-    # plain path: one landscape at a time (the first of the batch)
-    plain, pdt = None, None
-    if "plain" in legs:
-        pdt, _, _, _ = run_gpu(fs, Ls[:1], args.plain_steps, 1, profile=False)
-        plain = Ls[0]["nx"] * Ls[0]["ny"] * args.plain_steps / pdt
-
-    phases = {p: round(ms / max(1, args.prof_steps), 4) for p, (ms, c) in timers.items() if c}
-    out = {
-        "metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded uniform-noise landscapes)",
-        "config": {"workload": workload,
-                   "landscapes": args.ninst, "cells": ncells, "lanes": args.lanes},
-        "plain": ({"value": plain, "ms_per_step": pdt / args.plain_steps * 1e3,
-                   "workload": f"1 x {args.n}x{args.n}, same parameters"} if plain else None),
-        "batched_over_plain": value / plain if plain else None,
-        "roofline": dict(roofline(dom, live, [stats[k] for k in sel], sel_cells),
-                         traffic=measured_traffic(dom, workload, args.lanes),
-                         measured_on=f"lane 0 of {args.lanes}: {len(sel)} landscapes" if args.lanes > 1 else "all"),
-        "launch_shapes": {"per_lane": tuned,
-                          "chosen_by": "autotuner" if TUNE else ("given" if SHAPES else "defaults")},
-        "phase_ms_per_step": phases,
-        "phase_ms_per_step_note": f"profiled window of {args.prof_steps} steps, kernels launched one by one"
-                                  + (f"; lane 0 of {args.lanes} ({len(sel)} landscapes), the others beside it"
-                                     if args.lanes > 1 else ""),
-        "step_model_frac": round(336.0 * ncells / (dt / args.steps) / 1e9 / HBM_PEAK_GBS, 5),
-        "hbm_bytes_resident": mem[0],
-        "basins_per_landscape": int(np.mean([s["nbasins"] for s in stats])),
-        "levels_per_landscape": int(np.mean([s["levels"] for s in stats])),
-    }
+    Ls = W.ragged_batch()
+    ncells = W.cells(Ls)
+    out = {"metric": METRIC, "value": None, "unit": "cell-steps/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded landscapes, fastscapelib_fortran/workloads.py)",
+           "config": {"workload": RAGGED_WORKLOAD, "landscapes": len(Ls), "cells": ncells, "lanes": args.lanes,
+                      "window": f"{args.steps} timed steps of the workload's 300, after {args.warmup} warm-up "
+                                f"steps and the autotuner"}}
     if "ragged" in legs:
-        out["ragged"] = ragged_leg(fs, args.ragged_steps, sample=8, lanes=args.lanes)
+        note("ragged batch (headline)")
+        r = run_headline(fs, Ls, args.steps, args.warmup, args.prof_steps, args.lanes)
+        sel_cells = sum(Ls[k]["nx"] * Ls[k]["ny"] for k in r["sel"])
+        out["value"] = ncells * args.steps / r["dt"]
+        out["ms_per_step"] = r["dt"] / args.steps * 1e3
+        out["config"]["lanes"] = r["lanes"]
+        out["roofline"] = dict(roofline(r["dom"], r["live"], [r["stats"][k] for k in r["sel"]], sel_cells),
+                               traffic=measured_traffic(r["dom"], RAGGED_WORKLOAD, r["lanes"]),
+                               measured_on=(f"lane 0 of {r['lanes']}: {len(r['sel'])} landscapes, {sel_cells} nodes"
+                                            if r["lanes"] > 1 else "all landscapes"))
+        out["launch_shapes"] = {"per_lane": r["tuned"],
+                                "chosen_by": "autotuner" if TUNE else ("given" if SHAPES else "defaults")}
+        out["phase_ms_per_step"] = {p: round(ms / max(1, args.prof_steps), 4) for p, (ms, c) in r["prof"].items() if c}
+        out["phase_ms_per_step_note"] = (f"profiled window of {args.prof_steps} steps, kernels launched one by one"
+                                         + (f"; lane 0 of {r['lanes']} ({len(r['sel'])} landscapes), the others "
+                                            "beside it" if r["lanes"] > 1 else ""))
+        out["hbm_bytes_resident"] = r["mem"][0]
+        out["newton_iters_per_step"] = float(np.sum([s["newton_iters"] for s in r["stats"]]))
+        out["levels_max"] = int(max(s["levels"] for s in r["stats"]))
+        out["basins_per_landscape"] = int(np.mean([s["nbasins"] for s in r["stats"]]))
+    if "plain" in legs:
+        note("plain (one landscape at a time)")
+        pick = [Ls[i] for i in spread(Ls, args.plain_sample)]
+        plain = plain_rate(fs, pick, args.plain_steps)
+        out["plain"] = {"value": plain, "sample": f"{len(pick)} landscapes of the batch spread over its sizes, each "
+                                                   f"alone, {args.plain_steps} steps after 1 warm-up step and the autotuner"}
+        if out["value"]:
+            out["batched_over_plain"] = out["value"] / plain
+    if "uniform" in legs:
+        note("uniform batch")
+        out["uniform"] = uniform_leg(fs, args.uniform_steps, args.lanes)
+    if "mixed" in legs:
+        note("mixed batch")
+        out["mixed"] = mixed_leg(fs, args.mixed_steps)
     if "serving" in legs:
-        out["serving"] = serving_leg(fs, args.quantum, args.serve_seconds, sample=8)
+        note("serving")
+        out["serving"] = serving_leg(fs, args.serve_window, args.serve_deadline, args.quantum, sample=6)
     if "advect" in legs:
-        out["advect"] = advect_leg(fs, Ls, 20, args.lanes)
+        note("advection")
+        out["advect"] = advect_leg(fs, 10, args.lanes)
     if "readouts" in legs:
-        out["readouts"] = readout_leg(fs, Ls)
+        note("read-outs")
+        out["readouts"], L0 = readout_leg(fs)
         if not args.no_cpu:
-            out["readouts"]["cpu_baseline"] = readout_cpu_baseline(Ls[0])
+            out["readouts"]["cpu_baseline"] = readout_cpu_baseline(L0)
     if not args.no_cpu:
-        procs = args.cpu_procs or min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(Ls, args.cpu_steps, procs)
+        note("cpu baseline")
+        aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        procs = args.cpu_procs or max(1, min(16, aff))
+        out["cpu_baseline"] = cpu_baseline(Ls, procs)
     print(json.dumps(out))
 
 

@@ -34,6 +34,7 @@ static int fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
 }
+int fsx_fail(int code, const char* msg) { return fail(code, std::string(msg)); }
 
 #define HIPCHK(x)                                                                     \
   do {                                                                                \
@@ -184,7 +185,10 @@ static int upload_params(fsx_ctx* c) {
   if (!c->dirty) return 0;
   c->any_sfd = c->any_mfd = false;
   c->any_advect = false;
-  for (const InstParams& P : c->hp) {
+  for (InstParams& P : c->hp) {
+    // g = g1, or g2 where h - b > 1 when g2 > 0 (src/StreamPowerLaw.f90:236-239): zero everywhere exactly when
+    // g1 == 0 and g2 <= 0, and then the Gauss-Seidel loop runs once (:375-376)
+    P.flow = (P.sfd && P.g1 == 0.0 && !(P.g2 > 0.0)) ? 1 : 0;
     if (P.active && P.run_advect) c->any_advect = true;
     if (!P.active || !P.run_spl) continue;
     if (P.sfd) c->any_sfd = true; else c->any_mfd = true;
@@ -245,7 +249,14 @@ static int relayout(fsx_ctx* c) {
   for (int k = 0; k < S.ninst; k++)
     if (c->used[k]) by_off.push_back({c->hp[k].off, k});
   std::sort(by_off.begin(), by_off.end());
-  for (size_t r = 0; r < by_off.size(); r++) c->hp[by_off[r].second].aux = by_off[r].first + (int)r;
+  // the per-landscape level bounds of the last step live at aux: a landscape whose aux moves needs a
+  // new step before its chi and fluxes can be read out
+  for (size_t r = 0; r < by_off.size(); r++) {
+    InstParams& P = c->hp[by_off[r].second];
+    const int aux = by_off[r].first + (int)r;
+    if (P.aux != aux) c->routed[by_off[r].second] = 0;
+    P.aux = aux;
+  }
   for (int k = 0; k < S.ninst; k++) {
     InstParams& P = c->hp[k];
     P.c0 = (int)chunks.size();
@@ -357,7 +368,8 @@ static int slot_add(fsx_ctx* c, int nx, int ny, int* inst, bool relay) {
   c->used[k] = 1;
   c->steps[k] = 0;
   c->routed[k] = 0;
-  c->xl[k] = c->yl[k] = 0.0;
+  c->xl[k] = (double)(nx - 1);  // the geometry() default: dx = dy = 1 until set_xl_yl
+  c->yl[k] = (double)(ny - 1);
   c->ibc[k] = 1111;
   DevState& S = c->S;
   k_slot_init<<<(int)std::min<long long>(1024, (nn + 255) / 256), 256, 0, c->stream>>>(
@@ -673,7 +685,7 @@ static void enqueue_route_only(fsx_ctx* c, hipStream_t st) {
 
 static void enqueue_step(fsx_ctx* c, hipStream_t st) {
   DevState& S = c->S;
-  if (c->any_advect) PHASE(PH_UPLIFT, launch_advect(S, st));  // Advect precedes Uplift (api.f90:209-221)
+  if (c->any_advect) PHASE(PH_ADVECT, launch_advect(S, st));  // Advect precedes Uplift (api.f90:209-221)
   PHASE(PH_UPLIFT, launch_uplift(S, st));
   enqueue_routing(c, st);
   PHASE(PH_SPL_PREP, launch_spl_prep(S, st));

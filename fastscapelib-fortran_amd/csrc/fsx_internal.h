@@ -36,6 +36,8 @@ struct InstParams {
   int run_uplift, run_spl, run_diff, sfd;
   int run_advect;             // a velocity field was set (SetV)
   int active;
+  int flow;                  // single flow without deposition (g == 0 everywhere): one stream-power pass,
+                             // solved node by node as receivers complete (k_spl_flow)
   int aux;                   // offset of the nn+1-entry per-landscape arrays (level starts, groups)
   int c0, c1;                // chunk-table range of the landscape
   double dx, dy, dt;
@@ -205,6 +207,9 @@ __device__ __forceinline__ double dval(unsigned long long k) {
 }
 
 // ------------------------------------------------------------------------------------------
+// Records msg as the calling thread's last error (fsx_last_error) and returns code.
+int fsx_fail(int code, const char* msg);
+
 // Host-side launch wrappers (defined in the .hip files).
 struct DevState;
 void launch_advect(DevState& S, hipStream_t st);  // needs S.vx/S.vy
@@ -268,5 +273,6 @@ enum Phase {
   PH_DIFFUSION,    //   k_adi_prep, k_adi_x, k_adi_y, k_adi_post
   PH_MFD_ROUTE,    //   multiple flow direction: fill, receivers, donors, Kahn layers, accumulation
   PH_MFD_SPL,      //   multiple flow direction: edge factors + Gauss-Seidel stream power
+  PH_ADVECT,       // * k_advect (first in the step when a velocity field is set)
   PH_COUNT
 };

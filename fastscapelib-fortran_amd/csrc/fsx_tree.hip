@@ -232,6 +232,13 @@ __global__ __launch_bounds__(FSX_WG) void k_level_sort(const InstParams* __restr
 }
 
 // ---- per-position records -----------------------------------------------------------------------
+// A value of the dataflow sweep that is not written yet: a NaN whose payload no arithmetic produces from
+// the finite (or canonical NaN) values the sweep computes.
+#define FLOW_SENT 0xFFF8DEADBEEF0001ull
+__device__ __forceinline__ bool flow_sent(double x) {
+  return (unsigned long long)__double_as_longlong(x) == FLOW_SENT;
+}
+
 // Drainage-area inputs: own term dx*dy*precip and the positions (relative to the next level's start)
 // of the donors in descending node index; unused slots -1.
 __global__ __launch_bounds__(FSX_CHUNK) void k_area_prep(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
@@ -268,7 +275,8 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_spl_fact(const InstParams* __rest
                                                         const double* __restrict__ kfint, const double* __restrict__ ht,
                                                         SplRec32* __restrict__ srec, double* __restrict__ lsed,
                                                         const double* __restrict__ g, unsigned long long* __restrict__ redk,
-                                                        const int* __restrict__ flags) {
+                                                        const int* __restrict__ flags, double* __restrict__ Hl,
+                                                        double* __restrict__ Wl, double* __restrict__ HWl) {
   __shared__ unsigned long long smax[2][FSX_CHUNK / 64];
   const Chunk c = chunks[chunk_id()];
   const InstParams& P = PP[c.inst];
@@ -306,7 +314,8 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_spl_fact(const InstParams* __rest
   int q = -1;
   double f = 0.0;
   if (d > 0) {
-    q = pos[P.off + r] - lstart[P.aux + d - 1];
+    // the level sweep reads the receiver relative to its level's start, the dataflow sweep absolutely
+    q = P.flow ? pos[P.off + r] : pos[P.off + r] - lstart[P.aux + d - 1];
     const double am = pow(av, P.m);
     const double n = P.n;
     if (fabs(n - 1.0) < 2.2250738585072014e-308) {
@@ -319,6 +328,12 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_spl_fact(const InstParams* __rest
   w[0] = make_ulonglong2((unsigned long long)(unsigned)v | ((unsigned long long)(unsigned)q << 32),
                          (unsigned long long)__double_as_longlong(f));
   w[1] = make_ulonglong2((unsigned long long)__double_as_longlong(t), 0ull);
+  if (P.flow) {  // not yet solved: k_spl_flow waits for a receiver's values to leave the sentinel
+    const double sx = __longlong_as_double((long long)FLOW_SENT);
+    Hl[gp] = sx;
+    Wl[gp] = sx;
+    HWl[gp] = sx;
+  }
 }
 
 // ---- staged level groups ---------------------------------------------------------------------------
@@ -563,7 +578,7 @@ __global__ __launch_bounds__(FSX_WG) void k_spl_sweep(
   __shared__ unsigned long long s_newton;
   const int inst = blockIdx.x;
   const InstParams& P = PP[inst];
-  if (!P.active || !P.run_spl || !P.sfd) return;
+  if (!P.active || !P.run_spl || !P.sfd || P.flow) return;  // flow instances: k_spl_flow
   // per-level double buffers (h, water, hwater, lake sill) addressed by offset from the LDS base
 #define BH(par) (sm + (par) * vcap)
 #define BW(par) (sm + (2 + (par)) * vcap)
@@ -850,6 +865,357 @@ __global__ __launch_bounds__(FSX_WG) void k_spl_sweep(
   }
 }
 
+// ---- sweeps without levels: dataflow over positions ------------------------------------------------
+// The level sweeps pay one workgroup barrier per level, and every level waits for its slowest node (for
+// the stream-power law, the longest Newton iteration).  The two kernels below instead solve each node as
+// soon as the node(s) it depends on are done, so the critical path is the longest receiver chain's own
+// work, not the sum of per-level maxima.  One workgroup per landscape:
+//
+// * Positions (level order: every receiver before its donors) are cut into tiles of 64; waves take
+//   tiles in order from a workgroup counter (the stream-power pass from the first position up, the
+//   drainage area from the last position down), keep the current and the next tile's records in
+//   registers (one position per lane), and hand positions to their lanes one by one: a free lane takes
+//   the next position of its wave's pool (records fetched by cross-lane shuffles), waits for the nodes it
+//   depends on, solves, publishes and takes another.  Lanes of one wave thus never idle behind a slow
+//   lane (a Newton iteration of tens of passes, a chain inside the tile).
+// * A finished node publishes its values to global memory (set to a sentinel by the prep kernel) and to
+//   an LDS ring (slot = position mod R) under a sequence tag: the writer claims the slot by swapping its
+//   tag to -2 (compare-and-swap: one writer at a time), writes the values, then the position (release);
+//   a reader accepts the slot only when it reads its position as the tag both before and after the
+//   values.  A reader that finds a later position there reads global
+//   memory (the value left the ring); one that finds an earlier one (not published yet, as a rule) polls
+//   the ring again and global memory every FLOW_GLOBAL-th pass (slots are written out of order now and
+//   then; global memory always ends up holding the value).
+// * Progress: positions are issued in order within a wave, so a lane holding a position of a later tile
+//   implies all earlier positions of its pool were issued.  The lowest pending position of the landscape
+//   is therefore held by some lane (or its wave has only free lanes and issues it next pass), and what it
+//   depends on is published.  All waves are resident on one compute unit (one workgroup), whose L1 keeps
+//   their global stores and loads coherent (workgroup-scope atomics).  A wave that sees no progress for
+//   FLOW_SPIN passes reports FSX_ERR_LEVELS and the workgroup stops; with a valid level order that cannot
+//   happen.
+#define FLOW_WG 1024
+#define FLOW_SPIN (1 << 26)
+#define FLOW_GLOBAL 32
+
+__device__ __forceinline__ double flow_ld(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void flow_st(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+// LDS accesses (the address space is inferred after inlining)
+__device__ __forceinline__ double lds_ld(double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void lds_st(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ int lds_acq(int* p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ int lds_rlx(int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void lds_rel(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void lds_stw(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void wg_fence_rel() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup"); }
+__device__ __forceinline__ void wg_fence_acq() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup"); }
+
+// Ring of NV doubles per slot and a tag; writes and reads under the sequence tag (see above).
+template <int R, int NV>
+struct FlowRing {
+  double* v;  // NV * R, value k of slot s at v[k * R + s]
+  int* tag;   // R
+  // Claims the slot (tag -> -2 by compare-and-swap, so two writers never mix their values), writes the
+  // values, then the tag.  A slot being written by another lane is left to it (readers then find p's
+  // value in global memory).
+  __device__ __forceinline__ void put(int p, const double* x) {
+    const int s = p & (R - 1);
+    const int old = lds_rlx(tag + s);
+    if (old == -2 || atomicCAS(tag + s, old, -2) != old) return;
+#pragma unroll
+    for (int k = 0; k < NV; k++) lds_st(v + k * R + s, x[k]);
+    lds_rel(tag + s, p);
+  }
+  __device__ __forceinline__ bool get(int q, double* x, int& tg) {  // tg: the slot's tag
+    const int s = q & (R - 1);
+    tg = lds_acq(tag + s);
+    if (tg != q) return false;
+#pragma unroll
+    for (int k = 0; k < NV; k++) x[k] = lds_ld(v + k * R + s);
+    wg_fence_acq();
+    tg = lds_rlx(tag + s);
+    return tg == q;
+  }
+};
+
+struct FlowWin {
+  int next, abort;
+};
+__device__ __forceinline__ void flow_win_init(FlowWin& W) {
+  if (threadIdx.x == 0) {
+    W.next = 0;
+    W.abort = 0;
+  }
+}
+__device__ __forceinline__ int flow_take(FlowWin& W) {  // wave-uniform take rank
+  int k = 0;
+  if ((threadIdx.x & 63) == 0) k = atomicAdd(&W.next, 1);
+  return __builtin_amdgcn_readfirstlane(k);
+}
+__device__ __forceinline__ bool flow_aborted(FlowWin& W) {
+  return __hip_atomic_load(&W.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != 0;
+}
+// one pass without progress: false (and the workgroup aborted) past the spin budget
+__device__ __forceinline__ bool flow_idle(FlowWin& W, int& spin, int* err) {
+  if (++spin > FLOW_SPIN || flow_aborted(W)) {
+    if ((threadIdx.x & 63) == 0) {
+      __hip_atomic_store(&W.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      atomicOr(err, FSX_ERR_LEVELS);
+    }
+    return false;
+  }
+  __builtin_amdgcn_s_sleep(1);
+  return true;
+}
+
+__device__ __forceinline__ double shfl_d(double x, int src) {
+  const long long v = __double_as_longlong(x);
+  const int lo = __shfl((int)(unsigned)(v & 0xffffffffll), src, 64);
+  const int hi = __shfl((int)(v >> 32), src, 64);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// ---- top-down: the single stream-power pass of g == 0 ----------------------------------------------
+// With g == 0 everywhere the Gauss-Seidel loop runs once (src/StreamPowerLaw.f90:375-376) and a node's
+// solve needs only its receiver's h, water and lake surface (:296-366, the fill :73-89).  A Newton lane
+// advances SPL_NK iterations per pass of its wave's loop.
+#define SPL_R 4096  // ring slots
+#define SPL_NK 4
+
+struct FlowRec {
+  int o, q;  // node, absolute receiver position (-1: base level)
+  double f, t;
+};
+__device__ __forceinline__ void flow_load(FlowRec& r, const SplRec32* srec, long long gp) {
+  const ulonglong2* w = reinterpret_cast<const ulonglong2*>(srec + gp);
+  const ulonglong2 a = w[0], b = w[1];
+  r.o = (int)(unsigned)(a.x & 0xffffffffull);
+  r.q = (int)(unsigned)(a.x >> 32);
+  r.f = __longlong_as_double((long long)a.y);
+  r.t = __longlong_as_double((long long)b.x);
+}
+
+static size_t spl_flow_smem() { return (size_t)SPL_R * (sizeof(int) + 3 * sizeof(double)); }
+
+__global__ __launch_bounds__(FLOW_WG) void k_spl_flow(const InstParams* __restrict__ PP, const int* __restrict__ nlev,
+                                                      const SplRec32* __restrict__ srec, double* __restrict__ Hl,
+                                                      double* __restrict__ Wl, double* __restrict__ HWl,
+                                                      double* __restrict__ scal, int* __restrict__ ngs_out,
+                                                      long long* __restrict__ newton_out, int* __restrict__ err,
+                                                      int fill_only) {
+  extern __shared__ double spl_ring[];  // h[R], water[R], lake surface[R], tag[R]
+  __shared__ FlowWin W;
+  __shared__ unsigned long long s_newton;
+  FlowRing<SPL_R, 3> ring{spl_ring, reinterpret_cast<int*>(spl_ring + 3 * SPL_R)};
+  const int inst = blockIdx.x;
+  const InstParams& P = PP[inst];
+  if (!P.active || !P.run_spl || !P.sfd || !P.flow) return;
+  if (nlev[inst] == 0) return;  // no level order this step (receiver cycle, already reported)
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int nn = P.nn;
+  const long long off = P.off;
+  const int ntiles = (nn + 63) / 64;
+  const double n = P.n;
+  const bool n_is_one = fabs(n - 1.0) < 2.2250738585072014e-308;
+  const int nkind = npow_kind(n);
+  const double omega = 0.875 / n, tolp = P.newton_tol;
+  flow_win_init(W);
+  for (int i = tid; i < SPL_R; i += FLOW_WG) ring.tag[i] = -1;
+  if (tid == 0) s_newton = 0;
+  __syncthreads();
+  auto tsize = [&](int t) { return min(64, nn - 64 * t); };
+  // record of position 64 t + lane, read unconditionally (clamped into the landscape; positions past the
+  // end are never issued) so that the loads stay in flight until the bank is used
+  auto load = [&](int t, int& q, double& f, double& u) {
+    const int pp = min(64 * min(t, ntiles - 1) + lane, nn - 1);
+    FlowRec r;
+    flow_load(r, srec, off + pp);
+    q = r.q;
+    f = r.f;
+    u = r.t;
+  };
+  // lane state: 0 free, 1 waiting for the receiver, 2 Newton
+  int ph = 0, p = 0, q = -1, it = 0, polls = 0, iss = 0, spin = 0, npass = 0;
+  bool gpass = false;
+  double f = 0.0, tt = 0.0, hr = 0.0, wr = 0.0, hw = 0.0, hv = 0.0;
+  unsigned long long my_newton = 0;
+#define SPL_PUBLISH(HV, WV, HW)                \
+  do {                                         \
+    const double px_[3] = {(HV), (WV), (HW)};  \
+    ring.put(p, px_);                          \
+    flow_st(Hl + off + p, px_[0]);             \
+    flow_st(Wl + off + p, px_[1]);             \
+    flow_st(HWl + off + p, px_[2]);            \
+    ph = 0;                                    \
+  } while (0)
+  // One pass of the wave with tile TC (records BQ, BF, BU, one per lane) as the current tile: issue its
+  // next positions to free lanes, then poll, solve and publish.  ST becomes 1 when the tile is used up (the
+  // caller refills the bank), 2 when the wave is done (or the workgroup aborted), 0 otherwise.
+#define SPL_PASS(TC, BQ, BF, BU, ST)                                                       \
+  do {                                                                                     \
+    int used_ = 0;                                                                         \
+    bool prog_ = false;                                                                    \
+    gpass = (++npass & (FLOW_GLOBAL - 1)) == 0;                                            \
+    if ((TC) < ntiles) {                                                                   \
+      const unsigned long long fm = __ballot(ph == 0);                                    \
+      const int nfree = __popcll(fm);                                                      \
+      const int rank = __popcll(fm & ((1ull << lane) - 1ull));                             \
+      const int avail = tsize(TC) - iss;                                                   \
+      const int j = iss + (rank < avail ? rank : 0);                                       \
+      const int sq = __shfl((BQ), j, 64);                                                  \
+      const double sf = shfl_d((BF), j), su = shfl_d((BU), j);                             \
+      if (ph == 0 && rank < avail) {                                                       \
+        ph = 1;                                                                            \
+        p = 64 * (TC) + j;                                                                 \
+        q = sq;                                                                            \
+        f = sf;                                                                            \
+        tt = su;                                                                           \
+        polls = 0;                                                                         \
+      }                                                                                    \
+      const int took = nfree < avail ? nfree : avail;                                      \
+      if (took) prog_ = true;                                                              \
+      iss += took;                                                                         \
+      if (iss == tsize(TC)) {                                                              \
+        iss = 0;                                                                           \
+        used_ = 1;                                                                         \
+      }                                                                                    \
+    }                                                                                      \
+    spl_work(prog_);                                                                       \
+    if ((TC) >= ntiles && __ballot(ph != 0) == 0ull) {                                     \
+      ST = 2;                                                                              \
+    } else if (__ballot(prog_) != 0ull) {                                                  \
+      spin = 0;                                                                            \
+      ST = used_;                                                                          \
+    } else {                                                                               \
+      ST = flow_idle(W, spin, err + inst) ? used_ : 2;                                     \
+    }                                                                                      \
+  } while (0)
+  // poll, solve and publish (the part of a pass after the issue)
+  auto spl_work = [&](bool& prog) __attribute__((always_inline)) {
+    // waiting lanes poll their receiver
+    if (ph == 1) {
+      bool ok = true;
+      if (q >= 0) {
+        double x[3];
+        int tg;
+        ok = ring.get(q, x, tg);
+        // global memory: once when the slot holds a later position (q's value, as a rule, left the
+        // ring), and on every FLOW_GLOBAL-th pass of the wave (all lanes at once, one wait per wave)
+        if (!ok && ((tg > q && polls == 0) || gpass)) {
+          polls = 1;
+          x[0] = flow_ld(Hl + off + q);
+          x[1] = flow_ld(Wl + off + q);
+          x[2] = flow_ld(HWl + off + q);
+          ok = !flow_sent(x[0]) && !flow_sent(x[1]) && !flow_sent(x[2]);
+        }
+        hr = x[0];
+        wr = x[1];
+        hw = x[2];
+      }
+      if (ok) {
+        prog = true;
+        if (q < 0) {
+          SPL_PUBLISH(tt, tt, tt);
+        } else {
+          const double hwv = hw > tt ? hw + 1.0e-8 : tt;
+          hw = hwv;
+          const double e = tt;
+          if (fill_only) {
+            SPL_PUBLISH(tt, wr, hwv);
+          } else if (!(e > wr)) {
+            SPL_PUBLISH(e, wr > e ? wr : e, hwv);
+          } else if (n_is_one) {
+            double ff = e, df = 1.0;
+            ff = ff + f * hr;
+            df = df + f;
+            double h1 = ff / df;
+            if (h1 < wr) h1 = wr;
+            SPL_PUBLISH(h1, wr > h1 ? wr : h1, hwv);
+          } else {
+            ph = 2;
+            hv = tt;
+            it = 0;
+          }
+        }
+      }
+    }
+    // Newton lanes: up to SPL_NK iterations, then publish when converged (errp <= tolp)
+    if (ph == 2) {
+      prog = true;
+      const double h0 = tt;
+      const bool act = f >= 0.0;
+      bool conv = false;
+      for (int k = 0; k < SPL_NK && !conv; k++) {
+        double ff = hv - h0, df = 1.0;
+        if (act) {
+          const double x = fmax(0.0, hv - hr);
+          double pn, pn1;
+          npow(x, n, nkind, pn, pn1);
+          ff = ff + f * pn;
+          df = df + f * n * pn1;
+        }
+        const double hn = hv - ff / df;
+        const double errp = fabs(hn - hv);
+        hv = hv * (1.0 - omega) + hn * omega;
+        conv = !(errp > tolp);
+        if (++it > 100000) {
+          atomicOr(err + inst, FSX_ERR_NEWTON);
+          conv = true;
+        }
+      }
+      if (conv) {
+        my_newton += it;
+        double h1 = hv;
+        if (h1 < wr) h1 = wr;
+        SPL_PUBLISH(h1, wr > h1 ? wr : h1, hw);
+      }
+    }
+  };
+  // two banks of records, the current tile's and the next one's, used in turn; a bank is refilled right
+  // after its tile is used up, so its loads are waited for only when it is current again
+  int ta = flow_take(W), tb = flow_take(W);
+  int qa, qb;
+  double fa, fb, ua, ub;
+  load(ta, qa, fa, ua);
+  load(tb, qb, fb, ub);
+  while (true) {
+    int st = 0;
+    while (st == 0) SPL_PASS(ta, qa, fa, ua, st);
+    if (st == 2) break;
+    ta = flow_take(W);
+    load(ta, qa, fa, ua);
+    st = 0;
+    while (st == 0) SPL_PASS(tb, qb, fb, ub, st);
+    if (st == 2) break;
+    tb = flow_take(W);
+    load(tb, qb, fb, ub);
+  }
+#undef SPL_PASS
+#undef SPL_PUBLISH
+  if (my_newton) atomicAdd(&s_newton, my_newton);
+  __syncthreads();
+  if (tid == 0 && !fill_only) {
+    ngs_out[inst] = 1;
+    newton_out[inst] = (long long)s_newton;
+    scal[inst] = 0.0;  // g == 0: no lake sediment to spread (k_spl_post)
+  }
+}
+
 // ---------------------------------------------------------------------------------------------------
 static size_t area_smem(int vcap) {
   return 2 * vcap * sizeof(double) + GRP_STG * (sizeof(double) + 8 * sizeof(int)) + (GRP_BUCKET + 1) * sizeof(int);
@@ -868,6 +1234,10 @@ int sweep_set_lds(int smem_max) {
   for (const void* f : {(const void*)k_area_sweep<256>, (const void*)k_area_sweep<512>, (const void*)k_area_sweep<1024>})
     if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)area_smem(kAreaVcap)) != hipSuccess) return -1;
   if (hipFuncSetAttribute((const void*)k_spl_sweep, hipFuncAttributeMaxDynamicSharedMemorySize, (int)spl_smem(kSplVcap)) != hipSuccess) return -1;
+  if ((int)(spl_flow_smem() + sizeof(FlowWin) + 64) > smem_max) return -1;
+  if (hipFuncSetAttribute((const void*)k_spl_flow, hipFuncAttributeMaxDynamicSharedMemorySize, (int)spl_flow_smem()) !=
+      hipSuccess)
+    return -1;
   if (hipFuncSetAttribute((const void*)k_level_sort, hipFuncAttributeMaxDynamicSharedMemorySize, LVL_HCAP * 4) != hipSuccess) return -1;
   return 0;
 }
@@ -909,10 +1279,13 @@ void launch_area_sweep(DevState& S, hipStream_t st) {
 void launch_spl_fact(DevState& S, hipStream_t st) {
   launch_fill_int(reinterpret_cast<int*>(S.redk), 4LL * S.ninst, 0, st);
   k_spl_fact<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.J, S.rec, S.pos, S.lstart, S.Al, S.area, S.length,
-                                               S.kfint, S.ht, S.srec, S.lsed, S.g, S.redk, S.flags);
+                                               S.kfint, S.ht, S.srec, S.lsed, S.g, S.redk, S.flags, S.Hl, S.Wl, S.HWl);
 }
 
 static void spl_sweep_launch(DevState& S, hipStream_t st, int fill_only) {
+  // every landscape is taken by exactly one of the two kernels (InstParams::flow)
+  k_spl_flow<<<S.ninst, FLOW_WG, spl_flow_smem(), st>>>(S.P, S.nlev, S.srec, S.Hl, S.Wl, S.HWl, S.scal, S.ngs,
+                                                         S.newton, S.err, fill_only);
   k_spl_sweep<<<S.ninst, FSX_WG, spl_smem(kSplVcap), st>>>(
       S.P, S.lstart, S.nlev, S.grec, S.ngroups, S.srec, S.arec, S.bc, S.g, S.Al, S.h, S.hwater,
       S.lake_depth, S.water, S.lsill, S.lwv, S.lsed, S.Hl, S.Wl, S.HWl, S.LSl, S.HPl, S.DHl, S.ELl, S.scal, S.ngs,

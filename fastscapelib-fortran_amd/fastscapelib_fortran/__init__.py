@@ -62,6 +62,7 @@ class Batch:
         self._h = h
         self.sizes = dict(enumerate(sizes))
         self.nn = {k: a * b for k, (a, b) in self.sizes.items()}
+        self.slots = n
 
     # -- lifetime --
     def close(self):
@@ -219,28 +220,36 @@ class Batch:
     def catchment(self, k): return self.derived(k, _lib.FSX_CATCHMENT)
 
     def derived_all(self, which, ks=None):
-        """One read-out (FSX_SLOPE, FSX_CURVATURE, FSX_CHI or FSX_CATCHMENT) of the landscapes ks (all by
-        default) in one launch: a list of arrays in the order of ks."""
-        ks = list(range(len(self.nn))) if ks is None else list(ks)
+        """One read-out (FSX_SLOPE, FSX_CURVATURE, FSX_CHI or FSX_CATCHMENT) of the landscapes ks (every
+        landscape held, by id, by default) in one launch: a list of arrays in the order of ks."""
+        ks = sorted(self.nn) if ks is None else [int(k) for k in ks]
+        for k in ks:
+            if k not in self.nn:
+                raise ValueError(f"derived_all: no landscape {k}")
         outs = [np.empty(self.nn[k], np.float64) for k in ks]
-        ptrs = (C.c_void_p * len(self.nn))()
+        ptrs = (C.c_void_p * self.slots)()  # the library reads one entry per slot
         for k, o in zip(ks, outs):
             ptrs[k] = o.ctypes.data
         check(lib.fsx_copy_derived_batch(self._h, which, ptrs, 0), "copy_derived_batch")
         return outs
 
-    def fluxes_all(self):
-        """(tectonic, erosion, boundary) flux of every landscape in one launch: an array [n, 3]."""
-        out = np.empty((len(self.nn), 3), np.float64)
+    def fluxes_all(self, ks=None):
+        """(tectonic, erosion, boundary) flux of the landscapes ks (every landscape held, by id, by
+        default) in one launch: an array [len(ks), 3] in the order of ks."""
+        ks = sorted(self.nn) if ks is None else [int(k) for k in ks]
+        for k in ks:
+            if k not in self.nn:
+                raise ValueError(f"fluxes_all: no landscape {k}")
+        out = np.empty((self.slots, 3), np.float64)  # the library writes three values per slot
         check(lib.fsx_get_fluxes_batch(self._h, ptr(out)), "get_fluxes_batch")
-        return out
+        return out[ks] if ks else np.empty((0, 3), np.float64)
 
     def vtk(self, k, f, vex, directory="VTK"):
         """FastScape_VTK for landscape k: legacy-VTK files of h (and b, sea level for vex < 0) with the
         caller's field f, written on the host from the fields copied off the GPU."""
         from .vtk import write_vtk
         nx, ny = self.sizes[k]
-        xl, yl = self.__dict__.get("_xlyl", {}).get(k, (0.0, 0.0))
+        xl, yl = self.__dict__.get("_xlyl", {}).get(k, (float(nx - 1), float(ny - 1)))  # library default dx = 1
         return write_vtk(self.get_step(k), nx, ny, xl, yl, self.h(k), self.field(k, _lib.FSX_B), f, float(vex),
                          0.0, directory)
 
@@ -465,6 +474,7 @@ class Pool(Batch):
         check(lib.fsx_remove_instance(self._h, int(k)), "remove_instance")
         del self.sizes[k]
         del self.nn[k]
+        self.__dict__.get("_xlyl", {}).pop(k, None)
 
     def info(self):
         """(free nodes, largest free contiguous range, free slots)."""

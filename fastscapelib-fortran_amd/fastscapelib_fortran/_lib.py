@@ -21,7 +21,7 @@ FSX_SLOPE, FSX_CURVATURE, FSX_CHI, FSX_CATCHMENT = range(4)  # fsx_copy_derived
 # phase timers, in the library's order (include/fastscape_hip.h, fsx_get_timers)
 PHASES = ["uplift", "receivers", "donors", "lm_labels", "lm_basins", "lm_walk", "lm_edges", "lm_sort", "lm_mst",
           "lm_orient", "lm_correct", "levels", "area_prep", "area_sweep", "spl_prep", "spl_fact", "spl_sweep",
-          "spl_post", "diffusion", "mfd_route", "mfd_spl"]
+          "spl_post", "diffusion", "mfd_route", "mfd_spl", "advect"]
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)

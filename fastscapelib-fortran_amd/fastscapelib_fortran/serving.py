@@ -29,7 +29,8 @@ from . import _lib
 @dataclass
 class Run:
     """One landscape run: grid, step count and a ``setup(pool, k)`` that configures landscape k of
-    the pool (xl/yl, dt, init_h, erosional parameters, uplift, BC ...)."""
+    the pool (xl/yl, dt, init_h, erosional parameters, uplift, BC ...).  ``outputs`` names the fields
+    copied out when it completes: h, b, etot, erate, area, lake_depth, rec, ndon."""
     rid: int
     nx: int
     ny: int
@@ -54,6 +55,7 @@ class Run:
 
 _FIELDS = {"h": _lib.FSX_H, "b": _lib.FSX_B, "etot": _lib.FSX_ETOT, "erate": _lib.FSX_ERATE,
            "area": _lib.FSX_AREA, "lake_depth": _lib.FSX_LAKE_DEPTH}
+_IFIELDS = {"rec": _lib.FSX_REC, "ndon": _lib.FSX_NDON}  # receivers and donor counts of the last step
 
 
 def pool_capacity(hbm_fraction=0.8, slots=256):
@@ -82,6 +84,7 @@ class Server:
         self.rejected: List[Run] = []
         self.cell_steps = 0
         self.quanta = 0
+        self.busy = 0.0  # wall time of the quanta (admission, steps and retirement while runs are resident)
 
     def submit(self, run: Run) -> None:
         run.left = run.steps
@@ -110,7 +113,10 @@ class Server:
 
     def _retire(self, run: Run) -> None:
         for name in run.outputs:
-            run.result[name] = self.pool.field(run.slot, _FIELDS[name])
+            if name in _IFIELDS:
+                run.result[name] = self.pool.ifield(run.slot, _IFIELDS[name])
+            else:
+                run.result[name] = self.pool.field(run.slot, _FIELDS[name])
         self.pool.remove(run.slot)
         run.done = self.clock()
         self.active.pop(run.slot, None)
@@ -119,6 +125,7 @@ class Server:
     def tick(self) -> int:
         """Admit, advance every resident run by one quantum, retire finished runs.  Returns the
         number of steps taken (0 when nothing is resident)."""
+        t0 = self.clock()
         self._admit()
         if not self.active:
             return 0
@@ -130,12 +137,14 @@ class Server:
             self.cell_steps += run.cells * q
             if run.left == 0:
                 self._retire(run)
+        self.busy += self.clock() - t0
         return q
 
-    def serve(self, schedule: List[Run], realtime: bool = True) -> dict:
+    def serve(self, schedule: List[Run], realtime: bool = True, deadline: Optional[float] = None) -> dict:
         """Drive a whole arrival schedule (runs with ``arrival`` in seconds from the start).  With
         ``realtime`` the loop waits for arrivals on the wall clock; otherwise every run is available
-        at time 0."""
+        at time 0.  With ``deadline`` (seconds) the loop stops then, leaving unfinished runs where
+        they are (counted in the report)."""
         pending = sorted(schedule, key=lambda r: r.arrival)
         t0 = self.clock()
         i = 0
@@ -144,17 +153,22 @@ class Server:
             r.arrival = (base + r.arrival) if realtime else t0
         while i < len(pending) or self.queue or self.active:
             now = self.clock()
+            if deadline is not None and now - t0 >= deadline:
+                break
             while i < len(pending) and pending[i].arrival <= now:
                 self.submit(pending[i])
                 i += 1
             if self.tick() == 0 and i < len(pending) and not self.queue:
                 time.sleep(max(0.0, min(0.05, pending[i].arrival - self.clock())))
-        return self.report(self.clock() - t0)
+        rep = self.report(self.clock() - t0)
+        rep["runs_unfinished"] = len(pending) - len(self.finished) - len(self.rejected)
+        return rep
 
     def report(self, wall: float) -> dict:
-        lat = np.array([r.latency for r in self.finished]) if self.finished else np.zeros(1)
-        return {"runs": len(self.finished), "rejected": len(self.rejected), "cell_steps": int(self.cell_steps),
-                "wall_s": wall,
+        lat = np.array([r.latency for r in self.finished]) if self.finished else np.full(1, np.nan)
+        return {"runs": len(self.finished), "runs_finished": len(self.finished), "rejected": len(self.rejected),
+                "cell_steps": int(self.cell_steps), "wall_s": wall,
                 "cell_steps_per_s": self.cell_steps / wall if wall > 0 else 0.0,
+                "busy_cell_steps_per_s": self.cell_steps / self.busy if self.busy > 0 else 0.0,
                 "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
                 "quanta": self.quanta}

@@ -182,9 +182,9 @@ int fsx_get_stats(fsx_ctx *ctx, int inst, long long *out6);
  * (timeUplift/timeSPL/timeDiffusion, src/FastScape_api.f90:207-257; printed by FastScape_Debug,
  * src/FastScape_ctx.f90:553-559).  When on, every step records a HIP event pair around each phase
  * on the stream it runs on.  fsx_get_timers fills up to n accumulated milliseconds and counts
- * (phase order: uplift (with the advection before it when a velocity field is set), receivers,
+ * (phase order: uplift, receivers,
  * donors, lm_labels, lm_basins, lm_walk, lm_edges, lm_sort, lm_mst, lm_orient, lm_correct, levels, area_prep, area_sweep, spl_prep, spl_fact, spl_sweep, spl_post,
- * diffusion, mfd_route, mfd_spl) and returns the number of phases. */
+ * diffusion, mfd_route, mfd_spl, advect) and returns the number of phases. */
 int fsx_set_profiling(fsx_ctx *ctx, int on);
 /* Time one phase (0..fsx_get_timers' count-1; -1: none) live while steps replay HIP graphs: the
  * phases before it and after it run as two graphs and the phase itself is launched directly between

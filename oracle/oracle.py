@@ -19,6 +19,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 C_LIB = os.path.join(HERE, "liboracle.so")
 REF_LIB = os.path.join(HERE, "_ref", "libfsref.so")
+# the reference build with the stream-power Newton tolerance set to 1e-8 (make -C oracle ref8)
+REF8_LIB = os.path.join(HERE, "_ref", "tol8", "libfsref8.so")
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -39,8 +41,8 @@ def build_c(force: bool = False) -> str:
     return C_LIB
 
 
-def ref_available() -> bool:
-    return os.path.exists(REF_LIB)
+def ref_available(path: str = REF_LIB) -> bool:
+    return os.path.exists(path)
 
 
 class Stats(C.Structure):
@@ -314,14 +316,16 @@ class FortranRef:
     """The reference library compiled from its own sources.  Arguments go by reference; node indices
     in integer outputs are converted to 0-based."""
 
-    _lib = None
+    _libs: dict = {}
 
-    def __init__(self):
-        if FortranRef._lib is None:
-            if not ref_available():
-                raise FileNotFoundError(REF_LIB)
-            FortranRef._lib = C.CDLL(REF_LIB)
-        self.lib = FortranRef._lib
+    def __init__(self, path: str = REF_LIB):
+        """``path``: REF_LIB (the unmodified build) or REF8_LIB (Newton tolerance 1e-8).  Each build
+        keeps its own module state; use one build per process."""
+        if path not in FortranRef._libs:
+            if not ref_available(path):
+                raise FileNotFoundError(path)
+            FortranRef._libs[path] = C.CDLL(path)
+        self.lib = FortranRef._libs[path]
         self.nn = 0
 
     @staticmethod

@@ -58,6 +58,8 @@ def apply(obj, L, k=None):
                                  L["p"])
     obj.set_u(*a, L["u"])
     obj.set_bc(*a, L["ibc"])
+    if "newton_tol" in L:
+        obj.set_newton_tol(*a, float(L["newton_tol"]))
 
 
 def rel_err(a, b):

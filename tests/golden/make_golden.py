@@ -2,7 +2,9 @@
 (oracle/_ref/libfsref.so, built by `make -C oracle ref`).  Each fixture holds the seeded inputs and
 the reference's outputs, so tests can check the oracle and the GPU path without the reference.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py          # fixtures of the unmodified reference build
+    python tests/golden/make_golden.py tol8     # fixtures of the build with Newton tolerance 1e-8
+                                                # (make -C oracle ref8), names tol8_*
 """
 from __future__ import annotations
 
@@ -19,8 +21,8 @@ import oracle as O  # noqa: E402
 from conftest import landscape  # noqa: E402
 
 
-def run_ref(L, steps_to_save):
-    f = O.FortranRef()
+def run_ref(L, steps_to_save, lib=O.REF_LIB):
+    f = O.FortranRef(lib)
     f.setup(L["nx"], L["ny"])
     f.set_xl_yl(L["xl"], L["yl"])
     f.set_dt(L["dt"])
@@ -84,5 +86,28 @@ def main():
     np.savez_compressed(os.path.join(HERE, "indexx.npz"), **sorts)
 
 
+def main_tol8():
+    """The ragged-batch workload's Newton tolerance (1e-8) against the reference built with that
+    tolerance: single and multiple flow direction, n = 1.5 and 2, periodic and fixed boundaries."""
+    cases = {
+        "tol8_sfd_1010_n15": dict(ibc=1010, n=1.5, m=0.675),
+        "tol8_sfd_1111_n2": dict(ibc=1111, n=2.0, m=0.9),
+        "tol8_mfd_1010_n2": dict(ibc=1010, n=2.0, m=0.9, p=1.0),
+        "tol8_mfd_1111_n15": dict(ibc=1111, n=1.5, m=0.675, p=1.0),
+    }
+    for name, kw in cases.items():
+        L = landscape(47, 31, seed=len(name) + 100, dx=250.0, kf=2e-6, dt=2e3, sinus=(10.0, 4000.0), kd=3e-2, **kw)
+        L["newton_tol"] = 1e-8
+        np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **inputs(L), **run_ref(L, {1, 5, 20}, O.REF8_LIB))
+    L = landscape(97, 73, seed=808, dx=250.0, kf=3e-6, dt=2e3, sinus=(10.0, 9000.0), kd=1e-2, ibc=1010, n=2.0,
+                  m=0.9)
+    L["newton_tol"] = 1e-8
+    np.savez_compressed(os.path.join(HERE, "tol8_sfd_1010_n2_97x73.npz"), **inputs(L),
+                        **run_ref(L, {1, 30}, O.REF8_LIB))
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["tol8"]:
+        main_tol8()
+    else:
+        main()

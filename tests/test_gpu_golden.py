@@ -1,5 +1,6 @@
 """The HIP path against the reference Fortran's own outputs (tests/golden/*.npz, written from the
-unmodified reference build by tests/golden/make_golden.py), through the C ABI.
+unmodified reference build by tests/golden/make_golden.py, and tol8_* from the reference built with
+the Newton tolerance 1e-8, which these runs select with set_newton_tol), through the C ABI.
 
 Bar: receivers and donors bit-exact; h, b within 1e-10 relative; drainage area within 1e-12
 relative; etot/erate/lake_depth within 1e-10 of the elevation scale.
@@ -28,8 +29,9 @@ def _load(name):
     L = {k[3:]: g[k] for k in g if k.startswith("in_")}
     for k in ["nx", "ny", "ibc"]:
         L[k] = int(L[k])
-    for k in ["xl", "yl", "kfsed", "m", "n", "kdsed", "g1", "g2", "p", "dt"]:
-        L[k] = float(L[k])
+    for k in ["xl", "yl", "kfsed", "m", "n", "kdsed", "g1", "g2", "p", "dt", "newton_tol"]:
+        if k in L:
+            L[k] = float(L[k])
     return g, L
 
 
@@ -42,7 +44,8 @@ def test_trajectory_matches_reference(name):
     import fastscapelib_fortran as fs
     g, L = _load(name)
     saved = sorted({int(k.split("_")[-1]) for k in g if not k.startswith("in_")})
-    print(f"golden {name}: {L['nx']}x{L['ny']} ibc={L['ibc']} p={L['p']} n={L['n']} steps={saved}", flush=True)
+    print(f"golden {name}: {L['nx']}x{L['ny']} ibc={L['ibc']} p={L['p']} n={L['n']} "
+          f"tol={L.get('newton_tol', 1e-3)} steps={saved}", flush=True)
     with fs.Batch([(L["nx"], L["ny"])]) as b:
         apply(b, L, 0)
         done = 0

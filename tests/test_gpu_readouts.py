@@ -185,3 +185,51 @@ def test_batched_readouts_over_lanes():
         for k, L in enumerate(Ls):
             one = b.fluxes(k)
             assert all(abs(fa[k, q] - one[q]) <= _sum_tol(L["nx"] * L["ny"]) * max(abs(one[q]), 1.0) for q in range(3))
+
+
+def test_pool_readouts_with_free_slots_and_relayout():
+    """A pool with free slots and a removed landscape: the batched read-outs address landscapes by slot id
+    (one output entry per slot on the library side), and a landscape whose level bounds moved with the
+    re-layout refuses chi and fluxes until it is stepped again, then gives the values of a fresh step."""
+    import fastscapelib_fortran as fs
+    from fastscapelib_fortran import _lib
+    Ls = [landscape(30 + 4 * k, 22 + 3 * k, seed=40 + k, dx=300.0) for k in range(3)]
+    with fs.Pool(40000, slots=16) as p:
+        ids = [p.add(L["nx"], L["ny"]) for L in Ls]
+        for k, L in zip(ids, Ls):
+            apply(p, L, k)
+        p.step(2)
+        chi_before = {k: p.chi(k) for k in ids}
+        fl_before = {k: p.fluxes(k) for k in ids}
+        fa = p.fluxes_all()
+        assert fa.shape == (3, 3)
+        for row, k in zip(fa, sorted(ids)):
+            assert np.allclose(row, fl_before[k], rtol=1e-12, atol=0.0)
+        p.remove(ids[0])  # lowest node offset: the others' per-landscape offsets move
+        rest = ids[1:]
+        for k in rest:
+            with pytest.raises(fs.FastscapeError):
+                p.chi(k)
+            with pytest.raises(fs.FastscapeError):
+                p.fluxes(k)
+        assert np.isnan(p.fluxes_all()).all()
+        # the same landscapes set up again and stepped twice in a fresh pool give the reference values
+        p.step(0)
+        with fs.Pool(40000, slots=16) as q:
+            qid = [q.add(L["nx"], L["ny"]) for L in Ls[1:]]
+            for k, L in zip(qid, Ls[1:]):
+                apply(q, L, k)
+            q.step(2)
+            want_chi = [q.chi(k) for k in qid]
+            want_fl = [q.fluxes(k) for k in qid]
+        # p's landscapes are now at step 2; step p's and q's copies once more and compare
+        for k in rest:
+            assert np.array_equal(chi_before[k], want_chi[rest.index(k)])
+            assert np.allclose(fl_before[k], want_fl[rest.index(k)], rtol=1e-12, atol=0.0)
+        p.step(1)
+        outs = p.derived_all(_lib.FSX_CHI)
+        assert len(outs) == len(rest)
+        for o, k in zip(outs, sorted(rest)):
+            assert np.array_equal(o, p.chi(k))
+        fa = p.fluxes_all()
+        assert fa.shape == (len(rest), 3) and np.isfinite(fa).all()

@@ -28,8 +28,9 @@ def _L_from(g):
     L = {k[3:]: g[k] for k in g if k.startswith("in_")}
     for k in ["nx", "ny", "ibc"]:
         L[k] = int(L[k])
-    for k in ["xl", "yl", "kfsed", "m", "n", "kdsed", "g1", "g2", "p", "dt"]:
-        L[k] = float(L[k])
+    for k in ["xl", "yl", "kfsed", "m", "n", "kdsed", "g1", "g2", "p", "dt", "newton_tol"]:
+        if k in L:
+            L[k] = float(L[k])
     return L
 
 
@@ -277,3 +278,57 @@ def test_stream_power_satisfies_the_implicit_relation(oracle):
         assert abs(h[v] - (ht[v] + F * h[r]) / (1 + F)) <= 1e-12 * max(1.0, abs(h[v]))
         ok += 1
     assert ok > h.size // 4
+
+
+def _newton_residual(o, L, ht):
+    """|f(h)| / f'(h) of the implicit relation f(h) = h - ht + F*max(0, h - h_rec)**n, F =
+    kf*dt*a**m/length**n, over the nodes the relaxed Newton iteration solved and left above their receiver
+    (eroded below ht, strictly above its receiver's water level -- the highest h downstream, to which the
+    solve would clamp it -- and ht above the receiver's ht): the step it would still take."""
+    h, a, rec, ln = o.field("h"), o.field("area"), o.ifield("rec"), o.field("length")
+    n, m = L["n"], L["m"]
+    v = np.arange(h.size)
+    water, anc = h.copy(), rec.copy()  # max of h over each node's path to base level, by pointer doubling
+    for _ in range(int(np.ceil(np.log2(h.size))) + 1):
+        water = np.maximum(water, water[anc])
+        anc = anc[anc]
+    sel = (rec != v) & (h < ht) & (h > water[rec]) & (ht > ht[rec])
+    F = L["kf"][sel] * L["dt"] * a[sel] ** m / ln[sel] ** n
+    x = h[sel] - h[rec[sel]]
+    f = h[sel] - ht[sel] + F * x ** n
+    df = 1.0 + F * n * x ** (n - 1.0)
+    return np.abs(f) / df, int(sel.sum())
+
+
+@pytest.mark.parametrize("n", [1.5, 2.0])
+def test_newton_tolerance_bounds_the_residual(oracle, n):
+    """The relaxed Newton iteration stops once a full Newton step |f/f'| is below the tolerance and then
+    takes the relaxed step, so the step left at the result is below (1 - 0.875/n) * tol.  With the
+    reference's 1e-3 the residual steps reach far above 1e-8; with 1e-8 every one is below 1e-8: a
+    restatement that ignored the setting fails one of the two."""
+    L = landscape(61, 47, seed=12, dx=250.0, u=0.0, kd=0.0, kf=3e-6, n=n, m=0.45 * n, sinus=(10.0, 6000.0),
+                  dt=2e3)
+    steps = {}
+    for tol in (1e-3, 1e-8):
+        o = oracle.COracle(61, 47)
+        apply(o, L)
+        o.set_newton_tol(tol)
+        o.execute_step()
+        steps[tol], cnt = _newton_residual(o, L, L["h"])
+        assert cnt > 500
+    assert steps[1e-8].max() <= (1.0 - 0.875 / n) * 1e-8 * 1.01
+    assert steps[1e-3].max() > 1e-6
+
+
+def test_newton_tolerance_fixtures_depend_on_it(oracle):
+    """The tol8_* fixtures pin the tolerance: the same inputs at the reference's default 1e-3 end
+    elsewhere (far beyond the 1e-10 parity bar)."""
+    g = _golden("tol8_sfd_1111_n2.npz")
+    L = _L_from(g)
+    o = oracle.COracle(L["nx"], L["ny"])
+    apply(o, L)
+    o.set_newton_tol(1e-3)
+    for _ in range(20):
+        o.execute_step()
+    h = o.field("h")
+    assert np.max(np.abs(h - g["h_20"]) / np.abs(g["h_20"])) > 1e-8
