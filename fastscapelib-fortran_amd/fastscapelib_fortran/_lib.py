@@ -29,6 +29,7 @@ _vp = C.c_void_p
 
 
 FSX_OK, FSX_EINVAL, FSX_EHIP, FSX_ESTEP, FSX_ENOMEM = 0, -1, -2, -3, -4
+FSX_INDEXX_NOALLOC_MAX = 524288  # keys per fsx_indexx_noalloc call (include/fastscape_hip.h)
 
 
 class FastscapeError(RuntimeError):
@@ -67,6 +68,7 @@ def _load():
         "fsx_get_sizes": [_vp, C.c_int, _ip, _ip],
         "fsx_route": [_vp, _vp, C.c_int],
         "fsx_indexx": [C.c_int, _dp, _ip],
+        "fsx_indexx_noalloc": [C.c_int, _vp, _vp, _vp, C.c_int],
         "fsx_get_stats": [_vp, C.c_int, C.POINTER(C.c_longlong)],
         "fsx_memory": [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)],
         "fsx_set_profiling": [_vp, C.c_int],

@@ -174,6 +174,16 @@ int fsx_route(fsx_ctx *ctx, void *stream, int sync);
  * (the reference's indx) into perm.  Host arrays; runs the device sort. */
 int fsx_indexx(int n, const double *keys, int *perm);
 
+/* fsx_indexx without any device allocation.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops).  keys and
+ * perm are DEVICE arrays of n entries (perm receives the 1-based permutation); the work is enqueued on
+ * stream (0: the null stream) and, with sync != 0, waited for and checked.  The sort's working arrays
+ * are a fixed area in the library's device memory, one per GPU, held until the process ends: n is
+ * limited to FSX_INDEXX_NOALLOC_MAX (larger n returns FSX_EINVAL).  Calls that overlap on one GPU
+ * share that area and race; serialize them (one stream, or wait between calls). */
+#define FSX_INDEXX_NOALLOC_MAX 524288
+int fsx_indexx_noalloc(int n, const double *keys, int *perm, void *stream, int sync);
+
 /* Statistics of the last step of an instance: out[0]=nbasins, out[1]=nconn, out[2]=ntree,
  * out[3]=levels, out[4]=Gauss-Seidel iterations, out[5]=Newton iterations (total). */
 int fsx_get_stats(fsx_ctx *ctx, int inst, long long *out6);
