@@ -49,8 +49,7 @@ METRIC = "cell-steps/s batched+served vs plain on 1 GPU; p50/p99 run latency"
 # Algorithmic (compulsory) bytes of the single-kernel phases: every input read once and every
 # output written once, per unit (node of the batch unless stated).
 ALG_BYTES_PER_NODE = {
-    "uplift": 40,        # R h,u,b  W h,b
-    "receivers": 20,     # R h  W rec(i32), length
+    "receivers": 52,     # uplift fused: R h,u,b  W ht,b, rec(i32), length
     "donors": 5,         # R rec  W dmask(u8)
     "lm_walk": 17,       # R (h, basin, preorder successor) record, candidate mask (closed-basin nodes ~ all)
     "area_prep": 65,     # R J, dmask, pos, donor pos, precip  W record (a0, 8 donor slots)
@@ -67,12 +66,9 @@ RAGGED_WORKLOAD = ("ragged batch 200 x log-uniform[65,1025]^2 (dx 250 m, noise +
 
 
 def _sync():
-    try:
-        import torch
-        if torch.cuda.is_available():
-            torch.cuda.synchronize()
-    except Exception:
-        pass
+    """Wait for the device (the library's own HIP runtime; a PyTorch build may bundle another)."""
+    from fastscapelib_fortran import _lib
+    _lib.device_synchronize()
 
 
 GRAPHS = True  # replay captured HIP graphs (off: launch every kernel directly, for kernel-trace profilers)
@@ -398,16 +394,18 @@ def roofline(ph, timers, stats, ncells):
 
 def copy_leg(nbytes=1 << 30, reps=5):
     """A plain device copy of known size (the HBM-counter calibration for the roofline's traffic)."""
-    import torch
-    n = nbytes // 8
-    x = torch.ones(n, dtype=torch.float64, device="cuda")
-    y = torch.empty_like(x)
-    torch.cuda.synchronize()
+    from fastscapelib_fortran import _lib
+    x = _lib.DeviceBuffer(nbytes)
+    y = _lib.DeviceBuffer(nbytes)
+    _lib.device_copy(x, y, nbytes)  # first touch
+    _lib.device_synchronize()
     t0 = time.perf_counter()
     for _ in range(reps):
-        y.copy_(x)
-    torch.cuda.synchronize()
+        _lib.device_copy(y, x, nbytes)
+    _lib.device_synchronize()
     dt = (time.perf_counter() - t0) / reps
+    x.free()
+    y.free()
     return {"bytes_read": nbytes, "bytes_written": nbytes, "ms": dt * 1e3, "GBps": 2 * nbytes / dt / 1e9}
 
 

@@ -51,7 +51,7 @@ struct fsx_ctx {
   std::vector<int> ibc;
   std::vector<char> used;                    // slot holds a landscape
   std::map<long long, long long> free_ranges;  // free node ranges of the arena (offset -> length)
-  long long capacity = 0, chunk_cap = 0;
+  long long capacity = 0, chunk_cap = 0, tile_cap = 0;
   std::vector<void*> allocs;
   hipStream_t stream = nullptr;
   bool dirty = true;
@@ -266,6 +266,26 @@ static int relayout(fsx_ctx* c) {
   }
   if ((long long)chunks.size() > c->chunk_cap) return fail(FSX_EINVAL, "chunk table overflow");
   S.nchunks = (int)chunks.size();
+  std::vector<int4> tiles;  // 2-D tiles, row-major within a landscape
+  for (int k = 0; k < S.ninst; k++) {
+    if (!c->used[k]) continue;
+    const InstParams& P = c->hp[k];
+    for (int y0 = 0; y0 < P.ny; y0 += FSX_TH)
+      for (int x0 = 0; x0 < P.nx; x0 += FSX_TW) tiles.push_back(make_int4(k, x0, y0, 0));
+  }
+  if ((long long)tiles.size() > c->tile_cap) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (S.tiles) HIPCHK(hipFree(S.tiles));
+    S.tiles = nullptr;
+    c->tile_cap = 0;
+    const long long cap = (long long)tiles.size() + tiles.size() / 4 + 64;
+    hipError_t e = hipMalloc(&S.tiles, sizeof(int4) * cap);
+    if (e != hipSuccess) return fail(FSX_ENOMEM, std::string("hipMalloc (tile table): ") + hipGetErrorString(e));
+    c->tile_cap = cap;
+  }
+  S.ntiles = (int)tiles.size();
+  if (!tiles.empty())
+    HIPCHK(hipMemcpyAsync(S.tiles, tiles.data(), sizeof(int4) * tiles.size(), hipMemcpyHostToDevice, c->stream));
   if (!chunks.empty())
     HIPCHK(hipMemcpyAsync(S.chunks, chunks.data(), sizeof(Chunk) * chunks.size(), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -456,6 +476,7 @@ extern "C" int fsx_destroy(fsx_ctx* c) {
   if (c->gpost) (void)hipGraphExecDestroy(c->gpost);
   for (auto& e : c->ev) (void)hipEventDestroy(e);
   for (void* p : c->allocs) (void)hipFree(p);
+  if (c->S.tiles) (void)hipFree(c->S.tiles);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return 0;
@@ -654,9 +675,11 @@ static int check_errors(fsx_ctx* c) {
   return 0;
 }
 
-static void enqueue_routing(fsx_ctx* c, hipStream_t st) {
+// uplift != 0: the step's uplift runs fused into the receiver search (Uplift precedes FlowRouting,
+// src/FastScape_api.f90:216-226)
+static void enqueue_routing(fsx_ctx* c, hipStream_t st, int uplift) {
   DevState& S = c->S;
-  PHASE(PH_RECEIVERS, launch_receivers(S, st));
+  PHASE(PH_RECEIVERS, launch_receivers(S, st, uplift));
   PHASE(PH_DONORS, launch_donors(S, st));
   // LocalMinima, src/LocalMinima.f90:1-132
   PHASE(PH_LM_LABELS, launch_lm_labels(S, st));
@@ -678,7 +701,7 @@ static void enqueue_routing(fsx_ctx* c, hipStream_t st) {
 }
 
 static void enqueue_route_only(fsx_ctx* c, hipStream_t st) {
-  enqueue_routing(c, st);
+  enqueue_routing(c, st, 0);
   launch_spl_prep(c->S, st);
   if (c->any_sfd) launch_fill(c->S, st);  // lake surface / depth without eroding
 }
@@ -686,8 +709,7 @@ static void enqueue_route_only(fsx_ctx* c, hipStream_t st) {
 static void enqueue_step(fsx_ctx* c, hipStream_t st) {
   DevState& S = c->S;
   if (c->any_advect) PHASE(PH_ADVECT, launch_advect(S, st));  // Advect precedes Uplift (api.f90:209-221)
-  PHASE(PH_UPLIFT, launch_uplift(S, st));
-  enqueue_routing(c, st);
+  enqueue_routing(c, st, 1);
   PHASE(PH_SPL_PREP, launch_spl_prep(S, st));
   if (c->any_sfd) {
     PHASE(PH_SPL_FACT, launch_spl_fact(S, st));
@@ -1129,13 +1151,13 @@ extern "C" int fsx_autotune(fsx_ctx* c, int reps, int* chosen) {
     double best_ms = 1e300;
     for (int i = 0; i < k.ncand && !rc; i++) {
       c->S.*k.field = k.cand[i];
-      enqueue_routing(c, c->stream);  // untimed pass: caches and code warm for this shape
+      enqueue_routing(c, c->stream, 0);  // untimed pass: caches and code warm for this shape
       ++c->ring_used;
       if ((rc = fsx_get_timers(c, nullptr, nullptr, 0)) < 0) break;
       rc = 0;
       fsx_reset_timers(c);
       for (int r = 0; r < reps && !rc; r++) {
-        enqueue_routing(c, c->stream);
+        enqueue_routing(c, c->stream, 0);
         if (++c->ring_used == kRing) rc = flush_timers_any(c, c->stream);  // one ring slot per pass
       }
       if ((rc = fsx_get_timers(c, nullptr, nullptr, 0)) < 0) break;

@@ -1462,7 +1462,7 @@ void launch_lm_basins(DevState& S, hipStream_t st) {
   k_root_scan<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.chunkcnt, S.nbasins, S.basin0);
   k_root_rank<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.rec, S.bc, S.chunkcnt, S.rootrank, S.outlet, S.basin0);
   launch_fill_int(S.bsize, S.ntot, 0, st);
-  k_basin_label<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.J, S.rootrank, S.bc, S.h, S.dmask, S.rec, S.basin,
+  k_basin_label<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.J, S.rootrank, S.bc, S.ht, S.dmask, S.rec, S.basin,
                                           S.bsize, S.wnode);
   k_bsize_scan<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.nbasins, S.bsize, S.soff, S.ecount);
   k_walk_mask<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.basin, S.wmask);

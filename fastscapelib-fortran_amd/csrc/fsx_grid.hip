@@ -15,46 +15,101 @@
   if (!P.active || v >= P.nn) return;                 \
   const long long g_ = (long long)P.off + v;
 
-// Uplift, src/Uplift.f90:9-10: h = h + u*dt, b = b + u*dt.
-__global__ __launch_bounds__(FSX_CHUNK) void k_uplift(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
-                                                      double* __restrict__ h, double* __restrict__ b,
-                                                      const double* __restrict__ u) {
-  NODE_KERNEL_PROLOGUE
-  if (!P.run_uplift) return;
-  const double ud = u[g_] * P.dt;
-  h[g_] = h[g_] + ud;
-  b[g_] = b[g_] + ud;
-}
-
-// find_receiver, src/FlowRouting.f90:302-362.  Nodes outside the BC bounds keep rec=self,
-// length=0; inside, the first strict maximum of (h(ij)-h(ijk))/l above tiny wins, jj outer.
-__global__ __launch_bounds__(FSX_CHUNK) void k_receivers(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
-                                                         const double* __restrict__ h, int* __restrict__ rec,
-                                                         double* __restrict__ length) {
-  NODE_KERNEL_PROLOGUE
-  if (!P.run_spl) return;
-  const double* hi = h + P.off;
-  const int i = v % P.nx, j = v / P.nx;
-  int r = v;
-  double l_best = 0.0;
-  if (i >= P.i1 && i <= P.i2 && j >= P.j1 && j <= P.j2) {
-    const double hv = hi[v];
-    double smax = 2.2250738585072014e-308;  // tiny(smax)
+// Uplift and find_receiver fused, over 2-D tiles of FSX_TW x FSX_TH nodes staged in LDS with a one-node
+// halo (one workgroup per tile, tiles of a landscape dealt to XCDs in contiguous runs so that the halo
+// rows of neighbouring tiles come from one L2).  The halo cell of direction (ii, jj) holds h of the node
+// find_receiver takes there (cyclic wrap else clamp, src/FlowRouting.f90:336-345), uplifted with the
+// same expression as the node's own tile computes it (Uplift, src/Uplift.f90:9-10: h + u*dt).  The
+// receiver search then reads only LDS: jj outer, ii inner, the first strict maximum of (h(ij)-h(ijk))/l
+// above tiny wins (:347-353), with the neighbour distances computed on the host by the reference's
+// expression.  Nodes outside the BC bounds keep rec = self, length = 0 (:322-325).
+// The uplifted h of a routed landscape goes to ht (its neighbours' tiles still read h); every kernel up to
+// k_spl_prep reads ht, and k_spl_prep writes it back to h.  A landscape without routing (run_spl == 0)
+// reads no neighbours and is uplifted in place.  uplift == 0: the receivers of the current h (ht = h).
+__global__ __launch_bounds__(256) void k_route_tile(const InstParams* __restrict__ PP, const int4* __restrict__ tiles,
+                                                    double* __restrict__ h, double* __restrict__ b,
+                                                    const double* __restrict__ u, double* __restrict__ ht,
+                                                    int* __restrict__ rec, double* __restrict__ length, int uplift) {
+  constexpr int W = FSX_TW + 2, H = FSX_TH + 2;
+  __shared__ double sh[H][W];
+  __shared__ double sud[FSX_TH][FSX_TW];
+  const int4 T = tiles[chunk_id()];
+  const InstParams& P = PP[T.x];
+  if (!P.active) return;
+  const bool up = uplift && P.run_uplift;
+  const int x0 = T.y, y0 = T.z, nx = P.nx, ny = P.ny;
+  const long long off = P.off;
+  const double dt = P.dt;
+  if (!P.run_spl) {  // uplift alone, in place
+    if (!up) return;
+    for (int k = threadIdx.x; k < FSX_TW * FSX_TH; k += 256) {
+      const int i = x0 + (k % FSX_TW), j = y0 + k / FSX_TW;
+      if (i >= nx || j >= ny) continue;
+      const long long g = off + i + (long long)j * nx;
+      const double ud = u[g] * dt;
+      h[g] = h[g] + ud;
+      b[g] = b[g] + ud;
+    }
+    return;
+  }
+  for (int c = threadIdx.x; c < W * H; c += 256) {
+    const int ly = c / W, lx = c - ly * W;
+    int gx = x0 + lx - 1, gy = y0 + ly - 1;
+    if (gx < 0 && P.xcyc) gx += nx;
+    if (gx > nx - 1 && P.xcyc) gx -= nx;
+    gx = gx < 0 ? 0 : (gx > nx - 1 ? nx - 1 : gx);
+    if (gy < 0 && P.ycyc) gy += ny;
+    if (gy > ny - 1 && P.ycyc) gy -= ny;
+    gy = gy < 0 ? 0 : (gy > ny - 1 ? ny - 1 : gy);
+    const long long g = off + gx + (long long)gy * nx;
+    double hv = h[g];
+    if (up) {
+      const double ud = u[g] * dt;
+      hv = hv + ud;
+      if (lx >= 1 && lx <= FSX_TW && ly >= 1 && ly <= FSX_TH) sud[ly - 1][lx - 1] = ud;
+    }
+    sh[ly][lx] = hv;
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < FSX_TW * FSX_TH; k += 256) {
+    const int tx = k % FSX_TW, ty = k / FSX_TW;
+    const int i = x0 + tx, j = y0 + ty;
+    if (i >= nx || j >= ny) continue;
+    const int v = i + j * nx;
+    const long long g = off + v;
+    const double hv = sh[ty + 1][tx + 1];
+    ht[g] = hv;
+    if (up) b[g] = b[g] + sud[ty][tx];
+    int r = v;
+    double l_best = 0.0;
+    if (i >= P.i1 && i <= P.i2 && j >= P.j1 && j <= P.j2) {
+      double smax = 2.2250738585072014e-308;  // tiny(smax)
 #pragma unroll
-    for (int jj = -1; jj <= 1; jj++) {
+      for (int jj = -1; jj <= 1; jj++) {
+        int jjj = j + jj;
+        if (jjj < 0 && P.ycyc) jjj += ny;
+        if (jjj < 0) jjj = 0;
+        if (jjj > ny - 1 && P.ycyc) jjj -= ny;
+        if (jjj > ny - 1) jjj = ny - 1;
 #pragma unroll
-      for (int ii = -1; ii <= 1; ii++) {
-        const int q = nbr_wc(P, i, j, ii, jj);
-        if (q != v) {
-          const double l = P.len[(ii + 1) + 3 * (jj + 1)];
-          const double slope = (hv - hi[q]) / l;
-          if (slope > smax) { smax = slope; r = q; l_best = l; }
+        for (int ii = -1; ii <= 1; ii++) {
+          int iii = i + ii;
+          if (iii < 0 && P.xcyc) iii += nx;
+          if (iii < 0) iii = 0;
+          if (iii > nx - 1 && P.xcyc) iii -= nx;
+          if (iii > nx - 1) iii = nx - 1;
+          const int q = iii + jjj * nx;
+          if (q != v) {
+            const double l = P.len[(ii + 1) + 3 * (jj + 1)];
+            const double slope = (hv - sh[ty + 1 + jj][tx + 1 + ii]) / l;
+            if (slope > smax) { smax = slope; r = q; l_best = l; }
+          }
         }
       }
     }
+    rec[g] = r;
+    length[g] = l_best;
   }
-  rec[g_] = r;
-  length[g_] = l_best;
 }
 
 // Donor masks: bit dbit(d) set when the neighbour of v in direction d drains to v.  Listing the
@@ -76,17 +131,17 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_donors(const InstParams* __restri
   dmask[g_] = (uint8_t)m;
 }
 
-// Stream-power set-up, src/StreamPowerLaw.f90:236-250: g, kfint, ht.
+// Stream-power set-up, src/StreamPowerLaw.f90:236-250: g, kfint, ht (written by k_route_tile; h := ht).
 __global__ __launch_bounds__(FSX_CHUNK) void k_spl_prep(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
-                                                        const double* __restrict__ h, const double* __restrict__ b,
+                                                        double* __restrict__ h, const double* __restrict__ b,
                                                         const double* __restrict__ kf, double* __restrict__ kfint,
-                                                        double* __restrict__ g, double* __restrict__ ht) {
+                                                        double* __restrict__ g, const double* __restrict__ ht) {
   NODE_KERNEL_PROLOGUE
   if (!P.run_spl) return;
-  const double hv = h[g_], d = hv - b[g_];
+  const double hv = ht[g_], d = hv - b[g_];  // ht: h after uplift (k_route_tile)
   if (P.g1 >= 0.0) g[g_] = (P.g2 > 0.0 && d > 1.0) ? P.g2 : P.g1;
   kfint[g_] = (P.kfsed > 0.0 && d > 1.0) ? P.kfsed : kf[g_];
-  ht[g_] = hv;
+  h[g_] = hv;
 }
 
 // Stream-power bookkeeping, src/StreamPowerLaw.f90:380-393: b=min(h,b), lake sediment fill,
@@ -450,11 +505,8 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_adi_post(const InstParams* __rest
 }
 
 // ---------------------------------------------------------------------------------------------
-void launch_uplift(DevState& S, hipStream_t st) {
-  k_uplift<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.h, S.b, S.u);
-}
-void launch_receivers(DevState& S, hipStream_t st) {
-  k_receivers<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.h, S.rec, S.length);
+void launch_receivers(DevState& S, hipStream_t st, int uplift) {
+  if (S.ntiles) k_route_tile<<<S.ntiles, 256, 0, st>>>(S.P, S.tiles, S.h, S.b, S.u, S.ht, S.rec, S.length, uplift);
 }
 void launch_donors(DevState& S, hipStream_t st) {
   k_donors<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.rec, S.dmask);

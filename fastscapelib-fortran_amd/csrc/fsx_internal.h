@@ -16,6 +16,8 @@
 #define FSX_SLAB_CAP 8   // candidate-pass slots per closed-basin node
 #define FSX_CHUNK 256    // nodes per block in node-parallel kernels
 #define FSX_WG 1024      // threads of the per-landscape workgroups
+#define FSX_TW 64        // 2-D tiles of the stencil kernels: columns
+#define FSX_TH 8         //   rows
 
 // device-side error bits, one word per instance
 enum {
@@ -213,8 +215,7 @@ int fsx_fail(int code, const char* msg);
 // Host-side launch wrappers (defined in the .hip files).
 struct DevState;
 void launch_advect(DevState& S, hipStream_t st);  // needs S.vx/S.vy
-void launch_uplift(DevState& S, hipStream_t st);
-void launch_receivers(DevState& S, hipStream_t st);
+void launch_receivers(DevState& S, hipStream_t st, int uplift);  // uplift fused in when uplift != 0
 void launch_donors(DevState& S, hipStream_t st);  // donor mask of every node
 void launch_lm_labels(DevState& S, hipStream_t st);
 void launch_lm_basins(DevState& S, hipStream_t st);
@@ -252,8 +253,8 @@ void launch_fill(DevState& S, hipStream_t st);
 // Phases timed with HIP events when profiling is on (the reference times Uplift, SPL and Diffusion
 // with cpu_time, src/FastScape_api.f90:207-257).  Phases marked * are exactly one kernel.
 enum Phase {
-  PH_UPLIFT = 0,   // * k_uplift
-  PH_RECEIVERS,    // * k_receivers
+  PH_UPLIFT = 0,   //   (uplift runs fused into the receiver phase)
+  PH_RECEIVERS,    // * k_route_tile (uplift + D8 receivers)
   PH_DONORS,       // * k_donors (before and after rerouting)
   PH_LM_LABELS,    //   pointer jumping for basin roots
   PH_LM_BASINS,    //   basin ids, sizes

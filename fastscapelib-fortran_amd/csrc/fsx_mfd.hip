@@ -579,7 +579,7 @@ __global__ __launch_bounds__(FSX_WG) void k_mfd_spl(
 
 // ---------------------------------------------------------------------------------------------------
 void launch_mfd_route(DevState& S, hipStream_t st) {
-  k_mfd_fill<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.lstart, S.nlev, S.order, S.rec, S.h, S.hwater, S.lake_depth);
+  k_mfd_fill<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.lstart, S.nlev, S.order, S.rec, S.ht, S.hwater, S.lake_depth);
   launch_fill_int(S.mindeg, S.ntot, 0, st);
   k_mfd_rec<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.hwater, S.rec, S.mrec, S.mw, S.mdir, S.mnrec, S.mflag,
                                               S.mindeg, S.pexp);

@@ -10,6 +10,8 @@ struct DevState {
   int any_mfd = 0, any_gs = 0;
   InstParams* P = nullptr;  // [ninst]
   Chunk* chunks = nullptr;  // [nchunks]
+  int ntiles = 0;           // 2-D tiles (FSX_TW x FSX_TH nodes) covering every instance
+  int4* tiles = nullptr;    // [ntiles]: (instance, x0, y0, -)
   // launch shapes, chosen by fsx_autotune (no result depends on them)
   int walk_gx = 256;       // basin walk: blocks per landscape
   int jump_blocks = 2048;  // pointer jumping: grid cap

@@ -126,6 +126,71 @@ def dev_ptr(t) -> C.c_void_p:
     return C.c_void_p(t.data_ptr())
 
 
+class DeviceBuffer:
+    """Device memory from the HIP runtime the library itself is linked against (no torch: a PyTorch
+    build may bundle a HIP runtime of its own), for callers that hand device arrays to the C ABI."""
+
+    _hip = None
+
+    def __init__(self, nbytes: int):
+        if DeviceBuffer._hip is None:
+            h = C.CDLL("libamdhip64.so.7")  # the runtime libfastscape_hip.so links (already loaded)
+            h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+            h.hipFree.argtypes = [C.c_void_p]
+            h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+            h.hipDeviceSynchronize.argtypes = []
+            for f in (h.hipMalloc, h.hipFree, h.hipMemcpy, h.hipDeviceSynchronize):
+                f.restype = C.c_int
+            DeviceBuffer._hip = h
+        self.nbytes = int(nbytes)
+        self.ptr = C.c_void_p()
+        rc = DeviceBuffer._hip.hipMalloc(C.byref(self.ptr), max(1, self.nbytes))
+        if rc != 0:
+            raise FastscapeError(f"hipMalloc({self.nbytes}) failed with status {rc}")
+
+    def upload(self, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        if a.nbytes > self.nbytes:
+            raise ValueError("array larger than the buffer")
+        rc = DeviceBuffer._hip.hipMemcpy(self.ptr, a.ctypes.data, a.nbytes, 1)  # host to device
+        if rc != 0:
+            raise FastscapeError(f"hipMemcpy failed with status {rc}")
+        return self
+
+    def download(self, a: np.ndarray) -> np.ndarray:
+        if not a.flags.c_contiguous or a.nbytes > self.nbytes:
+            raise ValueError("target must be contiguous and fit the buffer")
+        rc = DeviceBuffer._hip.hipMemcpy(a.ctypes.data, self.ptr, a.nbytes, 2)  # device to host
+        if rc != 0:
+            raise FastscapeError(f"hipMemcpy failed with status {rc}")
+        return a
+
+    def free(self):
+        if self.ptr.value:
+            DeviceBuffer._hip.hipFree(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def device_synchronize() -> None:
+    """hipDeviceSynchronize on the library's HIP runtime."""
+    DeviceBuffer(0).free()  # loads the runtime handle
+    rc = DeviceBuffer._hip.hipDeviceSynchronize()
+    if rc != 0:
+        raise FastscapeError(f"hipDeviceSynchronize failed with status {rc}")
+
+
+def device_copy(dst: DeviceBuffer, src: DeviceBuffer, nbytes: int) -> None:
+    rc = DeviceBuffer._hip.hipMemcpy(dst.ptr, src.ptr, int(nbytes), 3)  # device to device
+    if rc != 0:
+        raise FastscapeError(f"hipMemcpy failed with status {rc}")
+
+
 def indexx(keys) -> np.ndarray:
     """loc_min_3_indexx on the device: the reference's 1-based sort permutation of keys."""
     k = np.ascontiguousarray(np.asarray(keys, dtype=np.float64).reshape(-1))

@@ -20,18 +20,18 @@ def _keys(n, seed):
 
 
 def test_indexx_noalloc_matches_indexx():
-    import torch
-
     from fastscapelib_fortran import _lib
     lib = _lib.lib
     for n, seed in [(300000, 1), (4999, 2)]:
         keys = _keys(n, seed)
         want = _lib.indexx(keys)
-        kd = torch.from_numpy(keys).cuda()
-        pd = torch.full((n,), -7, dtype=torch.int32, device="cuda")
+        kd = _lib.DeviceBuffer(8 * n).upload(keys)
+        pd = _lib.DeviceBuffer(4 * n).upload(np.full(n, -7, np.int32))
         print(f"indexx_noalloc: n={n}", flush=True)
-        _lib.check(lib.fsx_indexx_noalloc(n, _lib.dev_ptr(kd), _lib.dev_ptr(pd), None, 1), "fsx_indexx_noalloc")
-        got = pd.cpu().numpy()
+        _lib.check(lib.fsx_indexx_noalloc(n, kd.ptr, pd.ptr, None, 1), "fsx_indexx_noalloc")
+        got = pd.download(np.empty(n, np.int32))
+        kd.free()
+        pd.free()
         assert np.array_equal(got, want), n
         assert np.all(np.diff(keys[got - 1]) >= 0)
     big = _lib.FSX_INDEXX_NOALLOC_MAX + 1
