@@ -112,23 +112,67 @@ __global__ __launch_bounds__(256) void k_route_tile(const InstParams* __restrict
   }
 }
 
-// Donor masks: bit dbit(d) set when the neighbour of v in direction d drains to v.  Listing the
-// set bits in ascending neighbour index reproduces find_donor (src/FlowRouting.f90:366-389).
-// Also threads the donors as first[v] (smallest donor) and next[v] (next larger donor of rec(v)),
-// which lets a depth-first preorder walk advance with one load per step.  Every thread writes only
-// its own node: next[v] is gathered from the receivers around rec(v).
-__global__ __launch_bounds__(FSX_CHUNK) void k_donors(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
-                                                      const int* __restrict__ rec, uint8_t* __restrict__ dmask) {
-  NODE_KERNEL_PROLOGUE
-  if (!P.run_spl) return;
-  const int* ri = rec + P.off;
-  int q[8], dir[8];
-  const int c8 = neighbours_wc(P, v, q, dir);
-  unsigned m = 0;
+// Donor masks over the same 2-D tiles: the receivers of a tile and its one-node halo (wrapped or clamped
+// as find_receiver takes neighbours) in LDS, then for each node the distinct neighbours in direction
+// order (jj outer, ii inner, itself and repeated clamp targets skipped, as neighbours_wc) whose receiver
+// is the node: find_donor's donor set (src/FlowRouting.f90:366-389) as a mask, bit dbit(d) for direction d.
+__global__ __launch_bounds__(256) void k_donors_tile(const InstParams* __restrict__ PP, const int4* __restrict__ tiles,
+                                                     const int* __restrict__ rec, uint8_t* __restrict__ dmask) {
+  constexpr int W = FSX_TW + 2, H = FSX_TH + 2;
+  __shared__ int sr[H][W];
+  const int4 T = tiles[chunk_id()];
+  const InstParams& P = PP[T.x];
+  if (!P.active || !P.run_spl) return;
+  const int x0 = T.y, y0 = T.z, nx = P.nx, ny = P.ny;
+  const long long off = P.off;
+  for (int c = threadIdx.x; c < W * H; c += 256) {
+    const int ly = c / W, lx = c - ly * W;
+    int gx = x0 + lx - 1, gy = y0 + ly - 1;
+    if (gx < 0 && P.xcyc) gx += nx;
+    if (gx > nx - 1 && P.xcyc) gx -= nx;
+    gx = gx < 0 ? 0 : (gx > nx - 1 ? nx - 1 : gx);
+    if (gy < 0 && P.ycyc) gy += ny;
+    if (gy > ny - 1 && P.ycyc) gy -= ny;
+    gy = gy < 0 ? 0 : (gy > ny - 1 ? ny - 1 : gy);
+    sr[ly][lx] = rec[off + gx + (long long)gy * nx];
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < FSX_TW * FSX_TH; k += 256) {
+    const int tx = k % FSX_TW, ty = k / FSX_TW;
+    const int i = x0 + tx, j = y0 + ty;
+    if (i >= nx || j >= ny) continue;
+    const int v = i + j * nx;
+    unsigned m = 0;
+    if (i > 0 && j > 0 && i < nx - 1 && j < ny - 1) {  // interior: eight distinct neighbours
 #pragma unroll
-  for (int t = 0; t < 8; t++)
-    if (t < c8 && ri[q[t]] == v) m |= 1u << dbit(dir[t]);
-  dmask[g_] = (uint8_t)m;
+      for (int d = 0; d < 9; d++)
+        if (d != 4 && sr[ty + d / 3][tx + d % 3] == v) m |= 1u << dbit(d);
+    } else {
+      // Wrapped or clamped column / row of each offset.  Direction (ii, jj) repeats an earlier one
+      // of the jj-outer, ii-inner order exactly when an earlier ii' of the same row maps to the same
+      // column or an earlier jj' to the same row; it is the node itself when both map to it.
+      int ex[3], ey[3];
+#pragma unroll
+      for (int o = 0; o < 3; o++) {
+        int a = i + o - 1, c = j + o - 1;
+        if (a < 0 && P.xcyc) a += nx;
+        if (a > nx - 1 && P.xcyc) a -= nx;
+        ex[o] = a < 0 ? 0 : (a > nx - 1 ? nx - 1 : a);
+        if (c < 0 && P.ycyc) c += ny;
+        if (c > ny - 1 && P.ycyc) c -= ny;
+        ey[o] = c < 0 ? 0 : (c > ny - 1 ? ny - 1 : c);
+      }
+      const bool xd[3] = {false, ex[1] == ex[0], ex[2] == ex[0] || ex[2] == ex[1]};
+      const bool yd[3] = {false, ey[1] == ey[0], ey[2] == ey[0] || ey[2] == ey[1]};
+#pragma unroll
+      for (int d = 0; d < 9; d++) {
+        const int ii = d % 3, jj = d / 3;
+        if (d == 4 || xd[ii] || yd[jj] || (ex[ii] == i && ey[jj] == j)) continue;
+        if (sr[ty + jj][tx + ii] == v) m |= 1u << dbit(d);
+      }
+    }
+    dmask[off + v] = (uint8_t)m;
+  }
 }
 
 // Stream-power set-up, src/StreamPowerLaw.f90:236-250: g, kfint, ht (written by k_route_tile; h := ht).
@@ -509,7 +553,7 @@ void launch_receivers(DevState& S, hipStream_t st, int uplift) {
   if (S.ntiles) k_route_tile<<<S.ntiles, 256, 0, st>>>(S.P, S.tiles, S.h, S.b, S.u, S.ht, S.rec, S.length, uplift);
 }
 void launch_donors(DevState& S, hipStream_t st) {
-  k_donors<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.rec, S.dmask);
+  if (S.ntiles) k_donors_tile<<<S.ntiles, 256, 0, st>>>(S.P, S.tiles, S.rec, S.dmask);
 }
 void launch_spl_prep(DevState& S, hipStream_t st) {
   k_spl_prep<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.h, S.b, S.kf, S.kfint, S.g, S.ht);
