@@ -56,6 +56,7 @@ struct fsx_ctx {
   hipStream_t stream = nullptr;
   bool dirty = true;
   bool any_sfd = false, any_mfd = false;  // which flow-routing paths the active instances use
+  bool any_flow = false;                  // some active instance takes the dataflow stream-power sweep
   bool any_advect = false, g_adv = false;  // some active instance advects; as captured
   // one step captured as a HIP graph and replayed; rebuilt when the launch sequence changes
   bool use_graph = true, graph_valid = false, g_sfd = false, g_mfd = false;
@@ -183,7 +184,7 @@ static int check_inst(fsx_ctx* c, int inst) {
 
 static int upload_params(fsx_ctx* c) {
   if (!c->dirty) return 0;
-  c->any_sfd = c->any_mfd = false;
+  c->any_sfd = c->any_mfd = c->any_flow = false;
   c->any_advect = false;
   for (InstParams& P : c->hp) {
     // g = g1, or g2 where h - b > 1 when g2 > 0 (src/StreamPowerLaw.f90:236-239): zero everywhere exactly when
@@ -192,6 +193,7 @@ static int upload_params(fsx_ctx* c) {
     if (P.active && P.run_advect) c->any_advect = true;
     if (!P.active || !P.run_spl) continue;
     if (P.sfd) c->any_sfd = true; else c->any_mfd = true;
+    if (P.flow) c->any_flow = true;
   }
   HIPCHK(hipMemcpyAsync(c->S.P, c->hp.data(), sizeof(InstParams) * c->hp.size(), hipMemcpyHostToDevice, c->stream));
   c->dirty = false;
@@ -1137,6 +1139,7 @@ extern "C" int fsx_autotune(fsx_ctx* c, int reps, int* chosen) {
       {&DevState::nr_small_gx, {16, 32, 64, 128}, 4, PH_LM_SORT, -1},
       {&DevState::area_wg, {256, 512, 1024, 0}, 3, PH_AREA_SWEEP, -1},
       {&DevState::bor_gx, {2, 4, 8, 16}, 4, PH_LM_MST, -1},
+      {&DevState::flow_wg, {256, 512, 1024, 0}, 3, PH_SPL_SWEEP, -1},
   };
   // keep the caller's profiling state and accumulated timers
   if (int r = flush_timers_any(c, c->stream)) return r;
@@ -1147,17 +1150,29 @@ extern "C" int fsx_autotune(fsx_ctx* c, int reps, int* chosen) {
   int rc = fsx_set_profiling(c, 1);
   for (const TuneKnob& k : knobs) {
     if (rc) break;
+    // the stream-power sweep's shape is timed on the dataflow landscapes' solve of the routed state,
+    // without the step's per-landscape outputs (Newton counts) and without updating h
+    const bool spl = k.ph0 == PH_SPL_SWEEP;
+    if (spl && !c->any_flow) continue;
+    auto pass = [&]() {
+      enqueue_routing(c, c->stream, 0);
+      if (spl) {
+        hipStream_t st = c->stream;
+        launch_spl_fact(c->S, st);
+        PHASE(PH_SPL_SWEEP, launch_spl_flow(c->S, st, 2));
+      }
+    };
     int best = c->S.*k.field;
     double best_ms = 1e300;
     for (int i = 0; i < k.ncand && !rc; i++) {
       c->S.*k.field = k.cand[i];
-      enqueue_routing(c, c->stream, 0);  // untimed pass: caches and code warm for this shape
+      pass();  // untimed pass: caches and code warm for this shape
       ++c->ring_used;
       if ((rc = fsx_get_timers(c, nullptr, nullptr, 0)) < 0) break;
       rc = 0;
       fsx_reset_timers(c);
       for (int r = 0; r < reps && !rc; r++) {
-        enqueue_routing(c, c->stream, 0);
+        pass();
         if (++c->ring_used == kRing) rc = flush_timers_any(c, c->stream);  // one ring slot per pass
       }
       if ((rc = fsx_get_timers(c, nullptr, nullptr, 0)) < 0) break;
@@ -1185,6 +1200,7 @@ extern "C" int fsx_get_tuning(fsx_ctx* c, int* out) {
   out[3] = c->S.nr_small_gx;
   out[4] = c->S.area_wg;
   out[5] = c->S.bor_gx;
+  out[6] = c->S.flow_wg;
   return 0;
 }
 
@@ -1196,12 +1212,15 @@ extern "C" int fsx_set_tuning(fsx_ctx* c, const int* in) {
   if (in[4] != 256 && in[4] != 512 && in[4] != 1024)
     return fail(FSX_EINVAL, "fsx_set_tuning: area_wg must be 256, 512 or 1024");
   if (in[5] < 1 || in[5] > 1024) return fail(FSX_EINVAL, "fsx_set_tuning: bor_gx must be in [1, 1024]");
+  if (in[6] != 256 && in[6] != 512 && in[6] != 1024)
+    return fail(FSX_EINVAL, "fsx_set_tuning: flow_wg must be 256, 512 or 1024");
   c->S.walk_gx = in[0];
   c->S.jump_blocks = in[1];
   c->S.tour_gx = in[2];
   c->S.nr_small_gx = in[3];
   c->S.area_wg = in[4];
   c->S.bor_gx = in[5];
+  c->S.flow_wg = in[6];
   c->graph_valid = false;  // captured launches carry the old shapes
   return 0;
 }

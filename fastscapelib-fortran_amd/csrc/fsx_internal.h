@@ -236,6 +236,7 @@ void launch_area_prep(DevState& S, hipStream_t st);
 void launch_area_sweep(DevState& S, hipStream_t st);
 void launch_spl_fact(DevState& S, hipStream_t st);
 void launch_spl_sweep(DevState& S, hipStream_t st);
+void launch_spl_flow(DevState& S, hipStream_t st, int mode);  // mode: 0 step, 1 fill only, 2 timing
 void launch_spl(DevState& S, hipStream_t st);
 void launch_diffusion(DevState& S, hipStream_t st);
 void launch_mfd_route(DevState& S, hipStream_t st);

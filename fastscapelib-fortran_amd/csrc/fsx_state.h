@@ -19,6 +19,7 @@ struct DevState {
   int nr_small_gx = 64;    // edge sort: blocks per landscape for the segments in LDS
   int area_wg = 1024;      // drainage-area sweep: threads per landscape (256, 512 or 1024)
   int bor_gx = 4;          // spanning forest: workgroups per landscape for the first rounds' edge scans
+  int flow_wg = 1024;      // stream-power dataflow sweep: threads per landscape (256, 512 or 1024)
   int nr_gx = 8;           // edge sort: workgroups per landscape for the big ranges of a level
 
   // ---- per node (ntot) ----

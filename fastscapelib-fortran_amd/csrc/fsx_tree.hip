@@ -893,7 +893,6 @@ __global__ __launch_bounds__(FSX_WG) void k_spl_sweep(
 //   their global stores and loads coherent (workgroup-scope atomics).  A wave that sees no progress for
 //   FLOW_SPIN passes reports FSX_ERR_LEVELS and the workgroup stops; with a valid level order that cannot
 //   happen.
-#define FLOW_WG 1024
 #define FLOW_SPIN (1 << 26)
 #define FLOW_GLOBAL 32
 
@@ -1012,12 +1011,16 @@ __device__ __forceinline__ void flow_load(FlowRec& r, const SplRec32* srec, long
 
 static size_t spl_flow_smem() { return (size_t)SPL_R * (sizeof(int) + 3 * sizeof(double)); }
 
-__global__ __launch_bounds__(FLOW_WG) void k_spl_flow(const InstParams* __restrict__ PP, const int* __restrict__ nlev,
-                                                      const SplRec32* __restrict__ srec, double* __restrict__ Hl,
-                                                      double* __restrict__ Wl, double* __restrict__ HWl,
-                                                      double* __restrict__ scal, int* __restrict__ ngs_out,
-                                                      long long* __restrict__ newton_out, int* __restrict__ err,
-                                                      int fill_only) {
+// WG threads per landscape (the autotuner's flow_wg): fewer waves per SIMD shorten a pass of the wave
+// loop, and with it each hop of the critical receiver chain, more waves solve more nodes at once.
+// mode: 0 the step's solve, 1 lake fill only (no erosion), 2 the solve without the per-step outputs
+// (ngs, Newton count, scal; the autotuner's timing passes).
+template <int WG>
+__global__ __launch_bounds__(WG) void k_spl_flow(const InstParams* __restrict__ PP, const int* __restrict__ nlev,
+                                                 const SplRec32* __restrict__ srec, double* __restrict__ Hl,
+                                                 double* __restrict__ Wl, double* __restrict__ HWl,
+                                                 double* __restrict__ scal, int* __restrict__ ngs_out,
+                                                 long long* __restrict__ newton_out, int* __restrict__ err, int mode) {
   extern __shared__ double spl_ring[];  // h[R], water[R], lake surface[R], tag[R]
   __shared__ FlowWin W;
   __shared__ unsigned long long s_newton;
@@ -1035,7 +1038,8 @@ __global__ __launch_bounds__(FLOW_WG) void k_spl_flow(const InstParams* __restri
   const int nkind = npow_kind(n);
   const double omega = 0.875 / n, tolp = P.newton_tol;
   flow_win_init(W);
-  for (int i = tid; i < SPL_R; i += FLOW_WG) ring.tag[i] = -1;
+  const bool fill_only = mode == 1;
+  for (int i = tid; i < SPL_R; i += WG) ring.tag[i] = -1;
   if (tid == 0) s_newton = 0;
   __syncthreads();
   auto tsize = [&](int t) { return min(64, nn - 64 * t); };
@@ -1209,7 +1213,7 @@ __global__ __launch_bounds__(FLOW_WG) void k_spl_flow(const InstParams* __restri
 #undef SPL_PUBLISH
   if (my_newton) atomicAdd(&s_newton, my_newton);
   __syncthreads();
-  if (tid == 0 && !fill_only) {
+  if (tid == 0 && mode == 0) {
     ngs_out[inst] = 1;
     newton_out[inst] = (long long)s_newton;
     scal[inst] = 0.0;  // g == 0: no lake sediment to spread (k_spl_post)
@@ -1235,9 +1239,8 @@ int sweep_set_lds(int smem_max) {
     if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)area_smem(kAreaVcap)) != hipSuccess) return -1;
   if (hipFuncSetAttribute((const void*)k_spl_sweep, hipFuncAttributeMaxDynamicSharedMemorySize, (int)spl_smem(kSplVcap)) != hipSuccess) return -1;
   if ((int)(spl_flow_smem() + sizeof(FlowWin) + 64) > smem_max) return -1;
-  if (hipFuncSetAttribute((const void*)k_spl_flow, hipFuncAttributeMaxDynamicSharedMemorySize, (int)spl_flow_smem()) !=
-      hipSuccess)
-    return -1;
+  for (const void* f : {(const void*)k_spl_flow<256>, (const void*)k_spl_flow<512>, (const void*)k_spl_flow<1024>})
+    if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)spl_flow_smem()) != hipSuccess) return -1;
   if (hipFuncSetAttribute((const void*)k_level_sort, hipFuncAttributeMaxDynamicSharedMemorySize, LVL_HCAP * 4) != hipSuccess) return -1;
   return 0;
 }
@@ -1282,10 +1285,20 @@ void launch_spl_fact(DevState& S, hipStream_t st) {
                                                S.kfint, S.ht, S.srec, S.lsed, S.g, S.redk, S.flags, S.Hl, S.Wl, S.HWl);
 }
 
+void launch_spl_flow(DevState& S, hipStream_t st, int mode) {
+  const size_t sm = spl_flow_smem();
+  if (S.flow_wg == 256)
+    k_spl_flow<256><<<S.ninst, 256, sm, st>>>(S.P, S.nlev, S.srec, S.Hl, S.Wl, S.HWl, S.scal, S.ngs, S.newton, S.err, mode);
+  else if (S.flow_wg == 512)
+    k_spl_flow<512><<<S.ninst, 512, sm, st>>>(S.P, S.nlev, S.srec, S.Hl, S.Wl, S.HWl, S.scal, S.ngs, S.newton, S.err, mode);
+  else
+    k_spl_flow<1024><<<S.ninst, 1024, sm, st>>>(S.P, S.nlev, S.srec, S.Hl, S.Wl, S.HWl, S.scal, S.ngs, S.newton, S.err,
+                                                mode);
+}
+
 static void spl_sweep_launch(DevState& S, hipStream_t st, int fill_only) {
   // every landscape is taken by exactly one of the two kernels (InstParams::flow)
-  k_spl_flow<<<S.ninst, FLOW_WG, spl_flow_smem(), st>>>(S.P, S.nlev, S.srec, S.Hl, S.Wl, S.HWl, S.scal, S.ngs,
-                                                         S.newton, S.err, fill_only);
+  launch_spl_flow(S, st, fill_only);
   k_spl_sweep<<<S.ninst, FSX_WG, spl_smem(kSplVcap), st>>>(
       S.P, S.lstart, S.nlev, S.grec, S.ngroups, S.srec, S.arec, S.bc, S.g, S.Al, S.h, S.hwater,
       S.lake_depth, S.water, S.lsill, S.lwv, S.lsed, S.Hl, S.Wl, S.HWl, S.LSl, S.HPl, S.DHl, S.ELl, S.scal, S.ngs,

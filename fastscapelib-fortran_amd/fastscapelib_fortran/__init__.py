@@ -160,11 +160,12 @@ class Batch:
         k = -1 if name is None else _lib.PHASES.index(name)
         check(lib.fsx_time_phase(self._h, k), "time_phase")
 
-    TUNING = ("walk_gx", "jump_blocks", "tour_gx", "nr_small_gx", "area_wg", "bor_gx")
+    TUNING = ("walk_gx", "jump_blocks", "tour_gx", "nr_small_gx", "area_wg", "bor_gx", "flow_wg")
 
     def autotune(self, reps=2):
-        """Choose the launch shapes of the routing kernels by timing candidates on the current
-        landscapes (which it leaves unchanged); returns the chosen shapes."""
+        """Choose the launch shapes of the routing kernels and of the dataflow stream-power sweep by
+        timing candidates on the current landscapes (whose elevations it leaves unchanged); returns the
+        chosen shapes."""
         out = (C.c_int * len(self.TUNING))()
         check(lib.fsx_autotune(self._h, int(reps), out), "autotune")
         return dict(zip(self.TUNING, list(out)))

@@ -10,7 +10,8 @@ from conftest import apply, landscape
 pytestmark = pytest.mark.gpu
 
 CANDIDATES = {"walk_gx": {64, 128, 256, 512}, "jump_blocks": {1024, 2048, 4096, 8192}, "tour_gx": {8, 16, 32},
-              "nr_small_gx": {16, 32, 64, 128}, "area_wg": {256, 512, 1024}, "bor_gx": {2, 4, 8, 16}}
+              "nr_small_gx": {16, 32, 64, 128}, "area_wg": {256, 512, 1024}, "bor_gx": {2, 4, 8, 16},
+              "flow_wg": {256, 512, 1024}}
 
 
 def test_autotune_changes_shapes_not_results():
@@ -31,7 +32,7 @@ def test_autotune_changes_shapes_not_results():
             assert v in CANDIDATES[name], (name, v)
         assert b.tuning() == chosen
         for k in range(len(Ls)):
-            assert np.array_equal(b.h(k), h_before[k])  # tuning runs routing only
+            assert np.array_equal(b.h(k), h_before[k])  # tuning leaves the elevations alone
         a.step(2)
         b.step(2)
         for k in range(len(Ls)):
@@ -52,13 +53,15 @@ def test_set_tuning_roundtrip_and_bounds():
         a.step(1)
         b.step(1)  # b's default-shape graphs are captured here
         shapes = {"walk_gx": 64, "jump_blocks": 1024, "tour_gx": 32, "nr_small_gx": 16, "area_wg": 256,
-                  "bor_gx": 1}
+                  "bor_gx": 1, "flow_wg": 256}
         b.set_tuning(shapes)
         assert b.tuning() == shapes
         with pytest.raises(_lib.FastscapeError):
             b.set_tuning({"walk_gx": 0})
         with pytest.raises(_lib.FastscapeError):
             b.set_tuning({"area_wg": 384})
+        with pytest.raises(_lib.FastscapeError):
+            b.set_tuning({"flow_wg": 128})
         assert b.tuning() == shapes  # a rejected call changes nothing
         a.step(2)
         b.step(2)
