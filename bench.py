@@ -56,7 +56,7 @@ ALG_BYTES_PER_NODE = {
     "area_sweep": 48,    # R record  W Al
     "spl_prep": 48,      # R h, b, kf  W kfint, g, ht
     "spl_fact": 92,      # R J, rec, pos(x2), Al, length, kfint, ht(x2)  W area, lsed, record (node, rcv, f, ht)
-    "spl_sweep": 40,     # R record  W Hl, HWl (level order)
+    "spl_sweep": 44,     # dataflow sweep: R receiver position, factor, ht of the record  W Hl, Wl, HWl
     "spl_post": 100,     # R pos, Hl, HWl, ht, b, etot  W h, b, etot, erate, sedflux, hwater, lake_depth
 }
 ALG_BYTES_PER_EDGE = {}  # per basin-graph edge (the spanning-forest and tour phases are multi-kernel)
