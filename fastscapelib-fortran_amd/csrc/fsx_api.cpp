@@ -58,6 +58,7 @@ struct fsx_ctx {
   bool any_sfd = false, any_mfd = false;  // which flow-routing paths the active instances use
   bool any_flow = false;                  // some active instance takes the dataflow stream-power sweep
   bool any_advect = false, g_adv = false;  // some active instance advects; as captured
+  bool any_marine = false, g_mar = false;  // some active instance runs the marine transport; as captured
   // one step captured as a HIP graph and replayed; rebuilt when the launch sequence changes
   bool use_graph = true, graph_valid = false, g_sfd = false, g_mfd = false;
   hipGraphExec_t gexec = nullptr;
@@ -185,12 +186,13 @@ static int check_inst(fsx_ctx* c, int inst) {
 static int upload_params(fsx_ctx* c) {
   if (!c->dirty) return 0;
   c->any_sfd = c->any_mfd = c->any_flow = false;
-  c->any_advect = false;
+  c->any_advect = c->any_marine = false;
   for (InstParams& P : c->hp) {
     // g = g1, or g2 where h - b > 1 when g2 > 0 (src/StreamPowerLaw.f90:236-239): zero everywhere exactly when
     // g1 == 0 and g2 <= 0, and then the Gauss-Seidel loop runs once (:375-376)
     P.flow = (P.sfd && P.g1 == 0.0 && !(P.g2 > 0.0)) ? 1 : 0;
     if (P.active && P.run_advect) c->any_advect = true;
+    if (P.active && P.run_marine) c->any_marine = true;
     if (!P.active || !P.run_spl) continue;
     if (P.sfd) c->any_sfd = true; else c->any_mfd = true;
     if (P.flow) c->any_flow = true;
@@ -307,11 +309,12 @@ static int relayout(fsx_ctx* c) {
 // 0, precip = 1, kf = kd = 0; everything else is rewritten by each step.
 __global__ void k_slot_init(long long off, int nn, double* h, double* b, double* u, double* etot, double* erate,
                             double* area, double* lake_depth, double* sedflux, double* hwater, double* kf,
-                            double* kd, double* g, double* precip) {
+                            double* kd, double* g, double* precip, double* fmix) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += gridDim.x * blockDim.x) {
     const long long q = off + i;
     h[q] = 0.0; b[q] = 0.0; u[q] = 0.0; etot[q] = 0.0; erate[q] = 0.0; area[q] = 0.0; lake_depth[q] = 0.0;
     sedflux[q] = 0.0; hwater[q] = 0.0; kf[q] = 0.0; kd[q] = 0.0; g[q] = 0.0; precip[q] = 1.0;
+    fmix[q] = 0.5;  // Fmix = 0.5 (src/FastScape_ctx.f90:96)
   }
 }
 
@@ -344,7 +347,7 @@ static int ctx_alloc(long long capacity, int nslots, fsx_ctx** out) {
   A(h, N); A(b, N); A(u, N); A(kf, N); A(kd, N); A(precip, N); A(etot, N); A(erate, N); A(area, N);
   A(length, N); A(hwater, N); A(lake_depth, N); A(sedflux, N); A(ht, N); A(water, N); A(zint, N); A(zintp, N);
   A(gam, N); A(dh, N); A(hp, N); A(lwv, N); A(lsed, N); A(kfint, N); A(kdint, N); A(g, N); A(pexp, N); A(elev, N);
-  A(adx, N); A(ady, N); A(adg, N); A(adf, N);
+  A(adx, N); A(ady, N); A(adg, N); A(adf, N); A(fmix, N);
   A(rec, N); A(basin, N); A(rootrank, N); A(order, N); A(lstart, N + ninst); A(lsill, N);
   A(dmask, N); A(bc, N);
   A(outlet, N); A(bsize, N); A(soff, N); A(ecount, N); A(eoff, N); A(comp, N); A(bpar, N); A(bnew, N); A(best, N);
@@ -384,6 +387,7 @@ static int slot_add(fsx_ctx* c, int nx, int ny, int* inst, bool relay) {
   P.active = 1;
   P.newton_tol = 1.0e-3;
   P.p = 1.0;
+  P.p_raw = 1.0;  // p_mfd_exp = 1 (src/FastScape_ctx.f90:93)
   P.sfd = 1;
   decode_bc(P, 1111);
   geometry(P, (double)(P.nx - 1), (double)(P.ny - 1));
@@ -396,7 +400,7 @@ static int slot_add(fsx_ctx* c, int nx, int ny, int* inst, bool relay) {
   DevState& S = c->S;
   k_slot_init<<<(int)std::min<long long>(1024, (nn + 255) / 256), 256, 0, c->stream>>>(
       off, (int)nn, S.h, S.b, S.u, S.etot, S.erate, S.area, S.lake_depth, S.sedflux, S.hwater, S.kf, S.kd, S.g,
-      S.precip);
+      S.precip, S.fmix);
   HIPCHK(hipGetLastError());
   if (int r = upload_bc(c, k)) return r;
   if (relay)
@@ -535,6 +539,23 @@ static int advect_alloc(fsx_ctx* c) {
   return rc;
 }
 
+// Marine buffers, allocated the first time an instance sets marine parameters.
+static int marine_alloc(fsx_ctx* c) {
+  DevState& S = c->S;
+  if (S.mar_rec) return 0;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const long long N = S.ntot;
+  int rc = 0;
+  c->graph_valid = false;
+#define A(p, n) if (!rc) rc = dalloc(c, &S.p, (n))
+  A(mar_flux, N); A(mar_sd, N); A(mar_h0, N); A(mar_w, 8 * N); A(mar_rec, 8 * N); A(mar_dir, 8 * N);
+  A(mar_nrec, N); A(mar_indeg, N); A(mar_order, N); A(mar_ls, N + S.ninst); A(mar_nlev, S.ninst);
+  A(mar_niter, S.ninst); A(mar_q1, N); A(mar_qs, N); A(mar_ht, N); A(mar_ft, N); A(mar_hp, N); A(mar_fp, N);
+  A(mar_hh, N); A(mar_fh, N); A(mar_fhp, N); A(mar_tint, N); A(mar_gam, N);
+#undef A
+  return rc;
+}
+
 static int put(fsx_ctx* c, double* dst, const double* src, int n, int on_device) {
   HIPCHK(hipMemcpyAsync(dst, src, sizeof(double) * (size_t)n, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                         c->stream));
@@ -558,6 +579,7 @@ extern "C" int fsx_set_erosional_parameters(fsx_ctx* c, int inst, const double* 
   P.g1 = g1;
   P.g2 = g2;
   P.p = p;
+  P.p_raw = p;
   P.sfd = 0;
   if (p < -1.5) {
     P.sfd = 1;
@@ -577,6 +599,26 @@ extern "C" int fsx_set_erosional_parameters(fsx_ctx* c, int inst, const double* 
   c->dirty = true;
   if (!P.sfd)
     if (int r = mfd_alloc(c)) return r;
+  return 0;
+}
+
+extern "C" int fsx_set_marine_parameters(fsx_ctx* c, int inst, double sealevel, double poro1, double poro2,
+                                         double zporo1, double zporo2, double ratio, double layer, double kdsea1,
+                                         double kdsea2) {
+  if (int r = check_inst(c, inst)) return r;
+  if (int r = marine_alloc(c)) return r;
+  InstParams& P = c->hp[inst];
+  P.run_marine = 1;
+  P.sealevel = sealevel;
+  P.poro1 = poro1;
+  P.poro2 = poro2;
+  P.zporo1 = zporo1;
+  P.zporo2 = zporo2;
+  P.ratio = ratio;
+  P.layer = layer;
+  P.kdsea1 = kdsea1;
+  P.kdsea2 = kdsea2;
+  c->dirty = true;
   return 0;
 }
 
@@ -606,6 +648,7 @@ extern "C" int fsx_set_field(fsx_ctx* c, int inst, int field, const double* src,
     case FSX_B: dst = c->S.b; break;
     case FSX_U: dst = c->S.u; P.run_uplift = 1; c->dirty = true; break;
     case FSX_PRECIP: dst = c->S.precip; break;
+    case FSX_FMIX: dst = c->S.fmix; break;
     default: return fail(FSX_EINVAL, "field cannot be set");
   }
   return put(c, dst + P.off, src, P.nn, on_device);
@@ -672,6 +715,7 @@ static int check_errors(fsx_ctx* c) {
     if (e[k] & FSX_ERR_SORTSTACK) why += " edge-sort segment stack overflow;";
     if (e[k] & FSX_ERR_LEVELS) why += " receiver graph is not a forest;";
     if (e[k] & FSX_ERR_MFD) why += " multiple-flow order incomplete;";
+    if (e[k] & FSX_ERR_MARINE) why += " marine silt / sand diffusion did not converge in 1000 iterations;";
     return fail(FSX_ESTEP, "instance " + std::to_string(k) + ":" + why);
   }
   return 0;
@@ -720,6 +764,7 @@ static void enqueue_step(fsx_ctx* c, hipStream_t st) {
   if (c->any_mfd) PHASE(PH_MFD_SPL, launch_mfd_spl(S, st));
   PHASE(PH_SPL_POST, launch_spl_post(S, st));
   PHASE(PH_DIFFUSION, launch_diffusion(S, st));
+  if (c->any_marine) PHASE(PH_MARINE, launch_marine(S, st));  // Marine follows Diffusion (api.f90:245-250)
 }
 
 static int capture(fsx_ctx* c, hipStream_t st, int lo, int hi, hipGraphExec_t* out) {
@@ -775,6 +820,7 @@ static int build_graphs(fsx_ctx* c, hipStream_t st) {
   c->g_sfd = c->any_sfd;
   c->g_mfd = c->any_mfd;
   c->g_adv = c->any_advect;
+  c->g_mar = c->any_marine;
   c->g_timed = c->timed_phase;
   return 0;
 }
@@ -783,6 +829,10 @@ extern "C" int fsx_execute_steps(fsx_ctx* c, int nsteps, void* stream, int sync)
   if (!c) return fail(FSX_EINVAL, "null context");
   if (nsteps < 0) return fail(FSX_EINVAL, "nsteps < 0");
   hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+  // Marine reads the step's single-flow routing (rec, stack; src/Marine.f90:30-36,60-61)
+  for (int k = 0; k < c->S.ninst; k++)
+    if (c->used[k] && c->hp[k].active && c->hp[k].run_marine && !c->hp[k].run_spl)
+      return fail(FSX_EINVAL, "marine transport needs the flow routing: set the erosional parameters first");
   if (stream && c->dirty) HIPCHK(hipStreamSynchronize(c->stream));
   if (int r = upload_params(c)) return r;
   if (stream) HIPCHK(hipStreamSynchronize(c->stream));
@@ -790,7 +840,7 @@ extern "C" int fsx_execute_steps(fsx_ctx* c, int nsteps, void* stream, int sync)
     // the step's launch sequence depends on the layout and on which routing paths are in use; the
     // kernels read every other parameter from device memory, so one capture serves until those change
     if (!c->graph_valid || c->g_sfd != c->any_sfd || c->g_mfd != c->any_mfd || c->g_adv != c->any_advect ||
-        c->g_timed != c->timed_phase) {
+        c->g_mar != c->any_marine || c->g_timed != c->timed_phase) {
       if (int r = build_graphs(c, st)) return r;
     }
     if (c->gexec) {
@@ -896,6 +946,7 @@ extern "C" int fsx_copy_field(fsx_ctx* c, int inst, int field, double* dst, int 
     case FSX_HWATER: src = c->S.hwater; break;
     case FSX_KF: src = c->S.kf; break;
     case FSX_KD: src = c->S.kd; break;
+    case FSX_FMIX: src = c->S.fmix; break;
     default: return fail(FSX_EINVAL, "unknown field");
   }
   HIPCHK(hipMemcpyAsync(dst, src + P.off, sizeof(double) * P.nn, on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,

@@ -28,6 +28,7 @@ enum {
   FSX_ERR_SORTSTACK = 16,  // segment stack of the edge sort overflowed
   FSX_ERR_LEVELS = 32,     // receiver graph has a cycle (stack incomplete)
   FSX_ERR_MFD = 64,        // multiple-flow-direction order incomplete (reference STOPs)
+  FSX_ERR_MARINE = 128,    // silt / sand diffusion did not converge in 1000 iterations (reference STOPs)
 };
 
 struct InstParams {
@@ -40,12 +41,15 @@ struct InstParams {
   int active;
   int flow;                  // single flow without deposition (g == 0 everywhere): one stream-power pass,
                              // solved node by node as receivers complete (k_spl_flow)
+  int run_marine;            // marine transport after Diffusion (SetMarineParam)
   int aux;                   // offset of the nn+1-entry per-landscape arrays (level starts, groups)
   int c0, c1;                // chunk-table range of the landscape
   double dx, dy, dt;
   double len[9];             // neighbour distance by direction (ii+1)+3*(jj+1), host-computed
   double dxdy;               // dx*dy
   double kfsed, m, n, kdsed, g1, g2, p, newton_tol;
+  double p_raw;              // p as given to the erosional parameters (p_mfd_exp of single-flow landscapes)
+  double sealevel, poro1, poro2, zporo1, zporo2, ratio, layer, kdsea1, kdsea2;  // SetMarineParam
 };
 
 struct Chunk {
@@ -241,6 +245,7 @@ void launch_spl(DevState& S, hipStream_t st);
 void launch_diffusion(DevState& S, hipStream_t st);
 void launch_mfd_route(DevState& S, hipStream_t st);
 void launch_mfd_spl(DevState& S, hipStream_t st);
+void launch_marine(DevState& S, hipStream_t st);  // needs the marine buffers
 // read-outs (outside a step) of the landscapes listed in ids (device memory, n entries; outputs at each
 // landscape's node offset), or of the one landscape inst1 of n nodes when ids is null (outputs from 0)
 void launch_terrain(DevState& S, const int* ids, int n, int inst1, int which, double con, double* out, hipStream_t st);
@@ -276,5 +281,6 @@ enum Phase {
   PH_MFD_ROUTE,    //   multiple flow direction: fill, receivers, donors, Kahn layers, accumulation
   PH_MFD_SPL,      //   multiple flow direction: edge factors + Gauss-Seidel stream power
   PH_ADVECT,       // * k_advect (first in the step when a velocity field is set)
+  PH_MARINE,       //   marine transport: shelf flux, silt / sand diffusion (last in the step)
   PH_COUNT
 };

@@ -493,7 +493,9 @@ __global__ __launch_bounds__(FSX_WG) void k_mfd_spl(
         double hv = r.h;
         int sl;
         if (r.e > w_rcv) {
-          if (r.q[0] >= 0) {
+          // with Marine on, nodes below sea level keep their current h (src/StreamPowerLaw.f90:105,144:
+          // h for n == 1, ht otherwise)
+          if (r.q[0] >= 0 && !(P.run_marine && !((n_is_one ? r.h : r.t) >= P.sealevel))) {
             double hr[8];
 #pragma unroll
             for (int k = 0; k < 8; k++) hr[k] = r.q[k] >= 0 ? Hl[off + r.q[k]] : 0.0;

@@ -30,6 +30,7 @@ struct DevState {
   double *kfint = nullptr, *kdint = nullptr, *g = nullptr, *pexp = nullptr, *elev = nullptr;
   double *adx = nullptr, *ady = nullptr, *adg = nullptr, *adf = nullptr;  // ADI couplings, diagonal, rhs
   double *vx = nullptr, *vy = nullptr;  // velocity field of the advection (allocated when first set)
+  double* fmix = nullptr;               // silt fraction of the marine transport (Fmix, default 0.5)
   int *rec = nullptr, *basin = nullptr, *rootrank = nullptr, *order = nullptr;
   int *lstart = nullptr, *lsill = nullptr;
   uint8_t *dmask = nullptr, *bc = nullptr;
@@ -73,6 +74,18 @@ struct DevState {
   int *mnlev = nullptr;  // [ninst]
   int *mpos = nullptr, *mdp = nullptr;  // position of each node in the Kahn order; donor positions (8/node)
   double *mdw = nullptr;                // donor weights by position (8/node)
+
+  // ---- marine transport (allocated when an instance first sets marine parameters) ----
+  // single-flow flux to the sea, shelf depth, filled surface; multiple-flow receivers of the filled
+  // surface (8 slots per node: target, weight, direction), counts, in-degrees, Kahn order and layer
+  // starts; working arrays of the silt / sand diffusion
+  double *mar_flux = nullptr, *mar_sd = nullptr, *mar_h0 = nullptr, *mar_w = nullptr;
+  int *mar_rec = nullptr, *mar_nrec = nullptr, *mar_indeg = nullptr, *mar_order = nullptr, *mar_ls = nullptr;
+  uint8_t* mar_dir = nullptr;
+  int *mar_nlev = nullptr, *mar_niter = nullptr;  // [ninst]
+  double *mar_q1 = nullptr, *mar_qs = nullptr, *mar_ht = nullptr, *mar_ft = nullptr, *mar_hp = nullptr;
+  double *mar_fp = nullptr, *mar_hh = nullptr, *mar_fh = nullptr, *mar_fhp = nullptr, *mar_tint = nullptr;
+  double* mar_gam = nullptr;
 
   // ---- per instance ----
   int *nbasins = nullptr, *basin0 = nullptr, *nconn = nullptr, *nlev = nullptr, *err = nullptr;

@@ -725,7 +725,9 @@ __global__ __launch_bounds__(FSX_WG) void k_spl_sweep(
               else e = gzero ? t : ELl[off + b0 + i];
               if (e > w_rcv) {
                 const double fact = rr.f;
-                if (n_is_one) {
+                if (P.run_marine && !((n_is_one ? rr.h : t) >= P.sealevel)) {
+                  // below sea level with Marine on: h keeps its current value (:305,342)
+                } else if (n_is_one) {
                   double f = e, df = 1.0;
                   f = f + fact * hr;
                   df = df + fact;
@@ -1039,6 +1041,9 @@ __global__ __launch_bounds__(WG) void k_spl_flow(const InstParams* __restrict__ 
   const double omega = 0.875 / n, tolp = P.newton_tol;
   flow_win_init(W);
   const bool fill_only = mode == 1;
+  // with Marine on, the stream-power law leaves nodes below sea level alone (src/StreamPowerLaw.f90:305,342:
+  // h for n == 1, ht otherwise; the same here, where the single pass starts from h = ht)
+  const bool sea_skip = P.run_marine != 0;
   for (int i = tid; i < SPL_R; i += WG) ring.tag[i] = -1;
   if (tid == 0) s_newton = 0;
   __syncthreads();
@@ -1143,6 +1148,9 @@ __global__ __launch_bounds__(WG) void k_spl_flow(const InstParams* __restrict__ 
             SPL_PUBLISH(tt, wr, hwv);
           } else if (!(e > wr)) {
             SPL_PUBLISH(e, wr > e ? wr : e, hwv);
+          } else if (sea_skip && !(tt >= P.sealevel)) {  // below sea level with Marine on: h stays ht
+            const double h1 = tt < wr ? wr : tt;
+            SPL_PUBLISH(h1, wr > h1 ? wr : h1, hwv);
           } else if (n_is_one) {
             double ff = e, df = 1.0;
             ff = ff + f * hr;

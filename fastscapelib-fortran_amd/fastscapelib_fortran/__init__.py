@@ -12,7 +12,8 @@ The free functions keep the reference's call surface and f2py argument conventio
 Documented divergences: call-order and size errors raise ``RuntimeError``/``ValueError`` where the
 reference STOPs the process; ``fastscape_setup`` starts from h = 0 rather than Fortran random
 numbers (every run sets h with ``fastscape_init_h``); routines outside the per-step update
-(marine, stratigraphy, flexure) raise ``NotImplementedError``; ``fastscape_vtk`` writes the
+(stratigraphy, flexure) raise ``NotImplementedError``; ``fastscape_set_marine_parameters`` turns on the
+marine transport (Marine) after Diffusion in every step, on the GPU; ``fastscape_vtk`` writes the
 reference's legacy-VTK files on the host from the fields copied off the GPU; ``fastscape_set_v`` turns on the
 advection of h, b and etot (Advect) at the start of every step, on the GPU.  The slope,
 curvature, chi and catchment read-outs and the three fluxes are computed on the GPU from the resident
@@ -35,7 +36,8 @@ __all__ = [
     "fastscape_execute_step", "fastscape_copy_h", "fastscape_copy_basement", "fastscape_copy_total_erosion",
     "fastscape_copy_erosion_rate", "fastscape_copy_drainage_area", "fastscape_copy_lake_depth",
     "fastscape_copy_slope", "fastscape_copy_curvature", "fastscape_copy_chi", "fastscape_copy_catchment",
-    "fastscape_get_fluxes", "fastscape_set_v", "fastscape_vtk",
+    "fastscape_get_fluxes", "fastscape_set_v", "fastscape_vtk", "fastscape_set_marine_parameters",
+    "fastscape_init_f", "fastscape_copy_f",
     "fastscape_get_step", "fastscape_get_sizes", "fastscape_destroy", "fastscape_set_h", "fastscape_set_basement",
     "fastscape_set_precip", "fastscape_set_all_layers", "fastscape_reset_cumulative_erosion", "fastscape_view",
     "fastscape_debug", "Batch", "Lanes", "Pool", "FastscapeError", "indexx",
@@ -107,6 +109,12 @@ class Batch:
     def set_dt(self, k, dt): check(lib.fsx_set_dt(self._h, k, float(dt)), "set_dt")
     def set_bc(self, k, ibc): check(lib.fsx_set_bc(self._h, k, int(ibc)), "set_bc")
     def set_newton_tol(self, k, tol): check(lib.fsx_set_newton_tol(self._h, k, float(tol)), "set_newton_tol")
+
+    def set_marine_parameters(self, k, sealevel, poro1, poro2, zporo1, zporo2, ratio, layer, kdsea1, kdsea2):
+        """Marine transport of landscape k after Diffusion in every step (FastScape_Set_Marine_Parameters)."""
+        check(lib.fsx_set_marine_parameters(self._h, k, *[float(x) for x in (sealevel, poro1, poro2, zporo1, zporo2,
+                                                                          ratio, layer, kdsea1, kdsea2)]),
+              "set_marine_parameters")
     def set_active(self, k, on): check(lib.fsx_set_active(self._h, k, 1 if on else 0), "set_active")
 
     def set_erosional_parameters(self, k, kf, kfsed, m, n, kd, kdsed, g1, g2, p):
@@ -141,6 +149,7 @@ class Batch:
     def set_h(self, k, h): self.set_field(k, _lib.FSX_H, h, "h")
     def set_basement(self, k, b): self.set_field(k, _lib.FSX_B, b, "b")
     def set_precip(self, k, p): self.set_field(k, _lib.FSX_PRECIP, p, "precip")
+    def init_f(self, k, f): self.set_field(k, _lib.FSX_FMIX, f, "f")
 
     def set_all_layers(self, k, dh):
         p, d, _a = self._src(k, dh, "dh")
@@ -325,7 +334,8 @@ class Lanes:
     _PER = {"set_xl_yl", "set_dt", "set_bc", "set_newton_tol", "set_active", "set_erosional_parameters",
             "init_h", "set_field", "set_u", "set_v", "set_h", "set_basement", "set_precip", "set_all_layers",
             "reset_cumulative_erosion", "field", "ifield", "h", "rec", "donors", "order", "mrec", "morder",
-            "get_step", "stats", "vtk", "derived", "slope", "curvature", "chi", "catchment", "fluxes"}
+            "get_step", "stats", "vtk", "derived", "slope", "curvature", "chi", "catchment", "fluxes",
+            "set_marine_parameters", "init_f"}
     TUNING = Batch.TUNING
 
     @staticmethod
@@ -541,12 +551,19 @@ def fastscape_set_h(h): _need().set_h(0, h)
 def fastscape_set_basement(b): _need().set_basement(0, b)
 def fastscape_set_precip(p): _need().set_precip(0, p)
 def fastscape_set_all_layers(dh): _need().set_all_layers(0, dh)
+def fastscape_init_f(f): _need().init_f(0, f)
 def fastscape_reset_cumulative_erosion(): _need().reset_cumulative_erosion(0)
 
 
 def fastscape_set_erosional_parameters(kf, kfsed, m, n, kd, kdsed, g1, g2, p):
     """FastScape_Set_Erosional_Parameters (src/FastScape_api.f90:523-536)."""
     _need().set_erosional_parameters(0, kf, kfsed, m, n, kd, kdsed, g1, g2, p)
+
+
+def fastscape_set_marine_parameters(sl, p1, p2, z1, z2, r, l, kds1, kds2):
+    """FastScape_Set_Marine_Parameters (src/FastScape_api.f90:540-552): sea level, surface porosities and
+    e-folding depths of silt and sand, silt ratio, mixed-layer thickness, marine diffusivities."""
+    _need().set_marine_parameters(0, sl, p1, p2, z1, z2, r, l, kds1, kds2)
 
 
 def fastscape_execute_step():
@@ -569,6 +586,7 @@ def fastscape_copy_total_erosion(e): _copy_into(_lib.FSX_ETOT, e)
 def fastscape_copy_erosion_rate(e): _copy_into(_lib.FSX_ERATE, e)
 def fastscape_copy_drainage_area(a): _copy_into(_lib.FSX_AREA, a)
 def fastscape_copy_lake_depth(d): _copy_into(_lib.FSX_LAKE_DEPTH, d)
+def fastscape_copy_f(f): _copy_into(_lib.FSX_FMIX, f)
 
 
 def _derived_into(which, out):
@@ -654,6 +672,5 @@ def _outside(name):
     return f
 
 
-for _n in ["fastscape_set_marine_parameters", "fastscape_strati",
-           "fastscape_init_f", "fastscape_copy_f"]:
+for _n in ["fastscape_strati"]:
     globals()[_n] = _outside(_n)

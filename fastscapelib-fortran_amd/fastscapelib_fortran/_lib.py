@@ -14,14 +14,14 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libfastscap
 
 # field ids (include/fastscape_hip.h)
 FSX_H, FSX_B, FSX_U, FSX_PRECIP, FSX_ETOT, FSX_ERATE, FSX_AREA, FSX_LAKE_DEPTH, FSX_SEDFLUX, FSX_LENGTH, \
-    FSX_HWATER, FSX_KF, FSX_KD = range(13)
+    FSX_HWATER, FSX_KF, FSX_KD, FSX_FMIX = range(14)
 FSX_REC, FSX_NDON, FSX_DON, FSX_ORDER, FSX_MREC, FSX_MNREC, FSX_MSTACK = range(7)
 FSX_SLOPE, FSX_CURVATURE, FSX_CHI, FSX_CATCHMENT = range(4)  # fsx_copy_derived
 
 # phase timers, in the library's order (include/fastscape_hip.h, fsx_get_timers)
 PHASES = ["uplift", "receivers", "donors", "lm_labels", "lm_basins", "lm_walk", "lm_edges", "lm_sort", "lm_mst",
           "lm_orient", "lm_correct", "levels", "area_prep", "area_sweep", "spl_prep", "spl_fact", "spl_sweep",
-          "spl_post", "diffusion", "mfd_route", "mfd_spl", "advect"]
+          "spl_post", "diffusion", "mfd_route", "mfd_spl", "advect", "marine"]
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -50,6 +50,7 @@ def _load():
         "fsx_set_erosional_parameters": [_vp, C.c_int, _vp, C.c_double, C.c_double, C.c_double, _vp, C.c_double,
                                          C.c_double, C.c_double, C.c_double, C.c_int],
         "fsx_set_newton_tol": [_vp, C.c_int, C.c_double],
+        "fsx_set_marine_parameters": [_vp, C.c_int] + [C.c_double] * 9,
         "fsx_init_h": [_vp, C.c_int, _vp, C.c_int],
         "fsx_set_field": [_vp, C.c_int, C.c_int, _vp, C.c_int],
         "fsx_set_all_layers": [_vp, C.c_int, _vp, C.c_int],

@@ -47,6 +47,7 @@ enum {
   FSX_HWATER = 10,    /* lake surface (hwater, internal)                              */
   FSX_KF = 11,        /* kf (write through fsx_set_erosional_parameters)             */
   FSX_KD = 12,        /* kd (idem)                                                    */
+  FSX_FMIX = 13,      /* silt fraction of the marine transport (Fmix)   Init_F / Copy_F */
 };
 /* derived field ids of fsx_copy_derived */
 enum {
@@ -106,13 +107,23 @@ int fsx_set_bc(fsx_ctx *ctx, int inst, int ibc);
  * direction with weights slope**p (p < 0: exponent from the mean slope, src/FlowRouting.f90:232-243). */
 int fsx_set_erosional_parameters(fsx_ctx *ctx, int inst, const double *kf, double kfsed, double m, double n,
                                  const double *kd, double kdsed, double g1, double g2, double p, int on_device);
+/* FastScape_Set_Marine_Parameters (src/FastScape_api.f90:540-552; SetMarineParam
+ * src/FastScape_ctx.f90:454-472): sea level, surface porosities and their e-folding depths of silt
+ * and sand, silt ratio, thickness of the mixed layer, marine diffusivities of silt and sand.  Switches
+ * on the marine transport after Diffusion in every step (Marine, src/Marine.f90:1-167), which needs
+ * the erosional parameters (the step's flow routing); with it on, the stream-power law leaves nodes
+ * below sea level alone (src/StreamPowerLaw.f90:105,144,305,342).  The silt fraction is field
+ * FSX_FMIX (default 0.5). */
+int fsx_set_marine_parameters(fsx_ctx *ctx, int inst, double sealevel, double poro1, double poro2, double zporo1,
+                              double zporo2, double ratio, double layer, double kdsea1, double kdsea2);
 /* Newton tolerance of the n != 1 stream-power solve.  No reference setter: the reference
  * hard-codes 1e-3 (src/StreamPowerLaw.f90:344), which is the default here. */
 int fsx_set_newton_tol(fsx_ctx *ctx, int inst, double tol);
 /* FastScape_Init_H (src/FastScape_api.f90:267-279; InitH src/FastScape_ctx.f90:339-350): h, and b = h */
 int fsx_init_h(fsx_ctx *ctx, int inst, const double *h, int on_device);
-/* FastScape_Set_U / Set_H / Set_Basement / Set_Precip (src/FastScape_api.f90:618-630,664-676,
- * 696-724): field one of FSX_U, FSX_H, FSX_B, FSX_PRECIP.  Setting FSX_U switches uplift on. */
+/* FastScape_Set_U / Set_H / Set_Basement / Set_Precip / Init_F (src/FastScape_api.f90:618-630,664-676,
+ * 696-724,283-295): field one of FSX_U, FSX_H, FSX_B, FSX_PRECIP, FSX_FMIX.  Setting FSX_U switches
+ * uplift on. */
 int fsx_set_field(fsx_ctx *ctx, int inst, int field, const double *src, int on_device);
 /* FastScape_Set_V (src/FastScape_api.f90:634-646; SetV src/FastScape_ctx.f90:616-632): the velocity
  * field (vx, vy: nn values each); switches on the advection of h, b and etot at the start of every
@@ -126,7 +137,8 @@ int fsx_reset_cumulative_erosion(fsx_ctx *ctx, int inst);
 int fsx_set_active(fsx_ctx *ctx, int inst, int active);
 
 /* FastScape_Execute_Step (src/FastScape_api.f90:201-263), nsteps times, on every active
- * instance: Uplift, then routing + LocalMinima + accumulation + stream power, then Diffusion.
+ * instance: Advect, Uplift, then routing + LocalMinima + accumulation + stream power, then Diffusion,
+ * then Marine, each where switched on.
  * Asynchronous on `stream` unless `sync` != 0; device-side failures surface as FSX_ESTEP at the
  * next synchronous call. */
 int fsx_execute_steps(fsx_ctx *ctx, int nsteps, void *stream, int sync);
@@ -194,7 +206,7 @@ int fsx_get_stats(fsx_ctx *ctx, int inst, long long *out6);
  * on the stream it runs on.  fsx_get_timers fills up to n accumulated milliseconds and counts
  * (phase order: uplift, receivers,
  * donors, lm_labels, lm_basins, lm_walk, lm_edges, lm_sort, lm_mst, lm_orient, lm_correct, levels, area_prep, area_sweep, spl_prep, spl_fact, spl_sweep, spl_post,
- * diffusion, mfd_route, mfd_spl, advect) and returns the number of phases. */
+ * diffusion, mfd_route, mfd_spl, advect, marine) and returns the number of phases. */
 int fsx_set_profiling(fsx_ctx *ctx, int on);
 /* Time one phase (0..fsx_get_timers' count-1; -1: none) live while steps replay HIP graphs: the
  * phases before it and after it run as two graphs and the phase itself is launched directly between

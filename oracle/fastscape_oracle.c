@@ -29,12 +29,13 @@ struct fso_state {
   int ibc, i1, i2, j1, j2, xcyc, ycyc;
   unsigned char *bc;
   double *h, *u, *b, *etot, *erate, *a, *length, *precip, *kf, *kd, *sedflux, *lake_depth, *hwater,
-      *g, *p_mfd_exp, *vx, *vy;
+      *g, *p_mfd_exp, *vx, *vy, *fmix;
   int *rec, *ndon, *don, *stack;
   int *mrec, *mnrec, *mstack;
   double *mwrec, *mlrec;
   double kfsed, m, n, kdsed, g1, g2, p, sealevel, newton_tol;
-  int sfd, run_spl, run_uplift, run_diffusion, run_advect;
+  double poro1, poro2, zporo1, zporo2, ratio, layer, kdsea1, kdsea2; /* SetMarineParam */
+  int sfd, run_spl, run_uplift, run_diffusion, run_advect, run_marine;
   int step;
   fso_stats st;
 };
@@ -503,19 +504,20 @@ static void flow_acc_sfd(fso_state *s) {
   }
 }
 
-/* find_mult_rec, src/FlowRouting.f90:147-298 */
-static int find_mult_rec(fso_state *s) {
+/* find_mult_rec, src/FlowRouting.f90:147-298, on the state's h, rec and stack with exponent argument p,
+ * into the given arrays (h0 = the filled surface, "water"). */
+static int mult_rec(fso_state *s, double p, double *h0, int *mrec, int *mnrec, double *mwrec, double *mlrec,
+                    int *mstack) {
   double dx, dy;
   dxdy(s, &dx, &dy);
   int nx = s->nx, ny = s->ny, nn = s->nn;
-  double *h0 = s->hwater;
   for (int i = 0; i < nn; i++) h0[i] = s->h[i];
   for (int i = 0; i < nn; i++) { /* :187-196 */
     int ijk = s->stack[i], ijr = s->rec[ijk];
     if (h0[ijr] > h0[ijk]) h0[ijk] = h0[ijr] + 1.0e-8;
   }
-  for (int i = 0; i < nn; i++) s->mnrec[i] = 0;
-  for (int i = 0; i < NDMAX * nn; i++) s->mwrec[i] = 0.0;
+  for (int i = 0; i < nn; i++) mnrec[i] = 0;
+  for (int i = 0; i < NDMAX * nn; i++) mwrec[i] = 0.0;
   for (int j = s->j1; j <= s->j2; j++) { /* :204-230 */
     for (int i = s->i1; i <= s->i2; i++) {
       int ij = (j - 1) * nx + (i - 1);
@@ -533,22 +535,22 @@ static int find_mult_rec(fso_state *s) {
           if (iii > nx) iii = nx;
           int ijk = (jjj - 1) * nx + (iii - 1);
           if (h0[ij] > h0[ijk]) {
-            int k = s->mnrec[ij]++;
+            int k = mnrec[ij]++;
             if (k >= NDMAX) return -4;
-            s->mrec[NDMAX * ij + k] = ijk;
+            mrec[NDMAX * ij + k] = ijk;
             double ddx = (double)ii * dx, ddy = (double)jj * dy;
             double l = sqrt(ddx * ddx + ddy * ddy);
-            s->mlrec[NDMAX * ij + k] = l;
-            s->mwrec[NDMAX * ij + k] = (h0[ij] - h0[ijk]) / l;
+            mlrec[NDMAX * ij + k] = l;
+            mwrec[NDMAX * ij + k] = (h0[ij] - h0[ijk]) / l;
           }
         }
       }
     }
   }
   for (int ij = 0; ij < nn; ij++) { /* :232-243 */
-    int nr = s->mnrec[ij];
-    double *w = s->mwrec + NDMAX * ij;
-    if (s->p < 0.0) {
+    int nr = mnrec[ij];
+    double *w = mwrec + NDMAX * ij;
+    if (p < 0.0) {
       /* slope is double precision but rounded through real(); 0.5 and 0.6 are default-real
          literals promoted to double in the mixed-mode expression */
       double slope = 0.0;
@@ -567,17 +569,17 @@ static int find_mult_rec(fso_state *s) {
   /* donors (:245-255) and LIFO Kahn order (:257-292) */
   int *ndon = (int *)calloc((size_t)nn, sizeof(int));
   for (int ij = 0; ij < nn; ij++)
-    for (int k = 0; k < s->mnrec[ij]; k++) ndon[s->mrec[NDMAX * ij + k]]++;
+    for (int k = 0; k < mnrec[ij]; k++) ndon[mrec[NDMAX * ij + k]]++;
   int *vis = (int *)calloc((size_t)nn, sizeof(int)), *parse = (int *)malloc(sizeof(int) * (size_t)nn);
   int nparse = 0, nstack = 0;
-  for (int ij = 0; ij < nn; ij++) s->mstack[ij] = -1;
+  for (int ij = 0; ij < nn; ij++) mstack[ij] = -1;
   for (int ij = 0; ij < nn; ij++) {
     if (ndon[ij] == 0) parse[nparse++] = ij;
     while (nparse > 0) {
       int ijn = parse[--nparse];
-      s->mstack[nstack++] = ijn;
-      for (int k = 0; k < s->mnrec[ijn]; k++) {
-        int ijr = s->mrec[NDMAX * ijn + k];
+      mstack[nstack++] = ijn;
+      for (int k = 0; k < mnrec[ijn]; k++) {
+        int ijr = mrec[NDMAX * ijn + k];
         vis[ijr]++;
         if (vis[ijr] == ndon[ijr]) parse[nparse++] = ijr;
       }
@@ -585,6 +587,10 @@ static int find_mult_rec(fso_state *s) {
   }
   free(ndon); free(vis); free(parse);
   return nstack == nn ? 0 : -5; /* 'error in stack', :292 */
+}
+
+static int find_mult_rec(fso_state *s) {
+  return mult_rec(s, s->p, s->hwater, s->mrec, s->mnrec, s->mwrec, s->mlrec, s->mstack);
 }
 
 /* FlowRouting (multiple flow direction), src/FlowRouting.f90:3-39 */
@@ -638,8 +644,13 @@ static void spl_node(spl_ctx *x, int ijk) {
     return;
   }
   double w_rcv = water[ijr];
+  /* with Marine on, nodes below sea level keep their h (:105,144,305,342): n == 1 tests the current
+   * h, otherwise ht */
+  const int n1 = fabs(n - 1.0) < DBL_MIN;
+  const int sea = s->run_marine && !((n1 ? h[ijk] : ht[ijk]) >= s->sealevel);
   if (elev[ijk] > w_rcv) {
-    if (fabs(n - 1.0) < DBL_MIN) {
+    if (sea) {
+    } else if (n1) {
       if (!mfd) { /* :306-314 */
         double f = elev[ijk], df = 1.0;
         double fact = kfint[ijk] * dt * pow(a[ijk], m) / s->length[ijk];
@@ -992,6 +1003,249 @@ static int diffusion(fso_state *s) {
 }
 
 /* ------------------------------------------------------------------------------------------ */
+/* SiltSandCouplingDiffusion, src/Marine.f90:171-499: coupled silt / sand diffusion in the ocean,
+ * alternating-direction passes (h along rows, silt fraction f along rows, h along columns, f along
+ * columns) repeated until max|h - hp| <= 1 m.  Rows j = 2..ny-1 and columns i = 2..nx-1 (1-based) are
+ * solved; the others keep their values (hhalf's first and last rows stay at their initial h, columns
+ * 1 and nx of h take hhalf's), then the four corners copy their row neighbour.  Boundaries: a fixed
+ * side ('1' digit) holds h; otherwise no flux, never cyclic.  q1, q2: silt and sand supply.  Returns
+ * the iteration count, or -8 when it exceeds 1000 (the reference stops).                        */
+int fso_silt_sand(double *h, double *f, const double *q1, const double *q2, int nx, int ny, double dx, double dy,
+                  double dt, double sl, double L, double K1, double K2, int ibc) {
+  const int nn = nx * ny;
+  char cbc[5];
+  bc_chars(ibc, cbc);
+  const size_t sz = sizeof(double) * (size_t)nn;
+  double *hp = malloc(sz), *fp = malloc(sz), *ht = malloc(sz), *ft = malloc(sz), *hhalf = malloc(sz);
+  double *fhalf = malloc(sz), *fhalfp = malloc(sz), *tint = malloc(sz);
+  int nmax = nx > ny ? nx : ny;
+  double *diag = calloc((size_t)nmax, sizeof(double)), *sup = calloc((size_t)nmax, sizeof(double));
+  double *inf = calloc((size_t)nmax, sizeof(double)), *rhs = calloc((size_t)nmax, sizeof(double));
+  double *res = calloc((size_t)nmax, sizeof(double)), *gam = calloc((size_t)nmax, sizeof(double));
+  memcpy(ht, h, sz); memcpy(ft, f, sz); memcpy(hhalf, h, sz); memcpy(fhalf, f, sz);
+  const double tol = 1.0;
+  double err1 = 2 * tol;
+  int niter = 0, rc = 0;
+#define I(i, j) (((j)-1) * nx + (i)-1)
+  while (err1 > tol && !rc) {
+    memcpy(hp, h, sz); memcpy(fp, f, sz); memcpy(fhalfp, fhalf, sz);
+    niter++;
+    /* h along x, :215-277 */
+    for (int j = 2; j <= ny - 1 && !rc; j++) {
+      for (int i = 1; i <= nx; i++) {
+        int ij = I(i, j), ipj = I(i + 1, j), imj = I(i - 1, j), ijp = I(i, j + 1), ijm = I(i, j - 1);
+        if (ht[ij] <= sl) { /* flag is 0 everywhere (its setting is commented out, :39) */
+          double Ap, Bp, Cp;
+          if (i == 1) {
+            if (cbc[3] == '1') { diag[i - 1] = 1.0; sup[i - 1] = 0.0; rhs[i - 1] = ht[ij]; continue; }
+            Ap = dt / 2.0 * (K2 + (K1 - K2) * (fhalfp[ipj] + fhalfp[ij]) / 2.0) / (dx * dx);
+            diag[i - 1] = 1.0 + Ap;
+            sup[i - 1] = -Ap;
+          } else if (i == nx) {
+            if (cbc[1] == '1') { diag[i - 1] = 1.0; inf[i - 1] = 0.0; rhs[i - 1] = ht[ij]; continue; }
+            Bp = -(dt / 2.0 * (K2 + (K1 - K2) * (fhalfp[ij] + fhalfp[imj]) / 2.0) / (dx * dx));
+            diag[i - 1] = 1.0 - Bp;
+            inf[i - 1] = Bp;
+          } else {
+            Ap = dt / 2.0 * (K2 + (K1 - K2) * (fhalfp[ipj] + fhalfp[ij]) / 2.0) / (dx * dx);
+            Bp = -(dt / 2.0 * (K2 + (K1 - K2) * (fhalfp[ij] + fhalfp[imj]) / 2.0) / (dx * dx));
+            diag[i - 1] = 1.0 + Ap - Bp;
+            sup[i - 1] = -Ap;
+            inf[i - 1] = Bp;
+          }
+          Cp = dt / 2.0 * (K2 + (K1 - K2) * (ft[ijp] + ft[ij]) / 2.0) * (ht[ijp] - ht[ij]) / (dy * dy) -
+               dt / 2.0 * (K2 + (K1 - K2) * (ft[ij] + ft[ijm]) / 2.0) * (ht[ij] - ht[ijm]) / (dy * dy) +
+               (q1[ij] + q2[ij]) * dt / 2.0;
+          rhs[i - 1] = Cp + ht[ij];
+        } else {
+          diag[i - 1] = 1.0; sup[i - 1] = 0.0; inf[i - 1] = 0.0; rhs[i - 1] = ht[ij];
+        }
+      }
+      rc = tridag(inf, diag, sup, rhs, res, gam, nx);
+      for (int i = 1; i <= nx; i++) hhalf[I(i, j)] = res[i - 1];
+    }
+    memcpy(tint, hhalf, sz);
+    hhalf[I(1, 1)] = hhalf[I(2, 1)];
+    hhalf[I(1, ny)] = hhalf[I(2, ny)];
+    hhalf[I(nx, 1)] = hhalf[I(nx - 1, 1)];
+    hhalf[I(nx, ny)] = hhalf[I(nx - 1, ny)];
+    /* f along x, :287-340 */
+    for (int j = 2; j <= ny - 1 && !rc; j++) {
+      for (int i = 2; i <= nx - 1; i++) {
+        int ij = I(i, j), ipj = I(i + 1, j), imj = I(i - 1, j), ijp = I(i, j + 1), ijm = I(i, j - 1);
+        if (ht[ij] <= sl && hhalf[ij] >= (1.0 + 1.0e-6) * ht[ij]) {
+          double Dp = (hhalf[ij] - ht[ij]) / dt;
+          double Ep = K1 / 2.0 * (hhalf[ipj] - hhalf[ij]) / (dx * dx);
+          double Mp = -(K1 / 2.0 * (hhalf[ij] - hhalf[imj]) / (dx * dx));
+          double Np = K1 / 2.0 * (ft[ijp] + ft[ij]) * (ht[ijp] - ht[ij]) / (dy * dy) -
+                      K1 / 2.0 * (ft[ij] + ft[ijm]) * (ht[ij] - ht[ijm]) / (dy * dy) + q1[ij];
+          diag[i - 1] = 2.0 * L / dt + Dp - Mp - Ep;
+          sup[i - 1] = -Ep;
+          inf[i - 1] = -Mp;
+          rhs[i - 1] = Np - Dp * ft[ij] + 2.0 * L * ft[ij] / dt;
+        } else {
+          diag[i - 1] = 1.0; sup[i - 1] = 0.0; inf[i - 1] = 0.0; rhs[i - 1] = ft[ij];
+        }
+      }
+      diag[0] = 1.0; sup[0] = -1.0; rhs[0] = 0.0;
+      diag[nx - 1] = 1.0; inf[nx - 1] = -1.0; rhs[nx - 1] = 0.0;
+      rc = tridag(inf, diag, sup, rhs, res, gam, nx);
+      for (int i = 1; i <= nx; i++) fhalf[I(i, j)] = res[i - 1];
+    }
+    for (int q = 0; q < nn; q++) fhalf[q] = dmin(dmax(0.0, fhalf[q]), 1.0);
+    /* h along y, :345-405 */
+    for (int i = 2; i <= nx - 1 && !rc; i++) {
+      for (int j = 1; j <= ny; j++) {
+        int ij = I(i, j), ipj = I(i + 1, j), imj = I(i - 1, j), ijp = I(i, j + 1), ijm = I(i, j - 1);
+        if (ht[ij] <= sl) {
+          double Ap, Bp, Cp;
+          if (j == 1) {
+            if (cbc[0] == '1') { diag[j - 1] = 1.0; sup[j - 1] = 0.0; rhs[j - 1] = hhalf[ij]; continue; }
+            Ap = dt / 2.0 * (K2 + (K1 - K2) * (fp[ijp] + fp[ij]) / 2.0) / (dy * dy);
+            diag[j - 1] = 1.0 + Ap;
+            sup[j - 1] = -Ap;
+          } else if (j == ny) {
+            if (cbc[2] == '1') { diag[j - 1] = 1.0; inf[j - 1] = 0.0; rhs[j - 1] = hhalf[ij]; continue; }
+            Bp = -(dt / 2.0 * (K2 + (K1 - K2) * (fp[ij] + fp[ijm]) / 2.0) / (dy * dy));
+            diag[j - 1] = 1.0 - Bp;
+            inf[j - 1] = Bp;
+          } else {
+            Ap = dt / 2.0 * (K2 + (K1 - K2) * (fp[ijp] + fp[ij]) / 2.0) / (dy * dy);
+            Bp = -(dt / 2.0 * (K2 + (K1 - K2) * (fp[ij] + fp[ijm]) / 2.0) / (dy * dy));
+            diag[j - 1] = 1.0 + Ap - Bp;
+            sup[j - 1] = -Ap;
+            inf[j - 1] = Bp;
+          }
+          Cp = dt / 2.0 * (K2 + (K1 - K2) * (fhalf[ipj] + fhalf[ij]) / 2.0) * (hhalf[ipj] - hhalf[ij]) / (dx * dx) -
+               dt / 2.0 * (K2 + (K1 - K2) * (fhalf[ij] + fhalf[imj]) / 2.0) * (hhalf[ij] - hhalf[imj]) / (dx * dx) +
+               (q1[ij] + q2[ij]) * dt / 2.0;
+          rhs[j - 1] = Cp + hhalf[ij];
+        } else {
+          diag[j - 1] = 1.0; sup[j - 1] = 0.0; inf[j - 1] = 0.0; rhs[j - 1] = hhalf[ij];
+        }
+      }
+      rc = tridag(inf, diag, sup, rhs, res, gam, ny);
+      for (int j = 1; j <= ny; j++) tint[I(i, j)] = res[j - 1];
+    }
+    memcpy(h, tint, sz);
+    h[I(1, 1)] = h[I(2, 1)];
+    h[I(1, ny)] = h[I(2, ny)];
+    h[I(nx, 1)] = h[I(nx - 1, 1)];
+    h[I(nx, ny)] = h[I(nx - 1, ny)];
+    /* f along y, :415-468 */
+    for (int i = 2; i <= nx - 1 && !rc; i++) {
+      for (int j = 2; j <= ny - 1; j++) {
+        int ij = I(i, j), ipj = I(i + 1, j), imj = I(i - 1, j), ijp = I(i, j + 1), ijm = I(i, j - 1);
+        if (ht[ij] <= sl && h[ij] >= (1.0 + 1.0e-6) * hhalf[ij]) {
+          double Dp = (h[ij] - hhalf[ij]) / dt;
+          double Ep = K1 / 2.0 * (h[ijp] - h[ij]) / (dy * dy);
+          double Mp = -(K1 / 2.0 * (h[ij] - h[ijm]) / (dy * dy));
+          double Np = K1 / 2.0 * (fhalf[ipj] + fhalf[ij]) * (hhalf[ipj] - hhalf[ij]) / (dx * dx) -
+                      K1 / 2.0 * (fhalf[ij] + fhalf[imj]) * (hhalf[ij] - hhalf[imj]) / (dx * dx) + q1[ij];
+          diag[j - 1] = 2.0 * L / dt + Dp - Mp - Ep;
+          sup[j - 1] = -Ep;
+          inf[j - 1] = -Mp;
+          rhs[j - 1] = Np - Dp * fhalf[ij] + 2.0 * L * fhalf[ij] / dt;
+        } else {
+          diag[j - 1] = 1.0; sup[j - 1] = 0.0; inf[j - 1] = 0.0; rhs[j - 1] = fhalf[ij];
+        }
+      }
+      diag[0] = 1.0; sup[0] = -1.0; rhs[0] = 0.0;
+      diag[ny - 1] = 1.0; inf[ny - 1] = -1.0; rhs[ny - 1] = 0.0;
+      rc = tridag(inf, diag, sup, rhs, res, gam, ny);
+      for (int j = 1; j <= ny; j++) f[I(i, j)] = res[j - 1];
+    }
+    for (int q = 0; q < nn; q++) f[q] = dmin(dmax(0.0, f[q]), 1.0);
+    err1 = 0.0; /* :472 */
+    for (int q = 0; q < nn; q++) err1 = dmax(err1, fabs(h[q] - hp[q]));
+    if (niter > 1000) rc = -8;
+  }
+#undef I
+  for (int q = 0; q < nn; q++)
+    if (h[q] >= sl + 1.0e-3) f[q] = 0.0;
+  free(hp); free(fp); free(ht); free(ft); free(hhalf); free(fhalf); free(fhalfp); free(tint);
+  free(diag); free(sup); free(inf); free(rhs); free(res); free(gam);
+  return rc ? rc : niter;
+}
+
+/* Marine, src/Marine.f90:1-167: the sediment flux that reaches the sea along the step's single-flow
+ * receivers, spread over the shelf along the multiple-flow receivers of the current h (find_mult_rec
+ * with p = 0: weights slope**p_mfd_exp), then silt / sand diffusion in the ocean.  The compaction
+ * block (:120-146) only computes arrays that nothing reads afterwards; it is left out.            */
+static int marine(fso_state *s) {
+  const int nn = s->nn;
+  double dx, dy;
+  dxdy(s, &dx, &dy);
+  const double sl = s->sealevel;
+  const size_t sz = sizeof(double) * (size_t)nn;
+  double *flux = malloc(sz), *sd = malloc(sz), *ht = malloc(sz), *fs = malloc(sz), *q1 = malloc(sz), *q2 = malloc(sz);
+  double *mw = malloc(NDMAX * sz), *ml = malloc(NDMAX * sz), *h0 = malloc(sz);
+  int *mr = malloc(NDMAX * sizeof(int) * (size_t)nn), *mn = malloc(sizeof(int) * (size_t)nn);
+  int *ms = malloc(sizeof(int) * (size_t)nn);
+  for (int i = 0; i < nn; i++) flux[i] = s->h[i] > sl ? s->sedflux[i] : 0.0; /* :28-29 */
+  for (int ij = nn - 1; ij >= 0; ij--) {                                       /* :30-36 */
+    int ijk = s->stack[ij], ijr = s->rec[ijk];
+    if (ijr != ijk && s->h[ijk] > sl) flux[ijr] = flux[ijr] + flux[ijk];
+  }
+  for (int i = 0; i < nn; i++)
+    if (s->h[i] > sl) flux[i] = 0.0;
+  const double ratio1 = s->ratio / (1.0 - s->poro1), ratio2 = (1.0 - s->ratio) / (1.0 - s->poro2); /* :46-50 */
+  for (int i = 0; i < nn; i++) flux[i] = flux[i] * (ratio1 + ratio2);
+  int rc = mult_rec(s, 0.0, h0, mr, mn, mw, ml, ms); /* :60-61 */
+  if (!rc) {
+    const double shelfslope = -1.0e-4; /* :67-79 */
+    for (int i = 0; i < nn; i++) sd[i] = sl;
+    for (int ij = 0; ij < nn; ij++) {
+      int ijk = ms[ij];
+      for (int k = 0; k < mn[ijk]; k++) {
+        int ijr = mr[NDMAX * ijk + k];
+        if (s->h[ijk] < sl) {
+          sd[ijr] = dmin(sd[ijr], sd[ijk] + ml[NDMAX * ijk + k] * shelfslope);
+          sd[ijr] = dmax(sd[ijr], s->h[ijr]);
+        }
+      }
+    }
+    for (int i = 0; i < nn; i++) /* :85-92 */
+      if (s->h[i] < sl) flux[i] = flux[i] + (s->h[i] - sd[i]);
+    for (int ij = 0; ij < nn; ij++) {
+      int ijk = ms[ij];
+      for (int k = 0; k < mn[ijk]; k++) {
+        int ijr = mr[NDMAX * ijk + k];
+        flux[ijr] = flux[ijr] + dmax(0.0, flux[ijk] * mw[NDMAX * ijk + k]);
+      }
+    }
+    for (int i = 0; i < nn; i++) { /* :97-110 */
+      if (flux[i] > 0.0 && s->h[i] < sl) flux[i] = -(s->h[i] - sd[i]);
+      if (flux[i] <= 0.0 && s->h[i] < sl) flux[i] = flux[i] - (s->h[i] - sd[i]);
+      if (s->h[i] >= sl) flux[i] = 0.0;
+      flux[i] = dmax(flux[i], 0.0);
+      fs[i] = flux[i] > 0.0 ? ratio1 / (ratio1 + ratio2) : 0.0;
+      flux[i] = flux[i] / s->dt;
+      q1[i] = flux[i] * fs[i];
+      q2[i] = flux[i] * (1.0 - fs[i]);
+    }
+    memcpy(ht, s->h, sz); /* :113-114 */
+    int it = fso_silt_sand(s->h, s->fmix, q1, q2, s->nx, s->ny, dx, dy, s->dt, sl, s->layer, s->kdsea1, s->kdsea2,
+                           s->ibc);
+    if (it < 0) rc = it;
+    else s->st.marine_iters = it;
+  }
+  if (!rc) {
+    for (int i = 0; i < nn; i++) { /* :148-165 */
+      s->etot[i] = s->etot[i] + ht[i] - s->h[i];
+      s->erate[i] = s->erate[i] + (ht[i] - s->h[i]) / s->dt;
+      if (s->h[i] < sl) { s->sedflux[i] = 0.0; s->etot[i] = 0.0; s->erate[i] = 0.0; }
+      if (s->h[i] >= sl + 1.0e-3) s->fmix[i] = 0.0;
+      s->fmix[i] = dmin(dmax(0.0, s->fmix[i]), 1.0);
+      s->b[i] = dmin(s->h[i], s->b[i]);
+    }
+  }
+  free(flux); free(sd); free(ht); free(fs); free(q1); free(q2); free(mw); free(ml); free(h0); free(mr); free(mn);
+  free(ms);
+  return rc;
+}
+
+/* ------------------------------------------------------------------------------------------ */
 /* Init + SetNXNY + SetUp, src/FastScape_ctx.f90:40-113.  The reference fills h and catch0 with
  * random_number; callers here always provide h through init_h, so h starts at 0.               */
 fso_state *fso_create(int nx, int ny) {
@@ -1002,7 +1256,7 @@ fso_state *fso_create(int nx, int ny) {
 #define DA(p) s->p = (double *)calloc(nn, sizeof(double))
 #define IA(p) s->p = (int *)calloc(nn, sizeof(int))
   DA(h); DA(u); DA(b); DA(etot); DA(erate); DA(a); DA(length); DA(precip); DA(kf); DA(kd); DA(sedflux);
-  DA(lake_depth); DA(hwater); DA(g); DA(p_mfd_exp); DA(vx); DA(vy);
+  DA(lake_depth); DA(hwater); DA(g); DA(p_mfd_exp); DA(vx); DA(vy); DA(fmix);
   IA(rec); IA(ndon); IA(stack); IA(mnrec); IA(mstack);
 #undef DA
 #undef IA
@@ -1011,7 +1265,7 @@ fso_state *fso_create(int nx, int ny) {
   s->mwrec = (double *)calloc(NDMAX * nn, sizeof(double));
   s->mlrec = (double *)calloc(NDMAX * nn, sizeof(double));
   s->bc = (unsigned char *)calloc(nn, 1);
-  for (size_t i = 0; i < nn; i++) { s->precip[i] = 1.0; s->p_mfd_exp[i] = 1.0; }
+  for (size_t i = 0; i < nn; i++) { s->precip[i] = 1.0; s->p_mfd_exp[i] = 1.0; s->fmix[i] = 0.5; }
   s->newton_tol = 1.0e-3;
   s->sealevel = 0.0;
   fso_set_bc(s, 1111);
@@ -1022,7 +1276,7 @@ void fso_destroy(fso_state *s) {
   if (!s) return;
   free(s->h); free(s->u); free(s->b); free(s->etot); free(s->erate); free(s->a); free(s->length);
   free(s->precip); free(s->kf); free(s->kd); free(s->sedflux); free(s->lake_depth); free(s->hwater);
-  free(s->g); free(s->p_mfd_exp); free(s->rec); free(s->ndon); free(s->stack); free(s->mnrec);
+  free(s->g); free(s->p_mfd_exp); free(s->fmix); free(s->rec); free(s->ndon); free(s->stack); free(s->mnrec);
   free(s->mstack); free(s->don); free(s->mrec); free(s->mwrec); free(s->mlrec); free(s->bc);
   free(s->vx); free(s->vy);
   free(s);
@@ -1050,6 +1304,15 @@ void fso_init_h(fso_state *s, const double *h) {
 void fso_set_h(fso_state *s, const double *h) { memcpy(s->h, h, sizeof(double) * (size_t)s->nn); }
 void fso_set_basement(fso_state *s, const double *b) { memcpy(s->b, b, sizeof(double) * (size_t)s->nn); }
 void fso_set_newton_tol(fso_state *s, double tol) { s->newton_tol = tol; }
+/* SetMarineParam, src/FastScape_ctx.f90:454-472 */
+void fso_set_marine_parameters(fso_state *s, double sl, double p1, double p2, double z1, double z2, double r,
+                               double l, double kds1, double kds2) {
+  s->run_marine = 1;
+  s->sealevel = sl; s->poro1 = p1; s->poro2 = p2; s->zporo1 = z1; s->zporo2 = z2; s->ratio = r; s->layer = l;
+  s->kdsea1 = kds1; s->kdsea2 = kds2;
+}
+/* InitF, src/FastScape_ctx.f90:354-362 */
+void fso_init_f(fso_state *s, const double *f) { memcpy(s->fmix, f, sizeof(double) * (size_t)s->nn); }
 void fso_reset_cumulative_erosion(fso_state *s) { memset(s->etot, 0, sizeof(double) * (size_t)s->nn); }
 
 /* SetErosionalParam, src/FastScape_ctx.f90:423-450 */
@@ -1067,7 +1330,7 @@ void fso_set_erosional_parameters(fso_state *s, const double *kf, double kfsed, 
   if (kdmax > DBL_MIN || kdsed > DBL_MIN) s->run_diffusion = 1;
 }
 
-/* FastScape_Execute_Step, src/FastScape_api.f90:201-263 (Advect/Marine/Strati not on this path). */
+/* FastScape_Execute_Step, src/FastScape_api.f90:201-263 (Strati not on this path). */
 int fso_execute_step(fso_state *s) {
   int rc = 0;
   if (s->run_advect) {
@@ -1090,6 +1353,7 @@ int fso_execute_step(fso_state *s) {
     }
   }
   if (!rc && s->run_diffusion) rc = diffusion(s);
+  if (!rc && s->run_marine) rc = marine(s); /* :245-250 */
   s->step++;
   return rc;
 }
@@ -1109,6 +1373,7 @@ void fso_copy_field(const fso_state *s, int field, double *dst) {
     case FSO_LENGTH: src = s->length; break;
     case FSO_HWATER: src = s->hwater; break;
     case FSO_SEDFLUX: src = s->sedflux; break;
+    case FSO_FMIX: src = s->fmix; break;
     default: return;
   }
   memcpy(dst, src, sizeof(double) * (size_t)s->nn);

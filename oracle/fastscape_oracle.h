@@ -30,6 +30,7 @@ typedef struct {
   int depth;         /* longest receiver chain (nodes) after correction      */
   int ngs;           /* Gauss-Seidel iterations of the last SPL call         */
   long newton_iters; /* total Newton iterations of the last SPL call         */
+  int marine_iters;  /* silt / sand diffusion iterations of the last Marine call */
 } fso_stats;
 
 /* ---- state lifecycle (src/FastScape_ctx.f90 Init/SetUp/Destroy) ---- */
@@ -50,6 +51,14 @@ void fso_set_erosional_parameters(fso_state *s, const double *kf, double kfsed, 
 /* Extension: the reference hard-codes the Newton tolerance to 1e-3 (src/StreamPowerLaw.f90:146,344). */
 void fso_set_newton_tol(fso_state *s, double tol);
 void fso_reset_cumulative_erosion(fso_state *s);
+/* SetMarineParam (src/FastScape_ctx.f90:454-472): turns Marine on after Diffusion in every step. */
+void fso_set_marine_parameters(fso_state *s, double sealevel, double poro1, double poro2, double zporo1,
+                               double zporo2, double ratio, double layer, double kdsea1, double kdsea2);
+void fso_init_f(fso_state *s, const double *f); /* InitF: silt fraction Fmix (default 0.5) */
+/* SiltSandCouplingDiffusion (src/Marine.f90:171-499) on caller arrays: h and f in place, q1 / q2 the
+ * silt / sand supply; returns the iteration count, or < 0 (zero pivot, no convergence in 1000). */
+int fso_silt_sand(double *h, double *f, const double *q1, const double *q2, int nx, int ny, double dx, double dy,
+                  double dt, double sealevel, double layer, double kdsea1, double kdsea2, int ibc);
 
 int fso_execute_step(fso_state *s);
 int fso_get_step(const fso_state *s);
@@ -57,7 +66,7 @@ void fso_get_stats(const fso_state *s, fso_stats *out);
 
 /* field ids for fso_copy_field */
 enum {
-  FSO_H = 0, FSO_B, FSO_ETOT, FSO_ERATE, FSO_AREA, FSO_LAKE_DEPTH, FSO_LENGTH, FSO_HWATER, FSO_SEDFLUX
+  FSO_H = 0, FSO_B, FSO_ETOT, FSO_ERATE, FSO_AREA, FSO_LAKE_DEPTH, FSO_LENGTH, FSO_HWATER, FSO_SEDFLUX, FSO_FMIX
 };
 enum { FSO_REC = 0, FSO_NDON, FSO_STACK, FSO_MNREC, FSO_MSTACK };
 void fso_copy_field(const fso_state *s, int field, double *dst);

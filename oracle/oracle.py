@@ -48,7 +48,7 @@ def ref_available(path: str = REF_LIB) -> bool:
 class Stats(C.Structure):
     _fields_ = [("nbasins", C.c_int), ("nopen", C.c_int), ("nconn", C.c_int), ("ntree", C.c_int),
                 ("nlocmin_before", C.c_int), ("nlocmin_after", C.c_int), ("max_tie", C.c_int),
-                ("depth", C.c_int), ("ngs", C.c_int), ("newton_iters", C.c_long)]
+                ("depth", C.c_int), ("ngs", C.c_int), ("newton_iters", C.c_long), ("marine_iters", C.c_int)]
 
     def asdict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -69,6 +69,7 @@ def _c():
                            ("fso_set_precip", [_dp]),
                            ("fso_init_h", [_dp]), ("fso_set_h", [_dp]), ("fso_set_basement", [_dp]),
                            ("fso_set_newton_tol", [C.c_double]),
+                           ("fso_set_marine_parameters", [C.c_double] * 9), ("fso_init_f", [_dp]),
                            ("fso_set_erosional_parameters", [_dp, C.c_double, C.c_double, C.c_double, _dp,
                                                              C.c_double, C.c_double, C.c_double, C.c_double]),
                            ("fso_copy_field", [C.c_int, _dp]), ("fso_copy_ifield", [C.c_int, _ip]),
@@ -102,12 +103,14 @@ def _c():
         lib.fso_catchment.restype = None
         lib.fso_fluxes_sfd.argtypes = [_ip, _ip, _dp, _dp, C.c_void_p, C.c_int, C.c_double, _dp]
         lib.fso_fluxes_sfd.restype = None
+        lib.fso_silt_sand.argtypes = [_dp] * 4 + [C.c_int] * 2 + [C.c_double] * 7 + [C.c_int]
+        lib.fso_silt_sand.restype = C.c_int
         _clib = lib
     return _clib
 
 
 FIELDS = {"h": 0, "b": 1, "etot": 2, "erate": 3, "area": 4, "lake_depth": 5, "length": 6, "hwater": 7,
-          "sedflux": 8}
+          "sedflux": 8, "fmix": 9}
 IFIELDS = {"rec": 0, "ndon": 1, "stack": 2, "mnrec": 3, "mstack": 4}
 
 
@@ -141,6 +144,9 @@ class COracle:
     def set_basement(self, b): self.lib.fso_set_basement(self.h_, _d(self._arr(b)))
     def set_newton_tol(self, t): self.lib.fso_set_newton_tol(self.h_, t)
     def reset_cumulative_erosion(self): self.lib.fso_reset_cumulative_erosion(self.h_)
+    def set_marine_parameters(self, sl, p1, p2, z1, z2, r, l, kds1, kds2):
+        self.lib.fso_set_marine_parameters(self.h_, sl, p1, p2, z1, z2, r, l, kds1, kds2)
+    def init_f(self, f): self.lib.fso_init_f(self.h_, _d(self._arr(f)))
 
     def set_erosional_parameters(self, kf, kfsed, m, n, kd, kdsed, g1, g2, p):
         self.lib.fso_set_erosional_parameters(self.h_, _d(self._arr(kf)), kfsed, m, n, _d(self._arr(kd)),
@@ -237,6 +243,18 @@ def indexx(arr):
     out = np.empty(arr.size, np.int32)
     _c().fso_indexx(arr.size, _d(arr), _i(out))
     return out  # 1-based
+
+
+def silt_sand(h, f, q1, q2, *, nx, ny, dx, dy, dt, sealevel, layer, kdsea1, kdsea2, ibc):
+    """The restated SiltSandCouplingDiffusion on copies of h and f: (h, f, iterations)."""
+    h = np.ascontiguousarray(h, np.float64).copy()
+    f = np.ascontiguousarray(f, np.float64).copy()
+    it = _c().fso_silt_sand(_d(h), _d(f), _d(np.ascontiguousarray(q1, np.float64)),
+                            _d(np.ascontiguousarray(q2, np.float64)), nx, ny, dx, dy, dt, sealevel, layer, kdsea1,
+                            kdsea2, ibc)
+    if it < 0:
+        raise RuntimeError(f"silt_sand status {it}")
+    return h, f, it
 
 
 def local_minima(h, nx, ny, dx, dy, ibc=1111):
@@ -358,6 +376,11 @@ class FortranRef:
 
     def execute_step(self): self.lib.fastscape_execute_step_()
 
+    def set_marine_parameters(self, sl, p1, p2, z1, z2, r, l, kds1, kds2):
+        self.lib.fastscape_set_marine_parameters_(*[self._rd(v) for v in (sl, p1, p2, z1, z2, r, l, kds1, kds2)])
+
+    def init_f(self, f): self.lib.fastscape_init_f_(_d(self._arr(f)))
+
     def get_step(self):
         s = C.c_int()
         self.lib.fastscape_get_step_(C.byref(s))
@@ -365,7 +388,7 @@ class FortranRef:
 
     _COPY = {"h": "fastscape_copy_h_", "b": "fastscape_copy_basement_", "etot": "fastscape_copy_total_erosion_",
              "erate": "fastscape_copy_erosion_rate_", "area": "fastscape_copy_drainage_area_",
-             "lake_depth": "fastscape_copy_lake_depth_"}
+             "lake_depth": "fastscape_copy_lake_depth_", "fmix": "fastscape_copy_f_"}
 
     def field(self, name):
         out = np.empty(self.nn, np.float64)
@@ -425,3 +448,17 @@ class FortranRef:
         out = np.empty(arr.size, np.int32)
         self.lib.loc_min_3_indexx_(self._ri(arr.size), _d(arr), _i(out))
         return out  # 1-based
+
+    def silt_sand(self, h, f, q1, q2, *, nx, ny, dx, dy, dt, sealevel, layer, kdsea1, kdsea2, ibc):
+        """SiltSandCouplingDiffusion on copies of h and f: (h, f, iterations)."""
+        h = np.ascontiguousarray(h, np.float64).copy()
+        f = np.ascontiguousarray(f, np.float64).copy()
+        q1 = np.ascontiguousarray(q1, np.float64)
+        q2 = np.ascontiguousarray(q2, np.float64)
+        niter = C.c_int(0)
+        flag = np.zeros(nx * ny, np.int32)
+        self.lib.siltsandcouplingdiffusion_(_d(h), _d(f), _d(q1), _d(q2), self._ri(nx), self._ri(ny), self._rd(dx),
+                                            self._rd(dy), self._rd(dt), self._rd(sealevel), self._rd(layer),
+                                            self._rd(kdsea1), self._rd(kdsea2), C.byref(niter), _i(flag),
+                                            self._ri(ibc))
+        return h, f, niter.value

@@ -48,6 +48,20 @@ def landscape(nx, ny, seed, *, dx=1000.0, u=1e-3, kf=1e-5, m=0.4, n=1.0, kd=1e-2
                 m=m, n=n, kd=np.full(nx * ny, kd), kdsed=kdsed, g1=g, g2=g, p=p, ibc=ibc, dt=dt)
 
 
+def shelf(nx, ny, seed, *, marine=(0.0, 0.0, 0.0, 1e3, 1e3, 0.5, 1e2, 3e2, 1.5e2), fmix=None, **kw):
+    """A landscape with a sea: the southern half lowered by 1000 m with no uplift there (as the
+    reference's Margin example, examples/Margin.f90), and the marine parameters (sea level, porosities,
+    e-folding depths, silt ratio, mixed-layer thickness, diffusivities); fmix: initial silt fraction."""
+    L = landscape(nx, ny, seed, **kw)
+    south = np.repeat(np.arange(ny) < ny // 2, nx)
+    L["h"] = np.where(south, L["h"] - 1000.0, L["h"])
+    L["u"] = np.where(south, 0.0, L["u"])
+    L["marine"] = np.asarray(marine, np.float64)
+    if fmix is not None:
+        L["fmix"] = np.asarray(fmix, np.float64)
+    return L
+
+
 def apply(obj, L, k=None):
     """Configure an oracle / reference / Batch instance from a landscape description."""
     a = (k,) if k is not None else ()
@@ -60,6 +74,10 @@ def apply(obj, L, k=None):
     obj.set_bc(*a, L["ibc"])
     if "newton_tol" in L:
         obj.set_newton_tol(*a, float(L["newton_tol"]))
+    if "marine" in L:
+        obj.set_marine_parameters(*a, *[float(x) for x in L["marine"]])
+    if "fmix" in L:
+        obj.init_f(*a, L["fmix"])
 
 
 def rel_err(a, b):

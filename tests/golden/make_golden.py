@@ -5,6 +5,8 @@ the reference's outputs, so tests can check the oracle and the GPU path without 
     python tests/golden/make_golden.py          # fixtures of the unmodified reference build
     python tests/golden/make_golden.py tol8     # fixtures of the build with Newton tolerance 1e-8
                                                 # (make -C oracle ref8), names tol8_*
+    python tests/golden/make_golden.py marine   # runs with the marine transport, names marine_*, and
+                                                # vectors of its silt / sand diffusion (silt_sand.npz)
 """
 from __future__ import annotations
 
@@ -18,7 +20,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path[:0] = [os.path.join(ROOT, "oracle"), os.path.join(ROOT, "tests")]
 
 import oracle as O  # noqa: E402
-from conftest import landscape  # noqa: E402
+from conftest import landscape, shelf  # noqa: E402
 
 
 def run_ref(L, steps_to_save, lib=O.REF_LIB):
@@ -30,12 +32,17 @@ def run_ref(L, steps_to_save, lib=O.REF_LIB):
     f.set_erosional_parameters(L["kf"], L["kfsed"], L["m"], L["n"], L["kd"], L["kdsed"], L["g1"], L["g2"], L["p"])
     f.set_u(L["u"])
     f.set_bc(L["ibc"])
+    if "marine" in L:
+        f.set_marine_parameters(*L["marine"])
+    if "fmix" in L:
+        f.init_f(L["fmix"])
     out = {}
     last = max(steps_to_save)
+    fields = ["h", "b", "etot", "erate", "area", "lake_depth"] + (["fmix"] if "marine" in L else [])
     for s in range(1, last + 1):
         f.execute_step()
         if s in steps_to_save:
-            for k in ["h", "b", "etot", "erate", "area", "lake_depth"]:
+            for k in fields:
                 out[f"{k}_{s}"] = f.field(k)
     return out
 
@@ -106,8 +113,47 @@ def main_tol8():
                         **run_ref(L, {1, 30}, O.REF8_LIB))
 
 
+def main_marine():
+    """Landscapes with a sea (the southern half 1000 m down) and the marine transport: single and multiple
+    flow, n = 1, 1.5, 2, with and without deposition, several boundary conditions, sea levels, porosities
+    and initial silt fractions; and the silt / sand diffusion alone on seeded inputs."""
+    base = dict(dx=1000.0, kf=1e-5, dt=1e3, kd=1e-2)
+    cases = {
+        "marine_mfd_1000_p1_g1": dict(ibc=1000, p=1.0, n=2.0, m=0.8, g=1.0, kfsed=1e-5, kdsed=1e-2),
+        "marine_sfd_1000_n1": dict(ibc=1000),
+        "marine_sfd_1111_n15_g1": dict(ibc=1111, n=1.5, m=0.6, g=1.0,
+                                       marine=(0.0, 0.3, 0.45, 1e3, 2e3, 0.3, 50.0, 2e2, 1e2)),
+        "marine_mfd_1010_pneg": dict(ibc=1010, p=-1.0, fmix=np.random.default_rng(9).random(41 * 61)),
+        "marine_sfd_0101_n2_sl": dict(ibc=101, n=2.0, m=0.8, marine=(0.5, 0.1, 0.2, 1e3, 1e3, 0.6, 1e2, 2e2, 1e2)),
+    }
+    for name, kw in cases.items():
+        L = shelf(41, 61, seed=len(name) + 300, **dict(base, **kw))
+        # multiple flow with deposition fills lakes flat; from the second step the reference then reads its
+        # multiple-flow water level at exact flats before setting it (src/StreamPowerLaw.f90:22,98-100), so
+        # that run is kept to its first step
+        steps = {1} if name == "marine_mfd_1000_p1_g1" else {1, 5, 20}
+        np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **inputs(L), **run_ref(L, steps))
+    f = O.FortranRef()
+    vec = {}
+    for i, (nx, ny, ibc) in enumerate([(11, 13, 1000), (31, 23, 1111), (20, 20, 1010), (40, 30, 0)]):
+        rng = np.random.default_rng(40 + i)
+        h = rng.random(nx * ny) * 5.0 - 1000.0
+        h[: nx * ny // 3] += 1000.0 + rng.random(nx * ny // 3)
+        fr = rng.random(nx * ny)
+        q1 = np.where(rng.random(nx * ny) < 0.3, rng.random(nx * ny) * 1e-2, 0.0)
+        q2 = q1 * 0.7
+        args = dict(nx=nx, ny=ny, dx=1e3, dy=1.5e3, dt=1e3, sealevel=0.0, layer=1e2, kdsea1=3e2, kdsea2=1.5e2,
+                    ibc=ibc)
+        hr, fo, it = f.silt_sand(h, fr, q1, q2, **args)
+        vec.update({f"{k}_{i}": np.asarray(v) for k, v in dict(h=h, f=fr, q1=q1, q2=q2, h_out=hr, f_out=fo,
+                                                                   niter=it, **args).items()})
+    np.savez_compressed(os.path.join(HERE, "silt_sand.npz"), **vec)
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["tol8"]:
         main_tol8()
+    elif sys.argv[1:] == ["marine"]:
+        main_marine()
     else:
         main()

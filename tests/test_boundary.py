@@ -85,8 +85,11 @@ def test_facade_call_order_errors():
         fs.fastscape_copy_h(np.zeros(4))
     with pytest.raises(RuntimeError, match="SetUp"):
         fs.fastscape_get_fluxes()
-    for name in ["fastscape_set_advect_every_step", "fastscape_strati", "fastscape_set_marine_parameters",
-                 "fastscape_copy_f"]:
+    with pytest.raises(RuntimeError, match="SetUp"):
+        fs.fastscape_set_marine_parameters(0.0, 0.0, 0.0, 1e3, 1e3, 0.5, 1e2, 3e2, 1.5e2)
+    with pytest.raises(RuntimeError, match="SetUp"):
+        fs.fastscape_copy_f(np.zeros(4))
+    for name in ["fastscape_set_advect_every_step", "fastscape_strati"]:
         if hasattr(fs, name):
             with pytest.raises(NotImplementedError):
                 getattr(fs, name)()

@@ -3,7 +3,8 @@ unmodified reference build by tests/golden/make_golden.py, and tol8_* from the r
 the Newton tolerance 1e-8, which these runs select with set_newton_tol), through the C ABI.
 
 Bar: receivers and donors bit-exact; h, b within 1e-10 relative; drainage area within 1e-12
-relative; etot/erate/lake_depth within 1e-10 of the elevation scale.
+relative; etot/erate/lake_depth within 1e-10 of the elevation scale; the marine runs' silt fraction
+within 1e-10.
 """
 from __future__ import annotations
 
@@ -36,7 +37,7 @@ def _load(name):
 
 
 TRAJ = sorted(os.path.basename(p) for p in glob.glob(os.path.join(GOLDEN, "*.npz"))
-              if "routing" not in p and "indexx" not in p)
+              if "routing" not in p and "indexx" not in p and "silt_sand" not in p)
 
 
 @pytest.mark.parametrize("name", TRAJ)
@@ -60,6 +61,9 @@ def test_trajectory_matches_reference(name):
             for k in ["etot", "erate", "lake_depth"]:
                 ref = g[f"{k}_{s}"]
                 assert np.max(np.abs(got[k] - ref)) <= H_TOL * scale * max(1.0, float(np.abs(ref).max())), (name, s, k)
+            if f"fmix_{s}" in g:
+                from fastscapelib_fortran import _lib
+                assert np.max(np.abs(b.field(0, _lib.FSX_FMIX) - g[f"fmix_{s}"])) <= H_TOL, (name, s, "fmix")
 
 
 @pytest.mark.parametrize("ibc", [1111, 1010, 101, 1000, 11])

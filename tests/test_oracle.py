@@ -36,18 +36,31 @@ def _L_from(g):
 
 # ---- 1. against the reference ---------------------------------------------------------------------
 @pytest.mark.parametrize("name", sorted(os.path.basename(p) for p in glob.glob(os.path.join(GOLDEN, "*.npz"))
-                                        if "routing" not in p and "indexx" not in p))
+                                        if "routing" not in p and "indexx" not in p and "silt_sand" not in p))
 def test_oracle_matches_reference_fixtures(oracle, name):
     g = _golden(name)
     L = _L_from(g)
     o = oracle.COracle(L["nx"], L["ny"])
     apply(o, L)
     saved = sorted({int(k.split("_")[-1]) for k in g if not k.startswith("in_")})
+    fields = ["h", "b", "etot", "erate", "area", "lake_depth"] + (["fmix"] if "marine" in L else [])
     for s in range(1, max(saved) + 1):
         o.execute_step()
         if s in saved:
-            for k in ["h", "b", "etot", "erate", "area", "lake_depth"]:
+            for k in fields:
                 assert np.array_equal(o.field(k), g[f"{k}_{s}"]), f"{name}: {k} differs at step {s}"
+
+
+def test_oracle_silt_sand_matches_reference_fixtures(oracle):
+    """SiltSandCouplingDiffusion alone (src/Marine.f90:171-499) on seeded land / sea elevations, silt
+    fractions and supplies, four boundary conditions: h, f and the iteration count bit for bit."""
+    g = _golden("silt_sand.npz")
+    for i in range(4):
+        a = {k: g[f"{k}_{i}"] for k in ("nx", "ny", "dx", "dy", "dt", "sealevel", "layer", "kdsea1", "kdsea2", "ibc")}
+        a = {k: (int(v) if k in ("nx", "ny", "ibc") else float(v)) for k, v in a.items()}
+        h, f, it = oracle.silt_sand(g[f"h_{i}"], g[f"f_{i}"], g[f"q1_{i}"], g[f"q2_{i}"], **a)
+        assert it == int(g[f"niter_{i}"]), i
+        assert np.array_equal(h, g[f"h_out_{i}"]) and np.array_equal(f, g[f"f_out_{i}"]), i
 
 
 def test_oracle_routing_matches_reference_fixtures(oracle):
