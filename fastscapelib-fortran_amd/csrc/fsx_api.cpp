@@ -59,6 +59,7 @@ struct fsx_ctx {
   bool any_flow = false;                  // some active instance takes the dataflow stream-power sweep
   bool any_advect = false, g_adv = false;  // some active instance advects; as captured
   bool any_marine = false, g_mar = false;  // some active instance runs the marine transport; as captured
+  bool any_strati = false, g_str = false;  // some active instance tracks stratigraphic reflectors; as captured
   // one step captured as a HIP graph and replayed; rebuilt when the launch sequence changes
   bool use_graph = true, graph_valid = false, g_sfd = false, g_mfd = false;
   hipGraphExec_t gexec = nullptr;
@@ -186,13 +187,14 @@ static int check_inst(fsx_ctx* c, int inst) {
 static int upload_params(fsx_ctx* c) {
   if (!c->dirty) return 0;
   c->any_sfd = c->any_mfd = c->any_flow = false;
-  c->any_advect = c->any_marine = false;
+  c->any_advect = c->any_marine = c->any_strati = false;
   for (InstParams& P : c->hp) {
     // g = g1, or g2 where h - b > 1 when g2 > 0 (src/StreamPowerLaw.f90:236-239): zero everywhere exactly when
     // g1 == 0 and g2 <= 0, and then the Gauss-Seidel loop runs once (:375-376)
     P.flow = (P.sfd && P.g1 == 0.0 && !(P.g2 > 0.0)) ? 1 : 0;
     if (P.active && P.run_advect) c->any_advect = true;
     if (P.active && P.run_marine) c->any_marine = true;
+    if (P.active && P.run_strati) c->any_strati = true;
     if (!P.active || !P.run_spl) continue;
     if (P.sfd) c->any_sfd = true; else c->any_mfd = true;
     if (P.flow) c->any_flow = true;
@@ -622,6 +624,79 @@ extern "C" int fsx_set_marine_parameters(fsx_ctx* c, int inst, double sealevel, 
   return 0;
 }
 
+extern "C" int fsx_strati(fsx_ctx* c, int inst, int nstep, int nreflector, int nfreq, double vex) {
+  if (int r = check_inst(c, inst)) return r;
+  if (nreflector < 1 || nfreq < 1 || nstep / nreflector < 1)
+    return fail(FSX_EINVAL, "fsx_strati: need nreflector >= 1, nfreq >= 1 and nstep >= nreflector");
+  InstParams& P = c->hp[inst];
+  DevState& S = c->S;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  int rc = 0;
+  if (!S.strati_cnt) rc = dalloc(c, &S.strati_cnt, 1);
+  double *r = nullptr, *e = nullptr;
+  if (!rc) rc = dalloc(c, &r, (long long)nreflector * P.nn);  // zero-filled
+  if (!rc) rc = dalloc(c, &e, (long long)nreflector * P.nn);
+  if (rc) return rc;
+  P.srefl = r;
+  P.serod = e;
+  P.run_strati = 1;
+  P.nref = nreflector;
+  P.iref = 1;
+  c->dirty = true;
+  c->graph_valid = false;
+  return 0;
+}
+
+extern "C" int fsx_strati_set_active(fsx_ctx* c, int inst, int ireflector) {
+  if (int r = check_inst(c, inst)) return r;
+  InstParams& P = c->hp[inst];
+  if (!P.run_strati) return fail(FSX_EINVAL, "fsx_strati_set_active: no stratigraphy on this landscape");
+  if (ireflector < 1 || ireflector > P.nref + 1)
+    return fail(FSX_EINVAL, "fsx_strati_set_active: reflector out of range");
+  P.iref = ireflector;
+  c->dirty = true;
+  return 0;
+}
+
+extern "C" int fsx_strati_read(fsx_ctx* c, int inst, int which, double* dst) {
+  if (int r = check_inst(c, inst)) return r;
+  if (!dst) return fail(FSX_EINVAL, "fsx_strati_read: null destination");
+  const InstParams& P = c->hp[inst];
+  if (!P.run_strati) return fail(FSX_EINVAL, "fsx_strati_read: no stratigraphy on this landscape");
+  DevState& S = c->S;
+  const size_t nn = (size_t)P.nn, nr = (size_t)P.nref;
+  hipStream_t st = c->stream;
+  if (int r = upload_params(c)) return r;
+  switch (which) {
+    case FSX_STRATI_REFLECTORS:
+      HIPCHK(hipMemcpyAsync(dst, P.srefl, sizeof(double) * nn * nr, hipMemcpyDeviceToHost, st));
+      break;
+    case FSX_STRATI_ERODED:
+      HIPCHK(hipMemcpyAsync(dst, P.serod, sizeof(double) * nn * nr, hipMemcpyDeviceToHost, st));
+      break;
+    case FSX_STRATI_SLOPES:
+      for (size_t i = 0; i < nr; i++) {
+        launch_strati_slope(S, inst, P.nn, P.srefl + i * nn, S.zint + P.off, st);
+        HIPCHK(hipMemcpyAsync(dst + i * nn, S.zint + P.off, sizeof(double) * nn, hipMemcpyDeviceToHost, st));
+      }
+      break;
+    case FSX_STRATI_SHORE:
+      if (!c->routed[inst]) return fail(FSX_EINVAL, "fsx_strati_read: distances to shore need the routing of a step");
+      for (size_t i = 0; i <= nr; i++) {  // each reflector, then the basement
+        const double* surf = i < nr ? P.srefl + i * nn : S.b + P.off;
+        launch_strati_shore(S, inst, P.nn, c->xl[inst], c->yl[inst], surf, S.adx + P.off, S.ady + P.off,
+                            S.strati_cnt, S.zintp + P.off, st);
+        HIPCHK(hipMemcpyAsync(dst + i * nn, S.zintp + P.off, sizeof(double) * nn, hipMemcpyDeviceToHost, st));
+      }
+      break;
+    default:
+      return fail(FSX_EINVAL, "fsx_strati_read: unknown item");
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(st));
+  return 0;
+}
+
 extern "C" int fsx_set_newton_tol(fsx_ctx* c, int inst, double tol) {
   if (int r = check_inst(c, inst)) return r;
   if (!(tol > 0.0)) return fail(FSX_EINVAL, "Newton tolerance must be > 0");
@@ -765,6 +840,7 @@ static void enqueue_step(fsx_ctx* c, hipStream_t st) {
   PHASE(PH_SPL_POST, launch_spl_post(S, st));
   PHASE(PH_DIFFUSION, launch_diffusion(S, st));
   if (c->any_marine) PHASE(PH_MARINE, launch_marine(S, st));  // Marine follows Diffusion (api.f90:245-250)
+  if (c->any_strati) PHASE(PH_STRATI, launch_strati_step(S, st));  // then run_Strati (api.f90:252-257)
 }
 
 static int capture(fsx_ctx* c, hipStream_t st, int lo, int hi, hipGraphExec_t* out) {
@@ -821,6 +897,7 @@ static int build_graphs(fsx_ctx* c, hipStream_t st) {
   c->g_mfd = c->any_mfd;
   c->g_adv = c->any_advect;
   c->g_mar = c->any_marine;
+  c->g_str = c->any_strati;
   c->g_timed = c->timed_phase;
   return 0;
 }
@@ -840,7 +917,7 @@ extern "C" int fsx_execute_steps(fsx_ctx* c, int nsteps, void* stream, int sync)
     // the step's launch sequence depends on the layout and on which routing paths are in use; the
     // kernels read every other parameter from device memory, so one capture serves until those change
     if (!c->graph_valid || c->g_sfd != c->any_sfd || c->g_mfd != c->any_mfd || c->g_adv != c->any_advect ||
-        c->g_mar != c->any_marine || c->g_timed != c->timed_phase) {
+        c->g_mar != c->any_marine || c->g_str != c->any_strati || c->g_timed != c->timed_phase) {
       if (int r = build_graphs(c, st)) return r;
     }
     if (c->gexec) {

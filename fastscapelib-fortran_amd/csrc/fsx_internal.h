@@ -42,6 +42,7 @@ struct InstParams {
   int flow;                  // single flow without deposition (g == 0 everywhere): one stream-power pass,
                              // solved node by node as receivers complete (k_spl_flow)
   int run_marine;            // marine transport after Diffusion (SetMarineParam)
+  int run_strati, nref, iref;  // stratigraphy (Activate_Strati): reflectors, the active one (1-based)
   int aux;                   // offset of the nn+1-entry per-landscape arrays (level starts, groups)
   int c0, c1;                // chunk-table range of the landscape
   double dx, dy, dt;
@@ -50,6 +51,7 @@ struct InstParams {
   double kfsed, m, n, kdsed, g1, g2, p, newton_tol;
   double p_raw;              // p as given to the erosional parameters (p_mfd_exp of single-flow landscapes)
   double sealevel, poro1, poro2, zporo1, zporo2, ratio, layer, kdsea1, kdsea2;  // SetMarineParam
+  double *srefl, *serod;     // stratigraphy: reflectors and the erosion below each (nref x nn, device)
 };
 
 struct Chunk {
@@ -121,6 +123,21 @@ __device__ __forceinline__ int neighbours_wc(const InstParams& P, int v, int* q,
       c++;
     }
   return c;
+}
+
+// slope in degrees (con = 45/atan(1)) of node ij of the surface H (slope, src/TerrainDerivatives.f90:3-36):
+// Sobel-type gradient of the 3x3 block, atan of its norm; 0 on the outer ring
+__device__ __forceinline__ double slope_at(const double* H, int ij, int nx, int ny, double dx, double dy, double con) {
+  const int i = ij % nx, j = ij / nx;
+  if (!(i >= 1 && i < nx - 1 && j >= 1 && j < ny - 1)) return 0.0;
+  const double h1 = H[ij + nx - 1], h2 = H[ij + nx], h3 = H[ij + nx + 1];
+  const double h4 = H[ij - 1], h6 = H[ij + 1];
+  const double h7 = H[ij - nx - 1], h8 = H[ij - nx], h9 = H[ij - nx + 1];
+  const double dzdx = ((h3 + 2.0 * h6 + h9) - (h1 + 2.0 * h4 + h7)) / 8.0 / dx;
+  const double dzdy = ((h7 + 2.0 * h8 + h9) - (h1 + 2.0 * h2 + h3)) / 8.0 / dy;
+  double v = dzdx * dzdx + dzdy * dzdy;
+  if (v > 2.2250738585072014e-308) v = atan(sqrt(v)) * con;
+  return v;
 }
 
 // bit index of direction d in the 8-bit donor mask
@@ -246,6 +263,13 @@ void launch_diffusion(DevState& S, hipStream_t st);
 void launch_mfd_route(DevState& S, hipStream_t st);
 void launch_mfd_spl(DevState& S, hipStream_t st);
 void launch_marine(DevState& S, hipStream_t st);  // needs the marine buffers
+void launch_strati_step(DevState& S, hipStream_t st);
+// stratigraphy read-outs of landscape inst (outside a step; surf and out hold the landscape's nn nodes):
+// slope of surf; distance to shore of surf along the receivers (shore points staged in xs / ys, their
+// count in cnt; xl, yl the landscape's extent)
+void launch_strati_slope(DevState& S, int inst, int nn, const double* surf, double* out, hipStream_t st);
+void launch_strati_shore(DevState& S, int inst, int nn, double xl, double yl, const double* surf, double* xs,
+                         double* ys, int* cnt, double* out, hipStream_t st);
 // read-outs (outside a step) of the landscapes listed in ids (device memory, n entries; outputs at each
 // landscape's node offset), or of the one landscape inst1 of n nodes when ids is null (outputs from 0)
 void launch_terrain(DevState& S, const int* ids, int n, int inst1, int which, double con, double* out, hipStream_t st);
@@ -281,6 +305,7 @@ enum Phase {
   PH_MFD_ROUTE,    //   multiple flow direction: fill, receivers, donors, Kahn layers, accumulation
   PH_MFD_SPL,      //   multiple flow direction: edge factors + Gauss-Seidel stream power
   PH_ADVECT,       // * k_advect (first in the step when a velocity field is set)
-  PH_MARINE,       //   marine transport: shelf flux, silt / sand diffusion (last in the step)
+  PH_MARINE,       //   marine transport: shelf flux, silt / sand diffusion
+  PH_STRATI,       // * k_strati_step: reflectors (last in the step)
   PH_COUNT
 };

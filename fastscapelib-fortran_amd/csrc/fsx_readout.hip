@@ -27,10 +27,7 @@ __global__ __launch_bounds__(256) void k_terrain(const InstParams* __restrict__ 
     const double h4 = H[ij - 1], h5 = H[ij], h6 = H[ij + 1];
     const double h7 = H[ij - nx - 1], h8 = H[ij - nx], h9 = H[ij - nx + 1];
     if (which == 0) {
-      const double dzdx = ((h3 + 2.0 * h6 + h9) - (h1 + 2.0 * h4 + h7)) / 8.0 / dx;
-      const double dzdy = ((h7 + 2.0 * h8 + h9) - (h1 + 2.0 * h2 + h3)) / 8.0 / dy;
-      v = dzdx * dzdx + dzdy * dzdy;
-      if (v > DBL_MIN) v = atan(sqrt(v)) * con;
+      v = slope_at(H, ij, nx, P.ny, dx, dy, con);
     } else {
       const double a = (h1 + h3 + h4 + h6 + h7 + h9) / dx / dx / 12.0 - (h2 + h5 + h8) / dx / dx / 6.0;
       const double b = (h1 + h2 + h3 + h7 + h8 + h9) / dy / dy / 12.0 - (h4 + h5 + h6) / dy / dy / 6.0;

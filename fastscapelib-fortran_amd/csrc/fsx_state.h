@@ -86,6 +86,7 @@ struct DevState {
   double *mar_q1 = nullptr, *mar_qs = nullptr, *mar_ht = nullptr, *mar_ft = nullptr, *mar_hp = nullptr;
   double *mar_fp = nullptr, *mar_hh = nullptr, *mar_fh = nullptr, *mar_fhp = nullptr, *mar_tint = nullptr;
   double* mar_gam = nullptr;
+  int* strati_cnt = nullptr;  // stratigraphy: shore-point count of a distance read-out
 
   // ---- per instance ----
   int *nbasins = nullptr, *basin0 = nullptr, *nconn = nullptr, *nlev = nullptr, *err = nullptr;

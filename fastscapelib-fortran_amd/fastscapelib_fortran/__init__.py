@@ -12,8 +12,10 @@ The free functions keep the reference's call surface and f2py argument conventio
 Documented divergences: call-order and size errors raise ``RuntimeError``/``ValueError`` where the
 reference STOPs the process; ``fastscape_setup`` starts from h = 0 rather than Fortran random
 numbers (every run sets h with ``fastscape_init_h``); routines outside the per-step update
-(stratigraphy, flexure) raise ``NotImplementedError``; ``fastscape_set_marine_parameters`` turns on the
-marine transport (Marine) after Diffusion in every step, on the GPU; ``fastscape_vtk`` writes the
+(flexure) raise ``NotImplementedError``; ``fastscape_set_marine_parameters`` turns on the marine
+transport (Marine) after Diffusion in every step, on the GPU; ``fastscape_strati`` tracks stratigraphic
+reflectors on the GPU in every step and writes the reference's horizon files from the host at its output
+steps (fastscapelib_fortran.strati); ``fastscape_vtk`` writes the
 reference's legacy-VTK files on the host from the fields copied off the GPU; ``fastscape_set_v`` turns on the
 advection of h, b and etot (Advect) at the start of every step, on the GPU.  The slope,
 curvature, chi and catchment read-outs and the three fluxes are computed on the GPU from the resident
@@ -37,7 +39,7 @@ __all__ = [
     "fastscape_copy_erosion_rate", "fastscape_copy_drainage_area", "fastscape_copy_lake_depth",
     "fastscape_copy_slope", "fastscape_copy_curvature", "fastscape_copy_chi", "fastscape_copy_catchment",
     "fastscape_get_fluxes", "fastscape_set_v", "fastscape_vtk", "fastscape_set_marine_parameters",
-    "fastscape_init_f", "fastscape_copy_f",
+    "fastscape_init_f", "fastscape_copy_f", "fastscape_strati",
     "fastscape_get_step", "fastscape_get_sizes", "fastscape_destroy", "fastscape_set_h", "fastscape_set_basement",
     "fastscape_set_precip", "fastscape_set_all_layers", "fastscape_reset_cumulative_erosion", "fastscape_view",
     "fastscape_debug", "Batch", "Lanes", "Pool", "FastscapeError", "indexx",
@@ -106,7 +108,9 @@ class Batch:
     def set_xl_yl(self, k, xl, yl):
         check(lib.fsx_set_xl_yl(self._h, k, float(xl), float(yl)), "set_xl_yl")
         self.__dict__.setdefault("_xlyl", {})[k] = (float(xl), float(yl))  # Pool has its own __init__
-    def set_dt(self, k, dt): check(lib.fsx_set_dt(self._h, k, float(dt)), "set_dt")
+    def set_dt(self, k, dt):
+        check(lib.fsx_set_dt(self._h, k, float(dt)), "set_dt")
+        self.__dict__.setdefault("_dt", {})[k] = float(dt)
     def set_bc(self, k, ibc): check(lib.fsx_set_bc(self._h, k, int(ibc)), "set_bc")
     def set_newton_tol(self, k, tol): check(lib.fsx_set_newton_tol(self._h, k, float(tol)), "set_newton_tol")
 
@@ -115,6 +119,44 @@ class Batch:
         check(lib.fsx_set_marine_parameters(self._h, k, *[float(x) for x in (sealevel, poro1, poro2, zporo1, zporo2,
                                                                           ratio, layer, kdsea1, kdsea2)]),
               "set_marine_parameters")
+        self.__dict__.setdefault("_sealevel", {})[k] = float(sealevel)
+
+    def strati(self, k, nstep, nreflector, nfreq, vex, directory="VTK"):
+        """Stratigraphic horizons of landscape k (FastScape_Strati): nreflector reflectors, a new one active
+        every nstep // nreflector steps, the reference's VTK files written into ``directory`` every nfreq
+        steps (fastscapelib_fortran.strati)."""
+        check(lib.fsx_strati(self._h, k, int(nstep), int(nreflector), int(nfreq), float(vex)), "strati")
+        nref = int(nreflector)
+        self.__dict__.setdefault("_strati", {})[k] = dict(
+            nref=nref, nfreq=int(nfreq), nfreqref=int(nstep) // nref, iref=1, vex=float(vex), directory=directory,
+            last=None, fields=np.zeros((nref, 10, self.nn[k])))
+
+    def strati_read(self, k, which, rows):
+        out = np.empty((rows, self.nn[k]))
+        check(lib.fsx_strati_read(self._h, k, int(which), out.ctypes.data_as(C.POINTER(C.c_double))), "strati_read")
+        return out
+
+    def _strati_output(self, k, step):
+        """One output step of Strati (src/FastScape_ctx.f90:832-836; src/Strati.f90:1-82) for landscape k."""
+        from . import strati as S
+        st = self._strati[k]
+        if step % st["nfreqref"] == 0:
+            st["iref"] += 1
+            check(lib.fsx_strati_set_active(self._h, k, st["iref"]), "strati_set_active")
+        nref = st["nref"]
+        refl = self.strati_read(k, _lib.FSX_STRATI_REFLECTORS, nref)
+        st["fields"][:, 9] = self.strati_read(k, _lib.FSX_STRATI_ERODED, nref)
+        slopes = self.strati_read(k, _lib.FSX_STRATI_SLOPES, nref)
+        dist = self.strati_read(k, _lib.FSX_STRATI_SHORE, nref + 1)
+        b = self.field(k, _lib.FSX_B)
+        dt = self.__dict__.get("_dt", {}).get(k, 0.0)
+        S.update_fields(st["fields"], refl, slopes, dist[:nref], b, self.field(k, _lib.FSX_FMIX), st["iref"],
+                        self.__dict__.get("_sealevel", {}).get(k, 0.0), dt * st["nfreqref"])
+        nx, ny = self.sizes[k]
+        xl, yl = self.__dict__.get("_xlyl", {}).get(k, (float(nx - 1), float(ny - 1)))
+        S.write_output(step, nx, ny, xl, yl, refl, st["fields"], b, dist[nref], st["vex"], st["iref"],
+                       st["directory"])
+        st["last"] = step
     def set_active(self, k, on): check(lib.fsx_set_active(self._h, k, 1 if on else 0), "set_active")
 
     def set_erosional_parameters(self, k, kf, kfsed, m, n, kd, kdsed, g1, g2, p):
@@ -160,7 +202,23 @@ class Batch:
     # -- stepping --
     def step(self, nsteps=1, stream=None, sync=True):
         s = C.c_void_p(stream) if isinstance(stream, int) else (stream or C.c_void_p())
-        check(lib.fsx_execute_steps(self._h, int(nsteps), s, 1 if sync else 0), "execute_steps")
+        st = self.__dict__.get("_strati")
+        if not st:
+            check(lib.fsx_execute_steps(self._h, int(nsteps), s, 1 if sync else 0), "execute_steps")
+            return
+        # with stratigraphy on, the steps run in segments that end at output steps, where the files are
+        # written (the reference writes them from inside its step, src/FastScape_ctx.f90:832-836)
+        left = int(nsteps)
+        while left > 0:
+            n = min([left] + [q["nfreq"] - self.get_step(k) % q["nfreq"] for k, q in st.items()])
+            check(lib.fsx_execute_steps(self._h, n, s, 1), "execute_steps")
+            left -= n
+            for k, q in st.items():
+                t = self.get_step(k)
+                if t % q["nfreq"] == 0 and t != q["last"]:
+                    self._strati_output(k, t)
+        if nsteps == 0:
+            check(lib.fsx_execute_steps(self._h, 0, s, 1 if sync else 0), "execute_steps")
 
     def synchronize(self): check(lib.fsx_synchronize(self._h), "synchronize")
 
@@ -335,7 +393,7 @@ class Lanes:
             "init_h", "set_field", "set_u", "set_v", "set_h", "set_basement", "set_precip", "set_all_layers",
             "reset_cumulative_erosion", "field", "ifield", "h", "rec", "donors", "order", "mrec", "morder",
             "get_step", "stats", "vtk", "derived", "slope", "curvature", "chi", "catchment", "fluxes",
-            "set_marine_parameters", "init_f"}
+            "set_marine_parameters", "init_f", "strati"}
     TUNING = Batch.TUNING
 
     @staticmethod
@@ -485,7 +543,8 @@ class Pool(Batch):
         check(lib.fsx_remove_instance(self._h, int(k)), "remove_instance")
         del self.sizes[k]
         del self.nn[k]
-        self.__dict__.get("_xlyl", {}).pop(k, None)
+        for d in ("_xlyl", "_dt", "_sealevel", "_strati"):
+            self.__dict__.get(d, {}).pop(k, None)
 
     def info(self):
         """(free nodes, largest free contiguous range, free slots)."""
@@ -552,6 +611,12 @@ def fastscape_set_basement(b): _need().set_basement(0, b)
 def fastscape_set_precip(p): _need().set_precip(0, p)
 def fastscape_set_all_layers(dh): _need().set_all_layers(0, dh)
 def fastscape_init_f(f): _need().init_f(0, f)
+
+
+def fastscape_strati(nstepp, nreflectorp, nfreqp, vexp):
+    """FastScape_Strati (src/FastScape_api.f90:745-758): stratigraphic horizons written into VTK/ every
+    nfreqp steps (Horizon*.vtk, Strati-*.vtk, CUBE.vtk), as the reference's Execute_Step does."""
+    _need().strati(0, nstepp, nreflectorp, nfreqp, vexp)
 def fastscape_reset_cumulative_erosion(): _need().reset_cumulative_erosion(0)
 
 
@@ -663,14 +728,3 @@ def fastscape_debug():
     print("Total number of self donors", int((rec == np.arange(rec.size)).sum()))
     print("Number of basins / basin-graph edges / MST edges", st["nbasins"], st["nconn"], st["ntree"])
     print("Number of Gauss-Siedel iterations (SPL)", st["ngs"])
-
-
-def _outside(name):
-    def f(*a, **k):
-        raise NotImplementedError(f"{name} is outside the per-step update implemented on the GPU")
-    f.__name__ = name
-    return f
-
-
-for _n in ["fastscape_strati"]:
-    globals()[_n] = _outside(_n)

@@ -17,11 +17,12 @@ FSX_H, FSX_B, FSX_U, FSX_PRECIP, FSX_ETOT, FSX_ERATE, FSX_AREA, FSX_LAKE_DEPTH, 
     FSX_HWATER, FSX_KF, FSX_KD, FSX_FMIX = range(14)
 FSX_REC, FSX_NDON, FSX_DON, FSX_ORDER, FSX_MREC, FSX_MNREC, FSX_MSTACK = range(7)
 FSX_SLOPE, FSX_CURVATURE, FSX_CHI, FSX_CATCHMENT = range(4)  # fsx_copy_derived
+FSX_STRATI_REFLECTORS, FSX_STRATI_ERODED, FSX_STRATI_SLOPES, FSX_STRATI_SHORE = range(4)  # fsx_strati_read
 
 # phase timers, in the library's order (include/fastscape_hip.h, fsx_get_timers)
 PHASES = ["uplift", "receivers", "donors", "lm_labels", "lm_basins", "lm_walk", "lm_edges", "lm_sort", "lm_mst",
           "lm_orient", "lm_correct", "levels", "area_prep", "area_sweep", "spl_prep", "spl_fact", "spl_sweep",
-          "spl_post", "diffusion", "mfd_route", "mfd_spl", "advect", "marine"]
+          "spl_post", "diffusion", "mfd_route", "mfd_spl", "advect", "marine", "strati"]
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -51,6 +52,9 @@ def _load():
                                          C.c_double, C.c_double, C.c_double, C.c_int],
         "fsx_set_newton_tol": [_vp, C.c_int, C.c_double],
         "fsx_set_marine_parameters": [_vp, C.c_int] + [C.c_double] * 9,
+        "fsx_strati": [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double],
+        "fsx_strati_set_active": [_vp, C.c_int, C.c_int],
+        "fsx_strati_read": [_vp, C.c_int, C.c_int, _dp],
         "fsx_init_h": [_vp, C.c_int, _vp, C.c_int],
         "fsx_set_field": [_vp, C.c_int, C.c_int, _vp, C.c_int],
         "fsx_set_all_layers": [_vp, C.c_int, _vp, C.c_int],

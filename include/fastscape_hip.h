@@ -116,6 +116,26 @@ int fsx_set_erosional_parameters(fsx_ctx *ctx, int inst, const double *kf, doubl
  * FSX_FMIX (default 0.5). */
 int fsx_set_marine_parameters(fsx_ctx *ctx, int inst, double sealevel, double poro1, double poro2, double zporo1,
                               double zporo2, double ratio, double layer, double kdsea1, double kdsea2);
+/* FastScape_Strati (src/FastScape_api.f90:745-758; Activate_Strati src/FastScape_ctx.f90:781-805):
+ * nreflector stratigraphic reflectors, one more becoming active every nstep / nreflector steps, output
+ * every nfreq steps.  Every step then ends with run_Strati's reflector update
+ * (src/FastScape_ctx.f90:809-830) on the GPU; the reference also writes its horizon files from
+ * inside the step every nfreq steps (Strati, src/Strati.f90:1-82): here the caller does that between
+ * steps -- after a step whose count is a multiple of nfreq, it advances the active reflector when the
+ * count is also a multiple of nstep / nreflector (fsx_strati_set_active), reads the reflectors, the
+ * erosion below them, their slopes and the distances to shore (fsx_strati_read), and writes the files
+ * (fastscapelib_fortran.strati does so).  vex is the host writer's.  The reflectors start at zero
+ * (the reference reads them before it first sets them). */
+int fsx_strati(fsx_ctx *ctx, int inst, int nstep, int nreflector, int nfreq, double vex);
+/* The active reflector (1-based; ireflector in run_Strati). */
+int fsx_strati_set_active(fsx_ctx *ctx, int inst, int ireflector);
+/* Stratigraphy read-outs of one landscape into host memory, nreflector x nn values (reflector-major):
+ * FSX_STRATI_REFLECTORS, FSX_STRATI_ERODED (erosion below each reflector, Strati's field 10),
+ * FSX_STRATI_SLOPES (slope of each reflector in degrees, src/TerrainDerivatives.f90:3-36), or
+ * FSX_STRATI_SHORE (distance to shore of each reflector and then of the basement, (nreflector + 1) x nn,
+ * distance_to_shore src/Strati.f90:86-127, along the last step's receivers). */
+enum { FSX_STRATI_REFLECTORS = 0, FSX_STRATI_ERODED = 1, FSX_STRATI_SLOPES = 2, FSX_STRATI_SHORE = 3 };
+int fsx_strati_read(fsx_ctx *ctx, int inst, int which, double *dst);
 /* Newton tolerance of the n != 1 stream-power solve.  No reference setter: the reference
  * hard-codes 1e-3 (src/StreamPowerLaw.f90:344), which is the default here. */
 int fsx_set_newton_tol(fsx_ctx *ctx, int inst, double tol);
@@ -138,7 +158,7 @@ int fsx_set_active(fsx_ctx *ctx, int inst, int active);
 
 /* FastScape_Execute_Step (src/FastScape_api.f90:201-263), nsteps times, on every active
  * instance: Advect, Uplift, then routing + LocalMinima + accumulation + stream power, then Diffusion,
- * then Marine, each where switched on.
+ * then Marine, then the stratigraphic reflectors, each where switched on.
  * Asynchronous on `stream` unless `sync` != 0; device-side failures surface as FSX_ESTEP at the
  * next synchronous call. */
 int fsx_execute_steps(fsx_ctx *ctx, int nsteps, void *stream, int sync);
@@ -206,7 +226,7 @@ int fsx_get_stats(fsx_ctx *ctx, int inst, long long *out6);
  * on the stream it runs on.  fsx_get_timers fills up to n accumulated milliseconds and counts
  * (phase order: uplift, receivers,
  * donors, lm_labels, lm_basins, lm_walk, lm_edges, lm_sort, lm_mst, lm_orient, lm_correct, levels, area_prep, area_sweep, spl_prep, spl_fact, spl_sweep, spl_post,
- * diffusion, mfd_route, mfd_spl, advect, marine) and returns the number of phases. */
+ * diffusion, mfd_route, mfd_spl, advect, marine, strati) and returns the number of phases. */
 int fsx_set_profiling(fsx_ctx *ctx, int on);
 /* Time one phase (0..fsx_get_timers' count-1; -1: none) live while steps replay HIP graphs: the
  * phases before it and after it run as two graphs and the phase itself is launched directly between

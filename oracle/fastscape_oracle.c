@@ -36,6 +36,10 @@ struct fso_state {
   double kfsed, m, n, kdsed, g1, g2, p, sealevel, newton_tol;
   double poro1, poro2, zporo1, zporo2, ratio, layer, kdsea1, kdsea2; /* SetMarineParam */
   int sfd, run_spl, run_uplift, run_diffusion, run_advect, run_marine;
+  /* stratigraphy (Activate_Strati): reflectors (nn x nref) and fields (nn x 10 x nref), Fortran order */
+  int run_strati, nref, iref, nfreq, nfreqref, strati_out;
+  double vexref;
+  double *refl, *sfields;
   int step;
   fso_stats st;
 };
@@ -1246,6 +1250,124 @@ static int marine(fso_state *s) {
 }
 
 /* ------------------------------------------------------------------------------------------ */
+/* distance_to_shore, src/Strati.f90:86-127: for every node whose surface changes sign towards its
+ * receiver, the crossing point h(ij)/(h(rec)-h(ij)) of the way (0.5 when flat); every node's distance
+ * to the nearest crossing (squared distances, 2(xl^2+yl^2) when there is none, then square roots). */
+static void distance_to_shore(const double *h, double *d, int nx, int ny, const int *rec, double xl, double yl) {
+  const int nn = nx * ny;
+  double *x = malloc(sizeof(double) * (size_t)nn), *y = malloc(sizeof(double) * (size_t)nn);
+  for (int ij = 0; ij < nn; ij++) {
+    d[ij] = 2.0 * (xl * xl + yl * yl);
+    x[ij] = xl * (double)(float)(ij % nx) / (double)(nx - 1);
+    y[ij] = yl * (double)(float)(ij / nx) / (double)(ny - 1);
+  }
+  for (int ij = 0; ij < nn; ij++) {
+    const int r = rec[ij];
+    if (h[ij] * h[r] < 0.0) {
+      double rat = 0.5;
+      if (h[r] - h[ij] != 0.0) rat = h[ij] / (h[r] - h[ij]);
+      const double xs = x[ij] + rat * (x[r] - x[ij]), ys = y[ij] + rat * (y[r] - y[ij]);
+      for (int k = 0; k < nn; k++) {
+        const double dist = (x[k] - xs) * (x[k] - xs) + (y[k] - ys) * (y[k] - ys);
+        d[k] = dmin(d[k], dist);
+      }
+    }
+  }
+  for (int k = 0; k < nn; k++)
+    if (d[k] > 0.0) d[k] = sqrt(d[k]);
+  free(x); free(y);
+}
+
+#define SF(ij, l, i) s->sfields[(size_t)(ij) + (size_t)nn * ((l) + 10 * (size_t)(i))]
+/* The fields of Strati, src/Strati.f90:31-50 (1-based reflector i, fields 1..9; field 10 is kept by
+ * run_Strati): depth, slope, thickness to the reflector below (the basement for the first) and to the
+ * basement, and, while the reflector is at or above the active one, depositional bathymetry, slope,
+ * distance to shore and silt fraction; the age dt_ref * (nref - i). */
+static void strati_fields(fso_state *s) {
+  const int nx = s->nx, ny = s->ny, nn = s->nn;
+  double dx, dy;
+  dxdy(s, &dx, &dy);
+  double *sl = malloc(sizeof(double) * (size_t)nn), *dist = calloc((size_t)nn, sizeof(double));
+  const double dtref = s->dt * (double)s->nfreqref;
+  for (int i = 1; i <= s->nref; i++) {
+    const double *r = s->refl + (size_t)nn * (i - 1);
+    fso_slope(r, nx, ny, dx, dy, sl);
+    if (s->run_spl) distance_to_shore(r, dist, nx, ny, s->rec, s->xl, s->yl);
+    for (int ij = 0; ij < nn; ij++) {
+      SF(ij, 0, i - 1) = r[ij];
+      SF(ij, 1, i - 1) = sl[ij];
+      SF(ij, 2, i - 1) = i > 1 ? r[ij] - s->refl[(size_t)nn * (i - 2) + ij] : r[ij] - s->b[ij];
+      SF(ij, 3, i - 1) = r[ij] - s->b[ij];
+      if (i >= s->iref) {
+        SF(ij, 4, i - 1) = r[ij] + s->sealevel;
+        SF(ij, 5, i - 1) = sl[ij];
+        SF(ij, 6, i - 1) = dist[ij];
+        SF(ij, 7, i - 1) = s->fmix[ij];
+      }
+      SF(ij, 8, i - 1) = dmax(dtref * (double)(s->nref - i), 0.0);
+    }
+  }
+  free(sl); free(dist);
+}
+
+/* run_Strati, src/FastScape_ctx.f90:809-839: reflectors uplifted, erosion below each active or buried
+ * reflector accumulated (field 10), buried reflectors cut down to h, the active and later ones set to
+ * h; every nfreq steps the active reflector advances (every nfreq_ref steps) and the fields are
+ * recomputed for output.  The reference reads the reflectors before it first sets them; they start at
+ * zero here (as its zero-filled allocation of them). */
+static void run_strati(fso_state *s) {
+  const int nn = s->nn;
+  for (int i = 0; i < s->nref; i++)
+    for (int ij = 0; ij < nn; ij++) s->refl[(size_t)nn * i + ij] = s->refl[(size_t)nn * i + ij] + s->u[ij] * s->dt;
+  for (int i = 0; i < s->iref; i++)
+    for (int ij = 0; ij < nn; ij++)
+      SF(ij, 9, i) = SF(ij, 9, i) + dmax(0.0, s->refl[(size_t)nn * i + ij] - s->h[ij]);
+  for (int i = 0; i < s->iref - 1; i++)
+    for (int ij = 0; ij < nn; ij++) s->refl[(size_t)nn * i + ij] = dmin(s->refl[(size_t)nn * i + ij], s->h[ij]);
+  for (int i = s->iref - 1; i < s->nref; i++)
+    for (int ij = 0; ij < nn; ij++) s->refl[(size_t)nn * i + ij] = s->h[ij];
+  s->strati_out = 0;
+  if ((s->step + 1) % s->nfreq == 0) {
+    if ((s->step + 1) % s->nfreqref == 0) s->iref++;
+    strati_fields(s);
+    s->strati_out = 1;
+  }
+}
+#undef SF
+
+/* Activate_Strati, src/FastScape_ctx.f90:781-805: nfield = 10, nfreqref = nstep / nreflector,
+ * ireflector = 1, fields = 0. */
+int fso_set_strati(fso_state *s, int nstep, int nreflector, int nfreq, double vex) {
+  if (nreflector < 1 || nfreq < 1 || nstep / nreflector < 1) return -1;
+  free(s->refl); free(s->sfields);
+  s->nref = nreflector;
+  s->nfreq = nfreq;
+  s->nfreqref = nstep / nreflector;
+  s->iref = 1;
+  s->vexref = vex;
+  s->refl = calloc((size_t)s->nn * nreflector, sizeof(double));
+  s->sfields = calloc((size_t)s->nn * 10 * nreflector, sizeof(double));
+  s->run_strati = 1;
+  s->strati_out = 0;
+  return 0;
+}
+
+/* The stratigraphy state: reflectors (nref x nn), fields (nref x 10 x nn), active reflector and
+ * whether the last step was an output step. */
+void fso_strati_state(const fso_state *s, double *refl, double *fields, int *iref, int *out) {
+  const size_t nn = (size_t)s->nn;
+  if (refl) memcpy(refl, s->refl, sizeof(double) * nn * s->nref);
+  if (fields) memcpy(fields, s->sfields, sizeof(double) * nn * 10 * s->nref);
+  if (iref) *iref = s->iref;
+  if (out) *out = s->strati_out;
+}
+
+/* distance_to_shore on caller arrays (rec 0-based) */
+void fso_distance_to_shore(const double *h, double *d, int nx, int ny, const int *rec, double xl, double yl) {
+  distance_to_shore(h, d, nx, ny, rec, xl, yl);
+}
+
+/* ------------------------------------------------------------------------------------------ */
 /* Init + SetNXNY + SetUp, src/FastScape_ctx.f90:40-113.  The reference fills h and catch0 with
  * random_number; callers here always provide h through init_h, so h starts at 0.               */
 fso_state *fso_create(int nx, int ny) {
@@ -1276,7 +1398,7 @@ void fso_destroy(fso_state *s) {
   if (!s) return;
   free(s->h); free(s->u); free(s->b); free(s->etot); free(s->erate); free(s->a); free(s->length);
   free(s->precip); free(s->kf); free(s->kd); free(s->sedflux); free(s->lake_depth); free(s->hwater);
-  free(s->g); free(s->p_mfd_exp); free(s->fmix); free(s->rec); free(s->ndon); free(s->stack); free(s->mnrec);
+  free(s->g); free(s->p_mfd_exp); free(s->fmix); free(s->refl); free(s->sfields); free(s->rec); free(s->ndon); free(s->stack); free(s->mnrec);
   free(s->mstack); free(s->don); free(s->mrec); free(s->mwrec); free(s->mlrec); free(s->bc);
   free(s->vx); free(s->vy);
   free(s);
@@ -1330,7 +1452,7 @@ void fso_set_erosional_parameters(fso_state *s, const double *kf, double kfsed, 
   if (kdmax > DBL_MIN || kdsed > DBL_MIN) s->run_diffusion = 1;
 }
 
-/* FastScape_Execute_Step, src/FastScape_api.f90:201-263 (Strati not on this path). */
+/* FastScape_Execute_Step, src/FastScape_api.f90:201-263 (the stratigraphy's files are written by the host). */
 int fso_execute_step(fso_state *s) {
   int rc = 0;
   if (s->run_advect) {
@@ -1354,6 +1476,7 @@ int fso_execute_step(fso_state *s) {
   }
   if (!rc && s->run_diffusion) rc = diffusion(s);
   if (!rc && s->run_marine) rc = marine(s); /* :245-250 */
+  if (!rc && s->run_strati) run_strati(s);  /* :252-257 */
   s->step++;
   return rc;
 }

@@ -55,6 +55,13 @@ void fso_reset_cumulative_erosion(fso_state *s);
 void fso_set_marine_parameters(fso_state *s, double sealevel, double poro1, double poro2, double zporo1,
                                double zporo2, double ratio, double layer, double kdsea1, double kdsea2);
 void fso_init_f(fso_state *s, const double *f); /* InitF: silt fraction Fmix (default 0.5) */
+/* Activate_Strati (src/FastScape_ctx.f90:781-805): reflectors updated every step (run_Strati), fields of
+ * Strati (src/Strati.f90:31-50) recomputed every nfreq steps; 0, or -1 for a bad schedule. */
+int fso_set_strati(fso_state *s, int nstep, int nreflector, int nfreq, double vex);
+/* reflectors (nref x nn), fields (nref x 10 x nn, field l of reflector i at (i*10+l)*nn), the active
+ * reflector (1-based), and 1 when the last step was an output step; any pointer may be NULL */
+void fso_strati_state(const fso_state *s, double *refl, double *fields, int *iref, int *out);
+void fso_distance_to_shore(const double *h, double *d, int nx, int ny, const int *rec, double xl, double yl);
 /* SiltSandCouplingDiffusion (src/Marine.f90:171-499) on caller arrays: h and f in place, q1 / q2 the
  * silt / sand supply; returns the iteration count, or < 0 (zero pivot, no convergence in 1000). */
 int fso_silt_sand(double *h, double *f, const double *q1, const double *q2, int nx, int ny, double dx, double dy,

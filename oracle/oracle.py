@@ -105,6 +105,12 @@ def _c():
         lib.fso_fluxes_sfd.restype = None
         lib.fso_silt_sand.argtypes = [_dp] * 4 + [C.c_int] * 2 + [C.c_double] * 7 + [C.c_int]
         lib.fso_silt_sand.restype = C.c_int
+        lib.fso_set_strati.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double]
+        lib.fso_set_strati.restype = C.c_int
+        lib.fso_strati_state.argtypes = [C.c_void_p, _dp, _dp, _ip, _ip]
+        lib.fso_strati_state.restype = None
+        lib.fso_distance_to_shore.argtypes = [_dp, _dp, C.c_int, C.c_int, _ip, C.c_double, C.c_double]
+        lib.fso_distance_to_shore.restype = None
         _clib = lib
     return _clib
 
@@ -147,6 +153,19 @@ class COracle:
     def set_marine_parameters(self, sl, p1, p2, z1, z2, r, l, kds1, kds2):
         self.lib.fso_set_marine_parameters(self.h_, sl, p1, p2, z1, z2, r, l, kds1, kds2)
     def init_f(self, f): self.lib.fso_init_f(self.h_, _d(self._arr(f)))
+
+    def strati(self, nstep, nreflector, nfreq, vex):
+        if self.lib.fso_set_strati(self.h_, int(nstep), int(nreflector), int(nfreq), float(vex)):
+            raise ValueError("bad stratigraphy schedule")
+        self.nref = int(nreflector)
+
+    def strati_state(self):
+        """(reflectors (nref, nn), fields (nref, 10, nn), active reflector, last step was an output step)"""
+        refl = np.empty((self.nref, self.nn))
+        fields = np.empty((self.nref, 10, self.nn))
+        iref, out = C.c_int(), C.c_int()
+        self.lib.fso_strati_state(self.h_, _d(refl), _d(fields), C.byref(iref), C.byref(out))
+        return refl, fields, iref.value, bool(out.value)
 
     def set_erosional_parameters(self, kf, kfsed, m, n, kd, kdsed, g1, g2, p):
         self.lib.fso_set_erosional_parameters(self.h_, _d(self._arr(kf)), kfsed, m, n, _d(self._arr(kd)),
@@ -243,6 +262,13 @@ def indexx(arr):
     out = np.empty(arr.size, np.int32)
     _c().fso_indexx(arr.size, _d(arr), _i(out))
     return out  # 1-based
+
+
+def distance_to_shore(h, rec, nx, ny, xl, yl):
+    d = np.empty(nx * ny)
+    _c().fso_distance_to_shore(_d(np.ascontiguousarray(h, np.float64)), _d(d), nx, ny,
+                               _i(np.ascontiguousarray(rec, np.int32)), xl, yl)
+    return d
 
 
 def silt_sand(h, f, q1, q2, *, nx, ny, dx, dy, dt, sealevel, layer, kdsea1, kdsea2, ibc):
@@ -380,6 +406,10 @@ class FortranRef:
         self.lib.fastscape_set_marine_parameters_(*[self._rd(v) for v in (sl, p1, p2, z1, z2, r, l, kds1, kds2)])
 
     def init_f(self, f): self.lib.fastscape_init_f_(_d(self._arr(f)))
+
+    def strati(self, nstep, nreflector, nfreq, vex):
+        """fastscape_strati: output into VTK/ of the working directory at every nfreq-th step."""
+        self.lib.fastscape_strati_(self._ri(nstep), self._ri(nreflector), self._ri(nfreq), self._rd(vex))
 
     def get_step(self):
         s = C.c_int()

@@ -7,6 +7,8 @@ the reference's outputs, so tests can check the oracle and the GPU path without 
                                                 # (make -C oracle ref8), names tol8_*
     python tests/golden/make_golden.py marine   # runs with the marine transport, names marine_*, and
                                                 # vectors of its silt / sand diffusion (silt_sand.npz)
+    python tests/golden/make_golden.py strati   # the files the reference's stratigraphy writes
+                                                # (strati_*.npz)
 """
 from __future__ import annotations
 
@@ -150,10 +152,56 @@ def main_marine():
     np.savez_compressed(os.path.join(HERE, "silt_sand.npz"), **vec)
 
 
+def _strati_files(args):
+    """Run a landscape with stratigraphy in a scratch directory (one process per run: the reference's
+    module state); returns {file name: bytes} of what it wrote into VTK/."""
+    import tempfile
+    L, sched, steps = args
+    d = tempfile.mkdtemp()
+    os.chdir(d)
+    f = O.FortranRef()
+    f.setup(L["nx"], L["ny"])
+    f.set_xl_yl(L["xl"], L["yl"])
+    f.set_dt(L["dt"])
+    f.init_h(L["h"])
+    f.set_erosional_parameters(L["kf"], L["kfsed"], L["m"], L["n"], L["kd"], L["kdsed"], L["g1"], L["g2"], L["p"])
+    f.set_u(L["u"])
+    f.set_bc(L["ibc"])
+    if "marine" in L:
+        f.set_marine_parameters(*L["marine"])
+    f.strati(*sched)
+    for _ in range(steps):
+        f.execute_step()
+    out = {}
+    for n in sorted(os.listdir("VTK")):
+        with open(os.path.join("VTK", n), "rb") as fh:
+            out[n] = fh.read()
+    return out
+
+
+def main_strati():
+    """Stratigraphic horizons: the reference's VTK output (Horizon*, Strati-*, CUBE) of two small runs --
+    single flow with a sea and the marine transport, and single flow on land -- as byte arrays."""
+    import multiprocessing as mp
+    cases = {
+        "strati_marine_1000": (shelf(17, 13, seed=21, dx=1000.0, kf=1e-5, dt=1e3, kd=1e-2, ibc=1000), (12, 3, 4, 10.0), 12),
+        "strati_land_1111": (landscape(15, 12, seed=22, dx=500.0, sinus=(8.0, 3000.0)), (10, 2, 5, -5.0), 10),
+    }
+    ctx = mp.get_context("spawn")
+    for name, (L, sched, steps) in cases.items():
+        with ctx.Pool(1) as pool:
+            files = pool.map(_strati_files, [(L, sched, steps)])[0]
+        rec = {"sched": np.asarray(sched, np.float64), "steps": np.asarray(steps), "names": np.asarray(sorted(files))}
+        rec.update({"file_" + n: np.frombuffer(b, np.uint8) for n, b in files.items()})
+        np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **inputs(L), **rec)
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["tol8"]:
         main_tol8()
     elif sys.argv[1:] == ["marine"]:
         main_marine()
+    elif sys.argv[1:] == ["strati"]:
+        main_strati()
     else:
         main()

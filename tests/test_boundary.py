@@ -89,10 +89,8 @@ def test_facade_call_order_errors():
         fs.fastscape_set_marine_parameters(0.0, 0.0, 0.0, 1e3, 1e3, 0.5, 1e2, 3e2, 1.5e2)
     with pytest.raises(RuntimeError, match="SetUp"):
         fs.fastscape_copy_f(np.zeros(4))
-    for name in ["fastscape_set_advect_every_step", "fastscape_strati"]:
-        if hasattr(fs, name):
-            with pytest.raises(NotImplementedError):
-                getattr(fs, name)()
+    with pytest.raises(RuntimeError, match="SetUp"):
+        fs.fastscape_strati(10, 2, 5, 1.0)
 
 
 def test_bench_refuses_several_gpus():

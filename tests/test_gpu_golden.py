@@ -37,7 +37,7 @@ def _load(name):
 
 
 TRAJ = sorted(os.path.basename(p) for p in glob.glob(os.path.join(GOLDEN, "*.npz"))
-              if "routing" not in p and "indexx" not in p and "silt_sand" not in p)
+              if "routing" not in p and "indexx" not in p and "silt_sand" not in p and "strati_" not in p)
 
 
 @pytest.mark.parametrize("name", TRAJ)

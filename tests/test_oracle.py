@@ -36,7 +36,8 @@ def _L_from(g):
 
 # ---- 1. against the reference ---------------------------------------------------------------------
 @pytest.mark.parametrize("name", sorted(os.path.basename(p) for p in glob.glob(os.path.join(GOLDEN, "*.npz"))
-                                        if "routing" not in p and "indexx" not in p and "silt_sand" not in p))
+                                        if "routing" not in p and "indexx" not in p and "silt_sand" not in p
+                                        and "strati_" not in p))
 def test_oracle_matches_reference_fixtures(oracle, name):
     g = _golden(name)
     L = _L_from(g)
