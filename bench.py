@@ -73,7 +73,7 @@ def _sync():
 
 GRAPHS = True  # replay captured HIP graphs (off: launch every kernel directly, for kernel-trace profilers)
 TUNE = True    # run the launch-shape autotuner after the warm-up steps of a batch
-SHAPES = None  # or launch shapes given on the command line (walk_gx, jump_blocks, tour_gx, nr_small_gx, area_wg, bor_gx[, flow_wg])
+SHAPES = None  # or launch shapes given on the command line (walk_gx, jump_blocks, tour_gx, nr_small_gx, area_wg, bor_gx[, flow_wg, flow_split])
 
 
 def open_batch(fs, Ls, lanes=1):
@@ -433,7 +433,7 @@ def main():
     ap.add_argument("--lanes", type=int, default=2,
                     help="contexts stepping the batch side by side on their own streams (1: one context)")
     ap.add_argument("--shapes", default=None,
-                    help="walk_gx,jump_blocks,tour_gx,nr_small_gx,area_wg,bor_gx[,flow_wg] instead of the autotuner")
+                    help="walk_gx,jump_blocks,tour_gx,nr_small_gx,area_wg,bor_gx[,flow_wg,flow_split] instead of the autotuner")
     args = ap.parse_args()
     global GRAPHS, TUNE, SHAPES
     GRAPHS = not args.no_graphs

@@ -51,7 +51,8 @@ struct fsx_ctx {
   std::vector<int> ibc;
   std::vector<char> used;                    // slot holds a landscape
   std::map<long long, long long> free_ranges;  // free node ranges of the arena (offset -> length)
-  long long capacity = 0, chunk_cap = 0, tile_cap = 0;
+  long long capacity = 0, chunk_cap = 0, tile_cap = 0, flow_cap = 0;
+  std::vector<int2> flow_host;  // the dataflow sweep's workgroup table as uploaded
   std::vector<void*> allocs;
   hipStream_t stream = nullptr;
   bool dirty = true;
@@ -184,6 +185,8 @@ static int check_inst(fsx_ctx* c, int inst) {
   return 0;
 }
 
+static int build_flow_table(fsx_ctx* c);
+
 static int upload_params(fsx_ctx* c) {
   if (!c->dirty) return 0;
   c->any_sfd = c->any_mfd = c->any_flow = false;
@@ -199,6 +202,7 @@ static int upload_params(fsx_ctx* c) {
     if (P.sfd) c->any_sfd = true; else c->any_mfd = true;
     if (P.flow) c->any_flow = true;
   }
+  if (int r = build_flow_table(c)) return r;
   HIPCHK(hipMemcpyAsync(c->S.P, c->hp.data(), sizeof(InstParams) * c->hp.size(), hipMemcpyHostToDevice, c->stream));
   c->dirty = false;
   return 0;
@@ -246,6 +250,74 @@ static void arena_give(fsx_ctx* c, long long off, long long n) {
       c->free_ranges.erase(it);
     }
   }
+}
+
+// Workgroups of the dataflow stream-power sweep.  Every landscape that takes it gets one; with flow_split > 0
+// the compute units left over go to the heaviest (nodes, x4 with a Newton solve) as further workgroups of
+// flow_split nodes each, at most kFlowGMax per landscape (none for landscapes with more tiles than a
+// workgroup's own-tile bitmap holds).  A landscape's workgroups are consecutive in the table, so they are
+// dealt to one XCD.  Rebuilt with the parameters; a changed table invalidates the captured step.
+static const int kFlowGMax = 8;
+static const long long kFlowMaxTiles = 2048LL * 32;
+static int cu_count() {
+  static int n = 0;
+  if (!n) {
+    int dev = 0;
+    hipDeviceProp_t p;
+    n = (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&p, dev) == hipSuccess) ? p.multiProcessorCount
+                                                                                             : 256;
+  }
+  return n;
+}
+static int build_flow_table(fsx_ctx* c) {
+  DevState& S = c->S;
+  std::vector<int> idx;
+  auto weight = [&](int k) {
+    const InstParams& P = c->hp[k];
+    return (long long)P.nn * (std::fabs(P.n - 1.0) < 2.2250738585072014e-308 ? 1 : 4);
+  };
+  for (int k = 0; k < S.ninst; k++) {
+    InstParams& P = c->hp[k];
+    P.flow_g = (c->used[k] && P.active && P.run_spl && P.flow) ? 1 : 0;
+    if (P.flow_g) idx.push_back(k);
+  }
+  if (S.flow_split > 0) {
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return weight(a) > weight(b); });
+    long long extra = std::max(0LL, (long long)cu_count() - (long long)idx.size());
+    for (int k : idx) {
+      InstParams& P = c->hp[k];
+      if ((P.nn + 63) / 64 > kFlowMaxTiles) continue;
+      const long long want = std::min<long long>(kFlowGMax, (weight(k) + S.flow_split - 1) / S.flow_split);
+      const long long add = std::min(std::max(0LL, want - 1), extra);
+      P.flow_g += (int)add;
+      extra -= add;
+    }
+  }
+  std::vector<int2> tab;
+  for (int k = 0; k < S.ninst; k++)
+    for (int r = 0; r < c->hp[k].flow_g; r++) tab.push_back(make_int2(k, r));
+  S.nflow = (int)tab.size();
+  const bool same = tab.size() == c->flow_host.size() &&
+                    std::equal(tab.begin(), tab.end(), c->flow_host.begin(),
+                               [](const int2& a, const int2& b) { return a.x == b.x && a.y == b.y; });
+  if (same) return 0;
+  if ((long long)tab.size() > c->flow_cap) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (S.flow_tab) HIPCHK(hipFree(S.flow_tab));
+    S.flow_tab = nullptr;
+    c->flow_cap = 0;
+    const long long cap = (long long)tab.size() + 64;
+    hipError_t e = hipMalloc(&S.flow_tab, sizeof(int2) * cap);
+    if (e != hipSuccess) return fail(FSX_ENOMEM, std::string("hipMalloc (flow table): ") + hipGetErrorString(e));
+    c->flow_cap = cap;
+  }
+  if (!tab.empty()) {
+    HIPCHK(hipMemcpyAsync(S.flow_tab, tab.data(), sizeof(int2) * tab.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  c->flow_host = std::move(tab);
+  c->graph_valid = false;
+  return 0;
 }
 
 static int relayout(fsx_ctx* c) {
@@ -361,7 +433,7 @@ static int ctx_alloc(long long capacity, int nslots, fsx_ctx** out) {
   A(sh, FSX_SLAB_CAP * N);
   A(nbasins, ninst); A(basin0, ninst); A(nconn, ninst); A(nlev, ninst); A(err, ninst); A(ngs, ninst);
   A(ntree, ninst); A(chunkcnt, c->chunk_cap); A(newton, ninst); A(scal, ninst); A(redk, 2 * ninst); A(flags, 64);
-  A(nsegs, ninst); A(segctr, ninst); A(maxd, ninst); A(bstate, 4 * ninst); A(nrst, 32 * ninst);
+  A(nsegs, ninst); A(segctr, ninst); A(maxd, ninst); A(bstate, 4 * ninst); A(nrst, 32 * ninst); A(flow_next, ninst);
   A(J, N); A(ngroups, ninst); A(grec, 2 * (N + ninst)); A(gmeta, N + ninst); A(pos, N); A(lcur, N + ninst); A(rp, N); A(LSl, N); A(dn, 8 * N);
   A(A0, N); A(Al, N); A(F, N); A(HTl, N); A(Hl, N); A(Wl, N); A(HWl, N); A(HPl, N); A(DHl, N); A(ELl, N);
   A(arec, N); A(srec, N); A(wnode, N); A(wmask, N);
@@ -485,6 +557,7 @@ extern "C" int fsx_destroy(fsx_ctx* c) {
   for (auto& e : c->ev) (void)hipEventDestroy(e);
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->S.tiles) (void)hipFree(c->S.tiles);
+  if (c->S.flow_tab) (void)hipFree(c->S.flow_tab);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return 0;
@@ -1268,6 +1341,7 @@ extern "C" int fsx_autotune(fsx_ctx* c, int reps, int* chosen) {
       {&DevState::area_wg, {256, 512, 1024, 0}, 3, PH_AREA_SWEEP, -1},
       {&DevState::bor_gx, {2, 4, 8, 16}, 4, PH_LM_MST, -1},
       {&DevState::flow_wg, {256, 512, 1024, 0}, 3, PH_SPL_SWEEP, -1},
+      {&DevState::flow_split, {0, 262144, 131072, 65536}, 4, PH_SPL_SWEEP, -1},
   };
   // keep the caller's profiling state and accumulated timers
   if (int r = flush_timers_any(c, c->stream)) return r;
@@ -1292,8 +1366,13 @@ extern "C" int fsx_autotune(fsx_ctx* c, int reps, int* chosen) {
     };
     int best = c->S.*k.field;
     double best_ms = 1e300;
+    const bool split = k.field == &DevState::flow_split;
     for (int i = 0; i < k.ncand && !rc; i++) {
       c->S.*k.field = k.cand[i];
+      if (split) {
+        c->dirty = true;
+        if ((rc = upload_params(c))) break;
+      }
       pass();  // untimed pass: caches and code warm for this shape
       ++c->ring_used;
       if ((rc = fsx_get_timers(c, nullptr, nullptr, 0)) < 0) break;
@@ -1309,6 +1388,10 @@ extern "C" int fsx_autotune(fsx_ctx* c, int reps, int* chosen) {
       if (t < best_ms) { best_ms = t; best = k.cand[i]; }
     }
     c->S.*k.field = best;
+    if (split && !rc) {
+      c->dirty = true;
+      rc = upload_params(c);
+    }
   }
   if (!rc) rc = check_errors(c);
   const int rp = fsx_set_profiling(c, prof0 ? 1 : 0);
@@ -1329,6 +1412,7 @@ extern "C" int fsx_get_tuning(fsx_ctx* c, int* out) {
   out[4] = c->S.area_wg;
   out[5] = c->S.bor_gx;
   out[6] = c->S.flow_wg;
+  out[7] = c->S.flow_split;
   return 0;
 }
 
@@ -1342,6 +1426,7 @@ extern "C" int fsx_set_tuning(fsx_ctx* c, const int* in) {
   if (in[5] < 1 || in[5] > 1024) return fail(FSX_EINVAL, "fsx_set_tuning: bor_gx must be in [1, 1024]");
   if (in[6] != 256 && in[6] != 512 && in[6] != 1024)
     return fail(FSX_EINVAL, "fsx_set_tuning: flow_wg must be 256, 512 or 1024");
+  if (in[7] < 0) return fail(FSX_EINVAL, "fsx_set_tuning: flow_split must be >= 0");
   c->S.walk_gx = in[0];
   c->S.jump_blocks = in[1];
   c->S.tour_gx = in[2];
@@ -1349,6 +1434,8 @@ extern "C" int fsx_set_tuning(fsx_ctx* c, const int* in) {
   c->S.area_wg = in[4];
   c->S.bor_gx = in[5];
   c->S.flow_wg = in[6];
+  if (c->S.flow_split != in[7]) c->dirty = true;  // the sweep's workgroup table follows at the next upload
+  c->S.flow_split = in[7];
   c->graph_valid = false;  // captured launches carry the old shapes
   return 0;
 }

@@ -41,6 +41,7 @@ struct InstParams {
   int active;
   int flow;                  // single flow without deposition (g == 0 everywhere): one stream-power pass,
                              // solved node by node as receivers complete (k_spl_flow)
+  int flow_g;                // workgroups of k_spl_flow on this landscape
   int run_marine;            // marine transport after Diffusion (SetMarineParam)
   int run_strati, nref, iref;  // stratigraphy (Activate_Strati): reflectors, the active one (1-based)
   int aux;                   // offset of the nn+1-entry per-landscape arrays (level starts, groups)

@@ -12,6 +12,9 @@ struct DevState {
   Chunk* chunks = nullptr;  // [nchunks]
   int ntiles = 0;           // 2-D tiles (FSX_TW x FSX_TH nodes) covering every instance
   int4* tiles = nullptr;    // [ntiles]: (instance, x0, y0, -)
+  int nflow = 0;            // workgroups of the dataflow stream-power sweep over all instances
+  int2* flow_tab = nullptr; // [nflow]: (instance, rank among the instance's workgroups)
+  int* flow_next = nullptr; // [ninst]: the sweep's tile counter of each instance
   // launch shapes, chosen by fsx_autotune (no result depends on them)
   int walk_gx = 256;       // basin walk: blocks per landscape
   int jump_blocks = 2048;  // pointer jumping: grid cap
@@ -19,7 +22,8 @@ struct DevState {
   int nr_small_gx = 64;    // edge sort: blocks per landscape for the segments in LDS
   int area_wg = 1024;      // drainage-area sweep: threads per landscape (256, 512 or 1024)
   int bor_gx = 4;          // spanning forest: workgroups per landscape for the first rounds' edge scans
-  int flow_wg = 1024;      // stream-power dataflow sweep: threads per landscape (256, 512 or 1024)
+  int flow_wg = 1024;      // stream-power dataflow sweep: threads per workgroup (256, 512 or 1024)
+  int flow_split = 131072; // stream-power dataflow sweep: nodes per extra workgroup (0: one per landscape)
   int nr_gx = 8;           // edge sort: workgroups per landscape for the big ranges of a level
 
   // ---- per node (ntot) ----

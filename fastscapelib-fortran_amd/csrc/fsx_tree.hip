@@ -904,6 +904,13 @@ __device__ __forceinline__ double flow_ld(const double* p) {
 __device__ __forceinline__ void flow_st(double* p, double v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
+// the same between workgroups of one landscape (other compute units, possibly other XCDs)
+__device__ __forceinline__ double flow_ld_dev(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void flow_st_dev(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 // LDS accesses (the address space is inferred after inlining)
 __device__ __forceinline__ double lds_ld(double* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -971,9 +978,12 @@ __device__ __forceinline__ int flow_take(FlowWin& W) {  // wave-uniform take ran
 __device__ __forceinline__ bool flow_aborted(FlowWin& W) {
   return __hip_atomic_load(&W.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != 0;
 }
-// one pass without progress: false (and the workgroup aborted) past the spin budget
-__device__ __forceinline__ bool flow_idle(FlowWin& W, int& spin, int* err) {
-  if (++spin > FLOW_SPIN || flow_aborted(W)) {
+// one pass without progress: false (and the workgroup aborted) past the spin budget, or (multi) once
+// another workgroup of the landscape has reported
+__device__ __forceinline__ bool flow_idle(FlowWin& W, int& spin, int* err, bool multi = false) {
+  const bool other = multi && (spin & 1023) == 1023 &&
+                     (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & FSX_ERR_LEVELS);
+  if (++spin > FLOW_SPIN || other || flow_aborted(W)) {
     if ((threadIdx.x & 63) == 0) {
       __hip_atomic_store(&W.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       atomicOr(err, FSX_ERR_LEVELS);
@@ -1011,26 +1021,43 @@ __device__ __forceinline__ void flow_load(FlowRec& r, const SplRec32* srec, long
   r.t = __longlong_as_double((long long)b.x);
 }
 
-static size_t spl_flow_smem() { return (size_t)SPL_R * (sizeof(int) + 3 * sizeof(double)); }
+// tiles of one landscape that a workgroup can mark as its own (one bit each): landscapes with more tiles
+// run on one workgroup
+#define FLOW_OWN_WORDS 2048
+static size_t spl_flow_smem() {
+  return (size_t)SPL_R * (sizeof(int) + 3 * sizeof(double)) + FLOW_OWN_WORDS * sizeof(unsigned);
+}
 
 // WG threads per landscape (the autotuner's flow_wg): fewer waves per SIMD shorten a pass of the wave
 // loop, and with it each hop of the critical receiver chain, more waves solve more nodes at once.
 // mode: 0 the step's solve, 1 lake fill only (no erosion), 2 the solve without the per-step outputs
 // (ngs, Newton count, scal; the autotuner's timing passes).
+//
+// A large landscape runs on P.flow_g workgroups (other compute units; ftab lists (landscape, rank) per
+// workgroup): waves of all of them take tiles in order from the landscape's counter in global memory
+// (fnext), each workgroup marks the tiles its waves took in an LDS bitmap, and publishes its nodes' values
+// to global memory with device-scope stores.  A lane whose receiver lies in a tile of its own workgroup
+// reads the ring as above; one whose receiver's tile another workgroup took polls global memory every
+// pass.  The progress argument holds across workgroups: tiles are taken in order, only by running waves.
+// Newton iteration counts are summed into newton_out (zeroed before the launch).
 template <int WG>
-__global__ __launch_bounds__(WG) void k_spl_flow(const InstParams* __restrict__ PP, const int* __restrict__ nlev,
+__global__ __launch_bounds__(WG) void k_spl_flow(const InstParams* __restrict__ PP, const int2* __restrict__ ftab,
+                                                 int* __restrict__ fnext, const int* __restrict__ nlev,
                                                  const SplRec32* __restrict__ srec, double* __restrict__ Hl,
                                                  double* __restrict__ Wl, double* __restrict__ HWl,
                                                  double* __restrict__ scal, int* __restrict__ ngs_out,
                                                  long long* __restrict__ newton_out, int* __restrict__ err, int mode) {
-  extern __shared__ double spl_ring[];  // h[R], water[R], lake surface[R], tag[R]
+  extern __shared__ double spl_ring[];  // h[R], water[R], lake surface[R], tag[R], own-tile bitmap
   __shared__ FlowWin W;
   __shared__ unsigned long long s_newton;
   FlowRing<SPL_R, 3> ring{spl_ring, reinterpret_cast<int*>(spl_ring + 3 * SPL_R)};
-  const int inst = blockIdx.x;
+  unsigned* own = reinterpret_cast<unsigned*>(spl_ring + 3 * SPL_R) + SPL_R;
+  const int2 wk = ftab[chunk_id()];
+  const int inst = wk.x, rank = wk.y;
   const InstParams& P = PP[inst];
   if (!P.active || !P.run_spl || !P.sfd || !P.flow) return;
   if (nlev[inst] == 0) return;  // no level order this step (receiver cycle, already reported)
+  const bool multi = P.flow_g > 1;
   const int tid = threadIdx.x, lane = tid & 63;
   const int nn = P.nn;
   const long long off = P.off;
@@ -1045,8 +1072,20 @@ __global__ __launch_bounds__(WG) void k_spl_flow(const InstParams* __restrict__ 
   // h for n == 1, ht otherwise; the same here, where the single pass starts from h = ht)
   const bool sea_skip = P.run_marine != 0;
   for (int i = tid; i < SPL_R; i += WG) ring.tag[i] = -1;
+  if (multi)
+    for (int i = tid; i < FLOW_OWN_WORDS; i += WG) own[i] = 0u;
   if (tid == 0) s_newton = 0;
   __syncthreads();
+  // next tile of the landscape for this wave (wave-uniform)
+  auto take = [&]() __attribute__((always_inline)) {
+    if (!multi) return flow_take(W);
+    int k = 0;
+    if (lane == 0) {
+      k = atomicAdd(fnext + inst, 1);
+      if (k < ntiles) atomicOr(own + (k >> 5), 1u << (k & 31));
+    }
+    return __builtin_amdgcn_readfirstlane(k);
+  };
   auto tsize = [&](int t) { return min(64, nn - 64 * t); };
   // record of position 64 t + lane, read unconditionally (clamped into the landscape; positions past the
   // end are never issued) so that the loads stay in flight until the bank is used
@@ -1067,9 +1106,15 @@ __global__ __launch_bounds__(WG) void k_spl_flow(const InstParams* __restrict__ 
   do {                                         \
     const double px_[3] = {(HV), (WV), (HW)};  \
     ring.put(p, px_);                          \
-    flow_st(Hl + off + p, px_[0]);             \
-    flow_st(Wl + off + p, px_[1]);             \
-    flow_st(HWl + off + p, px_[2]);            \
+    if (multi) {                               \
+      flow_st_dev(Hl + off + p, px_[0]);       \
+      flow_st_dev(Wl + off + p, px_[1]);       \
+      flow_st_dev(HWl + off + p, px_[2]);      \
+    } else {                                   \
+      flow_st(Hl + off + p, px_[0]);           \
+      flow_st(Wl + off + p, px_[1]);           \
+      flow_st(HWl + off + p, px_[2]);          \
+    }                                          \
     ph = 0;                                    \
   } while (0)
   // One pass of the wave with tile TC (records BQ, BF, BU, one per lane) as the current tile: issue its
@@ -1111,7 +1156,7 @@ __global__ __launch_bounds__(WG) void k_spl_flow(const InstParams* __restrict__ 
       spin = 0;                                                                            \
       ST = used_;                                                                          \
     } else {                                                                               \
-      ST = flow_idle(W, spin, err + inst) ? used_ : 2;                                     \
+      ST = flow_idle(W, spin, err + inst, multi) ? used_ : 2;                              \
     }                                                                                      \
   } while (0)
   // poll, solve and publish (the part of a pass after the issue)
@@ -1121,15 +1166,29 @@ __global__ __launch_bounds__(WG) void k_spl_flow(const InstParams* __restrict__ 
       bool ok = true;
       if (q >= 0) {
         double x[3];
-        int tg;
-        ok = ring.get(q, x, tg);
-        // global memory: once when the slot holds a later position (q's value, as a rule, left the
-        // ring), and on every FLOW_GLOBAL-th pass of the wave (all lanes at once, one wait per wave)
-        if (!ok && ((tg > q && polls == 0) || gpass)) {
-          polls = 1;
-          x[0] = flow_ld(Hl + off + q);
-          x[1] = flow_ld(Wl + off + q);
-          x[2] = flow_ld(HWl + off + q);
+        const int tq = q >> 6;
+        if (!multi || ((own[tq >> 5] >> (tq & 31)) & 1u)) {
+          int tg;
+          ok = ring.get(q, x, tg);
+          // global memory: once when the slot holds a later position (q's value, as a rule, left the
+          // ring), and on every FLOW_GLOBAL-th pass of the wave (all lanes at once, one wait per wave)
+          if (!ok && ((tg > q && polls == 0) || gpass)) {
+            polls = 1;
+            if (multi) {
+              x[0] = flow_ld_dev(Hl + off + q);
+              x[1] = flow_ld_dev(Wl + off + q);
+              x[2] = flow_ld_dev(HWl + off + q);
+            } else {
+              x[0] = flow_ld(Hl + off + q);
+              x[1] = flow_ld(Wl + off + q);
+              x[2] = flow_ld(HWl + off + q);
+            }
+            ok = !flow_sent(x[0]) && !flow_sent(x[1]) && !flow_sent(x[2]);
+          }
+        } else {  // another workgroup's tile (or one whose mark is not visible yet): global memory
+          x[0] = flow_ld_dev(Hl + off + q);
+          x[1] = flow_ld_dev(Wl + off + q);
+          x[2] = flow_ld_dev(HWl + off + q);
           ok = !flow_sent(x[0]) && !flow_sent(x[1]) && !flow_sent(x[2]);
         }
         hr = x[0];
@@ -1200,7 +1259,7 @@ __global__ __launch_bounds__(WG) void k_spl_flow(const InstParams* __restrict__ 
   };
   // two banks of records, the current tile's and the next one's, used in turn; a bank is refilled right
   // after its tile is used up, so its loads are waited for only when it is current again
-  int ta = flow_take(W), tb = flow_take(W);
+  int ta = take(), tb = take();
   int qa, qb;
   double fa, fb, ua, ub;
   load(ta, qa, fa, ua);
@@ -1209,12 +1268,12 @@ __global__ __launch_bounds__(WG) void k_spl_flow(const InstParams* __restrict__ 
     int st = 0;
     while (st == 0) SPL_PASS(ta, qa, fa, ua, st);
     if (st == 2) break;
-    ta = flow_take(W);
+    ta = take();
     load(ta, qa, fa, ua);
     st = 0;
     while (st == 0) SPL_PASS(tb, qb, fb, ub, st);
     if (st == 2) break;
-    tb = flow_take(W);
+    tb = take();
     load(tb, qb, fb, ub);
   }
 #undef SPL_PASS
@@ -1222,9 +1281,11 @@ __global__ __launch_bounds__(WG) void k_spl_flow(const InstParams* __restrict__ 
   if (my_newton) atomicAdd(&s_newton, my_newton);
   __syncthreads();
   if (tid == 0 && mode == 0) {
-    ngs_out[inst] = 1;
-    newton_out[inst] = (long long)s_newton;
-    scal[inst] = 0.0;  // g == 0: no lake sediment to spread (k_spl_post)
+    if (s_newton) atomicAdd(reinterpret_cast<unsigned long long*>(newton_out + inst), s_newton);
+    if (rank == 0) {
+      ngs_out[inst] = 1;
+      scal[inst] = 0.0;  // g == 0: no lake sediment to spread (k_spl_post)
+    }
   }
 }
 
@@ -1294,14 +1355,20 @@ void launch_spl_fact(DevState& S, hipStream_t st) {
 }
 
 void launch_spl_flow(DevState& S, hipStream_t st, int mode) {
+  if (!S.nflow) return;
   const size_t sm = spl_flow_smem();
+  launch_fill_int(S.flow_next, S.ninst, 0, st);
+  if (mode == 0) launch_fill_int(reinterpret_cast<int*>(S.newton), 2LL * S.ninst, 0, st);  // summed per workgroup
+  const int g = S.nflow;
   if (S.flow_wg == 256)
-    k_spl_flow<256><<<S.ninst, 256, sm, st>>>(S.P, S.nlev, S.srec, S.Hl, S.Wl, S.HWl, S.scal, S.ngs, S.newton, S.err, mode);
+    k_spl_flow<256><<<g, 256, sm, st>>>(S.P, S.flow_tab, S.flow_next, S.nlev, S.srec, S.Hl, S.Wl, S.HWl, S.scal, S.ngs,
+                                        S.newton, S.err, mode);
   else if (S.flow_wg == 512)
-    k_spl_flow<512><<<S.ninst, 512, sm, st>>>(S.P, S.nlev, S.srec, S.Hl, S.Wl, S.HWl, S.scal, S.ngs, S.newton, S.err, mode);
+    k_spl_flow<512><<<g, 512, sm, st>>>(S.P, S.flow_tab, S.flow_next, S.nlev, S.srec, S.Hl, S.Wl, S.HWl, S.scal, S.ngs,
+                                        S.newton, S.err, mode);
   else
-    k_spl_flow<1024><<<S.ninst, 1024, sm, st>>>(S.P, S.nlev, S.srec, S.Hl, S.Wl, S.HWl, S.scal, S.ngs, S.newton, S.err,
-                                                mode);
+    k_spl_flow<1024><<<g, 1024, sm, st>>>(S.P, S.flow_tab, S.flow_next, S.nlev, S.srec, S.Hl, S.Wl, S.HWl, S.scal,
+                                          S.ngs, S.newton, S.err, mode);
 }
 
 static void spl_sweep_launch(DevState& S, hipStream_t st, int fill_only) {

@@ -239,21 +239,22 @@ int fsx_reset_timers(fsx_ctx *ctx);
  * change) under candidate launch shapes -- basin-walk blocks per landscape, the pointer-jumping grid
  * cap, Euler-tour blocks per landscape, edge-sort blocks per landscape, drainage-area sweep threads
  * per landscape, spanning-forest edge-scan workgroups per landscape -- and, when some landscape takes
- * the dataflow stream-power sweep (single flow direction, g = 0), that sweep's threads per landscape
- * (timed on solves of the routed state that leave h and the per-step counts alone), with reps timed
+ * the dataflow stream-power sweep (single flow direction, g = 0), that sweep's threads per workgroup and
+ * nodes per workgroup (timed on solves of the routed state that leave h and the per-step counts alone), with reps timed
  * passes per candidate, and keeps the fastest of each.  No result depends on these shapes.  Routing
  * read-outs (receivers, donors, area) then hold the tuning passes' routing of the current elevation
  * until the next step.  chosen (FSX_NTUNE ints, or NULL) receives walk_gx, jump_blocks, tour_gx,
- * nr_small_gx, area_wg, bor_gx, flow_wg.  No reference counterpart: the reference has no launch
- * shapes. */
-#define FSX_NTUNE 7
+ * nr_small_gx, area_wg, bor_gx, flow_wg, flow_split.  No reference counterpart: the reference has no
+ * launch shapes. */
+#define FSX_NTUNE 8
 int fsx_autotune(fsx_ctx *ctx, int reps, int *chosen);
 /* The launch shapes in use (FSX_NTUNE ints, same order as fsx_autotune's chosen). */
 int fsx_get_tuning(fsx_ctx *ctx, int *out);
 /* Set the launch shapes (FSX_NTUNE ints, same order; e.g. ones an earlier fsx_autotune chose for
    this workload).  Each must be >= 1; walk blocks <= 4096, jump grid <= 2^20, tour blocks <= 1024,
    sort blocks <= 4096; area_wg one of 256, 512, 1024; bor_gx in [1, 1024]; flow_wg one of 256, 512,
-   1024.  A rejected call changes nothing.
+   1024; flow_split >= 0 (nodes per workgroup of the dataflow sweep, 0: one workgroup per landscape; at
+   most 8 workgroups per landscape).  A rejected call changes nothing.
    No counterpart in the reference. */
 int fsx_set_tuning(fsx_ctx *ctx, const int *in);
 

@@ -11,7 +11,7 @@ pytestmark = pytest.mark.gpu
 
 CANDIDATES = {"walk_gx": {64, 128, 256, 512}, "jump_blocks": {1024, 2048, 4096, 8192}, "tour_gx": {8, 16, 32},
               "nr_small_gx": {16, 32, 64, 128}, "area_wg": {256, 512, 1024}, "bor_gx": {2, 4, 8, 16},
-              "flow_wg": {256, 512, 1024}}
+              "flow_wg": {256, 512, 1024}, "flow_split": {0, 262144, 131072, 65536}}
 
 
 def test_autotune_changes_shapes_not_results():
@@ -53,7 +53,7 @@ def test_set_tuning_roundtrip_and_bounds():
         a.step(1)
         b.step(1)  # b's default-shape graphs are captured here
         shapes = {"walk_gx": 64, "jump_blocks": 1024, "tour_gx": 32, "nr_small_gx": 16, "area_wg": 256,
-                  "bor_gx": 1, "flow_wg": 256}
+                  "bor_gx": 1, "flow_wg": 256, "flow_split": 1000}
         b.set_tuning(shapes)
         assert b.tuning() == shapes
         with pytest.raises(_lib.FastscapeError):
