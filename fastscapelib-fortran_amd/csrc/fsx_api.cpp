@@ -53,8 +53,11 @@ struct fsx_ctx {
   std::map<long long, long long> free_ranges;  // free node ranges of the arena (offset -> length)
   long long capacity = 0, chunk_cap = 0, tile_cap = 0, flow_cap = 0;
   std::vector<int2> flow_host;  // the dataflow sweep's workgroup table as uploaded
+  std::vector<int2> mfd_host;   // the multiple-flow sweeps' workgroup table as uploaded
+  long long mfd_cap = 0;
   std::vector<void*> allocs;
   hipStream_t stream = nullptr;
+  hipStream_t last = nullptr;  // the caller's stream of the last fsx_execute_steps that was given one
   bool dirty = true;
   bool any_sfd = false, any_mfd = false;  // which flow-routing paths the active instances use
   bool any_flow = false;                  // some active instance takes the dataflow stream-power sweep
@@ -83,9 +86,19 @@ struct fsx_ctx {
 };
 static const int kRing = 64;
 
+// Waits for every launch of the context: its own stream and the caller's stream that the last steps were
+// enqueued on.  Done before captured graphs, tables or buffers that queued launches may still read are
+// destroyed, freed or rewritten.
+static hipError_t quiesce(fsx_ctx* c) {
+  hipError_t e = c->stream ? hipStreamSynchronize(c->stream) : hipSuccess;
+  if (e == hipSuccess && c->last && c->last != c->stream) e = hipStreamSynchronize(c->last);
+  return e;
+}
+
 static int flush_timers_any(fsx_ctx* c, hipStream_t st) {
   if (c->ring_used == 0) return 0;
   HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(quiesce(c));
   for (int s = 0; s < c->ring_used; s++)
     for (int p = 0; p < PH_COUNT; p++) {
       const int k = s * PH_COUNT + p;
@@ -187,6 +200,9 @@ static int check_inst(fsx_ctx* c, int inst) {
 
 static int build_flow_table(fsx_ctx* c);
 
+static int build_flow_table(fsx_ctx* c);
+static int build_mfd_table(fsx_ctx* c);
+
 static int upload_params(fsx_ctx* c) {
   if (!c->dirty) return 0;
   c->any_sfd = c->any_mfd = c->any_flow = false;
@@ -203,6 +219,7 @@ static int upload_params(fsx_ctx* c) {
     if (P.flow) c->any_flow = true;
   }
   if (int r = build_flow_table(c)) return r;
+  if (int r = build_mfd_table(c)) return r;
   HIPCHK(hipMemcpyAsync(c->S.P, c->hp.data(), sizeof(InstParams) * c->hp.size(), hipMemcpyHostToDevice, c->stream));
   c->dirty = false;
   return 0;
@@ -302,7 +319,7 @@ static int build_flow_table(fsx_ctx* c) {
                                [](const int2& a, const int2& b) { return a.x == b.x && a.y == b.y; });
   if (same) return 0;
   if ((long long)tab.size() > c->flow_cap) {
-    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(quiesce(c));
     if (S.flow_tab) HIPCHK(hipFree(S.flow_tab));
     S.flow_tab = nullptr;
     c->flow_cap = 0;
@@ -312,10 +329,60 @@ static int build_flow_table(fsx_ctx* c) {
     c->flow_cap = cap;
   }
   if (!tab.empty()) {
+    HIPCHK(quiesce(c));  // launches queued on the caller's stream may still read the old table
     HIPCHK(hipMemcpyAsync(S.flow_tab, tab.data(), sizeof(int2) * tab.size(), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
   }
   c->flow_host = std::move(tab);
+  c->graph_valid = false;
+  return 0;
+}
+
+// Workgroups of the multiple-flow dataflow sweeps: one per nodes_per (mfd_split) nodes of each landscape
+// that routes by multiple flow directions, at most kMfdGMax (one XCD's compute units); the extra ones are
+// dealt to the largest landscapes first while the device has compute units for them.  A landscape's
+// workgroups are consecutive in the table (one XCD).  No result depends on the count.
+static const int kMfdGMax = 32;
+static int build_mfd_table(fsx_ctx* c) {
+  DevState& S = c->S;
+  std::vector<int> idx;
+  for (int k = 0; k < S.ninst; k++) {
+    InstParams& P = c->hp[k];
+    P.mfd_g = (c->used[k] && P.active && P.run_spl && !P.sfd) ? 1 : 0;
+    if (P.mfd_g) idx.push_back(k);
+  }
+  std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return c->hp[a].nn > c->hp[b].nn; });
+  long long extra = std::max(0LL, (long long)cu_count() - (long long)idx.size());
+  for (int k : idx) {
+    InstParams& P = c->hp[k];
+    const long long want = std::min<long long>(kMfdGMax, std::max(1LL, ((long long)P.nn + S.mfd_split - 1) / S.mfd_split));
+    const long long add = std::min(want - 1, extra);
+    P.mfd_g += (int)add;
+    extra -= add;
+  }
+  std::vector<int2> tab;
+  for (int k = 0; k < S.ninst; k++)
+    for (int r = 0; r < c->hp[k].mfd_g; r++) tab.push_back(make_int2(k, r));
+  S.nmfd = (int)tab.size();
+  const bool same = tab.size() == c->mfd_host.size() &&
+                    std::equal(tab.begin(), tab.end(), c->mfd_host.begin(),
+                               [](const int2& a, const int2& b) { return a.x == b.x && a.y == b.y; });
+  if (same) return 0;
+  HIPCHK(quiesce(c));  // queued launches may still read the old table
+  if ((long long)tab.size() > c->mfd_cap) {
+    if (S.mfd_tab) HIPCHK(hipFree(S.mfd_tab));
+    S.mfd_tab = nullptr;
+    c->mfd_cap = 0;
+    const long long cap = (long long)tab.size() + 64;
+    hipError_t e = hipMalloc(&S.mfd_tab, sizeof(int2) * cap);
+    if (e != hipSuccess) return fail(FSX_ENOMEM, std::string("hipMalloc (mfd table): ") + hipGetErrorString(e));
+    c->mfd_cap = cap;
+  }
+  if (!tab.empty()) {
+    HIPCHK(hipMemcpyAsync(S.mfd_tab, tab.data(), sizeof(int2) * tab.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  c->mfd_host = std::move(tab);
   c->graph_valid = false;
   return 0;
 }
@@ -352,7 +419,7 @@ static int relayout(fsx_ctx* c) {
       for (int x0 = 0; x0 < P.nx; x0 += FSX_TW) tiles.push_back(make_int4(k, x0, y0, 0));
   }
   if ((long long)tiles.size() > c->tile_cap) {
-    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(quiesce(c));
     if (S.tiles) HIPCHK(hipFree(S.tiles));
     S.tiles = nullptr;
     c->tile_cap = 0;
@@ -362,6 +429,7 @@ static int relayout(fsx_ctx* c) {
     c->tile_cap = cap;
   }
   S.ntiles = (int)tiles.size();
+  HIPCHK(quiesce(c));  // the chunk and tile tables are rewritten in place
   if (!tiles.empty())
     HIPCHK(hipMemcpyAsync(S.tiles, tiles.data(), sizeof(int4) * tiles.size(), hipMemcpyHostToDevice, c->stream));
   if (!chunks.empty())
@@ -514,13 +582,13 @@ extern "C" int fsx_create_pool(long long capacity_nodes, int nslots, fsx_ctx** o
 
 extern "C" int fsx_add_instance(fsx_ctx* c, int nx, int ny, int* inst) {
   if (!c || !inst) return fail(FSX_EINVAL, "fsx_add_instance: bad arguments");
-  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(quiesce(c));
   return slot_add(c, nx, ny, inst, true);
 }
 
 extern "C" int fsx_remove_instance(fsx_ctx* c, int inst) {
   if (int r = check_inst(c, inst)) return r;
-  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(quiesce(c));
   InstParams& P = c->hp[inst];
   arena_give(c, P.off, P.nn);
   memset(&P, 0, sizeof P);
@@ -550,7 +618,7 @@ extern "C" int fsx_pool_info(fsx_ctx* c, long long* free_nodes, long long* large
 
 extern "C" int fsx_destroy(fsx_ctx* c) {
   if (!c) return 0;
-  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  (void)quiesce(c);
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   if (c->gpre) (void)hipGraphExecDestroy(c->gpre);
   if (c->gpost) (void)hipGraphExecDestroy(c->gpost);
@@ -558,6 +626,7 @@ extern "C" int fsx_destroy(fsx_ctx* c) {
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->S.tiles) (void)hipFree(c->S.tiles);
   if (c->S.flow_tab) (void)hipFree(c->S.flow_tab);
+  if (c->S.mfd_tab) (void)hipFree(c->S.mfd_tab);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return 0;
@@ -598,7 +667,9 @@ static int mfd_alloc(fsx_ctx* c) {
   c->graph_valid = false;
 #define A(p, n) if (!rc) rc = dalloc(c, &S.p, (n))
   A(mrec, 8 * N); A(mdon, 8 * N); A(mw, 8 * N); A(mfact, 8 * N); A(mdir, 8 * N); A(mflag, N);
-  A(mnrec, N); A(mndon, N); A(mindeg, N); A(mstack, N); A(mls, N + S.ninst); A(mnlev, S.ninst);
+  A(mnrec, N); A(mndon, N); A(mstack, N); A(mls, N + S.ninst); A(mnlev, S.ninst);
+  A(mkahn, N); A(mq, N); A(mcur, N + S.ninst); A(mctr, FSX_MFD_CTR * S.ninst); A(mpart, c->chunk_cap);
+  A(mdec, 100 * S.ninst);
   A(mpos, N); A(mdp, 8 * N); A(mdw, 8 * N);
 #undef A
   return rc;
@@ -935,6 +1006,10 @@ static int capture(fsx_ctx* c, hipStream_t st, int lo, int hi, hipGraphExec_t* o
 
 // One graph for the whole step, or (a phase timed live) the graphs before and after that phase.
 static int build_graphs(fsx_ctx* c, hipStream_t st) {
+  // an executable graph is destroyed only once no launch of it is queued (the last steps may have been
+  // enqueued on another stream without a wait)
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(quiesce(c));
   for (hipGraphExec_t* g : {&c->gexec, &c->gpre, &c->gpost})
     if (*g) {
       HIPCHK(hipGraphExecDestroy(*g));
@@ -979,13 +1054,20 @@ extern "C" int fsx_execute_steps(fsx_ctx* c, int nsteps, void* stream, int sync)
   if (!c) return fail(FSX_EINVAL, "null context");
   if (nsteps < 0) return fail(FSX_EINVAL, "nsteps < 0");
   hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+  // nothing to step in a context without landscapes (no launch may have an empty grid)
+  bool any = false;
+  for (int k = 0; k < c->S.ninst && !any; k++) any = c->used[k] != 0;
+  if (!any) return 0;
   // Marine reads the step's single-flow routing (rec, stack; src/Marine.f90:30-36,60-61)
   for (int k = 0; k < c->S.ninst; k++)
     if (c->used[k] && c->hp[k].active && c->hp[k].run_marine && !c->hp[k].run_spl)
       return fail(FSX_EINVAL, "marine transport needs the flow routing: set the erosional parameters first");
+  // the parameter table is rewritten in place: launches still queued on either stream read it
+  if (c->dirty && c->last && c->last != c->stream) HIPCHK(hipStreamSynchronize(c->last));
   if (stream && c->dirty) HIPCHK(hipStreamSynchronize(c->stream));
   if (int r = upload_params(c)) return r;
   if (stream) HIPCHK(hipStreamSynchronize(c->stream));
+  if (stream) c->last = st;
   if (c->use_graph && !c->prof && nsteps > 0) {
     // the step's launch sequence depends on the layout and on which routing paths are in use; the
     // kernels read every other parameter from device memory, so one capture serves until those change

@@ -18,6 +18,7 @@
 #define FSX_WG 1024      // threads of the per-landscape workgroups
 #define FSX_TW 64        // 2-D tiles of the stencil kernels: columns
 #define FSX_TH 8         //   rows
+#define FSX_MFD_CTR 16   // counters per landscape of the multiple-flow dataflow launches
 
 // device-side error bits, one word per instance
 enum {
@@ -42,6 +43,7 @@ struct InstParams {
   int flow;                  // single flow without deposition (g == 0 everywhere): one stream-power pass,
                              // solved node by node as receivers complete (k_spl_flow)
   int flow_g;                // workgroups of k_spl_flow on this landscape
+  int mfd_g;                 // workgroups of the multiple-flow dataflow sweeps on this landscape
   int run_marine;            // marine transport after Diffusion (SetMarineParam)
   int run_strati, nref, iref;  // stratigraphy (Activate_Strati): reflectors, the active one (1-based)
   int aux;                   // offset of the nn+1-entry per-landscape arrays (level starts, groups)
@@ -219,6 +221,45 @@ __device__ __forceinline__ int chunk_id() {
   const int q = n / FSX_NXCD, r = n % FSX_NXCD;
   return x * q + (x < r ? x : r) + k;
 }
+
+// ---- values passed between waves inside one launch ------------------------------------------------
+// A value of a dataflow sweep that is not written yet: a NaN whose payload no arithmetic produces from
+// the finite (or canonical NaN) values the sweeps compute.
+#define FLOW_SENT 0xFFF8DEADBEEF0001ull
+__device__ __forceinline__ bool flow_sent(double x) {
+  return (unsigned long long)__double_as_longlong(x) == FLOW_SENT;
+}
+__device__ __forceinline__ double flow_sentinel() { return __longlong_as_double((long long)FLOW_SENT); }
+// between waves of one workgroup (one compute unit)
+__device__ __forceinline__ double flow_ld(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void flow_st(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+// between workgroups of one landscape (other compute units, possibly other XCDs): device-scope loads
+// and stores of one location are coherent, so a value read differs from the sentinel only once written
+__device__ __forceinline__ double flow_ld_dev(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void flow_st_dev(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int flow_ldi_dev(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void flow_sti_dev(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long flow_ldu_dev(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void flow_stu_dev(unsigned long long* p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Waits until every memory access this wave issued has completed: device-scope stores are then visible
+// to device-scope loads of any compute unit, so a counter incremented after this publishes them.
+__device__ __forceinline__ void wave_drain() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
 // order-preserving 64-bit image of a double (for atomic min/max on doubles); 0 is below every image
 __device__ __forceinline__ unsigned long long dkey(double x) {

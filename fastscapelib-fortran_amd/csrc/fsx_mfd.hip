@@ -6,16 +6,17 @@
 // lower neighbours of h0 as receivers with slope**p weights, orders the nodes with a LIFO Kahn
 // traversal (mstack) and walks that order serially.
 //
-// Here the receiver graph is a DAG whose Kahn layers replace mstack: layer 0 holds the nodes
-// without donors, layer l+1 the nodes whose last donor sits in layer l.  Every receiver of a node
-// lies in a later layer, and so does its single-flow receiver (a "ghost" edge is added in the rare
-// case where that receiver is not also a multiple-flow receiver), so
-//  * accumulation and the erosion push run layer by layer from layer 0 (each node pulls from its
-//    donors), and
-//  * the implicit solve runs from the last layer back (each node reads its receivers).
-// Sums over donors are taken in a fixed order (neighbour direction, then receiver slot), so the
-// results do not depend on the batch or on the order of nodes inside a layer; they differ from the
-// reference's mstack order only in the rounding of those sums.
+// Here the nodes are ordered by the Kahn layers of the receiver graph in place of mstack: layer 0 holds
+// the nodes without donors, layer l+1 the nodes whose last donor sits in layer l.  Every receiver of a
+// node lies in a later layer, and so does its single-flow receiver (a "ghost" edge is added in the rare
+// case where that receiver is not also a multiple-flow receiver).  Sums over donors are taken in a fixed
+// order (neighbour direction, then receiver slot), so the results do not depend on the order of nodes
+// inside a layer; they differ from the reference's mstack order only in the rounding of those sums.
+//
+// The reference's serial walks -- the fill, the Kahn traversal, the accumulation, the erosion push and the
+// implicit solve -- run as dataflow sweeps on several workgroups per landscape (below): a position is
+// computed as soon as the values it depends on are written, with no barrier per layer, so that a large
+// landscape spreads over many compute units and its time follows its longest chain of dependencies.
 //
 // Per-node edge records are 8 slots in the reference's receiver order (scan order jj outer, ii
 // inner, src/FlowRouting.f90:204-230): target (local index, -1 unused), weight, direction.
@@ -29,36 +30,169 @@
   if (!P.active || !P.run_spl || P.sfd || v >= P.nn) return; \
   const long long g_ = (long long)P.off + v;
 
-static __device__ __forceinline__ void mfd_barrier() { __syncthreads(); }
+// ---- dataflow sweeps on several workgroups per landscape -----------------------------------------------
+// A landscape's sweeps run on P.mfd_g workgroups (the table mtab lists (landscape, rank); a landscape's
+// entries are consecutive, so its workgroups share an XCD).  Work comes in tiles that waves take in order
+// from a counter of the landscape (ctr[0]); a wave that leaves a phase adds the tiles it finished to
+// ctr[1].  Tiles of the successive phases of a launch are numbered consecutively, and a wave starts a
+// tile of a phase once ctr[1] shows every tile of the earlier phases finished.  Inside a dataflow phase a
+// lane computes its position once the values it depends on -- earlier positions in the phase's order --
+// differ from the sentinel.  Progress does not need the workgroups to be resident together: tiles are
+// taken in order by running waves, and the lowest unfinished position depends only on finished ones.
+// Values that one wave writes and another reads are device-scope loads and stores, and a wave drains its
+// stores before it counts its tiles.  A wait of more than MFD_SPIN polls reports FSX_ERR_MFD (it cannot
+// happen with a valid order) and the landscape's waves stop.
+#define MFD_SPIN (1 << 22)
+#define MFD_WG 512   // threads per workgroup of the dataflow kernels
+#define MFD_MAXGS 100  // Gauss-Seidel decisions kept per landscape (the loop stops at 99)
+#define MFD_EB 4096  // positions per partial sum of the Gauss-Seidel error (a fixed blocking of the sum)
+
+struct MfdTiles {
+  int* claim;  // ctr[0]: next tile to take
+  int* done;   // ctr[1]: tiles finished
+  int* err;    // the landscape's error word
+  int base;    // first tile of the current phase
+  int t;       // the tile this wave holds
+  bool ok;     // false once the landscape reported a stall
+};
+
+__device__ __forceinline__ int mfd_claim(int* claim) {
+  int k = 0;
+  if ((threadIdx.x & 63) == 0) k = __hip_atomic_fetch_add(claim, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __builtin_amdgcn_readfirstlane(k);
+}
+
+// one more poll without progress: false past the budget (reported) or once the landscape reported a stall
+__device__ __forceinline__ bool mfd_idle(int& spin, int* err) {
+  if (++spin > MFD_SPIN) {
+    if ((threadIdx.x & 63) == 0) atomicOr(err, FSX_ERR_MFD);
+    return false;
+  }
+  if ((spin & 255) == 0 && (flow_ldi_dev(err) & FSX_ERR_MFD)) return false;
+  __builtin_amdgcn_s_sleep(1);
+  return true;
+}
+
+__device__ __forceinline__ void mfd_start(MfdTiles& T, int* ctr, int* err) {
+  T.claim = ctr;
+  T.done = ctr + 1;
+  T.err = err;
+  T.base = 0;
+  T.ok = true;
+  T.t = mfd_claim(T.claim);
+}
+
+// waits until `need` tiles are finished
+__device__ __forceinline__ bool mfd_wait(MfdTiles& T, int need) {
+  int spin = 0;
+  while (T.ok && flow_ldi_dev(T.done) < need)
+    if (!mfd_idle(spin, T.err)) T.ok = false;
+  return T.ok;
+}
+
+// The next phase, of n tiles: this wave runs body(tile) on the tiles it takes.  A body that cannot finish
+// sets T.ok = false.
+template <class F>
+__device__ __forceinline__ void mfd_phase(MfdTiles& T, int n, F&& body) {
+  const int end = T.base + n;
+  int nd = 0;
+  if (T.ok && T.t < end && mfd_wait(T, T.base)) {
+    while (T.t < end) {
+      body(T.t - T.base);
+      if (!T.ok) break;
+      nd++;
+      T.t = mfd_claim(T.claim);
+    }
+  }
+  if (nd) {
+    wave_drain();
+    if ((threadIdx.x & 63) == 0) __hip_atomic_fetch_add(T.done, nd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  T.base = end;
+}
+
+// Loop of a dataflow tile: step(prog) tries the lane's position (sets prog when it finished it) and
+// returns whether the lane still has work; the wave polls until every lane is done.
+template <class F>
+__device__ __forceinline__ void mfd_lanes(MfdTiles& T, F&& step) {
+  int spin = 0;
+  while (true) {
+    bool prog = false;
+    const bool more = step(prog);
+    if (__ballot(more) == 0ull) return;
+    if (__ballot(prog) != 0ull) spin = 0;
+    else if (!mfd_idle(spin, T.err)) {
+      T.ok = false;
+      return;
+    }
+  }
+}
+
+#define MFD_WAVE_PROLOGUE                         \
+  const int2 wk = mtab[chunk_id()];               \
+  const int inst = wk.x;                          \
+  const InstParams& P = PP[inst];                 \
+  if (!P.active || !P.run_spl || P.sfd) return;   \
+  const int nn = P.nn, lane = threadIdx.x & 63;   \
+  const long long off = P.off;
+
+// ---- set-up -----------------------------------------------------------------------------------------
+// Kahn state (layer so far << 32 | in-edges left), empty queue slots, the layer histogram, the fill's
+// sentinel.
+__global__ __launch_bounds__(FSX_CHUNK) void k_mfd_init(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
+                                                        unsigned long long* __restrict__ mkahn,
+                                                        unsigned long long* __restrict__ mq, int* __restrict__ mls,
+                                                        double* __restrict__ hwater) {
+  MFD_PROLOGUE
+  mkahn[g_] = 0ull;
+  mq[g_] = ~0ull;
+  hwater[g_] = flow_sentinel();
+  mls[P.aux + v] = 0;
+  if (v == 0) mls[P.aux + P.nn] = 0;
+}
 
 // ---- lake surface along the single-flow forest (find_mult_rec :187-196) ------------------------
 // h0 = h, then along the stack h0(ijk) = h0(rec)+1e-8 where h0(rec) > h0(ijk); lake_depth = h0 - h
-// (FlowRouting :34).  Levels of the single-flow forest from base level up.
-__global__ __launch_bounds__(FSX_WG) void k_mfd_fill(const InstParams* __restrict__ PP, const int* __restrict__ lstart,
+// (FlowRouting :34).  A dataflow sweep over the single-flow level order from base level up: a node waits
+// for its receiver's h0.
+__global__ __launch_bounds__(MFD_WG) void k_mfd_fill(const InstParams* __restrict__ PP, const int2* __restrict__ mtab,
+                                                     int* __restrict__ ctr, int* __restrict__ err,
                                                      const int* __restrict__ nlev, const int* __restrict__ order,
                                                      const int* __restrict__ rec, const double* __restrict__ h,
                                                      double* __restrict__ hwater, double* __restrict__ lake_depth) {
-  const int inst = blockIdx.x;
-  const InstParams& P = PP[inst];
-  if (!P.active || !P.run_spl || P.sfd) return;
-  const long long off = P.off;
-  const int* ls = lstart + P.aux;
-  const int L = nlev[inst];
-  for (int l = 0; l < L; l++) {
-    const int b0 = ls[l], b1 = ls[l + 1];
-    for (int p = b0 + (int)threadIdx.x; p < b1; p += FSX_WG) {
-      const int v = order[off + p];
-      const double hv = h[off + v];
-      double hw = hv;
-      if (l > 0) {
-        const double hr = hwater[off + rec[off + v]];
-        if (hr > hv) hw = hr + 1.0e-8;
-      }
-      hwater[off + v] = hw;
-      lake_depth[off + v] = hw - hv;
+  MFD_WAVE_PROLOGUE
+  if (nlev[inst] == 0) return;  // no level order this step (receiver cycle, already reported)
+  MfdTiles T;
+  mfd_start(T, ctr + FSX_MFD_CTR * inst, err + inst);
+  mfd_phase(T, (nn + 63) / 64, [&](int tt) {
+    const int p = 64 * tt + lane;
+    bool todo = p < nn;
+    int v = 0, r = 0;
+    double hv = 0.0;
+    if (todo) {
+      v = order[off + p];
+      r = rec[off + v];
+      hv = h[off + v];
     }
-    mfd_barrier();
-  }
+    mfd_lanes(T, [&](bool& prog) {
+      if (todo) {
+        double hw = hv;
+        bool ready = true;
+        if (r != v) {
+          const double hr = flow_ld_dev(hwater + off + r);
+          ready = !flow_sent(hr);
+          if (hr > hv) hw = hr + 1.0e-8;
+        }
+        if (ready) {
+          flow_st_dev(hwater + off + v, hw);
+          lake_depth[off + v] = hw - hv;
+          todo = false;
+          prog = true;
+        }
+      }
+      return todo;
+    });
+  });
 }
 
 // ---- receivers and weights (find_mult_rec :204-243) ---------------------------------------------
@@ -66,12 +200,13 @@ __global__ __launch_bounds__(FSX_WG) void k_mfd_fill(const InstParams* __restric
 // direction that lands on another node repeats that node, as the reference does).  Weights
 // slope**pexp normalised by their sum; pexp = p, or 0.5 + 0.6*mean slope when p < 0 with the mean
 // rounded through single precision and 0.6 a single-precision literal.  In-degrees (for the Kahn
-// layers) count every slot, plus the ghost edge to the single-flow receiver when needed.
+// layers, the low half of mkahn) count every slot, plus the ghost edge to the single-flow receiver when
+// needed.
 __global__ __launch_bounds__(FSX_CHUNK) void k_mfd_rec(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
                                                        const double* __restrict__ h0, const int* __restrict__ rec,
                                                        int* __restrict__ mrec, double* __restrict__ mw,
                                                        uint8_t* __restrict__ mdir, int* __restrict__ mnrec,
-                                                       uint8_t* __restrict__ mflag, int* __restrict__ mindeg,
+                                                       uint8_t* __restrict__ mflag, unsigned long long* __restrict__ mkahn,
                                                        double* __restrict__ pexp) {
   MFD_PROLOGUE
   const double* hi = h0 + P.off;
@@ -118,13 +253,13 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_mfd_rec(const InstParams* __restr
     mdir[8 * g_ + k] = on ? (uint8_t)dd[k] : (uint8_t)4;
     if (on) {
       hit |= t[k] == r;
-      atomicAdd(mindeg + P.off + t[k], 1);
+      atomicAdd(mkahn + P.off + t[k], 1ull);
     }
   }
   mnrec[g_] = nr;
   const bool ghost = !hit && r != v;
   mflag[g_] = ghost ? 1 : 0;
-  if (ghost) atomicAdd(mindeg + P.off + r, 1);
+  if (ghost) atomicAdd(mkahn + P.off + r, 1ull);
 }
 
 // ---- donors: the slots of neighbouring nodes that point here ------------------------------------
@@ -156,77 +291,188 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_mfd_don(const InstParams* __restr
   mndon[g_] = nd;
 }
 
-// ---- Kahn layers (one workgroup per landscape) -----------------------------------------------------
-// The layer of a node is final when its last incoming edge is removed; positions inside a layer
-// come from an LDS counter, so they vary from run to run while the layer sets do not.
-__global__ __launch_bounds__(FSX_WG) void k_mfd_levels(const InstParams* __restrict__ PP, int* __restrict__ mindeg,
-                                                       const int* __restrict__ mrec, const int* __restrict__ mnrec,
-                                                       const uint8_t* __restrict__ mflag, const int* __restrict__ rec,
-                                                       int* __restrict__ morder, int* __restrict__ mls,
-                                                       int* __restrict__ mnlev, int* __restrict__ err) {
-  __shared__ int s_tail;
+// ---- Kahn layers (find_mult_rec :245-292) -------------------------------------------------------------
+// The traversal as a dataflow over a queue per landscape: a node keeps (largest layer of its donors + 1)
+// << 32 | in-edges left, updated by compare-and-swap; the donor that removes its last in-edge appends it
+// to the queue with its layer and counts it in that layer's histogram.  Phase 0 queues the nodes without
+// in-edges (layer 0); in phase 1 a lane takes queue position p, waits until it is written and releases
+// the node's receivers.  The layer sets are the reference's Kahn layers; the order inside a layer varies
+// from run to run.
+__global__ __launch_bounds__(MFD_WG) void k_mfd_kahn(const InstParams* __restrict__ PP, const int2* __restrict__ mtab,
+                                                     int* __restrict__ ctr, int* __restrict__ err,
+                                                     unsigned long long* __restrict__ mkahn, const int* __restrict__ mrec,
+                                                     const int* __restrict__ mnrec, const uint8_t* __restrict__ mflag,
+                                                     const int* __restrict__ rec, unsigned long long* __restrict__ mq,
+                                                     int* __restrict__ mls) {
+  MFD_WAVE_PROLOGUE
+  int* cq = ctr + FSX_MFD_CTR * inst + 4;  // claim, done, queue tail, largest layer
+  int* hist = mls + P.aux;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  MfdTiles T;
+  mfd_start(T, cq, err + inst);
+  // the lanes with `pushme` append node u of layer lay: one queue reservation per wave, one histogram
+  // count per distinct layer, one maximum
+  auto push = [&](bool pushme, int u, unsigned lay) {
+    const unsigned long long m = __ballot(pushme);
+    if (m == 0ull) return;
+    const int first = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == first) base = __hip_atomic_fetch_add(cq + 2, __popcll(m), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    base = __shfl(base, first, 64);
+    if (pushme) {
+      const int idx = base + __popcll(m & below);
+      if (idx < nn) flow_stu_dev(mq + off + idx, ((unsigned long long)lay << 32) | (unsigned long long)(unsigned)u);
+    }
+    unsigned top = pushme ? lay : 0u;
+    for (int o = 32; o > 0; o >>= 1) {
+      const unsigned y = (unsigned)__shfl_xor((int)top, o, 64);
+      top = y > top ? y : top;
+    }
+    if (lane == first) atomicMax(cq + 3, (int)top);
+    unsigned long long rem = m;
+    while (rem) {
+      const int ld = __ffsll((long long)rem) - 1;
+      const unsigned l0 = (unsigned)__shfl((int)lay, ld, 64);
+      const unsigned long long same = __ballot(pushme && lay == l0) & rem;
+      if (lane == ld) atomicAdd(hist + l0, __popcll(same));
+      rem &= ~same;
+    }
+  };
+  mfd_phase(T, (nn + 255) / 256, [&](int tt) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int u = 256 * tt + 64 * k + lane;
+      push(u < nn && (unsigned)mkahn[off + (u < nn ? u : 0)] == 0u, u, 0u);
+    }
+  });
+  mfd_phase(T, (nn + 63) / 64, [&](int tt) {
+    const int p = 64 * tt + lane;
+    bool todo = p < nn;
+    mfd_lanes(T, [&](bool& prog) {
+      bool have = false;
+      int u = 0, nr = -1;
+      unsigned lay = 0;
+      if (todo) {
+        const unsigned long long e = flow_ldu_dev(mq + off + p);
+        if (e != ~0ull) {
+          todo = false;
+          prog = true;
+          have = true;
+          u = (int)(unsigned)(e & 0xffffffffull);
+          lay = (unsigned)(e >> 32);
+          nr = mnrec[off + u];
+          if (mflag[off + u]) nr++;  // the ghost edge to the single-flow receiver, last
+        }
+      }
+      // receiver slots in step over the wave, so that the nodes they release are appended together
+      for (int k = 0; __ballot(have && k < nr) != 0ull; k++) {
+        bool rel = false;
+        int t = 0;
+        unsigned nl = 0;
+        if (have && k < nr) {
+          const long long gu = off + u;
+          t = (k < mnrec[gu]) ? mrec[8 * gu + k] : rec[gu];
+          unsigned long long* s = mkahn + off + t;
+          unsigned long long old = flow_ldu_dev(s), seen, nw;
+          do {
+            seen = old;
+            const unsigned hl = (unsigned)(seen >> 32), want = lay + 1u;
+            nw = ((unsigned long long)(hl > want ? hl : want) << 32) |
+                 (unsigned long long)((unsigned)(seen & 0xffffffffull) - 1u);
+            old = atomicCAS(s, seen, nw);
+          } while (old != seen);
+          rel = (unsigned)(seen & 0xffffffffull) == 1u;
+          nl = (unsigned)(nw >> 32);
+        }
+        push(rel, t, nl);
+      }
+      return todo;
+    });
+  });
+}
+
+// Layer starts from the histogram (exclusive scan, one workgroup per landscape) and the scatter cursors;
+// an order that does not hold every node is the reference's 'error in stack' (src/FlowRouting.f90:292).
+__global__ __launch_bounds__(FSX_WG) void k_mfd_lscan(const InstParams* __restrict__ PP, const int* __restrict__ ctr,
+                                                      int* __restrict__ mls, int* __restrict__ mcur,
+                                                      int* __restrict__ mnlev, int* __restrict__ err) {
+  __shared__ int lds[FSX_WG / 64];
   const int inst = blockIdx.x;
   const InstParams& P = PP[inst];
   if (!P.active || !P.run_spl || P.sfd) return;
-  const int nn = P.nn, tid = threadIdx.x;
-  const long long off = P.off;
-  int* ls = mls + P.aux;  // nn+1 entries
-  if (tid == 0) s_tail = 0;
-  __syncthreads();
-  for (int v = tid; v < nn; v += FSX_WG)
-    if (mindeg[off + v] == 0) morder[off + atomicAdd(&s_tail, 1)] = v;
-  __syncthreads();
-  int lo = 0, hi = s_tail, L = 0;
-  __syncthreads();
-  while (lo < hi) {
-    if (tid == 0) ls[L] = lo;
-    L++;
-    for (int p = lo + tid; p < hi; p += FSX_WG) {
-      const int v = morder[off + p];
-      const long long gv = off + v;
-      const int nr = mnrec[gv];
-      for (int k = 0; k <= nr; k++) {
-        int t;
-        if (k < nr) t = mrec[8 * gv + k];
-        else if (mflag[gv]) t = rec[gv];
-        else break;
-        if (atomicSub(mindeg + off + t, 1) == 1) morder[off + atomicAdd(&s_tail, 1)] = t;
-      }
+  const int* cq = ctr + FSX_MFD_CTR * inst + 4;
+  const int queued = cq[2], L = cq[3] + 1;
+  int* hist = mls + P.aux;
+  int* cur = mcur + P.aux;
+  int carry = 0;
+  for (int l0 = 0; l0 < L; l0 += FSX_WG) {
+    const int l = l0 + (int)threadIdx.x;
+    const int x = l < L ? hist[l] : 0;
+    int tot;
+    const int incl = block_incl_scan<FSX_WG>(x, lds, tot);
+    if (l < L) {
+      hist[l] = carry + incl - x;
+      cur[l] = carry + incl - x;
     }
-    __syncthreads();
-    lo = hi;
-    hi = s_tail;
-    __syncthreads();
+    carry += tot;
   }
-  if (tid == 0) {
-    ls[L] = hi;
+  if (threadIdx.x == 0) {
+    hist[L] = carry;
     mnlev[inst] = L;
-    if (hi != nn) atomicOr(err + inst, FSX_ERR_MFD);  // 'error in stack', src/FlowRouting.f90:292
+    if (carry != P.nn || queued != P.nn) atomicOr(err + inst, FSX_ERR_MFD);
+  }
+}
+
+// queue position -> its layer's next slot of mstack (one cursor update per distinct layer of a wave: the
+// queue is close to layer order, so a wave's 64 positions hold few layers)
+__global__ __launch_bounds__(FSX_CHUNK) void k_mfd_scatter(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
+                                                           const unsigned long long* __restrict__ mq,
+                                                           int* __restrict__ mcur, int* __restrict__ morder) {
+  const Chunk c = chunks[chunk_id()];
+  const InstParams& P = PP[c.inst];
+  if (!P.active || !P.run_spl || P.sfd) return;  // uniform over the block
+  const int p = c.start + (int)threadIdx.x, lane = threadIdx.x & 63;
+  const unsigned long long e = p < P.nn ? mq[(long long)P.off + p] : ~0ull;
+  const bool ok = e != ~0ull;  // (an incomplete order is reported by k_mfd_lscan)
+  const int u = (int)(unsigned)(e & 0xffffffffull), lay = ok ? (int)(e >> 32) : 0;
+  unsigned long long rem = __ballot(ok);
+  while (rem) {
+    const int ld = __ffsll((long long)rem) - 1;
+    const int l0 = __shfl(lay, ld, 64);
+    const unsigned long long same = __ballot(ok && lay == l0) & rem;
+    int base = 0;
+    if (lane == ld) base = atomicAdd(mcur + P.aux + l0, __popcll(same));
+    base = __shfl(base, ld, 64);
+    if ((same >> lane) & 1ull) morder[P.off + base + __popcll(same & ((1ull << lane) - 1ull))] = u;
+    rem &= ~same;
   }
 }
 
 // ---- per-position records --------------------------------------------------------------------
-// The layer sweeps run over positions of the Kahn order (mstack): mpos is a node's position; every
-// per-node input a sweep needs is copied to its position by node-parallel kernels, so a sweep's only
-// dependent reads are its neighbours' values of earlier layers.
+// The sweeps run over positions of the Kahn order (mstack): mpos is a node's position; every per-node
+// input a sweep needs is copied to its position by node-parallel kernels, so a sweep's only dependent
+// reads are its neighbours' values of earlier layers.
 __global__ __launch_bounds__(FSX_CHUNK) void k_mfd_pos(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
                                                        const int* __restrict__ morder, int* __restrict__ mpos) {
   MFD_PROLOGUE
   mpos[P.off + morder[g_]] = v;  // here v runs over positions
 }
 
-// donors (position, weight) and receivers (positions) of each position; own area dx*dy*precip
+// donors (position, weight) and receivers (positions) of each position; own area dx*dy*precip; the
+// accumulation's output set to the sentinel
 __global__ __launch_bounds__(FSX_CHUNK) void k_mfd_prep(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
                                                         const int* __restrict__ mpos, const int* __restrict__ rec,
                                                         const int* __restrict__ mrec, const int* __restrict__ mnrec,
                                                         const double* __restrict__ mw, const int* __restrict__ mdon,
                                                         const int* __restrict__ mndon, const double* __restrict__ precip,
-                                                        double* __restrict__ A0, int* __restrict__ mdp,
+                                                        double* __restrict__ A0, double* __restrict__ Al,
+                                                        int* __restrict__ mdp,
                                                         double* __restrict__ mdw, int* __restrict__ dn,
                                                         int* __restrict__ rp) {
   MFD_PROLOGUE
   const long long off = P.off, gp = off + mpos[g_];
   A0[gp] = P.dxdy * precip[g_];
+  Al[gp] = flow_sentinel();  // the accumulation's dataflow waits for it to change
   const int nd = mndon[g_], nr = mnrec[g_];
   for (int j = 0; j < 8; j++) {
     int dpos = -1;
@@ -244,65 +490,64 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_mfd_prep(const InstParams* __rest
   rp[gp] = r == v ? -1 : mpos[off + r];
 }
 
-// ---- accumulation (FlowAccumulation :97-119), layer by layer from the sources ----------------------
-// Each position pulls its donors (earlier layers); the next layer's records are loaded while the
-// current one is summed.
-struct MfdAreaRec {
-  double a0;
-  int d[8];
-  double w[8];
-};
-__device__ __forceinline__ void mfd_area_load(MfdAreaRec& r, long long gp, const double* A0, const int* mdp,
-                                              const double* mdw) {
-  r.a0 = A0[gp];
+// ---- accumulation (FlowAccumulation :97-119) -------------------------------------------------------
+// Dataflow over positions from the sources: a position waits for its donors' areas, then sums them in
+// donor-slot order.
+__global__ __launch_bounds__(MFD_WG) void k_mfd_area(const InstParams* __restrict__ PP, const int2* __restrict__ mtab,
+                                                     int* __restrict__ ctr, int* __restrict__ err,
+                                                     const double* __restrict__ A0, const int* __restrict__ mdp,
+                                                     const double* __restrict__ mdw, double* __restrict__ Al) {
+  MFD_WAVE_PROLOGUE
+  MfdTiles T;
+  mfd_start(T, ctr + FSX_MFD_CTR * inst + 8, err + inst);
+  mfd_phase(T, (nn + 63) / 64, [&](int tt) {
+    const int p = 64 * tt + lane;
+    bool todo = p < nn;
+    const long long gp = off + (todo ? p : 0);
+    double a0 = 0.0, w[8], x[8];
+    int d[8];
+    unsigned need = 0;
 #pragma unroll
-  for (int j = 0; j < 8; j++) {
-    r.d[j] = mdp[8 * gp + j];
-    r.w[j] = mdw[8 * gp + j];
-  }
+    for (int j = 0; j < 8; j++) {
+      d[j] = todo ? mdp[8 * gp + j] : -1;
+      w[j] = todo ? mdw[8 * gp + j] : 0.0;
+      x[j] = 0.0;
+      if (d[j] >= 0) need |= 1u << j;
+    }
+    if (todo) a0 = A0[gp];
+    mfd_lanes(T, [&](bool& prog) {
+      if (todo) {
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+          if ((need >> j) & 1u) {
+            const double y = flow_ld_dev(Al + off + d[j]);
+            if (!flow_sent(y)) {
+              x[j] = y;
+              need &= ~(1u << j);
+            }
+          }
+        if (!need) {
+          double a = a0;
+#pragma unroll
+          for (int j = 0; j < 8; j++)
+            if (d[j] >= 0) a = a + x[j] * w[j];
+          flow_st_dev(Al + gp, a);
+          todo = false;
+          prog = true;
+        }
+      }
+      return todo;
+    });
+  });
 }
 
-__global__ __launch_bounds__(FSX_WG) void k_mfd_area(const InstParams* __restrict__ PP, const int* __restrict__ mls,
-                                                     const int* __restrict__ mnlev, const double* __restrict__ A0,
-                                                     const int* __restrict__ mdp, const double* __restrict__ mdw,
-                                                     double* __restrict__ Al) {
-  const int inst = blockIdx.x;
-  const InstParams& P = PP[inst];
-  if (!P.active || !P.run_spl || P.sfd) return;
-  const int tid = threadIdx.x;
-  const long long off = P.off;
-  const int* ls = mls + P.aux;
-  const int L = mnlev[inst];
-  MfdAreaRec nx;
-  int b0 = ls[0], b1 = L > 0 ? ls[1] : 0;
-  if (L > 0 && tid < b1 - b0) mfd_area_load(nx, off + b0 + tid, A0, mdp, mdw);
-  for (int l = 0; l < L; l++) {
-    const int c0 = b0, c1 = b1;
-    MfdAreaRec cur = nx;
-    if (l + 1 < L) {
-      b0 = c1;
-      b1 = ls[l + 2];
-      if (tid < b1 - b0) mfd_area_load(nx, off + b0 + tid, A0, mdp, mdw);
-    }
-    for (int p = c0 + tid; p < c1; p += FSX_WG) {
-      MfdAreaRec r;
-      if (p == c0 + tid) r = cur;
-      else mfd_area_load(r, off + p, A0, mdp, mdw);
-      double a = r.a0;
-#pragma unroll
-      for (int j = 0; j < 8; j++)
-        if (r.d[j] >= 0) a = a + Al[off + r.d[j]] * r.w[j];
-      Al[off + p] = a;
-    }
-    mfd_barrier();
-  }
-}
 
 // ---- per-position stream-power inputs (StreamPowerLaw :104-116 and :145-162) ----------------------
 // n == 1: kf*dt*(a*w)**m/l where ht(ijk) >= ht(rcv); otherwise kf*dt*(a*w)**m/l**n where
 // ht(ijk) > ht(rcv); -1 marks an edge that does not take part (constant over the Gauss-Seidel loop).
-// Also: area back to grid order, ht / h / lake sill in position order, the elevation coefficient
-// (g, NaN on fixed nodes: elev = ht there), and the landscape's maxval(abs(h)) and maxval(g).
+// Also: area back to grid order, ht in position order, the first pass's outputs set to the sentinel, the
+// elevation coefficient (g, NaN on fixed nodes: elev = ht there), and the landscape's maxval(abs(h)) and
+// maxval(g).
 __global__ __launch_bounds__(FSX_CHUNK) void k_mfd_fact(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
                                                         const int* __restrict__ mpos, const int* __restrict__ mrec,
                                                         const int* __restrict__ mnrec, const double* __restrict__ mw,
@@ -311,8 +556,10 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_mfd_fact(const InstParams* __rest
                                                         const double* __restrict__ ht, const double* __restrict__ g,
                                                         const uint8_t* __restrict__ bc, double* __restrict__ mfact,
                                                         double* __restrict__ HTl, double* __restrict__ Hl,
+                                                        double* __restrict__ Wl, double* __restrict__ DHl,
                                                         int* __restrict__ LSl, double* __restrict__ EC,
-                                                        double* __restrict__ lsed, unsigned long long* __restrict__ redk) {
+                                                        double* __restrict__ lsed, unsigned long long* __restrict__ redk,
+                                                        long long* __restrict__ newton) {
   __shared__ unsigned long long smax[2][FSX_CHUNK / 64];
   const Chunk c = chunks[chunk_id()];
   const InstParams& P = PP[c.inst];
@@ -336,12 +583,15 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_mfd_fact(const InstParams* __rest
     atomicMax(redk + 2 * c.inst, kh);
     atomicMax(redk + 2 * c.inst + 1, kg);
   }
+  if (v == 0) newton[c.inst] = 0;  // summed over the solve's waves
   if (!live) return;
   const long long gp = (long long)P.off + mpos[g_];
   const double a = Al[gp], t = ht[g_];
   area[g_] = a;
   HTl[gp] = t;
-  Hl[gp] = t;
+  Hl[gp] = flow_sentinel();  // values of the first Gauss-Seidel pass not written yet
+  Wl[gp] = flow_sentinel();
+  DHl[gp] = flow_sentinel();
   LSl[gp] = -1;
   lsed[g_] = 0.0;
   EC[gp] = bc[g_] ? __longlong_as_double(0x7ff8000000000000ll) : g[g_];
@@ -367,238 +617,316 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_mfd_fact(const InstParams* __rest
 // Per iteration: hp = h; erosion pushed from the sources (dh(rcv) += dh(ijk)*w, :62-82) with the
 // lake-sediment update of lake sills; elev (:84-90); solve from the last layer back (:94-176):
 // closed form over the receivers for n == 1, relaxed Newton (omega = 0.875/n) otherwise; lakes keep
-// elev and collect their water volume at their sill; err = rms(h-hp), zero when g == 0 everywhere.
-// Everything runs on positions; node-indexed water, lake sill and sediment are kept for the
-// lake fill of k_spl_post.
-struct MfdSolveRec {
-  int v, rp;
-  double t, e, h;
-  int q[8];
-  double f[8];
-};
-__device__ __forceinline__ void mfd_solve_load(MfdSolveRec& r, long long off, int p, const int* morder, const int* rp,
-                                               const double* HTl, const double* ELl, const double* Hl, const int* dn,
-                                               const double* mfact, bool gzero) {
-  const long long gp = off + p;
-  r.v = morder[gp];
-  r.rp = rp[gp];
-  r.t = HTl[gp];
-  r.e = gzero ? r.t : ELl[gp];
-  r.h = Hl[gp];
-#pragma unroll
-  for (int k = 0; k < 8; k++) {
-    r.q[k] = dn[8 * gp + k];
-    r.f[k] = mfact[8 * gp + k];
-  }
-}
 
-__global__ __launch_bounds__(FSX_WG) void k_mfd_spl(
-    const InstParams* __restrict__ PP, const int* __restrict__ mls, const int* __restrict__ mnlev,
+// ---- stream power, Gauss-Seidel loop ------------------------------------------------------------------
+// Per iteration (phases of one launch): hp = h (from the second iteration; h and the erosion push reset
+// to the sentinel); erosion pushed from the sources (dh(rcv) += dh(ijk)*w, :62-82) with the lake-sediment
+// update of lake sills; elev (:84-90), with water level, lake sill and lake volumes cleared; the solve from
+// base level up (:94-176): closed form over the receivers for n == 1, relaxed Newton (omega = 0.875/n)
+// otherwise; lakes keep elev and collect their water volume at their sill; err = rms(h-hp) from partial
+// sums over fixed blocks of positions, added in block order by every wave (so every wave takes the same
+// decision), zero when g == 0 everywhere.  Node-indexed water, lake sill and sediment are kept for the
+// lake fill of k_spl_post.
+__global__ __launch_bounds__(MFD_WG) void k_mfd_spl(
+    const InstParams* __restrict__ PP, const int2* __restrict__ mtab, int* __restrict__ ctr, int* __restrict__ err,
     const int* __restrict__ morder, const int* __restrict__ rp, const int* __restrict__ dn,
     const double* __restrict__ mfact, const int* __restrict__ mdp, const double* __restrict__ mdw,
     const double* __restrict__ EC, const double* __restrict__ Al, const double* __restrict__ HTl,
     double* __restrict__ Hl, double* __restrict__ HPl, double* __restrict__ DHl, double* __restrict__ ELl,
     double* __restrict__ Wl, int* __restrict__ LSl, double* __restrict__ water, int* __restrict__ lsill,
     double* __restrict__ lwv, double* __restrict__ lsed, const unsigned long long* __restrict__ redk,
-    double* __restrict__ scal, int* __restrict__ ngs_out, long long* __restrict__ newton_out, int* __restrict__ err) {
-  __shared__ double red[FSX_WG / 64];
-  __shared__ double s_tol, s_err, s_gmax;
-  __shared__ unsigned long long s_newton;
-  const int inst = blockIdx.x;
-  const InstParams& P = PP[inst];
-  if (!P.active || !P.run_spl || P.sfd) return;
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  const int nn = P.nn;
-  const long long off = P.off;
-  const int* ls = mls + P.aux;
-  const int L = mnlev[inst];
+    double* __restrict__ scal, double* __restrict__ part, int* __restrict__ mdec, int* __restrict__ ngs_out,
+    long long* __restrict__ newton_out) {
+  MFD_WAVE_PROLOGUE
   const double n = P.n;
   const bool n_is_one = fabs(n - 1.0) < 2.2250738585072014e-308;
   const int nkind = npow_kind(n);
-  if (tid == 0) {
-    s_tol = 1.0e-4 * (dval(redk[2 * inst]) + 1.0);
-    s_gmax = dval(redk[2 * inst + 1]);
-    s_err = 2.0 * s_tol;
-    s_newton = 0;
-  }
-  __syncthreads();
-  const bool gzero = s_gmax < 2.2250738585072014e-308;
+  const double tol = 1.0e-4 * (dval(redk[2 * inst]) + 1.0);
+  const bool gzero = dval(redk[2 * inst + 1]) < 2.2250738585072014e-308;
+  const double sent = flow_sentinel();
+  const int ntile = (nn + 63) / 64, nnode = (nn + 255) / 256, nblk = (nn + MFD_EB - 1) / MFD_EB;
+  MfdTiles T;
+  mfd_start(T, ctr + FSX_MFD_CTR * inst + 12, err + inst);
+  double errv = 2.0 * tol;
   int ngs = 0;
   unsigned long long my_newton = 0;
-  while (s_err > s_tol && ngs < 99) {
+  while (errv > tol && ngs < 99 && T.ok) {
     ngs++;
-    for (int p = tid; p < nn; p += FSX_WG) HPl[off + p] = Hl[off + p];
-    __syncthreads();
+    const double* prev = ngs == 1 ? HTl : HPl;  // h at the start of the iteration
+    if (ngs > 1)
+      mfd_phase(T, nnode, [&](int tt) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          const int p = 256 * tt + 64 * k + lane;
+          if (p < nn) {
+            const long long gp = off + p;
+            flow_st_dev(HPl + gp, flow_ld_dev(Hl + gp));
+            flow_st_dev(Hl + gp, sent);
+            flow_st_dev(DHl + gp, sent);
+          }
+        }
+      });
     if (!gzero) {
-      for (int l = 0; l < L; l++) {
-        const int b0 = ls[l], b1 = ls[l + 1];
-        for (int p = b0 + tid; p < b1; p += FSX_WG) {
-          const long long gp = off + p;
-          const int v = morder[gp];
-          const double own = HTl[gp] - HPl[gp];
-          double s = own;
+      mfd_phase(T, ntile, [&](int tt) {  // erosion push, from the sources
+        const int p = 64 * tt + lane;
+        bool todo = p < nn;
+        const long long gp = off + (todo ? p : 0);
+        double w[8], x[8], own = 0.0;
+        int d[8], v = 0, r = 0, sl = -1;
+        unsigned need = 0;
 #pragma unroll
-          for (int j = 0; j < 8; j++) {
-            const int dpos = mdp[8 * gp + j];
-            if (dpos >= 0) s = s + DHl[off + dpos] * mdw[8 * gp + j];
-          }
-          double dv = s;
-          if (rp[gp] >= 0) {
-            const double tt = s - own;
-            if (LSl[gp] == v) lsed[off + v] = tt <= 0.0 ? 0.0 : tt;
-            dv = tt + own;
-          } else {
-            lsed[off + v] = s;
-          }
-          DHl[gp] = dv;
+        for (int j = 0; j < 8; j++) {
+          d[j] = todo ? mdp[8 * gp + j] : -1;
+          w[j] = todo ? mdw[8 * gp + j] : 0.0;
+          x[j] = 0.0;
+          if (d[j] >= 0) need |= 1u << j;
         }
-        mfd_barrier();
-      }
-      for (int p = tid; p < nn; p += FSX_WG) {
-        const long long gp = off + p;
-        const double t = HTl[gp], gc = EC[gp];
-        ELl[gp] = isnan(gc) ? t : t + (DHl[gp] - (t - HPl[gp])) * gc * P.dx * P.dy / Al[gp];
-      }
-      __syncthreads();
-    }
-    // solve, last layer first; the next layer's records are loaded while this one is solved
-    MfdSolveRec nxr;
-    int b0 = L > 0 ? ls[L - 1] : 0, b1 = L > 0 ? ls[L] : 0;
-    if (L > 0 && tid < b1 - b0) mfd_solve_load(nxr, off, b0 + tid, morder, rp, HTl, ELl, Hl, dn, mfact, gzero);
-    for (int l = L - 1; l >= 0; l--) {
-      const int c0 = b0, c1 = b1;
-      MfdSolveRec cur = nxr;
-      if (l > 0) {
-        b1 = c0;
-        b0 = ls[l - 1];
-        if (tid < b1 - b0) mfd_solve_load(nxr, off, b0 + tid, morder, rp, HTl, ELl, Hl, dn, mfact, gzero);
-      }
-      for (int p = c0 + tid; p < c1; p += FSX_WG) {
-        MfdSolveRec r;
-        if (p == c0 + tid) r = cur;
-        else mfd_solve_load(r, off, p, morder, rp, HTl, ELl, Hl, dn, mfact, gzero);
-        const long long gp = off + p;
-        const int v = r.v;
-        if (r.rp < 0) {
-          Wl[gp] = r.t;
-          LSl[gp] = v;
-          if (!gzero) {
-            lwv[off + v] = 0.0;
-            water[off + v] = r.t;
-            lsill[off + v] = v;
-          }
-          continue;
+        if (todo) {
+          v = morder[gp];
+          r = rp[gp];
+          own = HTl[gp] - flow_ld_dev(prev + gp);
+          sl = flow_ldi_dev(LSl + gp);  // the lake sill of the previous iteration's solve
         }
-        const double w_rcv = Wl[off + r.rp];
-        double hv = r.h;
-        int sl;
-        if (r.e > w_rcv) {
-          // with Marine on, nodes below sea level keep their current h (src/StreamPowerLaw.f90:105,144:
-          // h for n == 1, ht otherwise)
-          if (r.q[0] >= 0 && !(P.run_marine && !((n_is_one ? r.h : r.t) >= P.sealevel))) {
-            double hr[8];
+        mfd_lanes(T, [&](bool& prog) {
+          if (todo) {
 #pragma unroll
-            for (int k = 0; k < 8; k++) hr[k] = r.q[k] >= 0 ? Hl[off + r.q[k]] : 0.0;
-            if (n_is_one) {
-              double f = r.e, df = 1.0;
-#pragma unroll
-              for (int k = 0; k < 8; k++)
-                if (r.q[k] >= 0 && r.f[k] >= 0.0) {
-                  f = f + r.f[k] * hr[k];
-                  df = df + r.f[k];
+            for (int j = 0; j < 8; j++)
+              if ((need >> j) & 1u) {
+                const double y = flow_ld_dev(DHl + off + d[j]);
+                if (!flow_sent(y)) {
+                  x[j] = y;
+                  need &= ~(1u << j);
                 }
-              hv = f / df;
-            } else {
-              const double omega = 0.875 / n, tolp = P.newton_tol;
-              double errp = 2.0 * tolp;
-              const double h0 = r.e;
-              int it = 0;
-              while (errp > tolp) {
-                double f = hv - h0, df = 1.0;
-#pragma unroll
-                for (int k = 0; k < 8; k++)
-                  if (r.q[k] >= 0 && r.f[k] >= 0.0) {
-                    const double x = fmax(0.0, hv - hr[k]);
-                    double pn, pn1;
-                    npow(x, n, nkind, pn, pn1);
-                    f = f + r.f[k] * pn;
-                    df = df + r.f[k] * n * pn1;
-                  }
-                const double hn = hv - f / df;
-                errp = fabs(hn - hv);
-                hv = hv * (1.0 - omega) + hn * omega;
-                if (++it > 100000) { atomicOr(err + inst, FSX_ERR_NEWTON); break; }
               }
-              my_newton += it;
+            if (!need) {
+              double s = own;
+#pragma unroll
+              for (int j = 0; j < 8; j++)
+                if (d[j] >= 0) s = s + x[j] * w[j];
+              double dv = s;
+              if (r >= 0) {
+                const double t2 = s - own;
+                if (sl == v) lsed[off + v] = t2 <= 0.0 ? 0.0 : t2;
+                dv = t2 + own;
+              } else {
+                lsed[off + v] = s;
+              }
+              flow_st_dev(DHl + gp, dv);
+              todo = false;
+              prog = true;
             }
           }
-          sl = v;
-          if (!gzero) lwv[off + v] = 0.0;
-          if (hv < w_rcv) hv = w_rcv;
-        } else {
-          hv = r.e;
-          sl = LSl[off + r.rp];
-          if (!gzero && sl >= 0) atomicAdd(lwv + off + sl, w_rcv - hv);
+          return todo;
+        });
+      });
+      mfd_phase(T, nnode, [&](int tt) {  // elev; the solve's water level, lake sill and lake volumes cleared
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          const int p = 256 * tt + 64 * k + lane;
+          if (p < nn) {
+            const long long gp = off + p;
+            const double t = HTl[gp], gc = EC[gp];
+            flow_st_dev(ELl + gp,
+                        isnan(gc) ? t : t + (flow_ld_dev(DHl + gp) - (t - flow_ld_dev(prev + gp))) * gc * P.dx * P.dy / Al[gp]);
+            flow_st_dev(Wl + gp, sent);
+            flow_sti_dev(LSl + gp, -1);
+            flow_st_dev(lwv + off + morder[gp], 0.0);
+          }
         }
-        const double wv = w_rcv > hv ? w_rcv : hv;
-        Hl[gp] = hv;
-        Wl[gp] = wv;
-        LSl[gp] = sl;
-        if (!gzero) {
-          water[off + v] = wv;
-          lsill[off + v] = sl;
-        }
-      }
-      mfd_barrier();
+      });
     }
+    mfd_phase(T, ntile, [&](int tt) {  // solve, from base level up
+      const int p = nn - 1 - (64 * tt + lane);
+      bool todo = p >= 0;
+      const long long gp = off + (todo ? p : 0);
+      int q[8], v = 0, r = -1;
+      double f[8], hr[8], t = 0.0, e = 0.0, h0 = 0.0, w_rcv = 0.0;
+      int sl_rcv = -1;
+      unsigned need = 0;
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        q[k] = todo ? dn[8 * gp + k] : -1;
+        f[k] = todo ? mfact[8 * gp + k] : -1.0;
+        hr[k] = 0.0;
+        if (q[k] >= 0 && f[k] >= 0.0) need |= 1u << k;
+      }
+      if (todo) {
+        v = morder[gp];
+        r = rp[gp];
+        t = HTl[gp];
+        e = gzero ? t : flow_ld_dev(ELl + gp);
+        h0 = flow_ld_dev(prev + gp);
+        if (r >= 0) need |= 3u << 8;  // the single-flow receiver's water level and lake sill
+      }
+      mfd_lanes(T, [&](bool& prog) {
+        if (todo) {
+#pragma unroll
+          for (int k = 0; k < 8; k++)
+            if ((need >> k) & 1u) {
+              const double y = flow_ld_dev(Hl + off + q[k]);
+              if (!flow_sent(y)) {
+                hr[k] = y;
+                need &= ~(1u << k);
+              }
+            }
+          if ((need >> 8) & 1u) {
+            const double y = flow_ld_dev(Wl + off + r);
+            if (!flow_sent(y)) {
+              w_rcv = y;
+              need &= ~(1u << 8);
+            }
+          }
+          if ((need >> 9) & 1u) {
+            const int y = flow_ldi_dev(LSl + off + r);
+            if (y >= 0) {
+              sl_rcv = y;
+              need &= ~(1u << 9);
+            }
+          }
+          if (!need) {
+            todo = false;
+            prog = true;
+            if (r < 0) {  // base level
+              flow_st_dev(Hl + gp, h0);
+              flow_st_dev(Wl + gp, t);
+              flow_sti_dev(LSl + gp, v);
+              if (!gzero) {
+                water[off + v] = t;
+                lsill[off + v] = v;
+              }
+            } else {
+              double hv = h0;
+              int sl;
+              if (e > w_rcv) {
+                // with Marine on, nodes below sea level keep their current h (src/StreamPowerLaw.f90:105,144:
+                // h for n == 1, ht otherwise)
+                if (q[0] >= 0 && !(P.run_marine && !((n_is_one ? h0 : t) >= P.sealevel))) {
+                  if (n_is_one) {
+                    double fs = e, df = 1.0;
+#pragma unroll
+                    for (int k = 0; k < 8; k++)
+                      if (q[k] >= 0 && f[k] >= 0.0) {
+                        fs = fs + f[k] * hr[k];
+                        df = df + f[k];
+                      }
+                    hv = fs / df;
+                  } else {
+                    const double omega = 0.875 / n, tolp = P.newton_tol;
+                    double errp = 2.0 * tolp;
+                    int it = 0;
+                    while (errp > tolp) {
+                      double fs = hv - e, df = 1.0;
+#pragma unroll
+                      for (int k = 0; k < 8; k++)
+                        if (q[k] >= 0 && f[k] >= 0.0) {
+                          const double x = fmax(0.0, hv - hr[k]);
+                          double pn, pn1;
+                          npow(x, n, nkind, pn, pn1);
+                          fs = fs + f[k] * pn;
+                          df = df + f[k] * n * pn1;
+                        }
+                      const double hn = hv - fs / df;
+                      errp = fabs(hn - hv);
+                      hv = hv * (1.0 - omega) + hn * omega;
+                      if (++it > 100000) {
+                        atomicOr(err + inst, FSX_ERR_NEWTON);
+                        break;
+                      }
+                    }
+                    my_newton += it;
+                  }
+                }
+                sl = v;
+                if (hv < w_rcv) hv = w_rcv;
+              } else {
+                hv = e;
+                sl = sl_rcv;
+                if (!gzero) atomicAdd(lwv + off + sl, w_rcv - hv);
+              }
+              const double wv = w_rcv > hv ? w_rcv : hv;
+              flow_st_dev(Hl + gp, hv);
+              flow_st_dev(Wl + gp, wv);
+              flow_sti_dev(LSl + gp, sl);
+              if (!gzero) {
+                water[off + v] = wv;
+                lsill[off + v] = sl;
+              }
+            }
+          }
+        }
+        return todo;
+      });
+    });
     if (gzero) {
-      if (tid == 0) s_err = 0.0;
-    } else {
-      double ss = 0.0;
-      for (int p = tid; p < nn; p += FSX_WG) {
-        const double d = Hl[off + p] - HPl[off + p];
-        ss += d * d;
-      }
-      for (int o = 32; o > 0; o >>= 1) ss += __shfl_xor(ss, o, 64);
-      if (lane == 0) red[wid] = ss;
-      __syncthreads();
-      if (tid == 0) {
-        double x = 0.0;
-        for (int q = 0; q < FSX_WG / 64; q++) x += red[q];
-        s_err = sqrt(x / (double)nn);
-      }
+      errv = 0.0;
+      break;
     }
-    __syncthreads();
+    mfd_phase(T, nblk, [&](int blk) {  // partial sums of (h - hp)**2
+      double s = 0.0;
+      for (int i = lane; i < MFD_EB; i += 64) {
+        const int p = blk * MFD_EB + i;
+        if (p < nn) {
+          const double dd = flow_ld_dev(Hl + off + p) - flow_ld_dev(prev + off + p);
+          s += dd * dd;
+        }
+      }
+      for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+      if (lane == 0) flow_st_dev(part + P.c0 + blk, s);
+    });
+    if (!mfd_wait(T, T.base)) break;
+    double x = 0.0;
+    for (int j = lane; j < nblk; j += 64) x += flow_ld_dev(part + P.c0 + j);
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    x = __shfl(x, 0, 64);  // lane 0's sum in every lane
+    // The iteration's decision (1 go on, 2 stop) is kept: a wave that reaches this point late, when later
+    // iterations may have overwritten the partial sums, takes the decision the others took (a wave writes
+    // it and drains the store before any of its later phases, so partials are overwritten only once it
+    // is readable).
+    int* dp = mdec + MFD_MAXGS * inst + ngs;
+    int dcs = flow_ldi_dev(dp);
+    if (dcs == 0) {
+      dcs = sqrt(x / (double)nn) > tol ? 1 : 2;
+      if (lane == 0) flow_sti_dev(dp, dcs);
+      wave_drain();
+    }
+    errv = dcs == 1 ? 2.0 * tol : 0.0;
   }
-  if (my_newton) atomicAdd(&s_newton, my_newton);
-  __syncthreads();
-  if (tid == 0) {
+  if (my_newton) atomicAdd(reinterpret_cast<unsigned long long*>(newton_out + inst), my_newton);
+  if (wk.y == 0 && threadIdx.x == 0) {
     ngs_out[inst] = ngs;
-    newton_out[inst] = (long long)s_newton;
     scal[inst] = gzero ? 0.0 : 1.0;
   }
 }
 
 // ---------------------------------------------------------------------------------------------------
 void launch_mfd_route(DevState& S, hipStream_t st) {
-  k_mfd_fill<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.lstart, S.nlev, S.order, S.rec, S.ht, S.hwater, S.lake_depth);
-  launch_fill_int(S.mindeg, S.ntot, 0, st);
+  if (!S.nmfd) return;
+  launch_fill_int(S.mctr, FSX_MFD_CTR * S.ninst, 0, st);
+  k_mfd_init<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.mkahn, S.mq, S.mls, S.hwater);
+  k_mfd_fill<<<S.nmfd, MFD_WG, 0, st>>>(S.P, S.mfd_tab, S.mctr, S.err, S.nlev, S.order, S.rec, S.ht,
+                                         S.hwater, S.lake_depth);
   k_mfd_rec<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.hwater, S.rec, S.mrec, S.mw, S.mdir, S.mnrec, S.mflag,
-                                              S.mindeg, S.pexp);
+                                              S.mkahn, S.pexp);
   k_mfd_don<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.mrec, S.mnrec, S.mdon, S.mndon, S.err);
-  k_mfd_levels<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.mindeg, S.mrec, S.mnrec, S.mflag, S.rec, S.mstack, S.mls, S.mnlev,
-                                            S.err);
+  k_mfd_kahn<<<S.nmfd, MFD_WG, 0, st>>>(S.P, S.mfd_tab, S.mctr, S.err, S.mkahn, S.mrec, S.mnrec, S.mflag, S.rec, S.mq,
+                                         S.mls);
+  k_mfd_lscan<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.mctr, S.mls, S.mcur, S.mnlev, S.err);
+  k_mfd_scatter<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.mq, S.mcur, S.mstack);
   k_mfd_pos<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.mstack, S.mpos);
   k_mfd_prep<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.mpos, S.rec, S.mrec, S.mnrec, S.mw, S.mdon, S.mndon,
-                                               S.precip, S.A0, S.mdp, S.mdw, S.dn, S.rp);
-  k_mfd_area<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.mls, S.mnlev, S.A0, S.mdp, S.mdw, S.Al);
+                                               S.precip, S.A0, S.Al, S.mdp, S.mdw, S.dn, S.rp);
+  k_mfd_area<<<S.nmfd, MFD_WG, 0, st>>>(S.P, S.mfd_tab, S.mctr, S.err, S.A0, S.mdp, S.mdw, S.Al);
 }
 
 void launch_mfd_spl(DevState& S, hipStream_t st) {
+  if (!S.nmfd) return;
   launch_fill_int(reinterpret_cast<int*>(S.redk), 4LL * S.ninst, 0, st);
+  launch_fill_int(S.mctr, FSX_MFD_CTR * S.ninst, 0, st);
+  launch_fill_int(S.mdec, MFD_MAXGS * S.ninst, 0, st);
   k_mfd_fact<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.mpos, S.mrec, S.mnrec, S.mw, S.mdir, S.Al, S.area,
-                                               S.kfint, S.ht, S.g, S.bc, S.mfact, S.HTl, S.Hl, S.LSl, S.F, S.lsed, S.redk);
-  k_mfd_spl<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.mls, S.mnlev, S.mstack, S.rp, S.dn, S.mfact, S.mdp, S.mdw, S.F, S.Al,
-                                         S.HTl, S.Hl, S.HPl, S.DHl, S.ELl, S.Wl, S.LSl, S.water, S.lsill, S.lwv, S.lsed,
-                                         S.redk, S.scal, S.ngs, S.newton, S.err);
+                                               S.kfint, S.ht, S.g, S.bc, S.mfact, S.HTl, S.Hl, S.Wl, S.DHl, S.LSl, S.F,
+                                               S.lsed, S.redk, S.newton);
+  k_mfd_spl<<<S.nmfd, MFD_WG, 0, st>>>(S.P, S.mfd_tab, S.mctr, S.err, S.mstack, S.rp, S.dn, S.mfact, S.mdp, S.mdw, S.F,
+                                        S.Al, S.HTl, S.Hl, S.HPl, S.DHl, S.ELl, S.Wl, S.LSl, S.water, S.lsill, S.lwv,
+                                        S.lsed, S.redk, S.scal, S.mpart, S.mdec, S.ngs, S.newton);
 }

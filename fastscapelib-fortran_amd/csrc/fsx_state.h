@@ -14,6 +14,8 @@ struct DevState {
   int4* tiles = nullptr;    // [ntiles]: (instance, x0, y0, -)
   int nflow = 0;            // workgroups of the dataflow stream-power sweep over all instances
   int2* flow_tab = nullptr; // [nflow]: (instance, rank among the instance's workgroups)
+  int nmfd = 0;             // workgroups of the multiple-flow dataflow sweeps over all instances
+  int2* mfd_tab = nullptr;  // [nmfd]: (instance, rank among the instance's workgroups)
   int* flow_next = nullptr; // [ninst]: the sweep's tile counter of each instance
   // launch shapes, chosen by fsx_autotune (no result depends on them)
   int walk_gx = 256;       // basin walk: blocks per landscape
@@ -25,6 +27,7 @@ struct DevState {
   int flow_wg = 1024;      // stream-power dataflow sweep: threads per workgroup (256, 512 or 1024)
   int flow_split = 131072; // stream-power dataflow sweep: nodes per extra workgroup (0: one per landscape)
   int nr_gx = 8;           // edge sort: workgroups per landscape for the big ranges of a level
+  int mfd_split = 131072;  // multiple-flow dataflow sweeps: nodes per workgroup of a landscape
 
   // ---- per node (ntot) ----
   double *h = nullptr, *b = nullptr, *u = nullptr, *kf = nullptr, *kd = nullptr, *precip = nullptr;
@@ -74,7 +77,13 @@ struct DevState {
   int *mrec = nullptr, *mdon = nullptr;
   double *mw = nullptr, *mfact = nullptr;
   uint8_t *mdir = nullptr, *mflag = nullptr;
-  int *mnrec = nullptr, *mndon = nullptr, *mindeg = nullptr, *mstack = nullptr, *mls = nullptr;
+  int *mnrec = nullptr, *mndon = nullptr, *mstack = nullptr, *mls = nullptr;
+  unsigned long long* mkahn = nullptr;  // Kahn traversal: (layer << 32 | in-edges left) per node
+  unsigned long long* mq = nullptr;     // Kahn traversal: queue of (layer << 32 | node)
+  int* mcur = nullptr;                  // layer cursors of the scatter into mstack (nn+1 per landscape)
+  int* mctr = nullptr;                  // dataflow counters, FSX_MFD_CTR per landscape
+  double* mpart = nullptr;              // Gauss-Seidel error partial sums (indexed by the chunk range)
+  int* mdec = nullptr;                  // Gauss-Seidel decisions per iteration (100 per landscape)
   int *mnlev = nullptr;  // [ninst]
   int *mpos = nullptr, *mdp = nullptr;  // position of each node in the Kahn order; donor positions (8/node)
   double *mdw = nullptr;                // donor weights by position (8/node)

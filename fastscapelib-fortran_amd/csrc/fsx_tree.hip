@@ -232,12 +232,6 @@ __global__ __launch_bounds__(FSX_WG) void k_level_sort(const InstParams* __restr
 }
 
 // ---- per-position records -----------------------------------------------------------------------
-// A value of the dataflow sweep that is not written yet: a NaN whose payload no arithmetic produces from
-// the finite (or canonical NaN) values the sweep computes.
-#define FLOW_SENT 0xFFF8DEADBEEF0001ull
-__device__ __forceinline__ bool flow_sent(double x) {
-  return (unsigned long long)__double_as_longlong(x) == FLOW_SENT;
-}
 
 // Drainage-area inputs: own term dx*dy*precip and the positions (relative to the next level's start)
 // of the donors in descending node index; unused slots -1.
@@ -898,19 +892,6 @@ __global__ __launch_bounds__(FSX_WG) void k_spl_sweep(
 #define FLOW_SPIN (1 << 26)
 #define FLOW_GLOBAL 32
 
-__device__ __forceinline__ double flow_ld(const double* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-__device__ __forceinline__ void flow_st(double* p, double v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-// the same between workgroups of one landscape (other compute units, possibly other XCDs)
-__device__ __forceinline__ double flow_ld_dev(const double* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void flow_st_dev(double* p, double v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 // LDS accesses (the address space is inferred after inlining)
 __device__ __forceinline__ double lds_ld(double* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
