@@ -274,8 +274,8 @@ static void arena_give(fsx_ctx* c, long long off, long long n) {
 // flow_split nodes each, at most kFlowGMax per landscape (none for landscapes with more tiles than a
 // workgroup's own-tile bitmap holds).  A landscape's workgroups are consecutive in the table, so they are
 // dealt to one XCD.  Rebuilt with the parameters; a changed table invalidates the captured step.
-static const int kFlowGMax = 8;
-static const long long kFlowMaxTiles = 2048LL * 32;
+static const int kFlowGMax = 16;
+static const long long kFlowMaxTiles = 4096LL * 32;  // the own-tile bitmap of k_spl_flow (FLOW_OWN_WORDS words)
 static int cu_count() {
   static int n = 0;
   if (!n) {
@@ -505,6 +505,7 @@ static int ctx_alloc(long long capacity, int nslots, fsx_ctx** out) {
   A(J, N); A(ngroups, ninst); A(grec, 2 * (N + ninst)); A(gmeta, N + ninst); A(pos, N); A(lcur, N + ninst); A(rp, N); A(LSl, N); A(dn, 8 * N);
   A(A0, N); A(Al, N); A(F, N); A(HTl, N); A(Hl, N); A(Wl, N); A(HWl, N); A(HPl, N); A(DHl, N); A(ELl, N);
   A(arec, N); A(srec, N); A(wnode, N); A(wmask, N);
+  A(wro, N); A(rjmp, N); A(wrank, N); A(rlist, N / 32 + ninst + 64); A(biglist, N / 2048 + ninst + 1); A(bigctr, 64);
 #undef A
   if (rc) { fsx_destroy(c); return rc; }
   *out = c;

@@ -26,6 +26,14 @@
   const int v = c.start + (int)threadIdx.x;  \
   const bool live = P.active && P.run_spl && v < P.nn;
 
+// Closed basins of more than WALK_BIG nodes are searched node-parallel (k_big_*), since one thread's walk
+// takes time in proportion to its basin.
+#define WALK_BIG 2048
+#define BIG_NBIG 0    // bigctr[0]: big basins listed
+#define BIG_NRUL 1    // bigctr[1]: rulers listed
+#define BIG_FLAG0 2   // bigctr[2..]: convergence flags of the ruler jumps
+#define BIG_ROUNDS 24
+
 __global__ void k_fill_int(int* __restrict__ a, long long n, int val) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
     a[i] = val;
@@ -169,10 +177,12 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_basin_label(const InstParams* __r
   }
 }
 
-// exclusive scan of closed-basin sizes -> candidate slab offsets; clears edge counts
+// exclusive scan of closed-basin sizes -> candidate slab offsets; clears edge counts; lists the closed
+// basins of more than WALK_BIG nodes
 __global__ __launch_bounds__(FSX_WG) void k_bsize_scan(const InstParams* __restrict__ PP, const int* __restrict__ nbasins,
                                                        const int* __restrict__ bsize, int* __restrict__ soff,
-                                                       int* __restrict__ ecount) {
+                                                       int* __restrict__ ecount, int2* __restrict__ biglist,
+                                                       int* __restrict__ bigctr, int bigcap) {
   __shared__ int lds[FSX_WG / 64];
   const int inst = blockIdx.x;
   const InstParams& P = PP[inst];
@@ -187,6 +197,10 @@ __global__ __launch_bounds__(FSX_WG) void k_bsize_scan(const InstParams* __restr
     if (b < nb) {
       soff[P.off + b] = carry + incl - x;
       ecount[P.off + b] = 0;
+      if (x > WALK_BIG) {  // searched by the big-basin kernels, not by the walk
+        const int e = atomicAdd(bigctr + BIG_NBIG, 1);
+        if (e < bigcap) biglist[e] = make_int2(inst, b);
+      }
     }
     carry += tot;
   }
@@ -262,6 +276,7 @@ void k_basin_walk(const InstParams* __restrict__ PP, const int* __restrict__ nba
     for (b += STRIDE; b < nb; b += STRIDE) {
       const int o = outlet[off + b];
       if (bc[off + o]) continue;  // open basin: no scan
+      if (bsize[off + b] > WALK_BIG) continue;  // k_big_* search it
       v = o;
       ne = 0;
       visits = 0;
@@ -367,6 +382,257 @@ void k_basin_walk(const InstParams* __restrict__ PP, const int* __restrict__ nba
       live = next_basin();
     }
   }
+}
+
+// ---- 2b. passes of big basins, node-parallel ---------------------------------------------------------
+// The walk's results for a basin A are, for every basin B that the search of A's nodes meets, the first
+// (preorder rank, k) at which it meets B -- the edge order -- and the first (rank, k) among the pairs of
+// least weight max(h(v), h(q)) -- the pass.  Both are minima over (rank, k), so once every node knows its
+// rank in A's depth-first preorder, the nodes of A can be searched in any order:
+//  1. ranks: rulers (A's root and nodes picked by a hash, about one in 64) walk the preorder successors
+//     up to the next ruler, giving every node its ruler and offset; pointer jumping over the rulers gives
+//     each its distance to the end of A's preorder, so rank = size(A) - that distance + offset;
+//  2. a hash table of the basins B met, in A's candidate slab (capacity the largest power of two not
+//     above 4 size(A)), takes the minima by atomics in three node-parallel passes: the first meeting and
+//     the least weight, then the first (rank, k) of least weight, then that pass's node;
+//  3. one workgroup per big basin ranks the table entries by first meeting and writes them out as the walk
+//     does (neighbour basin, weight, pass nodes at the slab's start, count in ecount).
+__device__ __forceinline__ bool big_ruler_pick(int v) { return (((unsigned)v * 0x9E3779B1u) >> 26) == 0u; }
+__device__ __forceinline__ int big_cap(int size) {  // largest power of two <= 4 size
+  return (int)(1u << (31 - __clz((unsigned)(4 * size))));
+}
+#define BIG_NONE 0xffffffffu
+
+// rulers of big basins walk their segment of the preorder; rulers are listed for the jumps
+__global__ __launch_bounds__(FSX_CHUNK) void k_big_rulers(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
+                                                          const WalkNode* __restrict__ wnode, const int* __restrict__ rec,
+                                                          const int* __restrict__ bsize, unsigned long long* __restrict__ wro,
+                                                          unsigned long long* __restrict__ rjmp, int2* __restrict__ rlist,
+                                                          int* __restrict__ bigctr, int rcap, int* __restrict__ err) {
+  NODE_PROLOGUE
+  const ulonglong2* WN = reinterpret_cast<const ulonglong2*>(wnode) + P.off;
+  bool ruler = false;
+  if (live) {
+    const int bv = (int)(unsigned)(WN[v].y & 0xffffffffull);
+    ruler = bv >= 0 && bsize[P.off + bv] > WALK_BIG && (rec[P.off + v] == v || big_ruler_pick(v));
+  }
+  const int lane = threadIdx.x & 63;
+  const unsigned long long m = __ballot(ruler);
+  if (m) {
+    const int first = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == first) base = atomicAdd(bigctr + BIG_NRUL, __popcll(m));
+    base = __shfl(base, first, 64);
+    const int idx = base + __popcll(m & ((1ull << lane) - 1ull));
+    if (ruler) {
+      if (idx < rcap) rlist[idx] = make_int2(c.inst, v);
+      else atomicOr(err + c.inst, FSX_ERR_NCONN);
+    }
+  }
+  if (!ruler) return;
+  int u = v;
+  for (unsigned d = 0; d <= (unsigned)P.nn; d++) {
+    wro[P.off + u] = ((unsigned long long)(unsigned)v << 32) | d;
+    const int su = (int)(unsigned)(WN[u].y >> 32);
+    if (su < 0 || big_ruler_pick(su)) {
+      rjmp[P.off + v] = ((unsigned long long)(su < 0 ? BIG_NONE : (unsigned)su) << 32) | (d + 1u);
+      return;
+    }
+    u = su;
+  }
+  atomicOr(err + c.inst, FSX_ERR_LEVELS);  // the successors do not end: not a tree
+}
+
+// (next ruler, nodes up to it) -> (end, nodes to the end of the preorder), two hops per launch
+__global__ __launch_bounds__(256) void k_big_jump(const InstParams* __restrict__ PP, const int2* __restrict__ rlist,
+                                                  unsigned long long* __restrict__ rjmp, int* __restrict__ bigctr,
+                                                  int rcap, int iter) {
+  if (iter > 0 && bigctr[BIG_FLAG0 + iter - 1] == 0) return;
+  const int n = min(bigctr[BIG_NRUL], rcap);
+  bool changed = false;
+  for (int i = blockIdx.x * 256 + (int)threadIdx.x; i < n; i += gridDim.x * 256) {
+    const int2 r = rlist[i];
+    unsigned long long* J = rjmp + PP[r.x].off;
+    unsigned long long w = J[r.y];
+#pragma unroll
+    for (int hop = 0; hop < 2; hop++) {
+      const unsigned x = (unsigned)(w >> 32);
+      if (x == BIG_NONE) break;
+      const unsigned long long wx = J[x];
+      w = (wx & 0xffffffff00000000ull) | (unsigned long long)((unsigned)w + (unsigned)wx);
+      J[r.y] = w;
+      changed = true;
+    }
+  }
+  if (changed) bigctr[BIG_FLAG0 + iter] = 1;
+}
+
+// preorder rank of every node of a big basin
+__global__ __launch_bounds__(FSX_CHUNK) void k_big_rank(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
+                                                        const WalkNode* __restrict__ wnode, const int* __restrict__ bsize,
+                                                        const unsigned long long* __restrict__ wro,
+                                                        const unsigned long long* __restrict__ rjmp, int* __restrict__ wrank) {
+  NODE_PROLOGUE
+  if (!live) return;
+  const int bv = wnode[P.off + v].basin;
+  if (bv < 0) return;
+  const int sz = bsize[P.off + bv];
+  if (sz <= WALK_BIG) return;
+  const unsigned long long ro = wro[P.off + v];
+  const unsigned long long rj = rjmp[P.off + (int)(unsigned)(ro >> 32)];
+  wrank[P.off + v] = sz - (int)(unsigned)(rj & 0xffffffffull) + (int)(unsigned)(ro & 0xffffffffull);
+}
+
+// empty tables of the big basins (one workgroup per listed basin)
+__global__ __launch_bounds__(256) void k_big_init(const InstParams* __restrict__ PP, const int2* __restrict__ biglist,
+                                                  const int* __restrict__ bigctr, int bigcap, const int* __restrict__ bsize,
+                                                  const int* __restrict__ soff, int* __restrict__ sB, int* __restrict__ sp,
+                                                  int* __restrict__ sq, double* __restrict__ sw, int* __restrict__ sh) {
+  const int n = min(bigctr[BIG_NBIG], bigcap);
+  for (int e = blockIdx.x; e < n; e += gridDim.x) {
+    const int2 ab = biglist[e];
+    const InstParams& P = PP[ab.x];
+    const long long sbase = (long long)FSX_SLAB_CAP * (P.off + soff[P.off + ab.y]);
+    const int C = big_cap(bsize[P.off + ab.y]);
+    for (int t = threadIdx.x; t < C; t += 256) {
+      sh[sbase + t] = -1;
+      sB[sbase + t] = -1;
+      sp[sbase + t] = INT_MAX;
+      sq[sbase + t] = INT_MAX;
+      reinterpret_cast<unsigned long long*>(sw)[sbase + t] = ~0ull;
+    }
+  }
+}
+
+// pass 0: first meeting and least weight; 1: first (rank, k) of least weight; 2: that pass's node
+__global__ __launch_bounds__(FSX_CHUNK) void k_big_cand(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
+                                                        const WalkNode* __restrict__ wnode, const uint8_t* __restrict__ wmask,
+                                                        const int* __restrict__ bsize, const int* __restrict__ soff,
+                                                        const int* __restrict__ wrank, int* __restrict__ sB,
+                                                        int* __restrict__ sp, int* __restrict__ sq, double* __restrict__ sw,
+                                                        int* __restrict__ sh, int* __restrict__ err, int pass) {
+  NODE_PROLOGUE
+  if (!live) return;
+  const unsigned mv = wmask[P.off + v];
+  if (!mv) return;
+  const ulonglong2* WN = reinterpret_cast<const ulonglong2*>(wnode) + P.off;
+  const ulonglong2 wv = WN[v];
+  const int A = (int)(unsigned)(wv.y & 0xffffffffull);
+  if (A < 0) return;
+  const int sz = bsize[P.off + A];
+  if (sz <= WALK_BIG) return;
+  constexpr int II[8] = {-1, 1, 0, 0, -1, 1, -1, 1};
+  constexpr int JJ[8] = {0, 0, -1, 1, -1, -1, 1, 1};
+  const double hv = __longlong_as_double((long long)wv.x);
+  const int i = v % P.nx, j = v / P.nx, rank = wrank[P.off + v];
+  const long long sbase = (long long)FSX_SLAB_CAP * (P.off + soff[P.off + A]);
+  const unsigned C = (unsigned)big_cap(sz);
+  unsigned long long* SW = reinterpret_cast<unsigned long long*>(sw);
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    if (!((mv >> k) & 1u)) continue;
+    const int q = (j + JJ[k]) * P.nx + (i + II[k]);
+    const ulonglong2 wq = WN[q];
+    const int Bs = (int)(unsigned)(wq.y & 0xffffffffull);
+    const int B = Bs >= 0 ? Bs : -(Bs + 1);
+    const double hq = __longlong_as_double((long long)wq.x);
+    const unsigned long long wk = dkey(hv > hq ? hv : hq);
+    const int key = rank * 8 + k;
+    // the slot of B (inserted on the first pass)
+    unsigned s = ((unsigned)B * 2654435761u) & (C - 1u);
+    long long slot = -1;
+    for (unsigned pr = 0; pr < C; pr++, s = (s + 1u) & (C - 1u)) {
+      const int cur = pass == 0 ? atomicCAS(sh + sbase + s, -1, B) : sh[sbase + s];
+      if (cur == B || (pass == 0 && cur == -1)) {
+        slot = sbase + s;
+        break;
+      }
+      if (cur == -1) break;  // (later passes: B is always present)
+    }
+    if (slot < 0) {
+      atomicOr(err + c.inst, FSX_ERR_NCONN);
+      continue;
+    }
+    if (pass == 0) {
+      atomicMin(sp + slot, key);
+      atomicMin(SW + slot, wk);
+    } else if (pass == 1) {
+      if (SW[slot] == wk) atomicMin(sq + slot, key);
+    } else {
+      if (sq[slot] == key) sB[slot] = v;
+    }
+  }
+}
+
+// the table of a big basin in the walk's output form: entries ranked by first meeting
+__global__ __launch_bounds__(256) void k_big_final(const InstParams* __restrict__ PP, const int2* __restrict__ biglist,
+                                                   const int* __restrict__ bigctr, int bigcap, const int* __restrict__ bsize,
+                                                   const int* __restrict__ soff, int* __restrict__ sB, int* __restrict__ sp,
+                                                   int* __restrict__ sq, double* __restrict__ sw, int* __restrict__ sh,
+                                                   int* __restrict__ ecount, int* __restrict__ err) {
+  __shared__ int s_cnt;
+  constexpr int II[8] = {-1, 1, 0, 0, -1, 1, -1, 1};
+  constexpr int JJ[8] = {0, 0, -1, 1, -1, -1, 1, 1};
+  const int n = min(bigctr[BIG_NBIG], bigcap);
+  unsigned long long* SW = reinterpret_cast<unsigned long long*>(sw);
+  for (int e = blockIdx.x; e < n; e += gridDim.x) {
+    const int2 ab = biglist[e];
+    const InstParams& P = PP[ab.x];
+    const int size = bsize[P.off + ab.y];
+    const long long sbase = (long long)FSX_SLAB_CAP * (P.off + soff[P.off + ab.y]);
+    const int C = big_cap(size), room = FSX_SLAB_CAP * size - C;
+    if (threadIdx.x == 0) s_cnt = 0;
+    __syncthreads();
+    // occupied slots, compacted after the table: B, first meeting, pass node, pass direction, weight
+    for (int t = threadIdx.x; t < C; t += 256) {
+      const int B = sh[sbase + t];
+      if (B < 0) continue;
+      const int x = atomicAdd(&s_cnt, 1);
+      if (x >= room) continue;
+      const long long o = sbase + C + x;
+      sh[o] = B;
+      sp[o] = sp[sbase + t];
+      sB[o] = sB[sbase + t];
+      sq[o] = sq[sbase + t] & 7;
+      const unsigned long long kw = SW[sbase + t];
+      sw[o] = __longlong_as_double((long long)((kw & 0x8000000000000000ull) ? (kw & 0x7fffffffffffffffull) : ~kw));
+    }
+    __syncthreads();
+    const int D = s_cnt;
+    if (D > room || 2 * D > C) {
+      if (threadIdx.x == 0) atomicOr(err + ab.x, FSX_ERR_NCONN);
+      __syncthreads();
+      continue;
+    }
+    const long long tb = sbase + C;
+    for (int x = threadIdx.x; x < D; x += 256) {
+      const int key = sp[tb + x];
+      int t = 0;
+      for (int y = 0; y < D; y++) t += sp[tb + y] < key;
+      const int v = sB[tb + x], k = sq[tb + x];
+      const int i = v % P.nx, jv = v / P.nx;
+      sB[sbase + t] = sh[tb + x];
+      sw[sbase + t] = sw[tb + x];
+      sp[sbase + t] = v;
+      sq[sbase + t] = (jv + JJ[k]) * P.nx + (i + II[k]);
+    }
+    if (threadIdx.x == 0) ecount[P.off + ab.y] = D;
+    __syncthreads();
+  }
+}
+
+void launch_lm_big(DevState& S, hipStream_t st) {
+  const int nb = S.nchunks, rcap = (int)(S.ntot / 32 + S.ninst + 64), bigcap = (int)(S.ntot / WALK_BIG + S.ninst + 1);
+  k_big_rulers<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.wnode, S.rec, S.bsize, S.wro, S.rjmp, S.rlist, S.bigctr, rcap,
+                                         S.err);
+  for (int it = 0; it < BIG_ROUNDS; it++) k_big_jump<<<1024, 256, 0, st>>>(S.P, S.rlist, S.rjmp, S.bigctr, rcap, it);
+  k_big_rank<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.wnode, S.bsize, S.wro, S.rjmp, S.wrank);
+  k_big_init<<<512, 256, 0, st>>>(S.P, S.biglist, S.bigctr, bigcap, S.bsize, S.soff, S.sB, S.sp, S.sq, S.sw, S.sh);
+  for (int pass = 0; pass < 3; pass++)
+    k_big_cand<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.wnode, S.wmask, S.bsize, S.soff, S.wrank, S.sB, S.sp, S.sq,
+                                         S.sw, S.sh, S.err, pass);
+  k_big_final<<<512, 256, 0, st>>>(S.P, S.biglist, S.bigctr, bigcap, S.bsize, S.soff, S.sB, S.sp, S.sq, S.sw, S.sh,
+                                   S.ecount, S.err);
 }
 
 // edge offsets: basin order, open basins (except basin0) one virtual edge each
@@ -1464,7 +1730,9 @@ void launch_lm_basins(DevState& S, hipStream_t st) {
   launch_fill_int(S.bsize, S.ntot, 0, st);
   k_basin_label<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.J, S.rootrank, S.bc, S.ht, S.dmask, S.rec, S.basin,
                                           S.bsize, S.wnode);
-  k_bsize_scan<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.nbasins, S.bsize, S.soff, S.ecount);
+  launch_fill_int(S.bigctr, BIG_FLAG0 + BIG_ROUNDS, 0, st);
+  k_bsize_scan<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.nbasins, S.bsize, S.soff, S.ecount, S.biglist, S.bigctr,
+                                           (int)(S.ntot / WALK_BIG + S.ninst + 1));
   k_walk_mask<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.basin, S.wmask);
 }
 
@@ -1472,6 +1740,7 @@ void launch_lm_walk(DevState& S, hipStream_t st) {
   k_basin_walk<<<xcd_grid(S.ninst, S.walk_gx), 64, 0, st>>>(S.P, S.nbasins, S.bc, S.outlet, S.wnode, S.wmask,
                                                            S.soff, S.bsize, S.ecount, S.sB, S.sp, S.sq, S.sw, S.sh,
                                                            S.err, S.ninst, S.walk_gx);
+  launch_lm_big(S, st);
 }
 
 void launch_lm_edges(DevState& S, hipStream_t st) {

@@ -50,6 +50,12 @@ struct DevState {
   SplRec32* srec = nullptr;   // per-position stream-power records
   WalkNode* wnode = nullptr;  // basin walk: (h, basin, preorder successor) per node
   uint8_t* wmask = nullptr;   // basin walk: neighbours k (bit k) that the pass search of a node examines
+  // pass search of basins too large for one thread's walk: (ruler << 32 | offset) per node, (next ruler <<
+  // 32 | nodes up to it) per ruler, preorder rank per node, the ruler and big-basin lists and counters
+  unsigned long long *wro = nullptr, *rjmp = nullptr;
+  int* wrank = nullptr;
+  int2 *rlist = nullptr, *biglist = nullptr;
+  int* bigctr = nullptr;
 
   // ---- per basin (node offsets) ----
   int *outlet = nullptr, *bsize = nullptr, *soff = nullptr, *ecount = nullptr, *eoff = nullptr;

@@ -1004,7 +1004,7 @@ __device__ __forceinline__ void flow_load(FlowRec& r, const SplRec32* srec, long
 
 // tiles of one landscape that a workgroup can mark as its own (one bit each): landscapes with more tiles
 // run on one workgroup
-#define FLOW_OWN_WORDS 2048
+#define FLOW_OWN_WORDS 4096
 static size_t spl_flow_smem() {
   return (size_t)SPL_R * (sizeof(int) + 3 * sizeof(double)) + FLOW_OWN_WORDS * sizeof(unsigned);
 }

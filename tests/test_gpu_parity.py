@@ -102,16 +102,24 @@ def _bowl(n, seed, depth, rad, noise):
     return (100.0 - depth * np.exp(-r2) + noise * rng.random((n, n))).ravel()
 
 
-@pytest.mark.parametrize("n,depth,rad,noise,min_edges", [(65, 20.0, 0.3, 0.5, 100), (97, 30.0, 0.25, 1.0, 20)])
+@pytest.mark.parametrize("n,depth,rad,noise,min_edges", [(65, 20.0, 0.3, 0.5, 100), (97, 30.0, 0.25, 1.0, 20),
+                                                         (161, 30.0, 0.3, 0.5, 30), (257, 60.0, 0.3, 0.3, 100)])
 def test_routing_basin_with_many_neighbours(fs, oracle, n, depth, rad, noise, min_edges):
     """One closed basin borders dozens of others, so its pass table outgrows the walk's per-thread
-    table and is found through the hash index: receivers, lengths and donors stay bit-exact."""
+    table and is found through the hash index; at 161 and 257 nodes a side the bowl holds more than
+    2048 nodes and is searched node-parallel (preorder ranks, hash table by atomics): receivers,
+    lengths and donors stay bit-exact."""
     L = landscape(n, n, seed=5, ibc=1111)
     L["h"] = _bowl(n, 5, depth, rad, noise)
     dx = L["xl"] / (n - 1)
     _, cb, cn, _ = oracle.connect_basins(L["h"], n, n, dx, dx, 1111)
     real = cn[:, 0] >= 0
     assert np.bincount(cb[real, 0]).max() >= min_edges  # the case the test is for
+    if n >= 161:  # the bowl is a basin of more than 2048 nodes (roots of the steepest-descent forest)
+        root, _ = oracle.find_receiver(L["h"], n, n, dx, dx, 1111)
+        for _ in range(64):
+            root = root[root]
+        assert np.bincount(root).max() > 2048
     rec, length, ndon, don, area, hw, order = _route_gpu(fs, L)
     r_rec, r_len, r_ndon, r_don, _, _ = oracle.local_minima(L["h"], n, n, dx, dx, 1111)
     assert np.array_equal(rec, r_rec), f"receivers differ at {np.flatnonzero(rec != r_rec)[:10]}"
