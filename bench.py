@@ -73,7 +73,7 @@ def _sync():
 
 GRAPHS = True  # replay captured HIP graphs (off: launch every kernel directly, for kernel-trace profilers)
 TUNE = True    # run the launch-shape autotuner after the warm-up steps of a batch
-SHAPES = None  # or launch shapes given on the command line (walk_gx, jump_blocks, tour_gx, nr_small_gx, area_wg, bor_gx[, flow_wg, flow_split])
+SHAPES = None  # or launch shapes given on the command line (walk_gx, jump_blocks, tour_gx, nr_small_gx, area_wg, bor_gx[, flow_wg, flow_split, walk_big, mfd_split])
 
 
 def open_batch(fs, Ls, lanes=1):

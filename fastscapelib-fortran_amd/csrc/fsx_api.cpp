@@ -505,7 +505,7 @@ static int ctx_alloc(long long capacity, int nslots, fsx_ctx** out) {
   A(J, N); A(ngroups, ninst); A(grec, 2 * (N + ninst)); A(gmeta, N + ninst); A(pos, N); A(lcur, N + ninst); A(rp, N); A(LSl, N); A(dn, 8 * N);
   A(A0, N); A(Al, N); A(F, N); A(HTl, N); A(Hl, N); A(Wl, N); A(HWl, N); A(HPl, N); A(DHl, N); A(ELl, N);
   A(arec, N); A(srec, N); A(wnode, N); A(wmask, N);
-  A(wro, N); A(rjmp, N); A(wrank, N); A(rlist, N / 32 + ninst + 64); A(biglist, N / 2048 + ninst + 1); A(bigctr, 64);
+  A(wro, N); A(rjmp, N); A(wrank, N); A(wseq, N); A(rlist, N); A(rchunk, c->chunk_cap); A(biglist, N / 64 + ninst + 1); A(bigctr, 64);
 #undef A
   if (rc) { fsx_destroy(c); return rc; }
   *out = c;
@@ -1425,6 +1425,8 @@ extern "C" int fsx_autotune(fsx_ctx* c, int reps, int* chosen) {
       {&DevState::bor_gx, {2, 4, 8, 16}, 4, PH_LM_MST, -1},
       {&DevState::flow_wg, {256, 512, 1024, 0}, 3, PH_SPL_SWEEP, -1},
       {&DevState::flow_split, {0, 262144, 131072, 65536}, 4, PH_SPL_SWEEP, -1},
+      {&DevState::walk_big, {64, 256, 1024, 8192}, 4, PH_LM_WALK, -1},
+      {&DevState::mfd_split, {262144, 131072, 65536, 32768}, 4, PH_MFD_ROUTE, -1},
   };
   // keep the caller's profiling state and accumulated timers
   if (int r = flush_timers_any(c, c->stream)) return r;
@@ -1439,6 +1441,7 @@ extern "C" int fsx_autotune(fsx_ctx* c, int reps, int* chosen) {
     // without the step's per-landscape outputs (Newton counts) and without updating h
     const bool spl = k.ph0 == PH_SPL_SWEEP;
     if (spl && !c->any_flow) continue;
+    if (k.ph0 == PH_MFD_ROUTE && !c->any_mfd) continue;
     auto pass = [&]() {
       enqueue_routing(c, c->stream, 0);
       if (spl) {
@@ -1449,7 +1452,7 @@ extern "C" int fsx_autotune(fsx_ctx* c, int reps, int* chosen) {
     };
     int best = c->S.*k.field;
     double best_ms = 1e300;
-    const bool split = k.field == &DevState::flow_split;
+    const bool split = k.field == &DevState::flow_split || k.field == &DevState::mfd_split;
     for (int i = 0; i < k.ncand && !rc; i++) {
       c->S.*k.field = k.cand[i];
       if (split) {
@@ -1496,6 +1499,8 @@ extern "C" int fsx_get_tuning(fsx_ctx* c, int* out) {
   out[5] = c->S.bor_gx;
   out[6] = c->S.flow_wg;
   out[7] = c->S.flow_split;
+  out[8] = c->S.walk_big;
+  out[9] = c->S.mfd_split;
   return 0;
 }
 
@@ -1510,6 +1515,8 @@ extern "C" int fsx_set_tuning(fsx_ctx* c, const int* in) {
   if (in[6] != 256 && in[6] != 512 && in[6] != 1024)
     return fail(FSX_EINVAL, "fsx_set_tuning: flow_wg must be 256, 512 or 1024");
   if (in[7] < 0) return fail(FSX_EINVAL, "fsx_set_tuning: flow_split must be >= 0");
+  if (in[8] < 64) return fail(FSX_EINVAL, "fsx_set_tuning: walk_big must be >= 64");
+  if (in[9] < 1024) return fail(FSX_EINVAL, "fsx_set_tuning: mfd_split must be >= 1024");
   c->S.walk_gx = in[0];
   c->S.jump_blocks = in[1];
   c->S.tour_gx = in[2];
@@ -1519,6 +1526,9 @@ extern "C" int fsx_set_tuning(fsx_ctx* c, const int* in) {
   c->S.flow_wg = in[6];
   if (c->S.flow_split != in[7]) c->dirty = true;  // the sweep's workgroup table follows at the next upload
   c->S.flow_split = in[7];
+  c->S.walk_big = in[8];
+  if (c->S.mfd_split != in[9]) c->dirty = true;  // so does the multiple-flow sweeps' table
+  c->S.mfd_split = in[9];
   c->graph_valid = false;  // captured launches carry the old shapes
   return 0;
 }

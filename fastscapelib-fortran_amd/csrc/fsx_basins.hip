@@ -26,13 +26,13 @@
   const int v = c.start + (int)threadIdx.x;  \
   const bool live = P.active && P.run_spl && v < P.nn;
 
-// Closed basins of more than WALK_BIG nodes are searched node-parallel (k_big_*), since one thread's walk
-// takes time in proportion to its basin.
-#define WALK_BIG 2048
+// Closed basins of more than `big` nodes (DevState::walk_big, at least 64) are searched node-parallel
+// (k_big_*), since one thread's walk takes time in proportion to its basin.
 #define BIG_NBIG 0    // bigctr[0]: big basins listed
 #define BIG_NRUL 1    // bigctr[1]: rulers listed
 #define BIG_FLAG0 2   // bigctr[2..]: convergence flags of the ruler jumps
 #define BIG_ROUNDS 24
+#define BIG_NCTR (BIG_FLAG0 + BIG_ROUNDS)
 
 __global__ void k_fill_int(int* __restrict__ a, long long n, int val) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
@@ -178,11 +178,11 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_basin_label(const InstParams* __r
 }
 
 // exclusive scan of closed-basin sizes -> candidate slab offsets; clears edge counts; lists the closed
-// basins of more than WALK_BIG nodes
+// basins of more than `big` nodes
 __global__ __launch_bounds__(FSX_WG) void k_bsize_scan(const InstParams* __restrict__ PP, const int* __restrict__ nbasins,
                                                        const int* __restrict__ bsize, int* __restrict__ soff,
                                                        int* __restrict__ ecount, int2* __restrict__ biglist,
-                                                       int* __restrict__ bigctr, int bigcap) {
+                                                       int* __restrict__ bigctr, int bigcap, int big) {
   __shared__ int lds[FSX_WG / 64];
   const int inst = blockIdx.x;
   const InstParams& P = PP[inst];
@@ -197,7 +197,7 @@ __global__ __launch_bounds__(FSX_WG) void k_bsize_scan(const InstParams* __restr
     if (b < nb) {
       soff[P.off + b] = carry + incl - x;
       ecount[P.off + b] = 0;
-      if (x > WALK_BIG) {  // searched by the big-basin kernels, not by the walk
+      if (x > big) {  // searched by the big-basin kernels, not by the walk
         const int e = atomicAdd(bigctr + BIG_NBIG, 1);
         if (e < bigcap) biglist[e] = make_int2(inst, b);
       }
@@ -245,7 +245,7 @@ void k_basin_walk(const InstParams* __restrict__ PP, const int* __restrict__ nba
                                                    const int* __restrict__ soff, const int* __restrict__ bsize,
                                                    int* __restrict__ ecount, int* __restrict__ sB, int* __restrict__ sp,
                                                    int* __restrict__ sq, double* __restrict__ sw, int* __restrict__ sh,
-                                                   int* __restrict__ err, int ninst, int gx) {
+                                                   int* __restrict__ err, int ninst, int gx, int big) {
   __shared__ int lB[WALK_TCAP * 64];
   __shared__ double lW[WALK_TCAP * 64];
   int* LB = lB + threadIdx.x;
@@ -276,7 +276,7 @@ void k_basin_walk(const InstParams* __restrict__ PP, const int* __restrict__ nba
     for (b += STRIDE; b < nb; b += STRIDE) {
       const int o = outlet[off + b];
       if (bc[off + o]) continue;  // open basin: no scan
-      if (bsize[off + b] > WALK_BIG) continue;  // k_big_* search it
+      if (bsize[off + b] > big) continue;  // k_big_* search it
       v = o;
       ne = 0;
       visits = 0;
@@ -389,75 +389,122 @@ void k_basin_walk(const InstParams* __restrict__ PP, const int* __restrict__ nba
 // (preorder rank, k) at which it meets B -- the edge order -- and the first (rank, k) among the pairs of
 // least weight max(h(v), h(q)) -- the pass.  Both are minima over (rank, k), so once every node knows its
 // rank in A's depth-first preorder, the nodes of A can be searched in any order:
-//  1. ranks: rulers (A's root and nodes picked by a hash, about one in 64) walk the preorder successors
+//  1. ranks: rulers (A's root and nodes picked by a hash, about one in 32) walk the preorder successors
 //     up to the next ruler, giving every node its ruler and offset; pointer jumping over the rulers gives
 //     each its distance to the end of A's preorder, so rank = size(A) - that distance + offset;
 //  2. a hash table of the basins B met, in A's candidate slab (capacity the largest power of two not
-//     above 4 size(A)), takes the minima by atomics in three node-parallel passes: the first meeting and
-//     the least weight, then the first (rank, k) of least weight, then that pass's node;
+//     above 4 size(A)), takes the minima by atomics in two node-parallel passes: the first meeting and
+//     the least weight, then the first (rank, k) of least weight (its node is the basin's preorder at
+//     that rank);
 //  3. one workgroup per big basin ranks the table entries by first meeting and writes them out as the walk
 //     does (neighbour basin, weight, pass nodes at the slab's start, count in ecount).
-__device__ __forceinline__ bool big_ruler_pick(int v) { return (((unsigned)v * 0x9E3779B1u) >> 26) == 0u; }
+__device__ __forceinline__ bool big_ruler_pick(int v) { return (((unsigned)v * 0x9E3779B1u) >> 27) == 0u; }
 __device__ __forceinline__ int big_cap(int size) {  // largest power of two <= 4 size
   return (int)(1u << (31 - __clz((unsigned)(4 * size))));
 }
 #define BIG_NONE 0xffffffffu
 
-// rulers of big basins walk their segment of the preorder; rulers are listed for the jumps
-__global__ __launch_bounds__(FSX_CHUNK) void k_big_rulers(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
-                                                          const WalkNode* __restrict__ wnode, const int* __restrict__ rec,
-                                                          const int* __restrict__ bsize, unsigned long long* __restrict__ wro,
-                                                          unsigned long long* __restrict__ rjmp, int2* __restrict__ rlist,
-                                                          int* __restrict__ bigctr, int rcap, int* __restrict__ err) {
+// Rulers of big basins, listed for the segment walks and the jumps without a shared counter: a count per
+// chunk (pass 0), an exclusive scan of the counts (k_big_rscan), the list entries (pass 1).  An entry is
+// (chunk << 8 | lane), so the list holds at most one entry per node.
+__device__ __forceinline__ bool big_is_ruler(const InstParams& P, int v, bool live, const WalkNode* wnode,
+                                             const int* rec, const int* bsize, int big) {
+  if (!live) return false;
+  const int bv = wnode[P.off + v].basin;
+  return bv >= 0 && bsize[P.off + bv] > big && (rec[P.off + v] == v || big_ruler_pick(v));
+}
+
+__global__ __launch_bounds__(FSX_CHUNK) void k_big_pick(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
+                                                        const WalkNode* __restrict__ wnode, const int* __restrict__ rec,
+                                                        const int* __restrict__ bsize, int* __restrict__ rchunk,
+                                                        int* __restrict__ rlist, int big, int pass) {
+  __shared__ int s_w[FSX_CHUNK / 64];
   NODE_PROLOGUE
-  const ulonglong2* WN = reinterpret_cast<const ulonglong2*>(wnode) + P.off;
-  bool ruler = false;
-  if (live) {
-    const int bv = (int)(unsigned)(WN[v].y & 0xffffffffull);
-    ruler = bv >= 0 && bsize[P.off + bv] > WALK_BIG && (rec[P.off + v] == v || big_ruler_pick(v));
-  }
-  const int lane = threadIdx.x & 63;
+  const bool ruler = big_is_ruler(P, v, live, wnode, rec, bsize, big);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const unsigned long long m = __ballot(ruler);
-  if (m) {
-    const int first = __ffsll((long long)m) - 1;
-    int base = 0;
-    if (lane == first) base = atomicAdd(bigctr + BIG_NRUL, __popcll(m));
-    base = __shfl(base, first, 64);
-    const int idx = base + __popcll(m & ((1ull << lane) - 1ull));
-    if (ruler) {
-      if (idx < rcap) rlist[idx] = make_int2(c.inst, v);
-      else atomicOr(err + c.inst, FSX_ERR_NCONN);
+  if (lane == 0) s_w[wid] = __popcll(m);
+  __syncthreads();
+  const int cid = chunk_id();
+  if (pass == 0) {
+    if (threadIdx.x == 0) {
+      int n = 0;
+      for (int w = 0; w < FSX_CHUNK / 64; w++) n += s_w[w];
+      rchunk[cid] = n;
     }
+    return;
   }
   if (!ruler) return;
-  int u = v;
-  for (unsigned d = 0; d <= (unsigned)P.nn; d++) {
-    wro[P.off + u] = ((unsigned long long)(unsigned)v << 32) | d;
-    const int su = (int)(unsigned)(WN[u].y >> 32);
-    if (su < 0 || big_ruler_pick(su)) {
-      rjmp[P.off + v] = ((unsigned long long)(su < 0 ? BIG_NONE : (unsigned)su) << 32) | (d + 1u);
-      return;
-    }
-    u = su;
+  int base = rchunk[cid];
+  for (int w = 0; w < wid; w++) base += s_w[w];
+  rlist[base + __popcll(m & ((1ull << lane) - 1ull))] = (cid << 8) | (int)threadIdx.x;
+}
+
+// exclusive scan of the per-chunk ruler counts (one workgroup); the total is the list's length
+__global__ __launch_bounds__(FSX_WG) void k_big_rscan(int* __restrict__ rchunk, int nchunks, int* __restrict__ bigctr) {
+  __shared__ int lds[FSX_WG / 64];
+  int carry = 0;
+  for (int c0 = 0; c0 < nchunks; c0 += FSX_WG) {
+    const int c = c0 + (int)threadIdx.x;
+    const int x = c < nchunks ? rchunk[c] : 0;
+    int tot;
+    const int incl = block_incl_scan<FSX_WG>(x, lds, tot);
+    if (c < nchunks) rchunk[c] = carry + incl - x;
+    carry += tot;
   }
-  atomicOr(err + c.inst, FSX_ERR_LEVELS);  // the successors do not end: not a tree
+  if (threadIdx.x == 0) bigctr[BIG_NRUL] = carry;
+}
+
+// the ruler of list entry e: (landscape, node)
+__device__ __forceinline__ int2 big_ruler(const Chunk* chunks, int code) {
+  const Chunk c = chunks[code >> 8];
+  return make_int2(c.inst, c.start + (code & 255));
+}
+
+// each listed ruler walks the preorder successors up to the next ruler: ruler and offset of the nodes on
+// the way, (next ruler, nodes up to it) of the ruler
+__global__ __launch_bounds__(256) void k_big_seg(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
+                                                 const int* __restrict__ rlist, const int* __restrict__ bigctr,
+                                                 const WalkNode* __restrict__ wnode, unsigned long long* __restrict__ wro,
+                                                 unsigned long long* __restrict__ rjmp, int* __restrict__ err) {
+  const int n = bigctr[BIG_NRUL];
+  for (int e = blockIdx.x * 256 + (int)threadIdx.x; e < n; e += gridDim.x * 256) {
+    const int2 r = big_ruler(chunks, rlist[e]);
+    const InstParams& P = PP[r.x];
+    const ulonglong2* WN = reinterpret_cast<const ulonglong2*>(wnode) + P.off;
+    const int v = r.y;
+    int u = v;
+    bool done = false;
+    for (unsigned d = 0; d <= (unsigned)P.nn; d++) {
+      wro[P.off + u] = ((unsigned long long)(unsigned)v << 32) | d;
+      const int su = (int)(unsigned)(WN[u].y >> 32);
+      if (su < 0 || big_ruler_pick(su)) {
+        rjmp[P.off + v] = ((unsigned long long)(su < 0 ? BIG_NONE : (unsigned)su) << 32) | (d + 1u);
+        done = true;
+        break;
+      }
+      u = su;
+    }
+    if (!done) atomicOr(err + r.x, FSX_ERR_LEVELS);  // the successors do not end: not a tree
+  }
 }
 
 // (next ruler, nodes up to it) -> (end, nodes to the end of the preorder), two hops per launch
-__global__ __launch_bounds__(256) void k_big_jump(const InstParams* __restrict__ PP, const int2* __restrict__ rlist,
-                                                  unsigned long long* __restrict__ rjmp, int* __restrict__ bigctr,
-                                                  int rcap, int iter) {
+__global__ __launch_bounds__(256) void k_big_jump(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
+                                                  const int* __restrict__ rlist, unsigned long long* __restrict__ rjmp,
+                                                  int* __restrict__ bigctr, int iter) {
   if (iter > 0 && bigctr[BIG_FLAG0 + iter - 1] == 0) return;
-  const int n = min(bigctr[BIG_NRUL], rcap);
+  const int n = bigctr[BIG_NRUL];
   bool changed = false;
-  for (int i = blockIdx.x * 256 + (int)threadIdx.x; i < n; i += gridDim.x * 256) {
-    const int2 r = rlist[i];
+  for (int e = blockIdx.x * 256 + (int)threadIdx.x; e < n; e += gridDim.x * 256) {
+    const int2 r = big_ruler(chunks, rlist[e]);
+    const int nn = PP[r.x].nn;
     unsigned long long* J = rjmp + PP[r.x].off;
     unsigned long long w = J[r.y];
 #pragma unroll
     for (int hop = 0; hop < 2; hop++) {
       const unsigned x = (unsigned)(w >> 32);
-      if (x == BIG_NONE) break;
+      if (x >= (unsigned)nn) break;  // the end (BIG_NONE)
       const unsigned long long wx = J[x];
       w = (wx & 0xffffffff00000000ull) | (unsigned long long)((unsigned)w + (unsigned)wx);
       J[r.y] = w;
@@ -467,20 +514,29 @@ __global__ __launch_bounds__(256) void k_big_jump(const InstParams* __restrict__
   if (changed) bigctr[BIG_FLAG0 + iter] = 1;
 }
 
-// preorder rank of every node of a big basin
+// preorder rank of every node of a big basin, and the node at each rank
 __global__ __launch_bounds__(FSX_CHUNK) void k_big_rank(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
                                                         const WalkNode* __restrict__ wnode, const int* __restrict__ bsize,
                                                         const unsigned long long* __restrict__ wro,
-                                                        const unsigned long long* __restrict__ rjmp, int* __restrict__ wrank) {
+                                                        const unsigned long long* __restrict__ rjmp, const int* __restrict__ soff,
+                                                        int* __restrict__ wrank, int* __restrict__ wseq, int* __restrict__ err,
+                                                        int big) {
   NODE_PROLOGUE
   if (!live) return;
   const int bv = wnode[P.off + v].basin;
   if (bv < 0) return;
   const int sz = bsize[P.off + bv];
-  if (sz <= WALK_BIG) return;
+  if (sz <= big) return;
   const unsigned long long ro = wro[P.off + v];
   const unsigned long long rj = rjmp[P.off + (int)(unsigned)(ro >> 32)];
-  wrank[P.off + v] = sz - (int)(unsigned)(rj & 0xffffffffull) + (int)(unsigned)(ro & 0xffffffffull);
+  const int rank = sz - (int)(unsigned)(rj & 0xffffffffull) + (int)(unsigned)(ro & 0xffffffffull);
+  if (rank < 0 || rank >= sz) {  // cannot happen for a tree; keeps every index below in bounds
+    atomicOr(err + c.inst, FSX_ERR_LEVELS);
+    wrank[P.off + v] = 0;
+    return;
+  }
+  wrank[P.off + v] = rank;
+  wseq[P.off + soff[P.off + bv] + rank] = v;  // the basin's preorder, in its own slice of node offsets
 }
 
 // empty tables of the big basins (one workgroup per listed basin)
@@ -504,13 +560,13 @@ __global__ __launch_bounds__(256) void k_big_init(const InstParams* __restrict__
   }
 }
 
-// pass 0: first meeting and least weight; 1: first (rank, k) of least weight; 2: that pass's node
+// pass 0: first meeting and least weight; 1: first (rank, k) of least weight
 __global__ __launch_bounds__(FSX_CHUNK) void k_big_cand(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
                                                         const WalkNode* __restrict__ wnode, const uint8_t* __restrict__ wmask,
                                                         const int* __restrict__ bsize, const int* __restrict__ soff,
                                                         const int* __restrict__ wrank, int* __restrict__ sB,
                                                         int* __restrict__ sp, int* __restrict__ sq, double* __restrict__ sw,
-                                                        int* __restrict__ sh, int* __restrict__ err, int pass) {
+                                                        int* __restrict__ sh, int* __restrict__ err, int pass, int big) {
   NODE_PROLOGUE
   if (!live) return;
   const unsigned mv = wmask[P.off + v];
@@ -520,7 +576,7 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_big_cand(const InstParams* __rest
   const int A = (int)(unsigned)(wv.y & 0xffffffffull);
   if (A < 0) return;
   const int sz = bsize[P.off + A];
-  if (sz <= WALK_BIG) return;
+  if (sz <= big) return;
   constexpr int II[8] = {-1, 1, 0, 0, -1, 1, -1, 1};
   constexpr int JJ[8] = {0, 0, -1, 1, -1, -1, 1, 1};
   const double hv = __longlong_as_double((long long)wv.x);
@@ -542,7 +598,8 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_big_cand(const InstParams* __rest
     unsigned s = ((unsigned)B * 2654435761u) & (C - 1u);
     long long slot = -1;
     for (unsigned pr = 0; pr < C; pr++, s = (s + 1u) & (C - 1u)) {
-      const int cur = pass == 0 ? atomicCAS(sh + sbase + s, -1, B) : sh[sbase + s];
+      int cur = sh[sbase + s];
+      if (pass == 0 && cur == -1) cur = atomicCAS(sh + sbase + s, -1, B);
       if (cur == B || (pass == 0 && cur == -1)) {
         slot = sbase + s;
         break;
@@ -553,13 +610,12 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_big_cand(const InstParams* __rest
       atomicOr(err + c.inst, FSX_ERR_NCONN);
       continue;
     }
+    // the atomics only where they can lower the value (most candidates of a busy pair cannot)
     if (pass == 0) {
-      atomicMin(sp + slot, key);
-      atomicMin(SW + slot, wk);
-    } else if (pass == 1) {
-      if (SW[slot] == wk) atomicMin(sq + slot, key);
-    } else {
-      if (sq[slot] == key) sB[slot] = v;
+      if (key < sp[slot]) atomicMin(sp + slot, key);
+      if (wk < SW[slot]) atomicMin(SW + slot, wk);
+    } else if (SW[slot] == wk && key < sq[slot]) {
+      atomicMin(sq + slot, key);
     }
   }
 }
@@ -569,7 +625,8 @@ __global__ __launch_bounds__(256) void k_big_final(const InstParams* __restrict_
                                                    const int* __restrict__ bigctr, int bigcap, const int* __restrict__ bsize,
                                                    const int* __restrict__ soff, int* __restrict__ sB, int* __restrict__ sp,
                                                    int* __restrict__ sq, double* __restrict__ sw, int* __restrict__ sh,
-                                                   int* __restrict__ ecount, int* __restrict__ err) {
+                                                   const int* __restrict__ wseq, int* __restrict__ ecount,
+                                                   int* __restrict__ err) {
   __shared__ int s_cnt;
   constexpr int II[8] = {-1, 1, 0, 0, -1, 1, -1, 1};
   constexpr int JJ[8] = {0, 0, -1, 1, -1, -1, 1, 1};
@@ -592,8 +649,9 @@ __global__ __launch_bounds__(256) void k_big_final(const InstParams* __restrict_
       const long long o = sbase + C + x;
       sh[o] = B;
       sp[o] = sp[sbase + t];
-      sB[o] = sB[sbase + t];
-      sq[o] = sq[sbase + t] & 7;
+      const int pk = sq[sbase + t];  // rank * 8 + k of the pass (a rank below size by k_big_rank)
+      sB[o] = wseq[P.off + soff[P.off + ab.y] + min(max(pk >> 3, 0), size - 1)];
+      sq[o] = pk & 7;
       const unsigned long long kw = SW[sbase + t];
       sw[o] = __longlong_as_double((long long)((kw & 0x8000000000000000ull) ? (kw & 0x7fffffffffffffffull) : ~kw));
     }
@@ -622,17 +680,20 @@ __global__ __launch_bounds__(256) void k_big_final(const InstParams* __restrict_
 }
 
 void launch_lm_big(DevState& S, hipStream_t st) {
-  const int nb = S.nchunks, rcap = (int)(S.ntot / 32 + S.ninst + 64), bigcap = (int)(S.ntot / WALK_BIG + S.ninst + 1);
-  k_big_rulers<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.wnode, S.rec, S.bsize, S.wro, S.rjmp, S.rlist, S.bigctr, rcap,
-                                         S.err);
-  for (int it = 0; it < BIG_ROUNDS; it++) k_big_jump<<<1024, 256, 0, st>>>(S.P, S.rlist, S.rjmp, S.bigctr, rcap, it);
-  k_big_rank<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.wnode, S.bsize, S.wro, S.rjmp, S.wrank);
+  const int nb = S.nchunks, bigcap = (int)(S.ntot / 64 + S.ninst + 1);
+  k_big_pick<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.wnode, S.rec, S.bsize, S.rchunk, S.rlist, S.walk_big, 0);
+  k_big_rscan<<<1, FSX_WG, 0, st>>>(S.rchunk, nb, S.bigctr);
+  k_big_pick<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.wnode, S.rec, S.bsize, S.rchunk, S.rlist, S.walk_big, 1);
+  k_big_seg<<<1024, 256, 0, st>>>(S.P, S.chunks, S.rlist, S.bigctr, S.wnode, S.wro, S.rjmp, S.err);
+  for (int it = 0; it < BIG_ROUNDS; it++) k_big_jump<<<1024, 256, 0, st>>>(S.P, S.chunks, S.rlist, S.rjmp, S.bigctr, it);
+  k_big_rank<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.wnode, S.bsize, S.wro, S.rjmp, S.soff, S.wrank, S.wseq, S.err,
+                                       S.walk_big);
   k_big_init<<<512, 256, 0, st>>>(S.P, S.biglist, S.bigctr, bigcap, S.bsize, S.soff, S.sB, S.sp, S.sq, S.sw, S.sh);
-  for (int pass = 0; pass < 3; pass++)
+  for (int pass = 0; pass < 2; pass++)
     k_big_cand<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.wnode, S.wmask, S.bsize, S.soff, S.wrank, S.sB, S.sp, S.sq,
-                                         S.sw, S.sh, S.err, pass);
+                                         S.sw, S.sh, S.err, pass, S.walk_big);
   k_big_final<<<512, 256, 0, st>>>(S.P, S.biglist, S.bigctr, bigcap, S.bsize, S.soff, S.sB, S.sp, S.sq, S.sw, S.sh,
-                                   S.ecount, S.err);
+                                   S.wseq, S.ecount, S.err);
 }
 
 // edge offsets: basin order, open basins (except basin0) one virtual edge each
@@ -1730,16 +1791,16 @@ void launch_lm_basins(DevState& S, hipStream_t st) {
   launch_fill_int(S.bsize, S.ntot, 0, st);
   k_basin_label<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.J, S.rootrank, S.bc, S.ht, S.dmask, S.rec, S.basin,
                                           S.bsize, S.wnode);
-  launch_fill_int(S.bigctr, BIG_FLAG0 + BIG_ROUNDS, 0, st);
+  launch_fill_int(S.bigctr, BIG_NCTR, 0, st);
   k_bsize_scan<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.nbasins, S.bsize, S.soff, S.ecount, S.biglist, S.bigctr,
-                                           (int)(S.ntot / WALK_BIG + S.ninst + 1));
+                                           (int)(S.ntot / 64 + S.ninst + 1), S.walk_big);
   k_walk_mask<<<nb, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.basin, S.wmask);
 }
 
 void launch_lm_walk(DevState& S, hipStream_t st) {
   k_basin_walk<<<xcd_grid(S.ninst, S.walk_gx), 64, 0, st>>>(S.P, S.nbasins, S.bc, S.outlet, S.wnode, S.wmask,
                                                            S.soff, S.bsize, S.ecount, S.sB, S.sp, S.sq, S.sw, S.sh,
-                                                           S.err, S.ninst, S.walk_gx);
+                                                           S.err, S.ninst, S.walk_gx, S.walk_big);
   launch_lm_big(S, st);
 }
 

@@ -28,6 +28,7 @@ struct DevState {
   int flow_split = 131072; // stream-power dataflow sweep: nodes per extra workgroup (0: one per landscape)
   int nr_gx = 8;           // edge sort: workgroups per landscape for the big ranges of a level
   int mfd_split = 131072;  // multiple-flow dataflow sweeps: nodes per workgroup of a landscape
+  int walk_big = 2048;     // closed basins larger than this are searched node-parallel, not walked
 
   // ---- per node (ntot) ----
   double *h = nullptr, *b = nullptr, *u = nullptr, *kf = nullptr, *kd = nullptr, *precip = nullptr;
@@ -51,10 +52,12 @@ struct DevState {
   WalkNode* wnode = nullptr;  // basin walk: (h, basin, preorder successor) per node
   uint8_t* wmask = nullptr;   // basin walk: neighbours k (bit k) that the pass search of a node examines
   // pass search of basins too large for one thread's walk: (ruler << 32 | offset) per node, (next ruler <<
-  // 32 | nodes up to it) per ruler, preorder rank per node, the ruler and big-basin lists and counters
+  // 32 | nodes up to it) per ruler, preorder rank per node and node per rank (by basin slice), the ruler
+  // and big-basin lists and counters
   unsigned long long *wro = nullptr, *rjmp = nullptr;
-  int* wrank = nullptr;
-  int2 *rlist = nullptr, *biglist = nullptr;
+  int *wrank = nullptr, *wseq = nullptr;
+  int *rlist = nullptr, *rchunk = nullptr;  // rulers (chunk << 8 | lane), per-chunk offsets into the list
+  int2* biglist = nullptr;
   int* bigctr = nullptr;
 
   // ---- per basin (node offsets) ----

@@ -227,7 +227,8 @@ class Batch:
         k = -1 if name is None else _lib.PHASES.index(name)
         check(lib.fsx_time_phase(self._h, k), "time_phase")
 
-    TUNING = ("walk_gx", "jump_blocks", "tour_gx", "nr_small_gx", "area_wg", "bor_gx", "flow_wg", "flow_split")
+    TUNING = ("walk_gx", "jump_blocks", "tour_gx", "nr_small_gx", "area_wg", "bor_gx", "flow_wg", "flow_split",
+              "walk_big", "mfd_split")
 
     def autotune(self, reps=2):
         """Choose the launch shapes of the routing kernels and of the dataflow stream-power sweep by

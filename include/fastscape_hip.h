@@ -243,10 +243,12 @@ int fsx_reset_timers(fsx_ctx *ctx);
  * nodes per workgroup (timed on solves of the routed state that leave h and the per-step counts alone), with reps timed
  * passes per candidate, and keeps the fastest of each.  No result depends on these shapes.  Routing
  * read-outs (receivers, donors, area) then hold the tuning passes' routing of the current elevation
- * until the next step.  chosen (FSX_NTUNE ints, or NULL) receives walk_gx, jump_blocks, tour_gx,
- * nr_small_gx, area_wg, bor_gx, flow_wg, flow_split.  No reference counterpart: the reference has no
- * launch shapes. */
-#define FSX_NTUNE 8
+ * until the next step.  Also the basin size above which the pass search runs node-parallel instead of
+ * one thread walking the basin (walk_big), and, when some landscape routes by multiple flow directions,
+ * the nodes per workgroup of its dataflow sweeps (mfd_split).  chosen (FSX_NTUNE ints, or NULL)
+ * receives walk_gx, jump_blocks, tour_gx, nr_small_gx, area_wg, bor_gx, flow_wg, flow_split, walk_big,
+ * mfd_split.  No reference counterpart: the reference has no launch shapes. */
+#define FSX_NTUNE 10
 int fsx_autotune(fsx_ctx *ctx, int reps, int *chosen);
 /* The launch shapes in use (FSX_NTUNE ints, same order as fsx_autotune's chosen). */
 int fsx_get_tuning(fsx_ctx *ctx, int *out);
@@ -254,7 +256,8 @@ int fsx_get_tuning(fsx_ctx *ctx, int *out);
    this workload).  Each must be >= 1; walk blocks <= 4096, jump grid <= 2^20, tour blocks <= 1024,
    sort blocks <= 4096; area_wg one of 256, 512, 1024; bor_gx in [1, 1024]; flow_wg one of 256, 512,
    1024; flow_split >= 0 (nodes per workgroup of the dataflow sweep, 0: one workgroup per landscape; at
-   most 8 workgroups per landscape).  A rejected call changes nothing.
+   most 16 workgroups per landscape); walk_big >= 64; mfd_split >= 1024 (at most 32 workgroups per
+   landscape).  A rejected call changes nothing.
    No counterpart in the reference. */
 int fsx_set_tuning(fsx_ctx *ctx, const int *in);
 
