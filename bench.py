@@ -274,24 +274,31 @@ def readout_cpu_baseline(L):
     return cpu
 
 
-def serving_leg(fs, window, deadline, quantum, sample):
-    """Arrivals of the serving stream during `window` seconds, served in real time; the loop stops at
+def serving_leg(fs, window, deadline, quantum, sample, lanes=2):
+    """Arrivals of the serving stream during `window` seconds, served in real time by `lanes` pools (each
+    stepping on its own stream, quanta launched as soon as a pool's last one finished); the loop stops at
     `deadline` seconds (runs still resident or waiting then are reported as unfinished)."""
     from fastscapelib_fortran import serving
     stream = [x for x in W.serving_stream(duration=window)]
-    cap, per_node = serving.pool_capacity(0.5)
+    cap, per_node = serving.pool_capacity(0.8 / max(1, lanes))
     need = sum(L["nx"] * L["ny"] for _, _, L in stream)
     cap = int(min(cap, max(need, max(L["nx"] * L["ny"] for _, _, L in stream))))
     runs = [serving.Run(rid=i, nx=L["nx"], ny=L["ny"], steps=st, arrival=t,
                         setup=(lambda p, k, L=L: W.configure(p, L, k))) for i, (t, st, L) in enumerate(stream)]
     offered = sum(L["nx"] * L["ny"] * st for _, st, L in stream)
-    with fs.Pool(cap, slots=256) as p:
-        p.graphs(GRAPHS)
-        srv = serving.Server(p, quantum=quantum)
+    pools = [fs.Pool(cap, slots=256) for _ in range(max(1, lanes))]
+    try:
+        for p in pools:
+            p.graphs(GRAPHS)
+        srv = serving.Server(pools, quantum=quantum)
         rep = srv.serve(runs, realtime=True, deadline=deadline)
+    finally:
+        for p in pools:
+            p.close()
     out = {"workload": f"Poisson 20 runs/s for {window:g} s; grids log-uniform[65,2049]^2, steps log-uniform[50,2000], "
                        "dt log-uniform[1e3,1e4], D8/MFD 50/50, n in {1,1.5,2}, Newton tol 1e-8",
-           "runs": len(stream), "quantum": quantum, "pool_nodes": cap, "bytes_per_node": round(per_node, 1),
+           "runs": len(stream), "quantum": quantum, "lanes": len(pools), "pool_nodes": cap,
+           "bytes_per_node": round(per_node, 1),
            "offered_cell_steps": offered, "deadline_s": deadline}
     out.update({k: rep[k] for k in ("runs_finished", "runs_unfinished", "cell_steps", "wall_s", "cell_steps_per_s",
                                     "p50_latency_s", "p99_latency_s", "busy_cell_steps_per_s")})
@@ -427,6 +434,7 @@ def main():
     ap.add_argument("--serve-window", type=float, default=5.0)
     ap.add_argument("--serve-deadline", type=float, default=60.0)
     ap.add_argument("--quantum", type=int, default=8)
+    ap.add_argument("--serve-lanes", type=int, default=2, help="pools of the serving leg, each on its own stream")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every kernel directly instead of replaying HIP graphs (for kernel-trace profiling)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (no autotuner)")
@@ -499,7 +507,8 @@ def main():
         out["mixed"] = mixed_leg(fs, args.mixed_steps)
     if "serving" in legs:
         note("serving")
-        out["serving"] = serving_leg(fs, args.serve_window, args.serve_deadline, args.quantum, sample=6)
+        out["serving"] = serving_leg(fs, args.serve_window, args.serve_deadline, args.quantum, sample=6,
+                                     lanes=args.serve_lanes)
     if "advect" in legs:
         note("advection")
         out["advect"] = advect_leg(fs, 10, args.lanes)

@@ -1144,6 +1144,15 @@ extern "C" int fsx_synchronize(fsx_ctx* c) {
   return check_errors(c);
 }
 
+extern "C" int fsx_query(fsx_ctx* c) {
+  if (!c) return fail(FSX_EINVAL, "null context");
+  const hipError_t e = hipStreamQuery(c->stream);
+  if (e == hipErrorNotReady) return 0;
+  HIPCHK(e);
+  if (int r = check_errors(c)) return r;
+  return 1;
+}
+
 extern "C" int fsx_route(fsx_ctx* c, void* stream, int sync) {
   if (!c) return fail(FSX_EINVAL, "null context");
   hipStream_t st = stream ? (hipStream_t)stream : c->stream;

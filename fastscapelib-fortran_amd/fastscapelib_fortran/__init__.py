@@ -547,6 +547,14 @@ class Pool(Batch):
         for d in ("_xlyl", "_dt", "_sealevel", "_strati"):
             self.__dict__.get(d, {}).pop(k, None)
 
+    def ready(self):
+        """True once every step queued on the pool's stream has finished (errors raised as after a
+        synchronous step), False while some still run."""
+        rc = lib.fsx_query(self._h)
+        if rc < 0:
+            check(rc, "query")
+        return rc == 1
+
     def info(self):
         """(free nodes, largest free contiguous range, free slots)."""
         a, b, c = C.c_longlong(), C.c_longlong(), C.c_int()

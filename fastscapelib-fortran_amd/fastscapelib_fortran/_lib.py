@@ -62,6 +62,7 @@ def _load():
         "fsx_set_active": [_vp, C.c_int, C.c_int],
         "fsx_execute_steps": [_vp, C.c_int, _vp, C.c_int],
         "fsx_synchronize": [_vp],
+        "fsx_query": [_vp],
         "fsx_copy_field": [_vp, C.c_int, C.c_int, _vp, C.c_int],
         "fsx_copy_ifield": [_vp, C.c_int, C.c_int, _vp, C.c_int],
         "fsx_copy_derived": [_vp, C.c_int, C.c_int, _vp, C.c_int],

@@ -11,6 +11,9 @@ and step count arrive at any time; at every quantum boundary the server
   letting a later, smaller run go ahead of one that does not fit yet,
 * advances every resident run by q = min(quantum, fewest steps left) steps.
 
+With several pools (lanes) each lane runs its own quanta on its own stream and is attended to as soon as its
+quantum finishes, so the device keeps stepping the other lanes while the host retires and admits.
+
 Each run's result does not depend on which other runs shared its steps (the kernels index every
 landscape by its own node range and reduce per landscape).
 """
@@ -71,74 +74,117 @@ def pool_capacity(hbm_fraction=0.8, slots=256):
     return int(free * hbm_fraction / per_node), per_node
 
 
+class _Lane:
+    """One pool of the server and the quantum it has in flight."""
+
+    def __init__(self, pool):
+        self.pool = pool
+        self.active: Dict[int, Run] = {}
+        self.q = 0  # steps in flight (0: idle)
+        self.cells = 0
+
+    def free(self) -> int:
+        cap = getattr(self.pool, "capacity", 0)
+        return cap - sum(r.cells for r in self.active.values())
+
+
 class Server:
-    """Continuous-batching loop over a landscape pool (see module docstring)."""
+    """Continuous-batching loop over one or several landscape pools (see module docstring).  With
+    several pools ("lanes", each stepping on its own HIP stream) a lane's next quantum is launched as soon
+    as its last one finished and its runs were retired and admitted, while the other lanes' quanta run:
+    the host's copy-outs, admissions and graph captures overlap the device's steps, and one lane's
+    per-landscape kernels overlap another's node-parallel ones."""
 
     def __init__(self, pool, quantum: int = 8, clock=time.perf_counter):
-        self.pool = pool
+        self.lanes = [_Lane(p) for p in (pool if isinstance(pool, (list, tuple)) else [pool])]
+        self.pool = self.lanes[0].pool
         self.quantum = int(quantum)
         self.clock = clock
         self.queue: deque = deque()
-        self.active: Dict[int, Run] = {}
         self.finished: List[Run] = []
         self.rejected: List[Run] = []
         self.cell_steps = 0
         self.quanta = 0
-        self.busy = 0.0  # wall time of the quanta (admission, steps and retirement while runs are resident)
+        self.busy = 0.0  # wall time with runs resident
+
+    @property
+    def active(self) -> Dict[int, Run]:
+        out = {}
+        for j, ln in enumerate(self.lanes):
+            out.update({(j, k): r for k, r in ln.active.items()})
+        return out
 
     def submit(self, run: Run) -> None:
         run.left = run.steps
         self.queue.append(run)
 
-    def _admit(self) -> None:
+    def _admit(self, ln: _Lane) -> None:
+        """Waiting runs, in arrival order, that fit the idle lane ln (a later, smaller run may go ahead of
+        one that does not fit yet); a run that fits no empty pool is rejected."""
         waiting = deque()
         while self.queue:
             run = self.queue.popleft()
-            k = self.pool.add(run.nx, run.ny)
+            k = ln.pool.add(run.nx, run.ny)
             if k is None:
-                if not self.active and not self.pool.nn:  # does not fit even an empty pool
+                if all(not x.active and not x.pool.nn for x in self.lanes):  # fits no empty pool
                     run.done = self.clock()
                     self.rejected.append(run)
                 else:
                     waiting.append(run)
                 continue
             run.slot = k
-            run.setup(self.pool, k)
+            run.setup(ln.pool, k)
             run.admitted = self.clock()
             if run.left == 0:
-                self._retire(run)
+                self._retire(ln, run)
             else:
-                self.active[k] = run
+                ln.active[k] = run
         self.queue = waiting
 
-    def _retire(self, run: Run) -> None:
+    def _retire(self, ln: _Lane, run: Run) -> None:
         for name in run.outputs:
             if name in _IFIELDS:
-                run.result[name] = self.pool.ifield(run.slot, _IFIELDS[name])
+                run.result[name] = ln.pool.ifield(run.slot, _IFIELDS[name])
             else:
-                run.result[name] = self.pool.field(run.slot, _FIELDS[name])
-        self.pool.remove(run.slot)
+                run.result[name] = ln.pool.field(run.slot, _FIELDS[name])
+        ln.pool.remove(run.slot)
         run.done = self.clock()
-        self.active.pop(run.slot, None)
+        ln.active.pop(run.slot, None)
         self.finished.append(run)
 
-    def tick(self) -> int:
-        """Admit, advance every resident run by one quantum, retire finished runs.  Returns the
-        number of steps taken (0 when nothing is resident)."""
-        t0 = self.clock()
-        self._admit()
-        if not self.active:
-            return 0
-        q = min(self.quantum, min(r.left for r in self.active.values()))
-        self.pool.step(q)
+    def _complete(self, ln: _Lane) -> None:
+        """The lane's quantum finished: count its steps, retire the runs that reached their step count."""
+        q = ln.q
+        ln.q = 0
         self.quanta += 1
-        for run in list(self.active.values()):
+        for run in list(ln.active.values()):
             run.left -= q
             self.cell_steps += run.cells * q
             if run.left == 0:
-                self._retire(run)
-        self.busy += self.clock() - t0
-        return q
+                self._retire(ln, run)
+
+    def tick(self) -> int:
+        """One pass over the lanes: a lane whose quantum finished is retired, refilled from the queue and
+        given its next quantum (the lane with the most free nodes is served first).  Returns the steps
+        launched (0 when nothing was launched)."""
+        launched = 0
+        for ln in sorted(self.lanes, key=lambda x: -x.free()):
+            if ln.q:
+                ready = getattr(ln.pool, "ready", None)
+                if ready is not None and not ready():
+                    continue  # still stepping
+                self._complete(ln)
+            self._admit(ln)
+            if not ln.active:
+                continue
+            q = min(self.quantum, min(r.left for r in ln.active.values()))
+            if hasattr(ln.pool, "ready"):
+                ln.pool.step(q, sync=False)
+            else:
+                ln.pool.step(q)
+            ln.q = q
+            launched += q
+        return launched
 
     def serve(self, schedule: List[Run], realtime: bool = True, deadline: Optional[float] = None) -> dict:
         """Drive a whole arrival schedule (runs with ``arrival`` in seconds from the start).  With
@@ -151,17 +197,28 @@ class Server:
         base = t0 if realtime else None
         for r in pending:
             r.arrival = (base + r.arrival) if realtime else t0
-        while i < len(pending) or self.queue or self.active:
+        last = t0
+        while i < len(pending) or self.queue or any(ln.active or ln.q for ln in self.lanes):
             now = self.clock()
+            if any(ln.q for ln in self.lanes):
+                self.busy += now - last
+            last = now
             if deadline is not None and now - t0 >= deadline:
                 break
             while i < len(pending) and pending[i].arrival <= now:
                 self.submit(pending[i])
                 i += 1
-            if self.tick() == 0 and i < len(pending) and not self.queue:
-                time.sleep(max(0.0, min(0.05, pending[i].arrival - self.clock())))
+            if self.tick() == 0:
+                if any(ln.q for ln in self.lanes):
+                    time.sleep(0.0002)  # every lane is stepping
+                elif i < len(pending) and not self.queue:
+                    time.sleep(max(0.0, min(0.05, pending[i].arrival - self.clock())))
+        for ln in self.lanes:  # let the quanta in flight finish (their steps are not counted past the deadline)
+            if ln.q and hasattr(ln.pool, "synchronize"):
+                ln.pool.synchronize()
         rep = self.report(self.clock() - t0)
         rep["runs_unfinished"] = len(pending) - len(self.finished) - len(self.rejected)
+        rep["lanes"] = len(self.lanes)
         return rep
 
     def report(self, wall: float) -> dict:

@@ -164,6 +164,11 @@ int fsx_set_active(fsx_ctx *ctx, int inst, int active);
 int fsx_execute_steps(fsx_ctx *ctx, int nsteps, void *stream, int sync);
 /* Wait for queued steps and report device-side failures. */
 int fsx_synchronize(fsx_ctx *ctx);
+/* Without waiting: 1 when every step queued on the context's own stream has finished (its error words
+   are then checked, as after a synchronous step), 0 while some still run, < 0 on an error.  Lets a
+   serving loop keep several pools' steps in flight and attend to whichever finishes.  No counterpart
+   in the reference (its steps are synchronous calls). */
+int fsx_query(fsx_ctx *ctx);
 /* Run each step as a replay of one captured HIP graph (1, the default) or as individual kernel
  * launches (0).  Results are the same; graphs are not used while phase profiling is on.  No
  * reference counterpart: an execution option of FastScape_Execute_Step's device implementation. */

@@ -44,6 +44,22 @@ class FakePool:
         return np.full(self.nn[k], float(self.steps[k]))
 
 
+class FakeAsyncPool(FakePool):
+    """A pool whose quanta run asynchronously: ready() turns true on the second poll after a step."""
+
+    def __init__(self, capacity, slots):
+        super().__init__(capacity, slots)
+        self.polls = 0
+
+    def step(self, q, sync=True):
+        super().step(q)
+        self.polls = 2
+
+    def ready(self):
+        self.polls -= 1
+        return self.polls <= 0
+
+
 @pytest.fixture
 def serving():
     from fastscapelib_fortran import serving
@@ -115,3 +131,28 @@ def test_lanes_deal_balances_cells_and_covers_every_landscape():
     assert all(lane == sorted(lane) for lane in m)
     assert Lanes.deal(sizes, 10) == [[0], [1], [2], [3], [4], [5]]  # no empty lanes
     assert Lanes.deal([(513, 513)] * 64, 2) == [list(range(0, 64, 2)), list(range(1, 64, 2))]
+
+
+def test_lanes_step_every_run_exactly_and_overlap(serving):
+    """Two lanes: each run lives in one lane and gets exactly its steps; both lanes have quanta in flight
+    at once; no lane ever holds more nodes than its capacity."""
+    pools = [FakeAsyncPool(capacity=6000, slots=4), FakeAsyncPool(capacity=6000, slots=4)]
+    srv = serving.Server(pools, quantum=4)
+    runs = _runs(serving, [(50, 50, 13, 0), (40, 40, 7, 0), (30, 30, 22, 0), (60, 60, 3, 0), (20, 20, 9, 0),
+                           (70, 70, 11, 0), (45, 45, 5, 0), (25, 25, 17, 0)])
+    seen_both = []
+    orig = srv.tick
+
+    def tick():
+        n = orig()
+        seen_both.append(all(ln.q for ln in srv.lanes))
+        return n
+    srv.tick = tick
+    rep = srv.serve(runs, realtime=False)
+    assert rep["runs"] == len(runs) and rep["rejected"] == 0 and rep["lanes"] == 2
+    for r in runs:
+        assert r.result["h"][0] == r.steps, r.rid
+    assert rep["cell_steps"] == sum(r.nx * r.ny * r.steps for r in runs)
+    assert any(seen_both)  # quanta of both lanes in flight together
+    for p in pools:
+        assert p.log and all(q <= 4 for q, _ in p.log)
