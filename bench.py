@@ -274,7 +274,10 @@ def readout_cpu_baseline(L):
     return cpu
 
 
-def serving_leg(fs, window, deadline, quantum, sample, lanes=2):
+HEAVY_CELLS = 1 << 20  # served runs above this many cells step in the first lane only
+
+
+def serving_leg(fs, window, deadline, quantum, sample, lanes=3):
     """Arrivals of the serving stream during `window` seconds, served in real time by `lanes` pools (each
     stepping on its own stream, quanta launched as soon as a pool's last one finished); the loop stops at
     `deadline` seconds (runs still resident or waiting then are reported as unfinished)."""
@@ -283,7 +286,7 @@ def serving_leg(fs, window, deadline, quantum, sample, lanes=2):
     cap, per_node = serving.pool_capacity(0.8 / max(1, lanes))
     need = sum(L["nx"] * L["ny"] for _, _, L in stream)
     cap = int(min(cap, max(need, max(L["nx"] * L["ny"] for _, _, L in stream))))
-    runs = [serving.Run(rid=i, nx=L["nx"], ny=L["ny"], steps=st, arrival=t,
+    runs = [serving.Run(rid=i, nx=L["nx"], ny=L["ny"], steps=st, arrival=t, heavy=L["nx"] * L["ny"] > HEAVY_CELLS,
                         setup=(lambda p, k, L=L: W.configure(p, L, k))) for i, (t, st, L) in enumerate(stream)]
     offered = sum(L["nx"] * L["ny"] * st for _, st, L in stream)
     pools = [fs.Pool(cap, slots=256) for _ in range(max(1, lanes))]
@@ -434,7 +437,8 @@ def main():
     ap.add_argument("--serve-window", type=float, default=5.0)
     ap.add_argument("--serve-deadline", type=float, default=60.0)
     ap.add_argument("--quantum", type=int, default=8)
-    ap.add_argument("--serve-lanes", type=int, default=2, help="pools of the serving leg, each on its own stream")
+    ap.add_argument("--serve-lanes", type=int, default=3,
+                    help="pools of the serving leg, each on its own stream (the first takes the large grids)")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every kernel directly instead of replaying HIP graphs (for kernel-trace profiling)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (no autotuner)")

@@ -492,7 +492,9 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_mfd_prep(const InstParams* __rest
 
 // ---- accumulation (FlowAccumulation :97-119) -------------------------------------------------------
 // Dataflow over positions from the sources: a position waits for its donors' areas, then sums them in
-// donor-slot order.
+// donor-slot order.  A donor inside the wave's own tile of 64 positions is read from its lane by a
+// register shuffle as soon as that lane finishes (rounds inside one pass), so a chain of positions
+// inside a tile costs no memory round trip per link; donors in other tiles are polled in memory.
 __global__ __launch_bounds__(MFD_WG) void k_mfd_area(const InstParams* __restrict__ PP, const int2* __restrict__ mtab,
                                                      int* __restrict__ ctr, int* __restrict__ err,
                                                      const double* __restrict__ A0, const int* __restrict__ mdp,
@@ -501,46 +503,64 @@ __global__ __launch_bounds__(MFD_WG) void k_mfd_area(const InstParams* __restric
   MfdTiles T;
   mfd_start(T, ctr + FSX_MFD_CTR * inst + 8, err + inst);
   mfd_phase(T, (nn + 63) / 64, [&](int tt) {
-    const int p = 64 * tt + lane;
+    const int lo = 64 * tt, p = lo + lane;
     bool todo = p < nn;
     const long long gp = off + (todo ? p : 0);
-    double a0 = 0.0, w[8], x[8];
+    double a0 = 0.0, w[8], x[8], mine = 0.0;
     int d[8];
-    unsigned need = 0;
+    unsigned needg = 0, needt = 0;  // donors in memory / in the tile
 #pragma unroll
     for (int j = 0; j < 8; j++) {
       d[j] = todo ? mdp[8 * gp + j] : -1;
       w[j] = todo ? mdw[8 * gp + j] : 0.0;
       x[j] = 0.0;
-      if (d[j] >= 0) need |= 1u << j;
+      if (d[j] >= lo) needt |= 1u << j;
+      else if (d[j] >= 0) needg |= 1u << j;
     }
     if (todo) a0 = A0[gp];
     mfd_lanes(T, [&](bool& prog) {
       if (todo) {
 #pragma unroll
         for (int j = 0; j < 8; j++)
-          if ((need >> j) & 1u) {
+          if ((needg >> j) & 1u) {
             const double y = flow_ld_dev(Al + off + d[j]);
             if (!flow_sent(y)) {
               x[j] = y;
-              need &= ~(1u << j);
+              needg &= ~(1u << j);
             }
           }
-        if (!need) {
+      }
+      for (int round = 0; round < 64; round++) {
+        bool did = false;
+        if (todo && !needg && !needt) {
           double a = a0;
 #pragma unroll
           for (int j = 0; j < 8; j++)
             if (d[j] >= 0) a = a + x[j] * w[j];
+          mine = a;
           flow_st_dev(Al + gp, a);
           todo = false;
           prog = true;
+          did = true;
         }
+        const unsigned long long fin = __ballot(!todo);
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+          const bool want = (needt >> j) & 1u;
+          const int src = want ? d[j] - lo : lane;
+          const double y = __shfl(mine, src, 64);
+          if (want && ((fin >> src) & 1ull)) {
+            x[j] = y;
+            needt &= ~(1u << j);
+            did = true;
+          }
+        }
+        if (__ballot(did) == 0ull) break;
       }
       return todo;
     });
   });
 }
-
 
 // ---- per-position stream-power inputs (StreamPowerLaw :104-116 and :145-162) ----------------------
 // n == 1: kf*dt*(a*w)**m/l where ht(ijk) >= ht(rcv); otherwise kf*dt*(a*w)**m/l**n where
@@ -667,19 +687,20 @@ __global__ __launch_bounds__(MFD_WG) void k_mfd_spl(
         }
       });
     if (!gzero) {
-      mfd_phase(T, ntile, [&](int tt) {  // erosion push, from the sources
-        const int p = 64 * tt + lane;
+      mfd_phase(T, ntile, [&](int tt) {  // erosion push, from the sources (in-tile donors by shuffles)
+        const int lo = 64 * tt, p = lo + lane;
         bool todo = p < nn;
         const long long gp = off + (todo ? p : 0);
-        double w[8], x[8], own = 0.0;
+        double w[8], x[8], own = 0.0, mine = 0.0;
         int d[8], v = 0, r = 0, sl = -1;
-        unsigned need = 0;
+        unsigned needg = 0, needt = 0;
 #pragma unroll
         for (int j = 0; j < 8; j++) {
           d[j] = todo ? mdp[8 * gp + j] : -1;
           w[j] = todo ? mdw[8 * gp + j] : 0.0;
           x[j] = 0.0;
-          if (d[j] >= 0) need |= 1u << j;
+          if (d[j] >= lo) needt |= 1u << j;
+          else if (d[j] >= 0) needg |= 1u << j;
         }
         if (todo) {
           v = morder[gp];
@@ -691,14 +712,17 @@ __global__ __launch_bounds__(MFD_WG) void k_mfd_spl(
           if (todo) {
 #pragma unroll
             for (int j = 0; j < 8; j++)
-              if ((need >> j) & 1u) {
+              if ((needg >> j) & 1u) {
                 const double y = flow_ld_dev(DHl + off + d[j]);
                 if (!flow_sent(y)) {
                   x[j] = y;
-                  need &= ~(1u << j);
+                  needg &= ~(1u << j);
                 }
               }
-            if (!need) {
+          }
+          for (int round = 0; round < 64; round++) {
+            bool did = false;
+            if (todo && !needg && !needt) {
               double s = own;
 #pragma unroll
               for (int j = 0; j < 8; j++)
@@ -711,10 +735,25 @@ __global__ __launch_bounds__(MFD_WG) void k_mfd_spl(
               } else {
                 lsed[off + v] = s;
               }
+              mine = dv;
               flow_st_dev(DHl + gp, dv);
               todo = false;
               prog = true;
+              did = true;
             }
+            const unsigned long long fin = __ballot(!todo);
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+              const bool want = (needt >> j) & 1u;
+              const int src = want ? d[j] - lo : lane;
+              const double y = __shfl(mine, src, 64);
+              if (want && ((fin >> src) & 1ull)) {
+                x[j] = y;
+                needt &= ~(1u << j);
+                did = true;
+              }
+            }
+            if (__ballot(did) == 0ull) break;
           }
           return todo;
         });
@@ -735,20 +774,23 @@ __global__ __launch_bounds__(MFD_WG) void k_mfd_spl(
         }
       });
     }
-    mfd_phase(T, ntile, [&](int tt) {  // solve, from base level up
-      const int p = nn - 1 - (64 * tt + lane);
+    mfd_phase(T, ntile, [&](int tt) {  // solve, from base level up (in-tile receivers by shuffles)
+      const int hi = nn - 1 - 64 * tt, p = hi - lane;
       bool todo = p >= 0;
       const long long gp = off + (todo ? p : 0);
       int q[8], v = 0, r = -1;
-      double f[8], hr[8], t = 0.0, e = 0.0, h0 = 0.0, w_rcv = 0.0;
-      int sl_rcv = -1;
-      unsigned need = 0;
+      double f[8], hr[8], t = 0.0, e = 0.0, h0 = 0.0, w_rcv = 0.0, myH = 0.0, myW = 0.0;
+      int sl_rcv = -1, mySL = -1;
+      unsigned needg = 0, needt = 0;  // bits 0-7 receivers' h, 8 water level and 9 lake sill of rp
 #pragma unroll
       for (int k = 0; k < 8; k++) {
         q[k] = todo ? dn[8 * gp + k] : -1;
         f[k] = todo ? mfact[8 * gp + k] : -1.0;
         hr[k] = 0.0;
-        if (q[k] >= 0 && f[k] >= 0.0) need |= 1u << k;
+        if (q[k] >= 0 && f[k] >= 0.0) {
+          if (q[k] <= hi) needt |= 1u << k;
+          else needg |= 1u << k;
+        }
       }
       if (todo) {
         v = morder[gp];
@@ -756,40 +798,47 @@ __global__ __launch_bounds__(MFD_WG) void k_mfd_spl(
         t = HTl[gp];
         e = gzero ? t : flow_ld_dev(ELl + gp);
         h0 = flow_ld_dev(prev + gp);
-        if (r >= 0) need |= 3u << 8;  // the single-flow receiver's water level and lake sill
+        if (r >= 0) {
+          if (r <= hi) needt |= 3u << 8;
+          else needg |= 3u << 8;
+        }
       }
       mfd_lanes(T, [&](bool& prog) {
         if (todo) {
 #pragma unroll
           for (int k = 0; k < 8; k++)
-            if ((need >> k) & 1u) {
+            if ((needg >> k) & 1u) {
               const double y = flow_ld_dev(Hl + off + q[k]);
               if (!flow_sent(y)) {
                 hr[k] = y;
-                need &= ~(1u << k);
+                needg &= ~(1u << k);
               }
             }
-          if ((need >> 8) & 1u) {
+          if ((needg >> 8) & 1u) {
             const double y = flow_ld_dev(Wl + off + r);
             if (!flow_sent(y)) {
               w_rcv = y;
-              need &= ~(1u << 8);
+              needg &= ~(1u << 8);
             }
           }
-          if ((need >> 9) & 1u) {
+          if ((needg >> 9) & 1u) {
             const int y = flow_ldi_dev(LSl + off + r);
             if (y >= 0) {
               sl_rcv = y;
-              need &= ~(1u << 9);
+              needg &= ~(1u << 9);
             }
           }
-          if (!need) {
+        }
+        for (int round = 0; round < 64; round++) {
+          bool did = false;
+          if (todo && !needg && !needt) {
             todo = false;
             prog = true;
+            did = true;
             if (r < 0) {  // base level
-              flow_st_dev(Hl + gp, h0);
-              flow_st_dev(Wl + gp, t);
-              flow_sti_dev(LSl + gp, v);
+              myH = h0;
+              myW = t;
+              mySL = v;
               if (!gzero) {
                 water[off + v] = t;
                 lsill[off + v] = v;
@@ -843,16 +892,43 @@ __global__ __launch_bounds__(MFD_WG) void k_mfd_spl(
                 sl = sl_rcv;
                 if (!gzero) atomicAdd(lwv + off + sl, w_rcv - hv);
               }
-              const double wv = w_rcv > hv ? w_rcv : hv;
-              flow_st_dev(Hl + gp, hv);
-              flow_st_dev(Wl + gp, wv);
-              flow_sti_dev(LSl + gp, sl);
+              myH = hv;
+              myW = w_rcv > hv ? w_rcv : hv;
+              mySL = sl;
               if (!gzero) {
-                water[off + v] = wv;
+                water[off + v] = myW;
                 lsill[off + v] = sl;
               }
             }
+            flow_st_dev(Hl + gp, myH);
+            flow_st_dev(Wl + gp, myW);
+            flow_sti_dev(LSl + gp, mySL);
           }
+          const unsigned long long fin = __ballot(!todo);
+#pragma unroll
+          for (int k = 0; k < 8; k++) {
+            const bool want = (needt >> k) & 1u;
+            const int src = want ? hi - q[k] : lane;
+            const double y = __shfl(myH, src, 64);
+            if (want && ((fin >> src) & 1ull)) {
+              hr[k] = y;
+              needt &= ~(1u << k);
+              did = true;
+            }
+          }
+          {
+            const bool want = (needt >> 8) & 1u;
+            const int src = want ? hi - r : lane;
+            const double y = __shfl(myW, src, 64);
+            const int z = __shfl(mySL, src, 64);
+            if (want && ((fin >> src) & 1ull)) {
+              w_rcv = y;
+              sl_rcv = z;
+              needt &= ~(3u << 8);
+              did = true;
+            }
+          }
+          if (__ballot(did) == 0ull) break;
         }
         return todo;
       });

@@ -41,6 +41,7 @@ class Run:
     setup: Callable
     arrival: float = 0.0
     outputs: tuple = ("h",)
+    heavy: bool = False  # a run whose steps take long (large grid): served in the first lane only
     admitted: Optional[float] = None
     done: Optional[float] = None
     left: int = 0
@@ -120,10 +121,17 @@ class Server:
 
     def _admit(self, ln: _Lane) -> None:
         """Waiting runs, in arrival order, that fit the idle lane ln (a later, smaller run may go ahead of
-        one that does not fit yet); a run that fits no empty pool is rejected."""
+        one that does not fit yet); a run that fits no empty pool is rejected.  With several lanes, heavy
+        runs go to the first lane and the others to the rest, so that the short steps of small grids are
+        not held to the pace of the largest ones."""
         waiting = deque()
+        split = len(self.lanes) > 1
+        first = ln is self.lanes[0]
         while self.queue:
             run = self.queue.popleft()
+            if split and run.heavy != first:
+                waiting.append(run)
+                continue
             k = ln.pool.add(run.nx, run.ny)
             if k is None:
                 if all(not x.active and not x.pool.nn for x in self.lanes):  # fits no empty pool

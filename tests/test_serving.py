@@ -134,12 +134,14 @@ def test_lanes_deal_balances_cells_and_covers_every_landscape():
 
 
 def test_lanes_step_every_run_exactly_and_overlap(serving):
-    """Two lanes: each run lives in one lane and gets exactly its steps; both lanes have quanta in flight
-    at once; no lane ever holds more nodes than its capacity."""
+    """Two lanes: each run lives in one lane and gets exactly its steps, heavy runs in the first lane and
+    the others in the second; both lanes have quanta in flight at once."""
     pools = [FakeAsyncPool(capacity=6000, slots=4), FakeAsyncPool(capacity=6000, slots=4)]
     srv = serving.Server(pools, quantum=4)
     runs = _runs(serving, [(50, 50, 13, 0), (40, 40, 7, 0), (30, 30, 22, 0), (60, 60, 3, 0), (20, 20, 9, 0),
                            (70, 70, 11, 0), (45, 45, 5, 0), (25, 25, 17, 0)])
+    for r in runs:
+        r.heavy = r.nx >= 50
     seen_both = []
     orig = srv.tick
 
@@ -156,3 +158,5 @@ def test_lanes_step_every_run_exactly_and_overlap(serving):
     assert any(seen_both)  # quanta of both lanes in flight together
     for p in pools:
         assert p.log and all(q <= 4 for q, _ in p.log)
+    heavy = {r.slot for r in runs if r.heavy}
+    assert all(set(ks) <= heavy for _, ks in pools[0].log)  # only heavy runs in the first lane
