@@ -488,7 +488,7 @@ static int ctx_alloc(long long capacity, int nslots, fsx_ctx** out) {
   A(P, ninst); A(chunks, c->chunk_cap);
   A(h, N); A(b, N); A(u, N); A(kf, N); A(kd, N); A(precip, N); A(etot, N); A(erate, N); A(area, N);
   A(length, N); A(hwater, N); A(lake_depth, N); A(sedflux, N); A(ht, N); A(water, N); A(zint, N); A(zintp, N);
-  A(gam, N); A(dh, N); A(hp, N); A(lwv, N); A(lsed, N); A(kfint, N); A(kdint, N); A(g, N); A(pexp, N); A(elev, N);
+  A(gam, N); A(lakec, N); A(lacc_lo, N); A(lacc_hi, N); A(lsed, N); A(kfint, N); A(kdint, N); A(g, N); A(pexp, N);
   A(adx, N); A(ady, N); A(adg, N); A(adf, N); A(fmix, N);
   A(rec, N); A(basin, N); A(rootrank, N); A(order, N); A(lstart, N + ninst); A(lsill, N);
   A(dmask, N); A(bc, N);
@@ -982,7 +982,7 @@ static void enqueue_step(fsx_ctx* c, hipStream_t st) {
     PHASE(PH_SPL_SWEEP, launch_spl_sweep(S, st));  // also fills hwater / lake_depth
   }
   if (c->any_mfd) PHASE(PH_MFD_SPL, launch_mfd_spl(S, st));
-  PHASE(PH_SPL_POST, launch_spl_post(S, st));
+  PHASE(PH_SPL_POST, launch_lake_acc(S, st); launch_spl_post(S, st));
   PHASE(PH_DIFFUSION, launch_diffusion(S, st));
   if (c->any_marine) PHASE(PH_MARINE, launch_marine(S, st));  // Marine follows Diffusion (api.f90:245-250)
   if (c->any_strati) PHASE(PH_STRATI, launch_strati_step(S, st));  // then run_Strati (api.f90:252-257)

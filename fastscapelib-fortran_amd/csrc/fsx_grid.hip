@@ -179,13 +179,43 @@ __global__ __launch_bounds__(256) void k_donors_tile(const InstParams* __restric
 __global__ __launch_bounds__(FSX_CHUNK) void k_spl_prep(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
                                                         double* __restrict__ h, const double* __restrict__ b,
                                                         const double* __restrict__ kf, double* __restrict__ kfint,
-                                                        double* __restrict__ g, const double* __restrict__ ht) {
+                                                        double* __restrict__ g, const double* __restrict__ ht,
+                                                        unsigned long long* __restrict__ lacc_lo,
+                                                        unsigned long long* __restrict__ lacc_hi) {
   NODE_KERNEL_PROLOGUE
   if (!P.run_spl) return;
+  lacc_lo[g_] = 0ull;  // lake volume sums of this step (k_lake_acc)
+  lacc_hi[g_] = 0ull;
   const double hv = ht[g_], d = hv - b[g_];  // ht: h after uplift (k_route_tile)
   if (P.g1 >= 0.0) g[g_] = (P.g2 > 0.0 && d > 1.0) ? P.g2 : P.g1;
   kfint[g_] = (P.kfsed > 0.0 && d > 1.0) ? P.kfsed : kf[g_];
   h[g_] = hv;
+}
+
+// Lake water volumes (src/StreamPowerLaw.f90:323-324, lake_water_volume(lake_sill) += water(rec) - h, summed
+// along the stack): every lake node of the last Gauss-Seidel pass adds its share to its sill as a 128-bit fixed-point
+// integer (units of 2^-64 m; shares below 2^-12 m lose their bits under 2^-64 m), by integer atomics that commute, so
+// the volume does not depend on the order of the additions (run, batch, lane split); the reference's serial double
+// sum and this exact sum, rounded once, agree to the last places.
+__device__ __forceinline__ double lake_volume(unsigned long long lo, unsigned long long hi) {
+  return (double)hi + (double)lo * 5.421010862427522170037e-20;  // 2^-64
+}
+__global__ __launch_bounds__(FSX_CHUNK) void k_lake_acc(const InstParams* __restrict__ PP, const Chunk* __restrict__ chunks,
+                                                        const double* __restrict__ lakec, const int* __restrict__ lsill,
+                                                        const double* __restrict__ gmax_flag,
+                                                        unsigned long long* __restrict__ lacc_lo,
+                                                        unsigned long long* __restrict__ lacc_hi) {
+  NODE_KERNEL_PROLOGUE
+  if (!P.run_spl || gmax_flag[c.inst] == 0.0) return;
+  const double x = lakec[g_];
+  const int sl = lsill[g_];
+  if (!(x >= 0.0) || sl < 0) return;
+  const double ip = floor(x);
+  const unsigned long long hi = (unsigned long long)ip;
+  const unsigned long long lo = (unsigned long long)((x - ip) * 18446744073709551616.0);  // exact scaling by 2^64
+  const long long s = (long long)P.off + sl;
+  const unsigned long long old = atomicAdd(lacc_lo + s, lo);
+  atomicAdd(lacc_hi + s, hi + (old + lo < old ? 1ull : 0ull));  // carry out of the low word
 }
 
 // Stream-power bookkeeping, src/StreamPowerLaw.f90:380-393: b=min(h,b), lake sediment fill,
@@ -196,7 +226,9 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_spl_post(const InstParams* __rest
                                                         double* __restrict__ h, double* __restrict__ b,
                                                         const double* __restrict__ ht, double* __restrict__ etot,
                                                         double* __restrict__ erate, double* __restrict__ sedflux,
-                                                        const int* __restrict__ lsill, const double* __restrict__ lwv,
+                                                        const int* __restrict__ lsill,
+                                                        const unsigned long long* __restrict__ lacc_lo,
+                                                        const unsigned long long* __restrict__ lacc_hi,
                                                         const double* __restrict__ lsed, const double* __restrict__ water,
                                                         const double* __restrict__ gmax_flag, const int* __restrict__ pos,
                                                         const double* __restrict__ Hl, const double* __restrict__ HWl,
@@ -218,7 +250,7 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_spl_post(const InstParams* __rest
   if (gmax_flag[c.inst] != 0.0) {  // the fill adds exactly zero when g == 0 everywhere
     const int sl = lsill[g_];
     if (sl >= 0) {
-      const double vol = lwv[P.off + sl];
+      const double vol = lake_volume(lacc_lo[P.off + sl], lacc_hi[P.off + sl]);
       if (vol > 0.0) {
         double s = lsed[P.off + sl];
         s = s < vol ? s : vol;
@@ -556,11 +588,14 @@ void launch_donors(DevState& S, hipStream_t st) {
   if (S.ntiles) k_donors_tile<<<S.ntiles, 256, 0, st>>>(S.P, S.tiles, S.rec, S.dmask);
 }
 void launch_spl_prep(DevState& S, hipStream_t st) {
-  k_spl_prep<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.h, S.b, S.kf, S.kfint, S.g, S.ht);
+  k_spl_prep<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.h, S.b, S.kf, S.kfint, S.g, S.ht, S.lacc_lo, S.lacc_hi);
+}
+void launch_lake_acc(DevState& S, hipStream_t st) {
+  k_lake_acc<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.lakec, S.lsill, S.scal, S.lacc_lo, S.lacc_hi);
 }
 void launch_spl_post(DevState& S, hipStream_t st) {
   k_spl_post<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.h, S.b, S.ht, S.etot, S.erate, S.sedflux, S.lsill,
-                                               S.lwv, S.lsed, S.water, S.scal, S.pos, S.Hl, S.HWl, S.hwater,
+                                               S.lacc_lo, S.lacc_hi, S.lsed, S.water, S.scal, S.pos, S.Hl, S.HWl, S.hwater,
                                                S.lake_depth, S.mpos);
 }
 void launch_diffusion(DevState& S, hipStream_t st) {

@@ -290,6 +290,7 @@ void launch_lm_correct(DevState& S, hipStream_t st);
 void launch_nr_sort(DevState& S, hipStream_t st);
 void launch_spl_prep(DevState& S, hipStream_t st);
 void launch_spl_post(DevState& S, hipStream_t st);
+void launch_lake_acc(DevState& S, hipStream_t st);
 void launch_expand_donors(DevState& S, hipStream_t st);
 void launch_levels(DevState& S, hipStream_t st);
 void launch_jump(DevState& S, hipStream_t st);

@@ -641,9 +641,9 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_mfd_fact(const InstParams* __rest
 // ---- stream power, Gauss-Seidel loop ------------------------------------------------------------------
 // Per iteration (phases of one launch): hp = h (from the second iteration; h and the erosion push reset
 // to the sentinel); erosion pushed from the sources (dh(rcv) += dh(ijk)*w, :62-82) with the lake-sediment
-// update of lake sills; elev (:84-90), with water level, lake sill and lake volumes cleared; the solve from
-// base level up (:94-176): closed form over the receivers for n == 1, relaxed Newton (omega = 0.875/n)
-// otherwise; lakes keep elev and collect their water volume at their sill; err = rms(h-hp) from partial
+// update of lake sills; elev (:84-90), with water level and lake sill cleared; the solve from base level up
+// (:94-176): closed form over the receivers for n == 1, relaxed Newton (omega = 0.875/n) otherwise; lakes keep
+// elev and leave their share of the sill's water volume for k_lake_acc; err = rms(h-hp) from partial
 // sums over fixed blocks of positions, added in block order by every wave (so every wave takes the same
 // decision), zero when g == 0 everywhere.  Node-indexed water, lake sill and sediment are kept for the
 // lake fill of k_spl_post.
@@ -654,7 +654,7 @@ __global__ __launch_bounds__(MFD_WG) void k_mfd_spl(
     const double* __restrict__ EC, const double* __restrict__ Al, const double* __restrict__ HTl,
     double* __restrict__ Hl, double* __restrict__ HPl, double* __restrict__ DHl, double* __restrict__ ELl,
     double* __restrict__ Wl, int* __restrict__ LSl, double* __restrict__ water, int* __restrict__ lsill,
-    double* __restrict__ lwv, double* __restrict__ lsed, const unsigned long long* __restrict__ redk,
+    double* __restrict__ lakec, double* __restrict__ lsed, const unsigned long long* __restrict__ redk,
     double* __restrict__ scal, double* __restrict__ part, int* __restrict__ mdec, int* __restrict__ ngs_out,
     long long* __restrict__ newton_out) {
   MFD_WAVE_PROLOGUE
@@ -758,7 +758,7 @@ __global__ __launch_bounds__(MFD_WG) void k_mfd_spl(
           return todo;
         });
       });
-      mfd_phase(T, nnode, [&](int tt) {  // elev; the solve's water level, lake sill and lake volumes cleared
+      mfd_phase(T, nnode, [&](int tt) {  // elev; the solve's water level and lake sill cleared
 #pragma unroll
         for (int k = 0; k < 4; k++) {
           const int p = 256 * tt + 64 * k + lane;
@@ -769,7 +769,6 @@ __global__ __launch_bounds__(MFD_WG) void k_mfd_spl(
                         isnan(gc) ? t : t + (flow_ld_dev(DHl + gp) - (t - flow_ld_dev(prev + gp))) * gc * P.dx * P.dy / Al[gp]);
             flow_st_dev(Wl + gp, sent);
             flow_sti_dev(LSl + gp, -1);
-            flow_st_dev(lwv + off + morder[gp], 0.0);
           }
         }
       });
@@ -842,6 +841,7 @@ __global__ __launch_bounds__(MFD_WG) void k_mfd_spl(
               if (!gzero) {
                 water[off + v] = t;
                 lsill[off + v] = v;
+                lakec[off + v] = -1.0;  // not a lake node
               }
             } else {
               double hv = h0;
@@ -887,10 +887,11 @@ __global__ __launch_bounds__(MFD_WG) void k_mfd_spl(
                 }
                 sl = v;
                 if (hv < w_rcv) hv = w_rcv;
+                if (!gzero) lakec[off + v] = -1.0;
               } else {
                 hv = e;
                 sl = sl_rcv;
-                if (!gzero) atomicAdd(lwv + off + sl, w_rcv - hv);
+                if (!gzero) lakec[off + v] = w_rcv - hv;  // its share of the sill's volume (k_lake_acc)
               }
               myH = hv;
               myW = w_rcv > hv ? w_rcv : hv;
@@ -1003,6 +1004,6 @@ void launch_mfd_spl(DevState& S, hipStream_t st) {
                                                S.kfint, S.ht, S.g, S.bc, S.mfact, S.HTl, S.Hl, S.Wl, S.DHl, S.LSl, S.F,
                                                S.lsed, S.redk, S.newton);
   k_mfd_spl<<<S.nmfd, MFD_WG, 0, st>>>(S.P, S.mfd_tab, S.mctr, S.err, S.mstack, S.rp, S.dn, S.mfact, S.mdp, S.mdw, S.F,
-                                        S.Al, S.HTl, S.Hl, S.HPl, S.DHl, S.ELl, S.Wl, S.LSl, S.water, S.lsill, S.lwv,
+                                        S.Al, S.HTl, S.Hl, S.HPl, S.DHl, S.ELl, S.Wl, S.LSl, S.water, S.lsill, S.lakec,
                                         S.lsed, S.redk, S.scal, S.mpart, S.mdec, S.ngs, S.newton);
 }

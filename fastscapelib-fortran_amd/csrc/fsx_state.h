@@ -34,8 +34,12 @@ struct DevState {
   double *h = nullptr, *b = nullptr, *u = nullptr, *kf = nullptr, *kd = nullptr, *precip = nullptr;
   double *etot = nullptr, *erate = nullptr, *area = nullptr, *length = nullptr, *hwater = nullptr;
   double *lake_depth = nullptr, *sedflux = nullptr, *ht = nullptr, *water = nullptr, *zint = nullptr;
-  double *zintp = nullptr, *gam = nullptr, *dh = nullptr, *hp = nullptr, *lwv = nullptr, *lsed = nullptr;
-  double *kfint = nullptr, *kdint = nullptr, *g = nullptr, *pexp = nullptr, *elev = nullptr;
+  double *zintp = nullptr, *gam = nullptr, *lsed = nullptr;
+  // lake water volumes (g > 0): each lake node's share (w_rcv - h, -1 when not in a lake) written by the solve,
+  // summed per lake sill exactly in 128-bit fixed point (units of 2^-64 m), so the sum is the same in any order
+  double* lakec = nullptr;
+  unsigned long long *lacc_lo = nullptr, *lacc_hi = nullptr;
+  double *kfint = nullptr, *kdint = nullptr, *g = nullptr, *pexp = nullptr;
   double *adx = nullptr, *ady = nullptr, *adg = nullptr, *adf = nullptr;  // ADI couplings, diagonal, rhs
   double *vx = nullptr, *vy = nullptr;  // velocity field of the advection (allocated when first set)
   double* fmix = nullptr;               // silt fraction of the marine transport (Fmix, default 0.5)

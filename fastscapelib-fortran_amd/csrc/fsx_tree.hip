@@ -561,7 +561,7 @@ __global__ __launch_bounds__(FSX_WG) void k_spl_sweep(
     const int4* __restrict__ grec, const int* __restrict__ ngroups, const SplRec32* __restrict__ srec,
     const AreaRec64* __restrict__ arec, const uint8_t* __restrict__ bc, const double* __restrict__ g,
     const double* __restrict__ Al, double* __restrict__ h, double* __restrict__ hwater, double* __restrict__ lake_depth,
-    double* __restrict__ water, int* __restrict__ lsill, double* __restrict__ lwv, double* __restrict__ lsed,
+    double* __restrict__ water, int* __restrict__ lsill, double* __restrict__ lakec, double* __restrict__ lsed,
     double* __restrict__ Hl, double* __restrict__ Wl, double* __restrict__ HWl, int* __restrict__ LSl,
     double* __restrict__ HPl, double* __restrict__ DHl, double* __restrict__ ELl, double* __restrict__ scal,
     int* __restrict__ ngs_out, long long* __restrict__ newton_out, int* __restrict__ err, int vcap, int fill_only,
@@ -699,7 +699,7 @@ __global__ __launch_bounds__(FSX_WG) void k_spl_sweep(
           int sl = v;
           if (l == 0) {
             wv = t;
-            if (!gzero && !fill_only) lwv[off + v] = 0.0;
+            if (!gzero && !fill_only) lakec[off + v] = -1.0;  // not a lake node
           } else {
             const int q = rr.q;
             double hr, w_rcv;
@@ -748,13 +748,13 @@ __global__ __launch_bounds__(FSX_WG) void k_spl_sweep(
                   }
                   my_newton += it;
                 }
-                if (!gzero) lwv[off + v] = 0.0;
+                if (!gzero) lakec[off + v] = -1.0;
                 if (hv < w_rcv) hv = w_rcv;
               } else {
                 hv = e;
                 if constexpr (FAST) sl = -1;
                 else sl = gzero ? -1 : (pin ? BLS(pb)[q] : LSl[off + bp + q]);
-                if (!gzero && sl >= 0) atomicAdd(lwv + off + sl, w_rcv - hv);
+                if (!gzero) lakec[off + v] = sl >= 0 ? w_rcv - hv : -1.0;  // its share of the sill's volume
               }
               wv = w_rcv > hv ? w_rcv : hv;
             }
@@ -1357,7 +1357,7 @@ static void spl_sweep_launch(DevState& S, hipStream_t st, int fill_only) {
   launch_spl_flow(S, st, fill_only);
   k_spl_sweep<<<S.ninst, FSX_WG, spl_smem(kSplVcap), st>>>(
       S.P, S.lstart, S.nlev, S.grec, S.ngroups, S.srec, S.arec, S.bc, S.g, S.Al, S.h, S.hwater,
-      S.lake_depth, S.water, S.lsill, S.lwv, S.lsed, S.Hl, S.Wl, S.HWl, S.LSl, S.HPl, S.DHl, S.ELl, S.scal, S.ngs,
+      S.lake_depth, S.water, S.lsill, S.lakec, S.lsed, S.Hl, S.Wl, S.HWl, S.LSl, S.HPl, S.DHl, S.ELl, S.scal, S.ngs,
       S.newton, S.err, kSplVcap, fill_only, S.redk);
 }
 
