@@ -28,7 +28,7 @@ struct DevState {
   int flow_split = 131072; // stream-power dataflow sweep: nodes per extra workgroup (0: one per landscape)
   int nr_gx = 8;           // edge sort: workgroups per landscape for the big ranges of a level
   int mfd_split = 131072;  // multiple-flow dataflow sweeps: nodes per workgroup of a landscape
-  int walk_big = 2048;     // closed basins larger than this are searched node-parallel, not walked
+  int walk_big = 256;      // closed basins larger than this are searched node-parallel, not walked
 
   // ---- per node (ntot) ----
   double *h = nullptr, *b = nullptr, *u = nullptr, *kf = nullptr, *kd = nullptr, *precip = nullptr;
