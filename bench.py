@@ -73,7 +73,7 @@ def _sync():
 
 GRAPHS = True  # replay captured HIP graphs (off: launch every kernel directly, for kernel-trace profilers)
 TUNE = True    # run the launch-shape autotuner after the warm-up steps of a batch
-SHAPES = None  # or launch shapes given on the command line (walk_gx, jump_blocks, tour_gx, nr_small_gx, area_wg, bor_gx[, flow_wg, flow_split, walk_big, mfd_split])
+SHAPES = None  # or launch shapes given on the command line (walk_gx, jump_blocks, tour_gx, nr_small_gx, area_wg, bor_gx[, flow_wg, flow_split, walk_big, mfd_split, tile_h])
 
 
 def open_batch(fs, Ls, lanes=1):
@@ -445,7 +445,8 @@ def main():
     ap.add_argument("--lanes", type=int, default=2,
                     help="contexts stepping the batch side by side on their own streams (1: one context)")
     ap.add_argument("--shapes", default=None,
-                    help="walk_gx,jump_blocks,tour_gx,nr_small_gx,area_wg,bor_gx[,flow_wg,flow_split] instead of the autotuner")
+                    help="walk_gx,jump_blocks,tour_gx,nr_small_gx,area_wg,bor_gx[,flow_wg,flow_split,walk_big,mfd_split,"
+                         "tile_h] instead of the autotuner (missing trailing values keep their defaults)")
     args = ap.parse_args()
     global GRAPHS, TUNE, SHAPES
     GRAPHS = not args.no_graphs
@@ -491,6 +492,15 @@ def main():
         out["phase_ms_per_step_note"] = (f"profiled window of {args.prof_steps} steps, kernels launched one by one"
                                          + (f"; lane 0 of {r['lanes']} ({len(r['sel'])} landscapes), the others "
                                             "beside it" if r["lanes"] > 1 else ""))
+        # every single-kernel phase with an algorithmic-bytes figure against the HBM peak, from the same window
+        # (mean launch; the donor masks run twice per step)
+        out["phase_hbm"] = {}
+        for p, (ms, c) in r["prof"].items():
+            nbytes = _alg_bytes(p, [r["stats"][k] for k in r["sel"]], sel_cells)
+            if c and nbytes:
+                gbs = nbytes / (ms / c * 1e-3) / 1e9
+                out["phase_hbm"][p] = {"GBps": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
+                                       "avg_launch_ms": round(ms / c, 4)}
         out["hbm_bytes_resident"] = r["mem"][0]
         out["newton_iters_per_step"] = float(np.sum([s["newton_iters"] for s in r["stats"]]))
         out["levels_max"] = int(max(s["levels"] for s in r["stats"]))

@@ -199,9 +199,8 @@ static int check_inst(fsx_ctx* c, int inst) {
 }
 
 static int build_flow_table(fsx_ctx* c);
-
-static int build_flow_table(fsx_ctx* c);
 static int build_mfd_table(fsx_ctx* c);
+static int build_tiles(fsx_ctx* c);
 
 static int upload_params(fsx_ctx* c) {
   if (!c->dirty) return 0;
@@ -220,6 +219,8 @@ static int upload_params(fsx_ctx* c) {
   }
   if (int r = build_flow_table(c)) return r;
   if (int r = build_mfd_table(c)) return r;
+  if (c->S.tile_h != c->S.tile_rows)
+    if (int r = build_tiles(c)) return r;
   HIPCHK(hipMemcpyAsync(c->S.P, c->hp.data(), sizeof(InstParams) * c->hp.size(), hipMemcpyHostToDevice, c->stream));
   c->dirty = false;
   return 0;
@@ -387,6 +388,39 @@ static int build_mfd_table(fsx_ctx* c) {
   return 0;
 }
 
+// The 2-D tiles of the stencil kernels, tile_h rows each, row-major within a landscape.  The table is
+// rewritten in place once queued launches are done; captured launches carry its size, so the graphs are
+// rebuilt.
+static int build_tiles(fsx_ctx* c) {
+  DevState& S = c->S;
+  std::vector<int4> tiles;
+  for (int k = 0; k < S.ninst; k++) {
+    if (!c->used[k]) continue;
+    const InstParams& P = c->hp[k];
+    for (int y0 = 0; y0 < P.ny; y0 += S.tile_h)
+      for (int x0 = 0; x0 < P.nx; x0 += FSX_TW) tiles.push_back(make_int4(k, x0, y0, 0));
+  }
+  HIPCHK(quiesce(c));
+  if ((long long)tiles.size() > c->tile_cap) {
+    if (S.tiles) HIPCHK(hipFree(S.tiles));
+    S.tiles = nullptr;
+    S.ntiles = 0;
+    c->tile_cap = 0;
+    const long long cap = (long long)tiles.size() + tiles.size() / 4 + 64;
+    hipError_t e = hipMalloc(&S.tiles, sizeof(int4) * cap);
+    if (e != hipSuccess) return fail(FSX_ENOMEM, std::string("hipMalloc (tile table): ") + hipGetErrorString(e));
+    c->tile_cap = cap;
+  }
+  if (!tiles.empty()) {
+    HIPCHK(hipMemcpyAsync(S.tiles, tiles.data(), sizeof(int4) * tiles.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  S.ntiles = (int)tiles.size();
+  S.tile_rows = S.tile_h;
+  c->graph_valid = false;
+  return 0;
+}
+
 static int relayout(fsx_ctx* c) {
   DevState& S = c->S;
   std::vector<Chunk> chunks;
@@ -411,27 +445,8 @@ static int relayout(fsx_ctx* c) {
   }
   if ((long long)chunks.size() > c->chunk_cap) return fail(FSX_EINVAL, "chunk table overflow");
   S.nchunks = (int)chunks.size();
-  std::vector<int4> tiles;  // 2-D tiles, row-major within a landscape
-  for (int k = 0; k < S.ninst; k++) {
-    if (!c->used[k]) continue;
-    const InstParams& P = c->hp[k];
-    for (int y0 = 0; y0 < P.ny; y0 += FSX_TH)
-      for (int x0 = 0; x0 < P.nx; x0 += FSX_TW) tiles.push_back(make_int4(k, x0, y0, 0));
-  }
-  if ((long long)tiles.size() > c->tile_cap) {
-    HIPCHK(quiesce(c));
-    if (S.tiles) HIPCHK(hipFree(S.tiles));
-    S.tiles = nullptr;
-    c->tile_cap = 0;
-    const long long cap = (long long)tiles.size() + tiles.size() / 4 + 64;
-    hipError_t e = hipMalloc(&S.tiles, sizeof(int4) * cap);
-    if (e != hipSuccess) return fail(FSX_ENOMEM, std::string("hipMalloc (tile table): ") + hipGetErrorString(e));
-    c->tile_cap = cap;
-  }
-  S.ntiles = (int)tiles.size();
-  HIPCHK(quiesce(c));  // the chunk and tile tables are rewritten in place
-  if (!tiles.empty())
-    HIPCHK(hipMemcpyAsync(S.tiles, tiles.data(), sizeof(int4) * tiles.size(), hipMemcpyHostToDevice, c->stream));
+  if (int r = build_tiles(c)) return r;
+  HIPCHK(quiesce(c));  // the chunk table is rewritten in place
   if (!chunks.empty())
     HIPCHK(hipMemcpyAsync(S.chunks, chunks.data(), sizeof(Chunk) * chunks.size(), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -1436,6 +1451,7 @@ extern "C" int fsx_autotune(fsx_ctx* c, int reps, int* chosen) {
       {&DevState::flow_split, {0, 262144, 131072, 65536}, 4, PH_SPL_SWEEP, -1},
       {&DevState::walk_big, {64, 256, 1024, 8192}, 4, PH_LM_WALK, -1},
       {&DevState::mfd_split, {262144, 131072, 65536, 32768}, 4, PH_MFD_ROUTE, -1},
+      {&DevState::tile_h, {4, 8, 16, 32}, 4, PH_RECEIVERS, PH_DONORS},
   };
   // keep the caller's profiling state and accumulated timers
   if (int r = flush_timers_any(c, c->stream)) return r;
@@ -1461,7 +1477,9 @@ extern "C" int fsx_autotune(fsx_ctx* c, int reps, int* chosen) {
     };
     int best = c->S.*k.field;
     double best_ms = 1e300;
-    const bool split = k.field == &DevState::flow_split || k.field == &DevState::mfd_split;
+    // these shapes change a work table (rebuilt by upload_params)
+    const bool split =
+        k.field == &DevState::flow_split || k.field == &DevState::mfd_split || k.field == &DevState::tile_h;
     for (int i = 0; i < k.ncand && !rc; i++) {
       c->S.*k.field = k.cand[i];
       if (split) {
@@ -1510,6 +1528,7 @@ extern "C" int fsx_get_tuning(fsx_ctx* c, int* out) {
   out[7] = c->S.flow_split;
   out[8] = c->S.walk_big;
   out[9] = c->S.mfd_split;
+  out[10] = c->S.tile_h;
   return 0;
 }
 
@@ -1526,6 +1545,8 @@ extern "C" int fsx_set_tuning(fsx_ctx* c, const int* in) {
   if (in[7] < 0) return fail(FSX_EINVAL, "fsx_set_tuning: flow_split must be >= 0");
   if (in[8] < 64) return fail(FSX_EINVAL, "fsx_set_tuning: walk_big must be >= 64");
   if (in[9] < 1024) return fail(FSX_EINVAL, "fsx_set_tuning: mfd_split must be >= 1024");
+  if (in[10] != 4 && in[10] != 8 && in[10] != 16 && in[10] != 32)
+    return fail(FSX_EINVAL, "fsx_set_tuning: tile_h must be 4, 8, 16 or 32");
   c->S.walk_gx = in[0];
   c->S.jump_blocks = in[1];
   c->S.tour_gx = in[2];
@@ -1538,6 +1559,8 @@ extern "C" int fsx_set_tuning(fsx_ctx* c, const int* in) {
   c->S.walk_big = in[8];
   if (c->S.mfd_split != in[9]) c->dirty = true;  // so does the multiple-flow sweeps' table
   c->S.mfd_split = in[9];
+  if (c->S.tile_h != in[10]) c->dirty = true;  // and the stencil kernels' tile table
+  c->S.tile_h = in[10];
   c->graph_valid = false;  // captured launches carry the old shapes
   return 0;
 }

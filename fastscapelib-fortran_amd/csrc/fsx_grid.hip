@@ -15,7 +15,8 @@
   if (!P.active || v >= P.nn) return;                 \
   const long long g_ = (long long)P.off + v;
 
-// Uplift and find_receiver fused, over 2-D tiles of FSX_TW x FSX_TH nodes staged in LDS with a one-node
+// Uplift and find_receiver fused, over 2-D tiles of FSX_TW x TH nodes (TH rows: 4, 8, 16 or 32, the tile
+// table's height, a launch shape the autotuner chooses) staged in LDS with a one-node
 // halo (one workgroup per tile, tiles of a landscape dealt to XCDs in contiguous runs so that the halo
 // rows of neighbouring tiles come from one L2).  The halo cell of direction (ii, jj) holds h of the node
 // find_receiver takes there (cyclic wrap else clamp, src/FlowRouting.f90:336-345), uplifted with the
@@ -26,13 +27,14 @@
 // The uplifted h of a routed landscape goes to ht (its neighbours' tiles still read h); every kernel up to
 // k_spl_prep reads ht, and k_spl_prep writes it back to h.  A landscape without routing (run_spl == 0)
 // reads no neighbours and is uplifted in place.  uplift == 0: the receivers of the current h (ht = h).
+template <int TH>
 __global__ __launch_bounds__(256) void k_route_tile(const InstParams* __restrict__ PP, const int4* __restrict__ tiles,
                                                     double* __restrict__ h, double* __restrict__ b,
                                                     const double* __restrict__ u, double* __restrict__ ht,
                                                     int* __restrict__ rec, double* __restrict__ length, int uplift) {
-  constexpr int W = FSX_TW + 2, H = FSX_TH + 2;
+  constexpr int W = FSX_TW + 2, H = TH + 2;
   __shared__ double sh[H][W];
-  __shared__ double sud[FSX_TH][FSX_TW];
+  __shared__ double sud[TH][FSX_TW];
   const int4 T = tiles[chunk_id()];
   const InstParams& P = PP[T.x];
   if (!P.active) return;
@@ -42,7 +44,7 @@ __global__ __launch_bounds__(256) void k_route_tile(const InstParams* __restrict
   const double dt = P.dt;
   if (!P.run_spl) {  // uplift alone, in place
     if (!up) return;
-    for (int k = threadIdx.x; k < FSX_TW * FSX_TH; k += 256) {
+    for (int k = threadIdx.x; k < FSX_TW * TH; k += 256) {
       const int i = x0 + (k % FSX_TW), j = y0 + k / FSX_TW;
       if (i >= nx || j >= ny) continue;
       const long long g = off + i + (long long)j * nx;
@@ -66,12 +68,12 @@ __global__ __launch_bounds__(256) void k_route_tile(const InstParams* __restrict
     if (up) {
       const double ud = u[g] * dt;
       hv = hv + ud;
-      if (lx >= 1 && lx <= FSX_TW && ly >= 1 && ly <= FSX_TH) sud[ly - 1][lx - 1] = ud;
+      if (lx >= 1 && lx <= FSX_TW && ly >= 1 && ly <= TH) sud[ly - 1][lx - 1] = ud;
     }
     sh[ly][lx] = hv;
   }
   __syncthreads();
-  for (int k = threadIdx.x; k < FSX_TW * FSX_TH; k += 256) {
+  for (int k = threadIdx.x; k < FSX_TW * TH; k += 256) {
     const int tx = k % FSX_TW, ty = k / FSX_TW;
     const int i = x0 + tx, j = y0 + ty;
     if (i >= nx || j >= ny) continue;
@@ -116,9 +118,10 @@ __global__ __launch_bounds__(256) void k_route_tile(const InstParams* __restrict
 // as find_receiver takes neighbours) in LDS, then for each node the distinct neighbours in direction
 // order (jj outer, ii inner, itself and repeated clamp targets skipped, as neighbours_wc) whose receiver
 // is the node: find_donor's donor set (src/FlowRouting.f90:366-389) as a mask, bit dbit(d) for direction d.
+template <int TH>
 __global__ __launch_bounds__(256) void k_donors_tile(const InstParams* __restrict__ PP, const int4* __restrict__ tiles,
                                                      const int* __restrict__ rec, uint8_t* __restrict__ dmask) {
-  constexpr int W = FSX_TW + 2, H = FSX_TH + 2;
+  constexpr int W = FSX_TW + 2, H = TH + 2;
   __shared__ int sr[H][W];
   const int4 T = tiles[chunk_id()];
   const InstParams& P = PP[T.x];
@@ -137,7 +140,7 @@ __global__ __launch_bounds__(256) void k_donors_tile(const InstParams* __restric
     sr[ly][lx] = rec[off + gx + (long long)gy * nx];
   }
   __syncthreads();
-  for (int k = threadIdx.x; k < FSX_TW * FSX_TH; k += 256) {
+  for (int k = threadIdx.x; k < FSX_TW * TH; k += 256) {
     const int tx = k % FSX_TW, ty = k / FSX_TW;
     const int i = x0 + tx, j = y0 + ty;
     if (i >= nx || j >= ny) continue;
@@ -581,11 +584,28 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_adi_post(const InstParams* __rest
 }
 
 // ---------------------------------------------------------------------------------------------
+// The tile table holds tiles of S.tile_rows rows (the height it was built with).
 void launch_receivers(DevState& S, hipStream_t st, int uplift) {
-  if (S.ntiles) k_route_tile<<<S.ntiles, 256, 0, st>>>(S.P, S.tiles, S.h, S.b, S.u, S.ht, S.rec, S.length, uplift);
+  if (!S.ntiles) return;
+#define ROUTE(TH) k_route_tile<TH><<<S.ntiles, 256, 0, st>>>(S.P, S.tiles, S.h, S.b, S.u, S.ht, S.rec, S.length, uplift)
+  switch (S.tile_rows) {
+    case 4: ROUTE(4); break;
+    case 16: ROUTE(16); break;
+    case 32: ROUTE(32); break;
+    default: ROUTE(8); break;
+  }
+#undef ROUTE
 }
 void launch_donors(DevState& S, hipStream_t st) {
-  if (S.ntiles) k_donors_tile<<<S.ntiles, 256, 0, st>>>(S.P, S.tiles, S.rec, S.dmask);
+  if (!S.ntiles) return;
+#define DONORS(TH) k_donors_tile<TH><<<S.ntiles, 256, 0, st>>>(S.P, S.tiles, S.rec, S.dmask)
+  switch (S.tile_rows) {
+    case 4: DONORS(4); break;
+    case 16: DONORS(16); break;
+    case 32: DONORS(32); break;
+    default: DONORS(8); break;
+  }
+#undef DONORS
 }
 void launch_spl_prep(DevState& S, hipStream_t st) {
   k_spl_prep<<<S.nchunks, FSX_CHUNK, 0, st>>>(S.P, S.chunks, S.h, S.b, S.kf, S.kfint, S.g, S.ht, S.lacc_lo, S.lacc_hi);

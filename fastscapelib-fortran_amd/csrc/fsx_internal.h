@@ -16,8 +16,7 @@
 #define FSX_SLAB_CAP 8   // candidate-pass slots per closed-basin node
 #define FSX_CHUNK 256    // nodes per block in node-parallel kernels
 #define FSX_WG 1024      // threads of the per-landscape workgroups
-#define FSX_TW 64        // 2-D tiles of the stencil kernels: columns
-#define FSX_TH 8         //   rows
+#define FSX_TW 64        // 2-D tiles of the stencil kernels: columns (rows: the context's tile_h, 4 to 32)
 #define FSX_MFD_CTR 16   // counters per landscape of the multiple-flow dataflow launches
 
 // device-side error bits, one word per instance

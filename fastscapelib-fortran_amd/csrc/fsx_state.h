@@ -10,8 +10,9 @@ struct DevState {
   int any_mfd = 0, any_gs = 0;
   InstParams* P = nullptr;  // [ninst]
   Chunk* chunks = nullptr;  // [nchunks]
-  int ntiles = 0;           // 2-D tiles (FSX_TW x FSX_TH nodes) covering every instance
+  int ntiles = 0;           // 2-D tiles (FSX_TW x tile_rows nodes) covering every instance
   int4* tiles = nullptr;    // [ntiles]: (instance, x0, y0, -)
+  int tile_rows = 8;        // rows of the tiles in the table
   int nflow = 0;            // workgroups of the dataflow stream-power sweep over all instances
   int2* flow_tab = nullptr; // [nflow]: (instance, rank among the instance's workgroups)
   int nmfd = 0;             // workgroups of the multiple-flow dataflow sweeps over all instances
@@ -29,6 +30,7 @@ struct DevState {
   int nr_gx = 8;           // edge sort: workgroups per landscape for the big ranges of a level
   int mfd_split = 131072;  // multiple-flow dataflow sweeps: nodes per workgroup of a landscape
   int walk_big = 256;      // closed basins larger than this are searched node-parallel, not walked
+  int tile_h = 8;          // stencil kernels (receivers, donor masks): rows per tile (4, 8, 16 or 32)
 
   // ---- per node (ntot) ----
   double *h = nullptr, *b = nullptr, *u = nullptr, *kf = nullptr, *kd = nullptr, *precip = nullptr;
