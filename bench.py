@@ -73,7 +73,7 @@ def _sync():
 
 GRAPHS = True  # replay captured HIP graphs (off: launch every kernel directly, for kernel-trace profilers)
 TUNE = True    # run the launch-shape autotuner after the warm-up steps of a batch
-SHAPES = None  # or launch shapes given on the command line (walk_gx, jump_blocks, tour_gx, nr_small_gx, area_wg, bor_gx[, flow_wg, flow_split, walk_big, mfd_split, tile_h])
+SHAPES = None  # or launch shapes given on the command line (walk_gx, jump_blocks, tour_gx, nr_small_gx, area_wg, bor_gx[, flow_wg, flow_split, walk_big, mfd_split, tile_h, flow_nk])
 
 
 def open_batch(fs, Ls, lanes=1):
@@ -313,6 +313,40 @@ def serving_leg(fs, window, deadline, quantum, sample, lanes=3):
     return out
 
 
+def serving_ragged_leg(fs, rate, steps, quantum, lanes, plain):
+    """The ragged batch's 200 landscapes as a served stream: Poisson arrivals at `rate` runs/s (seeded), each
+    run one landscape for `steps` steps, served in real time by `lanes` pools (the landscapes above HEAVY_CELLS in
+    the first); its rate over the busy time against the plain one-landscape-at-a-time rate of the same
+    landscapes."""
+    from fastscapelib_fortran import serving
+    Ls = W.ragged_batch()
+    rng = np.random.default_rng(2024)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, len(Ls)))
+    cap, per_node = serving.pool_capacity(0.8 / max(1, lanes))
+    cap = int(min(cap, W.cells(Ls)))
+    runs = [serving.Run(rid=k, nx=L["nx"], ny=L["ny"], steps=steps, arrival=float(arrivals[k]),
+                        heavy=L["nx"] * L["ny"] > HEAVY_CELLS, setup=(lambda p, i, L=L: W.configure(p, L, i)))
+            for k, L in enumerate(Ls)]
+    pools = [fs.Pool(cap, slots=256) for _ in range(max(1, lanes))]
+    try:
+        for p in pools:
+            p.graphs(GRAPHS)
+        rep = serving.Server(pools, quantum=quantum).serve(runs, realtime=True)
+    finally:
+        for p in pools:
+            p.close()
+    out = {"workload": f"{RAGGED_WORKLOAD}: each landscape a run of {steps} steps, Poisson arrivals {rate:g} runs/s",
+           "runs": len(runs), "quantum": quantum, "lanes": len(pools), "pool_nodes": cap,
+           "offered_cell_steps": W.cells(Ls) * steps}
+    out.update({k: rep[k] for k in ("runs_finished", "runs_unfinished", "cell_steps", "wall_s", "cell_steps_per_s",
+                                    "p50_latency_s", "p99_latency_s", "busy_cell_steps_per_s")})
+    out["latency_note"] = "completion latency (arrival to copy-out) of every run"
+    if plain:
+        out["plain"] = plain
+        out["served_over_plain"] = out["busy_cell_steps_per_s"] / plain
+    return out
+
+
 # ---- CPU baseline: the reference Fortran (oracle/_ref), one landscape per process -------------------
 def _cpu_worker(args):
     L, steps, lib = args
@@ -429,9 +463,12 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=8)
     ap.add_argument("--cpu-procs", type=int, default=0, help="0: the host's usable cores, at most 16")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--legs", default="ragged,plain,uniform,mixed,serving,advect,readouts",
-                    help="comma list of ragged (the headline), plain, uniform, mixed, serving, advect, readouts; "
-                         "copy alone: HBM-counter calibration")
+    ap.add_argument("--legs", default="ragged,plain,serve_ragged,uniform,mixed,serving,advect,readouts",
+                    help="comma list of ragged (the headline), plain, serve_ragged (the ragged landscapes served as "
+                         "an arriving stream), uniform, mixed, serving, advect, readouts; copy alone: HBM-counter "
+                         "calibration")
+    ap.add_argument("--serve-ragged-rate", type=float, default=50.0, help="arrivals per second of serve_ragged")
+    ap.add_argument("--serve-ragged-steps", type=int, default=300, help="steps per run of serve_ragged")
     ap.add_argument("--uniform-steps", type=int, default=20)
     ap.add_argument("--mixed-steps", type=int, default=3)
     ap.add_argument("--serve-window", type=float, default=5.0)
@@ -446,7 +483,7 @@ def main():
                     help="contexts stepping the batch side by side on their own streams (1: one context)")
     ap.add_argument("--shapes", default=None,
                     help="walk_gx,jump_blocks,tour_gx,nr_small_gx,area_wg,bor_gx[,flow_wg,flow_split,walk_big,mfd_split,"
-                         "tile_h] instead of the autotuner (missing trailing values keep their defaults)")
+                         "tile_h,flow_nk] instead of the autotuner (missing trailing values keep their defaults)")
     args = ap.parse_args()
     global GRAPHS, TUNE, SHAPES
     GRAPHS = not args.no_graphs
@@ -513,6 +550,14 @@ def main():
                                                    f"alone, {args.plain_steps} steps after 1 warm-up step and the autotuner"}
         if out["value"]:
             out["batched_over_plain"] = out["value"] / plain
+    if "serve_ragged" in legs:
+        note("ragged landscapes served as a stream")
+        if "plain" in out:
+            plain_v = out["plain"]["value"]
+        else:
+            plain_v = plain_rate(fs, [Ls[i] for i in spread(Ls, args.plain_sample)], args.plain_steps)
+        out["serve_ragged"] = serving_ragged_leg(fs, args.serve_ragged_rate, args.serve_ragged_steps, args.quantum,
+                                                 args.lanes, plain_v)
     if "uniform" in legs:
         note("uniform batch")
         out["uniform"] = uniform_leg(fs, args.uniform_steps, args.lanes)

@@ -1447,6 +1447,7 @@ extern "C" int fsx_autotune(fsx_ctx* c, int reps, int* chosen) {
       {&DevState::nr_small_gx, {16, 32, 64, 128}, 4, PH_LM_SORT, -1},
       {&DevState::area_wg, {256, 512, 1024, 0}, 3, PH_AREA_SWEEP, -1},
       {&DevState::bor_gx, {2, 4, 8, 16}, 4, PH_LM_MST, -1},
+      {&DevState::flow_nk, {4, 8, 16, 32}, 4, PH_SPL_SWEEP, -1},
       {&DevState::flow_wg, {256, 512, 1024, 0}, 3, PH_SPL_SWEEP, -1},
       {&DevState::flow_split, {0, 262144, 131072, 65536}, 4, PH_SPL_SWEEP, -1},
       {&DevState::walk_big, {64, 256, 1024, 8192}, 4, PH_LM_WALK, -1},
@@ -1529,6 +1530,7 @@ extern "C" int fsx_get_tuning(fsx_ctx* c, int* out) {
   out[8] = c->S.walk_big;
   out[9] = c->S.mfd_split;
   out[10] = c->S.tile_h;
+  out[11] = c->S.flow_nk;
   return 0;
 }
 
@@ -1547,6 +1549,7 @@ extern "C" int fsx_set_tuning(fsx_ctx* c, const int* in) {
   if (in[9] < 1024) return fail(FSX_EINVAL, "fsx_set_tuning: mfd_split must be >= 1024");
   if (in[10] != 4 && in[10] != 8 && in[10] != 16 && in[10] != 32)
     return fail(FSX_EINVAL, "fsx_set_tuning: tile_h must be 4, 8, 16 or 32");
+  if (in[11] < 1 || in[11] > 1024) return fail(FSX_EINVAL, "fsx_set_tuning: flow_nk must be in [1, 1024]");
   c->S.walk_gx = in[0];
   c->S.jump_blocks = in[1];
   c->S.tour_gx = in[2];
@@ -1561,6 +1564,7 @@ extern "C" int fsx_set_tuning(fsx_ctx* c, const int* in) {
   c->S.mfd_split = in[9];
   if (c->S.tile_h != in[10]) c->dirty = true;  // and the stencil kernels' tile table
   c->S.tile_h = in[10];
+  c->S.flow_nk = in[11];
   c->graph_valid = false;  // captured launches carry the old shapes
   return 0;
 }

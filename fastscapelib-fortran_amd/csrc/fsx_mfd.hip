@@ -112,7 +112,8 @@ __device__ __forceinline__ void mfd_phase(MfdTiles& T, int n, F&& body) {
 }
 
 // Loop of a dataflow tile: step(prog) tries the lane's position (sets prog when it finished it) and
-// returns whether the lane still has work; the wave polls until every lane is done.
+// returns whether the lane still has work; the wave polls until every lane is done, at a raised issue
+// priority while it makes progress.
 template <class F>
 __device__ __forceinline__ void mfd_lanes(MfdTiles& T, F&& step) {
   int spin = 0;
@@ -120,10 +121,15 @@ __device__ __forceinline__ void mfd_lanes(MfdTiles& T, F&& step) {
     bool prog = false;
     const bool more = step(prog);
     if (__ballot(more) == 0ull) return;
-    if (__ballot(prog) != 0ull) spin = 0;
-    else if (!mfd_idle(spin, T.err)) {
-      T.ok = false;
-      return;
+    if (__ballot(prog) != 0ull) {
+      spin = 0;
+      __builtin_amdgcn_s_setprio(2);
+    } else {
+      __builtin_amdgcn_s_setprio(0);
+      if (!mfd_idle(spin, T.err)) {
+        T.ok = false;
+        return;
+      }
     }
   }
 }

@@ -31,6 +31,7 @@ struct DevState {
   int mfd_split = 131072;  // multiple-flow dataflow sweeps: nodes per workgroup of a landscape
   int walk_big = 256;      // closed basins larger than this are searched node-parallel, not walked
   int tile_h = 8;          // stencil kernels (receivers, donor masks): rows per tile (4, 8, 16 or 32)
+  int flow_nk = 4;         // stream-power dataflow sweep: Newton iterations of a lane per pass of its wave
 
   // ---- per node (ntot) ----
   double *h = nullptr, *b = nullptr, *u = nullptr, *kf = nullptr, *kd = nullptr, *precip = nullptr;

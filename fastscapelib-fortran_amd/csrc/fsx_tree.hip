@@ -889,6 +889,8 @@ __global__ __launch_bounds__(FSX_WG) void k_spl_sweep(
 //   their global stores and loads coherent (workgroup-scope atomics).  A wave that sees no progress for
 //   FLOW_SPIN passes reports FSX_ERR_LEVELS and the workgroup stops; with a valid level order that cannot
 //   happen.
+// * A wave whose last pass made progress runs at a raised issue priority (s_setprio 2), an idle one at 0:
+//   on a compute unit crowded with polling waves, the waves that advance the front get the issue slots.
 #define FLOW_SPIN (1 << 26)
 #define FLOW_GLOBAL 32
 
@@ -985,9 +987,9 @@ __device__ __forceinline__ double shfl_d(double x, int src) {
 // ---- top-down: the single stream-power pass of g == 0 ----------------------------------------------
 // With g == 0 everywhere the Gauss-Seidel loop runs once (src/StreamPowerLaw.f90:375-376) and a node's
 // solve needs only its receiver's h, water and lake surface (:296-366, the fill :73-89).  A Newton lane
-// advances SPL_NK iterations per pass of its wave's loop.
+// advances nk iterations per pass of its wave's loop (a launch shape: few keep the passes of lanes that poll
+// short, more let a solve finish in the pass that found its receiver; the iterates are the same).
 #define SPL_R 4096  // ring slots
-#define SPL_NK 4
 
 struct FlowRec {
   int o, q;  // node, absolute receiver position (-1: base level)
@@ -1027,7 +1029,8 @@ __global__ __launch_bounds__(WG) void k_spl_flow(const InstParams* __restrict__ 
                                                  const SplRec32* __restrict__ srec, double* __restrict__ Hl,
                                                  double* __restrict__ Wl, double* __restrict__ HWl,
                                                  double* __restrict__ scal, int* __restrict__ ngs_out,
-                                                 long long* __restrict__ newton_out, int* __restrict__ err, int mode) {
+                                                 long long* __restrict__ newton_out, int* __restrict__ err, int mode,
+                                                 int nk) {
   extern __shared__ double spl_ring[];  // h[R], water[R], lake surface[R], tag[R], own-tile bitmap
   __shared__ FlowWin W;
   __shared__ unsigned long long s_newton;
@@ -1136,7 +1139,9 @@ __global__ __launch_bounds__(WG) void k_spl_flow(const InstParams* __restrict__ 
     } else if (__ballot(prog_) != 0ull) {                                                  \
       spin = 0;                                                                            \
       ST = used_;                                                                          \
+      __builtin_amdgcn_s_setprio(2);                                                       \
     } else {                                                                               \
+      __builtin_amdgcn_s_setprio(0);                                                       \
       ST = flow_idle(W, spin, err + inst, multi) ? used_ : 2;                              \
     }                                                                                      \
   } while (0)
@@ -1206,13 +1211,13 @@ __global__ __launch_bounds__(WG) void k_spl_flow(const InstParams* __restrict__ 
         }
       }
     }
-    // Newton lanes: up to SPL_NK iterations, then publish when converged (errp <= tolp)
+    // Newton lanes: up to nk iterations, then publish when converged (errp <= tolp)
     if (ph == 2) {
       prog = true;
       const double h0 = tt;
       const bool act = f >= 0.0;
       bool conv = false;
-      for (int k = 0; k < SPL_NK && !conv; k++) {
+      for (int k = 0; k < nk && !conv; k++) {
         double ff = hv - h0, df = 1.0;
         if (act) {
           const double x = fmax(0.0, hv - hr);
@@ -1343,13 +1348,13 @@ void launch_spl_flow(DevState& S, hipStream_t st, int mode) {
   const int g = S.nflow;
   if (S.flow_wg == 256)
     k_spl_flow<256><<<g, 256, sm, st>>>(S.P, S.flow_tab, S.flow_next, S.nlev, S.srec, S.Hl, S.Wl, S.HWl, S.scal, S.ngs,
-                                        S.newton, S.err, mode);
+                                        S.newton, S.err, mode, S.flow_nk);
   else if (S.flow_wg == 512)
     k_spl_flow<512><<<g, 512, sm, st>>>(S.P, S.flow_tab, S.flow_next, S.nlev, S.srec, S.Hl, S.Wl, S.HWl, S.scal, S.ngs,
-                                        S.newton, S.err, mode);
+                                        S.newton, S.err, mode, S.flow_nk);
   else
     k_spl_flow<1024><<<g, 1024, sm, st>>>(S.P, S.flow_tab, S.flow_next, S.nlev, S.srec, S.Hl, S.Wl, S.HWl, S.scal,
-                                          S.ngs, S.newton, S.err, mode);
+                                          S.ngs, S.newton, S.err, mode, S.flow_nk);
 }
 
 static void spl_sweep_launch(DevState& S, hipStream_t st, int fill_only) {

@@ -228,7 +228,7 @@ class Batch:
         check(lib.fsx_time_phase(self._h, k), "time_phase")
 
     TUNING = ("walk_gx", "jump_blocks", "tour_gx", "nr_small_gx", "area_wg", "bor_gx", "flow_wg", "flow_split",
-              "walk_big", "mfd_split", "tile_h")
+              "walk_big", "mfd_split", "tile_h", "flow_nk")
 
     def autotune(self, reps=2):
         """Choose the launch shapes of the routing kernels and of the dataflow stream-power sweep by

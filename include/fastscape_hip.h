@@ -250,11 +250,13 @@ int fsx_reset_timers(fsx_ctx *ctx);
  * read-outs (receivers, donors, area) then hold the tuning passes' routing of the current elevation
  * until the next step.  Also the basin size above which the pass search runs node-parallel instead of
  * one thread walking the basin (walk_big), and, when some landscape routes by multiple flow directions,
- * the nodes per workgroup of its dataflow sweeps (mfd_split), and the rows per 2-D tile of the
- * receiver and donor-mask stencil kernels (tile_h: 4, 8, 16 or 32; tiles are 64 columns wide).  chosen
+ * the nodes per workgroup of its dataflow sweeps (mfd_split), the rows per 2-D tile of the receiver and
+ * donor-mask stencil kernels (tile_h: 4, 8, 16 or 32; tiles are 64 columns wide), and the Newton iterations
+ * a lane of the dataflow stream-power sweep runs per pass of its wave (flow_nk: 4, 8, 16 or 32).  chosen
  * (FSX_NTUNE ints, or NULL) receives walk_gx, jump_blocks, tour_gx, nr_small_gx, area_wg, bor_gx, flow_wg,
- * flow_split, walk_big, mfd_split, tile_h.  No reference counterpart: the reference has no launch shapes. */
-#define FSX_NTUNE 11
+ * flow_split, walk_big, mfd_split, tile_h, flow_nk.  No reference counterpart: the reference has no launch
+ * shapes. */
+#define FSX_NTUNE 12
 int fsx_autotune(fsx_ctx *ctx, int reps, int *chosen);
 /* The launch shapes in use (FSX_NTUNE ints, same order as fsx_autotune's chosen). */
 int fsx_get_tuning(fsx_ctx *ctx, int *out);
@@ -263,7 +265,7 @@ int fsx_get_tuning(fsx_ctx *ctx, int *out);
    sort blocks <= 4096; area_wg one of 256, 512, 1024; bor_gx in [1, 1024]; flow_wg one of 256, 512,
    1024; flow_split >= 0 (nodes per workgroup of the dataflow sweep, 0: one workgroup per landscape; at
    most 16 workgroups per landscape); walk_big >= 64; mfd_split >= 1024 (at most 32 workgroups per
-   landscape); tile_h one of 4, 8, 16, 32.  A rejected call changes nothing.
+   landscape); tile_h one of 4, 8, 16, 32; flow_nk in [1, 1024].  A rejected call changes nothing.
    No counterpart in the reference. */
 int fsx_set_tuning(fsx_ctx *ctx, const int *in);
 

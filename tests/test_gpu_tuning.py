@@ -13,7 +13,7 @@ CANDIDATES = {"walk_gx": {64, 128, 256, 512}, "jump_blocks": {1024, 2048, 4096, 
               "nr_small_gx": {16, 32, 64, 128}, "area_wg": {256, 512, 1024}, "bor_gx": {2, 4, 8, 16},
               "flow_wg": {256, 512, 1024}, "flow_split": {0, 262144, 131072, 65536},
               "walk_big": {64, 256, 1024, 8192}, "mfd_split": {262144, 131072, 65536, 32768},
-              "tile_h": {4, 8, 16, 32}}
+              "tile_h": {4, 8, 16, 32}, "flow_nk": {4, 8, 16, 32}}
 
 
 def test_autotune_changes_shapes_not_results():
@@ -57,7 +57,7 @@ def test_set_tuning_roundtrip_and_bounds():
         # walk_big 256: every closed basin above 256 nodes takes the node-parallel pass search
         shapes = {"walk_gx": 64, "jump_blocks": 1024, "tour_gx": 32, "nr_small_gx": 16, "area_wg": 256,
                   "bor_gx": 1, "flow_wg": 256, "flow_split": 1000, "walk_big": 256, "mfd_split": 1024,
-                  "tile_h": 32}
+                  "tile_h": 32, "flow_nk": 9}
         b.set_tuning(shapes)
         assert b.tuning() == shapes
         with pytest.raises(_lib.FastscapeError):
@@ -70,6 +70,8 @@ def test_set_tuning_roundtrip_and_bounds():
             b.set_tuning({"walk_big": 32})
         with pytest.raises(_lib.FastscapeError):
             b.set_tuning({"tile_h": 12})
+        with pytest.raises(_lib.FastscapeError):
+            b.set_tuning({"flow_nk": 0})
         assert b.tuning() == shapes  # a rejected call changes nothing
         a.step(2)
         b.step(2)
