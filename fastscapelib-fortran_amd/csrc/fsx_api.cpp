@@ -65,7 +65,7 @@ struct fsx_ctx {
   bool any_marine = false, g_mar = false;  // some active instance runs the marine transport; as captured
   bool any_strati = false, g_str = false;  // some active instance tracks stratigraphic reflectors; as captured
   // one step captured as a HIP graph and replayed; rebuilt when the launch sequence changes
-  bool use_graph = true, graph_valid = false, g_sfd = false, g_mfd = false;
+  bool use_graph = true, graph_valid = false, g_sfd = false, g_mfd = false, g_sweep = false;
   hipGraphExec_t gexec = nullptr;
   // live timing of one phase inside graph replays: graph (phases before it), the phase launched
   // directly between two events, graph (phases after it)
@@ -205,6 +205,7 @@ static int build_tiles(fsx_ctx* c);
 static int upload_params(fsx_ctx* c) {
   if (!c->dirty) return 0;
   c->any_sfd = c->any_mfd = c->any_flow = false;
+  c->S.nsweep = 0;
   c->any_advect = c->any_marine = c->any_strati = false;
   for (InstParams& P : c->hp) {
     // g = g1, or g2 where h - b > 1 when g2 > 0 (src/StreamPowerLaw.f90:236-239): zero everywhere exactly when
@@ -216,6 +217,7 @@ static int upload_params(fsx_ctx* c) {
     if (!P.active || !P.run_spl) continue;
     if (P.sfd) c->any_sfd = true; else c->any_mfd = true;
     if (P.flow) c->any_flow = true;
+    if (P.sfd && !P.flow) c->S.nsweep++;
   }
   if (int r = build_flow_table(c)) return r;
   if (int r = build_mfd_table(c)) return r;
@@ -1059,6 +1061,7 @@ static int build_graphs(fsx_ctx* c, hipStream_t st) {
   c->graph_valid = true;
   c->g_sfd = c->any_sfd;
   c->g_mfd = c->any_mfd;
+  c->g_sweep = c->S.nsweep > 0;
   c->g_adv = c->any_advect;
   c->g_mar = c->any_marine;
   c->g_str = c->any_strati;
@@ -1087,7 +1090,8 @@ extern "C" int fsx_execute_steps(fsx_ctx* c, int nsteps, void* stream, int sync)
   if (c->use_graph && !c->prof && nsteps > 0) {
     // the step's launch sequence depends on the layout and on which routing paths are in use; the
     // kernels read every other parameter from device memory, so one capture serves until those change
-    if (!c->graph_valid || c->g_sfd != c->any_sfd || c->g_mfd != c->any_mfd || c->g_adv != c->any_advect ||
+    if (!c->graph_valid || c->g_sfd != c->any_sfd || c->g_mfd != c->any_mfd || c->g_sweep != (c->S.nsweep > 0) ||
+        c->g_adv != c->any_advect ||
         c->g_mar != c->any_marine || c->g_str != c->any_strati || c->g_timed != c->timed_phase) {
       if (int r = build_graphs(c, st)) return r;
     }

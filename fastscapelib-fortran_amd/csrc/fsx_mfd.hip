@@ -313,29 +313,30 @@ __global__ __launch_bounds__(MFD_WG) void k_mfd_kahn(const InstParams* __restric
   MFD_WAVE_PROLOGUE
   int* cq = ctr + FSX_MFD_CTR * inst + 4;  // claim, done, queue tail, largest layer
   int* hist = mls + P.aux;
-  const unsigned long long below = (1ull << lane) - 1ull;
   MfdTiles T;
   mfd_start(T, cq, err + inst);
-  // the lanes with `pushme` append node u of layer lay: one queue reservation per wave, one histogram
-  // count per distinct layer, one maximum
-  auto push = [&](bool pushme, int u, unsigned lay) {
-    const unsigned long long m = __ballot(pushme);
-    if (m == 0ull) return;
-    const int first = __ffsll((long long)m) - 1;
+  // Appending to the queue: a wave reserves the positions of all the nodes its lanes release in one step with
+  // one atomic on the tail (cq[2]), counts them in the layer histogram with one atomic per distinct layer, and
+  // keeps the largest layer it pushed in a register (one maximum per wave at the end): the tail, the histogram
+  // and the maximum are the contended words of the traversal.
+  unsigned mymax = 0;
+  auto reserve = [&](int cnt) {  // the first queue position of this lane's cnt entries
+    const int incl = wave_incl_scan(cnt);
+    const int total = __shfl(incl, 63, 64);
     int base = 0;
-    if (lane == first) base = __hip_atomic_fetch_add(cq + 2, __popcll(m), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    base = __shfl(base, first, 64);
-    if (pushme) {
-      const int idx = base + __popcll(m & below);
-      if (idx < nn) flow_stu_dev(mq + off + idx, ((unsigned long long)lay << 32) | (unsigned long long)(unsigned)u);
+    if (total) {
+      if (lane == 63) base = __hip_atomic_fetch_add(cq + 2, total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      base = __shfl(base, 63, 64);
     }
-    unsigned top = pushme ? lay : 0u;
-    for (int o = 32; o > 0; o >>= 1) {
-      const unsigned y = (unsigned)__shfl_xor((int)top, o, 64);
-      top = y > top ? y : top;
-    }
-    if (lane == first) atomicMax(cq + 3, (int)top);
-    unsigned long long rem = m;
+    return base + incl - cnt;
+  };
+  auto enqueue = [&](int& idx, int u, unsigned lay) {
+    if (idx < nn) flow_stu_dev(mq + off + idx, ((unsigned long long)lay << 32) | (unsigned long long)(unsigned)u);
+    idx++;
+    mymax = lay > mymax ? lay : mymax;
+  };
+  auto hist_add = [&](bool pushme, unsigned lay) {
+    unsigned long long rem = __ballot(pushme);
     while (rem) {
       const int ld = __ffsll((long long)rem) - 1;
       const unsigned l0 = (unsigned)__shfl((int)lay, ld, 64);
@@ -344,12 +345,22 @@ __global__ __launch_bounds__(MFD_WG) void k_mfd_kahn(const InstParams* __restric
       rem &= ~same;
     }
   };
+  // the sources (no in-edges), layer 0
   mfd_phase(T, (nn + 255) / 256, [&](int tt) {
+    bool z[4];
+    int cnt = 0;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
       const int u = 256 * tt + 64 * k + lane;
-      push(u < nn && (unsigned)mkahn[off + (u < nn ? u : 0)] == 0u, u, 0u);
+      z[k] = u < nn && (unsigned)mkahn[off + (u < nn ? u : 0)] == 0u;
+      cnt += z[k] ? 1 : 0;
     }
+    int idx = reserve(cnt);
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      if (z[k]) enqueue(idx, 256 * tt + 64 * k + lane, 0u);
+    const int all = __shfl(wave_incl_scan(cnt), 63, 64);
+    if (lane == 0 && all) atomicAdd(hist, all);
   });
   mfd_phase(T, (nn + 63) / 64, [&](int tt) {
     const int p = 64 * tt + lane;
@@ -370,14 +381,13 @@ __global__ __launch_bounds__(MFD_WG) void k_mfd_kahn(const InstParams* __restric
           if (mflag[off + u]) nr++;  // the ghost edge to the single-flow receiver, last
         }
       }
-      // receiver slots in step over the wave, so that the nodes they release are appended together
+      // receiver slots in step over the wave: each takes one in-edge of its node (layer at least lay + 1);
+      // the slots whose node this lane released (its last in-edge) are marked in relm
+      unsigned relm = 0;
       for (int k = 0; __ballot(have && k < nr) != 0ull; k++) {
-        bool rel = false;
-        int t = 0;
-        unsigned nl = 0;
         if (have && k < nr) {
           const long long gu = off + u;
-          t = (k < mnrec[gu]) ? mrec[8 * gu + k] : rec[gu];
+          const int t = (k < mnrec[gu]) ? mrec[8 * gu + k] : rec[gu];
           unsigned long long* s = mkahn + off + t;
           unsigned long long old = flow_ldu_dev(s), seen, nw;
           do {
@@ -387,14 +397,30 @@ __global__ __launch_bounds__(MFD_WG) void k_mfd_kahn(const InstParams* __restric
                  (unsigned long long)((unsigned)(seen & 0xffffffffull) - 1u);
             old = atomicCAS(s, seen, nw);
           } while (old != seen);
-          rel = (unsigned)(seen & 0xffffffffull) == 1u;
-          nl = (unsigned)(nw >> 32);
+          if ((unsigned)(seen & 0xffffffffull) == 1u) relm |= 1u << k;
         }
-        push(rel, t, nl);
+      }
+      // the released nodes, with their final layers, appended in slot order
+      int idx = reserve(__popc(relm));
+      for (int k = 0; __ballot((relm >> k) != 0u) != 0ull; k++) {
+        const bool pushme = (relm >> k) & 1u;
+        unsigned nl = 0;
+        if (pushme) {
+          const long long gu = off + u;
+          const int t = (k < mnrec[gu]) ? mrec[8 * gu + k] : rec[gu];
+          nl = (unsigned)(flow_ldu_dev(mkahn + off + t) >> 32);  // final: every in-edge of t is taken
+          enqueue(idx, t, nl);
+        }
+        hist_add(pushme, nl);
       }
       return todo;
     });
   });
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned y = (unsigned)__shfl_xor((int)mymax, o, 64);
+    mymax = y > mymax ? y : mymax;
+  }
+  if (lane == 0 && mymax) atomicMax(cq + 3, (int)mymax);
 }
 
 // Layer starts from the histogram (exclusive scan, one workgroup per landscape) and the scatter cursors;

@@ -18,6 +18,7 @@ struct DevState {
   int nmfd = 0;             // workgroups of the multiple-flow dataflow sweeps over all instances
   int2* mfd_tab = nullptr;  // [nmfd]: (instance, rank among the instance's workgroups)
   int* flow_next = nullptr; // [ninst]: the sweep's tile counter of each instance
+  int nsweep = 0;           // active single-flow landscapes with deposition (the level sweep k_spl_sweep)
   // launch shapes, chosen by fsx_autotune (no result depends on them)
   int walk_gx = 256;       // basin walk: blocks per landscape
   int jump_blocks = 2048;  // pointer jumping: grid cap

@@ -1358,9 +1358,11 @@ void launch_spl_flow(DevState& S, hipStream_t st, int mode) {
 }
 
 static void spl_sweep_launch(DevState& S, hipStream_t st, int fill_only) {
-  // every landscape is taken by exactly one of the two kernels (InstParams::flow)
+  // every landscape is taken by exactly one of the two kernels (InstParams::flow); the level sweep is not
+  // launched when no landscape takes it (its workgroups would wait for compute units with its LDS free)
   launch_spl_flow(S, st, fill_only);
-  k_spl_sweep<<<S.ninst, FSX_WG, spl_smem(kSplVcap), st>>>(
+  if (S.nsweep)
+    k_spl_sweep<<<S.ninst, FSX_WG, spl_smem(kSplVcap), st>>>(
       S.P, S.lstart, S.nlev, S.grec, S.ngroups, S.srec, S.arec, S.bc, S.g, S.Al, S.h, S.hwater,
       S.lake_depth, S.water, S.lsill, S.lakec, S.lsed, S.Hl, S.Wl, S.HWl, S.LSl, S.HPl, S.DHl, S.ELl, S.scal, S.ngs,
       S.newton, S.err, kSplVcap, fill_only, S.redk);
