@@ -54,7 +54,13 @@ __global__ __launch_bounds__(256) void k_route_tile(const InstParams* __restrict
     }
     return;
   }
-  for (int c = threadIdx.x; c < W * H; c += 256) {
+  // every load of the thread is issued before any is used (the tile's staging waits for memory once): the
+  // halo cells' h and u, and b of the thread's own nodes (updated after the barrier)
+  constexpr int NH = (W * H + 255) / 256, NPT = FSX_TW * TH / 256;
+  double hh[NH], uu[NH], bb[NPT];
+#pragma unroll
+  for (int m = 0; m < NH; m++) {
+    const int c = (int)threadIdx.x + 256 * m;
     const int ly = c / W, lx = c - ly * W;
     int gx = x0 + lx - 1, gy = y0 + ly - 1;
     if (gx < 0 && P.xcyc) gx += nx;
@@ -64,16 +70,33 @@ __global__ __launch_bounds__(256) void k_route_tile(const InstParams* __restrict
     if (gy > ny - 1 && P.ycyc) gy -= ny;
     gy = gy < 0 ? 0 : (gy > ny - 1 ? ny - 1 : gy);
     const long long g = off + gx + (long long)gy * nx;
-    double hv = h[g];
+    const bool in = c < W * H;
+    hh[m] = in ? h[g] : 0.0;
+    uu[m] = (in && up) ? u[g] : 0.0;
+  }
+#pragma unroll
+  for (int m = 0; m < NPT; m++) {
+    const int k = (int)threadIdx.x + 256 * m;
+    const int i = x0 + k % FSX_TW, j = y0 + k / FSX_TW;
+    bb[m] = (up && i < nx && j < ny) ? b[off + i + (long long)j * nx] : 0.0;
+  }
+#pragma unroll
+  for (int m = 0; m < NH; m++) {
+    const int c = (int)threadIdx.x + 256 * m;
+    if (c >= W * H) continue;
+    const int ly = c / W, lx = c - ly * W;
+    double hv = hh[m];
     if (up) {
-      const double ud = u[g] * dt;
+      const double ud = uu[m] * dt;
       hv = hv + ud;
       if (lx >= 1 && lx <= FSX_TW && ly >= 1 && ly <= TH) sud[ly - 1][lx - 1] = ud;
     }
     sh[ly][lx] = hv;
   }
   __syncthreads();
-  for (int k = threadIdx.x; k < FSX_TW * TH; k += 256) {
+#pragma unroll
+  for (int m = 0; m < NPT; m++) {
+    const int k = (int)threadIdx.x + 256 * m;
     const int tx = k % FSX_TW, ty = k / FSX_TW;
     const int i = x0 + tx, j = y0 + ty;
     if (i >= nx || j >= ny) continue;
@@ -81,7 +104,7 @@ __global__ __launch_bounds__(256) void k_route_tile(const InstParams* __restrict
     const long long g = off + v;
     const double hv = sh[ty + 1][tx + 1];
     ht[g] = hv;
-    if (up) b[g] = b[g] + sud[ty][tx];
+    if (up) b[g] = bb[m] + sud[ty][tx];
     int r = v;
     double l_best = 0.0;
     if (i >= P.i1 && i <= P.i2 && j >= P.j1 && j <= P.j2) {
@@ -128,7 +151,11 @@ __global__ __launch_bounds__(256) void k_donors_tile(const InstParams* __restric
   if (!P.active || !P.run_spl) return;
   const int x0 = T.y, y0 = T.z, nx = P.nx, ny = P.ny;
   const long long off = P.off;
-  for (int c = threadIdx.x; c < W * H; c += 256) {
+  constexpr int NH = (W * H + 255) / 256;
+  int rr[NH];  // every load issued before any is used
+#pragma unroll
+  for (int m = 0; m < NH; m++) {
+    const int c = (int)threadIdx.x + 256 * m;
     const int ly = c / W, lx = c - ly * W;
     int gx = x0 + lx - 1, gy = y0 + ly - 1;
     if (gx < 0 && P.xcyc) gx += nx;
@@ -137,10 +164,17 @@ __global__ __launch_bounds__(256) void k_donors_tile(const InstParams* __restric
     if (gy < 0 && P.ycyc) gy += ny;
     if (gy > ny - 1 && P.ycyc) gy -= ny;
     gy = gy < 0 ? 0 : (gy > ny - 1 ? ny - 1 : gy);
-    sr[ly][lx] = rec[off + gx + (long long)gy * nx];
+    rr[m] = c < W * H ? rec[off + gx + (long long)gy * nx] : 0;
+  }
+#pragma unroll
+  for (int m = 0; m < NH; m++) {
+    const int c = (int)threadIdx.x + 256 * m;
+    if (c < W * H) sr[c / W][c % W] = rr[m];
   }
   __syncthreads();
-  for (int k = threadIdx.x; k < FSX_TW * TH; k += 256) {
+#pragma unroll
+  for (int kk = 0; kk < FSX_TW * TH / 256; kk++) {
+    const int k = (int)threadIdx.x + 256 * kk;
     const int tx = k % FSX_TW, ty = k / FSX_TW;
     const int i = x0 + tx, j = y0 + ty;
     if (i >= nx || j >= ny) continue;
