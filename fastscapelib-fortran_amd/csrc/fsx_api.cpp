@@ -953,6 +953,7 @@ static int check_errors(fsx_ctx* c) {
     if (e[k] & FSX_ERR_LEVELS) why += " receiver graph is not a forest;";
     if (e[k] & FSX_ERR_MFD) why += " multiple-flow order incomplete;";
     if (e[k] & FSX_ERR_MARINE) why += " marine silt / sand diffusion did not converge in 1000 iterations;";
+    if (e[k] & FSX_ERR_NONFINITE) why += " elevation is not a number;";
     return fail(FSX_ESTEP, "instance " + std::to_string(k) + ":" + why);
   }
   return 0;

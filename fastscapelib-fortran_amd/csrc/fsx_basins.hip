@@ -584,11 +584,24 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_big_cand(const InstParams* __rest
   const long long sbase = (long long)FSX_SLAB_CAP * (P.off + soff[P.off + A]);
   const unsigned C = (unsigned)big_cap(sz);
   unsigned long long* SW = reinterpret_cast<unsigned long long*>(sw);
+  // the examined neighbours' records, then the first probe of each one's basin, all in flight together (a
+  // slot once taken keeps its basin, so a first probe read early is either final or still -1)
+  ulonglong2 wqs[8];
+  int first[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++)
+    if ((mv >> k) & 1u) wqs[k] = WN[(j + JJ[k]) * P.nx + (i + II[k])];
+#pragma unroll
+  for (int k = 0; k < 8; k++)
+    if ((mv >> k) & 1u) {
+      const int Bs = (int)(unsigned)(wqs[k].y & 0xffffffffull);
+      const int B = Bs >= 0 ? Bs : -(Bs + 1);
+      first[k] = sh[sbase + (((unsigned)B * 2654435761u) & (C - 1u))];
+    }
 #pragma unroll
   for (int k = 0; k < 8; k++) {
     if (!((mv >> k) & 1u)) continue;
-    const int q = (j + JJ[k]) * P.nx + (i + II[k]);
-    const ulonglong2 wq = WN[q];
+    const ulonglong2 wq = wqs[k];
     const int Bs = (int)(unsigned)(wq.y & 0xffffffffull);
     const int B = Bs >= 0 ? Bs : -(Bs + 1);
     const double hq = __longlong_as_double((long long)wq.x);
@@ -598,7 +611,7 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_big_cand(const InstParams* __rest
     unsigned s = ((unsigned)B * 2654435761u) & (C - 1u);
     long long slot = -1;
     for (unsigned pr = 0; pr < C; pr++, s = (s + 1u) & (C - 1u)) {
-      int cur = sh[sbase + s];
+      int cur = pr == 0 ? first[k] : sh[sbase + s];
       if (pass == 0 && cur == -1) cur = atomicCAS(sh + sbase + s, -1, B);
       if (cur == B || (pass == 0 && cur == -1)) {
         slot = sbase + s;
@@ -956,16 +969,22 @@ __device__ __forceinline__ int* nr_list(double* cw, long long eb, int n, int lev
 __global__ __launch_bounds__(FSX_WG) void k_nr_init(const InstParams* __restrict__ PP, const int* __restrict__ nconn,
                                                     double* __restrict__ cw, double* __restrict__ skey,
                                                     int* __restrict__ sid, int* __restrict__ segs,
-                                                    int* __restrict__ nrst) {
+                                                    int* __restrict__ nrst, int* __restrict__ err) {
   const int inst = blockIdx.x;
   const InstParams& P = PP[inst];
   if (!P.active || !P.run_spl) return;
   const int n = nconn[inst];
   const long long eb = (long long)FSX_EDGE_CAP * P.off;
+  // A pass height that is not a number (elevations that are not) would break the partition scans' sentinel
+  // (every comparison with it is false): it is sorted as +infinity and the step reports the landscape.
+  bool nan = false;
   for (int p = threadIdx.x; p < n; p += FSX_WG) {
-    skey[eb + p] = cw[eb + p];
+    const double kv = cw[eb + p];
+    nan |= kv != kv;
+    skey[eb + p] = kv == kv ? kv : __builtin_huge_val();
     sid[eb + p] = p;
   }
+  if (nan) atomicOr(err + inst, FSX_ERR_NONFINITE);
   __syncthreads();  // every key copied before the level lists overwrite cw
   if (threadIdx.x == 0) {
     int* st = nrst + NR_ST * inst;
@@ -1765,7 +1784,7 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_correct(const InstParams* __restr
 // ---------------------------------------------------------------------------------------------------
 void launch_nr_sort(DevState& S, hipStream_t st) {
   launch_fill_int(S.segctr, S.ninst, 0, st);
-  k_nr_init<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.nconn, S.cw, S.skey, S.sid, S.segs, S.nrst);
+  k_nr_init<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.nconn, S.cw, S.skey, S.sid, S.segs, S.nrst, S.err);
   // levels for the largest landscape's edge count (6 per node at most): log2(edges / NR_BIG) plus slack
   // for unbalanced partitions, within the per-landscape counters
   int nlev = 4;

@@ -29,6 +29,7 @@ enum {
   FSX_ERR_LEVELS = 32,     // receiver graph has a cycle (stack incomplete)
   FSX_ERR_MFD = 64,        // multiple-flow-direction order incomplete (reference STOPs)
   FSX_ERR_MARINE = 128,    // silt / sand diffusion did not converge in 1000 iterations (reference STOPs)
+  FSX_ERR_NONFINITE = 256, // a pass height (elevation) is not a number
 };
 
 struct InstParams {
