@@ -299,20 +299,35 @@ void k_basin_walk(const InstParams* __restrict__ PP, const int* __restrict__ nba
     ecount[off + b] = ne;
   };
   bool live = next_basin();
+  // the record and mask of v: loaded at a basin's first node, else prefetched with the previous node's
+  // neighbours (the preorder successor is known once v's record arrives), so the chase of successors and
+  // the neighbours' loads overlap
+  ulonglong2 wv = make_ulonglong2(0ull, 0ull);
+  unsigned mv = 0;
+  bool have = false;
   while (live) {
     if (++visits > P.nn) {  // not a tree: give up on this basin
       atomicOr(err + inst, FSX_ERR_LEVELS);
       finish();
       live = next_basin();
+      have = false;
       continue;
     }
     // scan the 8 neighbours of v (no wrap), src/LocalMinima.f90:281-320: only those k_walk_mask
     // marked (inside the grid, open basin or larger id) are loaded
     const int i = v % nx, j = v / nx;
-    const ulonglong2 wv = WN[v];
-    const unsigned mv = MK[v];
+    if (!have) {
+      wv = WN[v];
+      mv = MK[v];
+    }
     const double hv = __longlong_as_double((long long)wv.x);
     const int fv = (int)(unsigned)(wv.y >> 32);
+    ulonglong2 wnext = make_ulonglong2(0ull, 0ull);
+    unsigned mnext = 0;
+    if (fv >= 0) {
+      wnext = WN[fv];
+      mnext = MK[fv];
+    }
     int qs[8], bs[8];
     double hq[8];
 #pragma unroll
@@ -377,9 +392,13 @@ void k_basin_walk(const InstParams* __restrict__ PP, const int* __restrict__ nba
     // next node in depth-first preorder (k_basin_label's successor), -1 past the basin's last node
     if (fv >= 0) {
       v = fv;
+      wv = wnext;
+      mv = mnext;
+      have = true;
     } else {
       finish();
       live = next_basin();
+      have = false;
     }
   }
 }
