@@ -149,16 +149,20 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_basin_label(const InstParams* __r
     const uint8_t* DM = dmask + P.off;
     int sv = donor_after(P, v, DM[v], -1);
     if (sv < 0) {
-      int u = v;
+      // a climb: each step reads the receiver's donor mask and its own receiver together (one memory round
+      // trip per step)
+      int u = v, ru = rec[P.off + v];
       for (int guard = 0; guard < P.nn; guard++) {  // the receivers form a forest; bounded anyway
-        const int ru = rec[P.off + u];
         if (ru == u) break;
-        const int nu = donor_after(P, ru, DM[ru], u);
+        const unsigned mr = DM[ru];
+        const int rru = rec[P.off + ru];
+        const int nu = donor_after(P, ru, mr, u);
         if (nu >= 0) {
           sv = nu;
           break;
         }
         u = ru;
+        ru = rru;
       }
     }
     reinterpret_cast<ulonglong2*>(wnode)[P.off + v] =

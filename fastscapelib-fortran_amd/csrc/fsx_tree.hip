@@ -76,7 +76,7 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_jump(const InstParams* __restrict
 // (so a group has at most GRP_BUCKET levels and GRP_STG nodes; a level wider than GRP_HALF forms its
 // own group); the sweeps stage one group at a time into LDS.
 #define LVL_HCAP 32768
-#define LVL_UNROLL 8
+#define LVL_UNROLL 32
 #define GRP_HALF 1024                // a level wider than this is a group of its own
 #define GRP_BUCKET 1024              // levels starting in one bucket of positions share a group
 #define GRP_STG (GRP_HALF + GRP_BUCKET)  // positions of a group (staging capacity)
