@@ -278,6 +278,11 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_spl_fact(const InstParams* __rest
   if (!P.active || !P.run_spl || !P.sfd) return;  // uniform over the block: one chunk, one landscape
   const bool live = v < P.nn;
   const long long g_ = (long long)P.off + v;
+  // the node's level-order inputs, loaded before the block reduction so their latency overlaps it
+  const bool ordered = live && !flags[FSX_JUMP_ROUNDS - 1];
+  const int p = ordered ? pos[g_] : 0;
+  const int d = ordered ? jdist(J[g_]) : 0;
+  const int r = ordered ? rec[g_] : 0;
   // the landscape's maxval(abs(h)) and maxval(g) (src/StreamPowerLaw.f90:246,376), h == ht here:
   // reduced over the block, then one atomic per block
   unsigned long long kh = live ? dkey(fabs(ht[g_])) : 0ull, kg = live ? dkey(g[g_]) : 0ull;
@@ -296,11 +301,8 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_spl_fact(const InstParams* __rest
     atomicMax(redk + 2 * c.inst, kh);
     atomicMax(redk + 2 * c.inst + 1, kg);
   }
-  if (!live || flags[FSX_JUMP_ROUNDS - 1]) return;  // no level order this step (receiver cycle)
-  const int p = pos[g_];
-  const int d = jdist(J[g_]);
+  if (!ordered) return;  // no level order this step (receiver cycle)
   const long long gp = (long long)P.off + p;
-  const int r = rec[g_];
   const double t = ht[g_];
   const double av = Al[gp];  // drainage area, level order -> grid order
   area[g_] = av;
