@@ -304,6 +304,12 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_mfd_don(const InstParams* __restr
 // in-edges (layer 0); in phase 1 a lane takes queue position p, waits until it is written and releases
 // the node's receivers.  The layer sets are the reference's Kahn layers; the order inside a layer varies
 // from run to run.
+// Kahn state after one in-edge of a node of layer lay is taken: (max(layer, lay + 1) << 32) | (in-edges - 1)
+__device__ __forceinline__ unsigned long long kahn_take(unsigned long long w, unsigned lay) {
+  const unsigned hl = (unsigned)(w >> 32), want = lay + 1u;
+  return ((unsigned long long)(hl > want ? hl : want) << 32) | (unsigned long long)((unsigned)(w & 0xffffffffull) - 1u);
+}
+
 __global__ __launch_bounds__(MFD_WG) void k_mfd_kahn(const InstParams* __restrict__ PP, const int2* __restrict__ mtab,
                                                      int* __restrict__ ctr, int* __restrict__ err,
                                                      unsigned long long* __restrict__ mkahn, const int* __restrict__ mrec,
@@ -367,7 +373,7 @@ __global__ __launch_bounds__(MFD_WG) void k_mfd_kahn(const InstParams* __restric
     bool todo = p < nn;
     mfd_lanes(T, [&](bool& prog) {
       bool have = false;
-      int u = 0, nr = -1;
+      int u = 0, nr = -1, nm = 0;
       unsigned lay = 0;
       if (todo) {
         const unsigned long long e = flow_ldu_dev(mq + off + p);
@@ -377,41 +383,49 @@ __global__ __launch_bounds__(MFD_WG) void k_mfd_kahn(const InstParams* __restric
           have = true;
           u = (int)(unsigned)(e & 0xffffffffull);
           lay = (unsigned)(e >> 32);
-          nr = mnrec[off + u];
-          if (mflag[off + u]) nr++;  // the ghost edge to the single-flow receiver, last
+          nm = mnrec[off + u];
+          nr = nm + (mflag[off + u] ? 1 : 0);  // the ghost edge to the single-flow receiver, last
         }
       }
-      // receiver slots in step over the wave: each takes one in-edge of its node (layer at least lay + 1);
-      // the slots whose node this lane released (its last in-edge) are marked in relm
+      // Each receiver slot takes one in-edge of its node (layer at least lay + 1).  All slots of a lane go
+      // together: their targets, then their words, then one compare-and-swap each; a slot whose word changed
+      // since it was read retries from the value it found.  The slots whose node this lane released (its last
+      // in-edge) are marked in relm, with the node's final layer.
+      const long long gu = off + u;
       unsigned relm = 0;
-      for (int k = 0; __ballot(have && k < nr) != 0ull; k++) {
-        if (have && k < nr) {
-          const long long gu = off + u;
-          const int t = (k < mnrec[gu]) ? mrec[8 * gu + k] : rec[gu];
-          unsigned long long* s = mkahn + off + t;
-          unsigned long long old = flow_ldu_dev(s), seen, nw;
-          do {
-            seen = old;
-            const unsigned hl = (unsigned)(seen >> 32), want = lay + 1u;
-            nw = ((unsigned long long)(hl > want ? hl : want) << 32) |
-                 (unsigned long long)((unsigned)(seen & 0xffffffffull) - 1u);
-            old = atomicCAS(s, seen, nw);
-          } while (old != seen);
-          if ((unsigned)(seen & 0xffffffffull) == 1u) relm |= 1u << k;
+      int tk[9];
+      unsigned nlk[9];
+      unsigned long long sk[9], ok[9];
+#pragma unroll
+      for (int k = 0; k < 9; k++) {
+        tk[k] = 0;
+        nlk[k] = 0;
+        if (have && k < nr) tk[k] = k < nm ? mrec[8 * gu + k] : rec[gu];
+      }
+#pragma unroll
+      for (int k = 0; k < 9; k++) sk[k] = (have && k < nr) ? flow_ldu_dev(mkahn + off + tk[k]) : 0ull;
+#pragma unroll
+      for (int k = 0; k < 9; k++) ok[k] = (have && k < nr) ? atomicCAS(mkahn + off + tk[k], sk[k], kahn_take(sk[k], lay)) : 0ull;
+#pragma unroll
+      for (int k = 0; k < 9; k++) {
+        if (!(have && k < nr)) continue;
+        unsigned long long seen = sk[k], old = ok[k];
+        while (old != seen) {
+          seen = old;
+          old = atomicCAS(mkahn + off + tk[k], seen, kahn_take(seen, lay));
+        }
+        if ((unsigned)(seen & 0xffffffffull) == 1u) {
+          relm |= 1u << k;
+          nlk[k] = (unsigned)(kahn_take(seen, lay) >> 32);
         }
       }
       // the released nodes, with their final layers, appended in slot order
       int idx = reserve(__popc(relm));
-      for (int k = 0; __ballot((relm >> k) != 0u) != 0ull; k++) {
+#pragma unroll
+      for (int k = 0; k < 9; k++) {
         const bool pushme = (relm >> k) & 1u;
-        unsigned nl = 0;
-        if (pushme) {
-          const long long gu = off + u;
-          const int t = (k < mnrec[gu]) ? mrec[8 * gu + k] : rec[gu];
-          nl = (unsigned)(flow_ldu_dev(mkahn + off + t) >> 32);  // final: every in-edge of t is taken
-          enqueue(idx, t, nl);
-        }
-        hist_add(pushme, nl);
+        if (pushme) enqueue(idx, tk[k], nlk[k]);
+        hist_add(pushme, nlk[k]);
       }
       return todo;
     });
