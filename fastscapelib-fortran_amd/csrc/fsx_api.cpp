@@ -55,6 +55,8 @@ struct fsx_ctx {
   std::vector<int2> flow_host;  // the dataflow sweep's workgroup table as uploaded
   std::vector<int2> mfd_host;   // the multiple-flow sweeps' workgroup table as uploaded
   long long mfd_cap = 0;
+  std::vector<int2> lvl_host;   // the multi-workgroup level sort's table as uploaded
+  long long lvl_cap = 0;
   std::vector<void*> allocs;
   hipStream_t stream = nullptr;
   hipStream_t last = nullptr;  // the caller's stream of the last fsx_execute_steps that was given one
@@ -200,6 +202,7 @@ static int check_inst(fsx_ctx* c, int inst) {
 
 static int build_flow_table(fsx_ctx* c);
 static int build_mfd_table(fsx_ctx* c);
+static int build_lvl_table(fsx_ctx* c);
 static int build_tiles(fsx_ctx* c);
 
 static int upload_params(fsx_ctx* c) {
@@ -221,6 +224,7 @@ static int upload_params(fsx_ctx* c) {
   }
   if (int r = build_flow_table(c)) return r;
   if (int r = build_mfd_table(c)) return r;
+  if (int r = build_lvl_table(c)) return r;
   if (c->S.tile_h != c->S.tile_rows)
     if (int r = build_tiles(c)) return r;
   HIPCHK(hipMemcpyAsync(c->S.P, c->hp.data(), sizeof(InstParams) * c->hp.size(), hipMemcpyHostToDevice, c->stream));
@@ -386,6 +390,45 @@ static int build_mfd_table(fsx_ctx* c) {
     HIPCHK(hipStreamSynchronize(c->stream));
   }
   c->mfd_host = std::move(tab);
+  c->graph_valid = false;
+  return 0;
+}
+
+// Level sort of the large landscapes: one workgroup per FSX_LVL_SEG-node segment of every stream-power
+// landscape of at least FSX_LVL_BIG nodes (the others are sorted by one workgroup each), then one entry
+// (instance, segments) per such landscape for the level scan.
+static int build_lvl_table(fsx_ctx* c) {
+  DevState& S = c->S;
+  std::vector<int2> tab, big;
+  for (int k = 0; k < S.ninst; k++) {
+    InstParams& P = c->hp[k];
+    const bool on = c->used[k] && P.active && P.run_spl && P.nn >= FSX_LVL_BIG;
+    P.lvl_segs = on ? (P.nn + FSX_LVL_SEG - 1) / FSX_LVL_SEG : 0;
+    for (int r = 0; r < P.lvl_segs; r++) tab.push_back(make_int2(k, r));
+    if (on) big.push_back(make_int2(k, P.lvl_segs));
+  }
+  S.nlvl = (int)tab.size();
+  S.nlvl_big = (int)big.size();
+  tab.insert(tab.end(), big.begin(), big.end());
+  const bool same = tab.size() == c->lvl_host.size() &&
+                    std::equal(tab.begin(), tab.end(), c->lvl_host.begin(),
+                               [](const int2& a, const int2& b) { return a.x == b.x && a.y == b.y; });
+  if (same) return 0;
+  HIPCHK(quiesce(c));  // queued launches may still read the old table
+  if ((long long)tab.size() > c->lvl_cap) {
+    if (S.lvl_tab) HIPCHK(hipFree(S.lvl_tab));
+    S.lvl_tab = nullptr;
+    c->lvl_cap = 0;
+    const long long cap = (long long)tab.size() + 64;
+    hipError_t e = hipMalloc(&S.lvl_tab, sizeof(int2) * cap);
+    if (e != hipSuccess) return fail(FSX_ENOMEM, std::string("hipMalloc (level sort table): ") + hipGetErrorString(e));
+    c->lvl_cap = cap;
+  }
+  if (!tab.empty()) {
+    HIPCHK(hipMemcpyAsync(S.lvl_tab, tab.data(), sizeof(int2) * tab.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  c->lvl_host = std::move(tab);
   c->graph_valid = false;
   return 0;
 }
@@ -645,6 +688,7 @@ extern "C" int fsx_destroy(fsx_ctx* c) {
   if (c->S.tiles) (void)hipFree(c->S.tiles);
   if (c->S.flow_tab) (void)hipFree(c->S.flow_tab);
   if (c->S.mfd_tab) (void)hipFree(c->S.mfd_tab);
+  if (c->S.lvl_tab) (void)hipFree(c->S.lvl_tab);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return 0;

@@ -18,6 +18,8 @@
 #define FSX_WG 1024      // threads of the per-landscape workgroups
 #define FSX_TW 64        // 2-D tiles of the stencil kernels: columns (rows: the context's tile_h, 4 to 32)
 #define FSX_MFD_CTR 16   // counters per landscape of the multiple-flow dataflow launches
+#define FSX_LVL_BIG 131072  // landscapes of at least this many nodes are level-sorted by several workgroups
+#define FSX_LVL_SEG 16384   // nodes per workgroup of that sort
 
 // device-side error bits, one word per instance
 enum {
@@ -44,6 +46,7 @@ struct InstParams {
                              // solved node by node as receivers complete (k_spl_flow)
   int flow_g;                // workgroups of k_spl_flow on this landscape
   int mfd_g;                 // workgroups of the multiple-flow dataflow sweeps on this landscape
+  int lvl_segs;              // segments of the multi-workgroup level sort (0: sorted by one workgroup)
   int run_marine;            // marine transport after Diffusion (SetMarineParam)
   int run_strati, nref, iref;  // stratigraphy (Activate_Strati): reflectors, the active one (1-based)
   int aux;                   // offset of the nn+1-entry per-landscape arrays (level starts, groups)

@@ -17,6 +17,9 @@ struct DevState {
   int2* flow_tab = nullptr; // [nflow]: (instance, rank among the instance's workgroups)
   int nmfd = 0;             // workgroups of the multiple-flow dataflow sweeps over all instances
   int2* mfd_tab = nullptr;  // [nmfd]: (instance, rank among the instance's workgroups)
+  int nlvl = 0;             // workgroups of the multi-workgroup level sort (segments of the large landscapes)
+  int nlvl_big = 0;         // large landscapes (one workgroup each in the level scan)
+  int2* lvl_tab = nullptr;  // [nlvl + nlvl_big]: (instance, segment) per segment, then (instance, segments)
   int* flow_next = nullptr; // [ninst]: the sweep's tile counter of each instance
   int nsweep = 0;           // active single-flow landscapes with deposition (the level sweep k_spl_sweep)
   // launch shapes, chosen by fsx_autotune (no result depends on them)

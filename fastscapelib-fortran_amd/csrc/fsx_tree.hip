@@ -87,6 +87,45 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Level groups of one landscape from its level starts ls[0..L] (one workgroup): a new group starts at
+// level 0, at or after a wide level, and where the start position crosses a multiple of GRP_HALF.
+__device__ void level_groups(const InstParams& P, const int* ls, int L, int inst, int* __restrict__ gmeta,
+                             int* __restrict__ ngroups, int4* __restrict__ grec, int* lds) {
+  const int tid = threadIdx.x;
+  int* gm = gmeta + P.aux;  // <= L+1 entries
+  int gcarry = 0;
+  for (int s0 = 0; s0 < L; s0 += FSX_WG) {
+    const int l = s0 + tid;
+    int f = 0;
+    if (l < L) {
+      const int a0 = ls[l], w = ls[l + 1] - a0;
+      if (l == 0) f = 1;
+      else {
+        const int pa = ls[l - 1], pw = a0 - pa;
+        f = (w > GRP_HALF) || (pw > GRP_HALF) || (a0 / GRP_BUCKET != pa / GRP_BUCKET);
+      }
+    }
+    int tot;
+    const int incl = block_incl_scan<FSX_WG>(f, lds, tot);
+    if (f) gm[gcarry + incl - 1] = l;
+    gcarry += tot;
+  }
+  if (tid == 0) {
+    gm[gcarry] = L;
+    ngroups[inst] = gcarry;
+  }
+  __syncthreads();
+  // per-group records (lo, hi, p0, p1) + (pm, p2, -, -): level range, position range, start of the
+  // level below and end of the level above
+  int4* gr = grec + 2 * (long long)P.aux;
+  for (int g = tid; g < gcarry; g += FSX_WG) {
+    const int lo = gm[g], hi = gm[g + 1];
+    const int p0 = ls[lo], p1 = ls[hi];
+    gr[2 * g] = make_int4(lo, hi, p0, p1);
+    gr[2 * g + 1] = make_int4(lo > 0 ? ls[lo - 1] : 0, hi < L ? ls[hi + 1] : p1, 0, 0);
+  }
+}
+
 __global__ __launch_bounds__(FSX_WG) void k_level_sort(const InstParams* __restrict__ PP,
                                                        const unsigned long long* __restrict__ J,
                                                        const int* __restrict__ flags, int* __restrict__ lstart,
@@ -99,7 +138,7 @@ __global__ __launch_bounds__(FSX_WG) void k_level_sort(const InstParams* __restr
   __shared__ int s_max;
   const int inst = blockIdx.x;
   const InstParams& P = PP[inst];
-  if (!P.active || !P.run_spl) return;
+  if (!P.active || !P.run_spl || P.lvl_segs) return;  // large landscapes: k_lvl_count .. k_lvl_place
   const int nn = P.nn, tid = threadIdx.x;
   const long long off = P.off;
   int* ls = lstart + P.aux;  // nn+1 entries
@@ -195,39 +234,182 @@ __global__ __launch_bounds__(FSX_WG) void k_level_sort(const InstParams* __restr
     }
   }
   __syncthreads();
-  // level groups: a new group starts at level 0, at or after a wide level, and where the start
-  // position crosses a multiple of GRP_HALF
-  int* gm = gmeta + P.aux;  // <= L+1 entries
-  int gcarry = 0;
-  for (int s0 = 0; s0 < L; s0 += FSX_WG) {
-    const int l = s0 + tid;
-    int f = 0;
-    if (l < L) {
-      const int a0 = ls[l], w = ls[l + 1] - a0;
-      if (l == 0) f = 1;
-      else {
-        const int pa = ls[l - 1], pw = a0 - pa;
-        f = (w > GRP_HALF) || (pw > GRP_HALF) || (a0 / GRP_BUCKET != pa / GRP_BUCKET);
-      }
-    }
-    int tot;
-    const int incl = block_incl_scan<FSX_WG>(f, lds, tot);
-    if (f) gm[gcarry + incl - 1] = l;
-    gcarry += tot;
+  level_groups(P, ls, L, inst, gmeta, ngroups, grec, lds);
+}
+
+// ---- level order of the large landscapes (several workgroups each) -----------------------------
+// A landscape of at least FSX_LVL_BIG nodes is cut into segments of FSX_LVL_SEG nodes, one workgroup
+// each.  k_lvl_zero clears the landscape's depth counts; k_lvl_count adds each segment's depth histogram
+// (depths below LVL_BINS summed in LDS first: one global add per depth and segment) and the largest
+// depth; k_lvl_scan (one workgroup per landscape) turns the counts into level starts and cursors and
+// forms the level groups; k_lvl_place gives every node its position, a segment reserving each depth's
+// slots with one global add and dealing them out by LDS rank.  Positions within a level depend on
+// timing, as in k_level_sort; no result depends on them.
+#define LVL_BINS 8192
+#define LVL_PER (FSX_LVL_SEG / FSX_WG)  // nodes per thread
+
+__device__ __forceinline__ int clamp_depth(int d, int nn) { return d < 0 ? 0 : (d >= nn ? nn - 1 : d); }
+
+__global__ __launch_bounds__(FSX_WG) void k_lvl_zero(const InstParams* __restrict__ PP, const int2* __restrict__ tab,
+                                                     int* __restrict__ lcur, int* __restrict__ nlev) {
+  const int2 e = tab[blockIdx.x];
+  const InstParams& P = PP[e.x];
+  if (!P.active || !P.run_spl) return;
+  const int v0 = e.y * FSX_LVL_SEG, v1 = min(P.nn, v0 + FSX_LVL_SEG);
+  int* lc = lcur + P.aux;
+  for (int v = v0 + (int)threadIdx.x; v < v1; v += FSX_WG) lc[v] = 0;
+  if (e.y == 0 && threadIdx.x == 0) nlev[e.x] = 0;  // the largest depth, raised by k_lvl_count
+}
+
+__global__ __launch_bounds__(FSX_WG) void k_lvl_count(const InstParams* __restrict__ PP, const int2* __restrict__ tab,
+                                                      const unsigned long long* __restrict__ J,
+                                                      const int* __restrict__ flags, int* __restrict__ lcur,
+                                                      int* __restrict__ nlev) {
+  __shared__ int bins[LVL_BINS];
+  __shared__ int s_mb;
+  const int2 e = tab[blockIdx.x];
+  const InstParams& P = PP[e.x];
+  if (!P.active || !P.run_spl || flags[FSX_JUMP_ROUNDS - 1]) return;
+  const int nn = P.nn, tid = threadIdx.x, v0 = e.y * FSX_LVL_SEG;
+  const long long off = P.off;
+  int* lc = lcur + P.aux;
+  int dd[LVL_PER];
+#pragma unroll
+  for (int u = 0; u < LVL_PER; u++) {
+    const int v = v0 + u * FSX_WG + tid;
+    dd[u] = v < nn ? jdist(J[off + v]) : -1;
   }
-  if (tid == 0) {
-    gm[gcarry] = L;
-    ngroups[inst] = gcarry;
+  for (int l = tid; l < LVL_BINS; l += FSX_WG) bins[l] = 0;
+  if (tid == 0) s_mb = 0;
+  __syncthreads();
+  int mx = 0, mb = 0;
+#pragma unroll
+  for (int u = 0; u < LVL_PER; u++) {
+    if (v0 + u * FSX_WG + tid >= nn) continue;
+    const int d = clamp_depth(dd[u], nn);
+    mx = d > mx ? d : mx;
+    if (d < LVL_BINS) {
+      mb = d > mb ? d : mb;
+      atomicAdd(bins + d, 1);
+    } else {
+      atomicAdd(lc + d, 1);
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const int y = __shfl_xor(mx, o, 64), z = __shfl_xor(mb, o, 64);
+    mx = y > mx ? y : mx;
+    mb = z > mb ? z : mb;
+  }
+  if ((tid & 63) == 0) {
+    atomicMax(nlev + e.x, mx);
+    atomicMax(&s_mb, mb);
   }
   __syncthreads();
-  // per-group records (lo, hi, p0, p1) + (pm, p2, -, -): level range, position range, start of the
-  // level below and end of the level above
-  int4* gr = grec + 2 * (long long)P.aux;
-  for (int g = tid; g < gcarry; g += FSX_WG) {
-    const int lo = gm[g], hi = gm[g + 1];
-    const int p0 = ls[lo], p1 = ls[hi];
-    gr[2 * g] = make_int4(lo, hi, p0, p1);
-    gr[2 * g + 1] = make_int4(lo > 0 ? ls[lo - 1] : 0, hi < L ? ls[hi + 1] : p1, 0, 0);
+  for (int l = tid; l <= s_mb; l += FSX_WG) {
+    const int x = bins[l];
+    if (x) atomicAdd(lc + l, x);
+  }
+}
+
+__global__ __launch_bounds__(FSX_WG) void k_lvl_scan(const InstParams* __restrict__ PP, const int2* __restrict__ tab,
+                                                     const int* __restrict__ flags, int* __restrict__ lstart,
+                                                     int* __restrict__ lcur, int* __restrict__ nlev,
+                                                     int* __restrict__ gmeta, int* __restrict__ ngroups,
+                                                     int4* __restrict__ grec, int* __restrict__ err) {
+  __shared__ int lds[FSX_WG / 64];
+  const int inst = tab[blockIdx.x].x;
+  const InstParams& P = PP[inst];
+  if (!P.active || !P.run_spl) return;
+  const int tid = threadIdx.x;
+  if (flags[FSX_JUMP_ROUNDS - 1]) {  // jumping did not converge: a receiver cycle, so depths are meaningless
+    if (tid == 0) {
+      atomicOr(err + inst, FSX_ERR_LEVELS);
+      nlev[inst] = 0;
+      ngroups[inst] = 0;
+    }
+    return;
+  }
+  int* ls = lstart + P.aux;
+  int* lc = lcur + P.aux;
+  const int L = nlev[inst] + 1;
+  __syncthreads();  // every thread has read the largest depth before thread 0 replaces it by L
+  int carry = 0;
+  for (int s0 = 0; s0 < L; s0 += FSX_WG) {
+    const int l = s0 + tid;
+    const int x = l < L ? lc[l] : 0;
+    int tot;
+    const int incl = block_incl_scan<FSX_WG>(x, lds, tot);
+    if (l < L) {
+      const int st = carry + incl - x;
+      ls[l] = st;
+      lc[l] = st;
+    }
+    carry += tot;
+  }
+  if (tid == 0) {
+    ls[L] = carry;
+    nlev[inst] = L;
+  }
+  __syncthreads();
+  level_groups(P, ls, L, inst, gmeta, ngroups, grec, lds);
+}
+
+__global__ __launch_bounds__(FSX_WG) void k_lvl_place(const InstParams* __restrict__ PP, const int2* __restrict__ tab,
+                                                      const unsigned long long* __restrict__ J,
+                                                      const int* __restrict__ flags, int* __restrict__ lcur,
+                                                      int* __restrict__ order, int* __restrict__ pos) {
+  __shared__ int bins[LVL_BINS];
+  __shared__ int s_mb;
+  const int2 e = tab[blockIdx.x];
+  const InstParams& P = PP[e.x];
+  if (!P.active || !P.run_spl || flags[FSX_JUMP_ROUNDS - 1]) return;
+  const int nn = P.nn, tid = threadIdx.x, v0 = e.y * FSX_LVL_SEG;
+  const long long off = P.off;
+  int* lc = lcur + P.aux;
+  int dd[LVL_PER], rk[LVL_PER];
+#pragma unroll
+  for (int u = 0; u < LVL_PER; u++) {
+    const int v = v0 + u * FSX_WG + tid;
+    dd[u] = v < nn ? jdist(J[off + v]) : -1;
+  }
+  for (int l = tid; l < LVL_BINS; l += FSX_WG) bins[l] = 0;
+  if (tid == 0) s_mb = 0;
+  __syncthreads();
+  // rank of each node among the segment's nodes of its depth (shallow depths, in LDS), or its position
+  // straight from the global cursor (deep depths)
+  int mb = 0;
+#pragma unroll
+  for (int u = 0; u < LVL_PER; u++) {
+    rk[u] = 0;
+    if (v0 + u * FSX_WG + tid >= nn) continue;
+    const int d = clamp_depth(dd[u], nn);
+    dd[u] = d;
+    if (d < LVL_BINS) {
+      mb = d > mb ? d : mb;
+      rk[u] = atomicAdd(bins + d, 1);
+    } else {
+      rk[u] = atomicAdd(lc + d, 1);
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) { const int z = __shfl_xor(mb, o, 64); mb = z > mb ? z : mb; }
+  if ((tid & 63) == 0) atomicMax(&s_mb, mb);
+  __syncthreads();
+  // one reservation per shallow depth: the bin becomes the segment's first position at that depth
+  for (int l = tid; l <= s_mb; l += FSX_WG) {
+    const int x = bins[l];
+    if (x) bins[l] = atomicAdd(lc + l, x);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < LVL_PER; u++) {
+    const int v = v0 + u * FSX_WG + tid;
+    if (v >= nn) continue;
+    const int d = dd[u];
+    const int p = d < LVL_BINS ? bins[d] + rk[u] : rk[u];
+    if (p < nn) {
+      order[off + p] = v;
+      pos[off + v] = p;
+    }
   }
 }
 
@@ -1314,6 +1496,12 @@ void launch_levels(DevState& S, hipStream_t st) {
   launch_jump(S, st);
   k_level_sort<<<S.ninst, FSX_WG, LVL_HCAP * 4, st>>>(S.P, S.J, S.flags, S.lstart, S.lcur, S.nlev, S.gmeta, S.ngroups,
                                                         S.grec, S.order, S.pos, S.err);
+  if (!S.nlvl) return;
+  k_lvl_zero<<<S.nlvl, FSX_WG, 0, st>>>(S.P, S.lvl_tab, S.lcur, S.nlev);
+  k_lvl_count<<<S.nlvl, FSX_WG, 0, st>>>(S.P, S.lvl_tab, S.J, S.flags, S.lcur, S.nlev);
+  k_lvl_scan<<<S.nlvl_big, FSX_WG, 0, st>>>(S.P, S.lvl_tab + S.nlvl, S.flags, S.lstart, S.lcur, S.nlev, S.gmeta,
+                                             S.ngroups, S.grec, S.err);
+  k_lvl_place<<<S.nlvl, FSX_WG, 0, st>>>(S.P, S.lvl_tab, S.J, S.flags, S.lcur, S.order, S.pos);
 }
 
 void launch_area_prep(DevState& S, hipStream_t st) {

@@ -194,6 +194,43 @@ def test_deeper_than_lds_level_histogram(fs, oracle):
         assert b.stats(0)["levels"] > 32768
 
 
+@pytest.mark.parametrize("shape,ibc,deep", [((521, 261), 1010, False), ((50001, 3), 1, True)])
+def test_large_landscape_level_order(fs, oracle, shape, ibc, deep):
+    """Landscapes of 131072 nodes and more are level-sorted by one workgroup per 16384-node segment.
+    521x261: a 2-D landscape of 9 segments; 50001x3: a strip whose ~50000 levels are deeper than the
+    segments' LDS bins (8192) and the one-workgroup sort's LDS histogram (32768).  Elevations match the
+    oracle, the order is a permutation in which every receiver precedes its donors, and a batch with a
+    small landscape beside the large one gives the large one's result bit for bit."""
+    nx, ny = shape
+    if deep:
+        L = landscape(nx, ny, seed=78, dx=100.0, ibc=ibc, kf=1e-6, kd=1e-3)
+        L["h"] = (np.tile(np.arange(nx, dtype=np.float64), ny) * 0.5 + L["h"] * 0.1)
+        L["u"] = np.full(nx * ny, 1e-4)
+        L["u"][np.arange(nx * ny) % nx == 0] = 0.0
+        steps = 2
+    else:
+        L = landscape(nx, ny, seed=5, dx=250.0, ibc=ibc, sinus=(10.0, 20000.0), kf=2e-6)
+        steps = 4
+    print(f"large landscape {nx}x{ny}, ibc={ibc}, {steps} steps", flush=True)
+    got, ref = _run_both(fs, oracle, L, steps)
+    _check(got, ref, steps)
+    small = landscape(65, 90, seed=1, dx=250.0, ibc=1010, sinus=(10.0, 5000.0))
+    with fs.Batch([(nx, ny), (65, 90)]) as b:
+        apply(b, L, 0)
+        apply(b, small, 1)
+        b.step(steps)
+        h_mix = b.h(0)
+        rec, order = b.rec(0), b.order(0)
+        levels = b.stats(0)["levels"]
+    assert np.array_equal(h_mix, got["h"])
+    pos = np.empty(nx * ny, np.int64)
+    pos[order] = np.arange(nx * ny)
+    assert np.array_equal(np.sort(order), np.arange(nx * ny))
+    assert np.all(pos[rec] <= pos)
+    if deep:
+        assert levels > 32768
+
+
 def test_batch_equals_alone(fs, oracle):
     """Ragged batch: each landscape's result equals its run alone, bit for bit."""
     Ls = [landscape(65, 90, seed=1, dx=250.0, ibc=1010, sinus=(10.0, 5000.0)),
