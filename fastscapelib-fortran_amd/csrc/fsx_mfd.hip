@@ -516,24 +516,39 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_mfd_prep(const InstParams* __rest
                                                         double* __restrict__ mdw, int* __restrict__ dn,
                                                         int* __restrict__ rp) {
   MFD_PROLOGUE
-  const long long off = P.off, gp = off + mpos[g_];
-  A0[gp] = P.dxdy * precip[g_];
-  Al[gp] = flow_sentinel();  // the accumulation's dataflow waits for it to change
-  const int nd = mndon[g_], nr = mnrec[g_];
+  // every load is issued before any is used: the slots' indices, then the positions and weights they
+  // point at (the records are written to scattered positions, so nothing waits on a store)
+  const long long off = P.off;
+  const int nd = mndon[g_], nr = mnrec[g_], r = rec[g_], mp = mpos[g_];
+  const double pr = precip[g_];
+  int e[8], q[8];
+#pragma unroll
   for (int j = 0; j < 8; j++) {
-    int dpos = -1;
-    double w = 0.0;
-    if (j < nd) {
-      const int e = mdon[8 * g_ + j];
-      dpos = mpos[off + (e >> 3)];
-      w = mw[8 * off + e];
-    }
-    mdp[8 * gp + j] = dpos;
-    mdw[8 * gp + j] = w;
-    dn[8 * gp + j] = j < nr ? mpos[off + mrec[8 * g_ + j]] : -1;
+    e[j] = j < nd ? mdon[8 * g_ + j] : 0;
+    q[j] = j < nr ? mrec[8 * g_ + j] : 0;
   }
-  const int r = rec[g_];
-  rp[gp] = r == v ? -1 : mpos[off + r];
+  int dpos[8], rpos[8];
+  double w[8];
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    dpos[j] = j < nd ? mpos[off + (e[j] >> 3)] : -1;
+    w[j] = j < nd ? mw[8 * off + e[j]] : 0.0;
+    rpos[j] = j < nr ? mpos[off + q[j]] : -1;
+  }
+  const int rpv = r == v ? -1 : mpos[off + r];
+  const long long gp = off + mp;
+  A0[gp] = P.dxdy * pr;
+  Al[gp] = flow_sentinel();  // the accumulation's dataflow waits for it to change
+  int4* dp4 = reinterpret_cast<int4*>(mdp + 8 * gp);
+  dp4[0] = make_int4(dpos[0], dpos[1], dpos[2], dpos[3]);
+  dp4[1] = make_int4(dpos[4], dpos[5], dpos[6], dpos[7]);
+  double2* dw2 = reinterpret_cast<double2*>(mdw + 8 * gp);
+#pragma unroll
+  for (int j = 0; j < 4; j++) dw2[j] = make_double2(w[2 * j], w[2 * j + 1]);
+  int4* dn4 = reinterpret_cast<int4*>(dn + 8 * gp);
+  dn4[0] = make_int4(rpos[0], rpos[1], rpos[2], rpos[3]);
+  dn4[1] = make_int4(rpos[4], rpos[5], rpos[6], rpos[7]);
+  rp[gp] = rpv;
 }
 
 // ---- accumulation (FlowAccumulation :97-119) -------------------------------------------------------
