@@ -989,39 +989,59 @@ __device__ __forceinline__ int* nr_list(double* cw, long long eb, int n, int lev
   return reinterpret_cast<int*>(cw) + 2 * eb + ((level & 1) ? n + 2 : 0);
 }
 
-__global__ __launch_bounds__(FSX_WG) void k_nr_init(const InstParams* __restrict__ PP, const int* __restrict__ nconn,
-                                                    double* __restrict__ cw, double* __restrict__ skey,
-                                                    int* __restrict__ sid, int* __restrict__ segs,
-                                                    int* __restrict__ nrst, int* __restrict__ err) {
-  const int inst = blockIdx.x;
+// The keys and identities of the sort, copied out of cw: gridDim.x workgroups per landscape (blockIdx.y),
+// each thread taking 4 slots a stride apart at a time.  A pass height that is not a number (elevations that
+// are not) would break the partition scans' sentinel (every comparison with it is false): it is sorted as
+// +infinity and the step reports the landscape.
+__global__ __launch_bounds__(FSX_CHUNK) void k_nr_copy(const InstParams* __restrict__ PP, const int* __restrict__ nconn,
+                                                       const double* __restrict__ cw, double* __restrict__ skey,
+                                                       int* __restrict__ sid, int* __restrict__ err) {
+  const int inst = blockIdx.y;
   const InstParams& P = PP[inst];
   if (!P.active || !P.run_spl) return;
   const int n = nconn[inst];
   const long long eb = (long long)FSX_EDGE_CAP * P.off;
-  // A pass height that is not a number (elevations that are not) would break the partition scans' sentinel
-  // (every comparison with it is false): it is sorted as +infinity and the step reports the landscape.
+  const int stride = (int)gridDim.x * FSX_CHUNK;
   bool nan = false;
-  for (int p = threadIdx.x; p < n; p += FSX_WG) {
-    const double kv = cw[eb + p];
-    nan |= kv != kv;
-    skey[eb + p] = kv == kv ? kv : __builtin_huge_val();
-    sid[eb + p] = p;
+  for (int p0 = (int)blockIdx.x * FSX_CHUNK + (int)threadIdx.x; p0 < n; p0 += 4 * stride) {
+    double kv[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int p = p0 + k * stride;
+      kv[k] = p < n ? cw[eb + p] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int p = p0 + k * stride;
+      if (p >= n) continue;
+      nan |= kv[k] != kv[k];
+      skey[eb + p] = kv[k] == kv[k] ? kv[k] : __builtin_huge_val();
+      sid[eb + p] = p;
+    }
   }
   if (nan) atomicOr(err + inst, FSX_ERR_NONFINITE);
-  __syncthreads();  // every key copied before the level lists overwrite cw
-  if (threadIdx.x == 0) {
-    int* st = nrst + NR_ST * inst;
-    for (int r = 0; r < NR_ST; r++) st[r] = 0;
-    if (n > NR_BIG) {
-      int* L0 = nr_list(cw, eb, n, 0);
-      L0[0] = 0;
-      L0[1] = n - 1;
-      st[0] = 1;
-    } else if (n >= 2) {
-      segs[eb] = 0;
-      segs[eb + 1] = n - 1;
-      st[NR_ST - 1] = 1;
-    }
+}
+
+// The sort's start state, once k_nr_copy has taken the keys out of cw (the level lists overwrite it).
+__global__ __launch_bounds__(64) void k_nr_init(const InstParams* __restrict__ PP, const int* __restrict__ nconn,
+                                                double* __restrict__ cw, int* __restrict__ segs,
+                                                int* __restrict__ nrst) {
+  const int inst = blockIdx.x;
+  const InstParams& P = PP[inst];
+  if (!P.active || !P.run_spl || threadIdx.x != 0) return;
+  const int n = nconn[inst];
+  const long long eb = (long long)FSX_EDGE_CAP * P.off;
+  int* st = nrst + NR_ST * inst;
+  for (int r = 0; r < NR_ST; r++) st[r] = 0;
+  if (n > NR_BIG) {
+    int* L0 = nr_list(cw, eb, n, 0);
+    L0[0] = 0;
+    L0[1] = n - 1;
+    st[0] = 1;
+  } else if (n >= 2) {
+    segs[eb] = 0;
+    segs[eb + 1] = n - 1;
+    st[NR_ST - 1] = 1;
   }
 }
 
@@ -1807,7 +1827,10 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_correct(const InstParams* __restr
 // ---------------------------------------------------------------------------------------------------
 void launch_nr_sort(DevState& S, hipStream_t st) {
   launch_fill_int(S.segctr, S.ninst, 0, st);
-  k_nr_init<<<S.ninst, FSX_WG, 0, st>>>(S.P, S.nconn, S.cw, S.skey, S.sid, S.segs, S.nrst, S.err);
+  // about 16 slots per thread on the largest landscape's edge capacity, at most 1024 workgroups per landscape
+  const int gx = (int)std::min<long long>(1024, std::max<long long>(1, (long long)FSX_EDGE_CAP * S.max_nn / (FSX_CHUNK * 16)));
+  k_nr_copy<<<dim3(gx, S.ninst), FSX_CHUNK, 0, st>>>(S.P, S.nconn, S.cw, S.skey, S.sid, S.err);
+  k_nr_init<<<S.ninst, 64, 0, st>>>(S.P, S.nconn, S.cw, S.segs, S.nrst);
   // levels for the largest landscape's edge count (6 per node at most): log2(edges / NR_BIG) plus slack
   // for unbalanced partitions, within the per-landscape counters
   int nlev = 4;
