@@ -425,18 +425,42 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_area_prep(const InstParams* __res
   NODE_PROLOGUE
   if (!P.sfd || flags[FSX_JUMP_ROUNDS - 1]) return;  // no level order this step (receiver cycle)
   const int p = pos[g_];
-  const int d = jdist(J[g_]);
+  const unsigned long long jw = J[g_];
+  const unsigned mask = dmask[g_];
+  const double pr = precip[g_];
   const long long gp = (long long)P.off + p;
-  int ds[8];
-  const int nd = donors_sorted(P, v, dmask[g_], ds);
-  const int base = lstart[P.aux + d + 1];
+  const int i = v % P.nx, j = v / P.nx;
   int o[8];
+  if (i > 0 && j > 0 && i < P.nx - 1 && j < P.ny - 1) {
+    // interior: mask bit b is the b-th neighbour in ascending index order; the donors' positions are
+    // loaded into fixed slots together and placed by rank in registers (descending index order)
+    int pp[8];
 #pragma unroll
-  for (int k = 0; k < 8; k++) o[k] = k < nd ? pos[P.off + ds[nd - 1 - k]] - base : -1;
+    for (int b = 0; b < 8; b++) {
+      const int dd = b < 4 ? b : b + 1;
+      pp[b] = ((mask >> b) & 1u) ? pos[g_ + (dd % 3 - 1) + (long long)(dd / 3 - 1) * P.nx] : 0;
+    }
+    const int base = lstart[P.aux + jdist(jw) + 1];
+    const int nd = __popc(mask);
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      int x = -1;
+#pragma unroll
+      for (int b = 0; b < 8; b++)
+        if (((mask >> b) & 1u) && __popc(mask & ((1u << b) - 1u)) == nd - 1 - k) x = pp[b] - base;
+      o[k] = x;
+    }
+  } else {
+    int ds[8];
+    const int nd = donors_sorted(P, v, mask, ds);
+    const int base = lstart[P.aux + jdist(jw) + 1];
+#pragma unroll
+    for (int k = 0; k < 8; k++) o[k] = k < nd ? pos[P.off + ds[nd - 1 - k]] - base : -1;
+  }
   int4* w = reinterpret_cast<int4*>(arec + gp);
   w[0] = make_int4(o[0], o[1], o[2], o[3]);
   w[1] = make_int4(o[4], o[5], o[6], o[7]);
-  reinterpret_cast<double2*>(arec + gp)[2] = make_double2(P.dxdy * precip[g_], 0.0);
+  reinterpret_cast<double2*>(arec + gp)[2] = make_double2(P.dxdy * pr, 0.0);
   reinterpret_cast<double2*>(arec + gp)[3] = make_double2(0.0, 0.0);
 }
 
