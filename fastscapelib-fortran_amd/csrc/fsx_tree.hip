@@ -489,6 +489,7 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_spl_fact(const InstParams* __rest
   const int p = ordered ? pos[g_] : 0;
   const int d = ordered ? jdist(J[g_]) : 0;
   const int r = ordered ? rec[g_] : 0;
+  const double kfv = ordered ? kfint[g_] : 0.0, lv = ordered ? length[g_] : 0.0;
   // the landscape's maxval(abs(h)) and maxval(g) (src/StreamPowerLaw.f90:246,376), h == ht here:
   // reduced over the block, then one atomic per block
   unsigned long long kh = live ? dkey(fabs(ht[g_])) : 0ull, kg = live ? dkey(g[g_]) : 0ull;
@@ -509,21 +510,25 @@ __global__ __launch_bounds__(FSX_CHUNK) void k_spl_fact(const InstParams* __rest
   }
   if (!ordered) return;  // no level order this step (receiver cycle)
   const long long gp = (long long)P.off + p;
+  // the loads that follow the position, depth and receiver, issued together
   const double t = ht[g_];
   const double av = Al[gp];  // drainage area, level order -> grid order
+  const int pr = d > 0 ? pos[P.off + r] : 0;
+  const int l0 = (d > 0 && !P.flow) ? lstart[P.aux + d - 1] : 0;
+  const double tr = d > 0 ? ht[P.off + r] : 0.0;
   area[g_] = av;
   lsed[g_] = 0.0;  // no lake sediment before the first Gauss-Seidel pass
   int q = -1;
   double f = 0.0;
   if (d > 0) {
     // the level sweep reads the receiver relative to its level's start, the dataflow sweep absolutely
-    q = P.flow ? pos[P.off + r] : pos[P.off + r] - lstart[P.aux + d - 1];
+    q = pr - l0;
     const double am = pow(av, P.m);
     const double n = P.n;
     if (fabs(n - 1.0) < 2.2250738585072014e-308) {
-      f = kfint[g_] * P.dt * am / length[g_];
+      f = kfv * P.dt * am / lv;
     } else {
-      f = (t > ht[P.off + r]) ? kfint[g_] * P.dt * am / pow(length[g_], n) : -1.0;
+      f = (t > tr) ? kfv * P.dt * am / pow(lv, n) : -1.0;
     }
   }
   ulonglong2* w = reinterpret_cast<ulonglong2*>(srec + gp);
